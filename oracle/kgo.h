@@ -1,0 +1,207 @@
+/*
+ * kgo.h -- CPU oracle: a plain-C restatement of the reference engine's
+ * hash-join / hash-aggregate / runtime-Bloom / hash-repartition algorithms.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path (libkgpu.so) never links,
+ * loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository root).  Parity pins (what checks each function
+ * against something other than its own formula) are listed in the header
+ * comment of each section.
+ */
+#ifndef KGO_H
+#define KGO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Column types: the same numbering as include/kgpu.h (KG_T_*). */
+enum {
+  KGO_T_BOOL = 1,
+  KGO_T_INT8 = 2,
+  KGO_T_INT16 = 3,
+  KGO_T_INT32 = 4,
+  KGO_T_INT64 = 5,
+  KGO_T_UINT8 = 6,
+  KGO_T_UINT16 = 7,
+  KGO_T_UINT32 = 8,
+  KGO_T_UINT64 = 9,
+  KGO_T_FLOAT32 = 10,
+  KGO_T_FLOAT64 = 11,
+  KGO_T_DATE32 = 12,
+  KGO_T_TIMESTAMP = 13, /* any 64-bit temporal: date64, timestamps, durations */
+  KGO_T_DECIMAL64 = 14,
+  KGO_T_DECIMAL128 = 15
+};
+
+/* One Arrow-layout column: `values` (bit-packed for BOOL), optional LSB
+ * validity bitmap (NULL = all valid), element offset into both. */
+typedef struct kgo_col {
+  int32_t type;
+  int32_t precision;
+  int32_t scale;
+  int32_t _pad;
+  int64_t length;
+  int64_t offset;
+  const void *values;
+  const uint8_t *validity;
+} kgo_col;
+
+/* ---------------------------------------------------------------------
+ * Routing hash H_kg v1  (crates/krishiv-kernels/src/hash/routing.rs)
+ * Pinned by: routing tests' XXH64 vectors and H_kg goldens
+ * (crates/krishiv-kernels/tests/routing.rs:30-33,41-145), Python xxhash, and
+ * the partition/key-group tiling invariants (tests/routing.rs:385-420).
+ * --------------------------------------------------------------------- */
+uint64_t kgo_xxh64(const void *data, size_t len, uint64_t seed);
+uint64_t kgo_xxh64_fixed(uint64_t lo, uint64_t hi, int width);
+uint64_t kgo_fold_column(uint64_t h, uint64_t h_col);
+uint32_t kgo_key_group(uint64_t h);
+uint32_t kgo_radix(uint64_t h, uint32_t bits);
+uint32_t kgo_partition_of(uint64_t h, uint32_t n);
+void kgo_key_group_range(uint32_t p, uint32_t n, uint32_t *start, uint32_t *end);
+/* H_kg of each row of `ncols` key columns (NULL_HASH for a NULL value). */
+int kgo_hash_columns(int ncols, const kgo_col *cols, int64_t n, uint64_t *out);
+/* exchange::route for a Hash partitioning: partition of each row. */
+int kgo_route(int ncols, const kgo_col *cols, int64_t n, uint32_t partitions, uint32_t *out);
+/* Stable split of row ids by partition (ExchangeSink::sink's `split`):
+ * out_rows holds the row ids grouped by partition in input order,
+ * out_offsets[p]..out_offsets[p+1] its range. */
+void kgo_split(const uint32_t *pids, int64_t n, uint32_t partitions, int64_t *out_offsets,
+               int64_t *out_rows);
+
+/* ---------------------------------------------------------------------
+ * Row-layout table hash h and finalizer g
+ * (crates/krishiv-exec/src/rows/layout.rs:43-50,487-553,871-904,1080-1144)
+ * Pinned by: the known answer 4733552696251797926 for keys
+ * (float64 0.0, int32 NULL) (crates/krishiv-exec/src/join/keys.rs:348).
+ * --------------------------------------------------------------------- */
+uint64_t kgo_fold(uint64_t a, uint64_t b);
+int kgo_row_hash(int ncols, const kgo_col *cols, int64_t n, uint64_t *out);
+uint64_t kgo_finalize(uint64_t h, uint64_t seed);
+/* join::keys::Hashing::partition (join/keys.rs:118-128) for the row layout */
+uint32_t kgo_join_partition(uint64_t h, uint64_t seed, uint32_t radix_bits, uint32_t depth);
+
+/* ---------------------------------------------------------------------
+ * Split-block Bloom filter (crates/krishiv-io/src/runtime_filter/bloom.rs)
+ * Pinned by: the Parquet SBBF bit-for-bit vector (bloom.rs:329-347,
+ * regenerated from the Parquet specification with Python xxhash), the sizing
+ * closed form, and the reference's sizing/false-positive bounds
+ * (bloom.rs:349-378).
+ * --------------------------------------------------------------------- */
+size_t kgo_sbbf_bytes_for(uint64_t items, double fpp);
+size_t kgo_sbbf_blocks_for_bytes(size_t bytes);
+void kgo_sbbf_insert(uint32_t *words, size_t nblocks, uint64_t h);
+int kgo_sbbf_check(const uint32_t *words, size_t nblocks, uint64_t h);
+double kgo_sbbf_expected_fpp(size_t nblocks, uint64_t items);
+/* filter.rs:606-624 `bloom()`: build from hashes; returns 1 if kept (expected
+ * fpp <= max_fpp), 0 if dropped.  bytes==0 sizes it from the hash count. */
+int kgo_sbbf_build(const uint64_t *hashes, int64_t n, size_t bytes, double fpp, size_t max_bytes,
+                   double max_fpp, uint32_t *words_out, size_t *nblocks_out);
+
+/* ---------------------------------------------------------------------
+ * Synthetic data generators (counter-based; identical integer arithmetic on
+ * the device in libkgpu).
+ * --------------------------------------------------------------------- */
+uint64_t kgo_mix64(uint64_t z);
+uint64_t kgo_rand(uint64_t seed, uint64_t stream, uint64_t i);
+uint64_t kgo_below(uint64_t r, uint64_t n);
+double kgo_unit(uint64_t r);
+uint64_t kgo_perm(uint64_t i, uint64_t n, uint64_t seed);
+void kgo_zipf_cdf(uint64_t n, double s, double *cdf);
+uint64_t kgo_zipf_rank(const double *cdf, uint64_t n, double u);
+/* config generators (see bench.py and the tests for the shapes) */
+void kgo_gen_join_build(uint64_t seed, int64_t lo, int64_t hi, int64_t n_total, int64_t *key,
+                        int64_t *val);
+void kgo_gen_join_probe(uint64_t seed, int64_t lo, int64_t hi, int64_t key_range,
+                        uint32_t null_permille, int64_t *fk, uint8_t *fk_valid, int64_t *amt);
+void kgo_gen_agg(uint64_t seed, int64_t lo, int64_t hi, const double *cdf, uint64_t ndistinct,
+                 int64_t *key, int64_t *dec, double *f, int32_t *i32, int8_t *k8);
+
+/* ---------------------------------------------------------------------
+ * Hash aggregation with Spark semantics
+ * (crates/krishiv-exec/src/aggregate/mod.rs:588-775,929-1025,1083-1113;
+ *  accumulator/dense.rs:55-345; accumulator/helpers.rs:120-330,595-700;
+ *  accumulator/output.rs:230-285)
+ * Pinned by: Spark 4.2.0 goldens tests/fixtures/spark-4.2.0/decimal-sum-avg.json
+ * (172 cases), exact Python integer/decimal arithmetic, and math.fsum for
+ * float sums.
+ * --------------------------------------------------------------------- */
+enum {
+  KGO_AGG_COUNT_STAR = 1,
+  KGO_AGG_COUNT = 2,
+  KGO_AGG_SUM = 3,
+  KGO_AGG_MIN = 4,
+  KGO_AGG_MAX = 5,
+  KGO_AGG_AVG = 6
+};
+typedef struct kgo_agg_spec {
+  int32_t func;
+  int32_t input; /* input column index (ignored for COUNT_STAR) */
+} kgo_agg_spec;
+
+/* One output column of the aggregate: which buffer is used depends on the
+ * result type (i64 for integers/counts, f64 for doubles, lo/hi for
+ * decimals, lo for Decimal64 results). */
+typedef struct kgo_out {
+  int32_t type; /* KGO_T_* of the result */
+  int32_t precision;
+  int32_t scale;
+  int32_t _pad;
+  int64_t *i64;  /* also decimal low word */
+  int64_t *hi;   /* decimal high word */
+  double *f64;
+  uint8_t *valid; /* one byte per row */
+} kgo_out;
+
+typedef struct kgo_agg kgo_agg;
+/* key: the grouping column (NULL for a global aggregate). */
+kgo_agg *kgo_agg_new(const kgo_col *key_type_like, int ninputs, const kgo_col *input_types,
+                     int naggs, const kgo_agg_spec *aggs, int ansi);
+/* Raw rows, in order (HashAggregate::sink -> RowTable::apply). */
+int kgo_agg_update(kgo_agg *a, const kgo_col *key, const kgo_col *inputs, int64_t n);
+/* Partial states of `other`, merged into `a` in group-arrival order
+ * (combine/finalize of a two-phase aggregate). */
+int kgo_agg_merge(kgo_agg *a, const kgo_agg *other);
+int64_t kgo_agg_groups(const kgo_agg *a);
+/* Final values: out_key (i64 + valid) and one kgo_out per aggregate. */
+int kgo_agg_finish(const kgo_agg *a, int64_t *out_key, uint8_t *out_key_valid, kgo_out *outs);
+/* Result type of aggregate i: type, precision, scale. */
+void kgo_agg_result_type(const kgo_agg *a, int i, int32_t *type, int32_t *precision,
+                         int32_t *scale);
+void kgo_agg_free(kgo_agg *a);
+const char *kgo_last_error(void);
+
+/* decimal avg by one HALF_UP division (accumulator/helpers.rs:595-609).
+ * Returns 1 and *out on success, 0 when the shortcut does not apply or the
+ * value does not fit decimal(p, s). */
+int kgo_avg_fast(int64_t sum_lo, int64_t sum_hi, int sum_scale, int64_t count, int p, int s,
+                 int64_t *out_lo, int64_t *out_hi);
+
+/* ---------------------------------------------------------------------
+ * Hash join (crates/krishiv-exec/src/join/mod.rs:21-60,217-293;
+ * join/probe.rs:266-501).  Inner / left semi / left anti / left outer on
+ * `=` (NULL never matches) or `<=>` keys over integer columns.
+ * Pinned by: a nested-loop join on small inputs (the reference's own oracle,
+ * crates/krishiv-exec/tests/hash_join/main.rs:130-195).
+ * --------------------------------------------------------------------- */
+enum { KGO_JOIN_INNER = 0, KGO_JOIN_LEFT_OUTER = 1, KGO_JOIN_LEFT_SEMI = 2, KGO_JOIN_LEFT_ANTI = 3 };
+/* Emits (probe row, build row) pairs; build row -1 for an unmatched probe
+ * row of a left outer join; semi/anti emit probe rows only.  Returns the
+ * pair count, or -1 if `cap` is too small.  `nested` selects the
+ * nested-loop form (the reference test oracle) instead of the hash form. */
+int64_t kgo_join(int join_type, int nkeys, const kgo_col *probe_keys, const kgo_col *build_keys,
+                 const uint8_t *null_safe, int64_t n_probe, int64_t n_build, int nested,
+                 int64_t cap, int64_t *out_probe, int64_t *out_build);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

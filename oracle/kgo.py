@@ -1,0 +1,385 @@
+"""ctypes binding of the CPU oracle (oracle/libkgo.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the ``cpu_baseline`` leg of ``bench.py`` -- as the checker of the HIP
+path, never as the thing measured as the product or shipped.  Each wrapper
+names the reference file:line the C function restates (see oracle/kgo.c).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libkgo.so")
+
+T_BOOL, T_INT8, T_INT16, T_INT32, T_INT64 = 1, 2, 3, 4, 5
+T_UINT8, T_UINT16, T_UINT32, T_UINT64 = 6, 7, 8, 9
+T_FLOAT32, T_FLOAT64, T_DATE32, T_TIMESTAMP, T_DECIMAL64, T_DECIMAL128 = 10, 11, 12, 13, 14, 15
+
+AGG_COUNT_STAR, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_AVG = 1, 2, 3, 4, 5, 6
+JOIN_INNER, JOIN_LEFT_OUTER, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI = 0, 1, 2, 3
+
+_NP_OF = {
+    T_INT8: np.int8, T_INT16: np.int16, T_INT32: np.int32, T_INT64: np.int64,
+    T_UINT8: np.uint8, T_UINT16: np.uint16, T_UINT32: np.uint32, T_UINT64: np.uint64,
+    T_FLOAT32: np.float32, T_FLOAT64: np.float64, T_DATE32: np.int32, T_TIMESTAMP: np.int64,
+    T_DECIMAL64: np.int64,
+}
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc) and return the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+class _Col(C.Structure):
+    _fields_ = [("type", C.c_int32), ("precision", C.c_int32), ("scale", C.c_int32),
+                ("_pad", C.c_int32), ("length", C.c_int64), ("offset", C.c_int64),
+                ("values", C.c_void_p), ("validity", C.c_void_p)]
+
+
+class _Out(C.Structure):
+    _fields_ = [("type", C.c_int32), ("precision", C.c_int32), ("scale", C.c_int32),
+                ("_pad", C.c_int32), ("i64", C.c_void_p), ("hi", C.c_void_p),
+                ("f64", C.c_void_p), ("valid", C.c_void_p)]
+
+
+class _AggSpec(C.Structure):
+    _fields_ = [("func", C.c_int32), ("input", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        u64, u32, i64, vp = C.c_uint64, C.c_uint32, C.c_int64, C.c_void_p
+        L.kgo_xxh64.restype = u64
+        L.kgo_xxh64.argtypes = [vp, C.c_size_t, u64]
+        L.kgo_xxh64_fixed.restype = u64
+        L.kgo_xxh64_fixed.argtypes = [u64, u64, C.c_int]
+        L.kgo_fold_column.restype = u64
+        L.kgo_fold_column.argtypes = [u64, u64]
+        L.kgo_fold.restype = u64
+        L.kgo_fold.argtypes = [u64, u64]
+        L.kgo_key_group.restype = u32
+        L.kgo_key_group.argtypes = [u64]
+        L.kgo_radix.restype = u32
+        L.kgo_radix.argtypes = [u64, u32]
+        L.kgo_partition_of.restype = u32
+        L.kgo_partition_of.argtypes = [u64, u32]
+        L.kgo_key_group_range.argtypes = [u32, u32, vp, vp]
+        L.kgo_hash_columns.restype = C.c_int
+        L.kgo_hash_columns.argtypes = [C.c_int, vp, i64, vp]
+        L.kgo_route.restype = C.c_int
+        L.kgo_route.argtypes = [C.c_int, vp, i64, u32, vp]
+        L.kgo_split.argtypes = [vp, i64, u32, vp, vp]
+        L.kgo_row_hash.restype = C.c_int
+        L.kgo_row_hash.argtypes = [C.c_int, vp, i64, vp]
+        L.kgo_finalize.restype = u64
+        L.kgo_finalize.argtypes = [u64, u64]
+        L.kgo_join_partition.restype = u32
+        L.kgo_join_partition.argtypes = [u64, u64, u32, u32]
+        L.kgo_sbbf_bytes_for.restype = C.c_size_t
+        L.kgo_sbbf_bytes_for.argtypes = [u64, C.c_double]
+        L.kgo_sbbf_blocks_for_bytes.restype = C.c_size_t
+        L.kgo_sbbf_blocks_for_bytes.argtypes = [C.c_size_t]
+        L.kgo_sbbf_insert.argtypes = [vp, C.c_size_t, u64]
+        L.kgo_sbbf_check.restype = C.c_int
+        L.kgo_sbbf_check.argtypes = [vp, C.c_size_t, u64]
+        L.kgo_sbbf_expected_fpp.restype = C.c_double
+        L.kgo_sbbf_expected_fpp.argtypes = [C.c_size_t, u64]
+        L.kgo_sbbf_build.restype = C.c_int
+        L.kgo_sbbf_build.argtypes = [vp, i64, C.c_size_t, C.c_double, C.c_size_t, C.c_double,
+                                     vp, vp]
+        L.kgo_mix64.restype = u64
+        L.kgo_mix64.argtypes = [u64]
+        L.kgo_rand.restype = u64
+        L.kgo_rand.argtypes = [u64, u64, u64]
+        L.kgo_below.restype = u64
+        L.kgo_below.argtypes = [u64, u64]
+        L.kgo_unit.restype = C.c_double
+        L.kgo_unit.argtypes = [u64]
+        L.kgo_perm.restype = u64
+        L.kgo_perm.argtypes = [u64, u64, u64]
+        L.kgo_zipf_cdf.argtypes = [u64, C.c_double, vp]
+        L.kgo_zipf_rank.restype = u64
+        L.kgo_zipf_rank.argtypes = [vp, u64, C.c_double]
+        L.kgo_gen_join_build.argtypes = [u64, i64, i64, i64, vp, vp]
+        L.kgo_gen_join_probe.argtypes = [u64, i64, i64, i64, u32, vp, vp, vp]
+        L.kgo_gen_agg.argtypes = [u64, i64, i64, vp, u64, vp, vp, vp, vp, vp]
+        L.kgo_agg_new.restype = vp
+        L.kgo_agg_new.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int]
+        L.kgo_agg_update.restype = C.c_int
+        L.kgo_agg_update.argtypes = [vp, vp, vp, i64]
+        L.kgo_agg_merge.restype = C.c_int
+        L.kgo_agg_merge.argtypes = [vp, vp]
+        L.kgo_agg_groups.restype = i64
+        L.kgo_agg_groups.argtypes = [vp]
+        L.kgo_agg_finish.restype = C.c_int
+        L.kgo_agg_finish.argtypes = [vp, vp, vp, vp]
+        L.kgo_agg_result_type.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.kgo_agg_free.argtypes = [vp]
+        L.kgo_last_error.restype = C.c_char_p
+        L.kgo_avg_fast.restype = C.c_int
+        L.kgo_avg_fast.argtypes = [i64, i64, C.c_int, i64, C.c_int, C.c_int, vp, vp]
+        L.kgo_join.restype = i64
+        L.kgo_join.argtypes = [C.c_int, C.c_int, vp, vp, vp, i64, i64, C.c_int, i64, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Col:
+    """A host column: numpy values (+ optional bool validity), Arrow layout."""
+
+    def __init__(self, ctype, values, valid=None, precision=0, scale=0):
+        self.type = ctype
+        self.precision = precision
+        self.scale = scale
+        if ctype == T_BOOL:
+            v = np.asarray(values, dtype=bool)
+            self.length = len(v)
+            self.values = np.packbits(v, bitorder="little")
+        elif ctype == T_DECIMAL128:
+            v = np.ascontiguousarray(values)
+            if v.dtype != np.uint64 or v.ndim != 2:
+                v = dec128_words([int(x) for x in values])
+            self.length = v.shape[0]
+            self.values = v
+        else:
+            self.values = np.ascontiguousarray(values, dtype=_NP_OF[ctype])
+            self.length = len(self.values)
+        self.valid = None if valid is None else np.asarray(valid, dtype=bool)
+        self.bitmap = (None if self.valid is None
+                       else np.packbits(self.valid, bitorder="little"))
+
+    def c(self) -> _Col:
+        return _Col(self.type, self.precision, self.scale, 0, self.length, 0,
+                    _ptr(self.values), _ptr(self.bitmap))
+
+
+def dec128_words(ints) -> np.ndarray:
+    out = np.zeros((len(ints), 2), dtype=np.uint64)
+    for i, x in enumerate(ints):
+        u = x & ((1 << 128) - 1)
+        out[i, 0] = u & 0xFFFFFFFFFFFFFFFF
+        out[i, 1] = u >> 64
+    return out
+
+
+def _cols(cols):
+    arr = (_Col * max(1, len(cols)))()
+    for i, c in enumerate(cols):
+        arr[i] = c.c()
+    return arr
+
+
+def error() -> str:
+    return lib().kgo_last_error().decode()
+
+
+# --- hashing -------------------------------------------------------------
+def xxh64(data: bytes, seed: int = 0) -> int:
+    return lib().kgo_xxh64(data, len(data), seed)
+
+
+def xxh64_fixed(value: int, width: int) -> int:
+    u = value & ((1 << 128) - 1)
+    return lib().kgo_xxh64_fixed(u & 0xFFFFFFFFFFFFFFFF, u >> 64, width)
+
+
+def hash_columns(cols) -> np.ndarray:
+    n = cols[0].length
+    out = np.zeros(n, dtype=np.uint64)
+    ca = _cols(cols)
+    if lib().kgo_hash_columns(len(cols), C.addressof(ca), n, _ptr(out)) != 0:
+        raise RuntimeError(error())
+    return out
+
+
+def route(cols, partitions: int) -> np.ndarray:
+    n = cols[0].length if cols else 0
+    out = np.zeros(n, dtype=np.uint32)
+    ca = _cols(cols)
+    if lib().kgo_route(len(cols), C.addressof(ca), n, partitions, _ptr(out)) != 0:
+        raise RuntimeError(error())
+    return out
+
+
+def split(pids: np.ndarray, partitions: int):
+    pids = np.ascontiguousarray(pids, dtype=np.uint32)
+    offs = np.zeros(partitions + 1, dtype=np.int64)
+    rows = np.zeros(len(pids), dtype=np.int64)
+    lib().kgo_split(_ptr(pids), len(pids), partitions, _ptr(offs), _ptr(rows))
+    return offs, rows
+
+
+def row_hash(cols) -> np.ndarray:
+    n = cols[0].length
+    out = np.zeros(n, dtype=np.uint64)
+    ca = _cols(cols)
+    lib().kgo_row_hash(len(cols), C.addressof(ca), n, _ptr(out))
+    return out
+
+
+def key_group_range(p: int, n: int):
+    s, e = C.c_uint32(), C.c_uint32()
+    lib().kgo_key_group_range(p, n, C.byref(s), C.byref(e))
+    return s.value, e.value
+
+
+# --- SBBF ----------------------------------------------------------------
+def sbbf_build(hashes: np.ndarray, nbytes: int = 0, fpp: float = 0.01,
+               max_bytes: int = 8 << 20, max_fpp: float = 0.05):
+    hashes = np.ascontiguousarray(hashes, dtype=np.uint64)
+    b = nbytes or lib().kgo_sbbf_bytes_for(len(hashes), fpp)
+    nb = lib().kgo_sbbf_blocks_for_bytes(min(b, max_bytes))
+    words = np.zeros(max(nb, 1) * 8, dtype=np.uint32)
+    nbo = C.c_size_t()
+    kept = lib().kgo_sbbf_build(_ptr(hashes), len(hashes), nbytes, fpp, max_bytes, max_fpp,
+                                _ptr(words), C.byref(nbo))
+    return bool(kept), words[: nbo.value * 8]
+
+
+def sbbf_check(words: np.ndarray, h: int) -> bool:
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    return bool(lib().kgo_sbbf_check(_ptr(words), len(words) // 8, h))
+
+
+# --- generators ----------------------------------------------------------
+def zipf_cdf(n: int, s: float) -> np.ndarray:
+    cdf = np.zeros(n, dtype=np.float64)
+    lib().kgo_zipf_cdf(n, s, _ptr(cdf))
+    return cdf
+
+
+def gen_join_build(seed, lo, hi, n_total, with_val=True):
+    key = np.zeros(hi - lo, dtype=np.int64)
+    val = np.zeros(hi - lo, dtype=np.int64) if with_val else None
+    lib().kgo_gen_join_build(seed, lo, hi, n_total, _ptr(key), _ptr(val))
+    return key, val
+
+
+def gen_join_probe(seed, lo, hi, key_range, null_permille=10):
+    fk = np.zeros(hi - lo, dtype=np.int64)
+    ok = np.zeros(hi - lo, dtype=np.uint8)
+    amt = np.zeros(hi - lo, dtype=np.int64)
+    lib().kgo_gen_join_probe(seed, lo, hi, key_range, null_permille, _ptr(fk), _ptr(ok), _ptr(amt))
+    return fk, ok.astype(bool), amt
+
+
+def gen_agg(seed, lo, hi, cdf, ndistinct):
+    n = hi - lo
+    key = np.zeros(n, dtype=np.int64)
+    dec = np.zeros(n, dtype=np.int64)
+    f = np.zeros(n, dtype=np.float64)
+    i32 = np.zeros(n, dtype=np.int32)
+    k8 = np.zeros(n, dtype=np.int8)
+    lib().kgo_gen_agg(seed, lo, hi, _ptr(cdf), ndistinct, _ptr(key), _ptr(dec), _ptr(f),
+                      _ptr(i32), _ptr(k8))
+    return key, dec, f, i32, k8
+
+
+# --- aggregation ---------------------------------------------------------
+class Agg:
+    """The oracle's hash aggregate (Sink::sink = update, combine = merge,
+    finalize = finish)."""
+
+    def __init__(self, key_type, input_types, aggs, ansi=False):
+        self.key_type = key_type
+        kc = _Col(key_type[0], key_type[1], key_type[2], 0, 0, 0, None, None) if key_type else None
+        self._kc = kc
+        its = (_Col * max(1, len(input_types)))()
+        for i, (t, p, s) in enumerate(input_types):
+            its[i] = _Col(t, p, s, 0, 0, 0, None, None)
+        specs = (_AggSpec * max(1, len(aggs)))()
+        for i, (f, inp) in enumerate(aggs):
+            specs[i] = _AggSpec(f, inp)
+        self.naggs = len(aggs)
+        self.h = lib().kgo_agg_new(C.addressof(kc) if kc is not None else None, len(input_types),
+                                   C.addressof(its), len(aggs), C.addressof(specs), int(ansi))
+        if not self.h:
+            raise RuntimeError(error())
+
+    def update(self, key: Col | None, inputs):
+        n = key.length if key is not None else (inputs[0].length if inputs else 0)
+        kc = key.c() if key is not None else None
+        ca = _cols(inputs)
+        self._keep = (key, inputs)
+        rc = lib().kgo_agg_update(self.h, C.addressof(kc) if kc is not None else None,
+                                  C.addressof(ca), n)
+        if rc != 0:
+            raise ArithmeticError(error())
+
+    def merge(self, other: "Agg"):
+        if lib().kgo_agg_merge(self.h, other.h) != 0:
+            raise ArithmeticError(error())
+
+    def finish(self):
+        """Returns (keys, key_valid, [(values, valid, (type, p, s))...]).
+        Decimal results are Python ints."""
+        L = lib()
+        g = L.kgo_agg_groups(self.h)
+        keys = np.zeros(max(g, 1), dtype=np.int64)
+        kv = np.zeros(max(g, 1), dtype=np.uint8)
+        outs = (_Out * max(1, self.naggs))()
+        bufs = []
+        for i in range(self.naggs):
+            t, p, s = C.c_int32(), C.c_int32(), C.c_int32()
+            L.kgo_agg_result_type(self.h, i, C.byref(t), C.byref(p), C.byref(s))
+            lo = np.zeros(max(g, 1), dtype=np.int64)
+            hi = np.zeros(max(g, 1), dtype=np.int64)
+            f = np.zeros(max(g, 1), dtype=np.float64)
+            v = np.zeros(max(g, 1), dtype=np.uint8)
+            outs[i] = _Out(t.value, p.value, s.value, 0, _ptr(lo), _ptr(hi), _ptr(f), _ptr(v))
+            bufs.append((lo, hi, f, v, (t.value, p.value, s.value)))
+        if L.kgo_agg_finish(self.h, _ptr(keys), _ptr(kv), C.addressof(outs)) != 0:
+            raise ArithmeticError(error())
+        res = []
+        for lo, hi, f, v, (t, p, s) in bufs:
+            if t == T_FLOAT64:
+                vals = f[:g].copy()
+            elif t in (T_DECIMAL64, T_DECIMAL128):
+                vals = [((int(h) << 64) | (int(l) & 0xFFFFFFFFFFFFFFFF)) for l, h in zip(lo[:g], hi[:g])]
+                vals = [x - (1 << 128) if x >= (1 << 127) else x for x in vals]
+            else:
+                vals = lo[:g].copy()
+            res.append((vals, v[:g].astype(bool), (t, p, s)))
+        return keys[:g].copy(), kv[:g].astype(bool), res
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().kgo_agg_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def join(join_type, probe_keys, build_keys, null_safe=None, nested=False, cap=None):
+    n_p = probe_keys[0].length
+    n_b = build_keys[0].length
+    if cap is None:
+        cap = n_p * max(n_b, 1) if nested else max(16, 4 * (n_p + n_b))
+    op = np.zeros(cap, dtype=np.int64)
+    ob = np.zeros(cap, dtype=np.int64)
+    pk, bk = _cols(probe_keys), _cols(build_keys)
+    ns = None if null_safe is None else np.asarray(null_safe, dtype=np.uint8)
+    n = lib().kgo_join(join_type, len(probe_keys), C.addressof(pk), C.addressof(bk), _ptr(ns),
+                       n_p, n_b, int(nested), cap, _ptr(op), _ptr(ob))
+    if n < 0:
+        return join(join_type, probe_keys, build_keys, null_safe, nested, cap * 4)
+    return op[:n].copy(), ob[:n].copy()
