@@ -1,0 +1,171 @@
+"""Benchmark of the MI355X hash-aggregation hot path.
+
+Workload (one step): GROUP BY an int64 key drawn Zipf(s=1.1) over 10M
+distinct values, 200M rows, with a decimal(12,2) column (int64 cents in
+[1, 1e7)), a float64 column in [0, 1) and an int32 column in [0, 50];
+aggregates COUNT(*) and SUM, MIN, MAX, AVG of each value column (13
+aggregates).  Inputs are generated once into HBM (synthetic, seeded); a step
+creates the aggregate, sinks the 200M rows, finalizes and materializes the
+result groups in HBM.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--rows N]
+                    [--key int64|int8] [--cpu-rows N] [--no-cpu]
+
+Prints one JSON line (rank 0).  Only one GPU is used; --gpus above 1 exits.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "krishiv-engine_amd"))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--rows", type=int, default=200_000_000)
+    p.add_argument("--distinct", type=int, default=10_000_000)
+    p.add_argument("--key", choices=["int64", "int8"], default="int64")
+    p.add_argument("--cpu-rows", type=int, default=20_000_000)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(args, aggs):
+    """The oracle (single-threaded C restatement of the reference's CPU
+    aggregate) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import kgo
+    n = args.cpu_rows
+    cdf = kgo.zipf_cdf(args.distinct, 1.1)
+    key, dec, f, i32, k8 = kgo.gen_agg(3, 0, n, cdf, args.distinct)
+    kcol = kgo.Col(kgo.T_INT64, key) if args.key == "int64" else kgo.Col(kgo.T_INT8, k8)
+    types = [(kgo.T_DECIMAL64, 12, 2), (kgo.T_FLOAT64, 0, 0), (kgo.T_INT32, 0, 0)]
+    cols = [kgo.Col(kgo.T_DECIMAL64, dec, precision=12, scale=2), kgo.Col(kgo.T_FLOAT64, f),
+            kgo.Col(kgo.T_INT32, i32)]
+    t0 = time.perf_counter()
+    a = kgo.Agg((kcol.type, 0, 0), types, aggs)
+    a.update(kcol, cols)
+    a.finish()
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"{n} rows of the same generator (first {n} rows), oracle C aggregate, "
+                      f"single thread, {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import kgpu
+    from kgpu import Col, DeviceBuffer
+    from kgpu import abi as A
+    import ctypes as C
+
+    ctx = kgpu.Context(0)
+    n, nd = args.rows, args.distinct
+    # inputs resident in HBM
+    cdf = np.zeros(nd)
+    A.check(kgpu.lib().kg_zipf_cdf(nd, 1.1, cdf.ctypes.data))
+    cdf_d = DeviceBuffer.from_numpy(ctx, cdf)
+    key_d = DeviceBuffer(ctx, n * 8)
+    k8_d = DeviceBuffer(ctx, n)
+    dec_d = DeviceBuffer(ctx, n * 8)
+    f_d = DeviceBuffer(ctx, n * 8)
+    i32_d = DeviceBuffer(ctx, n * 4)
+    A.check(kgpu.lib().kg_gen_agg(ctx.h, 3, 0, n, cdf_d.ptr, nd,
+                                  key_d.ptr if args.key == "int64" else None, dec_d.ptr, f_d.ptr,
+                                  i32_d.ptr, k8_d.ptr if args.key == "int8" else None))
+    if args.key == "int64":
+        kcol = Col(A.T_INT64, key_d, length=n)
+        key_bytes = 8
+    else:
+        kcol = Col(A.T_INT8, k8_d, length=n)
+        key_bytes = 1
+    cols = [Col(A.T_DECIMAL64, dec_d, precision=12, scale=2, length=n),
+            Col(A.T_FLOAT64, f_d, length=n), Col(A.T_INT32, i32_d, length=n)]
+    types = [(A.T_DECIMAL64, 12, 2), (A.T_FLOAT64, 0, 0), (A.T_INT32, 0, 0)]
+    aggs = [(A.AGG_COUNT_STAR, 0)]
+    for c in range(3):
+        aggs += [(A.AGG_SUM, c), (A.AGG_MIN, c), (A.AGG_MAX, c), (A.AGG_AVG, c)]
+    expected = nd if args.key == "int64" else 4
+
+    groups = [0]
+
+    def step():
+        g = kgpu.HashAggregate(ctx, (kcol.type, 0, 0), types, aggs, expected_groups=expected)
+        g.sink(kcol, cols, n)
+        g.finalize()
+        r = g.result()
+        groups[0] = r.num_rows
+        del r
+        g.close()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    kgpu.prof_reset()
+    kgpu.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kgpu.prof_enable(False)
+    upd_ms, upd_n = kgpu.prof_get("agg_update")
+    out_ms, out_n = kgpu.prof_get("agg_output")
+    ms_per_step = dt * 1e3 / args.steps
+    rows_per_s = n * args.steps / dt
+    in_bytes = n * (key_bytes + 8 + 8 + 4)  # every input column read once
+    # the update kernel's algorithmic bytes per launch: its rows x 21-28 B;
+    # summed over a step's launches that is in_bytes
+    upd_avg_ms = upd_ms / max(upd_n, 1)
+    launches_per_step = upd_n / args.steps
+    bytes_per_launch = in_bytes / max(launches_per_step, 1)
+    achieved = bytes_per_launch / (upd_avg_ms / 1e3) if upd_ms > 0 else None
+    out = {
+        "metric": "hash aggregation rows/s (200M rows, Zipf 1.1 over 10M int64 keys, 13 aggregates)",
+        "value": rows_per_s,
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64/decimal(12,2)/f64/int32",
+        "data": "synthetic (seeded counter-based generator, generated in HBM)",
+        "config": {"workload": f"hash aggregation, {n} rows, key {args.key} "
+                               f"{'Zipf(1.1) over %d' % nd if args.key == 'int64' else '4 values'}, "
+                               "COUNT(*) + SUM/MIN/MAX/AVG of decimal(12,2), float64, int32",
+                   "groups": groups[0]},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+                     "kernel": "k_agg_update", "kernel_avg_ms": upd_avg_ms,
+                     "launches_per_step": launches_per_step,
+                     "algorithmic_bytes_per_step": in_bytes},
+        "step_hbm_frac": (in_bytes / (ms_per_step / 1e3)) / HBM_PEAK,
+        "output_kernel_ms": out_ms / max(out_n, 1),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, [(1, 0)] + [(f, c) for c in range(3)
+                                                             for f in (3, 4, 5, 6)])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,262 @@
+/*
+ * kgpu.h -- C ABI of the MI355X (gfx950) execution path for the Krishiv
+ * engine's vectorized hash-join / hash-aggregate / runtime-Bloom /
+ * hash-repartition operators.
+ *
+ * The reference engine is Rust and has no FFI for this path; these entry
+ * points are what a thin `extern "C"` crate binds.  Each
+ * declaration names the reference interface it replaces (paths relative to
+ * the reference repository root).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every function returning `int` returns KG_OK (0) or a KG_ERR_* code;
+ *    kg_last_error() then holds "<ERROR_CLASS>: <message>" for the calling
+ *    thread (the reference's Spark error classes, aggregate/mod.rs:1150-1192).
+ *    No C++ exception or abort crosses this boundary.
+ *  - Columns are Arrow layout (kg_column): values buffer, optional LSB
+ *    validity bitmap (NULL = no NULLs), an element offset applied to both.
+ *    KG_COL_DEVICE marks device (HBM) pointers; otherwise host memory, which
+ *    is copied to HBM before the call returns (borrowed input, as
+ *    Sink::sink borrows its Batch, crates/krishiv-exec/src/operator.rs:346-380).
+ *  - An operator handle is used by one thread at a time; distinct handles may
+ *    be used concurrently.  Each handle owns one HIP stream (default
+ *    priority).  Calls enqueue device work and return; calls that hand data
+ *    back to the host synchronize.
+ */
+#ifndef KGPU_H
+#define KGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KG_ABI_VERSION 1
+
+/* ---- status codes (Spark error classes in kg_last_error) -------------- */
+#define KG_OK 0
+#define KG_ERR_UNSUPPORTED 1  /* KRISHIV_UNSUPPORTED_FEATURE */
+#define KG_ERR_OVERFLOW 2     /* ARITHMETIC_OVERFLOW */
+#define KG_ERR_OUT_OF_RANGE 3 /* NUMERIC_VALUE_OUT_OF_RANGE.WITH_SUGGESTION */
+#define KG_ERR_MEMORY 4       /* UNABLE_TO_ACQUIRE_MEMORY (HBM budget exhausted) */
+#define KG_ERR_INTERNAL 5     /* INTERNAL_ERROR (bad argument, broken invariant) */
+#define KG_ERR_DEVICE 6       /* INTERNAL_ERROR raised by the HIP runtime */
+
+/* ---- column types (same numbering as the oracle's KGO_T_*) ------------- */
+#define KG_T_BOOL 1
+#define KG_T_INT8 2
+#define KG_T_INT16 3
+#define KG_T_INT32 4
+#define KG_T_INT64 5
+#define KG_T_UINT8 6
+#define KG_T_UINT16 7
+#define KG_T_UINT32 8
+#define KG_T_UINT64 9
+#define KG_T_FLOAT32 10
+#define KG_T_FLOAT64 11
+#define KG_T_DATE32 12
+#define KG_T_TIMESTAMP 13 /* any 64-bit temporal (date64, timestamp, duration) */
+#define KG_T_DECIMAL64 14
+#define KG_T_DECIMAL128 15
+
+#define KG_COL_DEVICE 1u /* values/validity are device pointers */
+
+typedef struct kg_column {
+  int32_t type;
+  int32_t precision; /* decimals */
+  int32_t scale;     /* decimals */
+  uint32_t flags;    /* KG_COL_* */
+  int64_t length;
+  int64_t offset; /* element offset into values and validity */
+  const void *values;
+  const uint8_t *validity; /* LSB bitmap, NULL = all valid */
+} kg_column;
+
+/* ---- errors and context --------------------------------------------- */
+const char *kg_last_error(void);
+int kg_abi_version(void);
+
+typedef struct kg_ctx kg_ctx;
+/* One GPU and an HBM budget for every operator of the context (0 = 90 % of
+ * free HBM): the analogue of the memory pool's reservations
+ * (crates/krishiv-memory/src/reservation.rs:12,135); past it operators fail
+ * with KG_ERR_MEMORY. */
+int kg_ctx_create(int device, uint64_t hbm_budget_bytes, kg_ctx **out);
+void kg_ctx_destroy(kg_ctx *ctx);
+/* Bytes of HBM currently held by the context's operators. */
+uint64_t kg_ctx_hbm_in_use(const kg_ctx *ctx);
+
+/* Plain device memory, for callers that keep inputs resident in HBM. */
+int kg_dev_alloc(kg_ctx *ctx, uint64_t bytes, void **out);
+int kg_dev_free(kg_ctx *ctx, void *p);
+int kg_memcpy_h2d(kg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int kg_memcpy_d2h(kg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int kg_ctx_synchronize(kg_ctx *ctx);
+
+/* ---- device batches (operator output) ----------------------------------
+ * A device-resident record batch owned by the library: a join's output,
+ * an aggregate's result, a repartition's partitions.  Its columns can be
+ * passed on to another operator as KG_COL_DEVICE columns without a copy. */
+typedef struct kg_batch kg_batch;
+int64_t kg_batch_num_rows(const kg_batch *b);
+int32_t kg_batch_num_columns(const kg_batch *b);
+/* A device view of column i (valid while the batch lives). */
+int kg_batch_column(const kg_batch *b, int32_t i, kg_column *out);
+/* Copies column i to host buffers: values (width x rows bytes; BOOL as one
+ * byte per row), validity as one byte per row (may be NULL). */
+int kg_batch_column_to_host(const kg_batch *b, int32_t i, void *values, uint8_t *valid_bytes);
+void kg_batch_release(kg_batch *b);
+
+/* ---- routing hash and hash repartition (exchange) ---------------------
+ * kg_hash_columns replaces routing::hash_columns
+ * (crates/krishiv-kernels/src/hash/routing.rs:258-282): H_kg v1 of each row
+ * (XXH64 seed 0 of the normalized key bytes, folded over several columns).
+ * out_hashes: device or host (out_flags & KG_COL_DEVICE). */
+int kg_hash_columns(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
+                    uint64_t *out_hashes, uint32_t out_flags);
+/* exchange::route for Partitioning::Hash (crates/krishiv-exec/src/exchange/mod.rs:159-212,
+ * physical/mod.rs:248-262): partition_of(H_kg(keys), partitions) per row. */
+int kg_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrows,
+                 uint32_t partitions, uint32_t *out_pids, uint32_t out_flags);
+/* ExchangeSink::sink's route + split (exchange/mod.rs:447-501): the rows of
+ * `cols` regrouped by partition, stable within a partition.  The output batch
+ * holds the permuted columns; out_offsets (host, partitions+1 entries) gives
+ * each partition's row range. */
+int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
+                   int32_t nkeys, const int32_t *key_cols, uint32_t partitions,
+                   int64_t *out_offsets, kg_batch **out);
+
+/* ---- hash aggregation ---------------------------------------------------
+ * HashAggregate (crates/krishiv-exec/src/aggregate/sink.rs:222,1459-1585) and
+ * PartialAggregate (aggregate/partial.rs): a Sink (kg_agg_sink*, called per
+ * batch) then finalize, then a Source (kg_agg_result).  Spark semantics and
+ * result types as aggregate/mod.rs:588-775. */
+#define KG_AGG_COUNT_STAR 1
+#define KG_AGG_COUNT 2
+#define KG_AGG_SUM 3
+#define KG_AGG_MIN 4
+#define KG_AGG_MAX 5
+#define KG_AGG_AVG 6
+
+#define KG_AGG_ANSI 1u /* spark.sql.ansi.enabled: overflow raises */
+
+typedef struct kg_agg_spec {
+  int32_t func;  /* KG_AGG_* */
+  int32_t input; /* input column index (ignored by COUNT_STAR) */
+} kg_agg_spec;
+
+typedef struct kg_hash_agg kg_hash_agg;
+/* key_type: the grouping column's type (type/precision/scale used; NULL for a
+ * global aggregate).  input_types: the value columns' types.
+ * expected_groups sizes the table (it grows if exceeded). */
+int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
+                  const kg_column *input_types, int32_t naggs, const kg_agg_spec *aggs,
+                  uint32_t flags, uint64_t expected_groups, kg_hash_agg **out);
+/* Sink::sink of raw rows. */
+int kg_agg_sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, int64_t nrows);
+/* Sink::finalize: checks overflow, fixes the group count. */
+int kg_agg_finalize(kg_hash_agg *a);
+/* Result type of aggregate i (type, precision, scale). */
+int kg_agg_result_type(const kg_hash_agg *a, int32_t i, kg_column *out_type);
+/* Source: the final values as a device batch [key, agg 0, agg 1, ...]. */
+int kg_agg_result(kg_hash_agg *a, kg_batch **out);
+void kg_agg_destroy(kg_hash_agg *a);
+
+/* ---- runtime Bloom filter (split-block, Parquet layout) ----------------
+ * RuntimeBloom (crates/krishiv-io/src/runtime_filter/bloom.rs:83-320) built
+ * as KeyFilterBuilder::finish_with/bloom() (runtime_filter/filter.rs:533-624)
+ * does: XXH64 seed 0 of the normalized key bytes, `bytes` = bytes_for(items,
+ * fpp) (or the planned size), kept only if expected_fpp(items) <= max_fpp. */
+typedef struct kg_bloom kg_bloom;
+int kg_bloom_create(kg_ctx *ctx, uint64_t nbytes, kg_bloom **out);
+/* bytes_for (bloom.rs:96-110). */
+uint64_t kg_bloom_bytes_for(uint64_t items, double fpp);
+/* Inserts the non-NULL keys of `key` (rows where `select` is set, if given:
+ * the build side's kept rows). */
+int kg_bloom_insert(kg_bloom *b, const kg_column *key, const kg_column *select, int64_t nrows);
+/* expected_fpp (bloom.rs:270-308) after `items` insertions. */
+double kg_bloom_expected_fpp(const kg_bloom *b, uint64_t items);
+/* The Parquet-layout bit array (bloom.rs:311-320) to host memory. */
+int kg_bloom_export(kg_bloom *b, uint8_t *bytes, uint64_t *len);
+/* check_batch (bloom.rs:211-247) fused with the scan row stage's selection
+ * (operators/runtime_filter.rs): writes the ids of rows that may match
+ * (NULL keys never pass) to out_rows (device) and their count to
+ * *out_count (host). */
+int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *out_rows,
+                   int64_t *out_count);
+void kg_bloom_destroy(kg_bloom *b);
+
+/* ---- hash join ----------------------------------------------------------
+ * HashJoin (crates/krishiv-exec/src/join/mod.rs:1014-1211): the build side is
+ * a Sink (kg_join_build_sink per batch, then kg_join_build_finalize,
+ * join/build.rs:1225-1695); the probe side an Operator (kg_join_probe per
+ * batch, join/probe.rs:784-948).  Output columns: the probe columns, then
+ * (inner, left outer) every build column (join/mod.rs:793-819). */
+#define KG_JOIN_INNER 0
+#define KG_JOIN_LEFT_OUTER 1
+#define KG_JOIN_LEFT_SEMI 2
+#define KG_JOIN_LEFT_ANTI 3
+
+typedef struct kg_join_spec {
+  int32_t join_type;
+  int32_t num_keys;       /* 1 or 2 */
+  int32_t probe_keys[4];  /* key column indices of the probe batch */
+  int32_t build_keys[4];  /* key column indices of the build batch */
+  uint8_t null_safe[4];   /* `<=>` keys */
+} kg_join_spec;
+
+typedef struct kg_hash_join kg_hash_join;
+int kg_join_create(kg_ctx *ctx, const kg_join_spec *spec, uint64_t expected_build_rows,
+                   kg_hash_join **out);
+int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows);
+int kg_join_build_finalize(kg_hash_join *j);
+/* Rows the build kept (NULL `=` keys dropped). */
+int64_t kg_join_build_rows(const kg_hash_join *j);
+int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                  kg_batch **out);
+void kg_join_destroy(kg_hash_join *j);
+
+/* ---- join fused with GROUP BY the build key -----------------------------
+ * The plan Aggregate(GROUP BY build key; SUM/COUNT/MIN/MAX/AVG of probe
+ * columns) over an inner HashJoin (the shape of SELECT key, SUM(amt),
+ * COUNT(*) FROM build JOIN probe ON key = fk GROUP BY key): the aggregate
+ * state lives in the build table's slots, so no join pair is materialized.
+ * Equal to kg_join_probe followed by kg_agg_sink, without the pairs. */
+int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_types,
+                      int32_t naggs, const kg_agg_spec *aggs, uint32_t flags);
+int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                     const int32_t *input_cols);
+int kg_joinagg_result(kg_hash_join *j, kg_batch **out);
+
+/* ---- synthetic data on the device (bench and tests) --------------------
+ * Counter-based generators; the integer ones are bit-identical to the
+ * CPU generators used by the tests. */
+int kg_gen_join_build(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, int64_t n_total,
+                      int64_t *key_dev, int64_t *val_dev);
+int kg_gen_join_probe(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, int64_t key_range,
+                      uint32_t null_permille, int64_t *fk_dev, uint8_t *fk_validity_dev,
+                      int64_t *amt_dev);
+/* The Zipf(s) cumulative distribution over n ranks (host, double). */
+int kg_zipf_cdf(uint64_t n, double s, double *cdf_host);
+int kg_gen_agg(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, const double *cdf_dev,
+               uint64_t ndistinct, int64_t *key_dev, int64_t *dec_dev, double *f_dev,
+               int32_t *i32_dev, int8_t *k8_dev);
+
+/* ---- profiling ----------------------------------------------------------
+ * Device time of the hot kernels, from HIP events recorded around each launch
+ * on the stream it runs on (the GPU analogue of the operators' EXPLAIN
+ * ANALYZE timers, crates/krishiv-exec/src/join/mod.rs:148-167).  Off by
+ * default. */
+void kg_prof_enable(int on);
+void kg_prof_reset(void);
+/* Total device milliseconds and launch count of a kernel family
+ * ("agg_update", "agg_flush", "join_build", "join_probe", "joinagg_probe",
+ * "bloom_insert", "bloom_probe", "hash_partition", "scatter"). */
+int kg_prof_get(const char *family, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

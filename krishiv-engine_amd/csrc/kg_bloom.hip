@@ -1,0 +1,277 @@
+// kg_bloom.hip -- the runtime Bloom filter: the Parquet split-block Bloom
+// filter (RuntimeBloom, crates/krishiv-io/src/runtime_filter/bloom.rs:83-320)
+// over XXH64 seed 0 of the normalized key bytes (runtime_filter/hash.rs:56-131),
+// built as KeyFilterBuilder::finish_with/bloom() (runtime_filter/filter.rs:533-624)
+// and probed as check_batch (bloom.rs:211-247) by the scan's row stage.
+//
+// Build: one thread per key sets one bit in each of the block's eight words
+// with atomicOr (OR is order-free, so the bit array is the CPU's byte for
+// byte).  Probe: pass 1 streams the key column, tests the 32-byte block
+// (L2/MALL resident for filters of a few MB) and writes a 64-row ballot word
+// per wave plus a per-tile count; pass 2 scans the counts and expands the
+// bitmap into the surviving row ids in input order.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+#include "kg_common.hpp"
+#include "kg_device.hpp"
+
+namespace kg {
+
+constexpr int kProbeBlock = 256;
+constexpr int kProbeTile = kProbeBlock * 32;  // rows per workgroup tile
+
+__device__ __forceinline__ uint64_t sbbf_block(uint64_t h, uint64_t nblocks) {
+  return ((h >> 32) * nblocks) >> 32;
+}
+
+__device__ __forceinline__ bool select_row(const DCol *sel, int64_t i) {
+  if (!sel) return true;
+  return dev::valid(*sel, i) && dev::load_i64(*sel, i) != 0;
+}
+
+__global__ void k_bloom_insert(DCol key, DCol sel, int has_sel, int64_t n, uint32_t *words,
+                               uint64_t nblocks, unsigned long long *items) {
+  uint64_t mine = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (!dev::valid(key, i) || !select_row(has_sel ? &sel : nullptr, i)) continue;
+    uint64_t h = dev::value_hash(key, i);
+    uint32_t *b = words + sbbf_block(h, nblocks) * 8;
+    uint32_t k = (uint32_t)h;
+#pragma unroll
+    for (int w = 0; w < 8; w++) atomicOr(b + w, 1u << ((k * dev::sbbf_salt(w)) >> 27));
+    mine++;
+  }
+  // one atomic per wave for the item count
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
+  if ((threadIdx.x & 63) == 0 && mine) atomicAdd(items, (unsigned long long)mine);
+}
+
+__device__ __forceinline__ bool sbbf_check(const uint32_t *words, uint64_t nblocks, uint64_t h) {
+  const uint4 *b = reinterpret_cast<const uint4 *>(words + sbbf_block(h, nblocks) * 8);
+  uint4 x = b[0], y = b[1];
+  uint32_t k = (uint32_t)h;
+  uint32_t miss = 0;
+  miss |= (1u << ((k * dev::sbbf_salt(0)) >> 27)) & ~x.x;
+  miss |= (1u << ((k * dev::sbbf_salt(1)) >> 27)) & ~x.y;
+  miss |= (1u << ((k * dev::sbbf_salt(2)) >> 27)) & ~x.z;
+  miss |= (1u << ((k * dev::sbbf_salt(3)) >> 27)) & ~x.w;
+  miss |= (1u << ((k * dev::sbbf_salt(4)) >> 27)) & ~y.x;
+  miss |= (1u << ((k * dev::sbbf_salt(5)) >> 27)) & ~y.y;
+  miss |= (1u << ((k * dev::sbbf_salt(6)) >> 27)) & ~y.z;
+  miss |= (1u << ((k * dev::sbbf_salt(7)) >> 27)) & ~y.w;
+  return miss == 0;
+}
+
+// Pass 1: tile t covers rows [t*kProbeTile, (t+1)*kProbeTile); bitmap word
+// (row / 64) holds the answers of 64 consecutive rows.
+__global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_bits(DCol key, int64_t n,
+                                                                  const uint32_t *words,
+                                                                  uint64_t nblocks,
+                                                                  uint64_t *bitmap,
+                                                                  uint32_t *tile_count) {
+  __shared__ uint32_t s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kProbeTile;
+  uint32_t cnt = 0;
+  const int lane = threadIdx.x & 63;
+  for (int k = 0; k < kProbeTile / kProbeBlock; k++) {
+    int64_t i = base + (int64_t)k * kProbeBlock + threadIdx.x;
+    bool pass = false;
+    if (i < n && dev::valid(key, i)) pass = sbbf_check(words, nblocks, dev::value_hash(key, i));
+    uint64_t m = __ballot(pass);
+    int64_t w0 = base + (int64_t)k * kProbeBlock + (threadIdx.x & ~63);
+    if (lane == 0 && w0 < n) {
+      bitmap[w0 >> 6] = m;
+      cnt += (uint32_t)__popcll(m);
+    }
+  }
+  if (lane == 0 && cnt) atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_count[blockIdx.x] = s_cnt;
+}
+
+// Pass 2: thread t of a tile owns 32 consecutive rows (half a bitmap word).
+__global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_rows(const uint64_t *bitmap, int64_t n,
+                                                                  const uint64_t *tile_off,
+                                                                  int32_t *out_rows) {
+  typedef hipcub::BlockScan<uint32_t, kProbeBlock> Scan;
+  __shared__ typename Scan::TempStorage tmp;
+  int64_t base = (int64_t)blockIdx.x * kProbeTile;
+  int64_t r0 = base + (int64_t)threadIdx.x * 32;
+  uint32_t bits = 0;
+  if (r0 < n) {
+    uint64_t w = bitmap[r0 >> 6];
+    bits = (uint32_t)(((r0 >> 5) & 1) ? (w >> 32) : w);
+  }
+  uint32_t cnt = (uint32_t)__popc(bits), off = 0;
+  Scan(tmp).ExclusiveSum(cnt, off);
+  int64_t pos = (int64_t)tile_off[blockIdx.x] + off;
+  while (bits) {
+    int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    out_rows[pos++] = (int32_t)(r0 + b);
+  }
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+struct kg_bloom {
+  kg_ctx *ctx = nullptr;
+  hipStream_t s = nullptr;
+  uint64_t nblocks = 0;
+  DevBuf words, items;
+  DevBuf bitmap, tiles, tile_off, temp;  // probe scratch
+};
+
+extern "C" {
+
+uint64_t kg_bloom_bytes_for(uint64_t items, double fpp) {
+  // bytes_for (bloom.rs:96-110): Parquet's -8n / ln(1 - fpp^(1/8)) bits in
+  // whole 32-byte blocks, at least one
+  if (fpp < 1e-9) fpp = 1e-9;
+  if (fpp > 0.5) fpp = 0.5;
+  double bits = -8.0 * (double)items / std::log(1.0 - std::pow(fpp, 1.0 / 8.0));
+  uint64_t bytes = (uint64_t)std::ceil(bits / 8.0);
+  uint64_t blocks = (bytes + 31) / 32;
+  if (blocks < 1) blocks = 1;
+  return blocks * 32;
+}
+
+int kg_bloom_create(kg_ctx *ctx, uint64_t nbytes, kg_bloom **out) {
+  return guarded([&] {
+    *out = nullptr;
+    auto b = std::make_unique<kg_bloom>();
+    b->ctx = ctx;
+    b->nblocks = (nbytes + 31) / 32;
+    if (b->nblocks < 1) b->nblocks = 1;
+    if (b->nblocks > (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "Bloom filter of 2^32 blocks or more");
+    b->s = make_stream(ctx->device);
+    b->words.reset(ctx, b->nblocks * 32);
+    b->items.reset(ctx, 8);
+    KG_HIP(hipMemsetAsync(b->words.get(), 0, b->nblocks * 32, b->s));
+    KG_HIP(hipMemsetAsync(b->items.get(), 0, 8, b->s));
+    KG_HIP(hipStreamSynchronize(b->s));
+    *out = b.release();
+  });
+}
+
+int kg_bloom_insert(kg_bloom *b, const kg_column *key, const kg_column *select, int64_t nrows) {
+  return guarded([&] {
+    if (nrows <= 0) return;
+    check_column(*key, nrows, "Bloom key");
+    Staged st(b->ctx, b->s);
+    DCol k = st.add(*key, nrows);
+    DCol sel{};
+    if (select) {
+      check_column(*select, nrows, "Bloom selection");
+      sel = st.add(*select, nrows);
+    }
+    {
+      ProfScope prof("bloom_insert", b->s);
+      k_bloom_insert<<<stream_grid(nrows, 256), 256, 0, b->s>>>(
+          k, sel, select != nullptr, nrows, b->words.get<uint32_t>(), b->nblocks,
+          b->items.get<unsigned long long>());
+      KG_HIP(hipGetLastError());
+    }
+    if (st.staged_host()) KG_HIP(hipStreamSynchronize(b->s));
+  });
+}
+
+double kg_bloom_expected_fpp(const kg_bloom *b, uint64_t items) {
+  // expected_fpp (bloom.rs:270-308): blocks receive Poisson(lambda) hashes
+  double lambda = (double)items / (double)b->nblocks;
+  if (lambda == 0.0) return 0.0;
+  if (lambda > 1024.0) return std::pow(1.0 - std::pow(31.0 / 32.0, lambda), 8);
+  double spread = 10.0 * std::sqrt(lambda) + 20.0;
+  double lo = std::floor(std::fmax(lambda - spread, 0.0)), hi = lambda + spread;
+  double ln_p = -lambda, k = 1.0;
+  while (k <= lo) {
+    ln_p += std::log(lambda) - std::log(k);
+    k += 1.0;
+  }
+  double total = 0.0;
+  k = lo;
+  while (k <= hi) {
+    double word = 1.0 - std::pow(31.0 / 32.0, k);
+    double w2 = word * word, w4 = w2 * w2;
+    total += std::exp(ln_p) * (w4 * w4);
+    k += 1.0;
+    ln_p += std::log(lambda) - std::log(k);
+  }
+  return total < 1.0 ? total : 1.0;
+}
+
+int kg_bloom_export(kg_bloom *b, uint8_t *bytes, uint64_t *len) {
+  return guarded([&] {
+    uint64_t need = b->nblocks * 32;
+    if (!bytes) {
+      *len = need;
+      return;
+    }
+    if (*len < need) fail(KG_ERR_INTERNAL, "export buffer too small");
+    KG_HIP(hipMemcpyAsync(bytes, b->words.get(), need, hipMemcpyDeviceToHost, b->s));
+    KG_HIP(hipStreamSynchronize(b->s));
+    *len = need;
+  });
+}
+
+int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *out_rows,
+                   int64_t *out_count) {
+  return guarded([&] {
+    *out_count = 0;
+    if (nrows <= 0) return;
+    if (nrows > INT32_MAX) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^31 rows or more");
+    check_column(*key, nrows, "probe key");
+    Staged st(b->ctx, b->s);
+    DCol k = st.add(*key, nrows);
+    int64_t tiles = ceil_div(nrows, kProbeTile);
+    b->bitmap.ensure(b->ctx, (uint64_t)ceil_div(nrows, 64) * 8 + 8);
+    b->tiles.ensure(b->ctx, (uint64_t)tiles * 4);
+    b->tile_off.ensure(b->ctx, (uint64_t)(tiles + 1) * 8);
+    {
+      ProfScope prof("bloom_probe", b->s);
+      k_bloom_probe_bits<<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
+          k, nrows, b->words.get<uint32_t>(), b->nblocks, b->bitmap.get<uint64_t>(),
+          b->tiles.get<uint32_t>());
+      KG_HIP(hipGetLastError());
+    }
+    size_t tb = 0;
+    KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, b->tiles.get<uint32_t>(),
+                                            b->tile_off.get<uint64_t>(), (int)tiles, b->s));
+    b->temp.ensure(b->ctx, tb + 16);
+    KG_HIP(hipcub::DeviceScan::ExclusiveSum(b->temp.get(), tb, b->tiles.get<uint32_t>(),
+                                            b->tile_off.get<uint64_t>(), (int)tiles, b->s));
+    {
+      ProfScope prof("bloom_compact", b->s);
+      k_bloom_probe_rows<<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
+          b->bitmap.get<uint64_t>(), nrows, b->tile_off.get<uint64_t>(), out_rows);
+      KG_HIP(hipGetLastError());
+    }
+    uint64_t last_off = 0;
+    uint32_t last_cnt = 0;
+    KG_HIP(hipMemcpyAsync(&last_off, b->tile_off.get<uint64_t>() + tiles - 1, 8,
+                          hipMemcpyDeviceToHost, b->s));
+    KG_HIP(hipMemcpyAsync(&last_cnt, b->tiles.get<uint32_t>() + tiles - 1, 4, hipMemcpyDeviceToHost,
+                          b->s));
+    KG_HIP(hipStreamSynchronize(b->s));
+    *out_count = (int64_t)(last_off + last_cnt);
+  });
+}
+
+void kg_bloom_destroy(kg_bloom *b) {
+  if (!b) return;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  hipStream_t s = b->s;
+  delete b;
+  destroy_stream(s);
+}
+
+}  // extern "C"

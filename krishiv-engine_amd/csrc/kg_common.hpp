@@ -1,0 +1,206 @@
+// kg_common.hpp -- shared runtime pieces of libkgpu: errors at the C ABI,
+// the context (device, HBM budget), device buffers, columns as kernels see
+// them, and launch profiling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kgpu.h"
+
+namespace kg {
+
+// ---------------------------------------------------------------- errors
+struct Error {
+  int code;
+  std::string msg;
+};
+
+const char *class_of(int code);
+void set_last_error(int code, const std::string &msg);
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+#define KG_HIP(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      ::kg::fail(KG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+// Wraps a C entry point body: any kg::Error (or other exception) becomes a
+// status code and the thread's last error.
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return KG_OK;
+  } catch (const Error &e) {
+    set_last_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error(KG_ERR_MEMORY, "host allocation failed");
+    return KG_ERR_MEMORY;
+  } catch (const std::exception &e) {
+    set_last_error(KG_ERR_INTERNAL, e.what());
+    return KG_ERR_INTERNAL;
+  } catch (...) {
+    set_last_error(KG_ERR_INTERNAL, "unknown failure");
+    return KG_ERR_INTERNAL;
+  }
+}
+
+}  // namespace kg
+
+// ---------------------------------------------------------------- context
+struct kg_ctx {
+  int device = 0;
+  uint64_t budget = 0;
+  std::atomic<uint64_t> in_use{0};
+  hipStream_t stream = nullptr;  // for context-level calls
+  int num_cus = 0;
+};
+
+namespace kg {
+
+void *dev_alloc(kg_ctx *ctx, uint64_t bytes);
+void dev_free(kg_ctx *ctx, void *p, uint64_t bytes);
+
+// RAII device buffer charged to a context's HBM budget.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(kg_ctx *ctx, uint64_t bytes) { reset(ctx, bytes); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept { swap(o); }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      swap(o);
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void reset(kg_ctx *ctx, uint64_t bytes) {
+    release();
+    ctx_ = ctx;
+    bytes_ = bytes;
+    p_ = bytes ? dev_alloc(ctx, bytes) : nullptr;
+  }
+  // Keeps the buffer when it is already large enough.
+  void ensure(kg_ctx *ctx, uint64_t bytes) {
+    if (bytes > bytes_) reset(ctx, bytes + bytes / 4);
+  }
+  void release() {
+    if (p_) dev_free(ctx_, p_, bytes_);
+    p_ = nullptr;
+    bytes_ = 0;
+  }
+  template <class T = void>
+  T *get() const {
+    return static_cast<T *>(p_);
+  }
+  uint64_t bytes() const { return bytes_; }
+  void *detach() {
+    void *p = p_;
+    p_ = nullptr;
+    bytes_ = 0;
+    return p;
+  }
+
+ private:
+  void swap(DevBuf &o) {
+    std::swap(ctx_, o.ctx_);
+    std::swap(p_, o.p_);
+    std::swap(bytes_, o.bytes_);
+  }
+  kg_ctx *ctx_ = nullptr;
+  void *p_ = nullptr;
+  uint64_t bytes_ = 0;
+};
+
+// ---------------------------------------------------------------- columns
+// A column as kernels read it (device pointers only).
+struct DCol {
+  int32_t type;
+  int32_t precision;
+  int32_t scale;
+  int32_t width;  // bytes per value (0 for bit-packed BOOL)
+  int64_t offset;
+  const void *values;
+  const uint8_t *validity;
+};
+
+int type_width(int t);  // -1 for an unknown type
+bool is_integral(int t);
+bool is_float(int t);
+bool is_decimal(int t);
+bool is_int_like(int t);  // integers, dates, timestamps, decimal64, bool
+
+// Host-side staging of kg_column inputs: host columns are copied to HBM
+// (synchronously, so the caller may free them on return); device columns
+// are used in place.
+class Staged {
+ public:
+  Staged(kg_ctx *ctx, hipStream_t s) : ctx_(ctx), s_(s) {}
+  DCol add(const kg_column &c, int64_t nrows);
+  bool staged_host() const { return !bufs_.empty(); }
+
+ private:
+  kg_ctx *ctx_;
+  hipStream_t s_;
+  std::vector<DevBuf> bufs_;
+};
+
+void check_column(const kg_column &c, int64_t nrows, const char *what);
+
+// ---------------------------------------------------------------- profiling
+// Records a HIP event pair around a launch when profiling is on.
+class ProfScope {
+ public:
+  ProfScope(const char *family, hipStream_t s);
+  ~ProfScope();
+
+ private:
+  const char *family_;
+  hipStream_t s_;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+// Streams: every operator owns one stream at the default priority.
+hipStream_t make_stream(int device);
+void destroy_stream(hipStream_t s);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Grid size for a streaming kernel over n items: enough workgroups to fill
+// the 256 CUs several times over, capped (grid-stride loops do the rest).
+inline unsigned stream_grid(int64_t n, int block, int64_t per_thread = 4) {
+  int64_t g = ceil_div(n, (int64_t)block * per_thread);
+  if (g < 1) g = 1;
+  if (g > 256 * 32) g = 256 * 32;
+  return (unsigned)g;
+}
+
+}  // namespace kg
+
+// ---------------------------------------------------------------- batches
+struct kg_batch {
+  kg_ctx *ctx = nullptr;
+  int64_t nrows = 0;
+  struct Col {
+    int32_t type = 0, precision = 0, scale = 0;
+    kg::DevBuf values;
+    kg::DevBuf validity;  // LSB bitmap, or empty when no NULLs
+  };
+  std::vector<Col> cols;
+};

@@ -1,0 +1,350 @@
+// kg_exchange.hip -- the routing hash H_kg and the hash repartition behind
+// exchanges (exchange::route + ExchangeSink's split,
+// crates/krishiv-exec/src/exchange/mod.rs:159-212,447-501).
+//
+// Repartition is a stable counting sort by partition id in three launches:
+//   1. k_route_hist   H_kg -> partition id per row (4 B/row written) and a
+//                     per-block histogram (LDS atomics);
+//   2. an exclusive scan of the partition-major histogram (hipcub);
+//   3. k_scatter      each wave replays its contiguous quarter of the
+//                     block's rows; the rank of a row among the rows of its
+//                     partition in the same 64-row round comes from a wave
+//                     multisplit (one ballot per partition-id bit), so rows
+//                     keep their input order inside a partition.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "kg_common.hpp"
+#include "kg_device.hpp"
+
+namespace kg {
+
+constexpr int kMaxKeys = 8;
+constexpr int kMaxCols = 32;
+constexpr int kPartBlock = 256;  // 4 waves
+constexpr uint32_t kMaxPartitions = 4096;
+
+struct KeyCols {
+  DCol c[kMaxKeys];
+  int n;
+};
+
+struct CopyCols {
+  DCol in[kMaxCols];
+  void *out_values[kMaxCols];
+  uint32_t *out_valid[kMaxCols];  // 32-bit words, zeroed; NULL when no validity
+  int n;
+};
+
+__device__ __forceinline__ uint64_t hash_row(const KeyCols &k, int64_t i) {
+  if (k.n == 1) return dev::value_hash(k.c[0], i);
+  uint64_t h = dev::FOLD_INIT;
+  for (int c = 0; c < k.n; c++) h = dev::fold(h ^ dev::value_hash(k.c[c], i), dev::FOLD_MUL);
+  return h;
+}
+
+__global__ void k_hash(KeyCols k, int64_t n, uint64_t *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = hash_row(k, i);
+}
+
+__global__ void k_pids(KeyCols k, int64_t n, uint32_t parts, uint32_t *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = dev::partition_of(hash_row(k, i), parts);
+}
+
+// Rows [lo, hi) of block b: contiguous ranges of `per_block` rows.
+__global__ void __launch_bounds__(kPartBlock) k_route_hist(KeyCols k, int64_t n, uint32_t parts,
+                                                           int64_t per_block, uint32_t *pids,
+                                                           uint32_t *hist) {
+  extern __shared__ uint32_t s_hist[];
+  for (uint32_t p = threadIdx.x; p < parts; p += blockDim.x) s_hist[p] = 0;
+  __syncthreads();
+  int64_t lo = blockIdx.x * per_block;
+  int64_t hi = lo + per_block < n ? lo + per_block : n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    uint32_t p = dev::partition_of(hash_row(k, i), parts);
+    pids[i] = p;
+    atomicAdd(&s_hist[p], 1u);
+  }
+  __syncthreads();
+  // partition-major layout: the scan then gives each (partition, block) its
+  // first output row.
+  for (uint32_t p = threadIdx.x; p < parts; p += blockDim.x)
+    hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_hist[p];
+}
+
+__device__ __forceinline__ void copy_value(const DCol &in, int64_t i, void *out, int64_t pos) {
+  int64_t j = i + in.offset;
+  switch (in.width) {
+    case 1: static_cast<uint8_t *>(out)[pos] = static_cast<const uint8_t *>(in.values)[j]; break;
+    case 2: static_cast<uint16_t *>(out)[pos] = static_cast<const uint16_t *>(in.values)[j]; break;
+    case 4: static_cast<uint32_t *>(out)[pos] = static_cast<const uint32_t *>(in.values)[j]; break;
+    case 8: static_cast<uint64_t *>(out)[pos] = static_cast<const uint64_t *>(in.values)[j]; break;
+    case 16: {
+      const uint64_t *s = static_cast<const uint64_t *>(in.values) + 2 * j;
+      uint64_t *d = static_cast<uint64_t *>(out) + 2 * pos;
+      d[0] = s[0];
+      d[1] = s[1];
+      break;
+    }
+    default: {  // bit-packed BOOL
+      if ((static_cast<const uint8_t *>(in.values)[j >> 3] >> (j & 7)) & 1)
+        atomicOr(static_cast<uint32_t *>(out) + (pos >> 5), 1u << (pos & 31));
+    }
+  }
+}
+
+// One wave per contiguous quarter of the block's rows; LDS holds, per wave,
+// the running output position of every partition.
+__global__ void __launch_bounds__(kPartBlock) k_scatter(const uint32_t *pids, int64_t n,
+                                                        uint32_t parts, int bits, int64_t per_block,
+                                                        const uint64_t *base, CopyCols cols,
+                                                        int64_t *out_rows) {
+  extern __shared__ uint32_t s_cnt[];  // [4][parts] counts, then offsets (low 32 bits)
+  __shared__ uint64_t s_base_hi;       // not used: offsets are kept relative
+  (void)s_base_hi;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t p = threadIdx.x; p < 4 * parts; p += blockDim.x) s_cnt[p] = 0;
+  __syncthreads();
+  int64_t lo = blockIdx.x * per_block;
+  int64_t hi = lo + per_block < n ? lo + per_block : n;
+  int64_t q = (hi - lo + 3) / 4;
+  int64_t wlo = lo + wave * q, whi = wlo + q < hi ? wlo + q : hi;
+  uint32_t *cnt = s_cnt + wave * parts;
+  // pass A: counts per partition of this wave's rows
+  for (int64_t r = wlo; r < whi; r += 64) {
+    int64_t i = r + lane;
+    bool live = i < whi;
+    uint32_t p = live ? pids[i] : 0;
+    uint64_t peers = __ballot(live);
+    for (int b = 0; b < bits; b++) {
+      uint64_t m = __ballot(live && ((p >> b) & 1));
+      peers &= ((p >> b) & 1) ? m : ~m;
+    }
+    if (live && (peers & lt) == 0) cnt[p] += (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // exclusive prefix across the four waves, per partition (relative to the
+  // block's first row of that partition)
+  for (uint32_t p = threadIdx.x; p < parts; p += blockDim.x) {
+    uint32_t a = 0;
+    for (int w = 0; w < 4; w++) {
+      uint32_t c = s_cnt[w * parts + p];
+      s_cnt[w * parts + p] = a;
+      a += c;
+    }
+  }
+  __syncthreads();
+  // pass B: replay in order and scatter
+  for (int64_t r = wlo; r < whi; r += 64) {
+    int64_t i = r + lane;
+    bool live = i < whi;
+    uint32_t p = live ? pids[i] : 0;
+    uint64_t peers = __ballot(live);
+    for (int b = 0; b < bits; b++) {
+      uint64_t m = __ballot(live && ((p >> b) & 1));
+      peers &= ((p >> b) & 1) ? m : ~m;
+    }
+    if (live) {
+      int64_t pos = (int64_t)base[(uint64_t)p * gridDim.x + blockIdx.x] + cnt[p] +
+                    __popcll(peers & lt);
+      for (int c = 0; c < cols.n; c++) {
+        copy_value(cols.in[c], i, cols.out_values[c], pos);
+        if (cols.out_valid[c] && dev::valid(cols.in[c], i))
+          atomicOr(cols.out_valid[c] + (pos >> 5), 1u << (pos & 31));
+      }
+      if (out_rows) out_rows[pos] = i;
+    }
+    // every lane has read cnt[p] above before the leaders advance it
+    __builtin_amdgcn_wave_barrier();
+    if (live && (peers & lt) == 0) cnt[p] += (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+static KeyCols stage_keys(Staged &st, int32_t nkeys, const kg_column *keys, int64_t nrows) {
+  if (nkeys < 1 || nkeys > kMaxKeys) fail(KG_ERR_UNSUPPORTED, "1 to 8 key columns");
+  KeyCols k{};
+  k.n = nkeys;
+  for (int c = 0; c < nkeys; c++) {
+    check_column(keys[c], nrows, "key");
+    k.c[c] = st.add(keys[c], nrows);
+  }
+  return k;
+}
+
+static int ceil_log2(uint32_t v) {
+  int b = 0;
+  while ((1u << b) < v) b++;
+  return b;
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+extern "C" {
+
+int kg_hash_columns(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
+                    uint64_t *out_hashes, uint32_t out_flags) {
+  return guarded([&] {
+    if (nrows <= 0) return;
+    hipStream_t s = ctx->stream;
+    Staged st(ctx, s);
+    KeyCols k = stage_keys(st, ncols, cols, nrows);
+    DevBuf tmp;
+    uint64_t *out = out_hashes;
+    if (!(out_flags & KG_COL_DEVICE)) {
+      tmp.reset(ctx, (uint64_t)nrows * 8);
+      out = tmp.get<uint64_t>();
+    }
+    {
+      ProfScope prof("hash", s);
+      k_hash<<<stream_grid(nrows, 256), 256, 0, s>>>(k, nrows, out);
+      KG_HIP(hipGetLastError());
+    }
+    if (!(out_flags & KG_COL_DEVICE))
+      KG_HIP(hipMemcpyAsync(out_hashes, out, (size_t)nrows * 8, hipMemcpyDeviceToHost, s));
+    KG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int kg_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrows,
+                 uint32_t partitions, uint32_t *out_pids, uint32_t out_flags) {
+  return guarded([&] {
+    if (nrows <= 0) return;
+    hipStream_t s = ctx->stream;
+    DevBuf tmp;
+    uint32_t *out = out_pids;
+    if (!(out_flags & KG_COL_DEVICE)) {
+      tmp.reset(ctx, (uint64_t)nrows * 4);
+      out = tmp.get<uint32_t>();
+    }
+    if (partitions <= 1 || nkeys == 0) {
+      KG_HIP(hipMemsetAsync(out, 0, (size_t)nrows * 4, s));
+    } else {
+      Staged st(ctx, s);
+      KeyCols k = stage_keys(st, nkeys, keys, nrows);
+      ProfScope prof("hash_partition", s);
+      k_pids<<<stream_grid(nrows, 256), 256, 0, s>>>(k, nrows, partitions, out);
+      KG_HIP(hipGetLastError());
+    }
+    if (!(out_flags & KG_COL_DEVICE))
+      KG_HIP(hipMemcpyAsync(out_pids, out, (size_t)nrows * 4, hipMemcpyDeviceToHost, s));
+    KG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
+                   int32_t nkeys, const int32_t *key_cols, uint32_t partitions,
+                   int64_t *out_offsets, kg_batch **out) {
+  return guarded([&] {
+    *out = nullptr;
+    if (ncols < 1 || ncols > kMaxCols) fail(KG_ERR_UNSUPPORTED, "1 to 32 columns");
+    if (partitions < 1) partitions = 1;
+    if (partitions > kMaxPartitions)
+      fail(KG_ERR_UNSUPPORTED, "more than 4096 partitions on the device repartition");
+    hipStream_t s = ctx->stream;
+    Staged st(ctx, s);
+    std::vector<DCol> in(ncols);
+    for (int c = 0; c < ncols; c++) {
+      check_column(cols[c], nrows, "repartition column");
+      in[c] = st.add(cols[c], nrows);
+    }
+    KeyCols k{};
+    k.n = nkeys;
+    if (nkeys < 0 || nkeys > kMaxKeys) fail(KG_ERR_UNSUPPORTED, "0 to 8 key columns");
+    for (int c = 0; c < nkeys; c++) {
+      if (key_cols[c] < 0 || key_cols[c] >= ncols) fail(KG_ERR_INTERNAL, "key column index");
+      k.c[c] = in[key_cols[c]];
+    }
+    auto b = std::make_unique<kg_batch>();
+    b->ctx = ctx;
+    b->nrows = nrows;
+    b->cols.resize(ncols);
+    CopyCols cc{};
+    cc.n = ncols;
+    for (int c = 0; c < ncols; c++) {
+      auto &oc = b->cols[c];
+      oc.type = cols[c].type;
+      oc.precision = cols[c].precision;
+      oc.scale = cols[c].scale;
+      int w = type_width(oc.type);
+      uint64_t vbytes = w == 0 ? (uint64_t)ceil_div(nrows, 32) * 4 : (uint64_t)nrows * w;
+      oc.values.reset(ctx, vbytes > 0 ? vbytes : 4);
+      if (w == 0) KG_HIP(hipMemsetAsync(oc.values.get(), 0, oc.values.bytes(), s));
+      cc.in[c] = in[c];
+      cc.out_values[c] = oc.values.get();
+      if (in[c].validity) {
+        oc.validity.reset(ctx, (uint64_t)ceil_div(nrows, 32) * 4 + 4);
+        KG_HIP(hipMemsetAsync(oc.validity.get(), 0, oc.validity.bytes(), s));
+        cc.out_valid[c] = oc.validity.get<uint32_t>();
+      }
+    }
+    if (nrows == 0 || partitions == 1 || nkeys == 0) {
+      // one partition: the rows in order (route() puts every row in 0)
+      for (uint32_t p = 0; p <= partitions; p++) out_offsets[p] = p == 0 ? 0 : nrows;
+      if (nrows > 0) {
+        std::vector<uint32_t> dummy;
+        DevBuf pids(ctx, (uint64_t)nrows * 4);
+        KG_HIP(hipMemsetAsync(pids.get(), 0, (size_t)nrows * 4, s));
+        DevBuf base(ctx, 8);
+        KG_HIP(hipMemsetAsync(base.get(), 0, 8, s));
+        ProfScope prof("scatter", s);
+        k_scatter<<<1, kPartBlock, 4 * sizeof(uint32_t), s>>>(pids.get<uint32_t>(), nrows, 1, 0,
+                                                              nrows, base.get<uint64_t>(), cc,
+                                                              nullptr);
+        KG_HIP(hipGetLastError());
+        KG_HIP(hipStreamSynchronize(s));
+      }
+      *out = b.release();
+      return;
+    }
+    // block count: the histogram is partitions x blocks entries
+    int64_t nb = ceil_div(nrows, 4096);
+    int64_t cap = (int64_t)(16u << 20) / partitions;
+    if (nb > cap) nb = cap;
+    if (nb > 65535) nb = 65535;
+    if (nb < 1) nb = 1;
+    int64_t per_block = ceil_div(nrows, nb);
+    nb = ceil_div(nrows, per_block);
+    DevBuf pids(ctx, (uint64_t)nrows * 4);
+    uint64_t hn = (uint64_t)partitions * nb;
+    DevBuf hist(ctx, hn * 4), base(ctx, hn * 8);
+    {
+      ProfScope prof("hash_partition", s);
+      k_route_hist<<<(unsigned)nb, kPartBlock, partitions * 4, s>>>(
+          k, nrows, partitions, per_block, pids.get<uint32_t>(), hist.get<uint32_t>());
+      KG_HIP(hipGetLastError());
+    }
+    size_t temp_bytes = 0;
+    KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, hist.get<uint32_t>(),
+                                            base.get<uint64_t>(), (int)hn, s));
+    DevBuf temp(ctx, temp_bytes + 16);
+    KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), temp_bytes, hist.get<uint32_t>(),
+                                            base.get<uint64_t>(), (int)hn, s));
+    {
+      ProfScope prof("scatter", s);
+      k_scatter<<<(unsigned)nb, kPartBlock, 4 * partitions * 4, s>>>(
+          pids.get<uint32_t>(), nrows, partitions, ceil_log2(partitions), per_block,
+          base.get<uint64_t>(), cc, nullptr);
+      KG_HIP(hipGetLastError());
+    }
+    // partition offsets: the scanned value at each partition's first block
+    std::vector<uint64_t> firsts(partitions);
+    for (uint32_t p = 0; p < partitions; p++)
+      KG_HIP(hipMemcpyAsync(&firsts[p], base.get<uint64_t>() + (uint64_t)p * nb, 8,
+                            hipMemcpyDeviceToHost, s));
+    KG_HIP(hipStreamSynchronize(s));
+    for (uint32_t p = 0; p < partitions; p++) out_offsets[p] = (int64_t)firsts[p];
+    out_offsets[partitions] = nrows;
+    *out = b.release();
+  });
+}
+
+}  // extern "C"

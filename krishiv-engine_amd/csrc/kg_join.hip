@@ -1,0 +1,709 @@
+// kg_join.hip -- hash join: the build Sink (join/build.rs:1225-1695) and the
+// probe Operator (join/probe.rs:784-948) of HashJoin (join/mod.rs:1014-1211),
+// inner / left outer / left semi / left anti on `=` or `<=>` keys with Spark's
+// NULL semantics (join/mod.rs:21-60), plus the join fused with GROUP BY the
+// build key.
+//
+// Layout
+//   build store     the build batches' columns, appended in HBM as they are
+//                   sunk and made contiguous at finalize (the row arena).
+//   slots[cap]      16-byte open-addressing slots {key, head, count}: the
+//                   packed key (one integer-like column, or two whose widths
+//                   sum to 8 bytes), the newest build row with that key, and
+//                   the key's row count.  Slot cap holds the key equal to the
+//                   EMPTY sentinel, slot cap+1 the NULL of a `<=>` key.
+//   next[rows]      chains of duplicate keys, newest first (rows/index.rs:237-279).
+// A probe is one 16-byte read for a unique key (key, head and count in one
+// slot).  Probe output is sized by a count pass and an exclusive scan, then
+// the pairs are written and the output columns gathered.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "kg_aggops.hpp"
+
+namespace kg {
+
+constexpr int kJoinCols = 32;
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+
+struct Slot {
+  uint64_t key;
+  uint32_t head;
+  uint32_t count;
+};
+
+struct KeySpec {
+  DCol c[2];
+  int n;
+  int shift;  // bits of the second column (two-column keys)
+  int null_safe;
+};
+
+struct JTable {
+  Slot *slots;
+  uint32_t *next;
+  uint64_t mask;
+  int log2cap;
+  uint64_t cap;
+  uint64_t seed;
+};
+
+// Packed key of row i.  Returns 0 (ordinary), 1 (the EMPTY value), 2 (a
+// `<=>` NULL) or 3 (an `=` NULL: matches nothing).
+__device__ __forceinline__ int pack_key(const KeySpec &K, int64_t i, uint64_t &key) {
+  uint64_t parts[2];
+  for (int c = 0; c < K.n; c++) {
+    const DCol &d = K.c[c];
+    if (!dev::valid(d, i)) {
+      key = 0;
+      return K.null_safe ? 2 : 3;
+    }
+    if (d.type == KG_T_FLOAT64) parts[c] = dev::norm_f64_bits(dev::load_f64(d, i));
+    else if (d.type == KG_T_FLOAT32) parts[c] = dev::norm_f32_bits((float)dev::load_f64(d, i));
+    else parts[c] = (uint64_t)dev::load_i64(d, i);
+  }
+  if (K.n == 1) {
+    key = parts[0];
+  } else {
+    uint64_t m = K.shift >= 64 ? ~0ull : ((1ull << K.shift) - 1);
+    key = (parts[0] << K.shift) | (parts[1] & m);
+  }
+  return key == kEmpty ? 1 : 0;
+}
+
+__device__ __forceinline__ uint64_t jslot_start(const JTable &T, uint64_t key) {
+  return dev::finalize(dev::row_hash1(key), T.seed) >> (64 - T.log2cap);
+}
+
+__global__ void k_join_init(Slot *slots, uint64_t n) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    slots[s] = Slot{kEmpty, kNoRow, 0};
+}
+
+__global__ void k_join_build(KeySpec K, int64_t n, JTable T) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t key;
+    int cls = pack_key(K, i, key);
+    if (cls == 3) {
+      T.next[i] = kNoRow;
+      continue;  // an `=` key with a NULL is never indexed
+    }
+    uint64_t s;
+    if (cls == 0) {
+      s = jslot_start(T, key);
+      while (true) {
+        uint64_t k = __hip_atomic_load(&T.slots[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == key) break;
+        if (k == kEmpty) {
+          unsigned long long prev =
+              atomicCAS((unsigned long long *)&T.slots[s].key, kEmpty, key);
+          if (prev == kEmpty || prev == key) break;
+        }
+        s = (s + 1) & T.mask;
+      }
+    } else {
+      s = T.cap + (cls == 2 ? 1 : 0);
+    }
+    uint32_t old = atomicExch(&T.slots[s].head, (uint32_t)i);
+    T.next[i] = old;
+    atomicAdd(&T.slots[s].count, 1u);
+  }
+}
+
+// Read-only lookup: slot index, or -1.
+__device__ __forceinline__ int64_t jfind(const JTable &T, int cls, uint64_t key) {
+  if (cls == 3) return -1;
+  if (cls != 0) return (int64_t)(T.cap + (cls == 2 ? 1 : 0));
+  uint64_t s = jslot_start(T, key);
+  while (true) {
+    uint64_t k = T.slots[s].key;
+    if (k == key) return (int64_t)s;
+    if (k == kEmpty) return -1;
+    s = (s + 1) & T.mask;
+  }
+}
+
+// Pass 1: matches and output rows of each probe row.
+__global__ void k_join_count(KeySpec K, int64_t n, JTable T, int join_type, uint32_t *head,
+                             uint32_t *nout) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t key;
+    int cls = pack_key(K, i, key);
+    int64_t s = jfind(T, cls, key);
+    uint32_t h = kNoRow, m = 0;
+    if (s >= 0) {
+      Slot sl = T.slots[s];
+      h = sl.head;
+      m = sl.count;
+    }
+    uint32_t o;
+    switch (join_type) {
+      case KG_JOIN_INNER: o = m; break;
+      case KG_JOIN_LEFT_OUTER: o = m ? m : 1; break;
+      case KG_JOIN_LEFT_SEMI: o = m ? 1 : 0; break;
+      default: o = m ? 0 : 1; break;  // anti
+    }
+    head[i] = h;
+    nout[i] = o;
+  }
+}
+
+// Pass 2: the (probe row, build row) pairs; build row -1 pads.
+__global__ void k_join_pairs(int64_t n, const uint32_t *head, const uint32_t *nout,
+                             const uint64_t *off, const uint32_t *next, int join_type,
+                             int32_t *pidx, int32_t *bidx) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t o = nout[i];
+    if (!o) continue;
+    uint64_t p = off[i];
+    if (join_type == KG_JOIN_LEFT_SEMI || join_type == KG_JOIN_LEFT_ANTI) {
+      pidx[p] = (int32_t)i;
+      if (bidx) bidx[p] = -1;
+      continue;
+    }
+    uint32_t r = head[i];
+    if (r == kNoRow) {  // left outer, unmatched
+      pidx[p] = (int32_t)i;
+      bidx[p] = -1;
+      continue;
+    }
+    for (uint32_t k = 0; k < o && r != kNoRow; k++, r = next[r]) {
+      pidx[p + k] = (int32_t)i;
+      bidx[p + k] = (int32_t)r;
+    }
+  }
+}
+
+struct GatherCols {
+  DCol src[kJoinCols];
+  int from_build[kJoinCols];
+  void *dst[kJoinCols];
+  uint32_t *dst_valid[kJoinCols];  // NULL when the output column has no NULLs
+  int n;
+};
+
+__device__ __forceinline__ void copy_elem(const DCol &in, int64_t j, void *out, int64_t pos) {
+  j += in.offset;
+  switch (in.width) {
+    case 1: static_cast<uint8_t *>(out)[pos] = static_cast<const uint8_t *>(in.values)[j]; break;
+    case 2: static_cast<uint16_t *>(out)[pos] = static_cast<const uint16_t *>(in.values)[j]; break;
+    case 4: static_cast<uint32_t *>(out)[pos] = static_cast<const uint32_t *>(in.values)[j]; break;
+    case 8: static_cast<uint64_t *>(out)[pos] = static_cast<const uint64_t *>(in.values)[j]; break;
+    case 16: {
+      const uint64_t *s = static_cast<const uint64_t *>(in.values) + 2 * j;
+      uint64_t *d = static_cast<uint64_t *>(out) + 2 * pos;
+      d[0] = s[0];
+      d[1] = s[1];
+      break;
+    }
+    default:
+      if ((static_cast<const uint8_t *>(in.values)[j >> 3] >> (j & 7)) & 1)
+        atomicOr(static_cast<uint32_t *>(out) + (pos >> 5), 1u << (pos & 31));
+  }
+}
+
+__global__ void k_join_gather(GatherCols G, int64_t nout, const int32_t *pidx,
+                              const int32_t *bidx) {
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < nout;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    int32_t pi = pidx[o];
+    int32_t bi = bidx ? bidx[o] : -1;
+    for (int c = 0; c < G.n; c++) {
+      int32_t r = G.from_build[c] ? bi : pi;
+      const DCol &s = G.src[c];
+      if (r < 0) continue;  // padded NULL (validity bit stays 0)
+      copy_elem(s, r, G.dst[c], o);
+      if (G.dst_valid[c] && dev::valid(s, r))
+        atomicOr(G.dst_valid[c] + (o >> 5), 1u << (o & 31));
+    }
+  }
+}
+
+// The join fused with GROUP BY the build key: each probe row's aggregate
+// ops go to its build key's record, once per matching build row.
+__global__ void k_joinagg_probe(KeySpec K, int64_t n, JTable T, AggProg P, uint64_t *rec) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t key;
+    int cls = pack_key(K, i, key);
+    int64_t s = jfind(T, cls, key);
+    if (s < 0) continue;
+    uint32_t m = T.slots[s].count;
+    if (!m) continue;
+    apply_row_mult(P, rec + (uint64_t)s * P.W, i, m);
+  }
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+struct kg_hash_join {
+  kg_ctx *ctx = nullptr;
+  hipStream_t s = nullptr;
+  kg_join_spec spec{};
+  uint64_t expected = 0;
+  // build store: per column, the appended chunks, then one contiguous buffer
+  struct BCol {
+    int32_t type = 0, precision = 0, scale = 0;
+    bool nullable = false;
+    std::vector<std::pair<DevBuf, int64_t>> chunks;      // values, rows
+    std::vector<std::pair<DevBuf, int64_t>> vchunks;     // validity bitmaps (bytes), rows
+    DevBuf values, validity;
+  };
+  std::vector<BCol> bcols;
+  int64_t build_rows = 0;
+  bool finalized = false;
+  // table
+  DevBuf slots, next;
+  uint64_t cap = 0;
+  int log2cap = 0;
+  uint64_t seed = 0x2545F4914F6CDD1DULL;
+  // fused aggregate
+  bool has_agg = false;
+  AggProg prog{};
+  OutProg out{};
+  std::vector<kg_column> agg_types;
+  DevBuf rec;
+  // probe scratch
+  DevBuf head, nout, off, temp, pidx, bidx;
+};
+
+namespace kg {
+
+static JTable jtable(kg_hash_join *j) {
+  JTable t{};
+  t.slots = j->slots.get<Slot>();
+  t.next = j->next.get<uint32_t>();
+  t.cap = j->cap;
+  t.log2cap = j->log2cap;
+  t.mask = j->cap - 1;
+  t.seed = j->seed;
+  return t;
+}
+
+static int jlog2(uint64_t v) {
+  int b = 0;
+  while ((1ull << b) < v) b++;
+  return b;
+}
+
+static KeySpec key_spec(kg_hash_join *j, const DCol *cols, const int32_t *which) {
+  KeySpec K{};
+  K.n = j->spec.num_keys;
+  K.null_safe = j->spec.null_safe[0];
+  for (int c = 0; c < K.n; c++) K.c[c] = cols[which[c]];
+  K.shift = K.n == 2 ? 8 * type_width(K.c[1].type) : 0;
+  return K;
+}
+
+static void check_keys(const kg_join_spec &sp, int32_t ncols, const kg_column *cols,
+                       const int32_t *which) {
+  int width = 0;
+  for (int c = 0; c < sp.num_keys; c++) {
+    if (which[c] < 0 || which[c] >= ncols) fail(KG_ERR_INTERNAL, "join key column index");
+    int t = cols[which[c]].type;
+    int w = type_width(t);
+    if (!(is_int_like(t) || is_float(t)) || w <= 0)
+      fail(KG_ERR_UNSUPPORTED, "join key type on the device");
+    width += w;
+  }
+  if (sp.num_keys == 2 && width > 8)
+    fail(KG_ERR_UNSUPPORTED, "two join keys wider than 8 bytes together");
+}
+
+static void stage_all(Staged &st, int32_t ncols, const kg_column *cols, int64_t nrows,
+                      std::vector<DCol> &out) {
+  out.resize(ncols);
+  for (int c = 0; c < ncols; c++) {
+    check_column(cols[c], nrows, "join column");
+    out[c] = st.add(cols[c], nrows);
+  }
+}
+
+__global__ void k_copy_bits(const uint8_t *src, int64_t src_off, int64_t n, uint32_t *dst,
+                            int64_t dst_off) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t s = src_off + i, d = dst_off + i;
+    if ((src[s >> 3] >> (s & 7)) & 1) atomicOr(dst + (d >> 5), 1u << (d & 31));
+  }
+}
+
+static void append_build(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows) {
+  if (j->bcols.empty()) {
+    j->bcols.resize(ncols);
+    for (int c = 0; c < ncols; c++) {
+      j->bcols[c].type = cols[c].type;
+      j->bcols[c].precision = cols[c].precision;
+      j->bcols[c].scale = cols[c].scale;
+    }
+  }
+  if ((int)j->bcols.size() != ncols) fail(KG_ERR_INTERNAL, "build batches differ in columns");
+  Staged st(j->ctx, j->s);
+  std::vector<DCol> d;
+  stage_all(st, ncols, cols, nrows, d);
+  for (int c = 0; c < ncols; c++) {
+    auto &bc = j->bcols[c];
+    if (cols[c].type != bc.type) fail(KG_ERR_INTERNAL, "build column type changed");
+    int w = type_width(bc.type);
+    // values: bit-packed booleans re-based to bit 0 as 32-bit words
+    uint64_t vbytes = w == 0 ? (uint64_t)ceil_div(nrows, 32) * 4 : (uint64_t)nrows * w;
+    DevBuf v(j->ctx, vbytes ? vbytes : 4);
+    if (w == 0) {
+      KG_HIP(hipMemsetAsync(v.get(), 0, v.bytes(), j->s));
+      k_copy_bits<<<stream_grid(nrows, 256), 256, 0, j->s>>>(
+          static_cast<const uint8_t *>(d[c].values), d[c].offset, nrows, v.get<uint32_t>(), 0);
+    } else {
+      KG_HIP(hipMemcpyAsync(v.get(), static_cast<const uint8_t *>(d[c].values) + d[c].offset * w,
+                            (size_t)nrows * w, hipMemcpyDeviceToDevice, j->s));
+    }
+    bc.chunks.emplace_back(std::move(v), nrows);
+    if (d[c].validity) {
+      bc.nullable = true;
+      DevBuf vb(j->ctx, (uint64_t)ceil_div(nrows, 32) * 4);
+      KG_HIP(hipMemsetAsync(vb.get(), 0, vb.bytes(), j->s));
+      k_copy_bits<<<stream_grid(nrows, 256), 256, 0, j->s>>>(d[c].validity, d[c].offset, nrows,
+                                                             vb.get<uint32_t>(), 0);
+      bc.vchunks.emplace_back(std::move(vb), nrows);
+    } else {
+      bc.vchunks.emplace_back(DevBuf(), nrows);  // all valid
+    }
+    KG_HIP(hipGetLastError());
+  }
+  j->build_rows += nrows;
+  KG_HIP(hipStreamSynchronize(j->s));
+}
+
+__global__ void k_set_bits(uint32_t *dst, int64_t off, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t d = off + i;
+    atomicOr(dst + (d >> 5), 1u << (d & 31));
+  }
+}
+
+// Concatenates the chunks of every build column (the arena made contiguous).
+static void concat_build(kg_hash_join *j) {
+  int64_t n = j->build_rows;
+  for (auto &bc : j->bcols) {
+    int w = type_width(bc.type);
+    uint64_t vbytes = w == 0 ? (uint64_t)ceil_div(n, 32) * 4 : (uint64_t)n * w;
+    bc.values.reset(j->ctx, vbytes + 16);
+    if (w == 0) KG_HIP(hipMemsetAsync(bc.values.get(), 0, bc.values.bytes(), j->s));
+    if (bc.nullable) {
+      bc.validity.reset(j->ctx, (uint64_t)ceil_div(n, 32) * 4 + 4);
+      KG_HIP(hipMemsetAsync(bc.validity.get(), 0, bc.validity.bytes(), j->s));
+    }
+    int64_t at = 0;
+    for (size_t k = 0; k < bc.chunks.size(); k++) {
+      int64_t rows = bc.chunks[k].second;
+      if (w == 0) {
+        k_copy_bits<<<stream_grid(rows, 256), 256, 0, j->s>>>(bc.chunks[k].first.get<uint8_t>(), 0,
+                                                             rows, bc.values.get<uint32_t>(), at);
+      } else if (rows) {
+        KG_HIP(hipMemcpyAsync(bc.values.get<uint8_t>() + at * w, bc.chunks[k].first.get(),
+                              (size_t)rows * w, hipMemcpyDeviceToDevice, j->s));
+      }
+      if (bc.nullable && rows) {
+        if (bc.vchunks[k].first.bytes())
+          k_copy_bits<<<stream_grid(rows, 256), 256, 0, j->s>>>(
+              bc.vchunks[k].first.get<uint8_t>(), 0, rows, bc.validity.get<uint32_t>(), at);
+        else
+          k_set_bits<<<stream_grid(rows, 256), 256, 0, j->s>>>(bc.validity.get<uint32_t>(), at, rows);
+      }
+      at += rows;
+    }
+    KG_HIP(hipGetLastError());
+    bc.chunks.clear();
+    bc.vchunks.clear();
+  }
+  KG_HIP(hipStreamSynchronize(j->s));
+}
+
+static DCol build_dcol(const kg_hash_join::BCol &bc) {
+  DCol d{};
+  d.type = bc.type;
+  d.precision = bc.precision;
+  d.scale = bc.scale;
+  d.width = type_width(bc.type);
+  d.offset = 0;
+  d.values = bc.values.get();
+  d.validity = bc.nullable ? bc.validity.get<uint8_t>() : nullptr;
+  return d;
+}
+
+static void scan_u32(kg_hash_join *j, const uint32_t *in, uint64_t *out, int64_t n) {
+  size_t tb = 0;
+  KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, j->s));
+  j->temp.ensure(j->ctx, tb + 16);
+  KG_HIP(hipcub::DeviceScan::ExclusiveSum(j->temp.get(), tb, in, out, (int)n, j->s));
+}
+
+}  // namespace kg
+
+extern "C" {
+
+int kg_join_create(kg_ctx *ctx, const kg_join_spec *spec, uint64_t expected_build_rows,
+                   kg_hash_join **out) {
+  return guarded([&] {
+    *out = nullptr;
+    if (spec->num_keys < 1 || spec->num_keys > 2)
+      fail(KG_ERR_UNSUPPORTED, "one or two join keys on the device");
+    if (spec->join_type < KG_JOIN_INNER || spec->join_type > KG_JOIN_LEFT_ANTI)
+      fail(KG_ERR_UNSUPPORTED, "join type on the device");
+    if (spec->num_keys == 2 && (spec->null_safe[0] || spec->null_safe[1]))
+      fail(KG_ERR_UNSUPPORTED, "null-safe keys in a two-key device join");
+    auto j = std::make_unique<kg_hash_join>();
+    j->ctx = ctx;
+    j->spec = *spec;
+    j->expected = expected_build_rows;
+    j->s = make_stream(ctx->device);
+    *out = j.release();
+  });
+}
+
+int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows) {
+  return guarded([&] {
+    if (j->finalized) fail(KG_ERR_INTERNAL, "build sink after finalize");
+    if (ncols < 1 || ncols > kJoinCols) fail(KG_ERR_UNSUPPORTED, "1 to 32 build columns");
+    check_keys(j->spec, ncols, cols, j->spec.build_keys);
+    if (j->build_rows + nrows > (int64_t)0x7FFFFFF0)
+      fail(KG_ERR_UNSUPPORTED, "a build side of 2^31 rows or more (grace partitions first)");
+    if (nrows > 0) append_build(j, ncols, cols, nrows);
+  });
+}
+
+int kg_join_build_finalize(kg_hash_join *j) {
+  return guarded([&] {
+    if (j->finalized) return;
+    int64_t n = j->build_rows;
+    concat_build(j);
+    uint64_t want = (uint64_t)(n > (int64_t)j->expected ? n : (int64_t)j->expected);
+    if (want < 512) want = 512;
+    j->cap = 1ull << jlog2(want * 2);
+    j->log2cap = jlog2(j->cap);
+    j->slots.reset(j->ctx, (j->cap + 2) * sizeof(Slot));
+    j->next.reset(j->ctx, (uint64_t)(n > 0 ? n : 1) * 4);
+    k_join_init<<<stream_grid(j->cap + 2, 256), 256, 0, j->s>>>(j->slots.get<Slot>(), j->cap + 2);
+    KG_HIP(hipGetLastError());
+    if (n > 0) {
+      std::vector<DCol> d(j->bcols.size());
+      for (size_t c = 0; c < d.size(); c++) d[c] = build_dcol(j->bcols[c]);
+      KeySpec K = key_spec(j, d.data(), j->spec.build_keys);
+      ProfScope prof("join_build", j->s);
+      k_join_build<<<stream_grid(n, 256), 256, 0, j->s>>>(K, n, jtable(j));
+      KG_HIP(hipGetLastError());
+    }
+    KG_HIP(hipStreamSynchronize(j->s));
+    j->finalized = true;
+  });
+}
+
+int64_t kg_join_build_rows(const kg_hash_join *j) { return j ? j->build_rows : 0; }
+
+int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                  kg_batch **out) {
+  return guarded([&] {
+    *out = nullptr;
+    if (!j->finalized) fail(KG_ERR_INTERNAL, "probe before the build finalized");
+    if (ncols < 1 || ncols + (int)j->bcols.size() > kJoinCols)
+      fail(KG_ERR_UNSUPPORTED, "too many output columns");
+    if (nrows > INT32_MAX) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^31 rows or more");
+    check_keys(j->spec, ncols, cols, j->spec.probe_keys);
+    for (int c = 0; c < j->spec.num_keys; c++) {
+      int bt = j->bcols.empty() ? cols[j->spec.probe_keys[c]].type
+                                : j->bcols[j->spec.build_keys[c]].type;
+      if (cols[j->spec.probe_keys[c]].type != bt)
+        fail(KG_ERR_INTERNAL, "probe and build keys differ in type");
+    }
+    Staged st(j->ctx, j->s);
+    std::vector<DCol> d;
+    stage_all(st, ncols, cols, nrows, d);
+    int jt = j->spec.join_type;
+    bool emits_build = jt == KG_JOIN_INNER || jt == KG_JOIN_LEFT_OUTER;
+    int64_t total = 0;
+    if (nrows > 0) {
+      KeySpec K = key_spec(j, d.data(), j->spec.probe_keys);
+      j->head.ensure(j->ctx, (uint64_t)nrows * 4);
+      j->nout.ensure(j->ctx, (uint64_t)nrows * 4);
+      j->off.ensure(j->ctx, (uint64_t)(nrows + 1) * 8);
+      {
+        ProfScope prof("join_probe", j->s);
+        k_join_count<<<stream_grid(nrows, 256), 256, 0, j->s>>>(
+            K, nrows, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
+        KG_HIP(hipGetLastError());
+      }
+      scan_u32(j, j->nout.get<uint32_t>(), j->off.get<uint64_t>(), nrows);
+      uint64_t last_off = 0;
+      uint32_t last_n = 0;
+      KG_HIP(hipMemcpyAsync(&last_off, j->off.get<uint64_t>() + nrows - 1, 8,
+                            hipMemcpyDeviceToHost, j->s));
+      KG_HIP(hipMemcpyAsync(&last_n, j->nout.get<uint32_t>() + nrows - 1, 4,
+                            hipMemcpyDeviceToHost, j->s));
+      KG_HIP(hipStreamSynchronize(j->s));
+      total = (int64_t)(last_off + last_n);
+    }
+    auto b = std::make_unique<kg_batch>();
+    b->ctx = j->ctx;
+    b->nrows = total;
+    int nb = emits_build ? (int)j->bcols.size() : 0;
+    b->cols.resize(ncols + nb);
+    GatherCols G{};
+    G.n = ncols + nb;
+    uint64_t vb = (uint64_t)ceil_div(total, 32) * 4 + 4;
+    for (int c = 0; c < ncols + nb; c++) {
+      bool from_build = c >= ncols;
+      DCol src = from_build ? build_dcol(j->bcols[c - ncols]) : d[c];
+      auto &oc = b->cols[c];
+      oc.type = src.type;
+      oc.precision = src.precision;
+      oc.scale = src.scale;
+      int w = type_width(src.type);
+      uint64_t bytes = w == 0 ? vb : (uint64_t)total * w + 16;
+      oc.values.reset(j->ctx, bytes);
+      if (w == 0) KG_HIP(hipMemsetAsync(oc.values.get(), 0, bytes, j->s));
+      bool nullable = src.validity != nullptr || (from_build && jt == KG_JOIN_LEFT_OUTER);
+      if (nullable) {
+        oc.validity.reset(j->ctx, vb);
+        KG_HIP(hipMemsetAsync(oc.validity.get(), 0, vb, j->s));
+      }
+      G.src[c] = src;
+      G.from_build[c] = from_build;
+      G.dst[c] = oc.values.get();
+      // padded rows (build row -1) keep a 0 validity bit: NULL
+      G.dst_valid[c] = nullable ? oc.validity.get<uint32_t>() : nullptr;
+    }
+    if (total > 0) {
+      j->pidx.ensure(j->ctx, (uint64_t)total * 4);
+      if (emits_build) j->bidx.ensure(j->ctx, (uint64_t)total * 4);
+      k_join_pairs<<<stream_grid(nrows, 256), 256, 0, j->s>>>(
+          nrows, j->head.get<uint32_t>(), j->nout.get<uint32_t>(), j->off.get<uint64_t>(),
+          j->next.get<uint32_t>(), jt, j->pidx.get<int32_t>(),
+          emits_build ? j->bidx.get<int32_t>() : nullptr);
+      KG_HIP(hipGetLastError());
+      ProfScope prof("join_gather", j->s);
+      k_join_gather<<<stream_grid(total, 256), 256, 0, j->s>>>(
+          G, total, j->pidx.get<int32_t>(), emits_build ? j->bidx.get<int32_t>() : nullptr);
+      KG_HIP(hipGetLastError());
+    }
+    KG_HIP(hipStreamSynchronize(j->s));
+    *out = b.release();
+  });
+}
+
+int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_types,
+                      int32_t naggs, const kg_agg_spec *aggs, uint32_t flags) {
+  return guarded([&] {
+    if (!j->finalized) fail(KG_ERR_INTERNAL, "aggregate before the build finalized");
+    if (j->spec.join_type != KG_JOIN_INNER || j->spec.num_keys != 1)
+      fail(KG_ERR_UNSUPPORTED, "the fused aggregate needs an inner join on one key");
+    if (ninputs > kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 aggregate inputs");
+    j->agg_types.assign(input_types, input_types + ninputs);
+    std::vector<CheckedSum> checked;
+    compile_aggs(j->agg_types, naggs, aggs, (flags & KG_AGG_ANSI) != 0, j->prog, j->out, checked);
+    if (!checked.empty())
+      fail(KG_ERR_UNSUPPORTED, "checked sums (decimal, ANSI) in the fused join aggregate");
+    for (int o = 0; o < j->prog.nops; o++)
+      if (j->prog.ops[o].kind == OP_ADD192)
+        fail(KG_ERR_UNSUPPORTED, "decimal128 sums in the fused join aggregate");
+    j->rec.reset(j->ctx, (j->cap + 2) * (uint64_t)j->prog.W * 8);
+    init_records(j->ctx, j->rec.get<uint64_t>(), j->cap + 2, j->prog, j->s);
+    j->has_agg = true;
+  });
+}
+
+int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                     const int32_t *input_cols) {
+  return guarded([&] {
+    if (!j->has_agg) fail(KG_ERR_INTERNAL, "no fused aggregate");
+    if (nrows <= 0) return;
+    check_keys(j->spec, ncols, cols, j->spec.probe_keys);
+    Staged st(j->ctx, j->s);
+    std::vector<DCol> d;
+    stage_all(st, ncols, cols, nrows, d);
+    KeySpec K = key_spec(j, d.data(), j->spec.probe_keys);
+    AggProg P = j->prog;
+    for (int c = 0; c < (int)j->agg_types.size(); c++) {
+      int ic = input_cols[c];
+      if (ic < 0 || ic >= ncols) fail(KG_ERR_INTERNAL, "aggregate input column index");
+      if (d[ic].type != j->agg_types[c].type) fail(KG_ERR_INTERNAL, "aggregate input type");
+      P.in[c] = d[ic];
+    }
+    {
+      ProfScope prof("joinagg_probe", j->s);
+      k_joinagg_probe<<<stream_grid(nrows, 256), 256, 0, j->s>>>(K, nrows, jtable(j), P,
+                                                                 j->rec.get<uint64_t>());
+      KG_HIP(hipGetLastError());
+    }
+    if (st.staged_host()) KG_HIP(hipStreamSynchronize(j->s));
+  });
+}
+
+int kg_joinagg_result(kg_hash_join *j, kg_batch **out) {
+  return guarded([&] {
+    *out = nullptr;
+    if (!j->has_agg) fail(KG_ERR_INTERNAL, "no fused aggregate");
+    DevBuf pos;
+    uint64_t ns = j->cap + 2;
+    int64_t n = group_positions(j->ctx, j->rec.get<uint64_t>(), ns, j->prog.W, false, pos, j->s);
+    auto b = std::make_unique<kg_batch>();
+    b->ctx = j->ctx;
+    b->nrows = n;
+    b->cols.resize(1 + j->out.n);
+    const auto &kb = j->bcols[j->spec.build_keys[0]];
+    auto &kc = b->cols[0];
+    kc.type = kb.type;
+    kc.precision = kb.precision;
+    kc.scale = kb.scale;
+    int kw = type_width(kb.type);
+    if (kw <= 0) fail(KG_ERR_UNSUPPORTED, "boolean join key in the fused aggregate");
+    uint64_t vb = (uint64_t)ceil_div(n, 32) * 4 + 4;
+    kc.values.reset(j->ctx, (uint64_t)n * kw + 8);
+    kc.validity.reset(j->ctx, vb);
+    KG_HIP(hipMemsetAsync(kc.validity.get(), 0, vb, j->s));
+    OutProg O = j->out;
+    O.cap = j->cap;
+    O.key_type = kb.type;
+    O.direct = 0;
+    O.global = 0;
+    for (int a = 0; a < O.n; a++) {
+      auto &c = b->cols[1 + a];
+      c.type = O.a[a].out_type;
+      c.precision = O.a[a].out_p;
+      c.scale = O.a[a].out_s;
+      int w = type_width(c.type);
+      c.values.reset(j->ctx, (uint64_t)n * (w > 0 ? w : 1) + 16);
+      c.validity.reset(j->ctx, vb);
+      KG_HIP(hipMemsetAsync(c.validity.get(), 0, vb, j->s));
+      O.a[a].values = c.values.get();
+      O.a[a].validity = c.validity.get<uint8_t>();
+    }
+    DevBuf err(j->ctx, 4);
+    KG_HIP(hipMemsetAsync(err.get(), 0, 4, j->s));
+    launch_agg_output(O, &j->slots.get<Slot>()->key, (int)(sizeof(Slot) / 8),
+                      j->rec.get<uint64_t>(), ns, pos.get<uint64_t>(), kc.values.get(),
+                      kc.validity.get<uint8_t>(), kw, err.get<unsigned int>(), j->s);
+    unsigned int e = 0;
+    KG_HIP(hipMemcpyAsync(&e, err.get(), 4, hipMemcpyDeviceToHost, j->s));
+    KG_HIP(hipStreamSynchronize(j->s));
+    if (e & 1u) fail(KG_ERR_OVERFLOW, "Overflow in sum");
+    if (e & 2u) fail(KG_ERR_OUT_OF_RANGE, "average out of the result decimal's range");
+    *out = b.release();
+  });
+}
+
+void kg_join_destroy(kg_hash_join *j) {
+  if (!j) return;
+  if (j->s) (void)hipStreamSynchronize(j->s);
+  hipStream_t s = j->s;
+  delete j;
+  destroy_stream(s);
+}
+
+}  // extern "C"

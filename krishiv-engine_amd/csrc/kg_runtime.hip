@@ -1,0 +1,329 @@
+// kg_runtime.hip -- context, HBM budget, errors, batches and profiling of
+// libkgpu (the C ABI declared in include/kgpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <string>
+
+#include "kg_common.hpp"
+
+namespace kg {
+
+static thread_local std::string g_last_error;
+
+const char *class_of(int code) {
+  switch (code) {
+    case KG_ERR_UNSUPPORTED: return "KRISHIV_UNSUPPORTED_FEATURE";
+    case KG_ERR_OVERFLOW: return "ARITHMETIC_OVERFLOW";
+    case KG_ERR_OUT_OF_RANGE: return "NUMERIC_VALUE_OUT_OF_RANGE.WITH_SUGGESTION";
+    case KG_ERR_MEMORY: return "UNABLE_TO_ACQUIRE_MEMORY";
+    default: return "INTERNAL_ERROR";
+  }
+}
+
+void set_last_error(int code, const std::string &msg) {
+  g_last_error = std::string(class_of(code)) + ": " + msg;
+}
+
+void *dev_alloc(kg_ctx *ctx, uint64_t bytes) {
+  if (!ctx) fail(KG_ERR_INTERNAL, "no context");
+  uint64_t prev = ctx->in_use.fetch_add(bytes);
+  if (ctx->budget && prev + bytes > ctx->budget) {
+    ctx->in_use.fetch_sub(bytes);
+    fail(KG_ERR_MEMORY, "HBM budget of " + std::to_string(ctx->budget) + " bytes exceeded (" +
+                            std::to_string(prev) + " held, " + std::to_string(bytes) +
+                            " requested)");
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    ctx->in_use.fetch_sub(bytes);
+    (void)hipGetLastError();
+    fail(KG_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+  }
+  return p;
+}
+
+void dev_free(kg_ctx *ctx, void *p, uint64_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  if (ctx) ctx->in_use.fetch_sub(bytes);
+}
+
+int type_width(int t) {
+  switch (t) {
+    case KG_T_BOOL: return 0;
+    case KG_T_INT8:
+    case KG_T_UINT8: return 1;
+    case KG_T_INT16:
+    case KG_T_UINT16: return 2;
+    case KG_T_INT32:
+    case KG_T_UINT32:
+    case KG_T_FLOAT32:
+    case KG_T_DATE32: return 4;
+    case KG_T_INT64:
+    case KG_T_UINT64:
+    case KG_T_FLOAT64:
+    case KG_T_TIMESTAMP:
+    case KG_T_DECIMAL64: return 8;
+    case KG_T_DECIMAL128: return 16;
+    default: return -1;
+  }
+}
+bool is_integral(int t) {
+  return t == KG_T_INT8 || t == KG_T_INT16 || t == KG_T_INT32 || t == KG_T_INT64;
+}
+bool is_float(int t) { return t == KG_T_FLOAT32 || t == KG_T_FLOAT64; }
+bool is_decimal(int t) { return t == KG_T_DECIMAL64 || t == KG_T_DECIMAL128; }
+bool is_int_like(int t) {
+  int w = type_width(t);
+  return w >= 0 && w <= 8 && !is_float(t);
+}
+
+void check_column(const kg_column &c, int64_t nrows, const char *what) {
+  if (type_width(c.type) < 0) fail(KG_ERR_UNSUPPORTED, std::string(what) + ": column type " + std::to_string(c.type));
+  if (c.length < nrows) fail(KG_ERR_INTERNAL, std::string(what) + ": column shorter than the batch");
+  if (c.offset < 0) fail(KG_ERR_INTERNAL, std::string(what) + ": negative offset");
+  if (nrows > 0 && !c.values) fail(KG_ERR_INTERNAL, std::string(what) + ": no values buffer");
+}
+
+DCol Staged::add(const kg_column &c, int64_t nrows) {
+  DCol d{};
+  d.type = c.type;
+  d.precision = c.precision;
+  d.scale = c.scale;
+  d.width = type_width(c.type);
+  if (c.flags & KG_COL_DEVICE) {
+    d.offset = c.offset;
+    d.values = c.values;
+    d.validity = c.validity;
+    return d;
+  }
+  // Host column: copy the rows [offset, offset + nrows) (bit-packed buffers
+  // keep their bit offset within the first byte).
+  d.offset = (d.width == 0 || c.validity) ? (c.offset & 7) : 0;
+  int64_t first = d.width == 0 ? c.offset >> 3 : c.offset - d.offset;
+  uint64_t vbytes = d.width == 0 ? (uint64_t)ceil_div(nrows + d.offset, 8)
+                                 : (uint64_t)(nrows + d.offset) * (uint64_t)d.width;
+  if (nrows > 0) {
+    bufs_.emplace_back(ctx_, vbytes);
+    const uint8_t *src = static_cast<const uint8_t *>(c.values) +
+                         (d.width == 0 ? first : first * d.width);
+    KG_HIP(hipMemcpyAsync(bufs_.back().get(), src, vbytes, hipMemcpyHostToDevice, s_));
+    d.values = bufs_.back().get();
+  }
+  if (c.validity && nrows > 0) {
+    uint64_t nb = (uint64_t)ceil_div(nrows + d.offset, 8);
+    bufs_.emplace_back(ctx_, nb);
+    KG_HIP(hipMemcpyAsync(bufs_.back().get(), c.validity + (c.offset >> 3), nb,
+                          hipMemcpyHostToDevice, s_));
+    d.validity = bufs_.back().get<uint8_t>();
+  }
+  // The caller may free its buffers once we return.
+  KG_HIP(hipStreamSynchronize(s_));
+  return d;
+}
+
+hipStream_t make_stream(int device) {
+  KG_HIP(hipSetDevice(device));
+  hipStream_t s;
+  KG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+void destroy_stream(hipStream_t s) {
+  if (s) (void)hipStreamDestroy(s);
+}
+
+// ---------------------------------------------------------------- profiling
+namespace {
+struct ProfEntry {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0.0;
+  int64_t launches = 0;
+};
+std::mutex g_prof_mu;
+std::map<std::string, ProfEntry> g_prof;
+std::atomic<bool> g_prof_on{false};
+}  // namespace
+
+ProfScope::ProfScope(const char *family, hipStream_t s) : family_(family), s_(s) {
+  if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+    a_ = b_ = nullptr;
+    return;
+  }
+  (void)hipEventRecord(a_, s_);
+}
+
+ProfScope::~ProfScope() {
+  if (!a_) return;
+  (void)hipEventRecord(b_, s_);
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof[family_].pending.emplace_back(a_, b_);
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+extern "C" {
+
+const char *kg_last_error(void) { return g_last_error.c_str(); }
+int kg_abi_version(void) { return KG_ABI_VERSION; }
+
+int kg_ctx_create(int device, uint64_t hbm_budget_bytes, kg_ctx **out) {
+  return guarded([&] {
+    if (!out) fail(KG_ERR_INTERNAL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    KG_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(KG_ERR_INTERNAL, "no such device");
+    KG_HIP(hipSetDevice(device));
+    auto *c = new kg_ctx();
+    c->device = device;
+    size_t free_b = 0, total_b = 0;
+    KG_HIP(hipMemGetInfo(&free_b, &total_b));
+    c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
+    hipDeviceProp_t prop;
+    KG_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount;
+    c->stream = make_stream(device);
+    *out = c;
+  });
+}
+
+void kg_ctx_destroy(kg_ctx *ctx) {
+  if (!ctx) return;
+  destroy_stream(ctx->stream);
+  delete ctx;
+}
+
+uint64_t kg_ctx_hbm_in_use(const kg_ctx *ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+int kg_dev_alloc(kg_ctx *ctx, uint64_t bytes, void **out) {
+  return guarded([&] { *out = dev_alloc(ctx, bytes); });
+}
+
+int kg_dev_free(kg_ctx *ctx, void *p) {
+  return guarded([&] {
+    // Plain allocations are not size-tracked by pointer; ask HIP for it.
+    if (!p) return;
+    size_t sz = 0;
+    hipDeviceptr_t base = nullptr;
+    if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)p) != hipSuccess) sz = 0;
+    dev_free(ctx, p, sz);
+  });
+}
+
+int kg_memcpy_h2d(kg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  return guarded([&] {
+    KG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int kg_memcpy_d2h(kg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  return guarded([&] {
+    KG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int kg_ctx_synchronize(kg_ctx *ctx) {
+  return guarded([&] {
+    KG_HIP(hipSetDevice(ctx->device));
+    KG_HIP(hipDeviceSynchronize());
+  });
+}
+
+int64_t kg_batch_num_rows(const kg_batch *b) { return b ? b->nrows : 0; }
+int32_t kg_batch_num_columns(const kg_batch *b) { return b ? (int32_t)b->cols.size() : 0; }
+
+int kg_batch_column(const kg_batch *b, int32_t i, kg_column *out) {
+  return guarded([&] {
+    if (!b || i < 0 || i >= (int32_t)b->cols.size()) fail(KG_ERR_INTERNAL, "column index");
+    const auto &c = b->cols[i];
+    out->type = c.type;
+    out->precision = c.precision;
+    out->scale = c.scale;
+    out->flags = KG_COL_DEVICE;
+    out->length = b->nrows;
+    out->offset = 0;
+    out->values = c.values.get();
+    out->validity = c.validity.bytes() ? c.validity.get<uint8_t>() : nullptr;
+  });
+}
+
+int kg_batch_column_to_host(const kg_batch *b, int32_t i, void *values, uint8_t *valid_bytes) {
+  return guarded([&] {
+    if (!b || i < 0 || i >= (int32_t)b->cols.size()) fail(KG_ERR_INTERNAL, "column index");
+    const auto &c = b->cols[i];
+    int w = type_width(c.type);
+    hipStream_t s = b->ctx->stream;
+    int64_t n = b->nrows;
+    if (n == 0) return;
+    if (w == 0) {
+      std::vector<uint8_t> bits((size_t)ceil_div(n, 8));
+      KG_HIP(hipMemcpyAsync(bits.data(), c.values.get(), bits.size(), hipMemcpyDeviceToHost, s));
+      KG_HIP(hipStreamSynchronize(s));
+      for (int64_t r = 0; r < n; r++)
+        static_cast<uint8_t *>(values)[r] = (bits[r >> 3] >> (r & 7)) & 1;
+    } else if (values) {
+      KG_HIP(hipMemcpyAsync(values, c.values.get(), (size_t)n * w, hipMemcpyDeviceToHost, s));
+    }
+    if (valid_bytes) {
+      if (c.validity.bytes()) {
+        std::vector<uint8_t> bits((size_t)ceil_div(n, 8));
+        KG_HIP(hipMemcpyAsync(bits.data(), c.validity.get(), bits.size(), hipMemcpyDeviceToHost, s));
+        KG_HIP(hipStreamSynchronize(s));
+        for (int64_t r = 0; r < n; r++) valid_bytes[r] = (bits[r >> 3] >> (r & 7)) & 1;
+      } else {
+        memset(valid_bytes, 1, (size_t)n);
+      }
+    }
+    KG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+void kg_batch_release(kg_batch *b) { delete b; }
+
+void kg_prof_enable(int on) { g_prof_on.store(on != 0); }
+
+void kg_prof_reset(void) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  for (auto &kv : g_prof)
+    for (auto &p : kv.second.pending) {
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+  g_prof.clear();
+}
+
+int kg_prof_get(const char *family, double *total_ms, int64_t *launches) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    auto it = g_prof.find(family);
+    if (it == g_prof.end()) {
+      *total_ms = 0.0;
+      *launches = 0;
+      return;
+    }
+    ProfEntry &e = it->second;
+    for (auto &p : e.pending) {
+      KG_HIP(hipEventSynchronize(p.second));
+      float ms = 0.f;
+      KG_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+      e.ms += ms;
+      e.launches += 1;
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+    e.pending.clear();
+    *total_ms = e.ms;
+    *launches = e.launches;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,9 @@
+"""kgpu: the MI355X (gfx950) execution path for Krishiv's hash join, hash
+aggregation, runtime Bloom filter and hash repartition, behind a C ABI
+(include/kgpu.h) implemented by libkgpu.so.  This package is the Python host
+mirror of the engine's operator interface over that ABI."""
+from ._lib import KgError, lib, exported_symbols  # noqa: F401
+from . import _lib as abi  # noqa: F401
+from .ops import (Batch, Col, Context, DeviceBuffer, HashAggregate, HashJoin,  # noqa: F401
+                  RuntimeBloom, hash_columns, prof_enable, prof_get, prof_reset, repartition,
+                  route)

@@ -1,0 +1,146 @@
+"""ctypes binding of libkgpu.so (include/kgpu.h).
+
+There is no fallback: if the shared library is missing or does not load,
+every entry point raises.  Build it with ``python -m kgpu.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkgpu.so")
+
+OK = 0
+ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_OUT_OF_RANGE, ERR_MEMORY, ERR_INTERNAL, ERR_DEVICE = 1, 2, 3, 4, 5, 6
+
+T_BOOL, T_INT8, T_INT16, T_INT32, T_INT64 = 1, 2, 3, 4, 5
+T_UINT8, T_UINT16, T_UINT32, T_UINT64 = 6, 7, 8, 9
+T_FLOAT32, T_FLOAT64, T_DATE32, T_TIMESTAMP, T_DECIMAL64, T_DECIMAL128 = 10, 11, 12, 13, 14, 15
+COL_DEVICE = 1
+
+AGG_COUNT_STAR, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_AVG = 1, 2, 3, 4, 5, 6
+AGG_ANSI = 1
+JOIN_INNER, JOIN_LEFT_OUTER, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI = 0, 1, 2, 3
+
+WIDTH = {T_BOOL: 0, T_INT8: 1, T_UINT8: 1, T_INT16: 2, T_UINT16: 2, T_INT32: 4, T_UINT32: 4,
+         T_FLOAT32: 4, T_DATE32: 4, T_INT64: 8, T_UINT64: 8, T_FLOAT64: 8, T_TIMESTAMP: 8,
+         T_DECIMAL64: 8, T_DECIMAL128: 16}
+
+
+class KgError(RuntimeError):
+    """A failed libkgpu call: ``code`` and the Spark error class in the text."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class Column(C.Structure):
+    _fields_ = [("type", C.c_int32), ("precision", C.c_int32), ("scale", C.c_int32),
+                ("flags", C.c_uint32), ("length", C.c_int64), ("offset", C.c_int64),
+                ("values", C.c_void_p), ("validity", C.c_void_p)]
+
+
+class AggSpec(C.Structure):
+    _fields_ = [("func", C.c_int32), ("input", C.c_int32)]
+
+
+class JoinSpec(C.Structure):
+    _fields_ = [("join_type", C.c_int32), ("num_keys", C.c_int32),
+                ("probe_keys", C.c_int32 * 4), ("build_keys", C.c_int32 * 4),
+                ("null_safe", C.c_uint8 * 4)]
+
+
+_lib = None
+
+_SIGS = {
+    # name: (restype, argtypes)
+    "kg_last_error": (C.c_char_p, []),
+    "kg_abi_version": (C.c_int, []),
+    "kg_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.c_void_p]),
+    "kg_ctx_destroy": (None, [C.c_void_p]),
+    "kg_ctx_hbm_in_use": (C.c_uint64, [C.c_void_p]),
+    "kg_dev_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kg_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kg_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kg_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "kg_batch_num_rows": (C.c_int64, [C.c_void_p]),
+    "kg_batch_num_columns": (C.c_int32, [C.c_void_p]),
+    "kg_batch_column": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "kg_batch_column_to_host": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "kg_batch_release": (None, [C.c_void_p]),
+    "kg_hash_columns": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                  C.c_uint32]),
+    "kg_partition": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_uint32,
+                               C.c_void_p, C.c_uint32]),
+    "kg_repartition": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32,
+                                 C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "kg_agg_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p]),
+    "kg_agg_sink": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "kg_agg_finalize": (C.c_int, [C.c_void_p]),
+    "kg_agg_result_type": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "kg_agg_result": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_agg_destroy": (None, [C.c_void_p]),
+    "kg_bloom_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kg_bloom_bytes_for": (C.c_uint64, [C.c_uint64, C.c_double]),
+    "kg_bloom_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "kg_bloom_expected_fpp": (C.c_double, [C.c_void_p, C.c_uint64]),
+    "kg_bloom_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_bloom_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "kg_bloom_destroy": (None, [C.c_void_p]),
+    "kg_join_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kg_join_build_sink": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    "kg_join_build_finalize": (C.c_int, [C.c_void_p]),
+    "kg_join_build_rows": (C.c_int64, [C.c_void_p]),
+    "kg_join_probe": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]),
+    "kg_join_destroy": (None, [C.c_void_p]),
+    "kg_joinagg_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                                    C.c_uint32]),
+    "kg_joinagg_probe": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]),
+    "kg_joinagg_result": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_gen_join_build": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
+                                    C.c_void_p, C.c_void_p]),
+    "kg_gen_join_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
+                                    C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_zipf_cdf": (C.c_int, [C.c_uint64, C.c_double, C.c_void_p]),
+    "kg_gen_agg": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_void_p,
+                             C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p]),
+    "kg_prof_enable": (None, [C.c_int]),
+    "kg_prof_reset": (None, []),
+    "kg_prof_get": (C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p]),
+}
+
+
+def exported_symbols():
+    """Every function name include/kgpu.h declares (parsed from the header)."""
+    import re
+    hdr = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "kgpu.h")
+    text = open(hdr).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kg_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """The loaded library (raises OSError when it is missing: no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built (python -m kgpu.build)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue  # reported by the export test; calling it raises AttributeError
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != OK:
+        raise KgError(rc, lib().kg_last_error().decode())

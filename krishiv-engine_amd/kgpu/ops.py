@@ -1,0 +1,477 @@
+"""Host-side mirror of the reference engine's operator interface for the GPU
+path: Arrow RecordBatch (or numpy columns) in, RecordBatch out.
+
+  HashAggregate  -- Sink (sink per batch, finalize) + Source (result):
+                    crates/krishiv-exec/src/aggregate/sink.rs:222,1459-1585
+  HashJoin       -- build Sink + probe Operator sharing one table:
+                    crates/krishiv-exec/src/join/mod.rs:1014-1211
+  JoinAggregate  -- inner join fused with GROUP BY the build key
+  RuntimeBloom   -- crates/krishiv-io/src/runtime_filter/bloom.rs
+  route / repartition -- crates/krishiv-exec/src/exchange/mod.rs:159-212,447-501
+
+Every method calls libkgpu through ctypes; nothing is computed here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, JoinSpec, KgError, check, lib,
+                   T_BOOL, T_DECIMAL128, T_DECIMAL64, T_INT8, T_INT16, T_INT32, T_INT64,
+                   T_UINT8, T_UINT16, T_UINT32, T_UINT64, T_FLOAT32, T_FLOAT64, T_DATE32,
+                   T_TIMESTAMP)
+
+_NP = {T_INT8: np.int8, T_INT16: np.int16, T_INT32: np.int32, T_INT64: np.int64,
+       T_UINT8: np.uint8, T_UINT16: np.uint16, T_UINT32: np.uint32, T_UINT64: np.uint64,
+       T_FLOAT32: np.float32, T_FLOAT64: np.float64, T_DATE32: np.int32, T_TIMESTAMP: np.int64,
+       T_DECIMAL64: np.int64, T_BOOL: np.uint8}
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One GPU and an HBM budget (kg_ctx)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        check(lib().kg_ctx_create(device, hbm_budget, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().kg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def hbm_in_use(self) -> int:
+        return lib().kg_ctx_hbm_in_use(self.h)
+
+    def synchronize(self):
+        check(lib().kg_ctx_synchronize(self.h))
+
+
+class DeviceBuffer:
+    """Plain HBM owned by the caller (inputs kept resident)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = max(int(nbytes), 8)
+        p = C.c_void_p()
+        check(lib().kg_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @staticmethod
+    def from_numpy(ctx: Context, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = DeviceBuffer(ctx, a.nbytes)
+        if a.nbytes:
+            check(lib().kg_memcpy_h2d(ctx.h, b.ptr, a.ctypes.data, a.nbytes))
+        return b
+
+    def to_numpy(self, dtype, count) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().kg_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().kg_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- columns
+class Col:
+    """A column for the C ABI: host numpy/pyarrow buffers, or device buffers.
+
+    ``values`` may be a numpy array (host), a DeviceBuffer (device), or None.
+    ``valid`` a bool numpy array (host, converted to an LSB bitmap), a
+    DeviceBuffer holding an LSB bitmap, or None (no NULLs)."""
+
+    def __init__(self, ctype, values, valid=None, precision=0, scale=0, length=None, offset=0):
+        self.type, self.precision, self.scale = ctype, precision, scale
+        self.offset = offset
+        self._keep = []
+        self.device = isinstance(values, DeviceBuffer)
+        if self.device:
+            self.length = int(length)
+            self.vptr = values.ptr
+            self._keep.append(values)
+            if isinstance(valid, DeviceBuffer):
+                self.bptr = valid.ptr
+                self._keep.append(valid)
+            elif valid is None:
+                self.bptr = None
+            else:
+                raise TypeError("a device column takes a device validity bitmap")
+            return
+        if ctype == T_BOOL:
+            v = np.asarray(values, dtype=bool)
+            self.length = len(v) if length is None else length
+            self.values = np.packbits(v, bitorder="little")
+        elif ctype == T_DECIMAL128:
+            v = np.asarray(values)
+            if v.dtype != np.uint64 or v.ndim != 2:
+                ints = [int(x) for x in values]
+                v = np.zeros((len(ints), 2), dtype=np.uint64)
+                for i, x in enumerate(ints):
+                    u = x & ((1 << 128) - 1)
+                    v[i, 0] = u & 0xFFFFFFFFFFFFFFFF
+                    v[i, 1] = u >> 64
+            self.values = np.ascontiguousarray(v)
+            self.length = self.values.shape[0] if length is None else length
+        else:
+            self.values = np.ascontiguousarray(values, dtype=_NP[ctype])
+            self.length = len(self.values) if length is None else length
+        self.vptr = self.values.ctypes.data if self.values.size else None
+        if valid is None:
+            self.bitmap = None
+        else:
+            self.bitmap = np.packbits(np.asarray(valid, dtype=bool), bitorder="little")
+        self.bptr = None if self.bitmap is None else self.bitmap.ctypes.data
+
+    @staticmethod
+    def from_arrow(arr) -> "Col":
+        """Zero-copy view of a pyarrow primitive array's buffers."""
+        import pyarrow as pa
+        arr = arr.combine_chunks() if isinstance(arr, pa.ChunkedArray) else arr
+        t = arr.type
+        m = {pa.int8(): T_INT8, pa.int16(): T_INT16, pa.int32(): T_INT32, pa.int64(): T_INT64,
+             pa.uint8(): T_UINT8, pa.uint16(): T_UINT16, pa.uint32(): T_UINT32,
+             pa.uint64(): T_UINT64, pa.float32(): T_FLOAT32, pa.float64(): T_FLOAT64,
+             pa.date32(): T_DATE32, pa.bool_(): T_BOOL}
+        prec = scale = 0
+        if t in m:
+            ctype = m[t]
+        elif pa.types.is_decimal(t):
+            prec, scale = t.precision, t.scale
+            ctype = T_DECIMAL128 if t.bit_width == 128 else T_DECIMAL64
+            if t.bit_width not in (64, 128):
+                raise KgError(1, "KRISHIV_UNSUPPORTED_FEATURE: decimal width")
+        elif pa.types.is_timestamp(t) or pa.types.is_date64(t) or pa.types.is_duration(t):
+            ctype = T_TIMESTAMP
+        else:
+            raise KgError(1, f"KRISHIV_UNSUPPORTED_FEATURE: {t}")
+        bufs = arr.buffers()
+        c = Col.__new__(Col)
+        c.type, c.precision, c.scale = ctype, prec, scale
+        c.length = len(arr)
+        c.offset = arr.offset
+        c.device = False
+        c._keep = [arr]
+        c.vptr = bufs[1].address if bufs[1] is not None else None
+        c.bptr = bufs[0].address if (bufs[0] is not None and arr.null_count > 0) else None
+        return c
+
+    def c(self) -> Column:
+        return Column(self.type, self.precision, self.scale, COL_DEVICE if self.device else 0,
+                      self.length, self.offset, self.vptr, self.bptr)
+
+
+def _cols(cols):
+    arr = (Column * max(1, len(cols)))()
+    for i, c in enumerate(cols):
+        arr[i] = c.c()
+    return arr
+
+
+def _type_col(ctype, precision=0, scale=0) -> Column:
+    return Column(ctype, precision, scale, 0, 0, 0, None, None)
+
+
+# ---------------------------------------------------------------- batches
+class Batch:
+    """A device-resident batch (kg_batch) owned by the library."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().kg_batch_release(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def num_rows(self) -> int:
+        return lib().kg_batch_num_rows(self.h)
+
+    @property
+    def num_columns(self) -> int:
+        return lib().kg_batch_num_columns(self.h)
+
+    def device_column(self, i) -> Column:
+        c = Column()
+        check(lib().kg_batch_column(self.h, i, C.byref(c)))
+        return c
+
+    def as_col(self, i) -> Col:
+        """Column i as a device Col (no copy) for another operator."""
+        d = self.device_column(i)
+        c = Col.__new__(Col)
+        c.type, c.precision, c.scale = d.type, d.precision, d.scale
+        c.length, c.offset, c.device = d.length, 0, True
+        c.vptr, c.bptr, c._keep = d.values, d.validity, [self]
+        return c
+
+    def column(self, i):
+        """(values, valid) on the host: numpy values (python ints for
+        decimal128) and a bool validity array."""
+        d = self.device_column(i)
+        n = self.num_rows
+        valid = np.ones(n, dtype=np.uint8)
+        if d.type == T_DECIMAL128:
+            raw = np.zeros((n, 2), dtype=np.uint64)
+            check(lib().kg_batch_column_to_host(self.h, i, raw.ctypes.data, valid.ctypes.data))
+            vals = [((int(h) << 64) | int(l)) for l, h in raw]
+            vals = [x - (1 << 128) if x >= (1 << 127) else x for x in vals]
+            return vals, valid.astype(bool)
+        dt = _NP[d.type]
+        vals = np.zeros(n, dtype=dt)
+        check(lib().kg_batch_column_to_host(self.h, i, vals.ctypes.data, valid.ctypes.data))
+        if d.type == T_BOOL:
+            vals = vals.astype(bool)
+        return vals, valid.astype(bool)
+
+    def column_type(self, i):
+        d = self.device_column(i)
+        return d.type, d.precision, d.scale
+
+    def to_arrow(self, names=None):
+        """The batch as a pyarrow RecordBatch (host copy)."""
+        import pyarrow as pa
+        arrays = []
+        for i in range(self.num_columns):
+            t, p, s = self.column_type(i)
+            vals, valid = self.column(i)
+            mask = ~valid
+            if t == T_DECIMAL128 or t == T_DECIMAL64:
+                from decimal import Decimal
+                typ = pa.decimal128(max(p, 1), s)
+                pyv = [None if m else Decimal(int(v)).scaleb(-s) for v, m in zip(vals, mask)]
+                arrays.append(pa.array(pyv, type=typ))
+            elif t == T_DATE32:
+                arrays.append(pa.array(vals, mask=mask, type=pa.date32()))
+            else:
+                arrays.append(pa.array(vals, mask=mask))
+        names = names or [f"c{i}" for i in range(len(arrays))]
+        return pa.RecordBatch.from_arrays(arrays, names=names)
+
+
+# ---------------------------------------------------------------- exchange
+def hash_columns(ctx: Context, cols) -> np.ndarray:
+    n = cols[0].length
+    out = np.zeros(n, dtype=np.uint64)
+    ca = _cols(cols)
+    check(lib().kg_hash_columns(ctx.h, len(cols), ca, n, out.ctypes.data, 0))
+    return out
+
+
+def route(ctx: Context, keys, partitions: int) -> np.ndarray:
+    n = keys[0].length
+    out = np.zeros(n, dtype=np.uint32)
+    ca = _cols(keys)
+    check(lib().kg_partition(ctx.h, len(keys), ca, n, partitions, out.ctypes.data, 0))
+    return out
+
+
+def repartition(ctx: Context, cols, key_cols, partitions: int):
+    """Rows regrouped by partition (stable): (Batch, offsets[partitions+1])."""
+    n = cols[0].length
+    ca = _cols(cols)
+    kc = (C.c_int32 * max(1, len(key_cols)))(*key_cols)
+    offs = np.zeros(partitions + 1, dtype=np.int64)
+    h = C.c_void_p()
+    check(lib().kg_repartition(ctx.h, len(cols), ca, n, len(key_cols), kc, partitions,
+                               offs.ctypes.data, C.byref(h)))
+    return Batch(h), offs
+
+
+# ---------------------------------------------------------------- aggregation
+class HashAggregate:
+    """GROUP BY one key column with count/sum/min/max/avg aggregates."""
+
+    def __init__(self, ctx: Context, key_type, input_types, aggs, ansi=False,
+                 expected_groups=1 << 16):
+        self.ctx = ctx
+        kt = _type_col(*key_type) if key_type is not None else None
+        its = (Column * max(1, len(input_types)))(*[_type_col(*t) for t in input_types])
+        sp = (AggSpec * max(1, len(aggs)))(*[AggSpec(f, i) for f, i in aggs])
+        h = C.c_void_p()
+        check(lib().kg_agg_create(ctx.h, C.byref(kt) if kt is not None else None,
+                                  len(input_types), its, len(aggs), sp,
+                                  AGG_ANSI if ansi else 0, expected_groups, C.byref(h)))
+        self.h = h
+        self.naggs = len(aggs)
+        self.has_key = key_type is not None
+
+    def sink(self, key: Col | None, inputs, nrows=None):
+        if nrows is None:
+            nrows = key.length if key is not None else inputs[0].length
+        kc = key.c() if key is not None else None
+        ca = _cols(inputs)
+        check(lib().kg_agg_sink(self.h, C.byref(kc) if kc is not None else None, ca, nrows))
+
+    def finalize(self):
+        check(lib().kg_agg_finalize(self.h))
+
+    def result_type(self, i):
+        c = Column()
+        check(lib().kg_agg_result_type(self.h, i, C.byref(c)))
+        return c.type, c.precision, c.scale
+
+    def result(self) -> Batch:
+        h = C.c_void_p()
+        check(lib().kg_agg_result(self.h, C.byref(h)))
+        return Batch(h)
+
+    def close(self):
+        if self.h:
+            lib().kg_agg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- join
+class HashJoin:
+    def __init__(self, ctx: Context, join_type, probe_keys, build_keys, null_safe=None,
+                 expected_build_rows=1 << 16):
+        self.ctx = ctx
+        spec = JoinSpec()
+        spec.join_type = join_type
+        spec.num_keys = len(probe_keys)
+        for i, (p, b) in enumerate(zip(probe_keys, build_keys)):
+            spec.probe_keys[i] = p
+            spec.build_keys[i] = b
+            spec.null_safe[i] = int(bool(null_safe[i])) if null_safe else 0
+        h = C.c_void_p()
+        check(lib().kg_join_create(ctx.h, C.byref(spec), expected_build_rows, C.byref(h)))
+        self.h = h
+
+    def build_sink(self, cols, nrows=None):
+        nrows = cols[0].length if nrows is None else nrows
+        check(lib().kg_join_build_sink(self.h, len(cols), _cols(cols), nrows))
+
+    def build_finalize(self):
+        check(lib().kg_join_build_finalize(self.h))
+
+    def build_rows(self) -> int:
+        return lib().kg_join_build_rows(self.h)
+
+    def probe(self, cols, nrows=None) -> Batch:
+        nrows = cols[0].length if nrows is None else nrows
+        h = C.c_void_p()
+        check(lib().kg_join_probe(self.h, len(cols), _cols(cols), nrows, C.byref(h)))
+        return Batch(h)
+
+    # join fused with GROUP BY the build key
+    def aggregate(self, input_types, aggs, ansi=False):
+        its = (Column * max(1, len(input_types)))(*[_type_col(*t) for t in input_types])
+        sp = (AggSpec * max(1, len(aggs)))(*[AggSpec(f, i) for f, i in aggs])
+        check(lib().kg_joinagg_create(self.h, len(input_types), its, len(aggs), sp,
+                                      AGG_ANSI if ansi else 0))
+
+    def probe_aggregate(self, cols, input_cols, nrows=None):
+        nrows = cols[0].length if nrows is None else nrows
+        ic = (C.c_int32 * max(1, len(input_cols)))(*input_cols)
+        check(lib().kg_joinagg_probe(self.h, len(cols), _cols(cols), nrows, ic))
+
+    def aggregate_result(self) -> Batch:
+        h = C.c_void_p()
+        check(lib().kg_joinagg_result(self.h, C.byref(h)))
+        return Batch(h)
+
+    def close(self):
+        if self.h:
+            lib().kg_join_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- bloom
+class RuntimeBloom:
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().kg_bloom_create(ctx.h, nbytes, C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def bytes_for(items: int, fpp: float) -> int:
+        return lib().kg_bloom_bytes_for(items, fpp)
+
+    def insert(self, key: Col, select: Col | None = None, nrows=None):
+        nrows = key.length if nrows is None else nrows
+        kc = key.c()
+        sc = select.c() if select is not None else None
+        check(lib().kg_bloom_insert(self.h, C.byref(kc), C.byref(sc) if sc is not None else None,
+                                    nrows))
+
+    def expected_fpp(self, items: int) -> float:
+        return lib().kg_bloom_expected_fpp(self.h, items)
+
+    def to_bytes(self) -> bytes:
+        n = C.c_uint64(0)
+        check(lib().kg_bloom_export(self.h, None, C.byref(n)))
+        buf = (C.c_uint8 * n.value)()
+        check(lib().kg_bloom_export(self.h, buf, C.byref(n)))
+        return bytes(buf)
+
+    def probe(self, key: Col, out_rows: DeviceBuffer | None = None, nrows=None):
+        """Row ids that may match: (DeviceBuffer of int32 ids, count)."""
+        nrows = key.length if nrows is None else nrows
+        out = out_rows or DeviceBuffer(self.ctx, max(nrows, 1) * 4)
+        cnt = C.c_int64(0)
+        kc = key.c()
+        check(lib().kg_bloom_probe(self.h, C.byref(kc), nrows, C.c_void_p(out.ptr), C.byref(cnt)))
+        return out, cnt.value
+
+    def close(self):
+        if self.h:
+            lib().kg_bloom_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- profiling
+def prof_enable(on=True):
+    lib().kg_prof_enable(int(on))
+
+
+def prof_reset():
+    lib().kg_prof_reset()
+
+
+def prof_get(family: str):
+    ms = C.c_double(0)
+    n = C.c_int64(0)
+    check(lib().kg_prof_get(family.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value

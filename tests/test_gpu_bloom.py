@@ -1,0 +1,91 @@
+"""GPU parity of the runtime Bloom filter: the device-built split-block bit
+array is byte-identical to the oracle's (and to the Parquet specification
+vector); the device probe selects exactly the rows the oracle's check
+passes, in input order."""
+import hashlib
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from gpu_util import ctx, say
+import kgo
+import kgpu
+from kgpu import Col
+from kgpu import abi as A
+
+pytestmark = pytest.mark.gpu
+V = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                "vectors.json")))
+
+
+def test_parquet_spec_vector_on_device():
+    vals = np.array([i * 7919 - 3 for i in range(500)], dtype=np.int64)
+    b = kgpu.RuntimeBloom(ctx(), 1024)
+    b.insert(Col(A.T_INT64, vals))
+    assert b.to_bytes() == bytes.fromhex(V["sbbf_1024"]["bytes_hex"])
+
+
+def test_config3_filter_size_and_digest():
+    say("bloom: 750K keys at fpp 0.03")
+    nbytes = kgpu.RuntimeBloom.bytes_for(750000, 0.03)
+    assert nbytes == 723968
+    keys = np.arange(750000, dtype=np.int64) * 20 + 3
+    b = kgpu.RuntimeBloom(ctx(), nbytes)
+    b.insert(Col(A.T_INT64, keys))
+    assert hashlib.sha256(b.to_bytes()).hexdigest() == V["sbbf_config3"]["sha256"]
+    assert b.expected_fpp(750000) == kgo.lib().kgo_sbbf_expected_fpp(nbytes // 32, 750000)
+
+
+def test_build_with_predicate_and_probe_matches_oracle():
+    """Config-3 shape at 1/100: build keys kept by attr < 5, probe column
+    over the full key range; survivors = the oracle's check, in order."""
+    say("bloom: build with selection, probe 6M rows")
+    nb, npr = 150_000, 6_000_000
+    key, _ = kgo.gen_join_build(4, 0, nb, nb, with_val=False)
+    attr = (np.arange(nb) * 7919 % 100).astype(np.int32)
+    keep = attr < 5
+    kept = key[keep]
+    nbytes = kgpu.RuntimeBloom.bytes_for(len(kept), 0.03)
+    b = kgpu.RuntimeBloom(ctx(), nbytes)
+    b.insert(Col(A.T_INT64, key), Col(A.T_BOOL, keep))
+    h = kgo.hash_columns([kgo.Col(kgo.T_INT64, kept)])
+    ok, words = kgo.sbbf_build(h, nbytes=nbytes)
+    assert b.to_bytes() == words.tobytes()
+    fk, fkv, _ = kgo.gen_join_probe(5, 0, npr, nb, 10)
+    rows, cnt = b.probe(Col(A.T_INT64, fk, fkv))
+    got = rows.to_numpy(np.int32, cnt)
+    ph = kgo.hash_columns([kgo.Col(kgo.T_INT64, fk)])
+    L = kgo.lib()
+    nblk = len(words) // 8
+    # vectorized oracle check over the same words (the per-hash C check on a sample)
+    blocks = ((ph >> np.uint64(32)) * np.uint64(nblk)) >> np.uint64(32)
+    salt = np.array([0x47b6137b, 0x44974d91, 0x8824ad5b, 0xa2b7289d, 0x705495c7, 0x2df1424b,
+                     0x9efc4947, 0x5c6bfb31], dtype=np.uint64)
+    key32 = (ph & np.uint64(0xFFFFFFFF))
+    bits = ((key32[:, None] * salt[None, :]) & np.uint64(0xFFFFFFFF)) >> np.uint64(27)
+    wv = words.reshape(-1, 8)[blocks.astype(np.int64)].astype(np.uint64)
+    passes = np.all((wv >> bits) & np.uint64(1), axis=1) & fkv
+    for i in np.random.default_rng(0).integers(0, npr, 2000):
+        assert bool(passes[i]) == (bool(fkv[i]) and bool(L.kgo_sbbf_check(
+            words.ctypes.data, nblk, int(ph[i]))))
+    want = np.nonzero(passes)[0]
+    assert np.array_equal(got, want)
+    # no false negatives: every probe key that is a kept build key passes
+    kept_set = np.zeros(nb, dtype=bool)
+    kept_set[kept] = True
+    true_hits = np.nonzero(fkv & kept_set[fk])[0]
+    assert np.all(np.isin(true_hits, got))
+
+
+def test_probe_edge_sizes():
+    b = kgpu.RuntimeBloom(ctx(), 4096)
+    b.insert(Col(A.T_INT64, np.arange(0, 3000, 3, dtype=np.int64)))
+    for n in (0, 1, 63, 64, 65, 8191, 8192, 8193, 20000):
+        keys = np.arange(n, dtype=np.int64)
+        rows, cnt = b.probe(Col(A.T_INT64, keys))
+        got = rows.to_numpy(np.int32, cnt)
+        assert np.all(np.diff(got) > 0)
+        assert np.all(np.isin(np.arange(0, min(n, 3000), 3), got))

@@ -1,0 +1,182 @@
+"""GPU parity of the hash join (build sink + probe operator) and of the join
+fused with GROUP BY the build key, against the oracle: join outputs compared
+as sorted row multisets with Spark NULL semantics, aggregates bit-exact."""
+import numpy as np
+import pytest
+
+from gpu_util import ctx, say
+import kgo
+import kgpu
+from kgpu import Col
+from kgpu import abi as A
+
+pytestmark = pytest.mark.gpu
+
+JT = {A.JOIN_INNER: kgo.JOIN_INNER, A.JOIN_LEFT_OUTER: kgo.JOIN_LEFT_OUTER,
+      A.JOIN_LEFT_SEMI: kgo.JOIN_LEFT_SEMI, A.JOIN_LEFT_ANTI: kgo.JOIN_LEFT_ANTI}
+
+
+def gpu_join(jt, probe_cols, build_cols, pk, bk, null_safe=None, build_batches=1):
+    j = kgpu.HashJoin(ctx(), jt, pk, bk, null_safe, expected_build_rows=1024)
+    n = build_cols[0].length
+    cuts = np.linspace(0, n, build_batches + 1).astype(int)
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        j.build_sink([Col(c.type, c.values_np[a:b], None if c.valid_np is None else c.valid_np[a:b],
+                          c.precision, c.scale) for c in build_cols], b - a)
+    j.build_finalize()
+    out = j.probe(probe_cols)
+    return j, out
+
+
+def mk(ctype, values, valid=None):
+    c = Col(ctype, values, valid)
+    c.values_np = np.asarray(values)
+    c.valid_np = None if valid is None else np.asarray(valid)
+    return c
+
+
+def rows_of(batch, ncols):
+    cols = [batch.column(i) for i in range(ncols)]
+    out = []
+    for r in range(batch.num_rows):
+        out.append(tuple((int(v[r]) if ok[r] else None) for v, ok in cols))
+    return sorted(out, key=lambda t: tuple((x is None, x or 0) for x in t))
+
+
+def expected_rows(jt, pairs, probe, build):
+    op, ob = pairs
+    out = []
+    for i, j in zip(op, ob):
+        row = [(int(v[i]) if ok is None or ok[i] else None) for v, ok in probe]
+        if jt in (A.JOIN_INNER, A.JOIN_LEFT_OUTER):
+            row += [None] * len(build) if j < 0 else \
+                [(int(v[j]) if ok is None or ok[j] else None) for v, ok in build]
+        out.append(tuple(row))
+    return sorted(out, key=lambda t: tuple((x is None, x or 0) for x in t))
+
+
+@pytest.mark.parametrize("jt", [A.JOIN_INNER, A.JOIN_LEFT_OUTER, A.JOIN_LEFT_SEMI,
+                                A.JOIN_LEFT_ANTI])
+def test_join_types_with_duplicates_and_nulls(jt):
+    say(f"join: type {jt}")
+    rng = np.random.default_rng(10 + jt)
+    nb, npr = 3000, 8000
+    bk = rng.integers(0, 2000, nb)
+    bv = rng.random(nb) > 0.05
+    bpay = rng.integers(-(1 << 40), 1 << 40, nb)
+    bpv = rng.random(nb) > 0.2
+    pk = rng.integers(0, 2500, npr)
+    pv = rng.random(npr) > 0.05
+    ppay = np.arange(npr, dtype=np.int64)
+    build = [mk(A.T_INT64, bk, bv), mk(A.T_INT64, bpay, bpv)]
+    probe = [mk(A.T_INT64, pk, pv), mk(A.T_INT64, ppay)]
+    j, out = gpu_join(jt, probe, build, [0], [0], build_batches=3)
+    pairs = kgo.join(JT[jt], [kgo.Col(kgo.T_INT64, pk, pv)], [kgo.Col(kgo.T_INT64, bk, bv)])
+    want = expected_rows(jt, pairs, [(pk, pv), (ppay, None)], [(bk, bv), (bpay, bpv)])
+    ncols = 4 if jt in (A.JOIN_INNER, A.JOIN_LEFT_OUTER) else 2
+    assert out.num_columns == ncols
+    assert rows_of(out, ncols) == want
+
+
+def test_null_safe_key_and_float_normalization():
+    probe = [mk(A.T_FLOAT64, np.array([0.0, np.nan, 1.0, 2.0]), np.array([1, 1, 1, 0], bool))]
+    build = [mk(A.T_FLOAT64, np.array([-0.0, -np.nan, 3.0, 0.0]), np.array([1, 1, 1, 0], bool)),
+             mk(A.T_INT64, np.array([10, 11, 12, 13]))]
+    for ns, want in ((None, [(0, 10), (1, 11)]), ([1], [(0, 10), (1, 11), (3, 13)])):
+        j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0], ns)
+        j.build_sink(build)
+        j.build_finalize()
+        out = j.probe(probe + [mk(A.T_INT64, np.arange(4))])
+        # columns: probe f64, probe row id, build f64, build payload
+        rid, _ = out.column(1)
+        pay, _ = out.column(3)
+        assert sorted(zip(rid.tolist(), pay.tolist())) == want
+
+
+def test_two_int32_keys():
+    rng = np.random.default_rng(5)
+    nb, npr = 2000, 5000
+    b1, b2 = rng.integers(0, 30, nb).astype(np.int32), rng.integers(-20, 20, nb).astype(np.int32)
+    p1, p2 = rng.integers(0, 35, npr).astype(np.int32), rng.integers(-25, 25, npr).astype(np.int32)
+    build = [mk(A.T_INT32, b1), mk(A.T_INT32, b2), mk(A.T_INT64, np.arange(nb))]
+    probe = [mk(A.T_INT32, p1), mk(A.T_INT32, p2), mk(A.T_INT64, np.arange(npr))]
+    j, out = gpu_join(A.JOIN_INNER, probe, build, [0, 1], [0, 1])
+    pairs = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT32, p1), kgo.Col(kgo.T_INT32, p2)],
+                     [kgo.Col(kgo.T_INT32, b1), kgo.Col(kgo.T_INT32, b2)])
+    got = sorted(zip(out.column(2)[0].tolist(), out.column(5)[0].tolist()))
+    assert got == sorted(zip(pairs[0].tolist(), pairs[1].tolist()))
+
+
+def test_config1_shape_join_then_aggregate_and_fused():
+    """SELECT key, SUM(amt), COUNT(*) FROM build JOIN probe ON key = fk
+    GROUP BY key: 100K build (a permutation), 800K probe (fk over 1.2x the
+    range, 1 % NULL): join -> device aggregate, and the fused form, both
+    bit-exact against the oracle."""
+    say("join: config-1 shape")
+    nb, npr = 100_000, 800_000
+    key, val = kgo.gen_join_build(1, 0, nb, nb)
+    fk, fkv, amt = kgo.gen_join_probe(2, 0, npr, int(nb * 1.2), 10)
+    # oracle: join pairs, then the aggregate over the pairs
+    op, ob = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, fk, fkv)], [kgo.Col(kgo.T_INT64, key)])
+    o = kgo.Agg((kgo.T_INT64, 0, 0), [(kgo.T_INT64, 0, 0)],
+                [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0)])
+    o.update(kgo.Col(kgo.T_INT64, key[ob]), [kgo.Col(kgo.T_INT64, amt[op])])
+    rk, _, rres = o.finish()
+    want = sorted(zip(rk.tolist(), rres[0][0].tolist(), rres[1][0].tolist()))
+    # device: join, then aggregate the join output in HBM (no copy)
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0], expected_build_rows=nb)
+    j.build_sink([Col(A.T_INT64, key), Col(A.T_INT64, val)])
+    j.build_finalize()
+    out = j.probe([Col(A.T_INT64, fk, fkv), Col(A.T_INT64, amt)])
+    g = kgpu.HashAggregate(ctx(), (A.T_INT64, 0, 0), [(A.T_INT64, 0, 0)],
+                           [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)], expected_groups=nb)
+    g.sink(out.as_col(2), [out.as_col(1)], out.num_rows)
+    g.finalize()
+    r = g.result()
+    got = sorted(zip(r.column(0)[0].tolist(), r.column(1)[0].tolist(), r.column(2)[0].tolist()))
+    assert got == want
+    # fused: the aggregate state in the build table's slots
+    j.aggregate([(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)])
+    j.probe_aggregate([Col(A.T_INT64, fk, fkv), Col(A.T_INT64, amt)], [1])
+    r = j.aggregate_result()
+    got = sorted(zip(r.column(0)[0].tolist(), r.column(1)[0].tolist(), r.column(2)[0].tolist()))
+    assert got == want
+
+
+def test_fused_aggregate_with_duplicate_build_keys():
+    rng = np.random.default_rng(9)
+    bk = rng.integers(0, 500, 3000)
+    pk = rng.integers(0, 600, 20000)
+    amt = rng.integers(-1000, 1000, 20000)
+    f = rng.random(20000)
+    op, ob = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, pk)], [kgo.Col(kgo.T_INT64, bk)])
+    aggs = [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0), (kgo.AGG_MIN, 0), (kgo.AGG_MAX, 1),
+            (kgo.AGG_AVG, 0)]
+    o = kgo.Agg((kgo.T_INT64, 0, 0), [(kgo.T_INT64, 0, 0), (kgo.T_FLOAT64, 0, 0)], aggs)
+    o.update(kgo.Col(kgo.T_INT64, bk[ob]), [kgo.Col(kgo.T_INT64, amt[op]),
+                                            kgo.Col(kgo.T_FLOAT64, f[op])])
+    rk, _, rres = o.finish()
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0])
+    j.build_sink([Col(A.T_INT64, bk)])
+    j.build_finalize()
+    j.aggregate([(A.T_INT64, 0, 0), (A.T_FLOAT64, 0, 0)], aggs)
+    j.probe_aggregate([Col(A.T_INT64, pk), Col(A.T_INT64, amt), Col(A.T_FLOAT64, f)], [1, 2])
+    r = j.aggregate_result()
+    gk = r.column(0)[0]
+    gi = {int(k): i for i, k in enumerate(gk)}
+    assert len(gi) == len(rk)
+    for ri, k in enumerate(rk):
+        i = gi[int(k)]
+        assert int(r.column(1)[0][i]) == int(rres[0][0][ri])
+        assert int(r.column(2)[0][i]) == int(rres[1][0][ri])
+        assert int(r.column(3)[0][i]) == int(rres[2][0][ri])
+        assert float(r.column(4)[0][i]) == float(rres[3][0][ri])
+        assert abs(float(r.column(5)[0][i]) - float(rres[4][0][ri])) <= 1e-12 * abs(float(rres[4][0][ri]))
+
+
+def test_empty_probe_and_no_match():
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0])
+    j.build_sink([Col(A.T_INT64, np.array([1, 2, 3]))])
+    j.build_finalize()
+    assert j.probe([Col(A.T_INT64, np.array([], dtype=np.int64))]).num_rows == 0
+    assert j.probe([Col(A.T_INT64, np.array([7, 8]))]).num_rows == 0
