@@ -141,6 +141,15 @@ int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nr
 #define KG_AGG_AVG 6
 
 #define KG_AGG_ANSI 1u /* spark.sql.ansi.enabled: overflow raises */
+/* PartialAggregate (aggregate/partial.rs): the result holds each aggregate's
+ * partial-state columns (state_fields, aggregate/mod.rs:747-775) instead of
+ * its final value: COUNT -> count; SUM int/float -> sum; SUM decimal ->
+ * (sum, nonempty); AVG -> (sum, count); MIN/MAX -> value. */
+#define KG_AGG_PARTIAL 2u
+/* The final phase of a two-phase aggregate: each sink call's inputs are the
+ * partial-state columns above, in aggregate order (input_types and the
+ * aggregate list stay the original argument types and functions). */
+#define KG_AGG_MERGE 4u
 
 typedef struct kg_agg_spec {
   int32_t func;  /* KG_AGG_* */

@@ -46,13 +46,43 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
     for (int t = threadIdx.x; t < SLOTS * W; t += blockDim.x) s_rec[t] = P.init[t % W];
     __syncthreads();
     int64_t end = base + chunk < nrows ? base + chunk : nrows;
+    if (P.direct || !P.has_key) {
+      // few groups: each wave reduces its rows per distinct group in
+      // registers and applies them once (lanes iterate in step: every lane
+      // takes part in the ballots and shuffles)
+      if (P.direct && threadIdx.x < 257) s_keys[threadIdx.x] = 0;  // every direct slot valid
+      for (int64_t b = base; b < end; b += blockDim.x) {
+        int64_t i = b + threadIdx.x;
+        bool active = i < end;
+        int ls = 0;
+        if (active && P.direct) {
+          uint64_t key;
+          int cls = row_key(P, i, key);
+          ls = cls == 2 ? 256 : (int)(key & 0xFF);
+        }
+        uint64_t todo = __ballot(active);
+        while (todo) {
+          int lead = __ffsll((unsigned long long)todo) - 1;
+          int ks = __shfl(ls, lead);
+          uint64_t peers = __ballot(active && ls == ks);
+          apply_wave<true>(P, s_rec + (uint64_t)ks * W, i, peers);
+          todo &= ~peers;
+        }
+      }
+      __syncthreads();
+      for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) {
+        const uint64_t *l = s_rec + (uint64_t)t * W;
+        if (l[0] == 0 || (!P.direct && t != 0)) continue;
+        merge_record(P, global_record(P, T, (P.direct && t == 256) ? 2 : 0, (uint64_t)t), l);
+      }
+      __syncthreads();
+      continue;
+    }
     for (int64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
       uint64_t key;
       int cls = row_key(P, i, key);
       int ls = -1;
-      if (P.direct) {
-        ls = cls == 2 ? 256 : (int)(key & 0xFF);
-      } else if (cls == 0) {
+      if (cls == 0) {
         uint64_t s = (P.has_key ? slot_hash(T, key) : 0) & (SLOTS - 1);
         for (int p = 0; p < kLdsProbe; p++) {
           uint64_t k = s_keys[s];
@@ -71,7 +101,6 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
         }
       }
       if (ls >= 0) {
-        if (P.direct) s_keys[ls] = cls == 2 ? 1 : 0;  // marks the direct slot used
         apply_row<true>(P, s_rec + (uint64_t)ls * W, i);
       } else {
         apply_row<false>(P, global_record(P, T, cls, key), i);
@@ -83,13 +112,7 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
       if (k == kEmpty) continue;
       const uint64_t *l = s_rec + (uint64_t)t * W;
       if (l[0] == 0) continue;  // no row
-      uint64_t *g;
-      if (P.direct) {
-        g = global_record(P, T, t == 256 ? 2 : 0, (uint64_t)t);
-      } else {
-        g = global_record(P, T, 0, k);
-      }
-      merge_record(P, g, l);
+      merge_record(P, global_record(P, T, 0, k), l);
     }
     __syncthreads();
   }
@@ -175,6 +198,39 @@ __global__ void k_abs_sum(DCol c, int64_t nrows, double *out) {
   if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, acc);
 }
 
+__device__ __forceinline__ void store_dec(int type, void *values, int64_t o, uint64_t lo,
+                                          uint64_t hi) {
+  if (type == KG_T_DECIMAL128) {
+    uint64_t *v = static_cast<uint64_t *>(values) + 2 * o;
+    v[0] = lo;
+    v[1] = hi;
+  } else {
+    static_cast<uint64_t *>(values)[o] = lo;
+  }
+}
+
+__device__ __forceinline__ void store_int(int type, void *values, int64_t o, int64_t v) {
+  switch (type) {
+    case KG_T_INT8:
+    case KG_T_UINT8:
+    case KG_T_BOOL: static_cast<int8_t *>(values)[o] = (int8_t)v; break;
+    case KG_T_INT16:
+    case KG_T_UINT16: static_cast<int16_t *>(values)[o] = (int16_t)v; break;
+    case KG_T_INT32:
+    case KG_T_UINT32:
+    case KG_T_DATE32: static_cast<int32_t *>(values)[o] = (int32_t)v; break;
+    default: static_cast<int64_t *>(values)[o] = v;
+  }
+}
+
+// The exact 128-bit sum rounded once to double.
+__device__ __forceinline__ double i128_to_double(uint64_t lo, uint64_t hi) {
+  bool n = neg128(lo, hi);
+  if (n) negate128(lo, hi);
+  double s = (double)(((unsigned __int128)hi << 64) | lo);
+  return n ? -s : s;
+}
+
 __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, const uint64_t *rec,
                              uint64_t nslots, const uint64_t *pos, void *out_key,
                              uint8_t *out_key_valid, int key_width, unsigned int *err) {
@@ -197,25 +253,77 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
         case 4: static_cast<uint32_t *>(out_key)[o] = (uint32_t)k; break;
         default: static_cast<uint64_t *>(out_key)[o] = k;
       }
-      if (O.key_type == KG_T_FLOAT32 && !null_key) {
-        static_cast<uint32_t *>(out_key)[o] = (uint32_t)k;
-      }
       if (!null_key && out_key_valid) set_bit(out_key_valid, o);
     }
     for (int j = 0; j < O.n; j++) {
       const OutAgg &a = O.a[j];
       const uint64_t *w = r + a.word;
-      uint64_t valid_rows = a.nulls_word >= 0 ? rows - r[a.nulls_word] : rows;
+      uint64_t valid_rows = a.valid_word >= 0   ? r[a.valid_word]
+                            : a.nulls_word >= 0 ? rows - r[a.nulls_word]
+                                                : rows;
+      uint64_t divisor = a.cnt_word >= 0 ? r[a.cnt_word] : valid_rows;
+      bool overflowed = a.st_word >= 0 && r[a.st_word] == 2;
       bool ok = true;
+      if (O.partial) {
+        // the partial state columns (accumulator/output.rs:8-60)
+        switch (a.kind) {
+          case R_COUNT_STAR:
+          case R_COUNT:
+            static_cast<int64_t *>(a.values)[o] = (int64_t)(a.kind == R_COUNT_STAR ? (a.cnt_word >= 0 ? divisor : rows) : divisor);
+            break;
+          case R_SUM_INT:
+            ok = valid_rows > 0;
+            if (ok && overflowed) atomicOr(err, 1u);
+            static_cast<int64_t *>(a.values)[o] = (int64_t)w[0];
+            break;
+          case R_SUM_F:
+            ok = valid_rows > 0;
+            static_cast<double *>(a.values)[o] = __longlong_as_double((long long)w[0]);
+            break;
+          case R_SUM_DEC:
+            ok = valid_rows > 0 && !overflowed;
+            if (ok) store_dec(a.st_type, a.values, o, w[0], w[1]);
+            static_cast<uint8_t *>(a.values2)[o] = valid_rows > 0 ? 1 : 0;
+            break;
+          case R_AVG_INT:
+            static_cast<double *>(a.values)[o] = i128_to_double(w[0], w[1]);
+            static_cast<int64_t *>(a.values2)[o] = (int64_t)divisor;
+            break;
+          case R_AVG_F:
+            static_cast<double *>(a.values)[o] = __longlong_as_double((long long)w[0]);
+            static_cast<int64_t *>(a.values2)[o] = (int64_t)divisor;
+            break;
+          case R_AVG_DEC:
+            ok = !overflowed;
+            if (ok) store_dec(a.st_type, a.values, o, w[0], w[1]);
+            static_cast<int64_t *>(a.values2)[o] = (int64_t)divisor;
+            break;
+          case R_MINMAX_I:
+            ok = valid_rows > 0;
+            store_int(a.out_type, a.values, o, (int64_t)w[0]);
+            break;
+          case R_MINMAX_F: {
+            ok = valid_rows > 0;
+            double d = f64_unorder(w[0]);
+            if (a.out_type == KG_T_FLOAT32) static_cast<float *>(a.values)[o] = (float)d;
+            else static_cast<double *>(a.values)[o] = d;
+            break;
+          }
+        }
+        if (ok) set_bit(a.validity, o);
+        if (a.validity2) set_bit(a.validity2, o);
+        continue;
+      }
       switch (a.kind) {
-        case R_COUNT_STAR: static_cast<int64_t *>(a.values)[o] = (int64_t)rows; break;
-        case R_COUNT: static_cast<int64_t *>(a.values)[o] = (int64_t)valid_rows; break;
+        case R_COUNT_STAR:
+          static_cast<int64_t *>(a.values)[o] = (int64_t)(a.cnt_word >= 0 ? divisor : rows);
+          break;
+        case R_COUNT: static_cast<int64_t *>(a.values)[o] = (int64_t)divisor; break;
         case R_SUM_INT: {
           ok = valid_rows > 0;
           // exact 128-bit sum; the result wraps (non-ANSI) or must fit (ANSI)
           bool fits = (int64_t)w[1] == ((int64_t)w[0] >> 63);
-          bool ovf = (a.st_word >= 0 && r[a.st_word] == 2) || (O.ansi && !fits);
-          if (ok && ovf) atomicOr(err, 1u);
+          if (ok && (overflowed || (O.ansi && !fits))) atomicOr(err, 1u);
           static_cast<int64_t *>(a.values)[o] = (int64_t)w[0];
           break;
         }
@@ -226,42 +334,28 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
         case R_SUM_DEC: {
           ok = valid_rows > 0;
           if (!ok) break;
-          if (a.st_word >= 0 && r[a.st_word] == 2) {
+          if (overflowed) {
             ok = false;
             if (O.ansi) atomicOr(err, 1u);
             break;
           }
           uint64_t lo = w[0], hi = w[1];
-          bool wide_ok = a.in_type != KG_T_DECIMAL128 || (int64_t)w[2] == ((int64_t)w[1] >> 63);
+          bool wide_ok = !(a.in_type == KG_T_DECIMAL128 || a.st_type == KG_T_DECIMAL128) ||
+                         (int64_t)w[2] == ((int64_t)w[1] >> 63);
           uint64_t mlo = lo, mhi = hi;
           if (neg128(lo, hi)) negate128(mlo, mhi);
-          bool fits = wide_ok && lt_mag(mlo, mhi, a.pow_lo, a.pow_hi);
-          if (!fits) {
+          if (!(wide_ok && lt_mag(mlo, mhi, a.pow_lo, a.pow_hi))) {
             ok = false;
             if (O.ansi) atomicOr(err, 1u);
             break;
           }
-          store_i128(a, o, lo, hi);
+          store_dec(a.out_type, a.values, o, lo, hi);
           break;
         }
-        case R_MINMAX_I: {
+        case R_MINMAX_I:
           ok = valid_rows > 0;
-          int64_t v = (int64_t)w[0];
-          switch (a.out_type == KG_T_INT8 || a.out_type == KG_T_UINT8 || a.out_type == KG_T_BOOL
-                      ? 1
-                      : a.out_type == KG_T_INT16 || a.out_type == KG_T_UINT16
-                            ? 2
-                            : a.out_type == KG_T_INT32 || a.out_type == KG_T_UINT32 ||
-                                      a.out_type == KG_T_DATE32
-                                  ? 4
-                                  : 8) {
-            case 1: static_cast<int8_t *>(a.values)[o] = (int8_t)v; break;
-            case 2: static_cast<int16_t *>(a.values)[o] = (int16_t)v; break;
-            case 4: static_cast<int32_t *>(a.values)[o] = (int32_t)v; break;
-            default: static_cast<int64_t *>(a.values)[o] = v;
-          }
+          store_int(a.out_type, a.values, o, (int64_t)w[0]);
           break;
-        }
         case R_MINMAX_F: {
           ok = valid_rows > 0;
           double d = f64_unorder(w[0]);
@@ -269,34 +363,26 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
           else static_cast<double *>(a.values)[o] = d;
           break;
         }
-        case R_AVG_INT: {
-          ok = valid_rows > 0;
-          if (!ok) break;
-          // the exact sum, rounded once to double
-          uint64_t lo = w[0], hi = w[1];
-          bool n = neg128(lo, hi);
-          if (n) negate128(lo, hi);
-          // one correctly rounded conversion of the 128-bit magnitude
-          double s = (double)(((unsigned __int128)hi << 64) | lo);
-          if (n) s = -s;
-          static_cast<double *>(a.values)[o] = s / (double)valid_rows;
+        case R_AVG_INT:
+          ok = divisor > 0;
+          if (ok) static_cast<double *>(a.values)[o] = i128_to_double(w[0], w[1]) / (double)divisor;
           break;
-        }
         case R_AVG_F:
-          ok = valid_rows > 0;
-          static_cast<double *>(a.values)[o] =
-              __longlong_as_double((long long)w[0]) / (double)valid_rows;
+          ok = divisor > 0;
+          if (ok)
+            static_cast<double *>(a.values)[o] =
+                __longlong_as_double((long long)w[0]) / (double)divisor;
           break;
         case R_AVG_DEC: {
-          ok = valid_rows > 0;
+          ok = divisor > 0;
           if (!ok) break;
-          if (a.st_word >= 0 && r[a.st_word] == 2) {
+          if (overflowed) {
             ok = false;
             if (O.ansi) atomicOr(err, 1u);
             break;
           }
           // avg_fast (accumulator/helpers.rs:595-609): |sum| * 10^(s - ss)
-          // / count, HALF_UP; the digit loop's value where that overflows.
+          // / count, HALF_UP; the digit-loop value where that overflows.
           uint64_t lo = w[0], hi = w[1];
           bool negative = neg128(lo, hi);
           if (negative) negate128(lo, hi);
@@ -305,38 +391,34 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
           for (int t = 0; t < k && !ovf; t++) ovf = mul128(lo, hi, 10);
           uint64_t qlo, qhi, rem;
           if (!ovf) {
-            divmod128(lo, hi, valid_rows, qlo, qhi, rem);
-            if (rem >= valid_rows - rem) {
+            divmod128(lo, hi, divisor, qlo, qhi, rem);
+            if (rem >= divisor - rem) {
               qlo += 1;
               if (qlo == 0) qhi += 1;
             }
           } else {
-            // |sum| >= 2^128 / 10^k: the scale-39 digit loop rounds the same
-            // quotient; compute it as (|sum| / count) * 10^k + digits.
             uint64_t slo = w[0], shi = w[1];
             if (negative) negate128(slo, shi);
-            divmod128(slo, shi, valid_rows, qlo, qhi, rem);
+            divmod128(slo, shi, divisor, qlo, qhi, rem);
+            ovf = false;
             for (int t = 0; t < k; t++) {
               if (mul128(qlo, qhi, 10)) {
                 ovf = true;
                 break;
               }
               unsigned __int128 r10 = (unsigned __int128)rem * 10;
-              uint64_t dgt = (uint64_t)(r10 / valid_rows);
-              rem = (uint64_t)(r10 % valid_rows);
+              uint64_t dgt = (uint64_t)(r10 / divisor);
+              rem = (uint64_t)(r10 % divisor);
               uint64_t old = qlo;
               qlo += dgt;
               if (qlo < old) qhi += 1;
             }
             if (!ovf) {
               unsigned __int128 r10 = (unsigned __int128)rem * 10;
-              if ((uint64_t)(r10 / valid_rows) >= 5) {
+              if ((uint64_t)(r10 / divisor) >= 5) {
                 qlo += 1;
                 if (qlo == 0) qhi += 1;
               }
-              ovf = false;
-            } else {
-              ovf = true;
             }
           }
           if (ovf || !lt_mag(qlo, qhi, a.pow_lo, a.pow_hi)) {
@@ -345,7 +427,7 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
             break;
           }
           if (negative) negate128(qlo, qhi);
-          store_i128(a, o, qlo, qhi);
+          store_dec(a.out_type, a.values, o, qlo, qhi);
           break;
         }
       }
@@ -366,6 +448,17 @@ __global__ void k_fill_words(uint64_t *rec, uint64_t nrec, int W, const uint64_t
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
        t += (uint64_t)gridDim.x * blockDim.x)
     rec[t] = init_dev[t % W];
+}
+
+// One byte per row (0/1) to an LSB bitmap: each thread owns a 32-bit word.
+__global__ void k_bytes_to_bits(const uint8_t *in, int64_t n, uint32_t *out) {
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w * 32 < n;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t bits = 0;
+    for (int k = 0; k < 32 && w * 32 + k < n; k++)
+      if (in[w * 32 + k]) bits |= 1u << k;
+    out[w] = bits;
+  }
 }
 
 __global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
@@ -410,9 +503,10 @@ using namespace kg;
 struct kg_hash_agg {
   kg_ctx *ctx = nullptr;
   hipStream_t s = nullptr;
-  bool has_key = false, direct = false, ansi = false;
+  bool has_key = false, direct = false, ansi = false, merge = false, partial = false;
   kg_column key_type{};
   std::vector<kg_column> in_types;
+  int nsink = 0;  // input columns per sink call (state columns when merging)
   AggProg prog{};
   OutProg out{};
   int slots = 512;  // LDS records per workgroup
@@ -582,9 +676,10 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
     if (key->type != a->key_type.type) fail(KG_ERR_INTERNAL, "grouping column type changed");
     P.key = st.add(*key, nrows);
   }
-  for (int c = 0; c < (int)a->in_types.size(); c++) {
+  for (int c = 0; c < a->nsink; c++) {
     check_column(inputs[c], nrows, "aggregate input");
-    if (inputs[c].type != a->in_types[c].type) fail(KG_ERR_INTERNAL, "input column type changed");
+    if (!a->merge && inputs[c].type != a->in_types[c].type)
+      fail(KG_ERR_INTERNAL, "input column type changed");
     P.in[c] = st.add(inputs[c], nrows);
   }
   bool replay = false;
@@ -630,12 +725,13 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
   if (st.staged_host()) KG_HIP(hipStreamSynchronize(a->s));
 }
 
-void compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg_agg_spec *aggs,
-                  bool ansi, AggProg &P, OutProg &O, std::vector<CheckedSum> &checked) {
-  int W = 1;  // word 0: rows
+int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg_agg_spec *aggs,
+                 bool ansi, bool merge, bool partial, AggProg &P, OutProg &O,
+                 std::vector<CheckedSum> &checked) {
+  int W = 1;  // word 0: rows (raw) or partial states (merge)
   std::vector<AggOp> ops;
   ops.push_back(AggOp{OP_COUNT_STAR, 0, 0, 0, 0, 0});
-  std::vector<int> nulls_word(in_types.size(), -1);
+  std::vector<int> nulls_word(kAggInputs, -1);
   std::vector<uint64_t> init(1, 0);
   auto words = [&](int n, uint64_t v) {
     int w = W;
@@ -647,20 +743,33 @@ void compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const k
     int kind, col, checked, avg;
   };
   std::vector<std::pair<Key, int>> shared;
-  auto find_or_add = [&](int kind, int col, bool checked, bool avg, int nwords, uint64_t iv,
-                         int &st_word) {
-    for (auto &kv : shared)
-      if (kv.first.kind == kind && kv.first.col == col && kv.first.checked == checked &&
-          kv.first.avg == avg && !checked)
-        return kv.second;
+  auto op_words = [&](int kind, int col, bool chk, bool avg, int nwords, uint64_t iv,
+                      int &st_word) {
+    if (!chk)
+      for (auto &kv : shared)
+        if (kv.first.kind == kind && kv.first.col == col && !kv.first.checked && kv.first.avg == avg)
+          return kv.second;
     int w = words(nwords, iv);
-    st_word = checked ? words(1, 0) : 0;
-    ops.push_back(AggOp{(uint8_t)kind, (uint8_t)col, (uint8_t)checked, 0, (uint16_t)w,
+    st_word = chk ? words(1, 0) : 0;
+    ops.push_back(AggOp{(uint8_t)kind, (uint8_t)col, (uint8_t)chk, 0, (uint16_t)w,
                         (uint16_t)st_word});
-    shared.push_back({Key{kind, col, checked, avg}, w});
+    shared.push_back({Key{kind, col, chk, avg}, w});
     return w;
   };
+  auto nulls_of = [&](int col) {
+    if (col >= kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 input columns per sink");
+    if (nulls_word[col] < 0) {
+      nulls_word[col] = words(1, 0);
+      ops.push_back(AggOp{OP_NULLS, (uint8_t)col, 0, 0, (uint16_t)nulls_word[col], 0});
+    }
+    return nulls_word[col];
+  };
+  auto plain = [&](int kind, int col, int word) {
+    ops.push_back(AggOp{(uint8_t)kind, (uint8_t)col, 0, 0, (uint16_t)word, 0});
+  };
+  int next_state = 0;  // merge: the next state column
   O.n = naggs;
+  O.partial = partial;
   for (int i = 0; i < naggs; i++) {
     OutAgg &o = O.a[i];
     o = OutAgg{};
@@ -668,57 +777,70 @@ void compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const k
     o.col = aggs[i].input;
     o.st_word = -1;
     o.nulls_word = -1;
-    if (aggs[i].func == KG_AGG_COUNT_STAR) {
-      o.kind = R_COUNT_STAR;
-      o.out_type = KG_T_INT64;
+    o.valid_word = -1;
+    o.cnt_word = -1;
+    int f = aggs[i].func;
+    if (f == KG_AGG_COUNT_STAR || f == KG_AGG_COUNT) {
+      o.kind = f == KG_AGG_COUNT_STAR ? R_COUNT_STAR : R_COUNT;
+      o.out_type = o.st_type = KG_T_INT64;
+      if (merge) {
+        // state `#count` (aggregate/mod.rs:755): add the counts
+        int c = next_state++;
+        o.word = words(1, 0);
+        plain(OP_ADDV, c, o.word);
+        o.cnt_word = o.word;
+      } else if (f == KG_AGG_COUNT) {
+        int c = aggs[i].input;
+        if (c < 0 || c >= (int)in_types.size()) fail(KG_ERR_INTERNAL, "aggregate input out of range");
+        o.nulls_word = nulls_of(c);
+      }
       continue;
     }
-    int c = aggs[i].input;
-    if (c < 0 || c >= (int)in_types.size()) fail(KG_ERR_INTERNAL, "aggregate input out of range");
-    const kg_column &t = in_types[c];
-    if (nulls_word[c] < 0) {
-      nulls_word[c] = words(1, 0);
-      ops.push_back(AggOp{OP_NULLS, (uint8_t)c, 0, 0, (uint16_t)nulls_word[c], 0});
-    }
-    o.nulls_word = nulls_word[c];
+    int c0 = aggs[i].input;
+    if (c0 < 0 || c0 >= (int)in_types.size()) fail(KG_ERR_INTERNAL, "aggregate input out of range");
+    const kg_column &t = in_types[c0];
+    int c = merge ? next_state : c0;  // the (first) column this aggregate reads
     o.in_type = t.type;
     o.in_s = t.scale;
     int st = 0;
-    switch (aggs[i].func) {
-      case KG_AGG_COUNT:
-        o.kind = R_COUNT;
-        o.out_type = KG_T_INT64;
-        break;
+    switch (f) {
       case KG_AGG_SUM:
       case KG_AGG_AVG: {
-        bool avg = aggs[i].func == KG_AGG_AVG;
-        if (is_float(t.type)) {
+        bool avg = f == KG_AGG_AVG;
+        if (is_float(t.type) || (avg && is_integral(t.type) && merge)) {
+          // float sums; a merged integer avg state sums in double (avg_sum_type)
           o.kind = avg ? R_AVG_F : R_SUM_F;
           o.out_type = KG_T_FLOAT64;
-          o.word = find_or_add(OP_ADDF64, c, false, false, 1, 0, st);
+          o.st_type = KG_T_FLOAT64;
+          o.word = op_words(OP_ADDF64, c, false, false, 1, 0, st);
+          o.nulls_word = nulls_of(c);
         } else if (is_integral(t.type)) {
-          // avg of integers sums in double in the reference (avg_sum_type);
-          // the exact 128-bit sum rounded once gives that double whenever
-          // the row-order double sum is exact (|sum| < 2^53)
           bool chk = !avg && ansi;
           o.kind = avg ? R_AVG_INT : R_SUM_INT;
           o.out_type = avg ? KG_T_FLOAT64 : KG_T_INT64;
-          o.word = find_or_add(OP_ADD128, c, chk, false, 2, 0, st);
+          o.st_type = avg ? KG_T_FLOAT64 : KG_T_INT64;
+          o.word = op_words(OP_ADD128, c, chk, false, 2, 0, st);
+          o.nulls_word = nulls_of(c);
           if (chk) {
             o.st_word = st;
-            int tw = type_width(t.type);
-            checked.push_back({c, o.word, st, false, false,
-                                  (unsigned __int128)1 << 63,
-                                  (unsigned __int128)1 << (8 * tw - 1)});
+            int tw = merge ? 8 : type_width(t.type);
+            checked.push_back({c, o.word, st, false, false, (unsigned __int128)1 << 63,
+                               (unsigned __int128)1 << (8 * tw - 1)});
           }
         } else if (is_decimal(t.type)) {
           if (t.scale < 0) fail(KG_ERR_UNSUPPORTED, "negative decimal scale");
           int sum_p = t.precision + 10 > 38 ? 38 : t.precision + 10;
-          bool wide = t.type == KG_T_DECIMAL128;
-          o.word = find_or_add(wide ? OP_ADD192 : OP_ADD128, c, true, avg, wide ? 3 : 2, 0, st);
+          // the sum state's carrier (decimal_on, aggregate/mod.rs:1004-1011)
+          int sum_type = (t.type == KG_T_DECIMAL64 && sum_p <= 18) ? KG_T_DECIMAL64 : KG_T_DECIMAL128;
+          bool wide = merge ? sum_type == KG_T_DECIMAL128 : t.type == KG_T_DECIMAL128;
+          o.word = op_words(wide ? OP_ADD192 : OP_ADD128, c, true, avg, wide ? 3 : 2, 0, st);
           o.st_word = st;
+          o.nulls_word = nulls_of(c);
           checked.push_back({c, o.word, st, avg, wide, pow10_u128(sum_p),
-                                pow10_u128(t.precision)});
+                             pow10_u128(merge ? sum_p : t.precision)});
+          o.st_type = sum_type;
+          o.st_p = sum_p;
+          o.st_s = t.scale;
           if (avg) {
             o.kind = R_AVG_DEC;
             o.out_p = t.precision + 4 > 38 ? 38 : t.precision + 4;
@@ -735,24 +857,47 @@ void compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const k
         } else {
           fail(KG_ERR_UNSUPPORTED, "sum/avg of this type");
         }
+        if (merge) {
+          next_state++;
+          bool two = avg || is_decimal(t.type);
+          if (two) {
+            int c2 = next_state++;
+            if (c2 >= kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 state columns per sink");
+            if (avg) {
+              // state (#sum, #count): the rows averaged are the summed counts
+              o.cnt_word = words(1, 0);
+              plain(OP_ADDV, c2, o.cnt_word);
+              o.valid_word = o.cnt_word;
+            } else {
+              // state (#sum, #nonempty): NULL only when no partial was nonempty
+              o.valid_word = words(1, 0);
+              plain(OP_COUNT_TRUE, c2, o.valid_word);
+            }
+            if (is_decimal(t.type))
+              ops.push_back(AggOp{OP_STICKY, (uint8_t)c, 0, (uint8_t)c2, 0, (uint16_t)st});
+          }
+        }
+        if (partial) o.st2 = (avg ? 2 : (is_decimal(t.type) ? 1 : 0));
         break;
       }
       case KG_AGG_MIN:
       case KG_AGG_MAX: {
-        bool mx = aggs[i].func == KG_AGG_MAX;
-        o.out_type = t.type;
-        o.out_p = t.precision;
-        o.out_s = t.scale;
+        bool mx = f == KG_AGG_MAX;
+        o.out_type = o.st_type = t.type;
+        o.out_p = o.st_p = t.precision;
+        o.out_s = o.st_s = t.scale;
+        o.nulls_word = nulls_of(c);
         if (is_float(t.type)) {
           o.kind = R_MINMAX_F;
-          o.word = find_or_add(mx ? OP_MAX_F : OP_MIN_F, c, false, false, 1, mx ? 0 : ~0ull, st);
+          o.word = op_words(mx ? OP_MAX_F : OP_MIN_F, c, false, false, 1, mx ? 0 : ~0ull, st);
         } else if (is_int_like(t.type) && t.type != KG_T_UINT64) {
           o.kind = R_MINMAX_I;
-          o.word = find_or_add(mx ? OP_MAX_I64 : OP_MIN_I64, c, false, false, 1,
-                               mx ? 0x8000000000000000ULL : 0x7FFFFFFFFFFFFFFFULL, st);
+          o.word = op_words(mx ? OP_MAX_I64 : OP_MIN_I64, c, false, false, 1,
+                            mx ? 0x8000000000000000ULL : 0x7FFFFFFFFFFFFFFFULL, st);
         } else {
           fail(KG_ERR_UNSUPPORTED, "min/max of this type on the device");
         }
+        if (merge) next_state++;
         break;
       }
       default: fail(KG_ERR_UNSUPPORTED, "aggregate function");
@@ -768,6 +913,7 @@ void compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const k
   for (int k = 0; k < W; k++) P.init[k] = init[k];
   O.W = W;
   O.ansi = ansi;
+  return merge ? next_state : (int)in_types.size();
 }
 
 void init_records(kg_ctx *ctx, uint64_t *rec, uint64_t nrec, const AggProg &P, hipStream_t s) {
@@ -829,7 +975,10 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       a->direct = type_width(t) <= 1;  // one-byte keys and booleans
     }
     for (int c = 0; c < ninputs; c++) a->in_types.push_back(input_types[c]);
-    compile_aggs(a->in_types, naggs, aggs, a->ansi, a->prog, a->out, a->checked);
+    a->merge = (flags & KG_AGG_MERGE) != 0;
+    a->partial = (flags & KG_AGG_PARTIAL) != 0;
+    a->nsink = compile_aggs(a->in_types, naggs, aggs, a->ansi, a->merge, a->partial, a->prog,
+                            a->out, a->checked);
     a->prog.has_key = a->has_key;
     a->prog.direct = a->direct;
     a->prog.key.type = a->has_key ? key_type->type : 0;
@@ -904,7 +1053,8 @@ int kg_agg_result(kg_hash_agg *a, kg_batch **out) {
     uint64_t vb = (uint64_t)ceil_div(n, 32) * 4 + 4;
     OutProg O = a->out;
     O.cap = a->cap;
-    int ncols = a->out.n + (a->has_key ? 1 : 0);
+    int ncols = (a->has_key ? 1 : 0);
+    for (int j = 0; j < a->out.n; j++) ncols += (a->partial && a->out.a[j].st2) ? 2 : 1;
     b->cols.resize(ncols);
     int kw = 8;
     void *key_vals = nullptr;
@@ -928,18 +1078,33 @@ int kg_agg_result(kg_hash_agg *a, kg_batch **out) {
       key_valid = kc.validity.get<uint8_t>();
       base = 1;
     }
+    int at = base;
+    std::vector<int> bool_cols;  // partial `nonempty` columns, written a byte per group
     for (int j = 0; j < a->out.n; j++) {
-      auto &c = b->cols[base + j];
       OutAgg &o = O.a[j];
-      c.type = o.out_type;
-      c.precision = o.out_p;
-      c.scale = o.out_s;
-      int w = type_width(o.out_type);
+      auto &c = b->cols[at++];
+      c.type = a->partial ? o.st_type : o.out_type;
+      c.precision = a->partial ? o.st_p : o.out_p;
+      c.scale = a->partial ? o.st_s : o.out_s;
+      int w = type_width(c.type);
       c.values.reset(a->ctx, (uint64_t)n * (w > 0 ? w : 1) + 16);
       c.validity.reset(a->ctx, vb);
       KG_HIP(hipMemsetAsync(c.validity.get(), 0, vb, a->s));
       o.values = c.values.get();
       o.validity = c.validity.get<uint8_t>();
+      o.values2 = nullptr;
+      o.validity2 = nullptr;
+      if (a->partial && o.st2) {
+        auto &c2 = b->cols[at];
+        c2.type = o.st2 == 1 ? KG_T_BOOL : KG_T_INT64;
+        c2.values.reset(a->ctx, (uint64_t)n * 8 + 16);
+        c2.validity.reset(a->ctx, vb);
+        KG_HIP(hipMemsetAsync(c2.validity.get(), 0, vb, a->s));
+        o.values2 = c2.values.get();
+        o.validity2 = c2.validity.get<uint8_t>();
+        if (o.st2 == 1) bool_cols.push_back(at);
+        at++;
+      }
     }
     DevBuf err(a->ctx, 4);
     KG_HIP(hipMemsetAsync(err.get(), 0, 4, a->s));
@@ -952,14 +1117,17 @@ int kg_agg_result(kg_hash_agg *a, kg_batch **out) {
     KG_HIP(hipStreamSynchronize(a->s));
     if (e & 1u) fail(KG_ERR_OVERFLOW, "Overflow in sum (use try_sum/try_avg to tolerate overflow)");
     if (e & 2u) fail(KG_ERR_OUT_OF_RANGE, "average out of the result decimal's range");
-    if (a->has_key && a->key_type.type == KG_T_BOOL) {
-      // repack the byte-per-group booleans as bits
-      std::vector<uint8_t> bytes((size_t)n), bits((size_t)ceil_div(n, 32) * 4 + 4, 0);
-      if (n) KG_HIP(hipMemcpy(bytes.data(), b->cols[0].values.get(), (size_t)n, hipMemcpyDeviceToHost));
-      for (int64_t r = 0; r < n; r++)
-        if (bytes[r]) bits[r >> 3] |= (uint8_t)(1u << (r & 7));
-      KG_HIP(hipMemcpy(b->cols[0].values.get(), bits.data(), bits.size() < b->cols[0].values.bytes() ? bits.size() : b->cols[0].values.bytes(), hipMemcpyHostToDevice));
+    // booleans (a boolean key, partial `nonempty` states) leave the kernel as
+    // one byte per group; Arrow packs them as bits
+    if (a->has_key && a->key_type.type == KG_T_BOOL) bool_cols.push_back(0);
+    for (int c : bool_cols) {
+      DevBuf bits(a->ctx, vb);
+      k_bytes_to_bits<<<stream_grid(ceil_div(n, 32), 256, 1), 256, 0, a->s>>>(
+          b->cols[c].values.get<uint8_t>(), n, bits.get<uint32_t>());
+      KG_HIP(hipGetLastError());
+      b->cols[c].values = std::move(bits);
     }
+    KG_HIP(hipStreamSynchronize(a->s));
     *out = b.release();
   });
 }

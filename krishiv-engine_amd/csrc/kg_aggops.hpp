@@ -30,13 +30,17 @@ enum OpKind : uint8_t {
   OP_MAX_I64,
   OP_MIN_F,  // order-preserving u64 of a double
   OP_MAX_F,
+  // merge of partial states (the final phase of a two-phase aggregate)
+  OP_ADDV,        // word += int64 value (counts)
+  OP_COUNT_TRUE,  // word += value != 0 (a `nonempty` flag)
+  OP_STICKY,      // st = 2 when the sum state is NULL and col2 is nonzero
 };
 
 struct AggOp {
   uint8_t kind;
   uint8_t col;
   uint8_t checked;  // sum whose running value is bounded (ANSI int, decimals)
-  uint8_t pad;
+  uint8_t col2;     // OP_STICKY: the `nonempty` / `count` state column
   uint16_t word;
   uint16_t st_word;  // overflow-state word of a checked sum
 };
@@ -179,9 +183,18 @@ __device__ __forceinline__ void apply_row(const AggProg &P, uint64_t *r, int64_t
       if (!ok) add_u64_nr<LDS>(r + op.word, 1);
       continue;
     }
+    if (op.kind == OP_STICKY) {
+      const DCol &c2 = P.in[op.col2];
+      if (!ok && dev::valid(c2, i) && dev::load_i64(c2, i) != 0) max_u64<LDS>(r + op.st_word, 2);
+      continue;
+    }
     if (!ok || (op.checked && P.skip_checked)) continue;
     uint64_t *w = r + op.word;
     switch (op.kind) {
+      case OP_ADDV: add_u64_nr<LDS>(w, lo); break;
+      case OP_COUNT_TRUE:
+        if (lo) add_u64_nr<LDS>(w, 1);
+        break;
       case OP_ADD128: add128<LDS>(w, lo, hi); break;
       case OP_ADD192: add192<LDS>(w, lo, hi, top); break;
       case OP_ADDF64: add_f64<LDS>(w, f); break;
@@ -189,6 +202,165 @@ __device__ __forceinline__ void apply_row(const AggProg &P, uint64_t *r, int64_t
       case OP_MAX_I64: max_i64<LDS>(w, (int64_t)lo); break;
       case OP_MIN_F: min_u64<LDS>(w, f64_order(f)); break;
       case OP_MAX_F: max_u64<LDS>(w, f64_order(f)); break;
+      default: break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- wave reduction
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    int64_t o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    int64_t o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// The rows of the lanes in `peers` (all of one group) reduced across the
+// wave, then applied once to record `r` by the lowest peer lane.  Every lane
+// of the wave must call it (lanes outside `peers` contribute neutral values).
+// Integer sums are reduced as four 32-bit limbs (exact), then added with the
+// carry-taking atomics; float sums add in a tree order.
+template <bool LDS>
+__device__ __forceinline__ void apply_wave(const AggProg &P, uint64_t *r, int64_t i,
+                                           uint64_t peers) {
+  const int lane = __lane_id();
+  const bool me = (peers >> lane) & 1;
+  const bool leader = me && (peers & ((1ull << lane) - 1)) == 0;
+  int last_col = -1;
+  bool ok = false;
+  uint64_t lo = 0, hi = 0, top = 0;
+  double f = 0.0;
+  for (int o = 0; o < P.nops; o++) {
+    const AggOp op = P.ops[o];
+    if (op.kind == OP_COUNT_STAR) {
+      if (leader) add_u64_nr<LDS>(r, (uint64_t)__popcll(peers));
+      continue;
+    }
+    if (op.col != last_col) {
+      last_col = op.col;
+      const DCol &c = P.in[op.col];
+      ok = me && dev::valid(c, i);
+      lo = hi = top = 0;
+      f = 0.0;
+      if (me) {
+        if (dev::is_f(c.type)) {
+          f = dev::load_f64(c, i);
+        } else {
+          int64_t h;
+          dev::load_i128(c, i, lo, h);
+          hi = (uint64_t)h;
+          top = (uint64_t)(h >> 63);
+        }
+      }
+    }
+    uint64_t *w = r + op.word;
+    switch (op.kind) {
+      case OP_NULLS: {
+        uint64_t n = (uint64_t)__popcll(__ballot(me && !ok));
+        if (leader && n) add_u64_nr<LDS>(w, n);
+        break;
+      }
+      case OP_STICKY: {
+        const DCol &c2 = P.in[op.col2];
+        bool hit = me && !ok && dev::valid(c2, i) && dev::load_i64(c2, i) != 0;
+        uint64_t any = __ballot(hit);  // every lane takes part in the ballot
+        if (leader && any) max_u64<LDS>(r + op.st_word, 2);
+        break;
+      }
+      case OP_COUNT_TRUE: {
+        uint64_t n = (uint64_t)__popcll(__ballot(ok && lo != 0));
+        if (leader && n) add_u64_nr<LDS>(w, n);
+        break;
+      }
+      case OP_ADDV: {
+        uint64_t s = wave_sum_u64(ok ? lo : 0);
+        if (leader && s) add_u64_nr<LDS>(w, s);
+        break;
+      }
+      case OP_ADD128:
+      case OP_ADD192: {
+        if (op.checked && P.skip_checked) break;
+        bool v = ok;
+        // limbs of the (sign-extended 192-bit) value
+        uint64_t l0 = v ? (lo & 0xFFFFFFFFull) : 0, l1 = v ? (lo >> 32) : 0;
+        uint64_t l2 = v ? (hi & 0xFFFFFFFFull) : 0;
+        int64_t l3 = v ? (int64_t)hi >> 32 : 0;
+        uint64_t s0 = wave_sum_u64(l0), s1 = wave_sum_u64(l1), s2 = wave_sum_u64(l2);
+        int64_t s3 = wave_sum_i64(l3);
+        if (leader) {
+          // S = s0 + s1*2^32 + s2*2^64 + s3*2^96 (s3 signed) as 192 bits:
+          // w0 the low word, U = the rest, sign-extended
+          unsigned __int128 a = (unsigned __int128)s0 + ((unsigned __int128)s1 << 32);
+          uint64_t w0 = (uint64_t)a;
+          __int128 U = (__int128)(a >> 64) + (__int128)s2 + ((__int128)s3 << 32);
+          uint64_t w1 = (uint64_t)U;
+          uint64_t w2 = (uint64_t)(U >> 64);
+          if (op.kind == OP_ADD128) add128<LDS>(w, w0, w1);
+          else add192<LDS>(w, w0, w1, w2);
+        }
+        break;
+      }
+      case OP_ADDF64: {
+        double s = wave_sum_f64(ok ? f : 0.0);
+        uint64_t any = __ballot(ok);
+        if (leader && any) add_f64<LDS>(w, s);
+        break;
+      }
+      case OP_MIN_I64: {
+        // the neutral values equal the initial state: applying them is harmless
+        int64_t m = wave_min_i64(ok ? (int64_t)lo : INT64_MAX);
+        if (leader) min_i64<LDS>(w, m);
+        break;
+      }
+      case OP_MAX_I64: {
+        int64_t m = wave_max_i64(ok ? (int64_t)lo : INT64_MIN);
+        if (leader) max_i64<LDS>(w, m);
+        break;
+      }
+      case OP_MIN_F: {
+        uint64_t m = wave_min_u64(ok ? f64_order(f) : ~0ull);
+        if (leader) min_u64<LDS>(w, m);
+        break;
+      }
+      case OP_MAX_F: {
+        uint64_t m = wave_max_u64(ok ? f64_order(f) : 0ull);
+        if (leader) max_u64<LDS>(w, m);
+        break;
+      }
       default: break;
     }
   }
@@ -254,7 +426,12 @@ __device__ __forceinline__ void merge_record(const AggProg &P, uint64_t *g, cons
     switch (op.kind) {
       case OP_COUNT_STAR:
       case OP_NULLS:
+      case OP_ADDV:
+      case OP_COUNT_TRUE:
         if (s[0]) add_u64_nr<false>(d, s[0]);
+        break;
+      case OP_STICKY:
+        if (l[op.st_word] == 2) max_u64<false>(g + op.st_word, 2);
         break;
       case OP_ADD128:
         if (!(op.checked && P.skip_checked)) add128<false>(d, s[0], s[1]);
@@ -372,10 +549,17 @@ struct OutAgg {
   int word;     // value word
   int st_word;  // checked-sum state word, -1 if none
   int nulls_word;
+  int valid_word;  // >= 0: the count that decides validity (merged states)
+  int cnt_word;    // >= 0: the row count an average divides by (merged states)
   int out_type, out_p, out_s, in_s, in_type;
   void *values;
-  uint8_t *validity;  // byte per row, compacted to bits on the host side of the batch
+  uint8_t *validity;  // LSB bitmap (32-bit words, atomically set)
   uint64_t pow_lo, pow_hi;  // 10^out_p (decimal result bound)
+  // partial output: the state columns (state_fields, aggregate/mod.rs:747-775)
+  int st_type, st_p, st_s;       // type of the first state column
+  int st2;                       // second state column: 0 none, 1 nonempty (bool), 2 count
+  void *values2;
+  uint8_t *validity2;
 };
 
 enum OutKind {
@@ -396,6 +580,7 @@ struct OutProg {
   int n;
   int W;
   int ansi;
+  int partial;  // emit partial states instead of final values
   int key_type;
   uint64_t cap;
   int direct;
@@ -456,8 +641,13 @@ struct CheckedSum {
 
 // Builds the update program (prog) and the final-value program (out) of an
 // aggregate list over input columns of the given types.
-void compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg_agg_spec *aggs,
-                  bool ansi, AggProg &prog, OutProg &out, std::vector<CheckedSum> &checked);
+// merge = the inputs are the partial-state columns of a first phase (in
+// state_fields order), not raw rows; in_types are the aggregates' original
+// argument types either way.  Returns the number of input columns a sink
+// call takes.
+int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg_agg_spec *aggs,
+                 bool ansi, bool merge, bool partial, AggProg &prog, OutProg &out,
+                 std::vector<CheckedSum> &checked);
 
 // Fills nrec records with the program's initial state words.
 void init_records(kg_ctx *ctx, uint64_t *rec, uint64_t nrec, const AggProg &P, hipStream_t s);

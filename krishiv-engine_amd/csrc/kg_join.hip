@@ -606,7 +606,8 @@ int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_t
     if (ninputs > kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 aggregate inputs");
     j->agg_types.assign(input_types, input_types + ninputs);
     std::vector<CheckedSum> checked;
-    compile_aggs(j->agg_types, naggs, aggs, (flags & KG_AGG_ANSI) != 0, j->prog, j->out, checked);
+    compile_aggs(j->agg_types, naggs, aggs, (flags & KG_AGG_ANSI) != 0, false, false, j->prog,
+                 j->out, checked);
     if (!checked.empty())
       fail(KG_ERR_UNSUPPORTED, "checked sums (decimal, ANSI) in the fused join aggregate");
     for (int o = 0; o < j->prog.nops; o++)

@@ -305,15 +305,18 @@ class HashAggregate:
     """GROUP BY one key column with count/sum/min/max/avg aggregates."""
 
     def __init__(self, ctx: Context, key_type, input_types, aggs, ansi=False,
-                 expected_groups=1 << 16):
+                 expected_groups=1 << 16, partial=False, merge=False):
+        """partial: emit partial states (PartialAggregate); merge: sink the
+        partial states of a first phase (the final phase)."""
         self.ctx = ctx
         kt = _type_col(*key_type) if key_type is not None else None
         its = (Column * max(1, len(input_types)))(*[_type_col(*t) for t in input_types])
         sp = (AggSpec * max(1, len(aggs)))(*[AggSpec(f, i) for f, i in aggs])
         h = C.c_void_p()
+        flags = (AGG_ANSI if ansi else 0) | (2 if partial else 0) | (4 if merge else 0)
         check(lib().kg_agg_create(ctx.h, C.byref(kt) if kt is not None else None,
                                   len(input_types), its, len(aggs), sp,
-                                  AGG_ANSI if ansi else 0, expected_groups, C.byref(h)))
+                                  flags, expected_groups, C.byref(h)))
         self.h = h
         self.naggs = len(aggs)
         self.has_key = key_type is not None

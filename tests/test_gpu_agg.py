@@ -283,3 +283,83 @@ def test_tpch_q1_matches_spark():
     with open(os.path.join(GOLD, "tpch_q1_expected.csv")) as fh:
         want = sorted(l.strip() for l in fh if l.strip())
     assert sorted(rows) == want
+
+
+def _two_phase(keys_parts, vals_parts, types, aggs, ansi):
+    """Partial aggregate per partition (device), then the final aggregate
+    merging the partial states in partition order."""
+    final = kgpu.HashAggregate(ctx(), (A.T_INT64, 0, 0), types, aggs, ansi=ansi, merge=True)
+    keep = []
+    for keys, vals in zip(keys_parts, vals_parts):
+        p = kgpu.HashAggregate(ctx(), (A.T_INT64, 0, 0), types, aggs, ansi=ansi, partial=True)
+        p.sink(Col(A.T_INT64, keys), vals)
+        p.finalize()
+        st = p.result()
+        keep += [p, st]
+        final.sink(st.as_col(0), [st.as_col(i) for i in range(1, st.num_columns)], st.num_rows)
+    final.finalize()
+    return final.result()
+
+
+def test_spark_decimal_goldens_two_phase():
+    """Every Spark decimal golden through PartialAggregate + the final merge
+    (contiguous partitions merged in order, as Spark's range partitioning)."""
+    with open(os.path.join(GOLD, "spark_decimal_sum_avg.json")) as f:
+        cases = json.load(f)
+    say(f"agg: {len(cases)} Spark decimal goldens, two-phase")
+    for c in cases:
+        p, s = c["type"]
+        func = A.AGG_SUM if c["fn"] == "sum" else A.AGG_AVG
+        vals = [int(x) for x in c["values"]]
+        groups, parts = c["groups"], c["partitions"]
+        n = len(vals)
+        kp, vp = [], []
+        for k in range(parts):
+            lo, hi = k * n // parts, (k + 1) * n // parts
+            kp.append(np.array([i % groups for i in range(lo, hi)], dtype=np.int64))
+            vp.append([Col(A.T_DECIMAL128, vals[lo:hi], precision=p, scale=s)])
+        try:
+            r = _two_phase(kp, vp, [(A.T_DECIMAL128, p, s)], [(func, 0)], c["ansi"])
+            err = None
+        except kgpu.KgError as e:
+            err = str(e)
+        if "error" in c:
+            assert err is not None and c["error"] in err, (c, err)
+            continue
+        assert err is None, (c, err)
+        t, op, os_ = r.column_type(1)
+        assert c["type_out"] == f"decimal({op},{os_})"
+        gk, _ = r.column(0)
+        gv, gok = r.column(1)
+        got = {int(k): (v if ok else None) for k, v, ok in zip(gk, gv, gok)}
+        for gi, want in enumerate(c["value"]):
+            v = got[gi]
+            if want is None:
+                assert v is None, (c, gi, v)
+            else:
+                neg = v < 0
+                d = str(abs(v)).rjust(os_ + 1, "0")
+                assert ("-" if neg else "") + (d[:-os_] + "." + d[-os_:] if os_ else d) == want
+
+
+def test_two_phase_equals_single_phase():
+    say("agg: two-phase vs oracle")
+    rng = np.random.default_rng(21)
+    n = 90000
+    key = rng.integers(0, 3000, n)
+    i64 = rng.integers(-(1 << 40), 1 << 40, n)
+    dec = rng.integers(-(10 ** 12) + 1, 10 ** 12, n)
+    f = rng.random(n)
+    oks = [rng.random(n) > 0.1 for _ in range(3)]
+    types = [(kgo.T_INT64, 0, 0), (kgo.T_DECIMAL64, 12, 2), (kgo.T_FLOAT64, 0, 0)]
+    aggs = [(kgo.AGG_COUNT_STAR, 0)] + [(fn, c) for c in range(3) for fn in ALL6]
+    o = kgo.Agg((kgo.T_INT64, 0, 0), types, aggs)
+    o.update(kgo.Col(kgo.T_INT64, key), [kgo.Col(t[0], v, ok, precision=t[1], scale=t[2])
+                                         for v, ok, t in zip((i64, dec, f), oks, types)])
+    ref = o.finish()
+    cuts = [0, 20000, 55000, n]
+    kp = [key[a:b] for a, b in zip(cuts[:-1], cuts[1:])]
+    vp = [[Col(T_MAP[t[0]], v[a:b], ok[a:b], t[1], t[2]) for v, ok, t in zip((i64, dec, f), oks, types)]
+          for a, b in zip(cuts[:-1], cuts[1:])]
+    r = _two_phase(kp, vp, [(T_MAP[t[0]], t[1], t[2]) for t in types], aggs, False)
+    compare(ref, r, aggs)
