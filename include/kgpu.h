@@ -197,6 +197,18 @@ int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *ou
                    int64_t *out_count);
 void kg_bloom_destroy(kg_bloom *b);
 
+/* The range form of a runtime filter and the scan's row stage
+ * (runtime_filter/member.rs Range, operators/runtime_filter.rs,
+ * operators/select.rs): ids of the rows whose value lies in [lo, hi]
+ * (integer-like columns, compared as int64; NULLs never pass), in input
+ * order, to out_rows (device); *out_count (host). */
+int kg_select_range(kg_ctx *ctx, const kg_column *col, int64_t nrows, int64_t lo, int64_t hi,
+                    int32_t *out_rows, int64_t *out_count);
+/* The selected rows (device row ids) of `cols` as a device batch: the take of
+ * a selection vector (operators/select.rs). */
+int kg_gather(kg_ctx *ctx, int32_t ncols, const kg_column *cols, const int32_t *rows,
+              int64_t nrows, kg_batch **out);
+
 /* ---- hash join ----------------------------------------------------------
  * HashJoin (crates/krishiv-exec/src/join/mod.rs:1014-1211): the build side is
  * a Sink (kg_join_build_sink per batch, then kg_join_build_finalize,
@@ -225,6 +237,10 @@ int kg_join_build_finalize(kg_hash_join *j);
 int64_t kg_join_build_rows(const kg_hash_join *j);
 int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                   kg_batch **out);
+/* The probe over the rows a selection vector keeps (device row ids, e.g. the
+ * runtime filter's survivors): the fused RowSelect of join/mod.rs:641-657. */
+int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                           const int32_t *sel_rows, int64_t nsel, kg_batch **out);
 void kg_join_destroy(kg_hash_join *j);
 
 /* ---- join fused with GROUP BY the build key -----------------------------

@@ -41,9 +41,10 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
   uint64_t *s_keys = smem;
   uint64_t *s_rec = smem + SLOTS;
   const int W = P.W;
+  const int WL = P.W | 1;  // odd LDS record stride: records spread over the LDS banks
   for (int64_t base = blockIdx.x * chunk; base < nrows; base += (int64_t)gridDim.x * chunk) {
     for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) s_keys[t] = kEmpty;
-    for (int t = threadIdx.x; t < SLOTS * W; t += blockDim.x) s_rec[t] = P.init[t % W];
+    for (int t = threadIdx.x; t < SLOTS * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
     __syncthreads();
     int64_t end = base + chunk < nrows ? base + chunk : nrows;
     if (P.direct || !P.has_key) {
@@ -65,13 +66,13 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
           int lead = __ffsll((unsigned long long)todo) - 1;
           int ks = __shfl(ls, lead);
           uint64_t peers = __ballot(active && ls == ks);
-          apply_wave<true>(P, s_rec + (uint64_t)ks * W, i, peers);
+          apply_wave<true>(P, s_rec + (uint64_t)ks * WL, i, peers);
           todo &= ~peers;
         }
       }
       __syncthreads();
       for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) {
-        const uint64_t *l = s_rec + (uint64_t)t * W;
+        const uint64_t *l = s_rec + (uint64_t)t * WL;
         if (l[0] == 0 || (!P.direct && t != 0)) continue;
         merge_record(P, global_record(P, T, (P.direct && t == 256) ? 2 : 0, (uint64_t)t), l);
       }
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
         }
       }
       if (ls >= 0) {
-        apply_row<true>(P, s_rec + (uint64_t)ls * W, i);
+        apply_row<true>(P, s_rec + (uint64_t)ls * WL, i);
       } else {
         apply_row<false>(P, global_record(P, T, cls, key), i);
       }
@@ -110,7 +111,7 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
     for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) {
       uint64_t k = s_keys[t];
       if (k == kEmpty) continue;
-      const uint64_t *l = s_rec + (uint64_t)t * W;
+      const uint64_t *l = s_rec + (uint64_t)t * WL;
       if (l[0] == 0) continue;  // no row
       merge_record(P, global_record(P, T, 0, k), l);
     }
@@ -906,7 +907,8 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
   // columns' ops adjacent, so a row loads each value once
   std::stable_sort(ops.begin() + 1, ops.end(),
                    [](const AggOp &x, const AggOp &y) { return x.col < y.col; });
-  if ((int)ops.size() > kAggOps || W > kAggWords) fail(KG_ERR_UNSUPPORTED, "too many aggregates");
+  // W < kAggWords: the LDS stride W|1 indexes init[] within bounds
+  if ((int)ops.size() > kAggOps || W >= kAggWords) fail(KG_ERR_UNSUPPORTED, "too many aggregates");
   P.nops = (int)ops.size();
   for (int k = 0; k < P.nops; k++) P.ops[k] = ops[k];
   P.W = W;
@@ -992,7 +994,7 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
     a->slots = 0;
     for (int slots : {512, 256, 128}) {
       if (a->direct && slots < 512) break;
-      size_t bytes = (size_t)slots * (W + 1) * 8;
+      size_t bytes = (size_t)slots * ((W | 1) + 1) * 8;
       if (bytes > 96 * 1024) continue;
       hipError_t e = hipSuccess;
       if (bytes > 64 * 1024) {

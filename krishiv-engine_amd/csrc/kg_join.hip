@@ -127,12 +127,12 @@ __device__ __forceinline__ int64_t jfind(const JTable &T, int cls, uint64_t key)
 }
 
 // Pass 1: matches and output rows of each probe row.
-__global__ void k_join_count(KeySpec K, int64_t n, JTable T, int join_type, uint32_t *head,
-                             uint32_t *nout) {
+__global__ void k_join_count(KeySpec K, int64_t n, const int32_t *sel, JTable T, int join_type,
+                             uint32_t *head, uint32_t *nout) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint64_t key;
-    int cls = pack_key(K, i, key);
+    int cls = pack_key(K, sel ? (int64_t)sel[i] : i, key);
     int64_t s = jfind(T, cls, key);
     uint32_t h = kNoRow, m = 0;
     if (s >= 0) {
@@ -153,27 +153,28 @@ __global__ void k_join_count(KeySpec K, int64_t n, JTable T, int join_type, uint
 }
 
 // Pass 2: the (probe row, build row) pairs; build row -1 pads.
-__global__ void k_join_pairs(int64_t n, const uint32_t *head, const uint32_t *nout,
-                             const uint64_t *off, const uint32_t *next, int join_type,
-                             int32_t *pidx, int32_t *bidx) {
+__global__ void k_join_pairs(int64_t n, const int32_t *sel, const uint32_t *head,
+                             const uint32_t *nout, const uint64_t *off, const uint32_t *next,
+                             int join_type, int32_t *pidx, int32_t *bidx) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint32_t o = nout[i];
     if (!o) continue;
     uint64_t p = off[i];
+    int32_t row = sel ? sel[i] : (int32_t)i;
     if (join_type == KG_JOIN_LEFT_SEMI || join_type == KG_JOIN_LEFT_ANTI) {
-      pidx[p] = (int32_t)i;
+      pidx[p] = row;
       if (bidx) bidx[p] = -1;
       continue;
     }
     uint32_t r = head[i];
     if (r == kNoRow) {  // left outer, unmatched
-      pidx[p] = (int32_t)i;
+      pidx[p] = row;
       bidx[p] = -1;
       continue;
     }
     for (uint32_t k = 0; k < o && r != kNoRow; k++, r = next[r]) {
-      pidx[p + k] = (int32_t)i;
+      pidx[p + k] = row;
       bidx[p + k] = (int32_t)r;
     }
   }
@@ -507,9 +508,9 @@ int kg_join_build_finalize(kg_hash_join *j) {
 
 int64_t kg_join_build_rows(const kg_hash_join *j) { return j ? j->build_rows : 0; }
 
-int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
-                  kg_batch **out) {
-  return guarded([&] {
+static void probe_impl(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                       const int32_t *sel, int64_t nsel, kg_batch **out) {
+  {
     *out = nullptr;
     if (!j->finalized) fail(KG_ERR_INTERNAL, "probe before the build finalized");
     if (ncols < 1 || ncols + (int)j->bcols.size() > kJoinCols)
@@ -528,23 +529,24 @@ int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t
     int jt = j->spec.join_type;
     bool emits_build = jt == KG_JOIN_INNER || jt == KG_JOIN_LEFT_OUTER;
     int64_t total = 0;
-    if (nrows > 0) {
+    int64_t np = sel ? nsel : nrows;  // probe rows (selected ones)
+    if (np > 0) {
       KeySpec K = key_spec(j, d.data(), j->spec.probe_keys);
-      j->head.ensure(j->ctx, (uint64_t)nrows * 4);
-      j->nout.ensure(j->ctx, (uint64_t)nrows * 4);
-      j->off.ensure(j->ctx, (uint64_t)(nrows + 1) * 8);
+      j->head.ensure(j->ctx, (uint64_t)np * 4);
+      j->nout.ensure(j->ctx, (uint64_t)np * 4);
+      j->off.ensure(j->ctx, (uint64_t)(np + 1) * 8);
       {
         ProfScope prof("join_probe", j->s);
-        k_join_count<<<stream_grid(nrows, 256), 256, 0, j->s>>>(
-            K, nrows, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
+        k_join_count<<<stream_grid(np, 256), 256, 0, j->s>>>(
+            K, np, sel, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
         KG_HIP(hipGetLastError());
       }
-      scan_u32(j, j->nout.get<uint32_t>(), j->off.get<uint64_t>(), nrows);
+      scan_u32(j, j->nout.get<uint32_t>(), j->off.get<uint64_t>(), np);
       uint64_t last_off = 0;
       uint32_t last_n = 0;
-      KG_HIP(hipMemcpyAsync(&last_off, j->off.get<uint64_t>() + nrows - 1, 8,
+      KG_HIP(hipMemcpyAsync(&last_off, j->off.get<uint64_t>() + np - 1, 8,
                             hipMemcpyDeviceToHost, j->s));
-      KG_HIP(hipMemcpyAsync(&last_n, j->nout.get<uint32_t>() + nrows - 1, 4,
+      KG_HIP(hipMemcpyAsync(&last_n, j->nout.get<uint32_t>() + np - 1, 4,
                             hipMemcpyDeviceToHost, j->s));
       KG_HIP(hipStreamSynchronize(j->s));
       total = (int64_t)(last_off + last_n);
@@ -582,8 +584,8 @@ int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t
     if (total > 0) {
       j->pidx.ensure(j->ctx, (uint64_t)total * 4);
       if (emits_build) j->bidx.ensure(j->ctx, (uint64_t)total * 4);
-      k_join_pairs<<<stream_grid(nrows, 256), 256, 0, j->s>>>(
-          nrows, j->head.get<uint32_t>(), j->nout.get<uint32_t>(), j->off.get<uint64_t>(),
+      k_join_pairs<<<stream_grid(np, 256), 256, 0, j->s>>>(
+          np, sel, j->head.get<uint32_t>(), j->nout.get<uint32_t>(), j->off.get<uint64_t>(),
           j->next.get<uint32_t>(), jt, j->pidx.get<int32_t>(),
           emits_build ? j->bidx.get<int32_t>() : nullptr);
       KG_HIP(hipGetLastError());
@@ -594,6 +596,20 @@ int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t
     }
     KG_HIP(hipStreamSynchronize(j->s));
     *out = b.release();
+  }
+}
+
+int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                  kg_batch **out) {
+  return guarded([&] { probe_impl(j, ncols, cols, nrows, nullptr, 0, out); });
+}
+
+int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
+                           const int32_t *sel_rows, int64_t nsel, kg_batch **out) {
+  return guarded([&] {
+    if (nsel > 0 && !sel_rows) fail(KG_ERR_INTERNAL, "no selection vector");
+    if (nsel > nrows) fail(KG_ERR_INTERNAL, "more selected rows than rows");
+    probe_impl(j, ncols, cols, nrows, sel_rows, nsel, out);
   });
 }
 

@@ -91,11 +91,17 @@ _SIGS = {
     "kg_bloom_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_bloom_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "kg_bloom_destroy": (None, [C.c_void_p]),
+    "kg_select_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
+                                  C.c_void_p, C.c_void_p]),
+    "kg_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                            C.c_void_p]),
     "kg_join_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kg_join_build_sink": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     "kg_join_build_finalize": (C.c_int, [C.c_void_p]),
     "kg_join_build_rows": (C.c_int64, [C.c_void_p]),
     "kg_join_probe": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]),
+    "kg_join_probe_selected": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64,
+                                         C.c_void_p, C.c_int64, C.c_void_p]),
     "kg_join_destroy": (None, [C.c_void_p]),
     "kg_joinagg_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                     C.c_uint32]),

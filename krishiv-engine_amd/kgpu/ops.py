@@ -300,6 +300,24 @@ def repartition(ctx: Context, cols, key_cols, partitions: int):
     return Batch(h), offs
 
 
+def select_range(ctx: Context, col: Col, lo: int, hi: int, out_rows: DeviceBuffer | None = None):
+    """Row ids whose value is in [lo, hi] (NULLs never pass), in order:
+    (DeviceBuffer of int32, count)."""
+    out = out_rows or DeviceBuffer(ctx, max(col.length, 1) * 4)
+    cnt = C.c_int64(0)
+    c = col.c()
+    check(lib().kg_select_range(ctx.h, C.byref(c), col.length, lo, hi, C.c_void_p(out.ptr),
+                                C.byref(cnt)))
+    return out, cnt.value
+
+
+def gather(ctx: Context, cols, rows: DeviceBuffer, n: int) -> Batch:
+    """The rows (device int32 ids) of `cols` as a device batch."""
+    h = C.c_void_p()
+    check(lib().kg_gather(ctx.h, len(cols), _cols(cols), C.c_void_p(rows.ptr), n, C.byref(h)))
+    return Batch(h)
+
+
 # ---------------------------------------------------------------- aggregation
 class HashAggregate:
     """GROUP BY one key column with count/sum/min/max/avg aggregates."""
@@ -383,6 +401,13 @@ class HashJoin:
         nrows = cols[0].length if nrows is None else nrows
         h = C.c_void_p()
         check(lib().kg_join_probe(self.h, len(cols), _cols(cols), nrows, C.byref(h)))
+        return Batch(h)
+
+    def probe_selected(self, cols, sel: DeviceBuffer, nsel: int, nrows=None) -> Batch:
+        nrows = cols[0].length if nrows is None else nrows
+        h = C.c_void_p()
+        check(lib().kg_join_probe_selected(self.h, len(cols), _cols(cols), nrows,
+                                           C.c_void_p(sel.ptr), nsel, C.byref(h)))
         return Batch(h)
 
     # join fused with GROUP BY the build key

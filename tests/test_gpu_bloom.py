@@ -89,3 +89,40 @@ def test_probe_edge_sizes():
         got = rows.to_numpy(np.int32, cnt)
         assert np.all(np.diff(got) > 0)
         assert np.all(np.isin(np.arange(0, min(n, 3000), 3), got))
+
+
+def test_config3_pipeline_small():
+    """Configuration 3 at 1/100: build keys kept by attr < 5 (range row stage),
+    an SBBF of the kept keys at fpp 0.03, the probe column pre-screened by it,
+    the survivors joined (selection-vector probe) and SUM(payload), COUNT(*)
+    aggregated on the device; equal to the oracle's join of every probe row."""
+    say("bloom: config-3 pipeline, 150K build, 6M probe")
+    nb, npr = 150_000, 6_000_000
+    key, _ = kgo.gen_join_build(4, 0, nb, nb, with_val=False)
+    attr = (np.random.default_rng(4).integers(0, 100, nb)).astype(np.int32)
+    fk, fkv, pay = kgo.gen_join_probe(5, 0, npr, nb, 0)
+    c = ctx()
+    rows, cnt = kgpu.select_range(c, Col(A.T_INT32, attr), 0, 4)
+    kept = kgpu.gather(c, [Col(A.T_INT64, key)], rows, cnt)
+    assert cnt == int(np.sum(attr < 5))
+    assert np.array_equal(kept.column(0)[0], key[attr < 5])
+    bloom = kgpu.RuntimeBloom(c, kgpu.RuntimeBloom.bytes_for(cnt, 0.03))
+    bloom.insert(kept.as_col(0), nrows=cnt)
+    j = kgpu.HashJoin(c, A.JOIN_INNER, [0], [0], expected_build_rows=cnt)
+    j.build_sink([kept.as_col(0)], cnt)
+    j.build_finalize()
+    fk_d = kgpu.DeviceBuffer.from_numpy(c, fk)
+    pay_d = kgpu.DeviceBuffer.from_numpy(c, pay)
+    probe_cols = [Col(A.T_INT64, fk_d, length=npr), Col(A.T_INT64, pay_d, length=npr)]
+    surv, ns = bloom.probe(probe_cols[0])
+    out = j.probe_selected(probe_cols, surv, ns, npr)
+    g = kgpu.HashAggregate(c, None, [(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)])
+    g.sink(None, [out.as_col(1)], out.num_rows)
+    g.finalize()
+    r = g.result()
+    keep = np.zeros(nb, dtype=bool)
+    keep[key[attr < 5]] = True
+    hit = keep[fk]
+    assert int(r.column(0)[0][0]) == int(pay[hit].sum())
+    assert int(r.column(1)[0][0]) == int(hit.sum())
+    assert ns >= hit.sum() and ns < 0.15 * npr  # 5 % true + about 3 % false positives
