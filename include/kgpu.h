@@ -255,6 +255,33 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
                      const int32_t *input_cols);
 int kg_joinagg_result(kg_hash_join *j, kg_batch **out);
 
+/* ---- grace hash join fused with the per-partition aggregate ------------
+ * The grace path of the hash join (join/grace.rs:99-1031: spill whole
+ * partitions, then per partition build a table and probe it) for the plan
+ *   SELECT p, COUNT(*), SUM((probe.pay * build.pay) % 2^31)
+ *   FROM build JOIN probe ON build.key = probe.key GROUP BY p
+ * with p = partition_of(H_kg(key), partitions) (exchange routing).  Rows are
+ * radix-partitioned on p into HBM partition buffers as they are sunk; past
+ * the context's HBM budget partitions go to pinned host memory.  finish()
+ * takes the partitions one at a time: the build partition is aggregated per
+ * key (row count, payload sum) in a table within `table_budget_bytes`, then
+ * the probe partition is streamed against it.  Per key, the pairs' count is
+ * nb*np and their payload-product sum Sb*Sp, so no pair is materialized;
+ * this equals the per-pair `% 2^31` only when every product is below 2^31
+ * and non-negative, which finish() checks from the payloads' ranges
+ * (KRISHIV_UNSUPPORTED_FEATURE otherwise).  Sums wrap at 64 bits (Spark
+ * non-ANSI bigint SUM).  Integer-like int64 keys and payloads. */
+typedef struct kg_grace kg_grace;
+int kg_grace_create(kg_ctx *ctx, uint32_t partitions, uint64_t table_budget_bytes, kg_grace **out);
+int kg_grace_build_sink(kg_grace *g, const kg_column *key, const kg_column *pay, int64_t nrows);
+int kg_grace_probe_sink(kg_grace *g, const kg_column *key, const kg_column *pay, int64_t nrows);
+/* Per output partition (host arrays of `partitions`): the pair count, the
+ * sum, and whether the sum is non-NULL (some pair had both payloads).
+ * *spilled_bytes: partition bytes that were kept in pinned host memory. */
+int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_valid,
+                    uint64_t *spilled_bytes);
+void kg_grace_destroy(kg_grace *g);
+
 /* ---- synthetic data on the device (bench and tests) --------------------
  * Counter-based generators; the integer ones are bit-identical to the
  * CPU generators used by the tests. */

@@ -150,3 +150,12 @@ def lib():
 def check(rc: int):
     if rc != OK:
         raise KgError(rc, lib().kg_last_error().decode())
+
+
+_SIGS.update({
+    "kg_grace_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p]),
+    "kg_grace_build_sink": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "kg_grace_probe_sink": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "kg_grace_finish": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_grace_destroy": (None, [C.c_void_p]),
+})

@@ -503,3 +503,47 @@ def prof_get(family: str):
     n = C.c_int64(0)
     check(lib().kg_prof_get(family.encode(), C.byref(ms), C.byref(n)))
     return ms.value, n.value
+
+
+# ---------------------------------------------------------------- grace join
+class GraceJoinAggregate:
+    """Grace hash join fused with SELECT p, COUNT(*), SUM(probe.pay *
+    build.pay % 2^31) ... GROUP BY p, p = partition_of(H_kg(key), partitions)."""
+
+    def __init__(self, ctx: Context, partitions=64, table_budget=8 << 30):
+        self.ctx = ctx
+        self.parts = partitions
+        h = C.c_void_p()
+        check(lib().kg_grace_create(ctx.h, partitions, table_budget, C.byref(h)))
+        self.h = h
+
+    def build_sink(self, key: Col, pay: Col, nrows=None):
+        nrows = key.length if nrows is None else nrows
+        k, p = key.c(), pay.c()
+        check(lib().kg_grace_build_sink(self.h, C.byref(k), C.byref(p), nrows))
+
+    def probe_sink(self, key: Col, pay: Col, nrows=None):
+        nrows = key.length if nrows is None else nrows
+        k, p = key.c(), pay.c()
+        check(lib().kg_grace_probe_sink(self.h, C.byref(k), C.byref(p), nrows))
+
+    def finish(self):
+        """(counts, sums, sum_valid, spilled_bytes)."""
+        counts = np.zeros(self.parts, dtype=np.int64)
+        sums = np.zeros(self.parts, dtype=np.int64)
+        valid = np.zeros(self.parts, dtype=np.uint8)
+        spilled = C.c_uint64(0)
+        check(lib().kg_grace_finish(self.h, counts.ctypes.data, sums.ctypes.data,
+                                    valid.ctypes.data, C.byref(spilled)))
+        return counts, sums, valid.astype(bool), spilled.value
+
+    def close(self):
+        if self.h:
+            lib().kg_grace_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
