@@ -282,6 +282,49 @@ int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_va
                     uint64_t *spilled_bytes);
 void kg_grace_destroy(kg_grace *g);
 
+/* ---- batched, continuously served join-aggregate requests ---------------
+ * A request is SELECT key, SUM(pay), COUNT(*) FROM build JOIN probe ON
+ * key = fk GROUP BY key (int64 keys/payloads, device-resident inputs).  The
+ * server packs every request waiting when the device frees into one batch:
+ * persistent kernels pull (request, row range) tiles from a device work list,
+ * so one launch per phase (table init, build, probe, output) serves requests
+ * of any size mix; the phase sequence is one captured hipGraph replayed per
+ * batch.  Requests are admitted while their tables, records and outputs fit
+ * the arena (the HBM budget, the analogue of the session's admission
+ * controller, crates/krishiv-session/src/admission.rs:168-191); build tables
+ * registered with kg_server_add_table stay resident and are shared by the
+ * requests that name them (the broadcast build tables of join/broadcast.rs).
+ * Two batches can be in flight (two streams), so requests that arrive while
+ * one batch runs are admitted into the next as arena space frees. */
+typedef struct kg_server kg_server;
+typedef struct kg_serve_request {
+  int64_t id;
+  const int64_t *build_keys; /* device; NULL: probe the resident table `table` */
+  int64_t build_rows;
+  int32_t table;
+  int32_t _pad;
+  const int64_t *probe_keys;   /* device */
+  const uint8_t *probe_valid;  /* device LSB bitmap or NULL */
+  const int64_t *probe_pay;    /* device */
+  int64_t probe_rows;
+} kg_serve_request;
+int kg_server_create(kg_ctx *ctx, uint64_t arena_bytes, int32_t max_batch, int32_t use_graph,
+                     kg_server **out);
+/* A resident build table over `n` device keys; *table_id names it. */
+int kg_server_add_table(kg_server *s, const int64_t *keys, int64_t n, int32_t *table_id);
+/* Queues a request (its inputs must stay valid until it completes). */
+int kg_server_submit(kg_server *s, const kg_serve_request *r);
+/* Non-blocking: retires finished batches (their ids to done_ids, at most
+ * max), then launches the next batch from the queue if a stream is free.
+ * wait != 0 blocks until at least one batch retires (or nothing is left). */
+int kg_server_progress(kg_server *s, int32_t wait, int64_t *done_ids, int32_t max,
+                       int32_t *ndone);
+/* The finished request's result [key, sum, count] (once; then forgotten). */
+int kg_server_take_result(kg_server *s, int64_t id, kg_batch **out);
+/* Requests queued and in flight. */
+int64_t kg_server_pending(const kg_server *s);
+void kg_server_destroy(kg_server *s);
+
 /* ---- synthetic data on the device (bench and tests) --------------------
  * Counter-based generators; the integer ones are bit-identical to the
  * CPU generators used by the tests. */

@@ -341,8 +341,7 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
             break;
           }
           uint64_t lo = w[0], hi = w[1];
-          bool wide_ok = !(a.in_type == KG_T_DECIMAL128 || a.st_type == KG_T_DECIMAL128) ||
-                         (int64_t)w[2] == ((int64_t)w[1] >> 63);
+          bool wide_ok = !a.wide || (int64_t)w[2] == ((int64_t)w[1] >> 63);
           uint64_t mlo = lo, mhi = hi;
           if (neg128(lo, hi)) negate128(mlo, mhi);
           if (!(wide_ok && lt_mag(mlo, mhi, a.pow_lo, a.pow_hi))) {
@@ -758,7 +757,7 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
     return w;
   };
   auto nulls_of = [&](int col) {
-    if (col >= kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 input columns per sink");
+    if (col >= kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 32 input columns per sink");
     if (nulls_word[col] < 0) {
       nulls_word[col] = words(1, 0);
       ops.push_back(AggOp{OP_NULLS, (uint8_t)col, 0, 0, (uint16_t)nulls_word[col], 0});
@@ -835,6 +834,7 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
           int sum_type = (t.type == KG_T_DECIMAL64 && sum_p <= 18) ? KG_T_DECIMAL64 : KG_T_DECIMAL128;
           bool wide = merge ? sum_type == KG_T_DECIMAL128 : t.type == KG_T_DECIMAL128;
           o.word = op_words(wide ? OP_ADD192 : OP_ADD128, c, true, avg, wide ? 3 : 2, 0, st);
+          o.wide = wide;
           o.st_word = st;
           o.nulls_word = nulls_of(c);
           checked.push_back({c, o.word, st, avg, wide, pow10_u128(sum_p),
@@ -863,7 +863,7 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
           bool two = avg || is_decimal(t.type);
           if (two) {
             int c2 = next_state++;
-            if (c2 >= kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 state columns per sink");
+            if (c2 >= kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 32 state columns per sink");
             if (avg) {
               // state (#sum, #count): the rows averaged are the summed counts
               o.cnt_word = words(1, 0);
@@ -964,7 +964,7 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
                   uint32_t flags, uint64_t expected_groups, kg_hash_agg **out) {
   return guarded([&] {
     *out = nullptr;
-    if (ninputs < 0 || ninputs > kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 input columns");
+    if (ninputs < 0 || ninputs > kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 32 input columns");
     auto a = std::make_unique<kg_hash_agg>();
     a->ctx = ctx;
     a->ansi = (flags & KG_AGG_ANSI) != 0;

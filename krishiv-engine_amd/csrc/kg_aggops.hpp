@@ -14,7 +14,7 @@
 namespace kg {
 
 constexpr uint64_t kEmpty = 0x8000000000000000ULL;
-constexpr int kAggInputs = 8;
+constexpr int kAggInputs = 32;
 constexpr int kAggOps = 48;
 constexpr int kAggWords = 64;
 constexpr int kAggBlock = 512;
@@ -552,6 +552,7 @@ struct OutAgg {
   int valid_word;  // >= 0: the count that decides validity (merged states)
   int cnt_word;    // >= 0: the row count an average divides by (merged states)
   int out_type, out_p, out_s, in_s, in_type;
+  int wide;  // the sum accumulator has three words (192 bits)
   void *values;
   uint8_t *validity;  // LSB bitmap (32-bit words, atomically set)
   uint64_t pow_lo, pow_hi;  // 10^out_p (decimal result bound)

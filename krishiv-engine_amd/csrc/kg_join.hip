@@ -619,7 +619,7 @@ int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_t
     if (!j->finalized) fail(KG_ERR_INTERNAL, "aggregate before the build finalized");
     if (j->spec.join_type != KG_JOIN_INNER || j->spec.num_keys != 1)
       fail(KG_ERR_UNSUPPORTED, "the fused aggregate needs an inner join on one key");
-    if (ninputs > kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 8 aggregate inputs");
+    if (ninputs > kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 32 aggregate inputs");
     j->agg_types.assign(input_types, input_types + ninputs);
     std::vector<CheckedSum> checked;
     compile_aggs(j->agg_types, naggs, aggs, (flags & KG_AGG_ANSI) != 0, false, false, j->prog,

@@ -1,0 +1,543 @@
+// kg_serve.hip -- batched, continuously served join-aggregate requests
+// (include/kgpu.h, kg_server_*): SELECT key, SUM(pay), COUNT(*) FROM build
+// JOIN probe ON key = fk GROUP BY key per request.
+//
+// A batch is every request admitted while a stream slot is free.  Its work
+// is four phases, each one persistent-kernel launch over a device work list
+// of (request, row range) tiles, so requests of 1K and 100M rows share a
+// launch and balance across the 256 CUs:
+//   init    the batch's own tables (16-byte slots {key, rows}) and records
+//   build   build keys inserted into their request's table (CAS); rows counted
+//   probe   probe rows looked up in their request's table (own or resident);
+//           a hit adds rows to COUNT and pay*rows to the exact 128-bit SUM
+//   output  occupied records compacted into the request's output region
+// The four launches read one device descriptor at a fixed address, so they
+// are captured once per stream slot as a hipGraph and replayed per batch
+// after the descriptor is uploaded.  Two slots (streams at default priority)
+// let the next batch form and launch while one runs.
+#include <hip/hip_runtime.h>
+
+#include <deque>
+#include <map>
+
+#include "kg_aggops.hpp"
+
+namespace kg {
+
+constexpr int kSrvBlock = 256;
+constexpr int64_t kSrvTile = 1 << 16;  // rows (or slots) per tile
+constexpr int kSrvSlots = 2;           // batches in flight
+
+struct SSlot {
+  uint64_t key;
+  uint64_t rows;
+};
+
+struct SReq {
+  const int64_t *bkeys;
+  int64_t nb;
+  const int64_t *pkeys;
+  const uint8_t *pvalid;
+  const int64_t *ppay;
+  int64_t np;
+  SSlot *slots;  // cap + 1 (slot cap: the key equal to EMPTY)
+  uint64_t cap;
+  int32_t log2cap;
+  int32_t own;
+  uint64_t *rec;  // (cap + 1) x {count, sum lo, sum hi}
+  int64_t *out_key, *out_sum, *out_cnt;
+};
+
+struct STile {
+  int32_t req;
+  int32_t pad;
+  int64_t lo, hi;
+};
+
+struct SDesc {
+  int32_t nreq;
+  int32_t ntiles[4];
+  unsigned int next[4];
+  SReq *reqs;
+  STile *tiles[4];
+  unsigned long long *groups;  // per request
+};
+
+__device__ __forceinline__ uint64_t sslot_start(uint64_t key, int log2cap) {
+  return dev::finalize(dev::row_hash1(key), 0x3C6EF372FE94F82BULL) >> (64 - log2cap);
+}
+
+// Claims tiles of phase `ph` until none is left (every workgroup exits).
+template <class F>
+__device__ __forceinline__ void for_tiles(SDesc *D, int ph, F &&f) {
+  __shared__ int s_t;
+  while (true) {
+    if (threadIdx.x == 0) s_t = (int)atomicAdd(&D->next[ph], 1u);
+    __syncthreads();
+    int t = s_t;
+    __syncthreads();
+    if (t >= D->ntiles[ph]) break;
+    const STile tile = D->tiles[ph][t];
+    f(D->reqs[tile.req], tile.req, tile.lo, tile.hi);
+  }
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_init(SDesc *D) {
+  for_tiles(D, 0, [&](const SReq &r, int, int64_t lo, int64_t hi) {
+    for (int64_t s = lo + threadIdx.x; s < hi; s += blockDim.x) {
+      if (r.own) r.slots[s] = SSlot{kEmpty, 0};
+      r.rec[3 * s] = 0;
+      r.rec[3 * s + 1] = 0;
+      r.rec[3 * s + 2] = 0;
+    }
+  });
+}
+
+__device__ __forceinline__ void sbuild_row(SSlot *slots, uint64_t cap, int log2cap, uint64_t key) {
+  uint64_t s;
+  if (key == kEmpty) {
+    s = cap;
+  } else {
+    s = sslot_start(key, log2cap);
+    uint64_t mask = cap - 1;
+    while (true) {
+      uint64_t k = __hip_atomic_load(&slots[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (k == key) break;
+      if (k == kEmpty) {
+        unsigned long long prev = atomicCAS((unsigned long long *)&slots[s].key, kEmpty, key);
+        if (prev == kEmpty || prev == key) break;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+  atomicAdd((unsigned long long *)&slots[s].rows, 1ull);
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_build(SDesc *D) {
+  for_tiles(D, 1, [&](const SReq &r, int, int64_t lo, int64_t hi) {
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+      sbuild_row(r.slots, r.cap, r.log2cap, (uint64_t)r.bkeys[i]);
+  });
+}
+
+__device__ __forceinline__ int64_t sfind(const SSlot *slots, uint64_t cap, int log2cap,
+                                         uint64_t key) {
+  if (key == kEmpty) return slots[cap].rows ? (int64_t)cap : -1;
+  uint64_t s = sslot_start(key, log2cap), mask = cap - 1;
+  while (true) {
+    uint64_t k = slots[s].key;
+    if (k == key) return (int64_t)s;
+    if (k == kEmpty) return -1;
+    s = (s + 1) & mask;
+  }
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_probe(SDesc *D) {
+  for_tiles(D, 2, [&](const SReq &r, int, int64_t lo, int64_t hi) {
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      if (r.pvalid && !((r.pvalid[i >> 3] >> (i & 7)) & 1)) continue;
+      int64_t s = sfind(r.slots, r.cap, r.log2cap, (uint64_t)r.pkeys[i]);
+      if (s < 0) continue;
+      uint64_t m = r.slots[s].rows;
+      if (!m) continue;
+      uint64_t *w = r.rec + 3 * s;
+      add_u64_nr<false>(w, m);
+      int64_t v = r.ppay[i];
+      // pay * m as 128 bits (two's complement)
+      uint64_t plo = (uint64_t)v * m;
+      uint64_t phi = __umul64hi((uint64_t)v, m) - (v < 0 ? m : 0);
+      add128<false>(w + 1, plo, phi);
+    }
+  });
+}
+
+__global__ void __launch_bounds__(kSrvBlock) k_srv_output(SDesc *D) {
+  for_tiles(D, 3, [&](const SReq &r, int q, int64_t lo, int64_t hi) {
+    for (int64_t s = lo + threadIdx.x; s < hi; s += blockDim.x) {
+      const uint64_t *w = r.rec + 3 * s;
+      if (!w[0]) continue;
+      uint64_t o = atomicAdd(D->groups + q, 1ull);
+      r.out_key[o] = s == (int64_t)r.cap ? (int64_t)kEmpty : (int64_t)r.slots[s].key;
+      r.out_sum[o] = (int64_t)w[1];  // SUM(bigint) wraps (non-ANSI)
+      r.out_cnt[o] = (int64_t)w[0];
+    }
+  });
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+static int slog2(uint64_t v) {
+  int b = 0;
+  while ((1ull << b) < v) b++;
+  return b;
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+struct kg_server {
+  kg_ctx *ctx = nullptr;
+  int32_t max_batch = 256;
+  bool use_graph = true;
+  uint64_t arena_bytes = 0;
+  struct Resident {
+    DevBuf slots;
+    uint64_t cap = 0;
+    int log2cap = 0;
+  };
+  std::vector<Resident> tables;
+  struct Pending {
+    kg_serve_request r;
+    uint64_t cap = 0;
+    uint64_t bytes = 0;
+  };
+  std::deque<Pending> queue;
+  struct Slot {
+    hipStream_t s = nullptr;
+    DevBuf arena;          // tables, records, outputs of the batch
+    DevBuf desc_dev;       // SDesc + SReq[max_batch] + tiles + group counters
+    uint8_t *desc_host = nullptr;  // pinned mirror of desc_dev
+    uint64_t desc_bytes = 0;
+    uint64_t max_tiles = 0;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<Pending> reqs;
+    std::vector<SReq> sreqs;
+  };
+  Slot slots[kSrvSlots];
+  std::map<int64_t, kg_batch *> results;
+  int64_t inflight = 0;
+  uint64_t launches = 0;
+};
+
+namespace kg {
+
+static uint64_t req_cap(kg_server *S, const kg_serve_request &r) {
+  if (!r.build_keys) {
+    if (r.table < 0 || r.table >= (int32_t)S->tables.size()) fail(KG_ERR_INTERNAL, "no such table");
+    return S->tables[r.table].cap;
+  }
+  uint64_t n = (uint64_t)(r.build_rows < 256 ? 256 : r.build_rows);
+  return 1ull << slog2(n * 2);
+}
+
+// Arena bytes of a request: own table slots, records and output columns.
+static uint64_t req_bytes(uint64_t cap, bool own) {
+  uint64_t per = (own ? 16 : 0) + 24 + 24;
+  return ((cap + 1) * per + 255) / 256 * 256;
+}
+
+static void layout_desc(kg_server *S, kg_server::Slot &sl) {
+  sl.max_tiles = 1 << 16;
+  sl.desc_bytes = sizeof(SDesc) + sizeof(SReq) * S->max_batch + 4 * sizeof(STile) * sl.max_tiles +
+                  8 * S->max_batch + 256;
+  sl.desc_dev.reset(S->ctx, sl.desc_bytes);
+  KG_HIP(hipHostMalloc((void **)&sl.desc_host, sl.desc_bytes, hipHostMallocDefault));
+}
+
+static SDesc *desc_ptr(kg_server::Slot &sl) { return sl.desc_dev.get<SDesc>(); }
+
+static void launch_phases(kg_server *S, kg_server::Slot &sl) {
+  unsigned grid = (unsigned)((S->ctx->num_cus > 0 ? S->ctx->num_cus : 256) * 4);
+  SDesc *D = desc_ptr(sl);
+  k_srv_init<<<grid, kSrvBlock, 0, sl.s>>>(D);
+  k_srv_build<<<grid, kSrvBlock, 0, sl.s>>>(D);
+  k_srv_probe<<<grid, kSrvBlock, 0, sl.s>>>(D);
+  k_srv_output<<<grid, kSrvBlock, 0, sl.s>>>(D);
+}
+
+static void capture_graph(kg_server *S, kg_server::Slot &sl) {
+  hipGraph_t g = nullptr;
+  KG_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
+  launch_phases(S, sl);
+  KG_HIP(hipStreamEndCapture(sl.s, &g));
+  KG_HIP(hipGraphInstantiate(&sl.graph, g, nullptr, nullptr, 0));
+  KG_HIP(hipGraphDestroy(g));
+}
+
+// Forms and launches one batch on a free slot from the head of the queue.
+static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
+  if (S->queue.empty()) return false;
+  sl.reqs.clear();
+  sl.sreqs.clear();
+  uint64_t used = 0;
+  while (!S->queue.empty() && (int32_t)sl.reqs.size() < S->max_batch) {
+    auto &p = S->queue.front();
+    if (used + p.bytes > S->arena_bytes) {
+      if (sl.reqs.empty()) fail(KG_ERR_MEMORY, "a request larger than the server's arena");
+      break;
+    }
+    used += p.bytes;
+    sl.reqs.push_back(p);
+    S->queue.pop_front();
+  }
+  // host descriptor
+  uint8_t *h = sl.desc_host;
+  SDesc *HD = reinterpret_cast<SDesc *>(h);
+  SReq *HR = reinterpret_cast<SReq *>(h + sizeof(SDesc));
+  STile *HT = reinterpret_cast<STile *>(h + sizeof(SDesc) + sizeof(SReq) * S->max_batch);
+  unsigned long long *HG = reinterpret_cast<unsigned long long *>(
+      h + sizeof(SDesc) + sizeof(SReq) * S->max_batch + 4 * sizeof(STile) * sl.max_tiles);
+  uint8_t *dbase = sl.desc_dev.get<uint8_t>();
+  *HD = SDesc{};
+  HD->nreq = (int32_t)sl.reqs.size();
+  HD->reqs = reinterpret_cast<SReq *>(dbase + sizeof(SDesc));
+  for (int ph = 0; ph < 4; ph++)
+    HD->tiles[ph] = reinterpret_cast<STile *>(dbase + sizeof(SDesc) + sizeof(SReq) * S->max_batch +
+                                              sizeof(STile) * sl.max_tiles * ph);
+  HD->groups = reinterpret_cast<unsigned long long *>(
+      dbase + sizeof(SDesc) + sizeof(SReq) * S->max_batch + 4 * sizeof(STile) * sl.max_tiles);
+  uint8_t *arena = sl.arena.get<uint8_t>();
+  uint64_t at = 0;
+  int ntiles[4] = {0, 0, 0, 0};
+  auto add_tiles = [&](int ph, int q, int64_t n) {
+    for (int64_t lo = 0; lo < n; lo += kSrvTile) {
+      if ((uint64_t)ntiles[ph] >= sl.max_tiles) fail(KG_ERR_UNSUPPORTED, "too many tiles in a batch");
+      STile &t = HT[sl.max_tiles * ph + ntiles[ph]++];
+      t.req = q;
+      t.lo = lo;
+      t.hi = lo + kSrvTile < n ? lo + kSrvTile : n;
+    }
+  };
+  for (int q = 0; q < (int)sl.reqs.size(); q++) {
+    const kg_serve_request &r = sl.reqs[q].r;
+    SReq s{};
+    bool own = r.build_keys != nullptr;
+    s.cap = sl.reqs[q].cap;
+    s.log2cap = slog2(s.cap);
+    s.own = own;
+    if (own) {
+      s.slots = reinterpret_cast<SSlot *>(arena + at);
+      at += (s.cap + 1) * 16;
+    } else {
+      s.slots = S->tables[r.table].slots.get<SSlot>();
+    }
+    s.rec = reinterpret_cast<uint64_t *>(arena + at);
+    at += (s.cap + 1) * 24;
+    s.out_key = reinterpret_cast<int64_t *>(arena + at);
+    at += (s.cap + 1) * 8;
+    s.out_sum = reinterpret_cast<int64_t *>(arena + at);
+    at += (s.cap + 1) * 8;
+    s.out_cnt = reinterpret_cast<int64_t *>(arena + at);
+    at += (s.cap + 1) * 8;
+    at = (at + 255) / 256 * 256;
+    s.bkeys = r.build_keys;
+    s.nb = own ? r.build_rows : 0;
+    s.pkeys = r.probe_keys;
+    s.pvalid = r.probe_valid;
+    s.ppay = r.probe_pay;
+    s.np = r.probe_rows;
+    HR[q] = s;
+    sl.sreqs.push_back(s);
+    add_tiles(0, q, (int64_t)s.cap + 1);
+    if (own) add_tiles(1, q, s.nb);
+    add_tiles(2, q, s.np);
+    add_tiles(3, q, (int64_t)s.cap + 1);
+    HG[q] = 0;
+  }
+  for (int ph = 0; ph < 4; ph++) HD->ntiles[ph] = ntiles[ph];
+  // upload the descriptor (every tile list up to its used length)
+  auto up = [&](uint64_t off, uint64_t bytes) {
+    if (bytes) KG_HIP(hipMemcpyAsync(dbase + off, h + off, bytes, hipMemcpyHostToDevice, sl.s));
+  };
+  up(0, sizeof(SDesc) + sizeof(SReq) * sl.reqs.size());
+  for (int ph = 0; ph < 4; ph++)
+    up(sizeof(SDesc) + sizeof(SReq) * S->max_batch + sizeof(STile) * sl.max_tiles * ph,
+       sizeof(STile) * ntiles[ph]);
+  up(sizeof(SDesc) + sizeof(SReq) * S->max_batch + 4 * sizeof(STile) * sl.max_tiles,
+     8 * sl.reqs.size());
+  if (S->use_graph) {
+    if (!sl.graph) capture_graph(S, sl);
+    KG_HIP(hipGraphLaunch(sl.graph, sl.s));
+  } else {
+    launch_phases(S, sl);
+    KG_HIP(hipGetLastError());
+  }
+  KG_HIP(hipEventRecord(sl.done, sl.s));
+  sl.busy = true;
+  S->inflight += (int64_t)sl.reqs.size();
+  S->launches++;
+  return true;
+}
+
+static void retire(kg_server *S, kg_server::Slot &sl, int64_t *done_ids, int32_t max,
+                   int32_t *ndone) {
+  std::vector<unsigned long long> groups(sl.reqs.size());
+  uint8_t *gdev = sl.desc_dev.get<uint8_t>() + sizeof(SDesc) + sizeof(SReq) * S->max_batch +
+                  4 * sizeof(STile) * sl.max_tiles;
+  KG_HIP(hipMemcpyAsync(groups.data(), gdev, 8 * groups.size(), hipMemcpyDeviceToHost, sl.s));
+  KG_HIP(hipStreamSynchronize(sl.s));
+  for (size_t q = 0; q < sl.reqs.size(); q++) {
+    int64_t n = (int64_t)groups[q];
+    auto b = std::make_unique<kg_batch>();
+    b->ctx = S->ctx;
+    b->nrows = n;
+    b->cols.resize(3);
+    const SReq &s = sl.sreqs[q];
+    const int64_t *src[3] = {s.out_key, s.out_sum, s.out_cnt};
+    for (int c = 0; c < 3; c++) {
+      b->cols[c].type = KG_T_INT64;
+      b->cols[c].values.reset(S->ctx, (uint64_t)n * 8 + 8);
+      if (n)
+        KG_HIP(hipMemcpyAsync(b->cols[c].values.get(), src[c], (size_t)n * 8,
+                              hipMemcpyDeviceToDevice, sl.s));
+    }
+    S->results[sl.reqs[q].r.id] = b.release();
+    if (*ndone < max) done_ids[(*ndone)++] = sl.reqs[q].r.id;
+  }
+  KG_HIP(hipStreamSynchronize(sl.s));
+  S->inflight -= (int64_t)sl.reqs.size();
+  sl.busy = false;
+  sl.reqs.clear();
+}
+
+}  // namespace kg
+
+extern "C" {
+
+int kg_server_create(kg_ctx *ctx, uint64_t arena_bytes, int32_t max_batch, int32_t use_graph,
+                     kg_server **out) {
+  return guarded([&] {
+    *out = nullptr;
+    if (max_batch < 1 || max_batch > 4096) fail(KG_ERR_UNSUPPORTED, "1 to 4096 requests per batch");
+    auto S = std::make_unique<kg_server>();
+    S->ctx = ctx;
+    S->max_batch = max_batch;
+    S->use_graph = use_graph != 0;
+    S->arena_bytes = arena_bytes / kSrvSlots;
+    for (auto &sl : S->slots) {
+      sl.s = make_stream(ctx->device);
+      sl.arena.reset(ctx, S->arena_bytes);
+      layout_desc(S.get(), sl);
+      KG_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+    *out = S.release();
+  });
+}
+
+int kg_server_add_table(kg_server *S, const int64_t *keys, int64_t n, int32_t *table_id) {
+  return guarded([&] {
+    kg_server::Resident t;
+    uint64_t want = (uint64_t)(n < 256 ? 256 : n);
+    t.cap = 1ull << slog2(want * 2);
+    t.log2cap = slog2(t.cap);
+    t.slots.reset(S->ctx, (t.cap + 1) * 16);
+    // one-request descriptor on slot 0's stream: init + build
+    kg_server::Slot &sl = S->slots[0];
+    if (sl.busy) {
+      KG_HIP(hipEventSynchronize(sl.done));
+    }
+    uint8_t *h = sl.desc_host;
+    uint8_t *dbase = sl.desc_dev.get<uint8_t>();
+    SDesc *HD = reinterpret_cast<SDesc *>(h);
+    SReq *HR = reinterpret_cast<SReq *>(h + sizeof(SDesc));
+    STile *HT = reinterpret_cast<STile *>(h + sizeof(SDesc) + sizeof(SReq) * S->max_batch);
+    *HD = SDesc{};
+    HD->nreq = 1;
+    HD->reqs = reinterpret_cast<SReq *>(dbase + sizeof(SDesc));
+    for (int ph = 0; ph < 4; ph++)
+      HD->tiles[ph] = reinterpret_cast<STile *>(dbase + sizeof(SDesc) + sizeof(SReq) * S->max_batch +
+                                                sizeof(STile) * sl.max_tiles * ph);
+    DevBuf scratch(S->ctx, (t.cap + 1) * 24);
+    SReq s{};
+    s.bkeys = keys;
+    s.nb = n;
+    s.slots = t.slots.get<SSlot>();
+    s.cap = t.cap;
+    s.log2cap = t.log2cap;
+    s.own = 1;
+    s.rec = scratch.get<uint64_t>();
+    HR[0] = s;
+    int nt[4] = {0, 0, 0, 0};
+    for (int64_t lo = 0; lo < (int64_t)t.cap + 1; lo += kSrvTile)
+      HT[nt[0]++] = STile{0, 0, lo, lo + kSrvTile < (int64_t)t.cap + 1 ? lo + kSrvTile : (int64_t)t.cap + 1};
+    for (int64_t lo = 0; lo < n; lo += kSrvTile)
+      HT[sl.max_tiles + nt[1]++] = STile{0, 0, lo, lo + kSrvTile < n ? lo + kSrvTile : n};
+    if ((uint64_t)nt[0] > sl.max_tiles || (uint64_t)nt[1] > sl.max_tiles)
+      fail(KG_ERR_UNSUPPORTED, "resident table too large for one batch descriptor");
+    HD->ntiles[0] = nt[0];
+    HD->ntiles[1] = nt[1];
+    KG_HIP(hipMemcpyAsync(dbase, h, sl.desc_bytes, hipMemcpyHostToDevice, sl.s));
+    unsigned grid = (unsigned)((S->ctx->num_cus > 0 ? S->ctx->num_cus : 256) * 4);
+    k_srv_init<<<grid, kSrvBlock, 0, sl.s>>>(desc_ptr(sl));
+    k_srv_build<<<grid, kSrvBlock, 0, sl.s>>>(desc_ptr(sl));
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipStreamSynchronize(sl.s));
+    *table_id = (int32_t)S->tables.size();
+    S->tables.push_back(std::move(t));
+  });
+}
+
+int kg_server_submit(kg_server *S, const kg_serve_request *r) {
+  return guarded([&] {
+    if (!r->probe_keys || !r->probe_pay) fail(KG_ERR_INTERNAL, "request without probe columns");
+    if (r->build_keys == nullptr && (r->table < 0 || r->table >= (int32_t)S->tables.size()))
+      fail(KG_ERR_INTERNAL, "request names no resident table");
+    kg_server::Pending p;
+    p.r = *r;
+    p.cap = req_cap(S, *r);
+    p.bytes = req_bytes(p.cap, r->build_keys != nullptr);
+    if (p.bytes > S->arena_bytes)
+      fail(KG_ERR_MEMORY, "request larger than the server's arena share");
+    S->queue.push_back(p);
+  });
+}
+
+int kg_server_progress(kg_server *S, int32_t wait, int64_t *done_ids, int32_t max,
+                       int32_t *ndone) {
+  return guarded([&] {
+    *ndone = 0;
+    bool waited = false;
+    for (int pass = 0; pass < 2; pass++) {
+      for (auto &sl : S->slots) {
+        if (!sl.busy) continue;
+        hipError_t q = hipEventQuery(sl.done);
+        if (q == hipErrorNotReady) {
+          if (!(wait && !waited)) continue;
+          KG_HIP(hipEventSynchronize(sl.done));
+          waited = true;
+        } else if (q != hipSuccess) {
+          KG_HIP(q);
+        }
+        retire(S, sl, done_ids, max, ndone);
+      }
+      for (auto &sl : S->slots)
+        if (!sl.busy && !S->queue.empty()) launch_batch(S, sl);
+      if (!wait || waited || S->inflight == 0) break;
+    }
+  });
+}
+
+int kg_server_take_result(kg_server *S, int64_t id, kg_batch **out) {
+  return guarded([&] {
+    auto it = S->results.find(id);
+    if (it == S->results.end()) fail(KG_ERR_INTERNAL, "no finished request with that id");
+    *out = it->second;
+    S->results.erase(it);
+  });
+}
+
+int64_t kg_server_pending(const kg_server *S) {
+  return S ? (int64_t)S->queue.size() + S->inflight : 0;
+}
+
+void kg_server_destroy(kg_server *S) {
+  if (!S) return;
+  for (auto &sl : S->slots) {
+    if (sl.s) (void)hipStreamSynchronize(sl.s);
+    if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.desc_host) (void)hipHostFree(sl.desc_host);
+  }
+  for (auto &kv : S->results) kg_batch_release(kv.second);
+  hipStream_t ss[kSrvSlots];
+  for (int i = 0; i < kSrvSlots; i++) ss[i] = S->slots[i].s;
+  delete S;
+  for (auto s : ss) destroy_stream(s);
+}
+
+}  // extern "C"

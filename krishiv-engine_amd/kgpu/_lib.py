@@ -159,3 +159,22 @@ _SIGS.update({
     "kg_grace_finish": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_grace_destroy": (None, [C.c_void_p]),
 })
+
+
+class ServeRequest(C.Structure):
+    """kg_serve_request (include/kgpu.h)."""
+    _fields_ = [("id", C.c_int64), ("build_keys", C.c_void_p), ("build_rows", C.c_int64),
+                ("table", C.c_int32), ("_pad", C.c_int32), ("probe_keys", C.c_void_p),
+                ("probe_valid", C.c_void_p), ("probe_pay", C.c_void_p),
+                ("probe_rows", C.c_int64)]
+
+
+_SIGS.update({
+    "kg_server_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p]),
+    "kg_server_add_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "kg_server_submit": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_server_progress": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+    "kg_server_take_result": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "kg_server_pending": (C.c_int64, [C.c_void_p]),
+    "kg_server_destroy": (None, [C.c_void_p]),
+})

@@ -547,3 +547,63 @@ class GraceJoinAggregate:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- serving
+class JoinAggServer:
+    """Batched, continuously served join-aggregate requests (kg_server_*):
+    each request is SELECT key, SUM(pay), COUNT(*) FROM build JOIN probe ON
+    key = fk GROUP BY key over device-resident int64 columns.  Requests are
+    packed into batches admitted by arena bytes; resident build tables are
+    shared by the requests that name them."""
+
+    def __init__(self, ctx: Context, arena_bytes=8 << 30, max_batch=256, use_graph=True):
+        from ._lib import ServeRequest
+        self._Req = ServeRequest
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().kg_server_create(ctx.h, arena_bytes, max_batch, 1 if use_graph else 0,
+                                     C.byref(h)))
+        self.h = h
+
+    def add_table(self, keys: DeviceBuffer, n: int) -> int:
+        tid = C.c_int32(-1)
+        check(lib().kg_server_add_table(self.h, keys.ptr, n, C.byref(tid)))
+        return tid.value
+
+    def submit(self, rid: int, probe_keys: DeviceBuffer, probe_pay: DeviceBuffer,
+               probe_rows: int, build_keys: DeviceBuffer | None = None, build_rows: int = 0,
+               table: int = -1, probe_valid: DeviceBuffer | None = None):
+        r = self._Req(rid, build_keys.ptr if build_keys is not None else None, build_rows,
+                      table, 0, probe_keys.ptr, probe_valid.ptr if probe_valid is not None else None,
+                      probe_pay.ptr, probe_rows)
+        check(lib().kg_server_submit(self.h, C.byref(r)))
+
+    def progress(self, wait=False, max_ids=4096):
+        """Ids of the requests finished since the last call (launches the
+        next batch when a stream slot is free)."""
+        ids = np.zeros(max_ids, dtype=np.int64)
+        nd = C.c_int32(0)
+        check(lib().kg_server_progress(self.h, 1 if wait else 0, ids.ctypes.data, max_ids,
+                                       C.byref(nd)))
+        return ids[:nd.value].tolist()
+
+    def result(self, rid: int) -> Batch:
+        """The finished request's (key, sum, count) batch, in HBM."""
+        h = C.c_void_p()
+        check(lib().kg_server_take_result(self.h, rid, C.byref(h)))
+        return Batch(h)
+
+    def pending(self) -> int:
+        return int(lib().kg_server_pending(self.h))
+
+    def close(self):
+        if self.h:
+            lib().kg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
