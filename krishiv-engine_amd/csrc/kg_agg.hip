@@ -32,6 +32,7 @@
 #include <memory>
 
 #include "kg_aggops.hpp"
+#include "kg_aggtile.hpp"
 
 namespace kg {
 template <int SLOTS>
@@ -116,6 +117,547 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
       merge_record(P, global_record(P, T, 0, k), l);
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- hashed keys
+// Pass 1 (k_agg_pass1): each workgroup takes a contiguous chunk and applies
+// it tile by tile (R rows a lane, op-major) to an LDS table of SLOTS records
+// plus two records for the key INT64_MIN and the NULL key.  Rows whose key
+// finds no LDS slot within kLdsProbe probes are appended to the row spill.
+// Every kEvictRows rows the records with fewer than kHotRows rows (keys seen
+// once or twice: the Zipf tail) move to the record spill and free their
+// slots, so the LDS table keeps the heavy keys; at the end every record
+// moves there.  The global table is not touched by ordinary keys.
+//
+// Spill space: each wave takes kWaveChunk rows of the row spill at a time
+// with one atomic; rows it leaves unused are marked dead (bit 63 of the
+// validity word) and skipped by the partition pass.
+//
+// Drain: both spills are partitioned by home-slot range (k_spill_hist,
+// k_spill_scatter); k_agg_merge gives each partition to one workgroup, which
+// aggregates its inputs in LDS and, when the LDS table fills or the
+// partition ends, merges each LDS record into the global record of its key
+// with plain loads and stores: every key's global record is written by its
+// partition's workgroup alone.  New keys take a slot with one CAS each,
+// within a per-partition quota that keeps the table below 7/8 full; a
+// workgroup that would pass it stops, leaving its unmerged records and
+// pending inputs for the host, which grows the table, merges them with
+// atomics (k_agg_retry) and resumes the partition.
+constexpr int kP1Block = 512;
+constexpr int kMergeBlock = 256;
+constexpr int kTileRows = 4;
+constexpr int kWaveChunk = 512;
+constexpr int64_t kEvictRows = 32768;
+constexpr uint64_t kHotRows = 4;
+constexpr uint64_t kDeadRow = 1ull << 63;
+
+struct SpillDev {
+  uint64_t *rows;
+  unsigned long long *nrows;
+  uint64_t cap_rows;
+  uint64_t *recs;
+  unsigned long long *nrecs;
+  uint64_t cap_recs;
+  unsigned int *overflow;
+  SpillFmt F;
+};
+
+// Slots for the lanes in `mask` (every lane of the wave calls it with the
+// same mask): one atomic per wave; returns this lane's index.
+__device__ __forceinline__ int64_t spill_reserve(unsigned long long *counter, uint64_t mask) {
+  const int lane = __lane_id();
+  const int lead = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == lead) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+  base = __shfl(base, lead);
+  return (int64_t)base + __popcll(mask & ((1ull << lane) - 1));
+}
+
+// A wave's current piece of the row spill: rows [base + used, base + kWaveChunk).
+struct WaveChunk {
+  int64_t base = -1;
+  int used = kWaveChunk;
+};
+
+// Marks the unused rows of the wave's piece dead (every lane calls it).
+__device__ __forceinline__ void chunk_retire(const SpillDev &S, WaveChunk &wc) {
+  if (wc.base < 0) return;
+  for (int j = wc.used + __lane_id(); j < kWaveChunk; j += 64) {
+    const int64_t pos = wc.base + j;
+    if ((uint64_t)pos < S.cap_rows) S.rows[pos * (int64_t)S.F.rw + 1] = kDeadRow;
+  }
+  wc.base = -1;
+}
+
+// Row-spill positions for the lanes in `mask` (every lane calls it).
+__device__ __forceinline__ int64_t chunk_take(const SpillDev &S, WaveChunk &wc, uint64_t mask) {
+  const int k = __popcll(mask);
+  if (wc.used + k > kWaveChunk) {
+    chunk_retire(S, wc);
+    unsigned long long b = 0;
+    if (__lane_id() == 0) b = atomicAdd(S.nrows, (unsigned long long)kWaveChunk);
+    wc.base = (int64_t)__shfl(b, 0);
+    wc.used = 0;
+  }
+  const int64_t pos = wc.base + wc.used + __popcll(mask & ((1ull << __lane_id()) - 1));
+  wc.used += k;
+  return pos;
+}
+
+// Writes the spill rows of the lane's rows with miss[r] (pos[r] their row):
+// key, validity bits, then the column values (one type dispatch a column).
+template <int R>
+__device__ __forceinline__ void write_spill_rows(const AggProg &P, const SpillDev &S,
+                                                 const ColSrc<R> &src, const bool (&miss)[R],
+                                                 const uint64_t (&key)[R],
+                                                 const int64_t (&pos)[R]) {
+  const int rw = S.F.rw;
+  uint64_t vbits[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) vbits[r] = 0;
+  uint64_t lo[R], hi[R];
+  double f[R];
+  bool ok[R];
+  for (int c = 0; c < S.F.ncols; c++) {
+    src.load(P, c, miss, lo, hi, f, ok);
+    const bool fl = dev::is_f(P.in[c].type), wide = P.in[c].type == KG_T_DECIMAL128;
+    const int o = S.F.off[c];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (!miss[r]) continue;
+      uint64_t *w = S.rows + pos[r] * rw + o;
+      if (ok[r]) vbits[r] |= 1ull << c;
+      w[0] = !ok[r] ? 0 : fl ? (uint64_t)__double_as_longlong(f[r]) : lo[r];
+      if (wide) w[1] = ok[r] ? hi[r] : 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (miss[r]) {
+      S.rows[pos[r] * rw] = key[r];
+      S.rows[pos[r] * rw + 1] = vbits[r];
+    }
+}
+
+// LDS home slot of a key among `slots` (low 32 bits of the slot hash; the
+// global home uses the top bits).
+__device__ __forceinline__ uint32_t lds_home(const Table &T, uint64_t key, uint32_t slots) {
+  return (uint32_t)(((slot_hash(T, key) & 0xFFFFFFFFull) * slots) >> 32);
+}
+
+__device__ __forceinline__ int lds_probe(uint64_t *s_keys, uint32_t slots, const Table &T,
+                                         uint64_t key) {
+  uint32_t s = lds_home(T, key, slots);
+  for (int p = 0; p < kLdsProbe; p++) {
+    uint64_t k = s_keys[s];
+    if (k == key) return (int)s;
+    if (k == kEmpty) {
+      unsigned long long prev = atomicCAS((unsigned long long *)&s_keys[s], kEmpty, key);
+      if (prev == kEmpty || prev == key) return (int)s;
+    }
+    s = s + 1 == slots ? 0 : s + 1;
+  }
+  return -1;
+}
+
+// Moves LDS records to the record spill and clears their slots: all of them
+// (all = true) or those with fewer than kHotRows rows.  Every thread calls it.
+__device__ __forceinline__ void spill_lds_records(const AggProg &P, const SpillDev &S,
+                                                  uint64_t *s_keys, uint64_t *s_rec, int SLOTS,
+                                                  bool all) {
+  const int W = P.W, WL = W | 1;
+  __syncthreads();
+  for (int t0 = 0; t0 < SLOTS; t0 += blockDim.x) {
+    const int t = t0 + threadIdx.x;
+    bool has = false;
+    if (t < SLOTS && s_keys[t] != kEmpty) {
+      const uint64_t rows = s_rec[t * WL];
+      has = rows != 0 && (all || rows < kHotRows);
+    }
+    const uint64_t m = __ballot(has);
+    if (!m) continue;
+    const int64_t pos = spill_reserve(S.nrecs, m);
+    if (has) {
+      uint64_t *l = s_rec + t * WL;
+      if ((uint64_t)pos < S.cap_recs) {
+        uint64_t *d = S.recs + pos * (int64_t)(1 + W);
+        d[0] = s_keys[t];
+        for (int w = 0; w < W; w++) d[1 + w] = l[w];
+      } else {
+        atomicOr(S.overflow, 2u);
+      }
+      s_keys[t] = kEmpty;
+      for (int w = 0; w < W; w++) l[w] = P.init[w];
+    }
+  }
+  __syncthreads();
+}
+
+template <int R>
+__global__ void __launch_bounds__(kP1Block) k_agg_pass1(AggProg P, Table T, SpillDev S,
+                                                        int64_t nrows, int64_t chunk, int SLOTS) {
+  extern __shared__ uint64_t smem[];
+  uint64_t *s_keys = smem;
+  uint64_t *s_rec = smem + SLOTS;  // SLOTS + 2 records
+  const int W = P.W;
+  const int WL = W | 1;  // odd LDS record stride: records spread over the banks
+  for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) s_keys[t] = kEmpty;
+  for (int t = threadIdx.x; t < (SLOTS + 2) * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < nrows ? lo + chunk : nrows;
+  WaveChunk wc;
+  int64_t next_evict = lo + kEvictRows;
+  for (int64_t base = lo; base < hi; base += (int64_t)R * blockDim.x) {
+    if (base >= next_evict) {
+      spill_lds_records(P, S, s_keys, s_rec, SLOTS, false);
+      next_evict = base + kEvictRows;
+    }
+    int ls[R];
+    bool miss[R];
+    uint64_t key[R];
+    int64_t pos[R];
+    ColSrc<R> src;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t i = base + (int64_t)r * blockDim.x + threadIdx.x;
+      src.idx[r] = i;
+      ls[r] = -1;
+      miss[r] = false;
+      key[r] = 0;
+      pos[r] = 0;
+      if (i < hi) {
+        const int cls = row_key(P, i, key[r]);
+        if (cls) {
+          ls[r] = SLOTS + cls - 1;
+        } else {
+          ls[r] = lds_probe(s_keys, (uint32_t)SLOTS, T, key[r]);
+          miss[r] = ls[r] < 0;
+        }
+      }
+    }
+    tile_apply<R>(P, s_rec, WL, ls, src);
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const uint64_t m = __ballot(miss[r]);
+      if (!m) continue;
+      pos[r] = chunk_take(S, wc, m);
+      if (miss[r] && (uint64_t)pos[r] >= S.cap_rows) {
+        atomicOr(S.overflow, 1u);
+        miss[r] = false;
+      }
+      any = true;
+    }
+    if (any) write_spill_rows<R>(P, S, src, miss, key, pos);
+  }
+  chunk_retire(S, wc);
+  spill_lds_records(P, S, s_keys, s_rec, SLOTS, true);
+  // the key INT64_MIN and the NULL key: their fixed slots, with atomics
+  if (threadIdx.x < 2) {
+    const uint64_t *l = s_rec + (SLOTS + threadIdx.x) * WL;
+    if (l[0]) merge_record(P, T.rec + (T.cap + threadIdx.x) * (uint64_t)W, l);
+  }
+}
+
+// Partition of a spilled key: its home slot's range, top log2p bits of the
+// slot hash (home slot = top log2cap bits).
+__device__ __forceinline__ uint32_t spill_part(const Table &T, uint64_t key, int log2p) {
+  return log2p ? (uint32_t)(slot_hash(T, key) >> (64 - log2p)) : 0u;
+}
+
+// Per-block partition histogram: hist[p * gridDim + b] (dead rows of the row
+// spill, `dead` set, are left out).
+__global__ void k_spill_hist(const uint64_t *rows, int w, uint64_t n, Table T, int log2p,
+                             int64_t per_block, int dead, uint32_t *hist) {
+  extern __shared__ uint32_t s_h[];
+  const uint32_t np = 1u << log2p;
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) s_h[p] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = lo + per_block < (int64_t)n ? lo + per_block : (int64_t)n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    if (dead && (rows[i * w + 1] & kDeadRow)) continue;
+    atomicAdd(&s_h[spill_part(T, rows[i * w], log2p)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_h[p];
+}
+
+// Moves each row to its partition (order within a partition is not kept).
+__global__ void k_spill_scatter(const uint64_t *rows, int w, uint64_t n, Table T, int log2p,
+                                int64_t per_block, int dead, const uint64_t *base, uint64_t *out) {
+  extern __shared__ uint32_t s_c[];
+  const uint32_t np = 1u << log2p;
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x)
+    s_c[p] = (uint32_t)base[(uint64_t)p * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = lo + per_block < (int64_t)n ? lo + per_block : (int64_t)n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const uint64_t *src = rows + i * w;
+    if (dead && (src[1] & kDeadRow)) continue;
+    const uint32_t pos = atomicAdd(&s_c[spill_part(T, src[0], log2p)], 1u);
+    uint64_t *dst = out + (uint64_t)pos * w;
+    for (int k = 0; k < w; k++) dst[k] = src[k];
+  }
+}
+
+struct MergeDev {
+  const uint64_t *rows;   // partitioned row spill
+  uint64_t nrows;
+  const uint64_t *rbase;  // scanned histogram: partition p starts at rbase[p * rnb]
+  uint32_t rnb;
+  const uint64_t *recs;   // partitioned record spill
+  uint64_t nrecs;
+  const uint64_t *cbase;
+  uint32_t cnb;
+  int log2p;
+  int64_t *resume;        // per partition: -1 merged, else the next input
+  uint32_t *inserted;     // per partition: keys inserted by this launch
+  uint32_t quota;         // keys a partition may insert in this launch
+  uint64_t *retry;        // per workgroup: up to S records (key + W words)
+  uint32_t *retry_n;
+  int64_t *pend;          // per workgroup: inputs of the stopped tile not applied
+  uint32_t *pend_n;
+  uint32_t *pend_part;
+  unsigned int *stops;
+  SpillFmt F;
+};
+
+__device__ __forceinline__ void part_range(const uint64_t *base, uint32_t nb, uint64_t n,
+                                           uint32_t p, uint32_t np, int64_t &a, int64_t &b) {
+  if (!base || n == 0) {
+    a = b = 0;
+    return;
+  }
+  a = (int64_t)base[(uint64_t)p * nb];
+  b = p + 1 < np ? (int64_t)base[(uint64_t)(p + 1) * nb] : (int64_t)n;
+}
+
+// Key slot in the global table, or -1 (relaxed device-scope loads: keys are
+// inserted by other workgroups while this one probes).
+__device__ __forceinline__ int64_t table_find(const Table &T, uint64_t key) {
+  uint64_t s = slot_hash(T, key) >> (64 - T.log2cap);
+  while (true) {
+    uint64_t k = __hip_atomic_load(&T.keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == key) return (int64_t)s;
+    if (k == kEmpty) return -1;
+    s = (s + 1) & T.mask;
+  }
+}
+
+// Find-or-insert without counting (the caller accounts for new keys).
+__device__ __forceinline__ uint64_t table_insert(const Table &T, uint64_t key) {
+  uint64_t s = slot_hash(T, key) >> (64 - T.log2cap);
+  while (true) {
+    uint64_t k = __hip_atomic_load(&T.keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == key) return s;
+    if (k == kEmpty) {
+      unsigned long long prev = atomicCAS((unsigned long long *)&T.keys[s], kEmpty, key);
+      if (prev == kEmpty || prev == key) return s;
+    }
+    s = (s + 1) & T.mask;
+  }
+}
+
+// Insert into the merge kernel's LDS table, unbounded probing while fewer
+// than `limit` keys are held (a full table fails the insert).
+__device__ __forceinline__ int lds_insert(uint64_t *s_keys, uint32_t S, int *s_occ, int limit,
+                                          const Table &T, uint64_t key) {
+  uint32_t s = lds_home(T, key, S);
+  while (true) {
+    uint64_t k = s_keys[s];
+    if (k == key) return (int)s;
+    if (k == kEmpty) {
+      if (atomicAdd(s_occ, 1) >= limit) {
+        atomicSub(s_occ, 1);
+        // a key another lane is inserting may still land here: look once more
+        return s_keys[s] == key ? (int)s : -1;
+      }
+      unsigned long long prev = atomicCAS((unsigned long long *)&s_keys[s], kEmpty, key);
+      if (prev == kEmpty) return (int)s;
+      atomicSub(s_occ, 1);
+      if (prev == key) return (int)s;
+    }
+    s = s + 1 == S ? 0 : s + 1;
+  }
+}
+
+// Merges every LDS record into the global table and clears the LDS table.
+// Returns false when the partition's quota of new keys would be passed: the
+// records of new keys then go to the workgroup's retry area.
+__device__ bool merge_flush(const AggProg &P, const Table &T, const MergeDev &M, int S,
+                            uint64_t *s_keys, uint64_t *s_rec, int *s_occ, uint32_t *s_new,
+                            uint32_t *s_ins, int *s_fail) {
+  const int W = P.W, WL = W | 1;
+  if (threadIdx.x == 0) *s_new = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < S; t += blockDim.x) {
+    const uint64_t k = s_keys[t];
+    if (k == kEmpty) continue;
+    const int64_t g = table_find(T, k);
+    if (g < 0) {
+      atomicAdd(s_new, 1u);
+      continue;
+    }
+    uint64_t *l = s_rec + t * WL;
+    merge_plain(P, T.rec + (uint64_t)g * W, l);
+    s_keys[t] = kEmpty;
+    for (int w = 0; w < W; w++) l[w] = P.init[w];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *s_fail = *s_ins + *s_new > M.quota;
+    if (!*s_fail) *s_ins += *s_new;
+  }
+  __syncthreads();
+  const bool fail = *s_fail;
+  for (int t = threadIdx.x; t < S; t += blockDim.x) {
+    const uint64_t k = s_keys[t];
+    if (k == kEmpty) continue;
+    uint64_t *l = s_rec + t * WL;
+    if (!fail) {
+      const uint64_t g = table_insert(T, k);
+      merge_plain(P, T.rec + g * W, l);
+    } else {
+      const uint32_t j = atomicAdd(&M.retry_n[blockIdx.x], 1u);
+      uint64_t *d = M.retry + ((uint64_t)blockIdx.x * S + j) * (1 + W);
+      d[0] = k;
+      for (int w = 0; w < W; w++) d[1 + w] = l[w];
+    }
+    s_keys[t] = kEmpty;
+    for (int w = 0; w < W; w++) l[w] = P.init[w];
+  }
+  if (threadIdx.x == 0) *s_occ = 0;
+  __syncthreads();
+  return !fail;
+}
+
+template <int R>
+__global__ void __launch_bounds__(kMergeBlock) k_agg_merge(AggProg P, Table T, MergeDev M, int S) {
+  extern __shared__ uint64_t smem[];
+  uint64_t *s_keys = smem;
+  uint64_t *s_rec = smem + S;
+  __shared__ int s_occ, s_fail;
+  __shared__ uint32_t s_new, s_ins;
+  const int W = P.W, WL = W | 1;
+  const int limit = S - S / 8;
+  const uint32_t np = 1u << M.log2p;
+  for (int t = threadIdx.x; t < S; t += blockDim.x) s_keys[t] = kEmpty;
+  for (int t = threadIdx.x; t < S * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
+  if (threadIdx.x == 0) s_occ = 0;
+  __syncthreads();
+  for (uint32_t p = blockIdx.x; p < np; p += gridDim.x) {
+    const int64_t start = M.resume[p];
+    if (start < 0) continue;
+    int64_t r0, r1, c0, c1;
+    part_range(M.rbase, M.rnb, M.nrows, p, np, r0, r1);
+    part_range(M.cbase, M.cnb, M.nrecs, p, np, c0, c1);
+    const int64_t nrec = c1 - c0, total = nrec + (r1 - r0);
+    if (threadIdx.x == 0) s_ins = 0;
+    __syncthreads();
+    bool stopped = false;
+    for (int64_t t0 = start; t0 < total && !stopped; t0 += (int64_t)R * blockDim.x) {
+      int64_t u[R];
+      bool done[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        u[r] = t0 + (int64_t)r * blockDim.x + threadIdx.x;
+        done[r] = u[r] >= total;
+      }
+      while (true) {
+        int ls[R], lsw[R];
+        SpillSrc<R> src;
+        src.rows = M.rows;
+        src.F = &M.F;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          ls[r] = -1;
+          src.idx[r] = r0 + u[r] - nrec;
+          if (!done[r]) {
+            const uint64_t key = u[r] < nrec ? M.recs[(c0 + u[r]) * (1 + W)]
+                                             : M.rows[src.idx[r] * M.F.rw];
+            ls[r] = lds_insert(s_keys, (uint32_t)S, &s_occ, limit, T, key);
+          }
+          if (ls[r] >= 0 && u[r] < nrec)
+            merge_into_lds(P, s_rec + ls[r] * WL, M.recs + (c0 + u[r]) * (1 + W) + 1);
+          lsw[r] = (ls[r] >= 0 && u[r] >= nrec) ? ls[r] : -1;
+        }
+        tile_apply<R>(P, s_rec, WL, lsw, src);
+        bool left = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (ls[r] >= 0) done[r] = true;
+          left |= !done[r];
+        }
+        if (!__syncthreads_or(left)) break;
+        // the LDS table is full: merge it into the global table, then retry
+        if (!merge_flush(P, T, M, S, s_keys, s_rec, &s_occ, &s_new, &s_ins, &s_fail)) {
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            if (!done[r]) {
+              const uint32_t j = atomicAdd(&M.pend_n[blockIdx.x], 1u);
+              M.pend[(uint64_t)blockIdx.x * R * kMergeBlock + j] = u[r];
+            }
+          if (threadIdx.x == 0) {
+            M.pend_part[blockIdx.x] = p;
+            M.resume[p] = t0 + (int64_t)R * blockDim.x;
+            M.inserted[p] = s_ins;
+            atomicAdd(M.stops, 1u);
+          }
+          stopped = true;
+          break;
+        }
+      }
+    }
+    if (stopped) break;
+    const bool ok = merge_flush(P, T, M, S, s_keys, s_rec, &s_occ, &s_new, &s_ins, &s_fail);
+    if (threadIdx.x == 0) {
+      M.resume[p] = -1;
+      M.inserted[p] = s_ins;
+      if (!ok) {
+        M.pend_part[blockIdx.x] = p;
+        atomicAdd(M.stops, 1u);
+      }
+    }
+    if (!ok) break;  // the retry area is in use: this workgroup stops
+  }
+}
+
+// The stopped workgroups' leftovers, merged with atomics after the table has
+// grown: retry records, then the pending inputs of the stopped tile.
+__global__ void k_agg_retry(AggProg P, Table T, MergeDev M, int S, uint32_t nwg) {
+  const int W = P.W;
+  for (uint32_t g = blockIdx.x; g < nwg; g += gridDim.x) {
+    const uint32_t nr = M.retry_n[g], npd = M.pend_n[g];
+    for (uint32_t j = threadIdx.x; j < nr; j += blockDim.x) {
+      const uint64_t *d = M.retry + ((uint64_t)g * S + j) * (1 + W);
+      merge_record(P, T.rec + global_slot(T, d[0]) * (uint64_t)W, d + 1);
+    }
+    if (npd == 0) continue;
+    const uint32_t p = M.pend_part[g];
+    const uint32_t np = 1u << M.log2p;
+    int64_t r0, r1, c0, c1;
+    part_range(M.rbase, M.rnb, M.nrows, p, np, r0, r1);
+    part_range(M.cbase, M.cnb, M.nrecs, p, np, c0, c1);
+    const int64_t nrec = c1 - c0;
+    for (uint32_t j = threadIdx.x; j < npd; j += blockDim.x) {
+      const int64_t u = M.pend[(uint64_t)g * kTileRows * kMergeBlock + j];
+      if (u < nrec) {
+        const uint64_t *d = M.recs + (c0 + u) * (1 + W);
+        merge_record(P, T.rec + global_slot(T, d[0]) * (uint64_t)W, d + 1);
+      } else {
+        SpillSrc<1> src;
+        src.rows = M.rows;
+        src.F = &M.F;
+        src.idx[0] = r0 + u - nrec;
+        uint64_t *r = T.rec + global_slot(T, M.rows[src.idx[0] * M.F.rw]) * (uint64_t)W;
+        int ls[1] = {0};
+        tile_apply<1, false>(P, r, 0, ls, src);
+      }
+    }
   }
 }
 
@@ -232,15 +774,26 @@ __device__ __forceinline__ double i128_to_double(uint64_t lo, uint64_t hi) {
   return n ? -s : s;
 }
 
+// One lane per output row o (the compacted slot list[o]).  Validity words
+// are whole 32-bit words from wave ballots (no atomics): lanes past the end
+// redo the last row, whose stores repeat the same values.
+__device__ __forceinline__ void wave_bits(uint8_t *bm, int64_t obase, int64_t n, bool bit) {
+  uint64_t m = __ballot(bit);
+  const int lane = __lane_id();
+  if (lane == 0) reinterpret_cast<uint32_t *>(bm)[obase >> 5] = (uint32_t)m;
+  if (lane == 32 && obase + 32 < n) reinterpret_cast<uint32_t *>(bm)[(obase >> 5) + 1] = (uint32_t)(m >> 32);
+}
+
 __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, const uint64_t *rec,
-                             uint64_t nslots, const uint64_t *pos, void *out_key,
+                             const uint64_t *list, int64_t n, void *out_key,
                              uint8_t *out_key_valid, int key_width, unsigned int *err) {
-  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nslots;
-       s += (uint64_t)gridDim.x * blockDim.x) {
+  for (int64_t o0 = blockIdx.x * (int64_t)blockDim.x; o0 < n; o0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t obase = o0 + (threadIdx.x & ~63);
+    const bool active = o0 + threadIdx.x < n;
+    const int64_t o = active ? o0 + threadIdx.x : n - 1;
+    const uint64_t s = list[o];
     const uint64_t *r = rec + s * (uint64_t)O.W;
     uint64_t rows = r[0];
-    if (rows == 0 && !(O.global && s == 0)) continue;
-    int64_t o = (int64_t)pos[s];
     // key
     bool null_key = O.direct ? s == 256 : s == O.cap + 1;
     if (out_key) {
@@ -254,8 +807,8 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
         case 4: static_cast<uint32_t *>(out_key)[o] = (uint32_t)k; break;
         default: static_cast<uint64_t *>(out_key)[o] = k;
       }
-      if (!null_key && out_key_valid) set_bit(out_key_valid, o);
     }
+    if (out_key_valid) wave_bits(out_key_valid, obase, n, active && !null_key);
     for (int j = 0; j < O.n; j++) {
       const OutAgg &a = O.a[j];
       const uint64_t *w = r + a.word;
@@ -311,8 +864,8 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
             break;
           }
         }
-        if (ok) set_bit(a.validity, o);
-        if (a.validity2) set_bit(a.validity2, o);
+        wave_bits(a.validity, obase, n, active && ok);
+        if (a.validity2) wave_bits(a.validity2, obase, n, active);
         continue;
       }
       switch (a.kind) {
@@ -431,7 +984,7 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
           break;
         }
       }
-      if (ok) set_bit(a.validity, o);
+      wave_bits(a.validity, obase, n, active && ok);
     }
   }
 }
@@ -477,7 +1030,7 @@ __global__ void k_rehash(AggProg P, const uint64_t *okeys, const uint64_t *orec,
     } else {
       uint64_t k = okeys[s];
       if (k == kEmpty) continue;
-      d = global_slot(T, k);
+      d = table_insert(T, k);  // the host sets the occupancy count
     }
     for (int w = 0; w < P.W; w++) T.rec[d * P.W + w] = orec[s * P.W + w];
   }
@@ -488,6 +1041,13 @@ __global__ void k_group_flags(const uint64_t *rec, uint64_t nslots, int W, int g
   for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nslots;
        s += (uint64_t)gridDim.x * blockDim.x)
     flags[s] = (rec[s * (uint64_t)W] > 0 || (global && s == 0)) ? 1u : 0u;
+}
+
+__global__ void k_compact_slots(const uint32_t *flags, const uint64_t *pos, uint64_t ns,
+                                uint64_t *list) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < ns;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    if (flags[s]) list[pos[s]] = s;
 }
 
 static unsigned __int128 pow10_u128(int p) {
@@ -504,19 +1064,28 @@ struct kg_hash_agg {
   kg_ctx *ctx = nullptr;
   hipStream_t s = nullptr;
   bool has_key = false, direct = false, ansi = false, merge = false, partial = false;
+  bool hashed = false;  // ordinary keys: pass 1 + spill + drain
   kg_column key_type{};
   std::vector<kg_column> in_types;
   int nsink = 0;  // input columns per sink call (state columns when merging)
   AggProg prog{};
   OutProg out{};
-  int slots = 512;  // LDS records per workgroup
+  int slots = 512;  // LDS records per workgroup (one-byte and global aggregates)
   size_t lds_bytes = 0;
+  int p1_slots = 0, m_slots = 0;  // LDS tables of pass 1 and of the merge
+  size_t p1_lds = 0, m_lds = 0;
   // table
   DevBuf keys, rec, occ;
   uint64_t cap = 0;
   int log2cap = 0;
   uint64_t seed = 0x9E3779B97F4A7C15ULL;
   uint64_t occ_bound = 0;
+  uint64_t merged_keys = 0;  // keys the merge inserted (not in the device count)
+  // spill (hashed keys)
+  SpillFmt fmt{};
+  DevBuf sp_rows, sp_recs, sp_cnt;  // sp_cnt: rows, records, overflow flags
+  uint64_t cap_rows = 0, cap_recs = 0, bound_rows = 0, bound_recs = 0;
+  uint64_t spill_budget = 0;
   // ordered sums
   std::vector<CheckedSum> checked;
   uint64_t rows_seen = 0;
@@ -558,8 +1127,18 @@ static int log2_ceil(uint64_t v) {
   return b;
 }
 
+// Keys in the table: the device count (inserts made with global_slot) plus
+// the merge's inserts.
+static uint64_t read_occupied(kg_hash_agg *a) {
+  uint64_t v = 0;
+  KG_HIP(hipMemcpyAsync(&v, a->occ.get(), 8, hipMemcpyDeviceToHost, a->s));
+  KG_HIP(hipStreamSynchronize(a->s));
+  return v + a->merged_keys;
+}
+
 // Doubles the table (rehash of every key and record).
 static void grow(kg_hash_agg *a, uint64_t min_cap) {
+  uint64_t keys_now = read_occupied(a);
   uint64_t ncap = a->cap * 2;
   while (ncap < min_cap) ncap *= 2;
   DevBuf nkeys, nrec;
@@ -570,23 +1149,17 @@ static void grow(kg_hash_agg *a, uint64_t min_cap) {
   a->rec = std::move(nrec);
   a->cap = ncap;
   a->log2cap = log2_ceil(ncap);
-  KG_HIP(hipMemsetAsync(a->occ.get(), 0, 8, a->s));
   Table t = table_of(a);
   k_rehash<<<stream_grid(ocap + 2, 256), 256, 0, a->s>>>(a->prog, okeys.get<uint64_t>(),
                                                           orec.get<uint64_t>(), ocap, t);
   KG_HIP(hipGetLastError());
+  KG_HIP(hipMemcpyAsync(a->occ.get(), &keys_now, 8, hipMemcpyHostToDevice, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
-}
-
-static uint64_t read_occupied(kg_hash_agg *a) {
-  uint64_t v = 0;
-  KG_HIP(hipMemcpyAsync(&v, a->occ.get(), 8, hipMemcpyDeviceToHost, a->s));
-  KG_HIP(hipStreamSynchronize(a->s));
-  return v;
+  a->merged_keys = 0;
 }
 
 // Rows the next launch may take so that the table stays below 3/4 full
-// whatever keys they hold.
+// whatever keys they hold (launches that insert into the table directly).
 static int64_t room(kg_hash_agg *a, int64_t want) {
   if (a->direct) return want;
   uint64_t limit = a->cap / 4 * 3;
@@ -606,7 +1179,6 @@ static int64_t room(kg_hash_agg *a, int64_t want) {
 // Guarantees room for n new keys (the replay path, which does not split).
 static void reserve(kg_hash_agg *a, int64_t n) {
   if (a->direct) return;
-  if (a->occ_bound + (uint64_t)n <= a->cap / 4 * 3) return;
   uint64_t occ = read_occupied(a);
   a->occ_bound = occ;
   while (occ + (uint64_t)n > a->cap / 4 * 3) grow(a, a->cap * 2);
@@ -634,6 +1206,262 @@ static void launch_update(kg_hash_agg *a, const AggProg &P, int64_t lo, int64_t 
   }
   KG_HIP(hipGetLastError());
   a->occ_bound += (uint64_t)n;
+}
+
+// ---------------------------------------------------------------- spill
+static SpillDev spill_of(kg_hash_agg *a) {
+  SpillDev S{};
+  S.rows = a->sp_rows.get<uint64_t>();
+  S.recs = a->sp_recs.get<uint64_t>();
+  S.nrows = a->sp_cnt.get<unsigned long long>();
+  S.nrecs = S.nrows + 1;
+  S.overflow = reinterpret_cast<unsigned int *>(S.nrows + 2);
+  S.cap_rows = a->cap_rows;
+  S.cap_recs = a->cap_recs;
+  S.F = a->fmt;
+  return S;
+}
+
+static void read_spill_counts(kg_hash_agg *a, uint64_t &nrows, uint64_t &nrecs) {
+  uint64_t c[3];
+  KG_HIP(hipMemcpyAsync(c, a->sp_cnt.get(), 24, hipMemcpyDeviceToHost, a->s));
+  KG_HIP(hipStreamSynchronize(a->s));
+  if (c[2]) fail(KG_ERR_INTERNAL, "aggregation spill overflow");
+  nrows = c[0];
+  nrecs = c[1];
+}
+
+// A buffer of `cap` spill entries of w words keeping the first `used`.
+static void resize_spill(kg_hash_agg *a, DevBuf &buf, uint64_t &cap, uint64_t want, uint64_t used,
+                         int w) {
+  DevBuf nb(a->ctx, want * w * 8 + 8);
+  if (used)
+    KG_HIP(hipMemcpyAsync(nb.get(), buf.get(), used * w * 8, hipMemcpyDeviceToDevice, a->s));
+  KG_HIP(hipStreamSynchronize(a->s));
+  buf = std::move(nb);
+  cap = want;
+}
+
+static void drain(kg_hash_agg *a);
+
+// Room in both spills for a pass-1 launch of n rows from `blocks`
+// workgroups; drains when the spill budget would be passed.
+static void spill_room(kg_hash_agg *a, int64_t n, int64_t blocks) {
+  const uint64_t rw = a->fmt.rw, cw = 1 + a->prog.W;
+  // rows: every row may miss, and each wave may leave a piece partly unused;
+  // records: every slot at each eviction and at the end
+  const int64_t chunk = ceil_div(n, blocks);
+  const uint64_t new_recs = (uint64_t)blocks * a->p1_slots * (uint64_t)(chunk / kEvictRows + 2);
+  n += blocks * (kP1Block / 64) * kWaveChunk;
+  if (a->bound_rows + n <= a->cap_rows && a->bound_recs + new_recs <= a->cap_recs) return;
+  uint64_t nr, nc;
+  read_spill_counts(a, nr, nc);
+  a->bound_rows = nr;
+  a->bound_recs = nc;
+  uint64_t want_rows = nr + n, want_recs = nc + new_recs;
+  if ((want_rows * rw + want_recs * cw) * 8 > a->spill_budget && (nr || nc)) {
+    drain(a);
+    nr = nc = 0;
+    want_rows = n;
+    want_recs = new_recs;
+  }
+  if (want_rows > a->cap_rows) {
+    uint64_t c = std::max<uint64_t>(want_rows, a->cap_rows + a->cap_rows / 2);
+    if (c * rw * 8 > a->spill_budget) c = want_rows;
+    resize_spill(a, a->sp_rows, a->cap_rows, c, nr, (int)rw);
+  }
+  if (want_recs > a->cap_recs) {
+    uint64_t c = std::max<uint64_t>(want_recs, a->cap_recs + a->cap_recs / 2);
+    resize_spill(a, a->sp_recs, a->cap_recs, c, nc, (int)cw);
+  }
+}
+
+// The sink's column types, kept for reading spill rows back; the spill row
+// layout follows them (key, validity bits, one word per input, two for
+// decimal128).
+static void record_types(kg_hash_agg *a, const AggProg &P) {
+  if (a->fmt.rw == 0) {
+    int w = 2;
+    a->fmt.ncols = a->nsink;
+    for (int c = 0; c < a->nsink; c++) {
+      a->prog.in[c].type = P.in[c].type;
+      a->prog.in[c].precision = P.in[c].precision;
+      a->prog.in[c].scale = P.in[c].scale;
+      a->prog.in[c].width = P.in[c].width;
+      a->fmt.off[c] = (uint8_t)w;
+      w += P.in[c].type == KG_T_DECIMAL128 ? 2 : 1;
+    }
+    a->fmt.rw = w;
+    return;
+  }
+  for (int c = 0; c < a->nsink; c++)
+    if (a->prog.in[c].type != P.in[c].type) fail(KG_ERR_INTERNAL, "input column type changed");
+}
+
+static void pass1(kg_hash_agg *a, const AggProg &P, int64_t nrows) {
+  record_types(a, P);
+  const int64_t per_wg = (int64_t)kTileRows * kP1Block * 8;
+  const int64_t maxb = (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256) * 2;
+  // pieces whose row spill fits the budget
+  const int64_t piece_max =
+      std::max<int64_t>(1 << 20, (int64_t)(a->spill_budget / 2 / (8 * (uint64_t)a->fmt.rw)));
+  for (int64_t done = 0; done < nrows;) {
+    const int64_t n = std::min<int64_t>(nrows - done, piece_max);
+    int64_t blocks = std::min<int64_t>(ceil_div(n, per_wg), maxb);
+    if (blocks < 1) blocks = 1;
+    const int64_t chunk = ceil_div(n, blocks);
+    spill_room(a, n, blocks);
+    AggProg Q = P;
+    for (int c = 0; c < kAggInputs; c++) Q.in[c].offset += done;
+    Q.key.offset += done;
+    {
+      ProfScope prof("agg_update", a->s);
+      k_agg_pass1<kTileRows><<<(unsigned)blocks, kP1Block, a->p1_lds, a->s>>>(
+          Q, table_of(a), spill_of(a), n, chunk, a->p1_slots);
+      KG_HIP(hipGetLastError());
+    }
+    a->bound_rows += (uint64_t)n + (uint64_t)blocks * (kP1Block / 64) * kWaveChunk;
+    a->bound_recs += (uint64_t)blocks * a->p1_slots * (uint64_t)(chunk / kEvictRows + 2);
+    done += n;
+  }
+}
+
+// Partitions one spill stream (n entries of w words) by home-slot range into
+// `out`; base receives the scanned per-block histogram (partition p starts
+// at base[p * nb]).  Returns nb.
+static uint32_t partition_spill(kg_hash_agg *a, const uint64_t *in, uint64_t n, int w, int log2p,
+                                bool dead, DevBuf &out, DevBuf &base, uint64_t &live) {
+  const uint32_t np = 1u << log2p;
+  // blocks enough to fill the device, each with about 16 entries a partition
+  const int64_t fill = (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256) * 2;
+  int64_t per_block = std::max<int64_t>(4096, std::min<int64_t>((int64_t)np * 16,
+                                                                ceil_div((int64_t)n, fill)));
+  int64_t nb = ceil_div((int64_t)n, per_block);
+  if (nb < 1) nb = 1;
+  per_block = ceil_div((int64_t)n, nb);
+  if (per_block < 1) per_block = 1;
+  const uint64_t hn = (uint64_t)np * nb;
+  DevBuf hist(a->ctx, hn * 4 + 4);
+  base.reset(a->ctx, hn * 8 + 8);
+  out.reset(a->ctx, n * w * 8 + 8);
+  Table t = table_of(a);
+  ProfScope prof("agg_partition", a->s);
+  k_spill_hist<<<(unsigned)nb, 256, np * 4, a->s>>>(in, w, n, t, log2p, per_block, dead ? 1 : 0,
+                                                    hist.get<uint32_t>());
+  KG_HIP(hipGetLastError());
+  size_t tb = 0;
+  KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist.get<uint32_t>(), base.get<uint64_t>(),
+                                          (int)hn, a->s));
+  DevBuf temp(a->ctx, tb + 16);
+  KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, hist.get<uint32_t>(),
+                                          base.get<uint64_t>(), (int)hn, a->s));
+  k_spill_scatter<<<(unsigned)nb, 256, np * 4, a->s>>>(in, w, n, t, log2p, per_block,
+                                                       dead ? 1 : 0, base.get<uint64_t>(),
+                                                       out.get<uint64_t>());
+  KG_HIP(hipGetLastError());
+  // live entries: the scanned total (last base + last count)
+  uint64_t lb = 0;
+  uint32_t lc = 0;
+  KG_HIP(hipMemcpyAsync(&lb, base.get<uint64_t>() + hn - 1, 8, hipMemcpyDeviceToHost, a->s));
+  KG_HIP(hipMemcpyAsync(&lc, hist.get<uint32_t>() + hn - 1, 4, hipMemcpyDeviceToHost, a->s));
+  KG_HIP(hipStreamSynchronize(a->s));
+  live = lb + lc;
+  return (uint32_t)nb;
+}
+
+// Merges both spills into the table (see k_agg_merge) and empties them.
+static void drain(kg_hash_agg *a) {
+  if (!a->hashed) return;
+  uint64_t nr, nc;
+  read_spill_counts(a, nr, nc);
+  if (nr == 0 && nc == 0) return;
+  if (nr + nc >= (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "more than 2^32 spilled entries");
+  if (read_occupied(a) > a->cap / 2) grow(a, a->cap * 2);
+  const int W = a->prog.W;
+  // partitions of about 4096 inputs (their keys fit the merge's LDS table)
+  int log2p = 0;
+  while (log2p < 14 && (1ull << log2p) * 4096 < nr + nc && (1ull << (log2p + 1)) <= a->cap) log2p++;
+  const uint32_t np = 1u << log2p;
+  DevBuf prow, pbase, prec, cbase;
+  uint32_t rnb = 0, cnb = 0;
+  uint64_t nr_live = 0, nc_live = 0;
+  if (nr)
+    rnb = partition_spill(a, a->sp_rows.get<uint64_t>(), nr, a->fmt.rw, log2p, true, prow, pbase,
+                          nr_live);
+  if (nc)
+    cnb = partition_spill(a, a->sp_recs.get<uint64_t>(), nc, 1 + W, log2p, false, prec, cbase,
+                          nc_live);
+  nr = nr_live;
+  nc = nc_live;
+  // merge, resuming stopped partitions after growth
+  const int64_t grid = std::min<int64_t>(np, (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256) * 2);
+  DevBuf resume(a->ctx, (uint64_t)np * 8), inserted(a->ctx, (uint64_t)np * 4);
+  DevBuf retry(a->ctx, (uint64_t)grid * a->m_slots * (1 + W) * 8);
+  DevBuf pend(a->ctx, (uint64_t)grid * kTileRows * kMergeBlock * 8);
+  DevBuf ctr(a->ctx, (uint64_t)grid * 12 + 16);  // retry_n, pend_n, pend_part, stops
+  KG_HIP(hipMemsetAsync(resume.get(), 0, resume.bytes(), a->s));
+  AggProg P = a->prog;
+  P.skip_checked = 0;
+  MergeDev M{};
+  M.rows = prow.get<uint64_t>();
+  M.nrows = nr;
+  M.rbase = nr ? pbase.get<uint64_t>() : nullptr;
+  M.rnb = rnb;
+  M.recs = prec.get<uint64_t>();
+  M.nrecs = nc;
+  M.cbase = nc ? cbase.get<uint64_t>() : nullptr;
+  M.cnb = cnb;
+  M.log2p = log2p;
+  M.resume = resume.get<int64_t>();
+  M.inserted = inserted.get<uint32_t>();
+  M.retry = retry.get<uint64_t>();
+  M.retry_n = ctr.get<uint32_t>();
+  M.pend_n = M.retry_n + grid;
+  M.pend_part = M.pend_n + grid;
+  M.stops = M.pend_part + grid;
+  M.pend = pend.get<int64_t>();
+  M.F = a->fmt;
+  for (int round = 0;; round++) {
+    if (round > 40) fail(KG_ERR_INTERNAL, "aggregation merge made no progress");
+    const uint64_t keys_now = read_occupied(a);
+    const uint64_t limit = a->cap - a->cap / 8;
+    M.quota = keys_now >= limit ? 0 : (uint32_t)std::min<uint64_t>((limit - keys_now) / np, 0xFFFFFFFFu);
+    KG_HIP(hipMemsetAsync(inserted.get(), 0, inserted.bytes(), a->s));
+    KG_HIP(hipMemsetAsync(ctr.get(), 0, ctr.bytes(), a->s));
+    {
+      ProfScope prof("agg_merge", a->s);
+      k_agg_merge<kTileRows><<<(unsigned)grid, kMergeBlock, a->m_lds, a->s>>>(P, table_of(a), M,
+                                                                            a->m_slots);
+      KG_HIP(hipGetLastError());
+    }
+    // keys inserted, and workgroups that stopped
+    std::vector<uint32_t> ins(np);
+    uint32_t stops = 0;
+    KG_HIP(hipMemcpyAsync(ins.data(), inserted.get(), (size_t)np * 4, hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipMemcpyAsync(&stops, M.stops, 4, hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    for (uint32_t v : ins) a->merged_keys += v;
+    if (stops == 0) break;
+    // the table is too full for some partition: grow so that every leftover
+    // (retry records, pending inputs) fits as a new key below 3/4, merge
+    // them with atomics, resume
+    std::vector<uint32_t> left((size_t)grid * 2);
+    KG_HIP(hipMemcpyAsync(left.data(), M.retry_n, (size_t)grid * 8, hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    uint64_t nleft = 0;
+    for (uint32_t v : left) nleft += v;
+    const uint64_t need = read_occupied(a) + nleft;
+    uint64_t want = a->cap * 2;
+    while (want / 4 * 3 < need) want *= 2;
+    grow(a, want);
+    ProfScope prof("agg_merge", a->s);
+    k_agg_retry<<<(unsigned)grid, 256, 0, a->s>>>(P, table_of(a), M, a->m_slots, (uint32_t)grid);
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipStreamSynchronize(a->s));
+  }
+  KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 24, a->s));
+  KG_HIP(hipStreamSynchronize(a->s));
+  a->bound_rows = a->bound_recs = 0;
 }
 
 // Whether no running sum of the checked op can leave its bound during this
@@ -687,15 +1515,20 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
     if (!proven(a, c, P.in[c.col], nrows)) replay = true;
   if (!replay) {
     P.skip_checked = 0;
-    int64_t done = 0;
-    while (done < nrows) {
-      int64_t n = room(a, nrows - done);
-      launch_update(a, P, done, n);
-      done += n;
+    if (a->hashed) {
+      pass1(a, P, nrows);
+    } else {
+      int64_t done = 0;
+      while (done < nrows) {
+        int64_t n = room(a, nrows - done);
+        launch_update(a, P, done, n);
+        done += n;
+      }
     }
   } else {
     // rows straight to the global table (their record index kept), then the
-    // checked sums in row order
+    // checked sums in row order, after every spilled row is in the table
+    drain(a);
     P.skip_checked = 1;
     reserve(a, nrows);
     a->scratch_slots.ensure(a->ctx, (uint64_t)nrows * 8);
@@ -926,10 +1759,11 @@ void init_records(kg_ctx *ctx, uint64_t *rec, uint64_t nrec, const AggProg &P, h
   KG_HIP(hipStreamSynchronize(s));
 }
 
+// The slots holding a group, in slot order: list[0..n) (flags, an exclusive
+// scan, then each flagged slot writes itself at its position).
 int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t ns, int W, bool global,
-                        DevBuf &pos, hipStream_t s) {
-  DevBuf flags(ctx, ns * 4);
-  pos.reset(ctx, (ns + 1) * 8);
+                        DevBuf &list, hipStream_t s) {
+  DevBuf flags(ctx, ns * 4), pos(ctx, (ns + 1) * 8);
   k_group_flags<<<stream_grid(ns, 256), 256, 0, s>>>(rec, ns, W, global, flags.get<uint32_t>());
   KG_HIP(hipGetLastError());
   size_t tb = 0;
@@ -943,15 +1777,24 @@ int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t ns, int W, bo
   KG_HIP(hipMemcpyAsync(&last_pos, pos.get<uint64_t>() + ns - 1, 8, hipMemcpyDeviceToHost, s));
   KG_HIP(hipMemcpyAsync(&last_flag, flags.get<uint32_t>() + ns - 1, 4, hipMemcpyDeviceToHost, s));
   KG_HIP(hipStreamSynchronize(s));
-  return (int64_t)(last_pos + last_flag);
+  int64_t n = (int64_t)(last_pos + last_flag);
+  list.reset(ctx, (uint64_t)n * 8 + 8);
+  if (n > 0) {
+    k_compact_slots<<<stream_grid(ns, 256), 256, 0, s>>>(flags.get<uint32_t>(), pos.get<uint64_t>(),
+                                                         ns, list.get<uint64_t>());
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipStreamSynchronize(s));
+  }
+  return n;
 }
 
 void launch_agg_output(const OutProg &O, const uint64_t *keys, int key_stride,
-                       const uint64_t *rec, uint64_t nslots, const uint64_t *pos, void *out_key,
+                       const uint64_t *rec, const uint64_t *list, int64_t n, void *out_key,
                        uint8_t *out_key_valid, int key_width, unsigned int *err, hipStream_t s) {
+  if (n <= 0) return;
   ProfScope prof("agg_output", s);
-  k_agg_output<<<stream_grid(nslots, 256), 256, 0, s>>>(O, keys, key_stride, rec, nslots, pos,
-                                                         out_key, out_key_valid, key_width, err);
+  k_agg_output<<<stream_grid(n, 256), 256, 0, s>>>(O, keys, key_stride, rec, list, n, out_key,
+                                                    out_key_valid, key_width, err);
   KG_HIP(hipGetLastError());
 }
 
@@ -1011,8 +1854,35 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       }
     }
     if (a->slots == 0) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the LDS table");
+    a->hashed = a->has_key && !a->direct;
+    if (a->hashed) {
+      // LDS tables sized so that two workgroups share a CU (80 KiB each)
+      const size_t budget = 80 * 1024 - 64;
+      const int WL = W | 1;
+      a->p1_slots = (int)((budget / 8 - 2 * (size_t)WL) / (1 + (size_t)WL));
+      a->m_slots = (int)(budget / 8 / (1 + (size_t)WL));
+      if (a->p1_slots < 64 || a->m_slots < 64)
+        fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the LDS table");
+      a->p1_lds = ((size_t)a->p1_slots + (size_t)(a->p1_slots + 2) * WL) * 8;
+      a->m_lds = (size_t)a->m_slots * (1 + WL) * 8;
+      for (const void *fn : {(const void *)k_agg_pass1<kTileRows>, (const void *)k_agg_merge<kTileRows>}) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)std::max(a->p1_lds, a->m_lds));
+        if (e != hipSuccess) {
+          (void)hipGetLastError();
+          fail(KG_ERR_DEVICE, "dynamic LDS above 64 KiB refused");
+        }
+      }
+      a->sp_cnt.reset(ctx, 32);
+      KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 32, a->s));
+      uint64_t free_b = 0, total_b = 0;
+      KG_HIP(hipMemGetInfo(&free_b, &total_b));
+      a->spill_budget = std::min<uint64_t>(16ull << 30, free_b / 4);
+      if (ctx->budget) a->spill_budget = std::min<uint64_t>(a->spill_budget, ctx->budget / 4);
+      if (a->spill_budget < (64u << 20)) a->spill_budget = 64u << 20;
+    }
     uint64_t eg = expected_groups < 512 ? 512 : expected_groups;
-    a->cap = a->direct ? 256 : (1ull << log2_ceil(eg * 2));
+    a->cap = a->direct ? 256 : (1ull << log2_ceil(eg + eg / 4));
     a->log2cap = log2_ceil(a->cap);
     a->occ.reset(ctx, 8);
     KG_HIP(hipMemsetAsync(a->occ.get(), 0, 8, a->s));
@@ -1028,6 +1898,7 @@ int kg_agg_sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, i
 int kg_agg_finalize(kg_hash_agg *a) {
   return guarded([&] {
     if (a->finalized) return;
+    drain(a);
     a->ngroups = group_positions(a->ctx, a->rec.get<uint64_t>(), nslots(a), a->prog.W,
                                  !a->has_key, a->pos, a->s);
     a->finalized = true;
@@ -1110,10 +1981,8 @@ int kg_agg_result(kg_hash_agg *a, kg_batch **out) {
     }
     DevBuf err(a->ctx, 4);
     KG_HIP(hipMemsetAsync(err.get(), 0, 4, a->s));
-    uint64_t ns = nslots(a);
-    launch_agg_output(O, a->keys.get<uint64_t>(), 1, a->rec.get<uint64_t>(), ns,
-                      a->pos.get<uint64_t>(), key_vals, key_valid, kw, err.get<unsigned int>(),
-                      a->s);
+    launch_agg_output(O, a->keys.get<uint64_t>(), 1, a->rec.get<uint64_t>(), a->pos.get<uint64_t>(),
+                      n, key_vals, key_valid, kw, err.get<unsigned int>(), a->s);
     unsigned int e = 0;
     KG_HIP(hipMemcpyAsync(&e, err.get(), 4, hipMemcpyDeviceToHost, a->s));
     KG_HIP(hipStreamSynchronize(a->s));

@@ -132,23 +132,34 @@ __device__ __forceinline__ void add_f64(uint64_t *w, double v) {
     unsafeAtomicAdd((double *)w, v);
 }
 
+// Min/max skip the atomic when the stored value already wins: a stale read
+// is an older, weaker value, so the test never drops an update.
+template <bool LDS>
+__device__ __forceinline__ uint64_t peek(const uint64_t *w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED,
+                           LDS ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
+}
 template <bool LDS>
 __device__ __forceinline__ void min_i64(uint64_t *w, int64_t v) {
+  if (v >= (int64_t)peek<LDS>(w)) return;
   __hip_atomic_fetch_min((long long *)w, (long long)v, __ATOMIC_RELAXED,
                          LDS ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
 }
 template <bool LDS>
 __device__ __forceinline__ void max_i64(uint64_t *w, int64_t v) {
+  if (v <= (int64_t)peek<LDS>(w)) return;
   __hip_atomic_fetch_max((long long *)w, (long long)v, __ATOMIC_RELAXED,
                          LDS ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
 }
 template <bool LDS>
 __device__ __forceinline__ void min_u64(uint64_t *w, uint64_t v) {
+  if (v >= peek<LDS>(w)) return;
   __hip_atomic_fetch_min((unsigned long long *)w, (unsigned long long)v, __ATOMIC_RELAXED,
                          LDS ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
 }
 template <bool LDS>
 __device__ __forceinline__ void max_u64(uint64_t *w, uint64_t v) {
+  if (v <= peek<LDS>(w)) return;
   __hip_atomic_fetch_max((unsigned long long *)w, (unsigned long long)v, __ATOMIC_RELAXED,
                          LDS ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -653,14 +664,14 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
 // Fills nrec records with the program's initial state words.
 void init_records(kg_ctx *ctx, uint64_t *rec, uint64_t nrec, const AggProg &P, hipStream_t s);
 
-// Output positions of the occupied records (rows > 0, or slot 0 of a global
-// aggregate): pos[s] = exclusive prefix count.  Returns the group count.
+// The occupied records (rows > 0, or slot 0 of a global aggregate) in slot
+// order: list[0..n) holds their slot indexes.  Returns n, the group count.
 int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t nslots, int W, bool global,
-                        DevBuf &pos, hipStream_t s);
+                        DevBuf &list, hipStream_t s);
 
-// Writes the final values of every occupied record (kernel k_agg_output).
+// Writes the final values of the n records named by list (kernel k_agg_output).
 void launch_agg_output(const OutProg &O, const uint64_t *keys, int key_stride,
-                       const uint64_t *rec, uint64_t nslots, const uint64_t *pos, void *out_key,
+                       const uint64_t *rec, const uint64_t *list, int64_t n, void *out_key,
                        uint8_t *out_key_valid, int key_width, unsigned int *err, hipStream_t s);
 
 }  // namespace kg

@@ -1,0 +1,307 @@
+// kg_aggtile.hpp -- op-major application of a tile of rows to LDS records,
+// and the row format of the aggregation spill.
+//
+// A lane holds R rows of a tile.  The op program is walked once per tile
+// (each op's uniform dispatch is paid once for R rows), and a column's
+// values are loaded once per tile with one type dispatch.  The rows come
+// either from the input columns (ColSrc) or from spill rows (SpillSrc).
+//
+// Spill row (64-bit words): [0] key, [1] validity of input column c in bit c,
+// then each input column's value at SpillFmt::off[c]: one word (integers
+// sign-extended, floats as float64 bits) or two (decimal128 lo, hi).
+// Spilled record: [0] key, then the W state words.
+#pragma once
+
+#include "kg_aggops.hpp"
+
+namespace kg {
+
+struct SpillFmt {
+  int rw;     // words per spill row
+  int ncols;  // input columns carried
+  uint8_t off[kAggInputs];
+};
+
+// Loads of input column c for the R rows of a lane (rows with act[r] false
+// are skipped; their ok[r] is false).
+template <int R>
+struct ColSrc {
+  int64_t idx[R];
+  __device__ __forceinline__ void load(const AggProg &P, int c, const bool (&act)[R],
+                                       uint64_t (&lo)[R], uint64_t (&hi)[R], double (&f)[R],
+                                       bool (&ok)[R]) const {
+    const DCol &d = P.in[c];
+    const int t = d.type;
+    const int64_t off = d.offset;
+    if (t == KG_T_FLOAT64) {
+      const double *v = static_cast<const double *>(d.values);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r]) f[r] = v[idx[r] + off];
+    } else if (t == KG_T_FLOAT32) {
+      const float *v = static_cast<const float *>(d.values);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r]) f[r] = (double)v[idx[r] + off];
+    } else if (t == KG_T_INT64 || t == KG_T_DECIMAL64 || t == KG_T_TIMESTAMP) {
+      const int64_t *v = static_cast<const int64_t *>(d.values);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r]) {
+          int64_t x = v[idx[r] + off];
+          lo[r] = (uint64_t)x;
+          hi[r] = (uint64_t)(x >> 63);
+        }
+    } else if (t == KG_T_INT32 || t == KG_T_DATE32) {
+      const int32_t *v = static_cast<const int32_t *>(d.values);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r]) {
+          int64_t x = v[idx[r] + off];
+          lo[r] = (uint64_t)x;
+          hi[r] = (uint64_t)(x >> 63);
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r]) {
+          int64_t h;
+          dev::load_i128(d, idx[r], lo[r], h);
+          hi[r] = (uint64_t)h;
+          if (dev::is_f(t)) f[r] = dev::load_f64(d, idx[r]);
+        }
+    }
+    if (d.validity) {
+#pragma unroll
+      for (int r = 0; r < R; r++) ok[r] = act[r] && dev::valid(d, idx[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) ok[r] = act[r];
+    }
+  }
+};
+
+// Loads from spill rows: rows[idx[r] * F.rw ...].
+template <int R>
+struct SpillSrc {
+  const uint64_t *rows;
+  int64_t idx[R];
+  const SpillFmt *F;
+  __device__ __forceinline__ void load(const AggProg &P, int c, const bool (&act)[R],
+                                       uint64_t (&lo)[R], uint64_t (&hi)[R], double (&f)[R],
+                                       bool (&ok)[R]) const {
+    const int t = P.in[c].type;
+    const int rw = F->rw, o = F->off[c];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      ok[r] = false;
+      if (!act[r]) continue;
+      const uint64_t *row = rows + idx[r] * (int64_t)rw;
+      ok[r] = (row[1] >> c) & 1;
+      uint64_t v = row[o];
+      if (dev::is_f(t)) {
+        f[r] = __longlong_as_double((long long)v);
+      } else {
+        lo[r] = v;
+        hi[r] = t == KG_T_DECIMAL128 ? row[o + 1]
+                : t == KG_T_UINT64   ? 0
+                                     : (uint64_t)((int64_t)v >> 63);
+      }
+    }
+  }
+};
+
+// Applies the op program to records srec + ls[r] * WL for the lane's rows
+// with ls[r] >= 0 (LDS records, or with LDS false global records).
+template <int R, bool LDS = true, class Src>
+__device__ __forceinline__ void tile_apply(const AggProg &P, uint64_t *srec, int WL,
+                                           const int (&ls)[R], const Src &src) {
+  bool act[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) act[r] = ls[r] >= 0;
+  uint64_t lo[R], hi[R];
+  double f[R];
+  bool ok[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    lo[r] = hi[r] = 0;
+    f[r] = 0.0;
+    ok[r] = false;
+  }
+  int last = -1;
+  for (int o = 0; o < P.nops; o++) {
+    const AggOp op = P.ops[o];
+    if (op.kind == OP_COUNT_STAR) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r]) add_u64_nr<LDS>(srec + ls[r] * WL, 1);
+      continue;
+    }
+    if (op.col != last) {
+      last = op.col;
+      src.load(P, op.col, act, lo, hi, f, ok);
+    }
+    if (op.kind == OP_NULLS) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r] && !ok[r]) add_u64_nr<LDS>(srec + ls[r] * WL + op.word, 1);
+      continue;
+    }
+    if (op.kind == OP_STICKY) {
+      uint64_t lo2[R], hi2[R];
+      double f2[R];
+      bool ok2[R];
+      src.load(P, op.col2, act, lo2, hi2, f2, ok2);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r] && !ok[r] && ok2[r] && lo2[r] != 0)
+          max_u64<LDS>(srec + ls[r] * WL + op.st_word, 2);
+      continue;
+    }
+    if (op.checked && P.skip_checked) continue;
+    switch (op.kind) {
+      case OP_ADDV:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r] && lo[r]) add_u64_nr<LDS>(srec + ls[r] * WL + op.word, lo[r]);
+        break;
+      case OP_COUNT_TRUE:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r] && lo[r]) add_u64_nr<LDS>(srec + ls[r] * WL + op.word, 1);
+        break;
+      case OP_ADD128:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) add128<LDS>(srec + ls[r] * WL + op.word, lo[r], hi[r]);
+        break;
+      case OP_ADD192:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r])
+            add192<LDS>(srec + ls[r] * WL + op.word, lo[r], hi[r],
+                         (uint64_t)((int64_t)hi[r] >> 63));
+        break;
+      case OP_ADDF64:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) add_f64<LDS>(srec + ls[r] * WL + op.word, f[r]);
+        break;
+      case OP_MIN_I64:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) min_i64<LDS>(srec + ls[r] * WL + op.word, (int64_t)lo[r]);
+        break;
+      case OP_MAX_I64:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) max_i64<LDS>(srec + ls[r] * WL + op.word, (int64_t)lo[r]);
+        break;
+      case OP_MIN_F:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) min_u64<LDS>(srec + ls[r] * WL + op.word, f64_order(f[r]));
+        break;
+      case OP_MAX_F:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) max_u64<LDS>(srec + ls[r] * WL + op.word, f64_order(f[r]));
+        break;
+      default: break;
+    }
+  }
+}
+
+// Merges a state record (W words) into an LDS record (LDS atomics).
+__device__ __forceinline__ void merge_into_lds(const AggProg &P, uint64_t *d, const uint64_t *s) {
+  for (int o = 0; o < P.nops; o++) {
+    const AggOp op = P.ops[o];
+    const uint64_t *a = s + op.word;
+    uint64_t *b = d + op.word;
+    switch (op.kind) {
+      case OP_COUNT_STAR:
+      case OP_NULLS:
+      case OP_ADDV:
+      case OP_COUNT_TRUE:
+        if (a[0]) add_u64_nr<true>(b, a[0]);
+        break;
+      case OP_STICKY:
+        if (s[op.st_word] == 2) max_u64<true>(d + op.st_word, 2);
+        break;
+      case OP_ADD128:
+        if (!(op.checked && P.skip_checked)) add128<true>(b, a[0], a[1]);
+        break;
+      case OP_ADD192:
+        if (!(op.checked && P.skip_checked)) add192<true>(b, a[0], a[1], a[2]);
+        break;
+      case OP_ADDF64: add_f64<true>(b, __longlong_as_double((long long)a[0])); break;
+      case OP_MIN_I64:
+        if (a[0] != P.init[op.word]) min_i64<true>(b, (int64_t)a[0]);
+        break;
+      case OP_MAX_I64:
+        if (a[0] != P.init[op.word]) max_i64<true>(b, (int64_t)a[0]);
+        break;
+      case OP_MIN_F:
+        if (a[0] != P.init[op.word]) min_u64<true>(b, a[0]);
+        break;
+      case OP_MAX_F:
+        if (a[0] != P.init[op.word]) max_u64<true>(b, a[0]);
+        break;
+      default: break;
+    }
+  }
+}
+
+// Merges record s into record d with plain loads and stores: d belongs to
+// the calling lane alone (its key's owner).
+__device__ __forceinline__ void merge_plain(const AggProg &P, uint64_t *d, const uint64_t *s) {
+  for (int o = 0; o < P.nops; o++) {
+    const AggOp op = P.ops[o];
+    const uint64_t *a = s + op.word;
+    uint64_t *b = d + op.word;
+    switch (op.kind) {
+      case OP_COUNT_STAR:
+      case OP_NULLS:
+      case OP_ADDV:
+      case OP_COUNT_TRUE: b[0] += a[0]; break;
+      case OP_STICKY:
+        if (s[op.st_word] == 2) d[op.st_word] = 2;
+        break;
+      case OP_ADD128:
+        if (!(op.checked && P.skip_checked)) {
+          uint64_t l = b[0] + a[0];
+          b[1] += a[1] + (l < b[0] ? 1 : 0);
+          b[0] = l;
+        }
+        break;
+      case OP_ADD192:
+        if (!(op.checked && P.skip_checked)) {
+          unsigned __int128 x = (unsigned __int128)b[0] + a[0];
+          unsigned __int128 y = (unsigned __int128)b[1] + a[1] + (uint64_t)(x >> 64);
+          b[0] = (uint64_t)x;
+          b[1] = (uint64_t)y;
+          b[2] += a[2] + (uint64_t)(y >> 64);
+        }
+        break;
+      case OP_ADDF64:
+        b[0] = (uint64_t)__double_as_longlong(__longlong_as_double((long long)b[0]) +
+                                              __longlong_as_double((long long)a[0]));
+        break;
+      case OP_MIN_I64:
+        if ((int64_t)a[0] < (int64_t)b[0]) b[0] = a[0];
+        break;
+      case OP_MAX_I64:
+        if ((int64_t)a[0] > (int64_t)b[0]) b[0] = a[0];
+        break;
+      case OP_MIN_F:
+        if (a[0] < b[0]) b[0] = a[0];
+        break;
+      case OP_MAX_F:
+        if (a[0] > b[0]) b[0] = a[0];
+        break;
+      default: break;
+    }
+  }
+}
+
+}  // namespace kg

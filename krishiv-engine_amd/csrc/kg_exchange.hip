@@ -188,6 +188,14 @@ static int ceil_log2(uint32_t v) {
 
 using namespace kg;
 
+namespace kg {
+// out[p] = in[p * stride] (each partition's first-block scan value)
+__global__ void k_stride_gather(const uint64_t *in, uint64_t stride, uint32_t n, uint64_t *out) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) out[p] = in[(uint64_t)p * stride];
+}
+}  // namespace kg
+
 extern "C" {
 
 int kg_hash_columns(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
@@ -337,9 +345,11 @@ int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nr
     }
     // partition offsets: the scanned value at each partition's first block
     std::vector<uint64_t> firsts(partitions);
-    for (uint32_t p = 0; p < partitions; p++)
-      KG_HIP(hipMemcpyAsync(&firsts[p], base.get<uint64_t>() + (uint64_t)p * nb, 8,
-                            hipMemcpyDeviceToHost, s));
+    DevBuf fd(ctx, (uint64_t)partitions * 8);
+    k_stride_gather<<<(unsigned)ceil_div(partitions, 256), 256, 0, s>>>(base.get<uint64_t>(), (uint64_t)nb,
+                                                              partitions, fd.get<uint64_t>());
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipMemcpyAsync(firsts.data(), fd.get(), (size_t)partitions * 8, hipMemcpyDeviceToHost, s));
     KG_HIP(hipStreamSynchronize(s));
     for (uint32_t p = 0; p < partitions; p++) out_offsets[p] = (int64_t)firsts[p];
     out_offsets[partitions] = nrows;

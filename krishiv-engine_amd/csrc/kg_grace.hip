@@ -173,6 +173,43 @@ static void free_piece(kg_grace::Piece &p) {
   p.hk = p.hv = p.hp = p.hpv = nullptr;
 }
 
+// Moves a partitioned piece from HBM to pinned host memory.
+static void spill_piece(kg_grace *g, kg_grace::Piece &piece) {
+  if (!piece.dev) return;
+  kg_column dk{}, dp{};
+  ck(kg_batch_column(piece.dev, 0, &dk));
+  ck(kg_batch_column(piece.dev, 1, &dp));
+  int64_t n = piece.rows;
+  size_t vb = (size_t)ceil_div(n, 32) * 4;
+  KG_HIP(hipHostMalloc((void **)&piece.hk, (size_t)n * 8, hipHostMallocDefault));
+  KG_HIP(hipHostMalloc((void **)&piece.hp, (size_t)n * 8, hipHostMallocDefault));
+  KG_HIP(hipMemcpyAsync(piece.hk, dk.values, (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
+  KG_HIP(hipMemcpyAsync(piece.hp, dp.values, (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
+  if (dk.validity) {
+    KG_HIP(hipHostMalloc((void **)&piece.hv, vb, hipHostMallocDefault));
+    KG_HIP(hipMemcpyAsync(piece.hv, dk.validity, vb, hipMemcpyDeviceToHost, g->s));
+  }
+  if (dp.validity) {
+    KG_HIP(hipHostMalloc((void **)&piece.hpv, vb, hipHostMallocDefault));
+    KG_HIP(hipMemcpyAsync(piece.hpv, dp.validity, vb, hipMemcpyDeviceToHost, g->s));
+  }
+  KG_HIP(hipStreamSynchronize(g->s));
+  kg_batch_release(piece.dev);
+  piece.dev = nullptr;
+  g->spilled += (uint64_t)n * 16 + (dk.validity ? vb : 0) + (dp.validity ? vb : 0);
+}
+
+// HBM a sub-batch of `rows` needs while it is partitioned: the staged key and
+// payload, their partitioned copy, partition ids and the block histograms.
+static uint64_t sink_bytes(int64_t rows) { return (uint64_t)rows * 40 + (4u << 20); }
+
+static bool fits(kg_grace *g, uint64_t need) {
+  return !g->ctx->budget || g->ctx->in_use.load() + need + g->table_budget <= g->ctx->budget;
+}
+
+// Past the HBM budget (with one partition's tables kept free), partitioned
+// pieces move to pinned host memory, oldest first; a batch is partitioned in
+// sub-batches that fit beside what stays in HBM.
 static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg_column *key,
                       const kg_column *pay, int64_t n, int side) {
   if (n <= 0) return;
@@ -181,48 +218,47 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
     fail(KG_ERR_UNSUPPORTED, "grace join of 64-bit integer keys and payloads");
   check_column(*key, n, "grace key");
   check_column(*pay, n, "grace payload");
-  // payload range for the factorization check
-  {
-    Staged st(g->ctx, g->s);
-    DCol pc = st.add(*pay, n);
-    k_minmax_i64<<<stream_grid(n, 256), 256, 0, g->s>>>(pc, n, g->mm.get<long long>() + 2 * side);
-    KG_HIP(hipGetLastError());
-    KG_HIP(hipStreamSynchronize(g->s));
-  }
-  kg_column cols[2] = {*key, *pay};
-  int32_t kc = 0;
-  kg_grace::Piece piece;
-  piece.off.assign(g->parts + 1, 0);
-  ck(kg_repartition(g->ctx, 2, cols, n, 1, &kc, g->parts, piece.off.data(), &piece.dev));
-  piece.rows = n;
-  kg_column dk{}, dp{};
-  ck(kg_batch_column(piece.dev, 0, &dk));
-  ck(kg_batch_column(piece.dev, 1, &dp));
-  piece.key_nullable = dk.validity != nullptr;
-  piece.pay_nullable = dp.validity != nullptr;
-  // past the HBM budget (with room for one partition's tables kept free),
-  // the partitioned piece moves to pinned host memory
-  uint64_t bytes = (uint64_t)n * 16 + (uint64_t)ceil_div(n, 8) * 2;
-  if (g->ctx->budget && g->ctx->in_use.load() + g->table_budget > g->ctx->budget) {
-    size_t vb = (size_t)ceil_div(n, 32) * 4;
-    KG_HIP(hipHostMalloc((void **)&piece.hk, (size_t)n * 8, hipHostMallocDefault));
-    KG_HIP(hipHostMalloc((void **)&piece.hp, (size_t)n * 8, hipHostMallocDefault));
-    KG_HIP(hipMemcpyAsync(piece.hk, dk.values, (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
-    KG_HIP(hipMemcpyAsync(piece.hp, dp.values, (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
-    if (dk.validity) {
-      KG_HIP(hipHostMalloc((void **)&piece.hv, vb, hipHostMallocDefault));
-      KG_HIP(hipMemcpyAsync(piece.hv, dk.validity, vb, hipMemcpyDeviceToHost, g->s));
+  for (int64_t done = 0; done < n;) {
+    int64_t m = n - done;
+    if (g->ctx->budget) {
+      for (auto *v : {&g->build, &g->probe})
+        for (auto &pc : *v)
+          if (!fits(g, sink_bytes(m < (1 << 16) ? m : (1 << 16)))) spill_piece(g, pc);
+      uint64_t used = g->ctx->in_use.load() + g->table_budget + (4u << 20);
+      int64_t room = used < g->ctx->budget ? (int64_t)((g->ctx->budget - used) / 40) : 0;
+      if (room < 1024 && room < m)
+        fail(KG_ERR_MEMORY, "HBM budget too small to partition a grace batch");
+      if (m > room) m = room;
     }
-    if (dp.validity) {
-      KG_HIP(hipHostMalloc((void **)&piece.hpv, vb, hipHostMallocDefault));
-      KG_HIP(hipMemcpyAsync(piece.hpv, dp.validity, vb, hipMemcpyDeviceToHost, g->s));
+    kg_column k = *key, v = *pay;
+    k.offset += done;
+    v.offset += done;
+    k.length -= done;
+    v.length -= done;
+    // payload range for the factorization check
+    {
+      Staged st(g->ctx, g->s);
+      DCol pc = st.add(v, m);
+      k_minmax_i64<<<stream_grid(m, 256), 256, 0, g->s>>>(pc, m,
+                                                          g->mm.get<long long>() + 2 * side);
+      KG_HIP(hipGetLastError());
+      KG_HIP(hipStreamSynchronize(g->s));
     }
-    KG_HIP(hipStreamSynchronize(g->s));
-    kg_batch_release(piece.dev);
-    piece.dev = nullptr;
-    g->spilled += bytes;
+    kg_column cols[2] = {k, v};
+    int32_t kc = 0;
+    kg_grace::Piece piece;
+    piece.off.assign(g->parts + 1, 0);
+    ck(kg_repartition(g->ctx, 2, cols, m, 1, &kc, g->parts, piece.off.data(), &piece.dev));
+    piece.rows = m;
+    kg_column dk{}, dp{};
+    ck(kg_batch_column(piece.dev, 0, &dk));
+    ck(kg_batch_column(piece.dev, 1, &dp));
+    piece.key_nullable = dk.validity != nullptr;
+    piece.pay_nullable = dp.validity != nullptr;
+    if (!fits(g, 0)) spill_piece(g, piece);
+    store.push_back(std::move(piece));
+    done += m;
   }
-  store.push_back(std::move(piece));
 }
 
 // Partition p's rows of a piece as key and payload columns (device or host).

@@ -704,7 +704,7 @@ int kg_joinagg_result(kg_hash_join *j, kg_batch **out) {
     DevBuf err(j->ctx, 4);
     KG_HIP(hipMemsetAsync(err.get(), 0, 4, j->s));
     launch_agg_output(O, &j->slots.get<Slot>()->key, (int)(sizeof(Slot) / 8),
-                      j->rec.get<uint64_t>(), ns, pos.get<uint64_t>(), kc.values.get(),
+                      j->rec.get<uint64_t>(), pos.get<uint64_t>(), n, kc.values.get(),
                       kc.validity.get<uint8_t>(), kw, err.get<unsigned int>(), j->s);
     unsigned int e = 0;
     KG_HIP(hipMemcpyAsync(&e, err.get(), 4, hipMemcpyDeviceToHost, j->s));

@@ -421,52 +421,55 @@ int kg_server_create(kg_ctx *ctx, uint64_t arena_bytes, int32_t max_batch, int32
 
 int kg_server_add_table(kg_server *S, const int64_t *keys, int64_t n, int32_t *table_id) {
   return guarded([&] {
+    if (n < 0) fail(KG_ERR_INTERNAL, "negative row count");
     kg_server::Resident t;
     uint64_t want = (uint64_t)(n < 256 ? 256 : n);
     t.cap = 1ull << slog2(want * 2);
     t.log2cap = slog2(t.cap);
     t.slots.reset(S->ctx, (t.cap + 1) * 16);
-    // one-request descriptor on slot 0's stream: init + build
-    kg_server::Slot &sl = S->slots[0];
-    if (sl.busy) {
-      KG_HIP(hipEventSynchronize(sl.done));
-    }
-    uint8_t *h = sl.desc_host;
-    uint8_t *dbase = sl.desc_dev.get<uint8_t>();
-    SDesc *HD = reinterpret_cast<SDesc *>(h);
-    SReq *HR = reinterpret_cast<SReq *>(h + sizeof(SDesc));
-    STile *HT = reinterpret_cast<STile *>(h + sizeof(SDesc) + sizeof(SReq) * S->max_batch);
-    *HD = SDesc{};
-    HD->nreq = 1;
-    HD->reqs = reinterpret_cast<SReq *>(dbase + sizeof(SDesc));
-    for (int ph = 0; ph < 4; ph++)
-      HD->tiles[ph] = reinterpret_cast<STile *>(dbase + sizeof(SDesc) + sizeof(SReq) * S->max_batch +
-                                                sizeof(STile) * sl.max_tiles * ph);
     DevBuf scratch(S->ctx, (t.cap + 1) * 24);
-    SReq s{};
-    s.bkeys = keys;
-    s.nb = n;
-    s.slots = t.slots.get<SSlot>();
-    s.cap = t.cap;
-    s.log2cap = t.log2cap;
-    s.own = 1;
-    s.rec = scratch.get<uint64_t>();
-    HR[0] = s;
-    int nt[4] = {0, 0, 0, 0};
+    // a one-request descriptor of its own (the stream slots' descriptors
+    // may still hold a batch's group counts): init + build phases
+    std::vector<STile> tiles[2];
     for (int64_t lo = 0; lo < (int64_t)t.cap + 1; lo += kSrvTile)
-      HT[nt[0]++] = STile{0, 0, lo, lo + kSrvTile < (int64_t)t.cap + 1 ? lo + kSrvTile : (int64_t)t.cap + 1};
+      tiles[0].push_back(STile{0, 0, lo, lo + kSrvTile < (int64_t)t.cap + 1 ? lo + kSrvTile
+                                                                          : (int64_t)t.cap + 1});
     for (int64_t lo = 0; lo < n; lo += kSrvTile)
-      HT[sl.max_tiles + nt[1]++] = STile{0, 0, lo, lo + kSrvTile < n ? lo + kSrvTile : n};
-    if ((uint64_t)nt[0] > sl.max_tiles || (uint64_t)nt[1] > sl.max_tiles)
-      fail(KG_ERR_UNSUPPORTED, "resident table too large for one batch descriptor");
-    HD->ntiles[0] = nt[0];
-    HD->ntiles[1] = nt[1];
-    KG_HIP(hipMemcpyAsync(dbase, h, sl.desc_bytes, hipMemcpyHostToDevice, sl.s));
+      tiles[1].push_back(STile{0, 0, lo, lo + kSrvTile < n ? lo + kSrvTile : n});
+    uint64_t bytes = sizeof(SDesc) + sizeof(SReq) +
+                     sizeof(STile) * (tiles[0].size() + tiles[1].size());
+    std::vector<uint8_t> h(bytes);
+    DevBuf d(S->ctx, bytes);
+    uint8_t *dbase = d.get<uint8_t>();
+    SDesc D{};
+    D.nreq = 1;
+    D.reqs = reinterpret_cast<SReq *>(dbase + sizeof(SDesc));
+    D.tiles[0] = reinterpret_cast<STile *>(dbase + sizeof(SDesc) + sizeof(SReq));
+    D.tiles[1] = D.tiles[0] + tiles[0].size();
+    D.tiles[2] = D.tiles[3] = D.tiles[1];
+    D.ntiles[0] = (int32_t)tiles[0].size();
+    D.ntiles[1] = (int32_t)tiles[1].size();
+    SReq r{};
+    r.bkeys = keys;
+    r.nb = n;
+    r.slots = t.slots.get<SSlot>();
+    r.cap = t.cap;
+    r.log2cap = t.log2cap;
+    r.own = 1;
+    r.rec = scratch.get<uint64_t>();
+    memcpy(h.data(), &D, sizeof(SDesc));
+    memcpy(h.data() + sizeof(SDesc), &r, sizeof(SReq));
+    memcpy(h.data() + sizeof(SDesc) + sizeof(SReq), tiles[0].data(), sizeof(STile) * tiles[0].size());
+    if (!tiles[1].empty())
+      memcpy(h.data() + sizeof(SDesc) + sizeof(SReq) + sizeof(STile) * tiles[0].size(),
+             tiles[1].data(), sizeof(STile) * tiles[1].size());
+    hipStream_t st = S->slots[0].s;
+    KG_HIP(hipMemcpyAsync(dbase, h.data(), bytes, hipMemcpyHostToDevice, st));
     unsigned grid = (unsigned)((S->ctx->num_cus > 0 ? S->ctx->num_cus : 256) * 4);
-    k_srv_init<<<grid, kSrvBlock, 0, sl.s>>>(desc_ptr(sl));
-    k_srv_build<<<grid, kSrvBlock, 0, sl.s>>>(desc_ptr(sl));
+    k_srv_init<<<grid, kSrvBlock, 0, st>>>(d.get<SDesc>());
+    k_srv_build<<<grid, kSrvBlock, 0, st>>>(d.get<SDesc>());
     KG_HIP(hipGetLastError());
-    KG_HIP(hipStreamSynchronize(sl.s));
+    KG_HIP(hipStreamSynchronize(st));
     *table_id = (int32_t)S->tables.size();
     S->tables.push_back(std::move(t));
   });

@@ -363,3 +363,22 @@ def test_two_phase_equals_single_phase():
           for a, b in zip(cuts[:-1], cuts[1:])]
     r = _two_phase(kp, vp, [(T_MAP[t[0]], t[1], t[2]) for t in types], aggs, False)
     compare(ref, r, aggs)
+
+
+def test_merge_resumes_after_table_growth():
+    """200K distinct keys into a table sized for 16 groups: the spill merge
+    runs out of table room, stops, the table grows, the leftover records and
+    pending rows merge with atomics and the stopped partitions resume."""
+    say("agg: merge quota stops and resumes")
+    rng = np.random.default_rng(11)
+    n = 600_000
+    key = (rng.integers(0, 200_000, n) * 7919 - (1 << 40)).astype(np.int64)
+    v = rng.integers(-(1 << 50), 1 << 50, n)
+    f = rng.random(n)
+    ok = rng.random(n) > 0.05
+    types = [(kgo.T_INT64, 0, 0), (kgo.T_FLOAT64, 0, 0)]
+    aggs = [(kgo.AGG_COUNT_STAR, 0), (kgo.AGG_SUM, 0), (kgo.AGG_MIN, 0), (kgo.AGG_MAX, 1),
+            (kgo.AGG_AVG, 1), (kgo.AGG_COUNT, 0)]
+    ref, b, _ = run_both(key, None, [(v, ok), (f, None)], types, aggs, batches=4,
+                         expected_groups=16)
+    compare(ref, b, aggs)
