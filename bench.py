@@ -39,7 +39,25 @@ def parse():
     p.add_argument("--key", choices=["int64", "int8"], default="int64")
     p.add_argument("--cpu-rows", type=int, default=20_000_000)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--workload", choices=["agg", "joinagg"], default="agg",
+                   help="agg: the hash aggregation (default); joinagg: the join of "
+                        "SELECT key, SUM(amt), COUNT(*) ... GROUP BY key over 1M build rows")
+    p.add_argument("--probe-rows", type=int, default=200_000_000)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a 1 GiB device-to-device copy (to scale FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
+
+
+def copy_calibration():
+    """A plain 1 GiB device copy: under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    its counters give the scale of those counters for known traffic."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 1 << 30}), flush=True)
 
 
 def cpu_baseline(args, aggs):
@@ -81,11 +99,83 @@ def cpu_baseline(args, aggs):
                       f"threads (per-thread tables merged at the end); {dt:.2f} s"}
 
 
+def joinagg(args):
+    """SELECT key, SUM(amt), COUNT(*) FROM build JOIN probe ON key = fk GROUP
+    BY key: build 1M rows (key a seeded permutation of 0..1M-1), probe rows
+    with fk uniform in [0, 1.2M) (1 % NULL) and amt in [1, 1e4], generated in
+    HBM; a step builds the table, probes every row through the join fused
+    with the aggregate and materializes the groups."""
+    import kgpu
+    from kgpu import Col, DeviceBuffer
+    from kgpu import abi as A
+    ctx = kgpu.Context(0)
+    nb, n = 1_000_000, args.probe_rows
+    L = kgpu.lib()
+    bk = DeviceBuffer(ctx, nb * 8)
+    bv = DeviceBuffer(ctx, nb * 8)
+    A.check(L.kg_gen_join_build(ctx.h, 1, 0, nb, nb, bk.ptr, bv.ptr))
+    fk = DeviceBuffer(ctx, n * 8)
+    fkv = DeviceBuffer(ctx, (n + 7) // 8 + 8)
+    amt = DeviceBuffer(ctx, n * 8)
+    A.check(L.kg_gen_join_probe(ctx.h, 2, 0, n, 1_200_000, 10, fk.ptr, fkv.ptr, amt.ptr))
+    bcol = Col(A.T_INT64, bk, length=nb)
+    pcols = [Col(A.T_INT64, fk, fkv, length=n), Col(A.T_INT64, amt, length=n)]
+    groups = [0]
+
+    def step():
+        j = kgpu.HashJoin(ctx, A.JOIN_INNER, [0], [0], expected_build_rows=nb)
+        j.build_sink([bcol], nb)
+        j.build_finalize()
+        j.aggregate([(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)])
+        j.probe_aggregate(pcols, [1], n)
+        r = j.aggregate_result()
+        groups[0] = r.num_rows
+        del r
+        j.close()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    kgpu.prof_reset()
+    kgpu.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kgpu.prof_enable(False)
+    pr_ms, pr_n = kgpu.prof_get("joinagg_probe")
+    in_bytes = n * 16 + n // 8  # fk, amt, fk validity
+    avg = pr_ms / max(pr_n, 1)
+    per_launch = in_bytes / max(pr_n / args.steps, 1)
+    achieved = per_launch / (avg / 1e3) if pr_ms > 0 else None
+    print(json.dumps({
+        "metric": "join probe rows/s (fused GROUP BY build key; 1M build rows)",
+        "value": n * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded counter-based generator, generated in HBM)",
+        "config": {"workload": f"SELECT key, SUM(amt), COUNT(*) join-aggregate, 1000000 build rows, "
+                               f"{n} probe rows, fk uniform [0, 1.2M), 1% NULL", "groups": groups[0]},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+                     "kernel": "k_joinagg_probe", "kernel_avg_ms": avg,
+                     "launches_per_step": pr_n / args.steps},
+    }), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "joinagg":
+        joinagg(args)
+        return
     import kgpu
     from kgpu import Col, DeviceBuffer
     from kgpu import abi as A

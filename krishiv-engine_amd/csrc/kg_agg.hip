@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
 // pending inputs for the host, which grows the table, merges them with
 // atomics (k_agg_retry) and resumes the partition.
 constexpr int kP1Block = 512;
-constexpr int kMergeBlock = 256;
+constexpr int kMergeBlock = 512;
 constexpr int kTileRows = 4;
 constexpr int kWaveChunk = 512;
 constexpr int64_t kEvictRows = 32768;
@@ -295,7 +295,7 @@ __device__ __forceinline__ void spill_lds_records(const AggProg &P, const SpillD
 }
 
 template <int R>
-__global__ void __launch_bounds__(kP1Block) k_agg_pass1(AggProg P, Table T, SpillDev S,
+__global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, SpillDev S,
                                                         int64_t nrows, int64_t chunk, int SLOTS) {
   extern __shared__ uint64_t smem[];
   uint64_t *s_keys = smem;
@@ -792,7 +792,14 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
     const bool active = o0 + threadIdx.x < n;
     const int64_t o = active ? o0 + threadIdx.x : n - 1;
     const uint64_t s = list[o];
-    const uint64_t *r = rec + s * (uint64_t)O.W;
+    // the record into LDS with back-to-back loads (its lines are read once;
+    // odd stride: lanes' records fall in different banks)
+    extern __shared__ uint64_t s_out_rec[];
+    uint64_t *r = s_out_rec + threadIdx.x * (uint64_t)(O.W | 1);
+    {
+      const uint64_t *g = rec + s * (uint64_t)O.W;
+      for (int w = 0; w < O.W; w++) r[w] = g[w];
+    }
     uint64_t rows = r[0];
     // key
     bool null_key = O.direct ? s == 256 : s == O.cap + 1;
@@ -1394,7 +1401,7 @@ static void drain(kg_hash_agg *a) {
   nr = nr_live;
   nc = nc_live;
   // merge, resuming stopped partitions after growth
-  const int64_t grid = std::min<int64_t>(np, (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256) * 2);
+  const int64_t grid = std::min<int64_t>(np, (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256));
   DevBuf resume(a->ctx, (uint64_t)np * 8), inserted(a->ctx, (uint64_t)np * 4);
   DevBuf retry(a->ctx, (uint64_t)grid * a->m_slots * (1 + W) * 8);
   DevBuf pend(a->ctx, (uint64_t)grid * kTileRows * kMergeBlock * 8);
@@ -1793,8 +1800,10 @@ void launch_agg_output(const OutProg &O, const uint64_t *keys, int key_stride,
                        uint8_t *out_key_valid, int key_width, unsigned int *err, hipStream_t s) {
   if (n <= 0) return;
   ProfScope prof("agg_output", s);
-  k_agg_output<<<stream_grid(n, 256), 256, 0, s>>>(O, keys, key_stride, rec, list, n, out_key,
-                                                    out_key_valid, key_width, err);
+  const int block = O.W > 31 ? 128 : 256;
+  const size_t lds = (size_t)block * (O.W | 1) * 8;
+  k_agg_output<<<stream_grid(n, block), block, lds, s>>>(O, keys, key_stride, rec, list, n,
+                                                         out_key, out_key_valid, key_width, err);
   KG_HIP(hipGetLastError());
 }
 
@@ -1860,24 +1869,32 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       const size_t budget = 80 * 1024 - 64;
       const int WL = W | 1;
       a->p1_slots = (int)((budget / 8 - 2 * (size_t)WL) / (1 + (size_t)WL));
-      a->m_slots = (int)(budget / 8 / (1 + (size_t)WL));
+      // the merge: one workgroup a CU with as much LDS as a workgroup may take
+      int optin = 0;
+      if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device) !=
+              hipSuccess || optin < 64 * 1024) {
+        (void)hipGetLastError();
+        optin = 64 * 1024;
+      }
+      a->m_slots = (int)(((size_t)std::min(optin, 160 * 1024) - 1024) / 8 / (1 + (size_t)WL));
       if (a->p1_slots < 64 || a->m_slots < 64)
         fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the LDS table");
       a->p1_lds = ((size_t)a->p1_slots + (size_t)(a->p1_slots + 2) * WL) * 8;
       a->m_lds = (size_t)a->m_slots * (1 + WL) * 8;
-      for (const void *fn : {(const void *)k_agg_pass1<kTileRows>, (const void *)k_agg_merge<kTileRows>}) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)std::max(a->p1_lds, a->m_lds));
+      const std::pair<const void *, size_t> fns[2] = {
+          {(const void *)k_agg_pass1<kTileRows>, a->p1_lds},
+          {(const void *)k_agg_merge<kTileRows>, a->m_lds}};
+      for (auto &f : fns) {
+        hipError_t e = hipFuncSetAttribute(f.first, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)f.second);
         if (e != hipSuccess) {
           (void)hipGetLastError();
-          fail(KG_ERR_DEVICE, "dynamic LDS above 64 KiB refused");
+          fail(KG_ERR_DEVICE, "dynamic LDS of " + std::to_string(f.second) + " bytes refused");
         }
       }
       a->sp_cnt.reset(ctx, 32);
       KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 32, a->s));
-      uint64_t free_b = 0, total_b = 0;
-      KG_HIP(hipMemGetInfo(&free_b, &total_b));
-      a->spill_budget = std::min<uint64_t>(16ull << 30, free_b / 4);
+      a->spill_budget = std::min<uint64_t>(16ull << 30, ctx->total_hbm / 16);
       if (ctx->budget) a->spill_budget = std::min<uint64_t>(a->spill_budget, ctx->budget / 4);
       if (a->spill_budget < (64u << 20)) a->spill_budget = 64u << 20;
     }

@@ -9,7 +9,9 @@
 #include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -67,6 +69,11 @@ struct kg_ctx {
   std::atomic<uint64_t> in_use{0};
   hipStream_t stream = nullptr;  // for context-level calls
   int num_cus = 0;
+  uint64_t total_hbm = 0;
+  // freed device blocks kept for reuse, by rounded size
+  std::mutex cache_mu;
+  std::multimap<uint64_t, void *> cache;
+  uint64_t cached = 0;
 };
 
 namespace kg {

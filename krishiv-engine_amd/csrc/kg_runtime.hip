@@ -26,29 +26,79 @@ void set_last_error(int code, const std::string &msg) {
   g_last_error = std::string(class_of(code)) + ": " + msg;
 }
 
+// Device memory with a per-context cache of freed blocks.  A block is
+// cached after a device synchronize (what hipFree would do), so no kernel
+// still uses it when it is handed out again; sizes are rounded (powers of
+// two up to 2 MiB, then multiples of 2 MiB) so blocks are reused across
+// operators of the same shapes.  The HBM budget counts live bytes.
+static uint64_t round_alloc(uint64_t b) {
+  if (b <= (2u << 20)) {
+    uint64_t r = 256;
+    while (r < b) r <<= 1;
+    return r;
+  }
+  return (b + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+}
+
+static void cache_release_all(kg_ctx *ctx) {
+  std::lock_guard<std::mutex> g(ctx->cache_mu);
+  for (auto &kv : ctx->cache) (void)hipFree(kv.second);
+  ctx->cache.clear();
+  ctx->cached = 0;
+}
+
 void *dev_alloc(kg_ctx *ctx, uint64_t bytes) {
   if (!ctx) fail(KG_ERR_INTERNAL, "no context");
-  uint64_t prev = ctx->in_use.fetch_add(bytes);
-  if (ctx->budget && prev + bytes > ctx->budget) {
-    ctx->in_use.fetch_sub(bytes);
+  const uint64_t rb = round_alloc(bytes);
+  uint64_t prev = ctx->in_use.fetch_add(rb);
+  if (ctx->budget && prev + rb > ctx->budget) {
+    ctx->in_use.fetch_sub(rb);
     fail(KG_ERR_MEMORY, "HBM budget of " + std::to_string(ctx->budget) + " bytes exceeded (" +
-                            std::to_string(prev) + " held, " + std::to_string(bytes) +
+                            std::to_string(prev) + " held, " + std::to_string(rb) +
                             " requested)");
   }
+  {
+    std::lock_guard<std::mutex> g(ctx->cache_mu);
+    auto it = ctx->cache.find(rb);
+    if (it != ctx->cache.end()) {
+      void *p = it->second;
+      ctx->cache.erase(it);
+      ctx->cached -= rb;
+      return p;
+    }
+  }
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
+  hipError_t e = hipMalloc(&p, rb);
   if (e != hipSuccess) {
-    ctx->in_use.fetch_sub(bytes);
     (void)hipGetLastError();
-    fail(KG_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    cache_release_all(ctx);
+    e = hipMalloc(&p, rb);
+  }
+  if (e != hipSuccess) {
+    ctx->in_use.fetch_sub(rb);
+    (void)hipGetLastError();
+    fail(KG_ERR_MEMORY, "hipMalloc of " + std::to_string(rb) + " bytes: " + hipGetErrorString(e));
   }
   return p;
 }
 
 void dev_free(kg_ctx *ctx, void *p, uint64_t bytes) {
   if (!p) return;
-  (void)hipFree(p);
-  if (ctx) ctx->in_use.fetch_sub(bytes);
+  if (!ctx) {
+    (void)hipFree(p);
+    return;
+  }
+  const uint64_t rb = round_alloc(bytes);
+  ctx->in_use.fetch_sub(rb);
+  (void)hipDeviceSynchronize();
+  std::lock_guard<std::mutex> g(ctx->cache_mu);
+  // keep at most a quarter of the device's memory cached
+  if (ctx->cached + rb > ctx->total_hbm / 4) {
+    (void)hipFree(p);
+    return;
+  }
+  ctx->cache.emplace(rb, p);
+  ctx->cached += rb;
 }
 
 int type_width(int t) {
@@ -186,6 +236,7 @@ int kg_ctx_create(int device, uint64_t hbm_budget_bytes, kg_ctx **out) {
     size_t free_b = 0, total_b = 0;
     KG_HIP(hipMemGetInfo(&free_b, &total_b));
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
+    c->total_hbm = total_b;
     hipDeviceProp_t prop;
     KG_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
@@ -196,6 +247,8 @@ int kg_ctx_create(int device, uint64_t hbm_budget_bytes, kg_ctx **out) {
 
 void kg_ctx_destroy(kg_ctx *ctx) {
   if (!ctx) return;
+  (void)hipDeviceSynchronize();
+  cache_release_all(ctx);
   destroy_stream(ctx->stream);
   delete ctx;
 }
