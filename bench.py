@@ -39,9 +39,14 @@ def parse():
     p.add_argument("--key", choices=["int64", "int8"], default="int64")
     p.add_argument("--cpu-rows", type=int, default=20_000_000)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workload", choices=["agg", "joinagg"], default="agg",
+    p.add_argument("--workload", choices=["agg", "joinagg", "bloom", "serve"], default="agg",
                    help="agg: the hash aggregation (default); joinagg: the join of "
-                        "SELECT key, SUM(amt), COUNT(*) ... GROUP BY key over 1M build rows")
+                        "SELECT key, SUM(amt), COUNT(*) ... GROUP BY key over 1M build rows; "
+                        "bloom: the runtime Bloom filter feeding a join (15M build, 600M probe); "
+                        "serve: the batched/served join-aggregate mix against the plain form")
+    p.add_argument("--requests", type=int, default=10_000)
+    p.add_argument("--qps", type=float, default=400.0)
+    p.add_argument("--chunk", type=int, default=48)
     p.add_argument("--probe-rows", type=int, default=200_000_000)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device-to-device copy (to scale FETCH_SIZE/WRITE_SIZE)")
@@ -165,6 +170,214 @@ def joinagg(args):
     }), flush=True)
 
 
+def bloom(args):
+    """Runtime Bloom filter feeding a join: build 15M rows (key unique, val
+    uniform in [0, 1e6)); the predicate val < 50000 keeps 5 % (the same
+    selectivity as attr < 5 over attr uniform in [0, 100)); a split-block
+    Bloom filter of the kept keys sized for 750K items at fpp 0.03; 600M probe
+    fk uniform over the 15M key range pre-screened by it; the survivors
+    joined (selection-vector probe) and SUM(amt), COUNT(*) aggregated."""
+    import kgpu
+    from kgpu import Col, DeviceBuffer
+    from kgpu import abi as A
+    ctx = kgpu.Context(0)
+    nb, n = 15_000_000, args.probe_rows if args.probe_rows != 200_000_000 else 600_000_000
+    L = kgpu.lib()
+    bk = DeviceBuffer(ctx, nb * 8)
+    bv = DeviceBuffer(ctx, nb * 8)
+    A.check(L.kg_gen_join_build(ctx.h, 4, 0, nb, nb, bk.ptr, bv.ptr))
+    fk = DeviceBuffer(ctx, n * 8)
+    fkv = DeviceBuffer(ctx, (n + 7) // 8 + 8)
+    amt = DeviceBuffer(ctx, n * 8)
+    A.check(L.kg_gen_join_probe(ctx.h, 5, 0, n, nb, 0, fk.ptr, fkv.ptr, amt.ptr))
+    nbytes = kgpu.RuntimeBloom.bytes_for(750_000, 0.03)
+    surv = DeviceBuffer(ctx, n * 4 + 64)
+    stats = {}
+
+    def step():
+        rows, kept = kgpu.select_range(ctx, Col(A.T_INT64, bv, length=nb), 0, 49_999)
+        keys = kgpu.gather(ctx, [Col(A.T_INT64, bk, length=nb)], rows, kept)
+        b = kgpu.RuntimeBloom(ctx, nbytes)
+        b.insert(keys.as_col(0), nrows=kept)
+        j = kgpu.HashJoin(ctx, A.JOIN_INNER, [0], [0], expected_build_rows=kept)
+        j.build_sink([keys.as_col(0)], kept)
+        j.build_finalize()
+        pc = [Col(A.T_INT64, fk, length=n), Col(A.T_INT64, amt, length=n)]
+        _, ns = b.probe(pc[0], out_rows=surv)
+        out = j.probe_selected(pc, surv, ns, n)
+        g = kgpu.HashAggregate(ctx, None, [(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)])
+        g.sink(None, [out.as_col(1)], out.num_rows)
+        g.finalize()
+        r = g.result()
+        stats.update(kept=kept, survivors=ns, joined=out.num_rows, sum=int(r.column(0)[0][0]))
+        del r, out
+        g.close()
+        j.close()
+        b.close()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    kgpu.prof_reset()
+    kgpu.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kgpu.prof_enable(False)
+    fams = {f: kgpu.prof_get(f) for f in ("bloom_probe", "bloom_compact", "bloom_insert",
+                                          "join_probe", "join_gather", "join_build",
+                                          "agg_update", "select_range", "gather")}
+    bp_ms, bp_n = fams["bloom_probe"]
+    bc_ms, _ = fams["bloom_compact"]
+    bp_ms += bc_ms
+    avg = bp_ms / max(bp_n, 1)
+    per_launch = n * 8 / max(bp_n / args.steps, 1)  # the probe key column
+    achieved = per_launch / (avg / 1e3) if bp_ms > 0 else None
+    print(json.dumps({
+        "metric": "Bloom-screened probe rows/s (600M probe, 15M build, fpp 0.03)",
+        "value": n * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded counter-based generator, generated in HBM)",
+        "config": {"workload": f"Bloom filter feeding a join, {nb} build rows, {n} probe rows",
+                   **stats},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+                     "kernel": "bloom probe (k_bloom_probe_bits + expand)", "kernel_avg_ms": avg},
+        "families_ms_per_step": {f: v[0] / args.steps for f, v in fams.items()},
+    }), flush=True)
+
+
+def serve(args):
+    """The batched/served mix: requests arrive as Poisson at --qps (seed 8);
+    each is SELECT key, SUM(pay), COUNT(*) FROM build JOIN probe ON key = fk
+    GROUP BY key with a build of log-uniform [1K, 10M] unique int64 keys and
+    a probe 10x as long (fk uniform over 1.1x the key range, int64 payload);
+    30 % probe one of 8 resident builds.  Inputs are generated in HBM per
+    chunk of --chunk requests before the chunk's arrivals start (generation
+    is timed apart and excluded from both forms).  The plain form runs one
+    join-aggregate launch sequence per request in arrival order on one
+    stream; the batched form submits each request to the server as it
+    arrives.  Latency = completion - arrival."""
+    import kgpu
+    from kgpu import Col, DeviceBuffer
+    from kgpu import abi as A
+    ctx = kgpu.Context(0)
+    L = kgpu.lib()
+    rng = np.random.default_rng(8)
+    R = args.requests
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e7), R)).astype(np.int64)
+    resident = rng.random(R) < 0.3
+    table_of = rng.integers(0, 8, R)
+    res_sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e7), 8)).astype(np.int64)
+    gaps = rng.exponential(1.0 / args.qps, R)
+    # resident builds (keys 0..n-1 permuted), built once for both forms
+    res_keys = []
+    for t in range(8):
+        k = DeviceBuffer(ctx, int(res_sizes[t]) * 8)
+        A.check(L.kg_gen_join_build(ctx.h, 1000 + t, 0, int(res_sizes[t]), int(res_sizes[t]), k.ptr, None))
+        res_keys.append(k)
+    plain_tables = []
+    for t in range(8):
+        j = kgpu.HashJoin(ctx, A.JOIN_INNER, [0], [0], expected_build_rows=int(res_sizes[t]))
+        j.build_sink([Col(A.T_INT64, res_keys[t], length=int(res_sizes[t]))], int(res_sizes[t]))
+        j.build_finalize()
+        plain_tables.append(j)
+    srv = kgpu.JoinAggServer(ctx, arena_bytes=64 << 30, max_batch=256, use_graph=True)
+    srv_tables = [srv.add_table(res_keys[t], int(res_sizes[t])) for t in range(8)]
+
+    def nb_of(r):
+        return int(res_sizes[table_of[r]]) if resident[r] else int(sizes[r])
+
+    def gen_chunk(lo, hi):
+        bufs = {}
+        for r in range(lo, hi):
+            nb = nb_of(r)
+            npr = 10 * nb
+            bk = None
+            if not resident[r]:
+                bk = DeviceBuffer(ctx, nb * 8)
+                A.check(L.kg_gen_join_build(ctx.h, 2 * r + 1, 0, nb, nb, bk.ptr, None))
+            fk = DeviceBuffer(ctx, npr * 8)
+            pay = DeviceBuffer(ctx, npr * 8)
+            A.check(L.kg_gen_join_probe(ctx.h, 2 * r + 2, 0, npr, int(nb * 1.1), 0, fk.ptr, None, pay.ptr))
+            bufs[r] = (bk, fk, pay, nb, npr)
+        ctx.synchronize()
+        return bufs
+
+    def run_form(batched):
+        lat = np.zeros(R)
+        gen_s = 0.0
+        busy_s = 0.0
+        for lo in range(0, R, args.chunk):
+            hi = min(R, lo + args.chunk)
+            t0 = time.perf_counter()
+            bufs = gen_chunk(lo, hi)
+            gen_s += time.perf_counter() - t0
+            arrive = np.cumsum(gaps[lo:hi]) - gaps[lo]
+            start = time.perf_counter()
+            if not batched:
+                for r in range(lo, hi):
+                    wait = arrive[r - lo] - (time.perf_counter() - start)
+                    if wait > 0:
+                        time.sleep(wait)
+                    bk, fk, pay, nb, npr = bufs[r]
+                    if resident[r]:
+                        j = plain_tables[table_of[r]]
+                    else:
+                        j = kgpu.HashJoin(ctx, A.JOIN_INNER, [0], [0], expected_build_rows=nb)
+                        j.build_sink([Col(A.T_INT64, bk, length=nb)], nb)
+                        j.build_finalize()
+                    j.aggregate([(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)])
+                    j.probe_aggregate([Col(A.T_INT64, fk, length=npr), Col(A.T_INT64, pay, length=npr)], [1], npr)
+                    res = j.aggregate_result()
+                    lat[r] = time.perf_counter() - start - arrive[r - lo]
+                    del res
+                    if not resident[r]:
+                        j.close()
+            else:
+                nxt, done = lo, 0
+                while done < hi - lo:
+                    now = time.perf_counter() - start
+                    while nxt < hi and arrive[nxt - lo] <= now:
+                        bk, fk, pay, nb, npr = bufs[nxt]
+                        if resident[nxt]:
+                            srv.submit(nxt, fk, pay, npr, table=srv_tables[table_of[nxt]])
+                        else:
+                            srv.submit(nxt, fk, pay, npr, build_keys=bk, build_rows=nb)
+                        nxt += 1
+                    ids = srv.progress(wait=nxt >= hi)
+                    t = time.perf_counter() - start
+                    for r in ids:
+                        lat[r] = t - arrive[r - lo]
+                        srv.result(r)
+                        done += 1
+            busy_s += time.perf_counter() - start
+            del bufs
+        return lat, busy_s, gen_s
+
+    out = {}
+    for name, batched in (("plain", False), ("batched", True)):
+        lat, busy, gen = run_form(batched)
+        out[name] = {"queries_per_s": R / busy, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3), "wall_s_excl_gen": busy,
+                     "gen_s": gen}
+    print(json.dumps({
+        "metric": "batched vs plain join-aggregate queries/s and p99 latency",
+        "value": out["batched"]["queries_per_s"], "unit": "queries/s", "n_gpus": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded counter-based generator, generated in HBM per chunk)",
+        "config": {"workload": f"{R} join+aggregate requests, Poisson {args.qps} q/s (seed 8), build "
+                               "log-uniform [1K, 10M], probe 10x, 30 % on 8 resident builds, "
+                               "arena 64 GB", "chunk": args.chunk},
+        "plain": out["plain"], "batched": out["batched"],
+        "batched_over_plain_qps": out["batched"]["queries_per_s"] / out["plain"]["queries_per_s"],
+    }), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -175,6 +388,12 @@ def main():
         sys.exit(2)
     if args.workload == "joinagg":
         joinagg(args)
+        return
+    if args.workload == "bloom":
+        bloom(args)
+        return
+    if args.workload == "serve":
+        serve(args)
         return
     import kgpu
     from kgpu import Col, DeviceBuffer

@@ -120,6 +120,16 @@ int kg_hash_columns(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t n
  * physical/mod.rs:248-262): partition_of(H_kg(keys), partitions) per row. */
 int kg_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrows,
                  uint32_t partitions, uint32_t *out_pids, uint32_t out_flags);
+/* The join's row-layout hash h of each row's key columns (RowLayout::encode's
+ * stored hash, crates/krishiv-exec/src/rows/layout.rs:487-553,787-797,871-904)
+ * and its grace partition (Hashing::partition, crates/krishiv-exec/src/join/
+ * keys.rs:118-128): depth 0 takes the top radix_bits (clamped to 3..6) of h,
+ * depth d >= 1 the 3 bits at 48 - 3 min(d, 6) of g = finalize(h, seed).
+ * out_hashes (may be NULL) and out_parts (one byte a row): device or host
+ * (out_flags & KG_COL_DEVICE). */
+int kg_join_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrows,
+                      uint32_t radix_bits, uint32_t depth, uint64_t seed, uint64_t *out_hashes,
+                      uint8_t *out_parts, uint32_t out_flags);
 /* ExchangeSink::sink's route + split (exchange/mod.rs:447-501): the rows of
  * `cols` regrouped by partition, stable within a partition.  The output batch
  * holds the permuted columns; out_offsets (host, partitions+1 entries) gives

@@ -377,9 +377,19 @@ __global__ void k_spill_hist(const uint64_t *rows, int w, uint64_t n, Table T, i
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * per_block;
   const int64_t hi = lo + per_block < (int64_t)n ? lo + per_block : (int64_t)n;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    if (dead && (rows[i * w + 1] & kDeadRow)) continue;
-    atomicAdd(&s_h[spill_part(T, rows[i * w], log2p)], 1u);
+  // four rows a thread: their loads are in flight together
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
+    uint64_t key[4];
+    bool live[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int64_t i = i0 + r * (int64_t)blockDim.x;
+      live[r] = i < hi && !(dead && (rows[i * w + 1] & kDeadRow));
+      key[r] = i < hi ? rows[i * w] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (live[r]) atomicAdd(&s_h[spill_part(T, key[r], log2p)], 1u);
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_h[p];
@@ -395,12 +405,27 @@ __global__ void k_spill_scatter(const uint64_t *rows, int w, uint64_t n, Table T
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * per_block;
   const int64_t hi = lo + per_block < (int64_t)n ? lo + per_block : (int64_t)n;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const uint64_t *src = rows + i * w;
-    if (dead && (src[1] & kDeadRow)) continue;
-    const uint32_t pos = atomicAdd(&s_c[spill_part(T, src[0], log2p)], 1u);
-    uint64_t *dst = out + (uint64_t)pos * w;
-    for (int k = 0; k < w; k++) dst[k] = src[k];
+  // four rows a thread: their loads and cursor atomics are in flight together
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
+    uint64_t key[4];
+    bool live[4];
+    uint32_t pos[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int64_t i = i0 + r * (int64_t)blockDim.x;
+      live[r] = i < hi && !(dead && (rows[i * w + 1] & kDeadRow));
+      key[r] = i < hi ? rows[i * w] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      pos[r] = live[r] ? atomicAdd(&s_c[spill_part(T, key[r], log2p)], 1u) : 0u;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (!live[r]) continue;
+      const uint64_t *src = rows + (i0 + r * (int64_t)blockDim.x) * w;
+      uint64_t *dst = out + (uint64_t)pos[r] * w;
+      for (int k = 0; k < w; k++) dst[k] = src[k];
+    }
   }
 }
 
@@ -1168,7 +1193,7 @@ static void grow(kg_hash_agg *a, uint64_t min_cap) {
 // Rows the next launch may take so that the table stays below 3/4 full
 // whatever keys they hold (launches that insert into the table directly).
 static int64_t room(kg_hash_agg *a, int64_t want) {
-  if (a->direct) return want;
+  if (a->direct || !a->has_key) return want;  // fixed slots: nothing is inserted
   uint64_t limit = a->cap / 4 * 3;
   if (a->occ_bound + (uint64_t)want <= limit) return want;
   uint64_t occ = read_occupied(a);
@@ -1185,7 +1210,7 @@ static int64_t room(kg_hash_agg *a, int64_t want) {
 
 // Guarantees room for n new keys (the replay path, which does not split).
 static void reserve(kg_hash_agg *a, int64_t n) {
-  if (a->direct) return;
+  if (a->direct || !a->has_key) return;
   uint64_t occ = read_occupied(a);
   a->occ_bound = occ;
   while (occ + (uint64_t)n > a->cap / 4 * 3) grow(a, a->cap * 2);
@@ -1289,6 +1314,7 @@ static void spill_room(kg_hash_agg *a, int64_t n, int64_t blocks) {
 static void record_types(kg_hash_agg *a, const AggProg &P) {
   if (a->fmt.rw == 0) {
     int w = 2;
+    a->fmt.vword = 1;
     a->fmt.ncols = a->nsink;
     for (int c = 0; c < a->nsink; c++) {
       a->prog.in[c].type = P.in[c].type;

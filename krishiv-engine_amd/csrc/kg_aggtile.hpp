@@ -17,8 +17,9 @@
 namespace kg {
 
 struct SpillFmt {
-  int rw;     // words per spill row
-  int ncols;  // input columns carried
+  int rw;      // words per spill row
+  int ncols;   // input columns carried
+  int vword;   // 1: word 1 holds the validity bits; 0: every value is valid
   uint8_t off[kAggInputs];
 };
 
@@ -97,7 +98,7 @@ struct SpillSrc {
       ok[r] = false;
       if (!act[r]) continue;
       const uint64_t *row = rows + idx[r] * (int64_t)rw;
-      ok[r] = (row[1] >> c) & 1;
+      ok[r] = F->vword ? ((row[1] >> c) & 1) : true;
       uint64_t v = row[o];
       if (dev::is_f(t)) {
         f[r] = __longlong_as_double((long long)v);
@@ -206,6 +207,85 @@ __device__ __forceinline__ void tile_apply(const AggProg &P, uint64_t *srec, int
 #pragma unroll
         for (int r = 0; r < R; r++)
           if (ok[r]) max_u64<LDS>(srec + ls[r] * WL + op.word, f64_order(f[r]));
+        break;
+      default: break;
+    }
+  }
+}
+
+// The join fused with GROUP BY the build key: row r counts m[r] times (the
+// join pairs it makes).  Counts and sums scale by m; min/max do not.
+// Programs with checked or 192-bit sums are not built for this path.
+template <int R, bool LDS = true, class Src>
+__device__ __forceinline__ void tile_apply_mult(const AggProg &P, uint64_t *srec, int WL,
+                                                const int (&ls)[R], const uint64_t (&m)[R],
+                                                const Src &src) {
+  bool act[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) act[r] = ls[r] >= 0;
+  uint64_t lo[R], hi[R];
+  double f[R];
+  bool ok[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    lo[r] = hi[r] = 0;
+    f[r] = 0.0;
+    ok[r] = false;
+  }
+  int last = -1;
+  for (int o = 0; o < P.nops; o++) {
+    const AggOp op = P.ops[o];
+    if (op.kind == OP_COUNT_STAR) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (act[r]) add_u64_nr<LDS>(srec + ls[r] * WL, m[r]);
+      continue;
+    }
+    if (op.col != last) {
+      last = op.col;
+      src.load(P, op.col, act, lo, hi, f, ok);
+    }
+    uint64_t *base = srec + op.word;
+    switch (op.kind) {
+      case OP_NULLS:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (act[r] && !ok[r]) add_u64_nr<LDS>(base + ls[r] * WL, m[r]);
+        break;
+      case OP_ADD128:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) {
+            // (hi:lo) * m modulo 2^128
+            const uint64_t plo = lo[r] * m[r];
+            const uint64_t phi = __umul64hi(lo[r], m[r]) + hi[r] * m[r];
+            add128<LDS>(base + ls[r] * WL, plo, phi);
+          }
+        break;
+      case OP_ADDF64:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) add_f64<LDS>(base + ls[r] * WL, f[r] * (double)m[r]);
+        break;
+      case OP_MIN_I64:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) min_i64<LDS>(base + ls[r] * WL, (int64_t)lo[r]);
+        break;
+      case OP_MAX_I64:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) max_i64<LDS>(base + ls[r] * WL, (int64_t)lo[r]);
+        break;
+      case OP_MIN_F:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) min_u64<LDS>(base + ls[r] * WL, f64_order(f[r]));
+        break;
+      case OP_MAX_F:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) max_u64<LDS>(base + ls[r] * WL, f64_order(f[r]));
         break;
       default: break;
     }

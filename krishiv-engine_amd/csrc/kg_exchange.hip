@@ -196,6 +196,64 @@ __global__ void k_stride_gather(const uint64_t *in, uint64_t stride, uint32_t n,
 }
 }  // namespace kg
 
+namespace kg {
+// The row layout's stored hash h over the key columns (layout.rs:487-553):
+// h = fold(0, MIX) = 0; per column a NULL mixes NULL_HASH, a value its LE
+// words (narrower integers zero-extended, floats normalized, decimal128 as
+// two words); h = fold(h, FINISH).
+__device__ __forceinline__ uint64_t layout_hash(const KeyCols &k, int64_t i) {
+  uint64_t h = 0;
+  for (int c = 0; c < k.n; c++) {
+    const DCol &d = k.c[c];
+    auto mix = [&](uint64_t v) { h = dev::fold(h ^ v, dev::L_MIX); };
+    if (!dev::valid(d, i)) {
+      mix(dev::NULL_HASH);
+      continue;
+    }
+    const int64_t j = i + d.offset;
+    switch (d.type) {
+      case KG_T_FLOAT64: mix(dev::norm_f64_bits(static_cast<const double *>(d.values)[j])); break;
+      case KG_T_FLOAT32: mix(dev::norm_f32_bits(static_cast<const float *>(d.values)[j])); break;
+      case KG_T_DECIMAL128: {
+        const uint64_t *w = static_cast<const uint64_t *>(d.values) + 2 * j;
+        mix(w[0]);
+        mix(w[1]);
+        break;
+      }
+      case KG_T_BOOL: mix((uint64_t)dev::load_i64(d, i)); break;
+      default: {
+        uint64_t v = 0;
+        switch (d.width) {
+          case 1: v = static_cast<const uint8_t *>(d.values)[j]; break;
+          case 2: v = static_cast<const uint16_t *>(d.values)[j]; break;
+          case 4: v = static_cast<const uint32_t *>(d.values)[j]; break;
+          default: v = static_cast<const uint64_t *>(d.values)[j];
+        }
+        mix(v);
+      }
+    }
+  }
+  return dev::fold(h, dev::L_FINISH);
+}
+
+__global__ void k_join_partition(KeyCols k, int64_t n, int radix, int depth, uint64_t seed,
+                                 uint64_t *hashes, uint8_t *parts) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h = layout_hash(k, i);
+    if (hashes) hashes[i] = h;
+    uint32_t p;
+    if (depth == 0) {
+      p = (uint32_t)(h >> (64 - radix));
+    } else {
+      const int d = depth < 6 ? depth : 6;
+      p = (uint32_t)((dev::finalize(h, seed) >> (48 - 3 * d)) & 7);
+    }
+    parts[i] = (uint8_t)p;
+  }
+}
+}  // namespace kg
+
 extern "C" {
 
 int kg_hash_columns(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
@@ -218,6 +276,43 @@ int kg_hash_columns(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t n
     }
     if (!(out_flags & KG_COL_DEVICE))
       KG_HIP(hipMemcpyAsync(out_hashes, out, (size_t)nrows * 8, hipMemcpyDeviceToHost, s));
+    KG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int kg_join_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrows,
+                      uint32_t radix_bits, uint32_t depth, uint64_t seed, uint64_t *out_hashes,
+                      uint8_t *out_parts, uint32_t out_flags) {
+  return guarded([&] {
+    if (nrows <= 0) return;
+    if (nkeys < 1) fail(KG_ERR_UNSUPPORTED, "at least one key column");
+    const int radix = radix_bits < 3 ? 3 : radix_bits > 6 ? 6 : (int)radix_bits;
+    hipStream_t s = ctx->stream;
+    Staged st(ctx, s);
+    KeyCols k = stage_keys(st, nkeys, keys, nrows);
+    const bool dev_out = (out_flags & KG_COL_DEVICE) != 0;
+    DevBuf th, tp;
+    uint64_t *hh = out_hashes;
+    uint8_t *pp = out_parts;
+    if (!dev_out) {
+      if (out_hashes) {
+        th.reset(ctx, (uint64_t)nrows * 8);
+        hh = th.get<uint64_t>();
+      }
+      tp.reset(ctx, (uint64_t)nrows);
+      pp = tp.get<uint8_t>();
+    }
+    {
+      ProfScope prof("join_partition", s);
+      k_join_partition<<<stream_grid(nrows, 256), 256, 0, s>>>(k, nrows, radix, (int)depth, seed,
+                                                               hh, pp);
+      KG_HIP(hipGetLastError());
+    }
+    if (!dev_out) {
+      if (out_hashes)
+        KG_HIP(hipMemcpyAsync(out_hashes, hh, (size_t)nrows * 8, hipMemcpyDeviceToHost, s));
+      KG_HIP(hipMemcpyAsync(out_parts, pp, (size_t)nrows, hipMemcpyDeviceToHost, s));
+    }
     KG_HIP(hipStreamSynchronize(s));
   });
 }

@@ -21,6 +21,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "kg_aggops.hpp"
+#include "kg_aggtile.hpp"
 
 namespace kg {
 
@@ -237,6 +238,201 @@ __global__ void k_joinagg_probe(KeySpec K, int64_t n, JTable T, AggProg P, uint6
     uint32_t m = T.slots[s].count;
     if (!m) continue;
     apply_row_mult(P, rec + (uint64_t)s * P.W, i, m);
+  }
+}
+
+// ---------------------------------------------------------------- partitioned
+// The join fused with GROUP BY the build key, for large probes: the probe
+// rows are radix-partitioned by the range of their key's home slot
+// (k_jp_hist, k_jp_scatter: one read of the inputs, one write of compact
+// rows [key, validity bits if any input has NULLs, aggregate inputs]).
+// k_jp_merge gives a partition to one workgroup, which loads the window of
+// the build slots it can reach (its C home slots plus kJMargin slots of
+// probe overflow: keys and row counts) with delta records into LDS, probes
+// and aggregates there (LDS atomics, times the key's build row count), then
+// adds each delta into its key's record with plain loads and stores: only
+// the workgroup of a key's home range has rows for it.  A probe that walks
+// past the window falls back to the global table with atomics.
+constexpr int kJPBlock = 256;
+constexpr int kJMBlock = 512;
+constexpr int kJMargin = 128;
+constexpr int kJTile = 4;
+
+__device__ __forceinline__ uint64_t jhash(const JTable &T, uint64_t key) {
+  return dev::finalize(dev::row_hash1(key), T.seed);
+}
+
+__device__ __forceinline__ uint32_t jpart(const JTable &T, uint64_t key, int log2p) {
+  return log2p ? (uint32_t)(jhash(T, key) >> (64 - log2p)) : 0u;
+}
+
+__global__ void k_jp_hist(KeySpec K, int64_t n, JTable T, int log2p, int64_t per_block,
+                          uint32_t *hist) {
+  extern __shared__ uint32_t s_h[];
+  const uint32_t np = 1u << log2p;
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) s_h[p] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = lo + per_block < n ? lo + per_block : n;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
+    uint64_t key[4];
+    bool live[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int64_t i = i0 + r * (int64_t)blockDim.x;
+      live[r] = i < hi && pack_key(K, i, key[r]) == 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (live[r]) atomicAdd(&s_h[jpart(T, key[r], log2p)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x)
+    hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_h[p];
+}
+
+// Compact rows by partition.  Keys that are NULL under `=` match nothing;
+// the key equal to the EMPTY sentinel and a `<=>` NULL have fixed slots and
+// are applied here with atomics.
+__global__ void k_jp_scatter(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt F, int log2p,
+                             int64_t per_block, const uint64_t *base, uint64_t *out,
+                             uint64_t *rec) {
+  extern __shared__ uint32_t s_c[];
+  const uint32_t np = 1u << log2p;
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x)
+    s_c[p] = (uint32_t)base[(uint64_t)p * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = lo + per_block < n ? lo + per_block : n;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
+   uint64_t keys[4];
+   int cls4[4];
+   uint32_t pos4[4];
+#pragma unroll
+   for (int r = 0; r < 4; r++) {
+     const int64_t i = i0 + r * (int64_t)blockDim.x;
+     cls4[r] = i < hi ? pack_key(K, i, keys[r]) : 3;
+   }
+#pragma unroll
+   for (int r = 0; r < 4; r++)
+     pos4[r] = cls4[r] == 0 ? atomicAdd(&s_c[jpart(T, keys[r], log2p)], 1u) : 0u;
+   for (int r = 0; r < 4; r++) {
+    const int64_t i = i0 + r * (int64_t)blockDim.x;
+    const int cls = cls4[r];
+    const uint64_t key = keys[r];
+    if (cls == 3) continue;
+    if (cls != 0) {
+      const uint64_t s = T.cap + (cls == 2 ? 1 : 0);
+      const uint32_t m = T.slots[s].count;
+      if (m) apply_row_mult(P, rec + s * (uint64_t)P.W, i, m);
+      continue;
+    }
+    const uint32_t pos = pos4[r];
+    uint64_t *row = out + (uint64_t)pos * F.rw;
+    row[0] = key;
+    uint64_t vbits = 0;
+    for (int c = 0; c < F.ncols; c++) {
+      const DCol &d = P.in[c];
+      const bool v = dev::valid(d, i);
+      uint64_t *w = row + F.off[c];
+      if (v) vbits |= 1ull << c;
+      if (dev::is_f(d.type)) {
+        w[0] = v ? (uint64_t)__double_as_longlong(dev::load_f64(d, i)) : 0;
+      } else {
+        uint64_t l = 0;
+        int64_t h = 0;
+        if (v) dev::load_i128(d, i, l, h);
+        w[0] = l;
+        if (d.type == KG_T_DECIMAL128) w[1] = (uint64_t)h;
+      }
+    }
+    if (F.vword) row[1] = vbits;
+   }
+  }
+}
+
+__global__ void __launch_bounds__(kJMBlock) k_jp_merge(JTable T, AggProg P, SpillFmt F,
+                                                       const uint64_t *rows, uint64_t nrows,
+                                                       const uint64_t *base, uint32_t nb,
+                                                       int log2p, int C, int Wn, uint64_t *rec) {
+  extern __shared__ uint64_t smem[];
+  const int W = P.W, WL = W | 1;
+  uint64_t *s_keys = smem;
+  uint64_t *s_rec = s_keys + Wn;
+  uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_rec + (uint64_t)Wn * WL);
+  const uint32_t np = 1u << log2p;
+  const bool whole = (uint64_t)Wn == T.cap;  // the window is the whole table
+  for (uint32_t p = blockIdx.x; p < np; p += gridDim.x) {
+    const uint64_t start = (uint64_t)p * C;
+    for (int l = threadIdx.x; l < Wn; l += blockDim.x) {
+      const uint64_t s = (start + l) & T.mask;
+      s_keys[l] = T.slots[s].key;
+      s_cnt[l] = T.slots[s].count;
+    }
+    for (int t = threadIdx.x; t < Wn * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
+    __syncthreads();
+    const int64_t a = (int64_t)base[(uint64_t)p * nb];
+    const int64_t b = p + 1 < np ? (int64_t)base[(uint64_t)(p + 1) * nb] : (int64_t)nrows;
+    for (int64_t t0 = a; t0 < b; t0 += (int64_t)kJTile * blockDim.x) {
+      int ls[kJTile];
+      uint64_t m[kJTile];
+      bool fb[kJTile];
+      SpillSrc<kJTile> src;
+      src.rows = rows;
+      src.F = &F;
+#pragma unroll
+      for (int r = 0; r < kJTile; r++) {
+        const int64_t i = t0 + (int64_t)r * blockDim.x + threadIdx.x;
+        src.idx[r] = i;
+        ls[r] = -1;
+        m[r] = 0;
+        fb[r] = false;
+        if (i >= b) continue;
+        const uint64_t key = rows[i * F.rw];
+        int l = (int)((jhash(T, key) >> (64 - T.log2cap)) - start);
+        while (true) {
+          if (l >= Wn) {
+            if (!whole) {
+              fb[r] = true;
+              break;
+            }
+            l = 0;
+          }
+          const uint64_t k = s_keys[l];
+          if (k == key) {
+            if (s_cnt[l]) {
+              ls[r] = l;
+              m[r] = s_cnt[l];
+            }
+            break;
+          }
+          if (k == kEmpty) break;
+          l++;
+        }
+      }
+      tile_apply_mult<kJTile>(P, s_rec, WL, ls, m, src);
+#pragma unroll
+      for (int r = 0; r < kJTile; r++) {
+        if (!fb[r]) continue;
+        const uint64_t key = rows[src.idx[r] * F.rw];
+        const int64_t s = jfind(T, 0, key);
+        if (s < 0 || T.slots[s].count == 0) continue;
+        SpillSrc<1> one;
+        one.rows = rows;
+        one.F = &F;
+        one.idx[0] = src.idx[r];
+        const int z[1] = {0};
+        const uint64_t mm[1] = {T.slots[s].count};
+        tile_apply_mult<1, false>(P, rec + (uint64_t)s * W, 0, z, mm, one);
+      }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < Wn; l += blockDim.x) {
+      const uint64_t *d = s_rec + (uint64_t)l * WL;
+      if (d[0] == 0) continue;
+      merge_plain(P, rec + ((start + l) & T.mask) * (uint64_t)W, d);
+    }
+    __syncthreads();
   }
 }
 
@@ -635,6 +831,74 @@ int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_t
   });
 }
 
+namespace kg {
+
+// The partitioned probe of the fused join aggregate (k_jp_hist, k_jp_scatter,
+// k_jp_merge).
+static void joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg &P,
+                                int64_t nrows) {
+  const int W = P.W, WL = W | 1;
+  const int ninputs = (int)j->agg_types.size();
+  SpillFmt F{};
+  F.ncols = ninputs;
+  F.vword = 0;
+  for (int c = 0; c < ninputs; c++)
+    if (P.in[c].validity) F.vword = 1;
+  int w = 1 + F.vword;
+  for (int c = 0; c < ninputs; c++) {
+    F.off[c] = (uint8_t)w;
+    w += P.in[c].type == KG_T_DECIMAL128 ? 2 : 1;
+  }
+  F.rw = w;
+  // window: C home slots + the probe overflow margin, within 80 KiB of LDS
+  const uint64_t per_slot = 8 + 4 + (uint64_t)WL * 8;
+  uint64_t C = 1;
+  while (C * 2 <= j->cap && (C * 2 + kJMargin) * per_slot <= 80 * 1024) C *= 2;
+  if ((C + kJMargin) * per_slot > 80 * 1024 && C > 1) C /= 2;
+  const int log2p = (int)(j->log2cap - __builtin_ctzll(C));
+  const uint32_t np = 1u << log2p;
+  const int Wn = (int)std::min<uint64_t>(C + kJMargin, j->cap);
+  const size_t lds = (size_t)Wn * per_slot;
+  if (lds > 80 * 1024) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the join window");
+  KG_HIP(hipFuncSetAttribute((const void *)k_jp_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds));
+  const int64_t fill = (int64_t)(j->ctx->num_cus > 0 ? j->ctx->num_cus : 256) * 2;
+  int64_t per_block = std::max<int64_t>(4096, std::min<int64_t>((int64_t)np * 32, ceil_div(nrows, fill)));
+  int64_t nb = ceil_div(nrows, per_block);
+  per_block = ceil_div(nrows, nb);
+  const uint64_t hn = (uint64_t)np * nb;
+  DevBuf hist(j->ctx, hn * 4 + 4), base(j->ctx, hn * 8 + 8);
+  DevBuf rows(j->ctx, (uint64_t)nrows * F.rw * 8 + 8);
+  JTable T = jtable(j);
+  ProfScope prof("joinagg_probe", j->s);
+  k_jp_hist<<<(unsigned)nb, kJPBlock, np * 4, j->s>>>(K, nrows, T, log2p, per_block,
+                                                      hist.get<uint32_t>());
+  KG_HIP(hipGetLastError());
+  size_t tb = 0;
+  KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist.get<uint32_t>(), base.get<uint64_t>(),
+                                          (int)hn, j->s));
+  DevBuf temp(j->ctx, tb + 16);
+  KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, hist.get<uint32_t>(), base.get<uint64_t>(),
+                                          (int)hn, j->s));
+  k_jp_scatter<<<(unsigned)nb, kJPBlock, np * 4, j->s>>>(K, nrows, T, P, F, log2p, per_block,
+                                                         base.get<uint64_t>(), rows.get<uint64_t>(),
+                                                         j->rec.get<uint64_t>());
+  KG_HIP(hipGetLastError());
+  uint64_t lb = 0;
+  uint32_t lc = 0;
+  KG_HIP(hipMemcpyAsync(&lb, base.get<uint64_t>() + hn - 1, 8, hipMemcpyDeviceToHost, j->s));
+  KG_HIP(hipMemcpyAsync(&lc, hist.get<uint32_t>() + hn - 1, 4, hipMemcpyDeviceToHost, j->s));
+  KG_HIP(hipStreamSynchronize(j->s));
+  const uint64_t live = lb + lc;
+  const unsigned grid = (unsigned)std::min<int64_t>(np, fill);
+  k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, P, F, rows.get<uint64_t>(), live, base.get<uint64_t>(),
+                                             (uint32_t)nb, log2p, (int)C, Wn, j->rec.get<uint64_t>());
+  KG_HIP(hipGetLastError());
+  KG_HIP(hipStreamSynchronize(j->s));
+}
+
+}  // namespace kg
+
 int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                      const int32_t *input_cols) {
   return guarded([&] {
@@ -652,11 +916,14 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
       if (d[ic].type != j->agg_types[c].type) fail(KG_ERR_INTERNAL, "aggregate input type");
       P.in[c] = d[ic];
     }
-    {
+    if (nrows >= (1ll << 32)) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^32 rows or more");
+    if (nrows < (1 << 20)) {
       ProfScope prof("joinagg_probe", j->s);
       k_joinagg_probe<<<stream_grid(nrows, 256), 256, 0, j->s>>>(K, nrows, jtable(j), P,
                                                                  j->rec.get<uint64_t>());
       KG_HIP(hipGetLastError());
+    } else {
+      joinagg_partitioned(j, K, P, nrows);
     }
     if (st.staged_host()) KG_HIP(hipStreamSynchronize(j->s));
   });

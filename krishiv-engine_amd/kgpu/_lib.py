@@ -178,3 +178,8 @@ _SIGS.update({
     "kg_server_pending": (C.c_int64, [C.c_void_p]),
     "kg_server_destroy": (None, [C.c_void_p]),
 })
+
+_SIGS.update({
+    "kg_join_partition": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_uint32,
+                                    C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint32]),
+})

@@ -288,6 +288,18 @@ def route(ctx: Context, keys, partitions: int) -> np.ndarray:
     return out
 
 
+def join_partition(ctx: Context, keys, radix_bits: int, depth: int, seed: int):
+    """(h, partition) of each row: the join's row-layout hash of the key
+    columns and its grace partition at `depth` (host numpy arrays)."""
+    n = keys[0].length
+    h = np.zeros(n, dtype=np.uint64)
+    p = np.zeros(n, dtype=np.uint8)
+    if n:
+        check(lib().kg_join_partition(ctx.h, len(keys), _cols(keys), n, radix_bits, depth,
+                                      seed, h.ctypes.data, p.ctypes.data, 0))
+    return h, p
+
+
 def repartition(ctx: Context, cols, key_cols, partitions: int):
     """Rows regrouped by partition (stable): (Batch, offsets[partitions+1])."""
     n = cols[0].length

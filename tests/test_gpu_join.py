@@ -4,7 +4,7 @@ as sorted row multisets with Spark NULL semantics, aggregates bit-exact."""
 import numpy as np
 import pytest
 
-from gpu_util import ctx, say
+from gpu_util import ctx, same_float, say
 import kgo
 import kgpu
 from kgpu import Col
@@ -180,3 +180,53 @@ def test_empty_probe_and_no_match():
     j.build_finalize()
     assert j.probe([Col(A.T_INT64, np.array([], dtype=np.int64))]).num_rows == 0
     assert j.probe([Col(A.T_INT64, np.array([7, 8]))]).num_rows == 0
+
+
+@pytest.mark.parametrize("nb_keys,nb_rows", [(5000, 20000), (300_000, 300_000)])
+def test_partitioned_fused_aggregate(nb_keys, nb_rows):
+    """Probe batches of a million rows and more take the radix-partitioned
+    fused aggregate (LDS windows of the build table): duplicate build keys
+    (the row count multiplies), NULL probe keys and NULL inputs, the key equal
+    to the table's sentinel, a small table (one window) and a large one."""
+    say(f"join: partitioned fused aggregate, {nb_rows} build rows over {nb_keys} keys")
+    rng = np.random.default_rng(nb_keys)
+    bk = rng.integers(0, nb_keys, nb_rows).astype(np.int64)
+    bk[::997] = np.iinfo(np.int64).min
+    npr = 1_300_000
+    pk = rng.integers(0, int(nb_keys * 1.2), npr).astype(np.int64)
+    pk[::1009] = np.iinfo(np.int64).min
+    pkv = rng.random(npr) > 0.02
+    amt = rng.integers(-(1 << 40), 1 << 40, npr)
+    amtv = rng.random(npr) > 0.03
+    f = rng.random(npr)
+    op, ob = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, pk, pkv)], [kgo.Col(kgo.T_INT64, bk)])
+    aggs = [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0), (kgo.AGG_MIN, 0), (kgo.AGG_MAX, 1),
+            (kgo.AGG_AVG, 1), (kgo.AGG_COUNT, 0)]
+    o = kgo.Agg((kgo.T_INT64, 0, 0), [(kgo.T_INT64, 0, 0), (kgo.T_FLOAT64, 0, 0)], aggs)
+    o.update(kgo.Col(kgo.T_INT64, bk[ob]),
+             [kgo.Col(kgo.T_INT64, amt[op], amtv[op]), kgo.Col(kgo.T_FLOAT64, f[op])])
+    rk, _, rres = o.finish()
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0], expected_build_rows=nb_rows)
+    j.build_sink([Col(A.T_INT64, bk)])
+    j.build_finalize()
+    j.aggregate([(A.T_INT64, 0, 0), (A.T_FLOAT64, 0, 0)], [(a, c) for a, c in aggs])
+    cut = 1_100_000  # one batch on the partitioned path, one on the direct path
+    for a, b in ((0, cut), (cut, npr)):
+        j.probe_aggregate([Col(A.T_INT64, pk[a:b], pkv[a:b]), Col(A.T_INT64, amt[a:b], amtv[a:b]),
+                           Col(A.T_FLOAT64, f[a:b])], [1, 2])
+    r = j.aggregate_result()
+    assert r.num_rows == len(rk)
+    gk = r.column(0)[0]
+    order = np.argsort(gk, kind="stable")
+    ro = np.argsort(np.asarray(rk), kind="stable")
+    assert np.array_equal(np.asarray(gk)[order], np.asarray(rk)[ro])
+    for c in range(len(aggs)):
+        rv, rok, (rt, _, _) = rres[c]
+        gv, gok = r.column(1 + c)
+        rv, rok, gv, gok = np.asarray(rv)[ro], np.asarray(rok)[ro], gv[order], gok[order]
+        assert np.array_equal(rok.astype(bool), gok.astype(bool)), c
+        if rt == kgo.T_FLOAT64:
+            a, b = rv[rok].astype(np.float64), gv[rok].astype(np.float64)
+            assert np.all(np.abs(a - b) <= 1e-12 * np.maximum(np.abs(a), np.abs(b)) + 1e-300), c
+        else:
+            assert np.array_equal(rv[rok].astype(np.int64), gv[rok].astype(np.int64)), c
