@@ -305,7 +305,10 @@ void kg_grace_destroy(kg_grace *g);
  * registered with kg_server_add_table stay resident and are shared by the
  * requests that name them (the broadcast build tables of join/broadcast.rs).
  * Two batches can be in flight (two streams), so requests that arrive while
- * one batch runs are admitted into the next as arena space frees. */
+ * one batch runs are admitted into the next as arena space frees.  A request
+ * whose probe has 4M rows or more is not batched: kg_server_progress runs it
+ * alone through the radix-partitioned fused aggregate (kg_joinagg_*) while
+ * the batches run on the device. */
 typedef struct kg_server kg_server;
 typedef struct kg_serve_request {
   int64_t id;

@@ -396,35 +396,47 @@ __global__ void k_spill_hist(const uint64_t *rows, int w, uint64_t n, Table T, i
 }
 
 // Moves each row to its partition (order within a partition is not kept).
+// Rows of up to kRegWords words are read into registers first, so every load
+// of a thread's four rows is in flight before the cursor atomics.
 __global__ void k_spill_scatter(const uint64_t *rows, int w, uint64_t n, Table T, int log2p,
                                 int64_t per_block, int dead, const uint64_t *base, uint64_t *out) {
   extern __shared__ uint32_t s_c[];
+  constexpr int kRegWords = 6;
   const uint32_t np = 1u << log2p;
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x)
     s_c[p] = (uint32_t)base[(uint64_t)p * gridDim.x + blockIdx.x];
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * per_block;
   const int64_t hi = lo + per_block < (int64_t)n ? lo + per_block : (int64_t)n;
-  // four rows a thread: their loads and cursor atomics are in flight together
+  const bool regs = w <= kRegWords;
   for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
-    uint64_t key[4];
+    uint64_t v[4][kRegWords];
     bool live[4];
-    uint32_t pos[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int64_t i = i0 + r * (int64_t)blockDim.x;
-      live[r] = i < hi && !(dead && (rows[i * w + 1] & kDeadRow));
-      key[r] = i < hi ? rows[i * w] : 0;
+      const bool in = i < hi;
+#pragma unroll
+      for (int k = 0; k < kRegWords; k++)
+        v[r][k] = (in && (k < w) && (regs || k < 2)) ? rows[i * w + k] : 0;
+      live[r] = in && !(dead && (v[r][1] & kDeadRow));
     }
+    uint32_t pos[4];
 #pragma unroll
     for (int r = 0; r < 4; r++)
-      pos[r] = live[r] ? atomicAdd(&s_c[spill_part(T, key[r], log2p)], 1u) : 0u;
+      pos[r] = live[r] ? atomicAdd(&s_c[spill_part(T, v[r][0], log2p)], 1u) : 0u;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       if (!live[r]) continue;
-      const uint64_t *src = rows + (i0 + r * (int64_t)blockDim.x) * w;
       uint64_t *dst = out + (uint64_t)pos[r] * w;
-      for (int k = 0; k < w; k++) dst[k] = src[k];
+      if (regs) {
+#pragma unroll
+        for (int k = 0; k < kRegWords; k++)
+          if (k < w) dst[k] = v[r][k];
+      } else {
+        const uint64_t *src = rows + (i0 + r * (int64_t)blockDim.x) * w;
+        for (int k = 0; k < w; k++) dst[k] = src[k];
+      }
     }
   }
 }
@@ -596,7 +608,6 @@ __global__ void __launch_bounds__(kMergeBlock) k_agg_merge(AggProg P, Table T, M
         int ls[R], lsw[R];
         SpillSrc<R> src;
         src.rows = M.rows;
-        src.F = &M.F;
 #pragma unroll
         for (int r = 0; r < R; r++) {
           ls[r] = -1;
@@ -676,7 +687,6 @@ __global__ void k_agg_retry(AggProg P, Table T, MergeDev M, int S, uint32_t nwg)
       } else {
         SpillSrc<1> src;
         src.rows = M.rows;
-        src.F = &M.F;
         src.idx[0] = r0 + u - nrec;
         uint64_t *r = T.rec + global_slot(T, M.rows[src.idx[0] * M.F.rw]) * (uint64_t)W;
         int ls[1] = {0};
@@ -1435,6 +1445,7 @@ static void drain(kg_hash_agg *a) {
   KG_HIP(hipMemsetAsync(resume.get(), 0, resume.bytes(), a->s));
   AggProg P = a->prog;
   P.skip_checked = 0;
+  set_spill_layout(P, a->fmt);
   MergeDev M{};
   M.rows = prow.get<uint64_t>();
   M.nrows = nr;

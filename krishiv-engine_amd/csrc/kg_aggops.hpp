@@ -55,6 +55,11 @@ struct AggProg {
   int direct;    // one-byte key: slot = (uint8)key, NULL -> 256
   int skip_checked;  // ordered-replay batches: checked sums are applied later
   uint64_t init[kAggWords];
+  // the spill row layout read back by SpillSrc (words per row, whether word 1
+  // holds validity bits, each input column's word offset)
+  int spill_rw;
+  int spill_vword;
+  uint8_t spill_off[kAggInputs];
 };
 
 struct Table {

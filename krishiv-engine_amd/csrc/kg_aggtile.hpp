@@ -23,6 +23,14 @@ struct SpillFmt {
   uint8_t off[kAggInputs];
 };
 
+// Copies a spill row layout into a program (host side, before a launch whose
+// kernels read spill rows).
+inline void set_spill_layout(AggProg &P, const SpillFmt &F) {
+  P.spill_rw = F.rw;
+  P.spill_vword = F.vword;
+  for (int c = 0; c < kAggInputs; c++) P.spill_off[c] = F.off[c];
+}
+
 // Loads of input column c for the R rows of a lane (rows with act[r] false
 // are skipped; their ok[r] is false).
 template <int R>
@@ -82,32 +90,31 @@ struct ColSrc {
   }
 };
 
-// Loads from spill rows: rows[idx[r] * F.rw ...].
+// Loads from spill rows: rows[idx[r] * P.spill_rw ...] (the layout is in the
+// program, a kernel argument: no pointer to a kernel argument is kept, which
+// would make the compiler copy it to scratch).
 template <int R>
 struct SpillSrc {
   const uint64_t *rows;
   int64_t idx[R];
-  const SpillFmt *F;
   __device__ __forceinline__ void load(const AggProg &P, int c, const bool (&act)[R],
                                        uint64_t (&lo)[R], uint64_t (&hi)[R], double (&f)[R],
                                        bool (&ok)[R]) const {
     const int t = P.in[c].type;
-    const int rw = F->rw, o = F->off[c];
+    const int rw = P.spill_rw, o = P.spill_off[c];
+    const bool wide = t == KG_T_DECIMAL128, uns = t == KG_T_UINT64;
+    // every array is indexed by the unrolled r only (a loop that is not
+    // unrolled would put lo/hi/f in scratch)
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      ok[r] = false;
-      if (!act[r]) continue;
       const uint64_t *row = rows + idx[r] * (int64_t)rw;
-      ok[r] = F->vword ? ((row[1] >> c) & 1) : true;
-      uint64_t v = row[o];
-      if (dev::is_f(t)) {
-        f[r] = __longlong_as_double((long long)v);
-      } else {
-        lo[r] = v;
-        hi[r] = t == KG_T_DECIMAL128 ? row[o + 1]
-                : t == KG_T_UINT64   ? 0
-                                     : (uint64_t)((int64_t)v >> 63);
-      }
+      const uint64_t v = act[r] ? row[o] : 0;
+      const uint64_t vb = act[r] && P.spill_vword ? row[1] : ~0ull;
+      const uint64_t w2 = act[r] && wide ? row[o + 1] : 0;
+      ok[r] = act[r] && ((vb >> c) & 1);
+      f[r] = __longlong_as_double((long long)v);
+      lo[r] = v;
+      hi[r] = wide ? w2 : uns ? 0 : (uint64_t)((int64_t)v >> 63);
     }
   }
 };
@@ -171,18 +178,42 @@ __device__ __forceinline__ void tile_apply(const AggProg &P, uint64_t *srec, int
         for (int r = 0; r < R; r++)
           if (ok[r] && lo[r]) add_u64_nr<LDS>(srec + ls[r] * WL + op.word, 1);
         break;
-      case OP_ADD128:
+      case OP_ADD128: {
+        // the rows' low-word adds issued together, one wait, then the carries
+        uint64_t old[R];
 #pragma unroll
         for (int r = 0; r < R; r++)
-          if (ok[r]) add128<LDS>(srec + ls[r] * WL + op.word, lo[r], hi[r]);
+          old[r] = ok[r] ? add_u64<LDS>(srec + ls[r] * WL + op.word, lo[r]) : 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint64_t h = hi[r] + ((old[r] + lo[r]) < old[r] ? 1 : 0);
+          if (ok[r] && h) add_u64_nr<LDS>(srec + ls[r] * WL + op.word + 1, h);
+        }
         break;
-      case OP_ADD192:
+      }
+      case OP_ADD192: {
+        uint64_t old[R], mid[R], c1[R];
 #pragma unroll
         for (int r = 0; r < R; r++)
-          if (ok[r])
-            add192<LDS>(srec + ls[r] * WL + op.word, lo[r], hi[r],
-                         (uint64_t)((int64_t)hi[r] >> 63));
+          old[r] = ok[r] ? add_u64<LDS>(srec + ls[r] * WL + op.word, lo[r]) : 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint64_t c0 = (old[r] + lo[r]) < old[r] ? 1 : 0;
+          mid[r] = hi[r] + c0;
+          c1[r] = mid[r] < hi[r] ? 1 : 0;  // hi = ~0 with a carry in
+        }
+        uint64_t old1[R];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          old1[r] = (ok[r] && mid[r]) ? add_u64<LDS>(srec + ls[r] * WL + op.word + 1, mid[r]) : 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint64_t c2 = (ok[r] && mid[r] && (old1[r] + mid[r]) < old1[r]) ? 1 : 0;
+          const uint64_t top = (uint64_t)((int64_t)hi[r] >> 63) + c1[r] + c2;
+          if (ok[r] && top) add_u64_nr<LDS>(srec + ls[r] * WL + op.word + 2, top);
+        }
         break;
+      }
       case OP_ADDF64:
 #pragma unroll
         for (int r = 0; r < R; r++)

@@ -293,7 +293,25 @@ __global__ void k_jp_hist(KeySpec K, int64_t n, JTable T, int log2p, int64_t per
 
 // Compact rows by partition.  Keys that are NULL under `=` match nothing;
 // the key equal to the EMPTY sentinel and a `<=>` NULL have fixed slots and
-// are applied here with atomics.
+// are applied here with atomics.  A thread takes four rows; their keys and
+// (up to kJPre inputs of one word) values are loaded before the cursor
+// atomics, so the loads are in flight together.
+constexpr int kJPre = 4;
+
+__device__ __forceinline__ void jp_value(const DCol &d, int64_t i, bool &v, uint64_t &lo,
+                                         uint64_t &hi) {
+  v = dev::valid(d, i);
+  lo = hi = 0;
+  if (!v) return;
+  if (dev::is_f(d.type)) {
+    lo = (uint64_t)__double_as_longlong(dev::load_f64(d, i));
+  } else {
+    int64_t h;
+    dev::load_i128(d, i, lo, h);
+    hi = (uint64_t)h;
+  }
+}
+
 __global__ void k_jp_scatter(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt F, int log2p,
                              int64_t per_block, const uint64_t *base, uint64_t *out,
                              uint64_t *rec) {
@@ -304,50 +322,85 @@ __global__ void k_jp_scatter(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * per_block;
   const int64_t hi = lo + per_block < n ? lo + per_block : n;
+  bool pre = F.ncols <= kJPre;
+  for (int c = 0; c < F.ncols && c < kJPre; c++)
+    if (P.in[c].type == KG_T_DECIMAL128) pre = false;
   for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
-   uint64_t keys[4];
-   int cls4[4];
-   uint32_t pos4[4];
+    uint64_t keys[4];
+    int cls4[4];
+    uint64_t val[4][kJPre];
+    uint64_t vb[4];
 #pragma unroll
-   for (int r = 0; r < 4; r++) {
-     const int64_t i = i0 + r * (int64_t)blockDim.x;
-     cls4[r] = i < hi ? pack_key(K, i, keys[r]) : 3;
-   }
+    for (int r = 0; r < 4; r++) {
+      const int64_t i = i0 + r * (int64_t)blockDim.x;
+      cls4[r] = i < hi ? pack_key(K, i, keys[r]) : 3;
+      vb[r] = 0;
 #pragma unroll
-   for (int r = 0; r < 4; r++)
-     pos4[r] = cls4[r] == 0 ? atomicAdd(&s_c[jpart(T, keys[r], log2p)], 1u) : 0u;
-   for (int r = 0; r < 4; r++) {
-    const int64_t i = i0 + r * (int64_t)blockDim.x;
-    const int cls = cls4[r];
-    const uint64_t key = keys[r];
-    if (cls == 3) continue;
-    if (cls != 0) {
-      const uint64_t s = T.cap + (cls == 2 ? 1 : 0);
-      const uint32_t m = T.slots[s].count;
-      if (m) apply_row_mult(P, rec + s * (uint64_t)P.W, i, m);
-      continue;
-    }
-    const uint32_t pos = pos4[r];
-    uint64_t *row = out + (uint64_t)pos * F.rw;
-    row[0] = key;
-    uint64_t vbits = 0;
-    for (int c = 0; c < F.ncols; c++) {
-      const DCol &d = P.in[c];
-      const bool v = dev::valid(d, i);
-      uint64_t *w = row + F.off[c];
-      if (v) vbits |= 1ull << c;
-      if (dev::is_f(d.type)) {
-        w[0] = v ? (uint64_t)__double_as_longlong(dev::load_f64(d, i)) : 0;
-      } else {
-        uint64_t l = 0;
-        int64_t h = 0;
-        if (v) dev::load_i128(d, i, l, h);
-        w[0] = l;
-        if (d.type == KG_T_DECIMAL128) w[1] = (uint64_t)h;
+      for (int c = 0; c < kJPre; c++) {
+        val[r][c] = 0;
+        if (pre && c < F.ncols && cls4[r] == 0) {
+          bool v;
+          uint64_t l, h;
+          jp_value(P.in[c], i, v, l, h);
+          val[r][c] = l;
+          if (v) vb[r] |= 1ull << c;
+        }
       }
     }
-    if (F.vword) row[1] = vbits;
-   }
+    uint32_t pos4[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      pos4[r] = cls4[r] == 0 ? atomicAdd(&s_c[jpart(T, keys[r], log2p)], 1u) : 0u;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int64_t i = i0 + r * (int64_t)blockDim.x;
+      const int cls = cls4[r];
+      if (cls == 3) continue;
+      if (cls != 0) {
+        const uint64_t s = T.cap + (cls == 2 ? 1 : 0);
+        const uint32_t m = T.slots[s].count;
+        if (m) apply_row_mult(P, rec + s * (uint64_t)P.W, i, m);
+        continue;
+      }
+      uint64_t *row = out + (uint64_t)pos4[r] * F.rw;
+      row[0] = keys[r];
+      if (pre) {
+#pragma unroll
+        for (int c = 0; c < kJPre; c++)
+          if (c < F.ncols) row[F.off[c]] = val[r][c];
+        if (F.vword) row[1] = vb[r];
+        continue;
+      }
+      uint64_t vbits = 0;
+      for (int c = 0; c < F.ncols; c++) {
+        bool v;
+        uint64_t l, h;
+        jp_value(P.in[c], i, v, l, h);
+        if (v) vbits |= 1ull << c;
+        uint64_t *w = row + F.off[c];
+        w[0] = l;
+        if (P.in[c].type == KG_T_DECIMAL128) w[1] = h;
+      }
+      if (F.vword) row[1] = vbits;
+    }
+  }
+}
+
+// Probe of the LDS window for `key`: (slot, build row count) on a match,
+// (-1, 0) on none, (-2, 0) when the probe walks past the window.
+__device__ __forceinline__ int2 window_probe(const JTable &T, const uint64_t *s_keys,
+                                             const uint32_t *s_cnt, int Wn, bool whole,
+                                             uint64_t start, uint64_t key) {
+  int l = (int)((jhash(T, key) >> (64 - T.log2cap)) - start);
+  while (true) {
+    if (l >= Wn) {
+      if (!whole) return make_int2(-2, 0);
+      l = 0;
+    }
+    const uint64_t k = s_keys[l];
+    if (k == key) return s_cnt[l] ? make_int2(l, (int)s_cnt[l]) : make_int2(-1, 0);
+    if (k == kEmpty) return make_int2(-1, 0);
+    l++;
   }
 }
 
@@ -379,47 +432,25 @@ __global__ void __launch_bounds__(kJMBlock) k_jp_merge(JTable T, AggProg P, Spil
       bool fb[kJTile];
       SpillSrc<kJTile> src;
       src.rows = rows;
-      src.F = &F;
 #pragma unroll
       for (int r = 0; r < kJTile; r++) {
         const int64_t i = t0 + (int64_t)r * blockDim.x + threadIdx.x;
         src.idx[r] = i;
-        ls[r] = -1;
-        m[r] = 0;
-        fb[r] = false;
-        if (i >= b) continue;
-        const uint64_t key = rows[i * F.rw];
-        int l = (int)((jhash(T, key) >> (64 - T.log2cap)) - start);
-        while (true) {
-          if (l >= Wn) {
-            if (!whole) {
-              fb[r] = true;
-              break;
-            }
-            l = 0;
-          }
-          const uint64_t k = s_keys[l];
-          if (k == key) {
-            if (s_cnt[l]) {
-              ls[r] = l;
-              m[r] = s_cnt[l];
-            }
-            break;
-          }
-          if (k == kEmpty) break;
-          l++;
-        }
+        const int2 hit = i < b ? window_probe(T, s_keys, s_cnt, Wn, whole, start, rows[i * F.rw])
+                               : make_int2(-1, 0);
+        ls[r] = hit.x >= 0 ? hit.x : -1;
+        m[r] = hit.x >= 0 ? (uint64_t)(uint32_t)hit.y : 0;
+        fb[r] = hit.x == -2;
       }
       tile_apply_mult<kJTile>(P, s_rec, WL, ls, m, src);
 #pragma unroll
       for (int r = 0; r < kJTile; r++) {
-        if (!fb[r]) continue;
+        if (!fb[r]) continue;  // (rare) the global table, with atomics
         const uint64_t key = rows[src.idx[r] * F.rw];
         const int64_t s = jfind(T, 0, key);
         if (s < 0 || T.slots[s].count == 0) continue;
         SpillSrc<1> one;
         one.rows = rows;
-        one.F = &F;
         one.idx[0] = src.idx[r];
         const int z[1] = {0};
         const uint64_t mm[1] = {T.slots[s].count};
@@ -850,16 +881,29 @@ static void joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
     w += P.in[c].type == KG_T_DECIMAL128 ? 2 : 1;
   }
   F.rw = w;
-  // window: C home slots + the probe overflow margin, within 80 KiB of LDS
+  AggProg Q = P;
+  set_spill_layout(Q, F);
+  // window: C home slots + the probe overflow margin, in as much LDS as a
+  // workgroup may take (fewer, larger partitions: longer runs per partition
+  // in the scatter's writes)
+  int optin = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, j->ctx->device) !=
+          hipSuccess || optin < 64 * 1024) {
+    (void)hipGetLastError();
+    optin = 64 * 1024;
+  }
+  // two workgroups a CU measured faster than one with twice the window
+  const uint64_t lds_cap = std::min<uint64_t>((uint64_t)std::min(optin, 160 * 1024) - 1024,
+                                              80 * 1024);
   const uint64_t per_slot = 8 + 4 + (uint64_t)WL * 8;
   uint64_t C = 1;
-  while (C * 2 <= j->cap && (C * 2 + kJMargin) * per_slot <= 80 * 1024) C *= 2;
-  if ((C + kJMargin) * per_slot > 80 * 1024 && C > 1) C /= 2;
+  while (C * 2 <= j->cap && (C * 2 + kJMargin) * per_slot <= lds_cap) C *= 2;
+  if ((C + kJMargin) * per_slot > lds_cap && C > 1) C /= 2;
   const int log2p = (int)(j->log2cap - __builtin_ctzll(C));
   const uint32_t np = 1u << log2p;
   const int Wn = (int)std::min<uint64_t>(C + kJMargin, j->cap);
   const size_t lds = (size_t)Wn * per_slot;
-  if (lds > 80 * 1024) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the join window");
+  if (lds > lds_cap) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the join window");
   KG_HIP(hipFuncSetAttribute((const void *)k_jp_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
   const int64_t fill = (int64_t)(j->ctx->num_cus > 0 ? j->ctx->num_cus : 256) * 2;
@@ -891,7 +935,7 @@ static void joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   KG_HIP(hipStreamSynchronize(j->s));
   const uint64_t live = lb + lc;
   const unsigned grid = (unsigned)std::min<int64_t>(np, fill);
-  k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, P, F, rows.get<uint64_t>(), live, base.get<uint64_t>(),
+  k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, Q, F, rows.get<uint64_t>(), live, base.get<uint64_t>(),
                                              (uint32_t)nb, log2p, (int)C, Wn, j->rec.get<uint64_t>());
   KG_HIP(hipGetLastError());
   KG_HIP(hipStreamSynchronize(j->s));

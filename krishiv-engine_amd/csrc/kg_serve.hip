@@ -44,7 +44,8 @@ struct SReq {
   uint64_t cap;
   int32_t log2cap;
   int32_t own;
-  uint64_t *rec;  // (cap + 1) x {count, sum lo, sum hi}
+  uint64_t *cnt;  // (cap + 1) pair counts (the output scans this array only)
+  uint64_t *sum;  // (cap + 1) x {sum lo, sum hi}
   int64_t *out_key, *out_sum, *out_cnt;
 };
 
@@ -86,9 +87,9 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_init(SDesc *D) {
   for_tiles(D, 0, [&](const SReq &r, int, int64_t lo, int64_t hi) {
     for (int64_t s = lo + threadIdx.x; s < hi; s += blockDim.x) {
       if (r.own) r.slots[s] = SSlot{kEmpty, 0};
-      r.rec[3 * s] = 0;
-      r.rec[3 * s + 1] = 0;
-      r.rec[3 * s + 2] = 0;
+      r.cnt[s] = 0;
+      r.sum[2 * s] = 0;
+      r.sum[2 * s + 1] = 0;
     }
   });
 }
@@ -140,26 +141,35 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_probe(SDesc *D) {
       if (s < 0) continue;
       uint64_t m = r.slots[s].rows;
       if (!m) continue;
-      uint64_t *w = r.rec + 3 * s;
-      add_u64_nr<false>(w, m);
+      add_u64_nr<false>(r.cnt + s, m);
       int64_t v = r.ppay[i];
       // pay * m as 128 bits (two's complement)
       uint64_t plo = (uint64_t)v * m;
       uint64_t phi = __umul64hi((uint64_t)v, m) - (v < 0 ? m : 0);
-      add128<false>(w + 1, plo, phi);
+      add128<false>(r.sum + 2 * s, plo, phi);
     }
   });
 }
 
+// Output positions are taken once per wave (a ballot and one atomic on the
+// request's counter), not once per group.
 __global__ void __launch_bounds__(kSrvBlock) k_srv_output(SDesc *D) {
   for_tiles(D, 3, [&](const SReq &r, int q, int64_t lo, int64_t hi) {
-    for (int64_t s = lo + threadIdx.x; s < hi; s += blockDim.x) {
-      const uint64_t *w = r.rec + 3 * s;
-      if (!w[0]) continue;
-      uint64_t o = atomicAdd(D->groups + q, 1ull);
+    for (int64_t s0 = lo; s0 < hi; s0 += blockDim.x) {
+      const int64_t s = s0 + threadIdx.x;
+      const bool has = s < hi && r.cnt[s] != 0;
+      const uint64_t m = __ballot(has);
+      if (!m) continue;
+      const int lane = __lane_id();
+      const int lead = __ffsll((unsigned long long)m) - 1;
+      unsigned long long base = 0;
+      if (lane == lead) base = atomicAdd(D->groups + q, (unsigned long long)__popcll(m));
+      base = __shfl(base, lead);
+      if (!has) continue;
+      const uint64_t o = base + __popcll(m & ((1ull << lane) - 1));
       r.out_key[o] = s == (int64_t)r.cap ? (int64_t)kEmpty : (int64_t)r.slots[s].key;
-      r.out_sum[o] = (int64_t)w[1];  // SUM(bigint) wraps (non-ANSI)
-      r.out_cnt[o] = (int64_t)w[0];
+      r.out_sum[o] = (int64_t)r.sum[2 * s];  // SUM(bigint) wraps (non-ANSI)
+      r.out_cnt[o] = (int64_t)r.cnt[s];
     }
   });
 }
@@ -187,6 +197,7 @@ struct kg_server {
     DevBuf slots;
     uint64_t cap = 0;
     int log2cap = 0;
+    kg_hash_join *join = nullptr;  // the same keys for large requests
   };
   std::vector<Resident> tables;
   struct Pending {
@@ -195,6 +206,7 @@ struct kg_server {
     uint64_t bytes = 0;
   };
   std::deque<Pending> queue;
+  std::deque<kg_serve_request> large;  // probes of kLargeProbe rows or more
   struct Slot {
     hipStream_t s = nullptr;
     DevBuf arena;          // tables, records, outputs of the batch
@@ -215,6 +227,59 @@ struct kg_server {
 };
 
 namespace kg {
+
+// Requests with probes this long run alone through the radix-partitioned
+// join aggregate (kg_joinagg_*): their time is the HBM traffic of their rows,
+// which the batch's tile kernels (one global atomic per matching row) would
+// only slow down.  Smaller requests are packed into batches.
+constexpr int64_t kLargeProbe = 4 << 20;
+
+static void ck(int rc) {
+  if (rc != KG_OK) throw Error{rc, std::string(kg_last_error())};
+}
+
+static kg_column dev_i64(const void *p, int64_t n, const uint8_t *valid = nullptr) {
+  kg_column c{};
+  c.type = KG_T_INT64;
+  c.flags = KG_COL_DEVICE;
+  c.length = n;
+  c.values = p;
+  c.validity = valid;
+  return c;
+}
+
+static kg_hash_join *build_join(kg_ctx *ctx, const int64_t *keys, int64_t n) {
+  kg_join_spec spec{};
+  spec.join_type = KG_JOIN_INNER;
+  spec.num_keys = 1;
+  kg_hash_join *j = nullptr;
+  ck(kg_join_create(ctx, &spec, (uint64_t)(n > 0 ? n : 1), &j));
+  std::unique_ptr<kg_hash_join, void (*)(kg_hash_join *)> guard(j, kg_join_destroy);
+  if (n > 0) {
+    kg_column k = dev_i64(keys, n);
+    ck(kg_join_build_sink(j, 1, &k, n));
+  }
+  ck(kg_join_build_finalize(j));
+  return guard.release();
+}
+
+// One large request, start to finish (the partitioned fused aggregate).
+static kg_batch *run_large(kg_server *S, const kg_serve_request &r) {
+  const bool own = r.build_keys != nullptr;
+  kg_hash_join *j = own ? build_join(S->ctx, r.build_keys, r.build_rows) : S->tables[r.table].join;
+  std::unique_ptr<kg_hash_join, void (*)(kg_hash_join *)> guard(own ? j : nullptr,
+                                                                 kg_join_destroy);
+  kg_column it = dev_i64(nullptr, 0);
+  kg_agg_spec aggs[2] = {{KG_AGG_SUM, 0}, {KG_AGG_COUNT_STAR, 0}};
+  ck(kg_joinagg_create(j, 1, &it, 2, aggs, 0));
+  kg_column cols[2] = {dev_i64(r.probe_keys, r.probe_rows, r.probe_valid),
+                       dev_i64(r.probe_pay, r.probe_rows)};
+  int32_t ic[1] = {1};
+  ck(kg_joinagg_probe(j, 2, cols, r.probe_rows, ic));
+  kg_batch *b = nullptr;
+  ck(kg_joinagg_result(j, &b));
+  return b;
+}
 
 static uint64_t req_cap(kg_server *S, const kg_serve_request &r) {
   if (!r.build_keys) {
@@ -316,8 +381,10 @@ static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
     } else {
       s.slots = S->tables[r.table].slots.get<SSlot>();
     }
-    s.rec = reinterpret_cast<uint64_t *>(arena + at);
-    at += (s.cap + 1) * 24;
+    s.cnt = reinterpret_cast<uint64_t *>(arena + at);
+    at += (s.cap + 1) * 8;
+    s.sum = reinterpret_cast<uint64_t *>(arena + at);
+    at += (s.cap + 1) * 16;
     s.out_key = reinterpret_cast<int64_t *>(arena + at);
     at += (s.cap + 1) * 8;
     s.out_sum = reinterpret_cast<int64_t *>(arena + at);
@@ -456,7 +523,8 @@ int kg_server_add_table(kg_server *S, const int64_t *keys, int64_t n, int32_t *t
     r.cap = t.cap;
     r.log2cap = t.log2cap;
     r.own = 1;
-    r.rec = scratch.get<uint64_t>();
+    r.cnt = scratch.get<uint64_t>();
+    r.sum = r.cnt + (t.cap + 1);
     memcpy(h.data(), &D, sizeof(SDesc));
     memcpy(h.data() + sizeof(SDesc), &r, sizeof(SReq));
     memcpy(h.data() + sizeof(SDesc) + sizeof(SReq), tiles[0].data(), sizeof(STile) * tiles[0].size());
@@ -470,6 +538,7 @@ int kg_server_add_table(kg_server *S, const int64_t *keys, int64_t n, int32_t *t
     k_srv_build<<<grid, kSrvBlock, 0, st>>>(d.get<SDesc>());
     KG_HIP(hipGetLastError());
     KG_HIP(hipStreamSynchronize(st));
+    t.join = build_join(S->ctx, keys, n);
     *table_id = (int32_t)S->tables.size();
     S->tables.push_back(std::move(t));
   });
@@ -480,6 +549,10 @@ int kg_server_submit(kg_server *S, const kg_serve_request *r) {
     if (!r->probe_keys || !r->probe_pay) fail(KG_ERR_INTERNAL, "request without probe columns");
     if (r->build_keys == nullptr && (r->table < 0 || r->table >= (int32_t)S->tables.size()))
       fail(KG_ERR_INTERNAL, "request names no resident table");
+    if (r->probe_rows >= kLargeProbe) {
+      S->large.push_back(*r);
+      return;
+    }
     kg_server::Pending p;
     p.r = *r;
     p.cap = req_cap(S, *r);
@@ -510,6 +583,14 @@ int kg_server_progress(kg_server *S, int32_t wait, int64_t *done_ids, int32_t ma
       }
       for (auto &sl : S->slots)
         if (!sl.busy && !S->queue.empty()) launch_batch(S, sl);
+      // one large request, while the batches run on the device
+      if (!S->large.empty() && *ndone < max) {
+        kg_serve_request r = S->large.front();
+        S->large.pop_front();
+        S->results[r.id] = run_large(S, r);
+        done_ids[(*ndone)++] = r.id;
+        break;
+      }
       if (!wait || waited || S->inflight == 0) break;
     }
   });
@@ -525,7 +606,7 @@ int kg_server_take_result(kg_server *S, int64_t id, kg_batch **out) {
 }
 
 int64_t kg_server_pending(const kg_server *S) {
-  return S ? (int64_t)S->queue.size() + S->inflight : 0;
+  return S ? (int64_t)S->queue.size() + S->inflight + (int64_t)S->large.size() : 0;
 }
 
 void kg_server_destroy(kg_server *S) {
@@ -537,6 +618,7 @@ void kg_server_destroy(kg_server *S) {
     if (sl.desc_host) (void)hipHostFree(sl.desc_host);
   }
   for (auto &kv : S->results) kg_batch_release(kv.second);
+  for (auto &t : S->tables) kg_join_destroy(t.join);
   hipStream_t ss[kSrvSlots];
   for (int i = 0; i < kSrvSlots; i++) ss[i] = S->slots[i].s;
   delete S;

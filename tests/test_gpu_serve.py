@@ -1,7 +1,8 @@
 """GPU parity of the batched join-aggregate server (kg_server_*): every
 request's (key, SUM(pay), COUNT(*)) equals the oracle's inner join of its
 build and probe columns aggregated per build key, whatever batch it lands
-in, with own and resident build tables, graph replay on or off."""
+in, with own and resident build tables, graph replay on or off, small
+requests batched and large ones on the partitioned path."""
 import numpy as np
 import pytest
 
@@ -61,7 +62,9 @@ def test_mixed_batch_own_and_resident_tables(use_graph):
         res_keys.append(k)
         res_dev.append((srv.add_table(d, nb), d))
     keep, want = [], {}
-    sizes = [1000, 4096, 1, 20000, 250, 70000, 9000, 1000, 3]
+    # requests 9 and 11 probe 4.5M and 5M rows: they run through the
+    # partitioned fused aggregate (own build, resident table)
+    sizes = [1000, 4096, 1, 20000, 250, 70000, 9000, 1000, 3, 450_000, 2000, 500_000]
     for rid, nb in enumerate(sizes):
         if rid % 3 == 2:
             t = rid % 2
