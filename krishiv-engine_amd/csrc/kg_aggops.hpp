@@ -474,7 +474,7 @@ __device__ __forceinline__ void merge_record(const AggProg &P, uint64_t *g, cons
 }
 
 __device__ __forceinline__ uint64_t slot_hash(const Table &T, uint64_t key) {
-  return dev::finalize(dev::row_hash1(key), T.seed);
+  return dev::table_hash(key, T.seed);
 }
 
 // Find-or-insert of `key` in the global table (never full: the host keeps

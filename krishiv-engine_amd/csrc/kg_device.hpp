@@ -170,6 +170,19 @@ __device__ __forceinline__ uint64_t row_hash2(uint64_t a, uint64_t b) {
 // finalize (layout.rs:1080-1082): g = fold(h ^ seed, FIN)
 __device__ __forceinline__ uint64_t finalize(uint64_t h, uint64_t seed) { return fold(h ^ seed, L_FIN); }
 
+// Placement hash of this library's own device tables (join directories,
+// aggregate tables): the SplitMix64 finalizer (Steele, Lea and Flood,
+// "Fast splittable pseudorandom number generators", 2014) of key ^ seed.
+// Slot placement is internal to a table and never visible in results, so it
+// need not be the reference's row hash; this one costs two 64-bit
+// multiplies where h and g take three 64x64->128 products.
+__device__ __forceinline__ uint64_t table_hash(uint64_t key, uint64_t seed) {
+  uint64_t z = key ^ seed;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
 // ---------------------------------------------------------------- SBBF
 __device__ __forceinline__ uint32_t sbbf_salt(int w) {
   // bloom.rs:15-24 (the Parquet specification's salts)

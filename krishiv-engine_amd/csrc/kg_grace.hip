@@ -28,7 +28,7 @@ struct GSlot {
 constexpr uint64_t kGEmpty = 0x8000000000000000ULL;
 
 __device__ __forceinline__ uint64_t gslot_start(uint64_t key, int log2cap) {
-  return dev::finalize(dev::row_hash1(key), 0x51ED27A1D3C4B9E5ULL) >> (64 - log2cap);
+  return dev::table_hash(key, 0x51ED27A1D3C4B9E5ULL) >> (64 - log2cap);
 }
 
 // Lookup table of the build partition's per-key aggregates (keys are

@@ -74,7 +74,7 @@ __device__ __forceinline__ int pack_key(const KeySpec &K, int64_t i, uint64_t &k
 }
 
 __device__ __forceinline__ uint64_t jslot_start(const JTable &T, uint64_t key) {
-  return dev::finalize(dev::row_hash1(key), T.seed) >> (64 - T.log2cap);
+  return dev::table_hash(key, T.seed) >> (64 - T.log2cap);
 }
 
 __global__ void k_join_init(Slot *slots, uint64_t n) {
@@ -259,13 +259,30 @@ constexpr int kJMargin = 128;
 constexpr int kJTile = 4;
 
 __device__ __forceinline__ uint64_t jhash(const JTable &T, uint64_t key) {
-  return dev::finalize(dev::row_hash1(key), T.seed);
+  return dev::table_hash(key, T.seed);
 }
 
 __device__ __forceinline__ uint32_t jpart(const JTable &T, uint64_t key, int log2p) {
   return log2p ? (uint32_t)(jhash(T, key) >> (64 - log2p)) : 0u;
 }
 
+// Key of probe row i for the partition kernels.  FAST: one 8-byte
+// integer-like key column, whose value is the packed key.
+template <bool FAST>
+__device__ __forceinline__ int jp_key(const KeySpec &K, int64_t i, uint64_t &key) {
+  if (FAST) {
+    const DCol &d = K.c[0];
+    if (!dev::valid(d, i)) {
+      key = 0;
+      return K.null_safe ? 2 : 3;
+    }
+    key = static_cast<const uint64_t *>(d.values)[i + d.offset];
+    return key == kEmpty ? 1 : 0;
+  }
+  return pack_key(K, i, key);
+}
+
+template <bool FAST>
 __global__ void k_jp_hist(KeySpec K, int64_t n, JTable T, int log2p, int64_t per_block,
                           uint32_t *hist) {
   extern __shared__ uint32_t s_h[];
@@ -280,7 +297,7 @@ __global__ void k_jp_hist(KeySpec K, int64_t n, JTable T, int log2p, int64_t per
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int64_t i = i0 + r * (int64_t)blockDim.x;
-      live[r] = i < hi && pack_key(K, i, key[r]) == 0;
+      live[r] = i < hi && jp_key<FAST>(K, i, key[r]) == 0;
     }
 #pragma unroll
     for (int r = 0; r < 4; r++)
@@ -312,6 +329,7 @@ __device__ __forceinline__ void jp_value(const DCol &d, int64_t i, bool &v, uint
   }
 }
 
+template <bool FAST>
 __global__ void k_jp_scatter(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt F, int log2p,
                              int64_t per_block, const uint64_t *base, uint64_t *out,
                              uint64_t *rec) {
@@ -333,7 +351,7 @@ __global__ void k_jp_scatter(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int64_t i = i0 + r * (int64_t)blockDim.x;
-      cls4[r] = i < hi ? pack_key(K, i, keys[r]) : 3;
+      cls4[r] = i < hi ? jp_key<FAST>(K, i, keys[r]) : 3;
       vb[r] = 0;
 #pragma unroll
       for (int c = 0; c < kJPre; c++) {
@@ -383,6 +401,158 @@ __global__ void k_jp_scatter(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt
       }
       if (F.vword) row[1] = vbits;
     }
+  }
+}
+
+// Staged form of k_jp_scatter for up to 2048 partitions and narrow rows: a
+// workgroup sorts a tile of kJSBlock * R rows by partition in LDS (counts,
+// block scan, staged rows), then writes each partition's run of the tile
+// with consecutive lanes on consecutive rows, so a wave's stores cover a few
+// contiguous runs instead of 64 scattered rows.  Rows keep their input order
+// within a partition.  FAST: one 8-byte key column and at most kJPre
+// one-word inputs of 8 bytes, loaded before the count atomics.
+constexpr int kJSBlock = 512;
+constexpr uint32_t kJSMaxParts = 2048;
+
+// Exclusive scan of c[0..np) into s (np <= 4 * kJSBlock); returns the total.
+__device__ __forceinline__ uint32_t js_block_scan(const uint32_t *c, uint32_t *s, uint32_t np,
+                                                  uint32_t *wsum) {
+  constexpr int PER = 4;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint32_t x[PER], tot = 0;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint32_t p = (uint32_t)t * PER + j;
+    x[j] = p < np ? c[p] : 0;
+    tot += x[j];
+  }
+  uint32_t inc = tot;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t woff = 0, all = 0;
+  for (int w = 0; w < kJSBlock / 64; w++) {
+    const uint32_t ws = wsum[w];
+    if (w < wv) woff += ws;
+    all += ws;
+  }
+  uint32_t run = woff + inc - tot;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint32_t p = (uint32_t)t * PER + j;
+    if (p < np) s[p] = run;
+    run += x[j];
+  }
+  __syncthreads();
+  return all;
+}
+
+__host__ __device__ constexpr size_t js_lds_bytes(int R, int rw, uint32_t np) {
+  return (size_t)kJSBlock * R * rw * 8 + (size_t)np * 16 + (size_t)kJSBlock * R * 2 +
+         kJSBlock / 64 * 4;
+}
+
+template <int R, bool FAST>
+__global__ void __launch_bounds__(kJSBlock)
+    k_jp_scatter_staged(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt F, int log2p,
+                        int64_t per_block, const uint64_t *base, uint64_t *out, uint64_t *rec) {
+  constexpr int TT = kJSBlock * R;
+  extern __shared__ uint64_t smem[];
+  const uint32_t np = 1u << log2p;
+  const int rw = F.rw;
+  uint64_t *st = smem;                                        // word w of staged row s: st[w * TT + s]
+  uint64_t *cur = st + (size_t)rw * TT;                       // np: next output row of the block
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(cur + np);     // np: rows of the tile
+  uint32_t *start = cnt + np;                                 // np: first staged row
+  uint16_t *st_p = reinterpret_cast<uint16_t *>(start + np);  // TT: partition of staged row s
+  uint32_t *wsum = reinterpret_cast<uint32_t *>(st_p + TT);   // kJSBlock / 64
+  for (uint32_t p = threadIdx.x; p < np; p += kJSBlock)
+    cur[p] = base[(uint64_t)p * gridDim.x + blockIdx.x];
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = lo + per_block < n ? lo + per_block : n;
+  const int nv = F.ncols;
+  for (int64_t t0 = lo; t0 < hi; t0 += TT) {
+    for (uint32_t p = threadIdx.x; p < np; p += kJSBlock) cnt[p] = 0;
+    __syncthreads();
+    uint64_t keys[R];
+    uint32_t pp[R], lp[R];
+    uint64_t val[R][FAST ? kJPre : 1];
+    uint64_t vb[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t i = t0 + (int64_t)r * kJSBlock + threadIdx.x;
+      const int cls = i < hi ? jp_key<FAST>(K, i, keys[r]) : 3;
+      pp[r] = cls == 0 ? jpart(T, keys[r], log2p) : 0xffffffffu;
+      vb[r] = 0;
+      if (FAST) {
+#pragma unroll
+        for (int c = 0; c < kJPre; c++) {
+          val[r][c] = 0;
+          if (c < nv && cls == 0) {
+            const DCol &d = P.in[c];
+            if (dev::valid(d, i)) {
+              val[r][c] = static_cast<const uint64_t *>(d.values)[i + d.offset];
+              vb[r] |= 1ull << c;
+            }
+          }
+        }
+      }
+      if (cls == 1 || cls == 2) {  // fixed slots (rare): atomics
+        const uint64_t s = T.cap + (cls == 2 ? 1 : 0);
+        const uint32_t m = T.slots[s].count;
+        if (m) apply_row_mult(P, rec + s * (uint64_t)P.W, i, m);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) lp[r] = pp[r] != 0xffffffffu ? atomicAdd(&cnt[pp[r]], 1u) : 0u;
+    __syncthreads();
+    const uint32_t tn = js_block_scan(cnt, start, np, wsum);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (pp[r] == 0xffffffffu) continue;
+      const uint32_t s = start[pp[r]] + lp[r];
+      st_p[s] = (uint16_t)pp[r];
+      st[s] = keys[r];
+      if (FAST) {
+#pragma unroll
+        for (int c = 0; c < kJPre; c++)
+          if (c < nv) st[(size_t)F.off[c] * TT + s] = val[r][c];
+      } else {
+        const int64_t i = t0 + (int64_t)r * kJSBlock + threadIdx.x;
+        for (int c = 0; c < nv; c++) {
+          bool v;
+          uint64_t l, h;
+          jp_value(P.in[c], i, v, l, h);
+          if (v) vb[r] |= 1ull << c;
+          st[(size_t)F.off[c] * TT + s] = l;
+          if (P.in[c].type == KG_T_DECIMAL128) st[(size_t)(F.off[c] + 1) * TT + s] = h;
+        }
+      }
+      if (F.vword) st[TT + s] = vb[r];
+    }
+    __syncthreads();
+    if (rw == 2) {
+      for (uint32_t s = threadIdx.x; s < tn; s += kJSBlock) {
+        const uint32_t p = st_p[s];
+        const uint64_t d = cur[p] + (s - start[p]);
+        const uint64_t a = st[s], b = st[TT + s];
+        reinterpret_cast<uint4 *>(out)[d] =
+            make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+      }
+    } else {
+      for (uint32_t s = threadIdx.x; s < tn; s += kJSBlock) {
+        const uint32_t p = st_p[s];
+        uint64_t *row = out + (cur[p] + (s - start[p])) * (uint64_t)rw;
+        for (int w = 0; w < rw; w++) row[w] = st[(size_t)w * TT + s];
+      }
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < np; p += kJSBlock) cur[p] += cnt[p];
+    __syncthreads();
   }
 }
 
@@ -865,8 +1035,10 @@ int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_t
 namespace kg {
 
 // The partitioned probe of the fused join aggregate (k_jp_hist, k_jp_scatter,
-// k_jp_merge).
-static void joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg &P,
+// k_jp_merge).  Returns false (nothing launched) when the table needs more
+// partitions than a workgroup can count in LDS; the caller then probes with
+// global atomics.
+static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg &P,
                                 int64_t nrows) {
   const int W = P.W, WL = W | 1;
   const int ninputs = (int)j->agg_types.size();
@@ -883,9 +1055,13 @@ static void joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   F.rw = w;
   AggProg Q = P;
   set_spill_layout(Q, F);
-  // window: C home slots + the probe overflow margin, in as much LDS as a
-  // workgroup may take (fewer, larger partitions: longer runs per partition
-  // in the scatter's writes)
+  // one 8-byte integer-like key and one-word 8-byte inputs: the loads need
+  // no type dispatch
+  bool fast = K.n == 1 && type_width(K.c[0].type) == 8 && !is_float(K.c[0].type) &&
+              ninputs <= kJPre;
+  for (int c = 0; c < ninputs; c++)
+    if (type_width(P.in[c].type) != 8) fast = false;
+  // window: C home slots + the probe overflow margin
   int optin = 0;
   if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, j->ctx->device) !=
           hipSuccess || optin < 64 * 1024) {
@@ -901,44 +1077,90 @@ static void joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   if ((C + kJMargin) * per_slot > lds_cap && C > 1) C /= 2;
   const int log2p = (int)(j->log2cap - __builtin_ctzll(C));
   const uint32_t np = 1u << log2p;
+  if (np > 16384) return false;  // the per-block counters would not fit in LDS
   const int Wn = (int)std::min<uint64_t>(C + kJMargin, j->cap);
   const size_t lds = (size_t)Wn * per_slot;
   if (lds > lds_cap) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the join window");
   KG_HIP(hipFuncSetAttribute((const void *)k_jp_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
-  const int64_t fill = (int64_t)(j->ctx->num_cus > 0 ? j->ctx->num_cus : 256) * 2;
-  int64_t per_block = std::max<int64_t>(4096, std::min<int64_t>((int64_t)np * 32, ceil_div(nrows, fill)));
-  int64_t nb = ceil_div(nrows, per_block);
-  per_block = ceil_div(nrows, nb);
+  const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
+  // staged scatter: tiles of kJSBlock * R rows sorted in LDS
+  const int R = F.rw <= 2 ? 12 : F.rw <= 3 ? 8 : 4;
+  const size_t slds = js_lds_bytes(R, F.rw, np);
+  const bool staged = np <= kJSMaxParts && F.rw <= 6 && slds + 1024 <= (size_t)optin;
+  int64_t per_block;
+  if (staged) {
+    const int64_t tt = (int64_t)kJSBlock * R;
+    per_block = ceil_div(ceil_div(nrows, cus * 4), tt) * tt;
+  } else {
+    per_block = std::max<int64_t>(4096, std::min<int64_t>((int64_t)np * 32, ceil_div(nrows, cus * 2)));
+  }
+  const int64_t nb = ceil_div(nrows, per_block);
+  if (!staged) per_block = ceil_div(nrows, nb);
   const uint64_t hn = (uint64_t)np * nb;
   DevBuf hist(j->ctx, hn * 4 + 4), base(j->ctx, hn * 8 + 8);
   DevBuf rows(j->ctx, (uint64_t)nrows * F.rw * 8 + 8);
   JTable T = jtable(j);
   ProfScope prof("joinagg_probe", j->s);
-  k_jp_hist<<<(unsigned)nb, kJPBlock, np * 4, j->s>>>(K, nrows, T, log2p, per_block,
-                                                      hist.get<uint32_t>());
-  KG_HIP(hipGetLastError());
+  {
+    ProfScope ph("jp_hist", j->s);
+    if (fast)
+      k_jp_hist<true><<<(unsigned)nb, kJPBlock, np * 4, j->s>>>(K, nrows, T, log2p, per_block,
+                                                                hist.get<uint32_t>());
+    else
+      k_jp_hist<false><<<(unsigned)nb, kJPBlock, np * 4, j->s>>>(K, nrows, T, log2p, per_block,
+                                                                 hist.get<uint32_t>());
+    KG_HIP(hipGetLastError());
+  }
   size_t tb = 0;
   KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist.get<uint32_t>(), base.get<uint64_t>(),
                                           (int)hn, j->s));
   DevBuf temp(j->ctx, tb + 16);
   KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, hist.get<uint32_t>(), base.get<uint64_t>(),
                                           (int)hn, j->s));
-  k_jp_scatter<<<(unsigned)nb, kJPBlock, np * 4, j->s>>>(K, nrows, T, P, F, log2p, per_block,
-                                                         base.get<uint64_t>(), rows.get<uint64_t>(),
-                                                         j->rec.get<uint64_t>());
-  KG_HIP(hipGetLastError());
+  {
+    ProfScope ps("jp_scatter", j->s);
+    const uint64_t *bp = base.get<uint64_t>();
+    uint64_t *rp = rows.get<uint64_t>(), *rec = j->rec.get<uint64_t>();
+    const unsigned g = (unsigned)nb;
+    if (staged) {
+      const void *fn;
+#define KG_JS(RR, FF)                                                                          \
+  fn = (const void *)k_jp_scatter_staged<RR, FF>;                                              \
+  KG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));     \
+  k_jp_scatter_staged<RR, FF><<<g, kJSBlock, slds, j->s>>>(K, nrows, T, P, F, log2p, per_block, \
+                                                           bp, rp, rec)
+      if (fast) {
+        if (R == 12) { KG_JS(12, true); } else if (R == 8) { KG_JS(8, true); } else { KG_JS(4, true); }
+      } else {
+        if (R == 12) { KG_JS(12, false); } else if (R == 8) { KG_JS(8, false); } else { KG_JS(4, false); }
+      }
+#undef KG_JS
+    } else if (fast) {
+      k_jp_scatter<true><<<g, kJPBlock, np * 4, j->s>>>(K, nrows, T, P, F, log2p, per_block, bp, rp,
+                                                        rec);
+    } else {
+      k_jp_scatter<false><<<g, kJPBlock, np * 4, j->s>>>(K, nrows, T, P, F, log2p, per_block, bp, rp,
+                                                         rec);
+    }
+    KG_HIP(hipGetLastError());
+  }
   uint64_t lb = 0;
   uint32_t lc = 0;
   KG_HIP(hipMemcpyAsync(&lb, base.get<uint64_t>() + hn - 1, 8, hipMemcpyDeviceToHost, j->s));
   KG_HIP(hipMemcpyAsync(&lc, hist.get<uint32_t>() + hn - 1, 4, hipMemcpyDeviceToHost, j->s));
   KG_HIP(hipStreamSynchronize(j->s));
   const uint64_t live = lb + lc;
-  const unsigned grid = (unsigned)std::min<int64_t>(np, fill);
-  k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, Q, F, rows.get<uint64_t>(), live, base.get<uint64_t>(),
-                                             (uint32_t)nb, log2p, (int)C, Wn, j->rec.get<uint64_t>());
-  KG_HIP(hipGetLastError());
+  const unsigned grid = (unsigned)std::min<int64_t>(np, cus * 2);
+  {
+    ProfScope pm("jp_merge", j->s);
+    k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, Q, F, rows.get<uint64_t>(), live,
+                                               base.get<uint64_t>(), (uint32_t)nb, log2p, (int)C,
+                                               Wn, j->rec.get<uint64_t>());
+    KG_HIP(hipGetLastError());
+  }
   KG_HIP(hipStreamSynchronize(j->s));
+  return true;
 }
 
 }  // namespace kg
@@ -961,13 +1183,11 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
       P.in[c] = d[ic];
     }
     if (nrows >= (1ll << 32)) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^32 rows or more");
-    if (nrows < (1 << 20)) {
+    if (nrows < (1 << 20) || !joinagg_partitioned(j, K, P, nrows)) {
       ProfScope prof("joinagg_probe", j->s);
       k_joinagg_probe<<<stream_grid(nrows, 256), 256, 0, j->s>>>(K, nrows, jtable(j), P,
                                                                  j->rec.get<uint64_t>());
       KG_HIP(hipGetLastError());
-    } else {
-      joinagg_partitioned(j, K, P, nrows);
     }
     if (st.staged_host()) KG_HIP(hipStreamSynchronize(j->s));
   });

@@ -65,7 +65,7 @@ struct SDesc {
 };
 
 __device__ __forceinline__ uint64_t sslot_start(uint64_t key, int log2cap) {
-  return dev::finalize(dev::row_hash1(key), 0x3C6EF372FE94F82BULL) >> (64 - log2cap);
+  return dev::table_hash(key, 0x3C6EF372FE94F82BULL) >> (64 - log2cap);
 }
 
 // Claims tiles of phase `ph` until none is left (every workgroup exits).

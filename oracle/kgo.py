@@ -350,7 +350,7 @@ class Agg:
             raise ArithmeticError(error())
         res = []
         for lo, hi, f, v, (t, p, s) in bufs:
-            if t == T_FLOAT64:
+            if t in (T_FLOAT64, T_FLOAT32):  # float32 MIN/MAX come back widened
                 vals = f[:g].copy()
             elif t in (T_DECIMAL64, T_DECIMAL128):
                 vals = [((int(h) << 64) | (int(l) & 0xFFFFFFFFFFFFFFFF)) for l, h in zip(lo[:g], hi[:g])]

@@ -182,16 +182,22 @@ def test_empty_probe_and_no_match():
     assert j.probe([Col(A.T_INT64, np.array([7, 8]))]).num_rows == 0
 
 
-@pytest.mark.parametrize("nb_keys,nb_rows", [(5000, 20000), (300_000, 300_000)])
+# build sizes: 128 and 2048 partitions (LDS-staged scatter), 4096 (direct
+# scatter), 32768 (more than the partition kernels count: global atomics)
+@pytest.mark.parametrize("nb_keys,nb_rows", [(5000, 20000), (300_000, 300_000),
+                                             (600_000, 600_000), (4_000_000, 4_300_000)])
 def test_partitioned_fused_aggregate(nb_keys, nb_rows):
     """Probe batches of a million rows and more take the radix-partitioned
     fused aggregate (LDS windows of the build table): duplicate build keys
     (the row count multiplies), NULL probe keys and NULL inputs, the key equal
-    to the table's sentinel, a small table (one window) and a large one."""
+    to the table's sentinel, a small table (one window) and larger ones."""
     say(f"join: partitioned fused aggregate, {nb_rows} build rows over {nb_keys} keys")
     rng = np.random.default_rng(nb_keys)
     bk = rng.integers(0, nb_keys, nb_rows).astype(np.int64)
-    bk[::997] = np.iinfo(np.int64).min
+    # at most ~300 build rows hold the sentinel key: its group gets ~1300 probe
+    # rows times that many pairs, and the oracle's float sum over them in pair
+    # order must stay within the tolerance of a sum in any other order
+    bk[::max(997, nb_rows // 300)] = np.iinfo(np.int64).min
     npr = 1_300_000
     pk = rng.integers(0, int(nb_keys * 1.2), npr).astype(np.int64)
     pk[::1009] = np.iinfo(np.int64).min
@@ -230,3 +236,57 @@ def test_partitioned_fused_aggregate(nb_keys, nb_rows):
             assert np.all(np.abs(a - b) <= 1e-12 * np.maximum(np.abs(a), np.abs(b)) + 1e-300), c
         else:
             assert np.array_equal(rv[rok].astype(np.int64), gv[rok].astype(np.int64)), c
+
+
+def _groups(keys, kvalid, cols):
+    """{key or None: tuple of (value or None)} of an aggregate result."""
+    out = {}
+    for g in range(len(keys)):
+        k = int(keys[g]) if kvalid[g] else None
+        out[k] = tuple((v[g] if ok[g] else None) for v, ok in cols)
+    return out
+
+
+def test_partitioned_fused_aggregate_type_dispatch():
+    """The partitioned fused aggregate on rows whose loads need the type
+    dispatch: an int32 `<=>` key with NULLs on both sides (the NULL key's
+    fixed slot, applied with atomics), int16 and float32 inputs with NULLs."""
+    say("join: partitioned fused aggregate, int32 null-safe key, int16/float32 inputs")
+    rng = np.random.default_rng(77)
+    nb, npr = 40_000, 1_200_000
+    bk = rng.integers(-30000, 30000, nb).astype(np.int32)
+    bkv = rng.random(nb) > 0.001
+    pk = rng.integers(-33000, 33000, npr).astype(np.int32)
+    pkv = rng.random(npr) > 0.01
+    a16 = rng.integers(-30000, 30000, npr).astype(np.int16)
+    a16v = rng.random(npr) > 0.05
+    f32 = rng.standard_normal(npr).astype(np.float32)
+    f32v = rng.random(npr) > 0.05
+    op, ob = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT32, pk, pkv)],
+                      [kgo.Col(kgo.T_INT32, bk, bkv)], null_safe=[1])
+    aggs = [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0), (kgo.AGG_MIN, 1), (kgo.AGG_MAX, 1),
+            (kgo.AGG_SUM, 1), (kgo.AGG_AVG, 0), (kgo.AGG_COUNT, 1)]
+    o = kgo.Agg((kgo.T_INT32, 0, 0), [(kgo.T_INT16, 0, 0), (kgo.T_FLOAT32, 0, 0)], aggs)
+    o.update(kgo.Col(kgo.T_INT32, bk[ob], bkv[ob]),
+             [kgo.Col(kgo.T_INT16, a16[op], a16v[op]), kgo.Col(kgo.T_FLOAT32, f32[op], f32v[op])])
+    rk, rkv, rres = o.finish()
+    want = _groups(rk, rkv, [(np.asarray(v), np.asarray(ok)) for v, ok, _ in rres])
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0], [1], expected_build_rows=nb)
+    j.build_sink([Col(A.T_INT32, bk, bkv)])
+    j.build_finalize()
+    j.aggregate([(A.T_INT16, 0, 0), (A.T_FLOAT32, 0, 0)], aggs)
+    j.probe_aggregate([Col(A.T_INT32, pk, pkv), Col(A.T_INT16, a16, a16v),
+                       Col(A.T_FLOAT32, f32, f32v)], [1, 2])
+    r = j.aggregate_result()
+    gk, gkv = r.column(0)
+    got = _groups(gk, gkv, [r.column(1 + c) for c in range(len(aggs))])
+    assert None in want and set(got) == set(want)
+    for k, wv in want.items():
+        for c, (a, b) in enumerate(zip(wv, got[k])):
+            assert (a is None) == (b is None), (k, c)
+            if a is None:
+                continue
+            if isinstance(a, (float, np.floating)):
+                assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)) + 1e-300, (k, c, a, b)
+            else:
+                assert int(a) == int(b), (k, c, a, b)

@@ -439,6 +439,18 @@ def test_float_min_max_nan_is_largest():
     assert res[0][0][0] == -2.0 and math.isnan(res[1][0][0])
 
 
+def test_float32_min_max_per_group_match_numpy():
+    rng = np.random.default_rng(4)
+    keys = rng.integers(0, 50, 5000).astype(np.int64)
+    v = rng.standard_normal(5000).astype(np.float32)
+    a = kgo.Agg((kgo.T_INT64, 0, 0), [(kgo.T_FLOAT32, 0, 0)], [(kgo.AGG_MIN, 0), (kgo.AGG_MAX, 0)])
+    a.update(kgo.Col(kgo.T_INT64, keys), [kgo.Col(kgo.T_FLOAT32, v)])
+    rk, _, res = a.finish()
+    for g, k in enumerate(rk):
+        sel = v[keys == k]
+        assert res[0][0][g] == float(sel.min()) and res[1][0][g] == float(sel.max())
+
+
 def test_global_aggregate_on_empty_input_is_one_row_of_nulls():
     a = kgo.Agg(None, [(kgo.T_INT64, 0, 0)], [(kgo.AGG_COUNT_STAR, 0), (kgo.AGG_SUM, 0)])
     a.update(None, [kgo.Col(kgo.T_INT64, [1])])  # a global aggregate has its group once seen
