@@ -104,6 +104,53 @@ def cpu_baseline(args, aggs):
                       f"threads (per-thread tables merged at the end); {dt:.2f} s"}
 
 
+def cpu_joinagg():
+    """The oracle (C restatement of the reference's join and aggregate, one
+    thread) on SELECT key, SUM(amt), COUNT(*) over 1M build rows and the
+    first 8M probe rows of the same generators: the reference's own
+    CPU-runnable case of this workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import kgo
+    nb, n = 1_000_000, 8_000_000
+    bk, _ = kgo.gen_join_build(1, 0, nb, nb)
+    fk, ok, amt = kgo.gen_join_probe(2, 0, n, 1_200_000, 10)
+    t0 = time.perf_counter()
+    pi, bi = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, fk, ok)], [kgo.Col(kgo.T_INT64, bk)],
+                      cap=n + 16)
+    a = kgo.Agg((kgo.T_INT64, 0, 0), [(kgo.T_INT64, 0, 0)], [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0)])
+    a.update(kgo.Col(kgo.T_INT64, bk[bi]), [kgo.Col(kgo.T_INT64, amt[pi])])
+    a.finish()
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"1M build rows and the first 8M probe rows of the same generators "
+                      f"(configuration 1's exact shape); oracle hash join + aggregate, one thread; "
+                      f"{dt:.2f} s"}
+
+
+def cpu_bloom(nbytes):
+    """The oracle on the Bloom workload's first 20M probe rows: filter built
+    from the same 15M build rows (val < 50000), H_kg of each probe key, SBBF
+    check, then the survivors' join and SUM(amt), COUNT(*); one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import kgo
+    nb, n = 15_000_000, 20_000_000
+    bk, bv = kgo.gen_join_build(4, 0, nb, nb)
+    fk, ok, amt = kgo.gen_join_probe(5, 0, n, nb, 0)
+    t0 = time.perf_counter()
+    kept = bk[bv < 50_000]
+    _, words = kgo.sbbf_build(kgo.hash_columns([kgo.Col(kgo.T_INT64, kept)]), nbytes=nbytes)
+    hit = kgo.sbbf_check_batch(words, kgo.hash_columns([kgo.Col(kgo.T_INT64, fk)]))
+    sel = np.nonzero(hit)[0]
+    pi, bi = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, fk[sel])], [kgo.Col(kgo.T_INT64, kept)])
+    a = kgo.Agg(None, [(kgo.T_INT64, 0, 0)], [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0)])
+    a.update(None, [kgo.Col(kgo.T_INT64, amt[sel][pi])])
+    a.finish()
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"the same 15M build rows and the first 20M probe rows; oracle H_kg, SBBF "
+                      f"build/check, join and aggregate, one thread; {dt:.2f} s"}
+
+
 def joinagg(args):
     """SELECT key, SUM(amt), COUNT(*) FROM build JOIN probe ON key = fk GROUP
     BY key: build 1M rows (key a seeded permutation of 0..1M-1), probe rows
@@ -149,24 +196,32 @@ def joinagg(args):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     kgpu.prof_enable(False)
-    pr_ms, pr_n = kgpu.prof_get("joinagg_probe")
-    in_bytes = n * 16 + n // 8  # fk, amt, fk validity
-    avg = pr_ms / max(pr_n, 1)
-    per_launch = in_bytes / max(pr_n / args.steps, 1)
-    achieved = per_launch / (avg / 1e3) if pr_ms > 0 else None
+    sc_ms, sc_n = kgpu.prof_get("jp_scatter")
+    in_bytes = n * 16 + n // 8  # fk, amt, fk validity: each read once
+    out_bytes = groups[0] * 24 + nb * 8  # (key, sum, count) rows; the build keys
+    avg = sc_ms / max(sc_n, 1)
+    per_launch = in_bytes / max(sc_n / args.steps, 1)
+    achieved = per_launch / (avg / 1e3) if sc_ms > 0 else None
+    step_ms = dt * 1e3 / args.steps
     print(json.dumps({
         "metric": "join probe rows/s (fused GROUP BY build key; 1M build rows)",
         "value": n * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic (seeded counter-based generator, generated in HBM)",
         "config": {"workload": f"SELECT key, SUM(amt), COUNT(*) join-aggregate, 1000000 build rows, "
                                f"{n} probe rows, fk uniform [0, 1.2M), 1% NULL", "groups": groups[0]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
-                     "kernel": "k_joinagg_probe", "kernel_avg_ms": avg,
-                     "launches_per_step": pr_n / args.steps},
+                     "frac": achieved / HBM_PEAK if achieved else None,
+                     "traffic": traffic_of("k_jp_scatter_staged_joinagg_200M_traffic",
+                                           int(per_launch // 16.125) if sc_n else -1),
+                     "kernel": "k_jp_scatter_staged", "kernel_avg_ms": avg,
+                     "launches_per_step": sc_n / args.steps,
+                     "algorithmic_bytes_per_step": in_bytes},
+        "step_hbm_frac": ((in_bytes + out_bytes) / (step_ms / 1e3)) / HBM_PEAK,
+        "phases": phases(["join_build", "jp_hist", "jp_scatter", "jp_merge", "agg_output"]),
+        **({} if args.no_cpu else {"cpu_baseline": cpu_joinagg()}),
     }), flush=True)
 
 
@@ -248,6 +303,7 @@ def bloom(args):
                      "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
                      "kernel": "bloom probe (k_bloom_probe_bits + expand)", "kernel_avg_ms": avg},
         "families_ms_per_step": {f: v[0] / args.steps for f, v in fams.items()},
+        **({} if args.no_cpu else {"cpu_baseline": cpu_bloom(nbytes)}),
     }), flush=True)
 
 
@@ -382,19 +438,39 @@ def serve(args):
     }), flush=True)
 
 
-def traffic_of(kernel: str, launch_rows: int):
-    """HBM bytes per launch of `kernel` from its committed counter summary
-    (profiles/<kernel>_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per
-    dispatch from two rocprofv3 --pmc runs of the same workload, scaled by
-    what the counters read for a 1 GiB device copy), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+def traffic_of(summary: str, launch_rows: int):
+    """HBM bytes per launch from a committed counter summary
+    (profiles/<summary>.md: `key: value` lines rows_per_launch, fetch_kib and
+    write_kib per dispatch from two rocprofv3 --pmc runs of the same workload,
+    fetch_scale and write_scale from what those counters read for a 1 GiB
+    device copy), or None when absent or taken at another size."""
+    path = os.path.join(ROOT, "profiles", summary + ".md")
     if not os.path.exists(path):
         return None
+    t = {}
     with open(path) as f:
-        t = json.load(f)
-    if t.get("rows_per_launch") != launch_rows:
+        for line in f:
+            parts = line.strip().split(": ")
+            if len(parts) == 2 and parts[0].isidentifier():
+                try:
+                    t[parts[0]] = float(parts[1])
+                except ValueError:
+                    pass
+    if int(t.get("rows_per_launch", -1)) != launch_rows:
         return None
     return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024
+
+
+def phases(names):
+    """Average ms per launch of the library's timed regions (HIP events on
+    the stream the kernels run on)."""
+    import kgpu
+    out = {}
+    for name in names:
+        ms, cnt = kgpu.prof_get(name)
+        if cnt:
+            out[name] = {"avg_ms": ms / cnt, "launches": cnt}
+    return out
 
 
 def main():
@@ -470,7 +546,7 @@ def main():
     dt = time.perf_counter() - t0
     kgpu.prof_enable(False)
     upd_ms, upd_n = kgpu.prof_get("agg_update")
-    out_ms, out_n = kgpu.prof_get("agg_output")
+    ph = phases(["agg_update", "agg_partition", "agg_merge", "agg_output"])
     ms_per_step = dt * 1e3 / args.steps
     rows_per_s = n * args.steps / dt
     in_bytes = n * (key_bytes + 8 + 8 + 4)  # every input column read once
@@ -500,14 +576,19 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None,
-                     "traffic": traffic_of("k_agg_pass1", int(bytes_per_launch // (key_bytes + 20)))
+                     "traffic": traffic_of("k_agg_pass1_hash_agg_200M_traffic",
+                                           int(bytes_per_launch // (key_bytes + 20)))
                      if args.key == "int64" else None,
                      "kernel": "k_agg_pass1" if args.key == "int64" else "k_agg_update",
                      "kernel_avg_ms": upd_avg_ms,
                      "launches_per_step": launches_per_step,
                      "algorithmic_bytes_per_step": in_bytes},
-        "step_hbm_frac": (in_bytes / (ms_per_step / 1e3)) / HBM_PEAK,
-        "output_kernel_ms": out_ms / max(out_n, 1),
+        # whole operator: inputs read once + result rows written once (key,
+        # COUNT(*), and SUM 16 B, MIN, MAX, AVG of decimal(12,2), float64, int32:
+        # 112 B a group)
+        "step_hbm_frac": ((in_bytes + groups[0] * (112 if args.key == "int64" else 105))
+                          / (ms_per_step / 1e3)) / HBM_PEAK,
+        "phases": ph,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, [(1, 0)] + [(f, c) for c in range(3)

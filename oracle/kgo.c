@@ -458,6 +458,12 @@ int kgo_sbbf_check(const uint32_t *words, size_t nblocks, uint64_t h) {
   return miss == 0;
 }
 
+/* check_batch (bloom.rs:203-247): one result byte per hash */
+void kgo_sbbf_check_batch(const uint32_t *words, size_t nblocks, const uint64_t *h, int64_t n,
+                          uint8_t *out) {
+  for (int64_t i = 0; i < n; i++) out[i] = (uint8_t)kgo_sbbf_check(words, nblocks, h[i]);
+}
+
 /* expected_fpp (bloom.rs:270-308) */
 double kgo_sbbf_expected_fpp(size_t nblocks, uint64_t items) {
   double lambda = (double)items / (double)nblocks;

@@ -100,6 +100,8 @@ size_t kgo_sbbf_bytes_for(uint64_t items, double fpp);
 size_t kgo_sbbf_blocks_for_bytes(size_t bytes);
 void kgo_sbbf_insert(uint32_t *words, size_t nblocks, uint64_t h);
 int kgo_sbbf_check(const uint32_t *words, size_t nblocks, uint64_t h);
+void kgo_sbbf_check_batch(const uint32_t *words, size_t nblocks, const uint64_t *h, int64_t n,
+                          uint8_t *out);
 double kgo_sbbf_expected_fpp(size_t nblocks, uint64_t items);
 /* filter.rs:606-624 `bloom()`: build from hashes; returns 1 if kept (expected
  * fpp <= max_fpp), 0 if dropped.  bytes==0 sizes it from the hash count. */

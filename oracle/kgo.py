@@ -96,6 +96,8 @@ def lib():
         L.kgo_sbbf_insert.argtypes = [vp, C.c_size_t, u64]
         L.kgo_sbbf_check.restype = C.c_int
         L.kgo_sbbf_check.argtypes = [vp, C.c_size_t, u64]
+        L.kgo_sbbf_check_batch.restype = None
+        L.kgo_sbbf_check_batch.argtypes = [vp, C.c_size_t, vp, C.c_int64, vp]
         L.kgo_sbbf_expected_fpp.restype = C.c_double
         L.kgo_sbbf_expected_fpp.argtypes = [C.c_size_t, u64]
         L.kgo_sbbf_build.restype = C.c_int
@@ -257,6 +259,14 @@ def sbbf_build(hashes: np.ndarray, nbytes: int = 0, fpp: float = 0.01,
 def sbbf_check(words: np.ndarray, h: int) -> bool:
     words = np.ascontiguousarray(words, dtype=np.uint32)
     return bool(lib().kgo_sbbf_check(_ptr(words), len(words) // 8, h))
+
+
+def sbbf_check_batch(words: np.ndarray, hashes: np.ndarray) -> np.ndarray:
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    hashes = np.ascontiguousarray(hashes, dtype=np.uint64)
+    out = np.zeros(len(hashes), dtype=np.uint8)
+    lib().kgo_sbbf_check_batch(_ptr(words), len(words) // 8, _ptr(hashes), len(hashes), _ptr(out))
+    return out.astype(bool)
 
 
 # --- generators ----------------------------------------------------------

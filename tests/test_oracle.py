@@ -208,6 +208,10 @@ def test_sbbf_sizing_and_false_positives():
     probe = kgo.hash_columns([kgo.Col(kgo.T_UINT64, np.arange(n, 2 * n, dtype=np.uint64))])
     fp = sum(kgo.sbbf_check(words, int(x)) for x in probe)
     assert fp < 1600
+    batch = kgo.sbbf_check_batch(words, probe)
+    assert batch.sum() == fp and all(batch[i] == kgo.sbbf_check(words, int(probe[i]))
+                                     for i in range(0, n, 97))
+    assert kgo.sbbf_check_batch(words, h).all()
     nb = len(words) // 8
     est = L.kgo_sbbf_expected_fpp(nb, n)
     assert 0.005 < est < 0.016
