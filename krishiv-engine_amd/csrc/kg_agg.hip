@@ -1877,6 +1877,7 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
     a->out.direct = a->direct;
     a->out.global = !a->has_key;
     a->s = make_stream(ctx->device);
+    StreamScope kg_scope(a->s);
     int W = a->prog.W;
     // the largest LDS table (records + keys) the device grants a workgroup:
     // 64 KiB always, more where the dynamic-LDS attribute is accepted
@@ -1946,11 +1947,12 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
 }
 
 int kg_agg_sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, int64_t nrows) {
-  return guarded([&] { sink(a, key, inputs, nrows); });
+  return guarded([&] { StreamScope kg_scope(a->s); sink(a, key, inputs, nrows); });
 }
 
 int kg_agg_finalize(kg_hash_agg *a) {
   return guarded([&] {
+    StreamScope kg_scope(a->s);
     if (a->finalized) return;
     drain(a);
     a->ngroups = group_positions(a->ctx, a->rec.get<uint64_t>(), nslots(a), a->prog.W,
@@ -1971,6 +1973,7 @@ int kg_agg_result_type(const kg_hash_agg *a, int32_t i, kg_column *out_type) {
 
 int kg_agg_result(kg_hash_agg *a, kg_batch **out) {
   return guarded([&] {
+    StreamScope kg_scope(a->s);
     *out = nullptr;
     if (!a->finalized) fail(KG_ERR_INTERNAL, "result before finalize");
     auto b = std::make_unique<kg_batch>();

@@ -154,6 +154,7 @@ int kg_bloom_create(kg_ctx *ctx, uint64_t nbytes, kg_bloom **out) {
     if (b->nblocks < 1) b->nblocks = 1;
     if (b->nblocks > (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "Bloom filter of 2^32 blocks or more");
     b->s = make_stream(ctx->device);
+    StreamScope kg_scope(b->s);
     b->words.reset(ctx, b->nblocks * 32);
     b->items.reset(ctx, 8);
     KG_HIP(hipMemsetAsync(b->words.get(), 0, b->nblocks * 32, b->s));
@@ -165,6 +166,7 @@ int kg_bloom_create(kg_ctx *ctx, uint64_t nbytes, kg_bloom **out) {
 
 int kg_bloom_insert(kg_bloom *b, const kg_column *key, const kg_column *select, int64_t nrows) {
   return guarded([&] {
+    StreamScope kg_scope(b->s);
     if (nrows <= 0) return;
     check_column(*key, nrows, "Bloom key");
     Staged st(b->ctx, b->s);
@@ -211,6 +213,7 @@ double kg_bloom_expected_fpp(const kg_bloom *b, uint64_t items) {
 
 int kg_bloom_export(kg_bloom *b, uint8_t *bytes, uint64_t *len) {
   return guarded([&] {
+    StreamScope kg_scope(b->s);
     uint64_t need = b->nblocks * 32;
     if (!bytes) {
       *len = need;
@@ -226,6 +229,7 @@ int kg_bloom_export(kg_bloom *b, uint8_t *bytes, uint64_t *len) {
 int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *out_rows,
                    int64_t *out_count) {
   return guarded([&] {
+    StreamScope kg_scope(b->s);
     *out_count = 0;
     if (nrows <= 0) return;
     if (nrows > INT32_MAX) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^31 rows or more");

@@ -63,6 +63,21 @@ int guarded(F &&f) {
 }  // namespace kg
 
 // ---------------------------------------------------------------- context
+namespace kg {
+// An event recorded on a stream; destroyed with its last holder.
+class Fence {
+ public:
+  explicit Fence(hipStream_t s);
+  ~Fence();
+  bool done() const;
+  Fence(const Fence &) = delete;
+  Fence &operator=(const Fence &) = delete;
+
+ private:
+  hipEvent_t e_ = nullptr;
+};
+}  // namespace kg
+
 struct kg_ctx {
   int device = 0;
   uint64_t budget = 0;
@@ -74,12 +89,43 @@ struct kg_ctx {
   std::mutex cache_mu;
   std::multimap<uint64_t, void *> cache;
   uint64_t cached = 0;
+  // blocks freed with stream order: reusable once every event completes
+  struct Pending {
+    uint64_t rb;
+    void *p;
+    std::vector<std::shared_ptr<kg::Fence>> waits;
+  };
+  std::vector<Pending> pending;
+  // events after the last enqueued reads of caller-owned device columns
+  std::vector<std::shared_ptr<kg::Fence>> use_fences;
 };
 
 namespace kg {
 
 void *dev_alloc(kg_ctx *ctx, uint64_t bytes);
-void dev_free(kg_ctx *ctx, void *p, uint64_t bytes);
+// Frees a block.  With a stream, the block is reused only after the work
+// enqueued on that stream so far, and every read of caller columns enqueued
+// so far (use fences), has completed; without one, after a device
+// synchronize.
+void dev_free(kg_ctx *ctx, void *p, uint64_t bytes, hipStream_t stream = nullptr);
+// Records that work reading caller-owned device columns has been enqueued
+// on stream s (blocks freed later wait for it).
+void note_column_use(kg_ctx *ctx, hipStream_t s);
+
+// The stream whose work uses the device buffers allocated in the current
+// scope of this thread: an operator entry point sets its own stream, so the
+// buffers it allocates are freed in that stream's order.
+hipStream_t alloc_stream();
+class StreamScope {
+ public:
+  explicit StreamScope(hipStream_t s);
+  ~StreamScope();
+  StreamScope(const StreamScope &) = delete;
+  StreamScope &operator=(const StreamScope &) = delete;
+
+ private:
+  hipStream_t prev_;
+};
 
 // RAII device buffer charged to a context's HBM budget.
 class DevBuf {
@@ -101,6 +147,7 @@ class DevBuf {
     release();
     ctx_ = ctx;
     bytes_ = bytes;
+    s_ = alloc_stream();
     p_ = bytes ? dev_alloc(ctx, bytes) : nullptr;
   }
   // Keeps the buffer when it is already large enough.
@@ -108,7 +155,7 @@ class DevBuf {
     if (bytes > bytes_) reset(ctx, bytes + bytes / 4);
   }
   void release() {
-    if (p_) dev_free(ctx_, p_, bytes_);
+    if (p_) dev_free(ctx_, p_, bytes_, s_);
     p_ = nullptr;
     bytes_ = 0;
   }
@@ -129,10 +176,12 @@ class DevBuf {
     std::swap(ctx_, o.ctx_);
     std::swap(p_, o.p_);
     std::swap(bytes_, o.bytes_);
+    std::swap(s_, o.s_);
   }
   kg_ctx *ctx_ = nullptr;
   void *p_ = nullptr;
   uint64_t bytes_ = 0;
+  hipStream_t s_ = nullptr;  // the stream whose order frees it (none: device-wide)
 };
 
 // ---------------------------------------------------------------- columns
@@ -158,13 +207,19 @@ bool is_int_like(int t);  // integers, dates, timestamps, decimal64, bool
 // are used in place.
 class Staged {
  public:
-  Staged(kg_ctx *ctx, hipStream_t s) : ctx_(ctx), s_(s) {}
+  Staged(kg_ctx *ctx, hipStream_t s) : ctx_(ctx), s_(s), scope_(s) {}
+  // the caller's device columns are read by work enqueued on s_ by now
+  ~Staged() {
+    if (device_) note_column_use(ctx_, s_);
+  }
   DCol add(const kg_column &c, int64_t nrows);
   bool staged_host() const { return !bufs_.empty(); }
 
  private:
   kg_ctx *ctx_;
   hipStream_t s_;
+  StreamScope scope_;
+  bool device_ = false;
   std::vector<DevBuf> bufs_;
 };
 

@@ -290,6 +290,7 @@ int kg_grace_create(kg_ctx *ctx, uint32_t partitions, uint64_t table_budget_byte
     g->parts = partitions;
     g->table_budget = table_budget_bytes ? table_budget_bytes : (8ull << 30);
     g->s = make_stream(ctx->device);
+    StreamScope kg_scope(g->s);
     g->mm.reset(ctx, 32);
     long long init[4] = {INT64_MAX, INT64_MIN, INT64_MAX, INT64_MIN};
     KG_HIP(hipMemcpy(g->mm.get(), init, 32, hipMemcpyHostToDevice));
@@ -298,16 +299,17 @@ int kg_grace_create(kg_ctx *ctx, uint32_t partitions, uint64_t table_budget_byte
 }
 
 int kg_grace_build_sink(kg_grace *g, const kg_column *key, const kg_column *pay, int64_t nrows) {
-  return guarded([&] { sink_rows(g, g->build, key, pay, nrows, 0); });
+  return guarded([&] { StreamScope kg_scope(g->s); sink_rows(g, g->build, key, pay, nrows, 0); });
 }
 
 int kg_grace_probe_sink(kg_grace *g, const kg_column *key, const kg_column *pay, int64_t nrows) {
-  return guarded([&] { sink_rows(g, g->probe, key, pay, nrows, 1); });
+  return guarded([&] { StreamScope kg_scope(g->s); sink_rows(g, g->probe, key, pay, nrows, 1); });
 }
 
 int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_valid,
                     uint64_t *spilled_bytes) {
   return guarded([&] {
+    StreamScope kg_scope(g->s);
     long long mm[4];
     KG_HIP(hipMemcpy(mm, g->mm.get(), 32, hipMemcpyDeviceToHost));
     bool any_b = mm[0] <= mm[1], any_p = mm[2] <= mm[3];

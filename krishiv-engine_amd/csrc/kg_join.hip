@@ -575,9 +575,12 @@ __device__ __forceinline__ int2 window_probe(const JTable &T, const uint64_t *s_
 }
 
 __global__ void __launch_bounds__(kJMBlock) k_jp_merge(JTable T, AggProg P, SpillFmt F,
-                                                       const uint64_t *rows, uint64_t nrows,
+                                                       const uint64_t *rows,
+                                                       const uint32_t *hist, uint64_t hn,
                                                        const uint64_t *base, uint32_t nb,
                                                        int log2p, int C, int Wn, uint64_t *rec) {
+  // rows written by the scatter: the exclusive scan's last offset + count
+  const uint64_t nrows = base[hn - 1] + hist[hn - 1];
   extern __shared__ uint64_t smem[];
   const int W = P.W, WL = W | 1;
   uint64_t *s_keys = smem;
@@ -868,6 +871,7 @@ int kg_join_create(kg_ctx *ctx, const kg_join_spec *spec, uint64_t expected_buil
 
 int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows) {
   return guarded([&] {
+    StreamScope kg_scope(j->s);
     if (j->finalized) fail(KG_ERR_INTERNAL, "build sink after finalize");
     if (ncols < 1 || ncols > kJoinCols) fail(KG_ERR_UNSUPPORTED, "1 to 32 build columns");
     check_keys(j->spec, ncols, cols, j->spec.build_keys);
@@ -879,6 +883,7 @@ int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, in
 
 int kg_join_build_finalize(kg_hash_join *j) {
   return guarded([&] {
+    StreamScope kg_scope(j->s);
     if (j->finalized) return;
     int64_t n = j->build_rows;
     concat_build(j);
@@ -998,12 +1003,13 @@ static void probe_impl(kg_hash_join *j, int32_t ncols, const kg_column *cols, in
 
 int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                   kg_batch **out) {
-  return guarded([&] { probe_impl(j, ncols, cols, nrows, nullptr, 0, out); });
+  return guarded([&] { StreamScope kg_scope(j->s); probe_impl(j, ncols, cols, nrows, nullptr, 0, out); });
 }
 
 int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                            const int32_t *sel_rows, int64_t nsel, kg_batch **out) {
   return guarded([&] {
+    StreamScope kg_scope(j->s);
     if (nsel > 0 && !sel_rows) fail(KG_ERR_INTERNAL, "no selection vector");
     if (nsel > nrows) fail(KG_ERR_INTERNAL, "more selected rows than rows");
     probe_impl(j, ncols, cols, nrows, sel_rows, nsel, out);
@@ -1013,6 +1019,7 @@ int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols
 int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_types,
                       int32_t naggs, const kg_agg_spec *aggs, uint32_t flags) {
   return guarded([&] {
+    StreamScope kg_scope(j->s);
     if (!j->finalized) fail(KG_ERR_INTERNAL, "aggregate before the build finalized");
     if (j->spec.join_type != KG_JOIN_INNER || j->spec.num_keys != 1)
       fail(KG_ERR_UNSUPPORTED, "the fused aggregate needs an inner join on one key");
@@ -1145,21 +1152,15 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
     }
     KG_HIP(hipGetLastError());
   }
-  uint64_t lb = 0;
-  uint32_t lc = 0;
-  KG_HIP(hipMemcpyAsync(&lb, base.get<uint64_t>() + hn - 1, 8, hipMemcpyDeviceToHost, j->s));
-  KG_HIP(hipMemcpyAsync(&lc, hist.get<uint32_t>() + hn - 1, 4, hipMemcpyDeviceToHost, j->s));
-  KG_HIP(hipStreamSynchronize(j->s));
-  const uint64_t live = lb + lc;
   const unsigned grid = (unsigned)std::min<int64_t>(np, cus * 2);
   {
     ProfScope pm("jp_merge", j->s);
-    k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, Q, F, rows.get<uint64_t>(), live,
-                                               base.get<uint64_t>(), (uint32_t)nb, log2p, (int)C,
-                                               Wn, j->rec.get<uint64_t>());
+    k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, Q, F, rows.get<uint64_t>(), hist.get<uint32_t>(),
+                                               hn, base.get<uint64_t>(), (uint32_t)nb, log2p,
+                                               (int)C, Wn, j->rec.get<uint64_t>());
     KG_HIP(hipGetLastError());
   }
-  KG_HIP(hipStreamSynchronize(j->s));
+  // the temporaries are freed in this stream's order: no synchronize
   return true;
 }
 
@@ -1168,6 +1169,7 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
 int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                      const int32_t *input_cols) {
   return guarded([&] {
+    StreamScope kg_scope(j->s);
     if (!j->has_agg) fail(KG_ERR_INTERNAL, "no fused aggregate");
     if (nrows <= 0) return;
     check_keys(j->spec, ncols, cols, j->spec.probe_keys);
@@ -1195,6 +1197,7 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
 
 int kg_joinagg_result(kg_hash_join *j, kg_batch **out) {
   return guarded([&] {
+    StreamScope kg_scope(j->s);
     *out = nullptr;
     if (!j->has_agg) fail(KG_ERR_INTERNAL, "no fused aggregate");
     DevBuf pos;

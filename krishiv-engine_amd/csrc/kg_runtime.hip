@@ -26,11 +26,51 @@ void set_last_error(int code, const std::string &msg) {
   g_last_error = std::string(class_of(code)) + ": " + msg;
 }
 
-// Device memory with a per-context cache of freed blocks.  A block is
-// cached after a device synchronize (what hipFree would do), so no kernel
-// still uses it when it is handed out again; sizes are rounded (powers of
-// two up to 2 MiB, then multiples of 2 MiB) so blocks are reused across
-// operators of the same shapes.  The HBM budget counts live bytes.
+// Device memory with a per-context cache of freed blocks; sizes are rounded
+// (powers of two up to 2 MiB, then multiples of 2 MiB) so blocks are reused
+// across operators of the same shapes.  The HBM budget counts live bytes.
+//
+// A block may be handed out again only when no kernel still uses it.  A
+// buffer freed by an operator whose work all runs on one stream waits, as a
+// pending block, for an event recorded on that stream at the free, and for
+// the use fences then outstanding (reads of caller-owned device columns,
+// which the caller may free once the call returns, as the reference's
+// `Sink::sink` borrows its batch).  Other frees synchronize the device,
+// as hipFree would.
+Fence::Fence(hipStream_t s) {
+  if (hipEventCreateWithFlags(&e_, hipEventDisableTiming) != hipSuccess) {
+    e_ = nullptr;
+    (void)hipGetLastError();
+    (void)hipDeviceSynchronize();  // no event: order by a full synchronize
+    return;
+  }
+  if (hipEventRecord(e_, s) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipEventDestroy(e_);
+    e_ = nullptr;
+    (void)hipDeviceSynchronize();
+  }
+}
+
+Fence::~Fence() {
+  if (e_) (void)hipEventDestroy(e_);
+}
+
+bool Fence::done() const {
+  if (!e_) return true;
+  hipError_t q = hipEventQuery(e_);
+  if (q == hipErrorNotReady) return false;
+  if (q != hipSuccess) (void)hipGetLastError();
+  return true;
+}
+
+static thread_local hipStream_t g_alloc_stream = nullptr;
+
+hipStream_t alloc_stream() { return g_alloc_stream; }
+
+StreamScope::StreamScope(hipStream_t s) : prev_(g_alloc_stream) { g_alloc_stream = s; }
+StreamScope::~StreamScope() { g_alloc_stream = prev_; }
+
 static uint64_t round_alloc(uint64_t b) {
   if (b <= (2u << 20)) {
     uint64_t r = 256;
@@ -40,11 +80,61 @@ static uint64_t round_alloc(uint64_t b) {
   return (b + (2u << 20) - 1) / (2u << 20) * (2u << 20);
 }
 
+// (cache_mu held) moves the pending blocks whose events have completed (all
+// of them with `all`, after a device synchronize) into the cache, keeping
+// at most a quarter of the device's memory cached.
+static void reclaim_locked(kg_ctx *ctx, bool all) {
+  size_t keep = 0;
+  for (size_t i = 0; i < ctx->pending.size(); i++) {
+    auto &pb = ctx->pending[i];
+    bool ready = all;
+    if (!ready) {
+      ready = true;
+      for (auto &f : pb.waits)
+        if (!f->done()) {
+          ready = false;
+          break;
+        }
+    }
+    if (!ready) {
+      if (keep != i) ctx->pending[keep] = std::move(pb);
+      keep++;
+      continue;
+    }
+    if (ctx->cached + pb.rb > ctx->total_hbm / 4) {
+      (void)hipFree(pb.p);
+    } else {
+      ctx->cache.emplace(pb.rb, pb.p);
+      ctx->cached += pb.rb;
+    }
+  }
+  ctx->pending.resize(keep);
+  size_t k = 0;
+  for (size_t i = 0; i < ctx->use_fences.size(); i++)
+    if (!ctx->use_fences[i]->done()) ctx->use_fences[k++] = ctx->use_fences[i];
+  ctx->use_fences.resize(k);
+}
+
 static void cache_release_all(kg_ctx *ctx) {
+  (void)hipDeviceSynchronize();
   std::lock_guard<std::mutex> g(ctx->cache_mu);
+  reclaim_locked(ctx, true);
   for (auto &kv : ctx->cache) (void)hipFree(kv.second);
   ctx->cache.clear();
   ctx->cached = 0;
+}
+
+void note_column_use(kg_ctx *ctx, hipStream_t s) {
+  if (!ctx) return;
+  auto f = std::make_shared<Fence>(s);
+  std::lock_guard<std::mutex> g(ctx->cache_mu);
+  if (ctx->use_fences.size() >= 64) {  // drop the completed ones
+    size_t k = 0;
+    for (size_t i = 0; i < ctx->use_fences.size(); i++)
+      if (!ctx->use_fences[i]->done()) ctx->use_fences[k++] = ctx->use_fences[i];
+    ctx->use_fences.resize(k);
+  }
+  ctx->use_fences.push_back(std::move(f));
 }
 
 void *dev_alloc(kg_ctx *ctx, uint64_t bytes) {
@@ -60,6 +150,10 @@ void *dev_alloc(kg_ctx *ctx, uint64_t bytes) {
   {
     std::lock_guard<std::mutex> g(ctx->cache_mu);
     auto it = ctx->cache.find(rb);
+    if (it == ctx->cache.end() && !ctx->pending.empty()) {
+      reclaim_locked(ctx, false);
+      it = ctx->cache.find(rb);
+    }
     if (it != ctx->cache.end()) {
       void *p = it->second;
       ctx->cache.erase(it);
@@ -82,7 +176,7 @@ void *dev_alloc(kg_ctx *ctx, uint64_t bytes) {
   return p;
 }
 
-void dev_free(kg_ctx *ctx, void *p, uint64_t bytes) {
+void dev_free(kg_ctx *ctx, void *p, uint64_t bytes, hipStream_t stream) {
   if (!p) return;
   if (!ctx) {
     (void)hipFree(p);
@@ -90,15 +184,24 @@ void dev_free(kg_ctx *ctx, void *p, uint64_t bytes) {
   }
   const uint64_t rb = round_alloc(bytes);
   ctx->in_use.fetch_sub(rb);
-  (void)hipDeviceSynchronize();
-  std::lock_guard<std::mutex> g(ctx->cache_mu);
-  // keep at most a quarter of the device's memory cached
-  if (ctx->cached + rb > ctx->total_hbm / 4) {
-    (void)hipFree(p);
+  if (!stream) {
+    (void)hipDeviceSynchronize();
+    std::lock_guard<std::mutex> g(ctx->cache_mu);
+    reclaim_locked(ctx, true);
+    if (ctx->cached + rb > ctx->total_hbm / 4) {
+      (void)hipFree(p);
+      return;
+    }
+    ctx->cache.emplace(rb, p);
+    ctx->cached += rb;
     return;
   }
-  ctx->cache.emplace(rb, p);
-  ctx->cached += rb;
+  auto f = std::make_shared<Fence>(stream);
+  std::lock_guard<std::mutex> g(ctx->cache_mu);
+  kg_ctx::Pending pb{rb, p, {std::move(f)}};
+  for (auto &u : ctx->use_fences)
+    if (!u->done()) pb.waits.push_back(u);
+  ctx->pending.push_back(std::move(pb));
 }
 
 int type_width(int t) {
@@ -148,6 +251,7 @@ DCol Staged::add(const kg_column &c, int64_t nrows) {
     d.offset = c.offset;
     d.values = c.values;
     d.validity = c.validity;
+    device_ = true;
     return d;
   }
   // Host column: copy the rows [offset, offset + nrows) (bit-packed buffers
@@ -175,15 +279,40 @@ DCol Staged::add(const kg_column &c, int64_t nrows) {
   return d;
 }
 
+// Streams are pooled per device and never destroyed: a freed buffer may
+// still order its reuse by an event on the stream of an operator that is
+// gone, so a stream handle stays valid for the life of the process.
+namespace {
+std::mutex g_stream_mu;
+std::map<int, std::vector<hipStream_t>> g_stream_pool;
+std::map<hipStream_t, int> g_stream_device;
+}  // namespace
+
 hipStream_t make_stream(int device) {
   KG_HIP(hipSetDevice(device));
+  {
+    std::lock_guard<std::mutex> g(g_stream_mu);
+    auto &v = g_stream_pool[device];
+    if (!v.empty()) {
+      hipStream_t s = v.back();
+      v.pop_back();
+      return s;
+    }
+  }
   hipStream_t s;
   KG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  std::lock_guard<std::mutex> g(g_stream_mu);
+  g_stream_device[s] = device;
   return s;
 }
 
 void destroy_stream(hipStream_t s) {
-  if (s) (void)hipStreamDestroy(s);
+  if (!s) return;
+  (void)hipStreamSynchronize(s);
+  std::lock_guard<std::mutex> g(g_stream_mu);
+  auto it = g_stream_device.find(s);
+  if (it == g_stream_device.end()) return;
+  g_stream_pool[it->second].push_back(s);
 }
 
 // ---------------------------------------------------------------- profiling

@@ -17,8 +17,10 @@
 // let the next batch form and launch while one runs.
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <deque>
 #include <map>
+#include <thread>
 
 #include "kg_aggops.hpp"
 
@@ -206,7 +208,25 @@ struct kg_server {
     uint64_t bytes = 0;
   };
   std::deque<Pending> queue;
-  std::deque<kg_serve_request> large;  // probes of kLargeProbe rows or more
+  // Probes of kLargeProbe rows or more run one at a time on a worker
+  // thread (their own streams), so batches keep launching meanwhile.
+  struct Large {
+    kg_serve_request r;
+    kg_hash_join *table = nullptr;  // the resident table's join, if any
+  };
+  struct LargeDone {
+    int64_t id;
+    kg_batch *b;
+    int code;
+    std::string msg;
+  };
+  std::mutex large_mu;
+  std::condition_variable large_cv;
+  std::deque<Large> large;
+  std::deque<LargeDone> large_done;
+  int64_t large_outstanding = 0;  // queued + running (main thread's count)
+  bool large_stop = false;
+  std::thread large_worker;
   struct Slot {
     hipStream_t s = nullptr;
     DevBuf arena;          // tables, records, outputs of the batch
@@ -233,6 +253,7 @@ namespace kg {
 // which the batch's tile kernels (one global atomic per matching row) would
 // only slow down.  Smaller requests are packed into batches.
 constexpr int64_t kLargeProbe = 4 << 20;
+constexpr int64_t kBatchProbeRows = 8 << 20;
 
 static void ck(int rc) {
   if (rc != KG_OK) throw Error{rc, std::string(kg_last_error())};
@@ -264,9 +285,9 @@ static kg_hash_join *build_join(kg_ctx *ctx, const int64_t *keys, int64_t n) {
 }
 
 // One large request, start to finish (the partitioned fused aggregate).
-static kg_batch *run_large(kg_server *S, const kg_serve_request &r) {
+static kg_batch *run_large(kg_server *S, const kg_serve_request &r, kg_hash_join *table) {
   const bool own = r.build_keys != nullptr;
-  kg_hash_join *j = own ? build_join(S->ctx, r.build_keys, r.build_rows) : S->tables[r.table].join;
+  kg_hash_join *j = own ? build_join(S->ctx, r.build_keys, r.build_rows) : table;
   std::unique_ptr<kg_hash_join, void (*)(kg_hash_join *)> guard(own ? j : nullptr,
                                                                  kg_join_destroy);
   kg_column it = dev_i64(nullptr, 0);
@@ -279,6 +300,52 @@ static kg_batch *run_large(kg_server *S, const kg_serve_request &r) {
   kg_batch *b = nullptr;
   ck(kg_joinagg_result(j, &b));
   return b;
+}
+
+static void large_loop(kg_server *S) {
+  (void)hipSetDevice(S->ctx->device);
+  while (true) {
+    kg_server::Large L;
+    {
+      std::unique_lock<std::mutex> g(S->large_mu);
+      S->large_cv.wait(g, [&] { return S->large_stop || !S->large.empty(); });
+      if (S->large.empty()) return;  // stopping
+      L = S->large.front();
+      S->large.pop_front();
+    }
+    kg_server::LargeDone d{L.r.id, nullptr, KG_OK, {}};
+    try {
+      d.b = run_large(S, L.r, L.table);
+    } catch (const Error &e) {
+      d.code = e.code;
+      d.msg = e.msg;
+    } catch (const std::exception &e) {
+      d.code = KG_ERR_INTERNAL;
+      d.msg = e.what();
+    }
+    std::lock_guard<std::mutex> g(S->large_mu);
+    S->large_done.push_back(std::move(d));
+    S->large_cv.notify_all();
+  }
+}
+
+// Moves finished large requests into the results (waiting for one when
+// `block`); a failed one is reported by this call.
+static void collect_large(kg_server *S, bool block, int64_t *done_ids, int32_t max,
+                          int32_t *ndone) {
+  std::unique_lock<std::mutex> g(S->large_mu);
+  if (block) S->large_cv.wait(g, [&] { return !S->large_done.empty(); });
+  while (!S->large_done.empty() && *ndone < max) {
+    kg_server::LargeDone d = std::move(S->large_done.front());
+    S->large_done.pop_front();
+    S->large_outstanding--;
+    if (d.code != KG_OK) {
+      g.unlock();
+      fail(d.code, "request " + std::to_string(d.id) + ": " + d.msg);
+    }
+    S->results[d.id] = d.b;
+    done_ids[(*ndone)++] = d.id;
+  }
 }
 
 static uint64_t req_cap(kg_server *S, const kg_serve_request &r) {
@@ -327,15 +394,21 @@ static void capture_graph(kg_server *S, kg_server::Slot &sl) {
 // Forms and launches one batch on a free slot from the head of the queue.
 static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
   if (S->queue.empty()) return false;
+  StreamScope kg_scope(sl.s);
   sl.reqs.clear();
   sl.sreqs.clear();
   uint64_t used = 0;
+  int64_t rows = 0;
   while (!S->queue.empty() && (int32_t)sl.reqs.size() < S->max_batch) {
     auto &p = S->queue.front();
     if (used + p.bytes > S->arena_bytes) {
       if (sl.reqs.empty()) fail(KG_ERR_MEMORY, "a request larger than the server's arena");
       break;
     }
+    // a batch completes as a whole: bound its probe rows so that a small
+    // request does not wait long for the largest one beside it
+    if (!sl.reqs.empty() && rows + p.r.probe_rows > kBatchProbeRows) break;
+    rows += p.r.probe_rows;
     used += p.bytes;
     sl.reqs.push_back(p);
     S->queue.pop_front();
@@ -433,6 +506,7 @@ static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
 
 static void retire(kg_server *S, kg_server::Slot &sl, int64_t *done_ids, int32_t max,
                    int32_t *ndone) {
+  StreamScope kg_scope(sl.s);
   std::vector<unsigned long long> groups(sl.reqs.size());
   uint8_t *gdev = sl.desc_dev.get<uint8_t>() + sizeof(SDesc) + sizeof(SReq) * S->max_batch +
                   4 * sizeof(STile) * sl.max_tiles;
@@ -550,7 +624,12 @@ int kg_server_submit(kg_server *S, const kg_serve_request *r) {
     if (r->build_keys == nullptr && (r->table < 0 || r->table >= (int32_t)S->tables.size()))
       fail(KG_ERR_INTERNAL, "request names no resident table");
     if (r->probe_rows >= kLargeProbe) {
-      S->large.push_back(*r);
+      kg_server::Large L{*r, r->build_keys ? nullptr : S->tables[r->table].join};
+      std::lock_guard<std::mutex> g(S->large_mu);
+      if (!S->large_worker.joinable()) S->large_worker = std::thread(large_loop, S);
+      S->large.push_back(L);
+      S->large_outstanding++;
+      S->large_cv.notify_all();
       return;
     }
     kg_server::Pending p;
@@ -583,15 +662,13 @@ int kg_server_progress(kg_server *S, int32_t wait, int64_t *done_ids, int32_t ma
       }
       for (auto &sl : S->slots)
         if (!sl.busy && !S->queue.empty()) launch_batch(S, sl);
-      // one large request, while the batches run on the device
-      if (!S->large.empty() && *ndone < max) {
-        kg_serve_request r = S->large.front();
-        S->large.pop_front();
-        S->results[r.id] = run_large(S, r);
-        done_ids[(*ndone)++] = r.id;
+      collect_large(S, false, done_ids, max, ndone);
+      if (!wait || waited || *ndone > 0) break;
+      if (S->inflight == 0) {
+        // nothing batched in flight: wait for a large request, if any
+        if (S->large_outstanding > 0 && *ndone < max) collect_large(S, true, done_ids, max, ndone);
         break;
       }
-      if (!wait || waited || S->inflight == 0) break;
     }
   });
 }
@@ -606,11 +683,19 @@ int kg_server_take_result(kg_server *S, int64_t id, kg_batch **out) {
 }
 
 int64_t kg_server_pending(const kg_server *S) {
-  return S ? (int64_t)S->queue.size() + S->inflight + (int64_t)S->large.size() : 0;
+  return S ? (int64_t)S->queue.size() + S->inflight + S->large_outstanding : 0;
 }
 
 void kg_server_destroy(kg_server *S) {
   if (!S) return;
+  {
+    std::lock_guard<std::mutex> g(S->large_mu);
+    S->large_stop = true;
+    S->large.clear();
+    S->large_cv.notify_all();
+  }
+  if (S->large_worker.joinable()) S->large_worker.join();
+  for (auto &d : S->large_done) kg_batch_release(d.b);
   for (auto &sl : S->slots) {
     if (sl.s) (void)hipStreamSynchronize(sl.s);
     if (sl.graph) (void)hipGraphExecDestroy(sl.graph);
