@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--qps", type=float, default=400.0)
     p.add_argument("--chunk", type=int, default=48)
     p.add_argument("--probe-rows", type=int, default=200_000_000)
+    p.add_argument("--build-rows", type=int, default=1_000_000,
+                   help="joinagg: build rows (keys a permutation of 0..n-1; fk uniform over 1.2n); "
+                        "4096 or fewer take the direct index")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device-to-device copy (to scale FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -161,7 +164,7 @@ def joinagg(args):
     from kgpu import Col, DeviceBuffer
     from kgpu import abi as A
     ctx = kgpu.Context(0)
-    nb, n = 1_000_000, args.probe_rows
+    nb, n = args.build_rows, args.probe_rows
     L = kgpu.lib()
     bk = DeviceBuffer(ctx, nb * 8)
     bv = DeviceBuffer(ctx, nb * 8)
@@ -169,7 +172,7 @@ def joinagg(args):
     fk = DeviceBuffer(ctx, n * 8)
     fkv = DeviceBuffer(ctx, (n + 7) // 8 + 8)
     amt = DeviceBuffer(ctx, n * 8)
-    A.check(L.kg_gen_join_probe(ctx.h, 2, 0, n, 1_200_000, 10, fk.ptr, fkv.ptr, amt.ptr))
+    A.check(L.kg_gen_join_probe(ctx.h, 2, 0, n, nb * 12 // 10, 10, fk.ptr, fkv.ptr, amt.ptr))
     bcol = Col(A.T_INT64, bk, length=nb)
     pcols = [Col(A.T_INT64, fk, fkv, length=n), Col(A.T_INT64, amt, length=n)]
     groups = [0]
@@ -196,7 +199,12 @@ def joinagg(args):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     kgpu.prof_enable(False)
-    sc_ms, sc_n = kgpu.prof_get("jp_scatter")
+    # the roofline kernel: the staged scatter, or the direct-index kernel
+    # for a tiny build (which reads the inputs once and writes nothing)
+    rk_name, fam = ("k_jp_scatter_staged", "jp_scatter")
+    if kgpu.prof_get("joinagg_direct")[1]:
+        rk_name, fam = ("k_joinagg_direct", "joinagg_direct")
+    sc_ms, sc_n = kgpu.prof_get(fam)
     in_bytes = n * 16 + n // 8  # fk, amt, fk validity: each read once
     out_bytes = groups[0] * 24 + nb * 8  # (key, sum, count) rows; the build keys
     avg = sc_ms / max(sc_n, 1)
@@ -204,24 +212,26 @@ def joinagg(args):
     achieved = per_launch / (avg / 1e3) if sc_ms > 0 else None
     step_ms = dt * 1e3 / args.steps
     print(json.dumps({
-        "metric": "join probe rows/s (fused GROUP BY build key; 1M build rows)",
+        "metric": f"join probe rows/s (fused GROUP BY build key; {nb} build rows)",
         "value": n * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic (seeded counter-based generator, generated in HBM)",
-        "config": {"workload": f"SELECT key, SUM(amt), COUNT(*) join-aggregate, 1000000 build rows, "
-                               f"{n} probe rows, fk uniform [0, 1.2M), 1% NULL", "groups": groups[0]},
+        "config": {"workload": f"SELECT key, SUM(amt), COUNT(*) join-aggregate, {nb} build rows, "
+                               f"{n} probe rows, fk uniform [0, {nb * 12 // 10}), 1% NULL",
+                   "groups": groups[0]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None,
-                     "traffic": traffic_of("k_jp_scatter_staged_joinagg_200M_traffic",
+                     "traffic": traffic_of(f"{rk_name}_joinagg_200M_traffic",
                                            int(per_launch // 16.125) if sc_n else -1),
-                     "kernel": "k_jp_scatter_staged", "kernel_avg_ms": avg,
+                     "kernel": rk_name, "kernel_avg_ms": avg,
                      "launches_per_step": sc_n / args.steps,
                      "algorithmic_bytes_per_step": in_bytes},
         "step_hbm_frac": ((in_bytes + out_bytes) / (step_ms / 1e3)) / HBM_PEAK,
-        "phases": phases(["join_build", "jp_hist", "jp_scatter", "jp_merge", "agg_output"]),
-        **({} if args.no_cpu else {"cpu_baseline": cpu_joinagg()}),
+        "phases": phases(["join_build", "jp_hist", "jp_scatter", "jp_merge", "joinagg_direct",
+                          "agg_output"]),
+        **({} if args.no_cpu or nb != 1_000_000 else {"cpu_baseline": cpu_joinagg()}),
     }), flush=True)
 
 

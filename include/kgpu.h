@@ -107,6 +107,10 @@ int kg_batch_column(const kg_batch *b, int32_t i, kg_column *out);
 /* Copies column i to host buffers: values (width x rows bytes; BOOL as one
  * byte per row), validity as one byte per row (may be NULL). */
 int kg_batch_column_to_host(const kg_batch *b, int32_t i, void *values, uint8_t *valid_bytes);
+/* Releases a batch.  Its device memory is reused only after the library's
+ * own work that reads or writes it has completed (stream order plus the
+ * reads of every operator call made before the release); device work the
+ * caller enqueued itself on the columns must be complete first. */
 void kg_batch_release(kg_batch *b);
 
 /* ---- routing hash and hash repartition (exchange) ---------------------
@@ -236,6 +240,12 @@ typedef struct kg_join_spec {
   int32_t probe_keys[4];  /* key column indices of the probe batch */
   int32_t build_keys[4];  /* key column indices of the build batch */
   uint8_t null_safe[4];   /* `<=>` keys */
+  /* 1: never probe through a direct index (krishiv.exec.join.directIndex =
+   * false, crates/krishiv-common/src/config/registry_join.rs:26).  With 0,
+   * a build of at most 4,096 rows on one integer `=` key whose values span
+   * at most 16,384 slots and 4 a row is probed by key - min
+   * (crates/krishiv-exec/src/join/direct.rs:1-160). */
+  uint8_t no_direct_index;
 } kg_join_spec;
 
 typedef struct kg_hash_join kg_hash_join;

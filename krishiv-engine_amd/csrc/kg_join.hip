@@ -48,6 +48,11 @@ struct JTable {
   int log2cap;
   uint64_t cap;
   uint64_t seed;
+  // direct index of a tiny build (join/direct.rs): slot of key dmin + d at
+  // dslot[d] (-1: none), d < dspan; null when the build does not qualify
+  const int32_t *dslot;
+  int64_t dmin;
+  uint64_t dspan;
 };
 
 // Packed key of row i.  Returns 0 (ordinary), 1 (the EMPTY value), 2 (a
@@ -114,9 +119,17 @@ __global__ void k_join_build(KeySpec K, int64_t n, JTable T) {
   }
 }
 
+// Direct-index lookup (join/direct.rs:199-230: key - min, one load): the
+// slot of an integer key, or -1 outside the span or for an absent key.
+__device__ __forceinline__ int64_t dfind(const JTable &T, uint64_t key) {
+  const uint64_t d = key - (uint64_t)T.dmin;  // wraps outside the span
+  return d < T.dspan ? (int64_t)T.dslot[d] : -1;
+}
+
 // Read-only lookup: slot index, or -1.
 __device__ __forceinline__ int64_t jfind(const JTable &T, int cls, uint64_t key) {
   if (cls == 3) return -1;
+  if (T.dslot) return dfind(T, key);  // no `<=>` NULL with a direct index
   if (cls != 0) return (int64_t)(T.cap + (cls == 2 ? 1 : 0));
   uint64_t s = jslot_start(T, key);
   while (true) {
@@ -238,6 +251,93 @@ __global__ void k_joinagg_probe(KeySpec K, int64_t n, JTable T, AggProg P, uint6
     uint32_t m = T.slots[s].count;
     if (!m) continue;
     apply_row_mult(P, rec + (uint64_t)s * P.W, i, m);
+  }
+}
+
+// min/max of the keys in the table (as signed 64-bit values; the EMPTY
+// value's fixed slot counts when it holds rows): out = {min, max}, preset
+// to {INT64_MAX, INT64_MIN}.
+__global__ void k_slot_minmax(JTable T, long long *out) {
+  long long lo = LLONG_MAX, hi = LLONG_MIN;
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s <= T.cap;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = s < T.cap ? T.slots[s].key : kEmpty;
+    if (s < T.cap ? key == kEmpty : T.slots[s].count == 0) continue;
+    const long long v = (long long)key;
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    const long long a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (lo != LLONG_MAX) atomicMin(&out[0], lo);
+    if (hi != LLONG_MIN) atomicMax(&out[1], hi);
+  }
+}
+
+// The direct index: every occupied slot (and the EMPTY value's fixed slot)
+// writes its slot number at key - min.  Keys in slots are distinct.
+__global__ void k_join_direct(JTable T, int32_t *dslot) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s <= T.cap;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = s < T.cap ? T.slots[s].key : kEmpty;
+    if (s < T.cap && key == kEmpty) continue;
+    if (s == T.cap && T.slots[s].count == 0) continue;
+    const uint64_t d = key - (uint64_t)T.dmin;
+    if (d < T.dspan) dslot[d] = (int32_t)s;
+  }
+}
+
+// The fused aggregate over a direct index whose records fit in LDS: each
+// workgroup keeps a private record per key of the span (LDS atomics), reads
+// its share of the probe rows once, and adds its records into the global
+// ones at the end (global atomics: one set per key a workgroup touched).
+constexpr int kJDBlock = 512;
+constexpr int kJDTile = 4;
+
+__global__ void __launch_bounds__(kJDBlock) k_joinagg_direct(KeySpec K, int64_t n, JTable T,
+                                                             AggProg P, uint64_t *rec) {
+  extern __shared__ uint64_t smem[];
+  const int W = P.W, WL = W | 1;
+  const int span = (int)T.dspan;
+  uint64_t *s_rec = smem;                                                // span x WL
+  uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_rec + (size_t)span * WL);  // span
+  for (int t = threadIdx.x; t < span * WL; t += kJDBlock) s_rec[t] = P.init[t % WL];
+  for (int d = threadIdx.x; d < span; d += kJDBlock) {
+    const int32_t sl = T.dslot[d];
+    s_cnt[d] = sl >= 0 ? T.slots[sl].count : 0u;
+  }
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kJDBlock * kJDTile;
+  for (int64_t t0 = (int64_t)blockIdx.x * kJDBlock * kJDTile; t0 < n; t0 += stride) {
+    int ls[kJDTile];
+    uint64_t m[kJDTile];
+    ColSrc<kJDTile> src;
+#pragma unroll
+    for (int r = 0; r < kJDTile; r++) {
+      const int64_t i = t0 + (int64_t)r * kJDBlock + threadIdx.x;
+      src.idx[r] = i < n ? i : 0;
+      ls[r] = -1;
+      m[r] = 0;
+      uint64_t key;
+      if (i < n && pack_key(K, i, key) != 3) {
+        const uint64_t d = key - (uint64_t)T.dmin;
+        if (d < T.dspan && s_cnt[d]) {
+          ls[r] = (int)d;
+          m[r] = s_cnt[d];
+        }
+      }
+    }
+    tile_apply_mult<kJDTile>(P, s_rec, WL, ls, m, src);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < span; d += kJDBlock) {
+    const uint64_t *l = s_rec + (size_t)d * WL;
+    if (l[0] == 0) continue;  // no row of this workgroup reached the key
+    merge_record(P, rec + (uint64_t)T.dslot[d] * W, l);
   }
 }
 
@@ -665,6 +765,10 @@ struct kg_hash_join {
   uint64_t cap = 0;
   int log2cap = 0;
   uint64_t seed = 0x2545F4914F6CDD1DULL;
+  // direct index (tiny integer builds)
+  DevBuf dslot;
+  int64_t dmin = 0;
+  uint64_t dspan = 0;
   // fused aggregate
   bool has_agg = false;
   AggProg prog{};
@@ -685,6 +789,9 @@ static JTable jtable(kg_hash_join *j) {
   t.log2cap = j->log2cap;
   t.mask = j->cap - 1;
   t.seed = j->seed;
+  t.dslot = j->dspan ? j->dslot.get<int32_t>() : nullptr;
+  t.dmin = j->dmin;
+  t.dspan = j->dspan;
   return t;
 }
 
@@ -839,6 +946,47 @@ static DCol build_dcol(const kg_hash_join::BCol &bc) {
   return d;
 }
 
+// The direct index of a tiny build (join/direct.rs:96-160): one integer
+// key, `=` only, at most kDirectRows build rows whose keys span at most
+// kDirectSlots values and at most kDirectPerRow slots a row (a floor of 16
+// rows).  Probes then find a key's slot with one subtraction and one load;
+// chains, counts and emitted rows stay those of the hashed table.
+constexpr int64_t kDirectRows = 4096;
+constexpr uint64_t kDirectSlots = 16 * 1024;
+constexpr uint64_t kDirectPerRow = 4;
+
+static void build_direct(kg_hash_join *j) {
+  j->dspan = 0;
+  const int64_t n = j->build_rows;
+  if (j->spec.no_direct_index || j->spec.num_keys != 1 || j->spec.null_safe[0] || n <= 0 ||
+      n > kDirectRows)
+    return;
+  const int t = j->bcols[j->spec.build_keys[0]].type;
+  const int w = type_width(t);
+  if (w <= 0 || w > 8 || is_float(t)) return;  // integers, dates, timestamps, decimal64
+  DevBuf mm(j->ctx, 16);
+  const long long init[2] = {LLONG_MAX, LLONG_MIN};
+  long long got[2];
+  KG_HIP(hipMemcpyAsync(mm.get(), init, 16, hipMemcpyHostToDevice, j->s));
+  k_slot_minmax<<<stream_grid(j->cap + 1, 256), 256, 0, j->s>>>(jtable(j), mm.get<long long>());
+  KG_HIP(hipGetLastError());
+  KG_HIP(hipMemcpyAsync(got, mm.get(), 16, hipMemcpyDeviceToHost, j->s));
+  KG_HIP(hipStreamSynchronize(j->s));
+  if (got[0] > got[1]) return;  // every key NULL
+  const uint64_t span = (uint64_t)got[1] - (uint64_t)got[0] + 1;
+  const uint64_t rows = (uint64_t)(n > 16 ? n : 16);
+  if (span == 0 || span > kDirectSlots || span > kDirectPerRow * rows) return;
+  j->dslot.reset(j->ctx, span * 4);
+  KG_HIP(hipMemsetAsync(j->dslot.get(), 0xFF, span * 4, j->s));  // -1: no key
+  JTable T = jtable(j);
+  T.dmin = got[0];
+  T.dspan = span;
+  k_join_direct<<<stream_grid(j->cap + 1, 256), 256, 0, j->s>>>(T, j->dslot.get<int32_t>());
+  KG_HIP(hipGetLastError());
+  j->dmin = got[0];
+  j->dspan = span;
+}
+
 static void scan_u32(kg_hash_join *j, const uint32_t *in, uint64_t *out, int64_t n) {
   size_t tb = 0;
   KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, j->s));
@@ -903,6 +1051,7 @@ int kg_join_build_finalize(kg_hash_join *j) {
       k_join_build<<<stream_grid(n, 256), 256, 0, j->s>>>(K, n, jtable(j));
       KG_HIP(hipGetLastError());
     }
+    build_direct(j);
     KG_HIP(hipStreamSynchronize(j->s));
     j->finalized = true;
   });
@@ -938,7 +1087,7 @@ static void probe_impl(kg_hash_join *j, int32_t ncols, const kg_column *cols, in
       j->nout.ensure(j->ctx, (uint64_t)np * 4);
       j->off.ensure(j->ctx, (uint64_t)(np + 1) * 8);
       {
-        ProfScope prof("join_probe", j->s);
+        ProfScope prof(j->dspan ? "join_probe_direct" : "join_probe", j->s);
         k_join_count<<<stream_grid(np, 256), 256, 0, j->s>>>(
             K, np, sel, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
         KG_HIP(hipGetLastError());
@@ -1082,6 +1231,10 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   uint64_t C = 1;
   while (C * 2 <= j->cap && (C * 2 + kJMargin) * per_slot <= lds_cap) C *= 2;
   if ((C + kJMargin) * per_slot > lds_cap && C > 1) C /= 2;
+  // at least two partitions a CU where the table allows: a small table cut
+  // into a few large windows would leave most CUs without a partition
+  const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
+  while (C > 1 && j->cap / C < (uint64_t)cus * 2) C /= 2;
   const int log2p = (int)(j->log2cap - __builtin_ctzll(C));
   const uint32_t np = 1u << log2p;
   if (np > 16384) return false;  // the per-block counters would not fit in LDS
@@ -1090,7 +1243,6 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   if (lds > lds_cap) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the join window");
   KG_HIP(hipFuncSetAttribute((const void *)k_jp_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
-  const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
   // staged scatter: tiles of kJSBlock * R rows sorted in LDS
   const int R = F.rw <= 2 ? 12 : F.rw <= 3 ? 8 : 4;
   const size_t slds = js_lds_bytes(R, F.rw, np);
@@ -1185,8 +1337,28 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
       P.in[c] = d[ic];
     }
     if (nrows >= (1ll << 32)) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^32 rows or more");
-    if (nrows < (1 << 20) || !joinagg_partitioned(j, K, P, nrows)) {
-      ProfScope prof("joinagg_probe", j->s);
+    const int WL = P.W | 1;
+    const size_t dlds = (size_t)j->dspan * (WL * 8 + 4);
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, j->ctx->device) !=
+            hipSuccess || optin < 64 * 1024) {
+      (void)hipGetLastError();
+      optin = 64 * 1024;
+    }
+    if (j->dspan && dlds + 1024 <= (size_t)optin && nrows >= (1 << 16)) {
+      // direct index, records privatized in LDS
+      ProfScope prof("joinagg_direct", j->s);
+      KG_HIP(hipFuncSetAttribute((const void *)k_joinagg_direct,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)dlds));
+      const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
+      const int64_t per_cu = dlds <= 40 * 1024 ? 4 : dlds <= 80 * 1024 ? 2 : 1;
+      const int64_t grid = std::min<int64_t>(cus * per_cu,
+                                             ceil_div(nrows, (int64_t)kJDBlock * kJDTile * 8));
+      k_joinagg_direct<<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
+          K, nrows, jtable(j), P, j->rec.get<uint64_t>());
+      KG_HIP(hipGetLastError());
+    } else if (nrows < (1 << 20) || !joinagg_partitioned(j, K, P, nrows)) {
+      ProfScope prof(j->dspan ? "joinagg_probe_direct" : "joinagg_probe", j->s);
       k_joinagg_probe<<<stream_grid(nrows, 256), 256, 0, j->s>>>(K, nrows, jtable(j), P,
                                                                  j->rec.get<uint64_t>());
       KG_HIP(hipGetLastError());

@@ -49,7 +49,7 @@ class AggSpec(C.Structure):
 class JoinSpec(C.Structure):
     _fields_ = [("join_type", C.c_int32), ("num_keys", C.c_int32),
                 ("probe_keys", C.c_int32 * 4), ("build_keys", C.c_int32 * 4),
-                ("null_safe", C.c_uint8 * 4)]
+                ("null_safe", C.c_uint8 * 4), ("no_direct_index", C.c_uint8)]
 
 
 _lib = None

@@ -386,11 +386,12 @@ class HashAggregate:
 # ---------------------------------------------------------------- join
 class HashJoin:
     def __init__(self, ctx: Context, join_type, probe_keys, build_keys, null_safe=None,
-                 expected_build_rows=1 << 16):
+                 expected_build_rows=1 << 16, direct_index=True):
         self.ctx = ctx
         spec = JoinSpec()
         spec.join_type = join_type
         spec.num_keys = len(probe_keys)
+        spec.no_direct_index = 0 if direct_index else 1
         for i, (p, b) in enumerate(zip(probe_keys, build_keys)):
             spec.probe_keys[i] = p
             spec.build_keys[i] = b

@@ -16,8 +16,8 @@ JT = {A.JOIN_INNER: kgo.JOIN_INNER, A.JOIN_LEFT_OUTER: kgo.JOIN_LEFT_OUTER,
       A.JOIN_LEFT_SEMI: kgo.JOIN_LEFT_SEMI, A.JOIN_LEFT_ANTI: kgo.JOIN_LEFT_ANTI}
 
 
-def gpu_join(jt, probe_cols, build_cols, pk, bk, null_safe=None, build_batches=1):
-    j = kgpu.HashJoin(ctx(), jt, pk, bk, null_safe, expected_build_rows=1024)
+def gpu_join(jt, probe_cols, build_cols, pk, bk, null_safe=None, build_batches=1, direct=True):
+    j = kgpu.HashJoin(ctx(), jt, pk, bk, null_safe, expected_build_rows=1024, direct_index=direct)
     n = build_cols[0].length
     cuts = np.linspace(0, n, build_batches + 1).astype(int)
     for a, b in zip(cuts[:-1], cuts[1:]):
@@ -55,10 +55,31 @@ def expected_rows(jt, pairs, probe, build):
     return sorted(out, key=lambda t: tuple((x is None, x or 0) for x in t))
 
 
+def probed_direct(run):
+    """Runs `run()` with the library's timers on; returns whether the probe
+    went through the direct index (its own timer family) or the table."""
+    kgpu.prof_reset()
+    kgpu.prof_enable(True)
+    try:
+        run()
+    finally:
+        kgpu.prof_enable(False)
+    d = kgpu.prof_get("join_probe_direct")[1] + kgpu.prof_get("joinagg_direct")[1] + \
+        kgpu.prof_get("joinagg_probe_direct")[1]
+    h = kgpu.prof_get("join_probe")[1] + kgpu.prof_get("joinagg_probe")[1] + \
+        kgpu.prof_get("jp_merge")[1]
+    assert d + h > 0
+    return d > 0 and h == 0
+
+
+@pytest.mark.parametrize("direct", [True, False])
 @pytest.mark.parametrize("jt", [A.JOIN_INNER, A.JOIN_LEFT_OUTER, A.JOIN_LEFT_SEMI,
                                 A.JOIN_LEFT_ANTI])
-def test_join_types_with_duplicates_and_nulls(jt):
-    say(f"join: type {jt}")
+def test_join_types_with_duplicates_and_nulls(jt, direct):
+    """3,000 build rows on keys spanning 2,000 values: the direct index
+    (join/direct.rs) when allowed, the hashed table otherwise; the same rows
+    either way."""
+    say(f"join: type {jt}, direct index {direct}")
     rng = np.random.default_rng(10 + jt)
     nb, npr = 3000, 8000
     bk = rng.integers(0, 2000, nb)
@@ -70,12 +91,96 @@ def test_join_types_with_duplicates_and_nulls(jt):
     ppay = np.arange(npr, dtype=np.int64)
     build = [mk(A.T_INT64, bk, bv), mk(A.T_INT64, bpay, bpv)]
     probe = [mk(A.T_INT64, pk, pv), mk(A.T_INT64, ppay)]
-    j, out = gpu_join(jt, probe, build, [0], [0], build_batches=3)
+    res = {}
+    assert probed_direct(lambda: res.update(out=gpu_join(jt, probe, build, [0], [0], build_batches=3,
+                                                         direct=direct)[1])) == direct
+    out = res["out"]
     pairs = kgo.join(JT[jt], [kgo.Col(kgo.T_INT64, pk, pv)], [kgo.Col(kgo.T_INT64, bk, bv)])
     want = expected_rows(jt, pairs, [(pk, pv), (ppay, None)], [(bk, bv), (bpay, bpv)])
     ncols = 4 if jt in (A.JOIN_INNER, A.JOIN_LEFT_OUTER) else 2
     assert out.num_columns == ncols
     assert rows_of(out, ncols) == want
+
+
+@pytest.mark.parametrize("case", ["sentinel", "int32", "too_sparse", "too_many_rows"])
+def test_direct_index_qualification_and_edges(case):
+    """Direct-index edges (join/direct.rs:96-160): keys at the bottom of the
+    int64 range (the table's EMPTY value among them), an int32 key with
+    probe keys outside the span, and builds that keep the hashed table (a
+    span above 4 slots a row; more than 4,096 rows)."""
+    say(f"join: direct index, {case}")
+    rng = np.random.default_rng(len(case))
+    t, kt = (A.T_INT32, kgo.T_INT32) if case == "int32" else (A.T_INT64, kgo.T_INT64)
+    nb = 5000 if case == "too_many_rows" else 1200
+    if case == "sentinel":
+        bk = np.iinfo(np.int64).min + rng.integers(0, 900, nb)
+        pk = np.iinfo(np.int64).min + rng.integers(0, 1000, 30000)
+        pk[::7] = rng.integers(-(1 << 62), 1 << 62, len(pk[::7]))
+    elif case == "int32":
+        bk = rng.integers(-2000, 2000, nb).astype(np.int32)
+        pk = rng.integers(-(1 << 31), (1 << 31) - 1, 30000).astype(np.int32)
+        pk[::2] = rng.integers(-2100, 2100, 15000)
+    elif case == "too_sparse":
+        bk = rng.integers(0, 1_000_000, nb)
+        pk = np.concatenate([bk[rng.integers(0, nb, 20000)], rng.integers(0, 1_000_000, 10000)])
+    else:
+        bk = rng.integers(0, 6000, nb)
+        pk = rng.integers(-100, 6100, 30000)
+    bv = rng.random(nb) > 0.03
+    pv = rng.random(len(pk)) > 0.03
+    bpay = np.arange(nb, dtype=np.int64)
+    ppay = np.arange(len(pk), dtype=np.int64)
+    build = [mk(t, bk, bv), mk(A.T_INT64, bpay)]
+    probe = [mk(t, pk, pv), mk(A.T_INT64, ppay)]
+    res = {}
+    used = probed_direct(lambda: res.update(out=gpu_join(A.JOIN_LEFT_OUTER, probe, build, [0], [0])[1]))
+    assert used == (case in ("sentinel", "int32"))
+    pairs = kgo.join(kgo.JOIN_LEFT_OUTER, [kgo.Col(kt, pk, pv)], [kgo.Col(kt, bk, bv)])
+    want = expected_rows(A.JOIN_LEFT_OUTER, pairs, [(pk, pv), (ppay, None)], [(bk, bv), (bpay, None)])
+    assert rows_of(res["out"], 4) == want
+
+
+@pytest.mark.parametrize("nprobe", [200_000, 3_000_000])
+def test_direct_index_fused_aggregate(nprobe):
+    """The fused GROUP BY build key over a direct index: records privatized
+    per workgroup in LDS (probes of 64K rows and more), duplicate build keys
+    (the row count multiplies), NULL probe keys and inputs, probe keys
+    outside the span."""
+    say(f"join: direct-index fused aggregate, {nprobe} probe rows")
+    rng = np.random.default_rng(nprobe)
+    nb = 2000
+    bk = rng.integers(-700, 1300, nb).astype(np.int32)
+    pk = rng.integers(-1000, 1600, nprobe).astype(np.int32)
+    pkv = rng.random(nprobe) > 0.02
+    amt = rng.integers(-(1 << 40), 1 << 40, nprobe)
+    amtv = rng.random(nprobe) > 0.03
+    f = rng.random(nprobe)
+    aggs = [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0), (kgo.AGG_MAX, 1), (kgo.AGG_AVG, 1)]
+    op, ob = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT32, pk, pkv)], [kgo.Col(kgo.T_INT32, bk)])
+    o = kgo.Agg((kgo.T_INT32, 0, 0), [(kgo.T_INT64, 0, 0), (kgo.T_FLOAT64, 0, 0)], aggs)
+    o.update(kgo.Col(kgo.T_INT32, bk[ob]),
+             [kgo.Col(kgo.T_INT64, amt[op], amtv[op]), kgo.Col(kgo.T_FLOAT64, f[op])])
+    rk, rkv, rres = o.finish()
+    want = _groups(rk, rkv, [(np.asarray(v), np.asarray(ok)) for v, ok, _ in rres])
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0], expected_build_rows=nb)
+    j.build_sink([Col(A.T_INT32, bk)])
+    j.build_finalize()
+    j.aggregate([(A.T_INT64, 0, 0), (A.T_FLOAT64, 0, 0)], aggs)
+    assert probed_direct(lambda: j.probe_aggregate(
+        [Col(A.T_INT32, pk, pkv), Col(A.T_INT64, amt, amtv), Col(A.T_FLOAT64, f)], [1, 2]))
+    r = j.aggregate_result()
+    gk, gkv = r.column(0)
+    got = _groups(gk, gkv, [r.column(1 + c) for c in range(len(aggs))])
+    assert set(got) == set(want)
+    for k, wv in want.items():
+        for c, (a, b) in enumerate(zip(wv, got[k])):
+            assert (a is None) == (b is None), (k, c)
+            if a is None:
+                continue
+            if isinstance(a, (float, np.floating)):
+                assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)) + 1e-300, (k, c, a, b)
+            else:
+                assert int(a) == int(b), (k, c, a, b)
 
 
 def test_null_safe_key_and_float_normalization():
