@@ -211,6 +211,20 @@ int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *ou
                    int64_t *out_count);
 void kg_bloom_destroy(kg_bloom *b);
 
+/* Spark-layout Bloom filter (optional).  The reference holds only the hash
+ * Spark's filter uses, Murmur3 x86_32 (crates/krishiv-kernels/src/hash/
+ * murmur3.rs:9-52); the layout is Spark's sketch BloomFilterImpl (version 1):
+ * a long[] of ceil(num_bits / 64) words, putLong(item) sets bits
+ * (h1 + i*h2, ~ when negative) mod bitSize for i = 1..num_hashes with
+ * h1 = murmur3(item, 0), h2 = murmur3(item, h1); the item of a key is
+ * Spark's xxhash64(key) (seed 42), as InjectRuntimeFilter builds it.  Keys:
+ * int8/16/32, date32 (hashed as int), int64, timestamp, decimal64 (as long).
+ * kg_bloom_insert, kg_bloom_probe, kg_bloom_export (the words, little-endian)
+ * and kg_bloom_destroy apply to it. */
+uint64_t kg_spark_bloom_bits_for(uint64_t items, double fpp);   /* optimalNumOfBits */
+int32_t kg_spark_bloom_hashes_for(uint64_t items, uint64_t num_bits); /* optimalNumOfHashFunctions */
+int kg_spark_bloom_create(kg_ctx *ctx, uint64_t num_bits, int32_t num_hashes, kg_bloom **out);
+
 /* The range form of a runtime filter and the scan's row stage
  * (runtime_filter/member.rs Range, operators/runtime_filter.rs,
  * operators/select.rs): ids of the rows whose value lies in [lo, hi]

@@ -119,6 +119,85 @@ __global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_rows(const uint64_t
   }
 }
 
+// ---------------------------------------------------------------- Spark layout
+// Spark's sketch BloomFilterImpl (version 1): a long[] bit array of
+// `nwords` words; putLong(item) sets bit (h1 + i*h2, ~ when negative,
+// 32-bit) mod (64 nwords) for i = 1..k, h1 = murmur3(item, 0), h2 =
+// murmur3(item, h1).  A runtime filter's item is Spark's xxhash64(key), seed
+// 42 (of the int value for keys of 4 bytes or less, of the long otherwise).
+__device__ __forceinline__ int64_t spark_item(const DCol &c, int64_t i) {
+  const int64_t v = dev::load_i64(c, i);
+  return c.width <= 4 ? (int64_t)dev::xxh64_seeded((uint32_t)(int32_t)v, 4, 42)
+                      : (int64_t)dev::xxh64_seeded((uint64_t)v, 8, 42);
+}
+
+__device__ __forceinline__ uint64_t spark_bit(int32_t h1, int32_t h2, int i, uint64_t bits) {
+  int32_t c = (int32_t)((uint32_t)h1 + (uint32_t)i * (uint32_t)h2);
+  if (c < 0) c = ~c;
+  return (uint64_t)c % bits;
+}
+
+__global__ void k_spark_insert(DCol key, DCol sel, int has_sel, int64_t n,
+                               unsigned long long *words, uint64_t nwords, int k,
+                               unsigned long long *items) {
+  uint64_t mine = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (!dev::valid(key, i) || !select_row(has_sel ? &sel : nullptr, i)) continue;
+    const int64_t item = spark_item(key, i);
+    const int32_t h1 = dev::murmur3_long((uint64_t)item, 0);
+    const int32_t h2 = dev::murmur3_long((uint64_t)item, h1);
+    for (int j = 1; j <= k; j++) {
+      const uint64_t b = spark_bit(h1, h2, j, nwords * 64);
+      atomicOr(words + (b >> 6), 1ull << (b & 63));
+    }
+    mine++;
+  }
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
+  if ((threadIdx.x & 63) == 0 && mine) atomicAdd(items, (unsigned long long)mine);
+}
+
+__device__ __forceinline__ bool spark_check(const unsigned long long *words, uint64_t nwords, int k,
+                                            int64_t item) {
+  const int32_t h1 = dev::murmur3_long((uint64_t)item, 0);
+  const int32_t h2 = dev::murmur3_long((uint64_t)item, h1);
+  bool all = true;
+  for (int j = 1; j <= k; j++) {
+    const uint64_t b = spark_bit(h1, h2, j, nwords * 64);
+    all = all && ((words[b >> 6] >> (b & 63)) & 1);
+  }
+  return all;
+}
+
+// Pass 1 of the probe for the Spark layout (the same bitmap and tile counts
+// as k_bloom_probe_bits).
+__global__ void __launch_bounds__(kProbeBlock) k_spark_probe_bits(DCol key, int64_t n,
+                                                                  const unsigned long long *words,
+                                                                  uint64_t nwords, int k,
+                                                                  uint64_t *bitmap,
+                                                                  uint32_t *tile_count) {
+  __shared__ uint32_t s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kProbeTile;
+  uint32_t cnt = 0;
+  const int lane = threadIdx.x & 63;
+  for (int t = 0; t < kProbeTile / kProbeBlock; t++) {
+    int64_t i = base + (int64_t)t * kProbeBlock + threadIdx.x;
+    bool pass = false;
+    if (i < n && dev::valid(key, i)) pass = spark_check(words, nwords, k, spark_item(key, i));
+    uint64_t m = __ballot(pass);
+    int64_t w0 = base + (int64_t)t * kProbeBlock + (threadIdx.x & ~63);
+    if (lane == 0 && w0 < n) {
+      bitmap[w0 >> 6] = m;
+      cnt += (uint32_t)__popcll(m);
+    }
+  }
+  if (lane == 0 && cnt) atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_count[blockIdx.x] = s_cnt;
+}
+
 }  // namespace kg
 
 using namespace kg;
@@ -126,10 +205,21 @@ using namespace kg;
 struct kg_bloom {
   kg_ctx *ctx = nullptr;
   hipStream_t s = nullptr;
-  uint64_t nblocks = 0;
+  bool spark = false;     // Spark's BloomFilterImpl layout instead of the SBBF
+  uint64_t nblocks = 0;   // SBBF: 32-byte blocks
+  uint64_t nwords = 0;    // Spark: 64-bit words
+  int32_t k = 0;          // Spark: hash functions
   DevBuf words, items;
   DevBuf bitmap, tiles, tile_off, temp;  // probe scratch
 };
+
+static void check_spark_key(const kg_column &c) {
+  const int t = c.type;
+  const int w = type_width(t);
+  if (!(t == KG_T_INT8 || t == KG_T_INT16 || t == KG_T_INT32 || t == KG_T_DATE32 ||
+        t == KG_T_INT64 || t == KG_T_TIMESTAMP || t == KG_T_DECIMAL64) || w <= 0)
+    fail(KG_ERR_UNSUPPORTED, "Spark-layout Bloom keys of this type");
+}
 
 extern "C" {
 
@@ -143,6 +233,43 @@ uint64_t kg_bloom_bytes_for(uint64_t items, double fpp) {
   uint64_t blocks = (bytes + 31) / 32;
   if (blocks < 1) blocks = 1;
   return blocks * 32;
+}
+
+uint64_t kg_spark_bloom_bits_for(uint64_t items, double fpp) {
+  // optimalNumOfBits: (long)(-n ln p / (ln 2)^2)
+  if (fpp <= 0.0 || fpp >= 1.0) return 0;
+  const double l2 = std::log(2.0);
+  return (uint64_t)(int64_t)(-(double)items * std::log(fpp) / (l2 * l2));
+}
+
+int32_t kg_spark_bloom_hashes_for(uint64_t items, uint64_t num_bits) {
+  // optimalNumOfHashFunctions: max(1, round(m / n * ln 2))
+  const double k = (double)num_bits / (double)(items ? items : 1) * std::log(2.0);
+  const int32_t r = (int32_t)std::llround(k);
+  return r < 1 ? 1 : r;
+}
+
+int kg_spark_bloom_create(kg_ctx *ctx, uint64_t num_bits, int32_t num_hashes, kg_bloom **out) {
+  return guarded([&] {
+    *out = nullptr;
+    if (num_hashes < 1 || num_hashes > 64) fail(KG_ERR_UNSUPPORTED, "1 to 64 hash functions");
+    auto b = std::make_unique<kg_bloom>();
+    b->ctx = ctx;
+    b->spark = true;
+    b->k = num_hashes;
+    b->nwords = (num_bits + 63) / 64;
+    if (b->nwords < 1) b->nwords = 1;
+    // Spark indexes bits with a non-negative int: more than 2^31 bits are unreachable
+    if (b->nwords > (1ull << 26)) fail(KG_ERR_UNSUPPORTED, "Spark-layout Bloom of 2^32 bits or more");
+    b->s = make_stream(ctx->device);
+    StreamScope kg_scope(b->s);
+    b->words.reset(ctx, b->nwords * 8);
+    b->items.reset(ctx, 8);
+    KG_HIP(hipMemsetAsync(b->words.get(), 0, b->nwords * 8, b->s));
+    KG_HIP(hipMemsetAsync(b->items.get(), 0, 8, b->s));
+    KG_HIP(hipStreamSynchronize(b->s));
+    *out = b.release();
+  });
 }
 
 int kg_bloom_create(kg_ctx *ctx, uint64_t nbytes, kg_bloom **out) {
@@ -176,7 +303,14 @@ int kg_bloom_insert(kg_bloom *b, const kg_column *key, const kg_column *select, 
       check_column(*select, nrows, "Bloom selection");
       sel = st.add(*select, nrows);
     }
-    {
+    if (b->spark) {
+      check_spark_key(*key);
+      ProfScope prof("bloom_insert", b->s);
+      k_spark_insert<<<stream_grid(nrows, 256), 256, 0, b->s>>>(
+          k, sel, select != nullptr, nrows, b->words.get<unsigned long long>(), b->nwords, b->k,
+          b->items.get<unsigned long long>());
+      KG_HIP(hipGetLastError());
+    } else {
       ProfScope prof("bloom_insert", b->s);
       k_bloom_insert<<<stream_grid(nrows, 256), 256, 0, b->s>>>(
           k, sel, select != nullptr, nrows, b->words.get<uint32_t>(), b->nblocks,
@@ -188,6 +322,9 @@ int kg_bloom_insert(kg_bloom *b, const kg_column *key, const kg_column *select, 
 }
 
 double kg_bloom_expected_fpp(const kg_bloom *b, uint64_t items) {
+  if (b->spark)  // Spark's expectedFpp: (1 - e^(-k n / m))^k
+    return std::pow(1.0 - std::exp(-(double)b->k * (double)items / (64.0 * (double)b->nwords)),
+                    (double)b->k);
   // expected_fpp (bloom.rs:270-308): blocks receive Poisson(lambda) hashes
   double lambda = (double)items / (double)b->nblocks;
   if (lambda == 0.0) return 0.0;
@@ -214,7 +351,7 @@ double kg_bloom_expected_fpp(const kg_bloom *b, uint64_t items) {
 int kg_bloom_export(kg_bloom *b, uint8_t *bytes, uint64_t *len) {
   return guarded([&] {
     StreamScope kg_scope(b->s);
-    uint64_t need = b->nblocks * 32;
+    uint64_t need = b->spark ? b->nwords * 8 : b->nblocks * 32;
     if (!bytes) {
       *len = need;
       return;
@@ -240,7 +377,14 @@ int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *ou
     b->bitmap.ensure(b->ctx, (uint64_t)ceil_div(nrows, 64) * 8 + 8);
     b->tiles.ensure(b->ctx, (uint64_t)tiles * 4);
     b->tile_off.ensure(b->ctx, (uint64_t)(tiles + 1) * 8);
-    {
+    if (b->spark) {
+      check_spark_key(*key);
+      ProfScope prof("bloom_probe", b->s);
+      k_spark_probe_bits<<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
+          k, nrows, b->words.get<unsigned long long>(), b->nwords, b->k, b->bitmap.get<uint64_t>(),
+          b->tiles.get<uint32_t>());
+      KG_HIP(hipGetLastError());
+    } else {
       ProfScope prof("bloom_probe", b->s);
       k_bloom_probe_bits<<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
           k, nrows, b->words.get<uint32_t>(), b->nblocks, b->bitmap.get<uint64_t>(),

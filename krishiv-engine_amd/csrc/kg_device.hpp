@@ -118,6 +118,41 @@ __device__ __forceinline__ uint64_t xxh64_fixed(uint64_t lo, uint64_t hi, int wi
   return avalanche(h);
 }
 
+// XXH64 with a seed of a 4- or 8-byte value (Spark's xxhash64 of an int or
+// a bigint uses seed 42): the same short-input path with h = seed + P5 + len.
+__device__ __forceinline__ uint64_t xxh64_seeded(uint64_t v, int width, uint64_t seed) {
+  uint64_t h = seed + P5 + (uint64_t)width;
+  if (width == 8) {
+    h = lane8(h, v);
+  } else {
+    h = rotl(h ^ ((v & 0xFFFFFFFFULL) * P1), 23) * P2 + P3;
+  }
+  return avalanche(h);
+}
+
+// Murmur3 x86_32 as Spark runs it (crates/krishiv-kernels/src/hash/murmur3.rs:9-52).
+__device__ __forceinline__ uint32_t m3_rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+__device__ __forceinline__ uint32_t m3_mix_k1(uint32_t k1) {
+  return m3_rotl(k1 * 0xcc9e2d51u, 15) * 0x1b873593u;
+}
+__device__ __forceinline__ uint32_t m3_mix_h1(uint32_t h1, uint32_t k1) {
+  return m3_rotl(h1 ^ k1, 13) * 5u + 0xe6546b64u;
+}
+__device__ __forceinline__ uint32_t m3_fmix(uint32_t h1, uint32_t len) {
+  uint32_t h = h1 ^ len;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  return h ^ (h >> 16);
+}
+// hash_long: the low, then the high 32 bits
+__device__ __forceinline__ int32_t murmur3_long(uint64_t v, int32_t seed) {
+  uint32_t h1 = m3_mix_h1((uint32_t)seed, m3_mix_k1((uint32_t)v));
+  h1 = m3_mix_h1(h1, m3_mix_k1((uint32_t)(v >> 32)));
+  return (int32_t)m3_fmix(h1, 8);
+}
+
 // The folded multiply (layout.rs:1131-1139; fold_column, routing.rs:172-183).
 __device__ __forceinline__ uint64_t fold(uint64_t a, uint64_t b) {
   return (a * b) ^ __umul64hi(a, b);

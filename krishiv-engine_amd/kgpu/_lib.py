@@ -86,6 +86,9 @@ _SIGS = {
     "kg_agg_destroy": (None, [C.c_void_p]),
     "kg_bloom_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "kg_bloom_bytes_for": (C.c_uint64, [C.c_uint64, C.c_double]),
+    "kg_spark_bloom_bits_for": (C.c_uint64, [C.c_uint64, C.c_double]),
+    "kg_spark_bloom_hashes_for": (C.c_int32, [C.c_uint64, C.c_uint64]),
+    "kg_spark_bloom_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p]),
     "kg_bloom_insert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "kg_bloom_expected_fpp": (C.c_double, [C.c_void_p, C.c_uint64]),
     "kg_bloom_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -454,15 +454,30 @@ class HashJoin:
 
 # ---------------------------------------------------------------- bloom
 class RuntimeBloom:
-    def __init__(self, ctx: Context, nbytes: int):
+    """The runtime filter's split-block Bloom filter (Parquet layout), or with
+    spark_bits > 0 Spark's BloomFilterImpl layout of that many bits and
+    spark_hashes hash functions (items: Spark's xxhash64 of the key)."""
+
+    def __init__(self, ctx: Context, nbytes: int = 0, spark_bits: int = 0, spark_hashes: int = 0):
         self.ctx = ctx
         h = C.c_void_p()
-        check(lib().kg_bloom_create(ctx.h, nbytes, C.byref(h)))
+        if spark_bits:
+            check(lib().kg_spark_bloom_create(ctx.h, spark_bits, spark_hashes, C.byref(h)))
+        else:
+            check(lib().kg_bloom_create(ctx.h, nbytes, C.byref(h)))
         self.h = h
 
     @staticmethod
     def bytes_for(items: int, fpp: float) -> int:
         return lib().kg_bloom_bytes_for(items, fpp)
+
+    @staticmethod
+    def spark_bits_for(items: int, fpp: float) -> int:
+        return lib().kg_spark_bloom_bits_for(items, fpp)
+
+    @staticmethod
+    def spark_hashes_for(items: int, bits: int) -> int:
+        return lib().kg_spark_bloom_hashes_for(items, bits)
 
     def insert(self, key: Col, select: Col | None = None, nrows=None):
         nrows = key.length if nrows is None else nrows

@@ -508,6 +508,101 @@ int kgo_sbbf_build(const uint64_t *hashes, int64_t n, size_t bytes, double fpp, 
 }
 
 /* ------------------------------------------------------------------ */
+/* Spark-layout Bloom filter (optional).  The reference holds only the    */
+/* hash primitive: Murmur3 x86_32 as Spark runs it                        */
+/* (crates/krishiv-kernels/src/hash/murmur3.rs:9-52).  The filter is       */
+/* Spark's sketch BloomFilterImpl (version 1; Apache Spark, not in the     */
+/* reference): putLong(item) takes h1 = hashLong(item, 0), h2 =            */
+/* hashLong(item, h1) and sets bit (h1 + i*h2, negated by ~ when           */
+/* negative, 32-bit) mod bitSize for i = 1..k in a long[] bit array; a     */
+/* runtime filter's item is xxhash64(key) with seed 42 (Spark's            */
+/* InjectRuntimeFilter).  Parity of the layout is unpinned by the          */
+/* reference; the hashes are pinned by Spark's hash()/xxhash64() goldens.  */
+/* ------------------------------------------------------------------ */
+static uint32_t m3_rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+/* mix_k1 / mix_h1 / fmix (murmur3.rs:12-30) */
+static uint32_t m3_mix_k1(uint32_t k1) { return m3_rotl(k1 * 0xcc9e2d51u, 15) * 0x1b873593u; }
+static uint32_t m3_mix_h1(uint32_t h1, uint32_t k1) { return m3_rotl(h1 ^ k1, 13) * 5u + 0xe6546b64u; }
+static uint32_t m3_fmix(uint32_t h1, uint32_t len) {
+  uint32_t h = h1 ^ len;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  return h ^ (h >> 16);
+}
+
+/* hash_int (murmur3.rs:39-42) */
+int32_t kgo_murmur3_int(int32_t v, int32_t seed) {
+  return (int32_t)m3_fmix(m3_mix_h1((uint32_t)seed, m3_mix_k1((uint32_t)v)), 4);
+}
+
+/* hash_long (murmur3.rs:44-52): the low, then the high 32 bits */
+int32_t kgo_murmur3_long(int64_t v, int32_t seed) {
+  uint64_t b = (uint64_t)v;
+  uint32_t h1 = m3_mix_h1((uint32_t)seed, m3_mix_k1((uint32_t)(b & 0xffffffffu)));
+  h1 = m3_mix_h1(h1, m3_mix_k1((uint32_t)(b >> 32)));
+  return (int32_t)m3_fmix(h1, 8);
+}
+
+/* Spark's xxhash64(v) of an int (4 bytes) or a bigint (8 bytes), seed 42 */
+int64_t kgo_spark_xxhash64(int64_t v, int width) {
+  uint8_t b[8];
+  uint64_t u = (uint64_t)v;
+  for (int i = 0; i < 8; i++) b[i] = (uint8_t)(u >> (8 * i));
+  return (int64_t)kgo_xxh64(b, (size_t)width, 42);
+}
+
+void kgo_spark_xxhash64_batch(const int64_t *v, int64_t n, int width, int64_t *out) {
+  for (int64_t i = 0; i < n; i++) out[i] = kgo_spark_xxhash64(v[i], width);
+}
+
+/* optimalNumOfBits: (long)(-n ln p / (ln 2)^2) */
+uint64_t kgo_spark_bloom_bits_for(uint64_t items, double fpp) {
+  double l2 = log(2.0);
+  return (uint64_t)(int64_t)(-(double)items * log(fpp) / (l2 * l2));
+}
+
+/* optimalNumOfHashFunctions: max(1, round(m / n * ln 2)) */
+int32_t kgo_spark_bloom_hashes_for(uint64_t items, uint64_t bits) {
+  double k = (double)bits / (double)(items ? items : 1) * log(2.0);
+  int32_t r = (int32_t)llround(k);
+  return r < 1 ? 1 : r;
+}
+
+/* words of the bit array: ceil(bits / 64); bitSize = words * 64 */
+uint64_t kgo_spark_bloom_words(uint64_t bits) { return (bits + 63) / 64; }
+
+static uint64_t spark_bit(int32_t h1, int32_t h2, int i, uint64_t bit_size) {
+  int32_t c = (int32_t)((uint32_t)h1 + (uint32_t)i * (uint32_t)h2);
+  if (c < 0) c = ~c;
+  return (uint64_t)c % bit_size;
+}
+
+void kgo_spark_bloom_put(uint64_t *words, uint64_t nwords, int32_t k, int64_t item) {
+  int32_t h1 = kgo_murmur3_long(item, 0), h2 = kgo_murmur3_long(item, h1);
+  for (int i = 1; i <= k; i++) {
+    uint64_t b = spark_bit(h1, h2, i, nwords * 64);
+    words[b >> 6] |= 1ull << (b & 63);
+  }
+}
+
+int kgo_spark_bloom_check(const uint64_t *words, uint64_t nwords, int32_t k, int64_t item) {
+  int32_t h1 = kgo_murmur3_long(item, 0), h2 = kgo_murmur3_long(item, h1);
+  for (int i = 1; i <= k; i++) {
+    uint64_t b = spark_bit(h1, h2, i, nwords * 64);
+    if (!((words[b >> 6] >> (b & 63)) & 1)) return 0;
+  }
+  return 1;
+}
+
+void kgo_spark_bloom_check_batch(const uint64_t *words, uint64_t nwords, int32_t k,
+                                 const int64_t *items, int64_t n, uint8_t *out) {
+  for (int64_t i = 0; i < n; i++) out[i] = (uint8_t)kgo_spark_bloom_check(words, nwords, k, items[i]);
+}
+
+/* ------------------------------------------------------------------ */
 /* generators                                                           */
 /* ------------------------------------------------------------------ */
 /* SplitMix64's output function (Steele, Lea, Flood 2014). */

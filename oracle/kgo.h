@@ -187,6 +187,20 @@ const char *kgo_last_error(void);
 int kgo_avg_fast(int64_t sum_lo, int64_t sum_hi, int sum_scale, int64_t count, int p, int s,
                  int64_t *out_lo, int64_t *out_hi);
 
+/* Spark-layout Bloom filter (optional row; layout parity unpinned by the
+ * reference, which holds only Murmur3 x86_32, murmur3.rs:9-52). */
+int32_t kgo_murmur3_int(int32_t v, int32_t seed);
+int32_t kgo_murmur3_long(int64_t v, int32_t seed);
+int64_t kgo_spark_xxhash64(int64_t v, int width);
+void kgo_spark_xxhash64_batch(const int64_t *v, int64_t n, int width, int64_t *out);
+uint64_t kgo_spark_bloom_bits_for(uint64_t items, double fpp);
+int32_t kgo_spark_bloom_hashes_for(uint64_t items, uint64_t bits);
+uint64_t kgo_spark_bloom_words(uint64_t bits);
+void kgo_spark_bloom_put(uint64_t *words, uint64_t nwords, int32_t k, int64_t item);
+int kgo_spark_bloom_check(const uint64_t *words, uint64_t nwords, int32_t k, int64_t item);
+void kgo_spark_bloom_check_batch(const uint64_t *words, uint64_t nwords, int32_t k,
+                                 const int64_t *items, int64_t n, uint8_t *out);
+
 /* ---------------------------------------------------------------------
  * Hash join (crates/krishiv-exec/src/join/mod.rs:21-60,217-293;
  * join/probe.rs:266-501).  Inner / left semi / left anti / left outer on

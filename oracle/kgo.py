@@ -96,6 +96,26 @@ def lib():
         L.kgo_sbbf_insert.argtypes = [vp, C.c_size_t, u64]
         L.kgo_sbbf_check.restype = C.c_int
         L.kgo_sbbf_check.argtypes = [vp, C.c_size_t, u64]
+        L.kgo_murmur3_int.restype = C.c_int32
+        L.kgo_murmur3_int.argtypes = [C.c_int32, C.c_int32]
+        L.kgo_murmur3_long.restype = C.c_int32
+        L.kgo_murmur3_long.argtypes = [i64, C.c_int32]
+        L.kgo_spark_xxhash64.restype = i64
+        L.kgo_spark_xxhash64.argtypes = [i64, C.c_int]
+        L.kgo_spark_xxhash64_batch.restype = None
+        L.kgo_spark_xxhash64_batch.argtypes = [vp, i64, C.c_int, vp]
+        L.kgo_spark_bloom_bits_for.restype = u64
+        L.kgo_spark_bloom_bits_for.argtypes = [u64, C.c_double]
+        L.kgo_spark_bloom_hashes_for.restype = C.c_int32
+        L.kgo_spark_bloom_hashes_for.argtypes = [u64, u64]
+        L.kgo_spark_bloom_words.restype = u64
+        L.kgo_spark_bloom_words.argtypes = [u64]
+        L.kgo_spark_bloom_put.restype = None
+        L.kgo_spark_bloom_put.argtypes = [vp, u64, C.c_int32, i64]
+        L.kgo_spark_bloom_check.restype = C.c_int
+        L.kgo_spark_bloom_check.argtypes = [vp, u64, C.c_int32, i64]
+        L.kgo_spark_bloom_check_batch.restype = None
+        L.kgo_spark_bloom_check_batch.argtypes = [vp, u64, C.c_int32, vp, i64, vp]
         L.kgo_sbbf_check_batch.restype = None
         L.kgo_sbbf_check_batch.argtypes = [vp, C.c_size_t, vp, C.c_int64, vp]
         L.kgo_sbbf_expected_fpp.restype = C.c_double
@@ -267,6 +287,31 @@ def sbbf_check_batch(words: np.ndarray, hashes: np.ndarray) -> np.ndarray:
     out = np.zeros(len(hashes), dtype=np.uint8)
     lib().kgo_sbbf_check_batch(_ptr(words), len(words) // 8, _ptr(hashes), len(hashes), _ptr(out))
     return out.astype(bool)
+
+
+def spark_bloom_build(items: np.ndarray, nbits: int, k: int) -> np.ndarray:
+    """Spark BloomFilterImpl bit array (uint64 words) after putLong of each item."""
+    L = lib()
+    words = np.zeros(L.kgo_spark_bloom_words(nbits), dtype=np.uint64)
+    for it in np.asarray(items, dtype=np.int64):
+        L.kgo_spark_bloom_put(_ptr(words), len(words), k, int(it))
+    return words
+
+
+def spark_bloom_check_batch(words: np.ndarray, k: int, items: np.ndarray) -> np.ndarray:
+    words = np.ascontiguousarray(words, dtype=np.uint64)
+    items = np.ascontiguousarray(items, dtype=np.int64)
+    out = np.zeros(len(items), dtype=np.uint8)
+    lib().kgo_spark_bloom_check_batch(_ptr(words), len(words), k, _ptr(items), len(items), _ptr(out))
+    return out.astype(bool)
+
+
+def spark_xxhash64(values: np.ndarray, width: int) -> np.ndarray:
+    """Spark's xxhash64 (seed 42) of int (width 4) or bigint (width 8) values."""
+    v = np.ascontiguousarray(np.asarray(values).astype(np.int64))
+    out = np.zeros(len(v), dtype=np.int64)
+    lib().kgo_spark_xxhash64_batch(_ptr(v), len(v), width, _ptr(out))
+    return out
 
 
 # --- generators ----------------------------------------------------------

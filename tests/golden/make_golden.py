@@ -116,6 +116,17 @@ def main(ref: str):
     # -- Spark 4.2.0 decimal sum/avg goldens (data file of the reference's tests)
     shutil.copyfile(os.path.join(ref, "tests/fixtures/spark-4.2.0/decimal-sum-avg.json"),
                     os.path.join(HERE, "spark_decimal_sum_avg.json"))
+    # -- Spark 4.2.0 hash()/xxhash64() of single int and bigint values (the int
+    # and bigint cases of the reference's data file): Murmur3 x86_32 and
+    # XXH64, both seed 42
+    with open(os.path.join(ref, "tests/fixtures/spark-4.2.0/hash.json")) as f:
+        hj = json.load(f)
+    cases = [{"type": c["category"], "value": int(c["expr"].rstrip("L")), "hash": c["hash"],
+              "xxhash64": c["xxhash64"]}
+             for c in hj["cases"] if c["category"] in ("int", "bigint") and not c["conf"]]
+    with open(os.path.join(HERE, "spark_hash_int_bigint.json"), "w") as f:
+        json.dump({"source": "tests/fixtures/spark-4.2.0/hash.json (int and bigint cases)",
+                   "spark_version": hj["spark_version"], "cases": cases}, f, indent=1)
     # -- TPC-H Q1 aggregate input (Spark 4.2.0, SF 0.01) and Spark's answer
     tp = os.path.join(ref, "crates/krishiv-exec/tests/fixtures/tpch-sf0.01")
     for name in ("q1_input.arrow", "q1_expected.csv"):

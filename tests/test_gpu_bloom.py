@@ -126,3 +126,35 @@ def test_config3_pipeline_small():
     assert int(r.column(0)[0][0]) == int(pay[hit].sum())
     assert int(r.column(1)[0][0]) == int(hit.sum())
     assert ns >= hit.sum() and ns < 0.15 * npr  # 5 % true + about 3 % false positives
+
+
+@pytest.mark.parametrize("t,kt,width", [(A.T_INT64, kgo.T_INT64, 8), (A.T_INT32, kgo.T_INT32, 4)])
+def test_spark_layout_bits_and_probe_match_oracle(t, kt, width):
+    """Spark's BloomFilterImpl layout (optional): items are Spark's
+    xxhash64(key) (seed 42); the device bit array equals the oracle's word
+    for word, and the probe keeps exactly the rows the oracle's
+    mightContainLong passes (NULL keys never), in order."""
+    say(f"bloom: Spark layout, {width}-byte keys")
+    rng = np.random.default_rng(width)
+    nb, npr = 40_000, 2_000_000
+    dt = np.int64 if width == 8 else np.int32
+    lim = (1 << 40) if width == 8 else (1 << 30)
+    key = rng.integers(-lim, lim, nb).astype(dt)
+    kv = rng.random(nb) > 0.02
+    keep = rng.random(nb) > 0.5
+    items = kgo.spark_xxhash64(key[kv & keep], width)
+    bits = kgpu.RuntimeBloom.spark_bits_for(int((kv & keep).sum()), 0.03)
+    k = kgpu.RuntimeBloom.spark_hashes_for(int((kv & keep).sum()), bits)
+    assert bits == kgo.lib().kgo_spark_bloom_bits_for(int((kv & keep).sum()), 0.03)
+    assert k == kgo.lib().kgo_spark_bloom_hashes_for(int((kv & keep).sum()), bits)
+    b = kgpu.RuntimeBloom(ctx(), spark_bits=bits, spark_hashes=k)
+    b.insert(Col(t, key, kv), Col(A.T_BOOL, keep))
+    want = kgo.spark_bloom_build(items, bits, k)
+    assert np.frombuffer(b.to_bytes(), dtype=np.uint64).tolist() == want.tolist()
+    pk = np.concatenate([key[rng.integers(0, nb, npr // 4)],
+                         rng.integers(-lim, lim, npr - npr // 4).astype(dt)])
+    pkv = rng.random(npr) > 0.01
+    rows, cnt = b.probe(Col(t, pk, pkv))
+    got = rows.to_numpy(np.int32, cnt)
+    hit = kgo.spark_bloom_check_batch(want, k, kgo.spark_xxhash64(pk, width)) & pkv
+    assert np.array_equal(got, np.nonzero(hit)[0].astype(np.int32))

@@ -220,6 +220,43 @@ def test_sbbf_sizing_and_false_positives():
     assert L.kgo_sbbf_expected_fpp(1, 5000) > 0.99
 
 
+# ---------------------------------------------------------------- Spark-layout Bloom
+def test_spark_murmur3_and_xxhash64_match_spark_goldens():
+    """Spark 4.2.0's hash() (Murmur3 x86_32, seed 42) and xxhash64() (seed
+    42) of int and bigint values: the hashes the Spark-layout filter uses."""
+    with open(os.path.join(GOLD, "spark_hash_int_bigint.json")) as f:
+        g = json.load(f)
+    L = kgo.lib()
+    assert len(g["cases"]) >= 12
+    for c in g["cases"]:
+        v = c["value"]
+        if c["type"] == "int":
+            assert L.kgo_murmur3_int(v, 42) == c["hash"], c
+            assert kgo.spark_xxhash64([v], 4)[0] == c["xxhash64"], c
+        else:
+            assert L.kgo_murmur3_long(v, 42) == c["hash"], c
+            assert kgo.spark_xxhash64([v], 8)[0] == c["xxhash64"], c
+
+
+def test_spark_bloom_has_no_false_negatives_and_its_expected_fpp():
+    """Layout parity is unpinned (the reference has no Spark filter); what
+    any correct filter must do: every inserted item passes, and the rate of
+    passing non-members is near (1 - e^(-kn/m))^k."""
+    L = kgo.lib()
+    n = 20000
+    m = L.kgo_spark_bloom_bits_for(n, 0.03)
+    k = L.kgo_spark_bloom_hashes_for(n, m)
+    assert abs(m - (-n * math.log(0.03) / math.log(2) ** 2)) < 1 and k == 5
+    items = kgo.spark_xxhash64(np.arange(n), 8)
+    words = kgo.spark_bloom_build(items, m, k)
+    assert len(words) == (m + 63) // 64
+    assert kgo.spark_bloom_check_batch(words, k, items).all()
+    other = kgo.spark_xxhash64(np.arange(n, n + 200000), 8)
+    fp = kgo.spark_bloom_check_batch(words, k, other).mean()
+    want = (1 - math.exp(-k * n / (len(words) * 64))) ** k
+    assert 0.7 * want < fp < 1.3 * want
+
+
 # ---------------------------------------------------------------- generators
 def test_perm_is_a_bijection():
     L = kgo.lib()
