@@ -39,11 +39,17 @@ def parse():
     p.add_argument("--key", choices=["int64", "int8"], default="int64")
     p.add_argument("--cpu-rows", type=int, default=20_000_000)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workload", choices=["agg", "joinagg", "bloom", "serve"], default="agg",
+    p.add_argument("--workload", choices=["agg", "joinagg", "bloom", "serve", "grace"], default="agg",
                    help="agg: the hash aggregation (default); joinagg: the join of "
                         "SELECT key, SUM(amt), COUNT(*) ... GROUP BY key over 1M build rows; "
                         "bloom: the runtime Bloom filter feeding a join (15M build, 600M probe); "
-                        "serve: the batched/served join-aggregate mix against the plain form")
+                        "serve: the batched/served join-aggregate mix against the plain form; "
+                        "grace: the grace join + per-partition aggregate at --scale of "
+                        "configuration 4")
+    p.add_argument("--scale", type=float, default=0.1,
+                   help="grace: fraction of configuration 4 (1.5B build, 6B probe rows)")
+    p.add_argument("--grace-hbm-budget", type=int, default=0,
+                   help="grace: context HBM budget in bytes (partitions past it go to pinned host)")
     p.add_argument("--requests", type=int, default=10_000)
     p.add_argument("--qps", type=float, default=400.0)
     p.add_argument("--chunk", type=int, default=48)
@@ -325,6 +331,83 @@ def bloom(args):
     }), flush=True)
 
 
+def grace(args):
+    """Configuration 4 at --scale: build rows (int64 key Zipf(0.8) over 0.8x
+    the build rows' distinct keys, int64 payload in [1, 1e4]) and 4x as many
+    probe rows from the same Zipf with payloads in [1, 1e4]; SELECT p,
+    COUNT(*), SUM(probe.pay * build.pay % 2^31) GROUP BY p = partition_of(
+    H_kg(key), 64), through the grace join (64 partitions in HBM, pinned host
+    past the context budget) fused with the per-partition aggregate.  A step
+    sinks every batch and finishes."""
+    import kgpu
+    from kgpu import Col, DeviceBuffer
+    from kgpu import abi as A
+    ctx = kgpu.Context(0, args.grace_hbm_budget) if args.grace_hbm_budget else kgpu.Context(0)
+    L = kgpu.lib()
+    nb = int(1_500_000_000 * args.scale)
+    npr = 4 * nb
+    nd = int(1_200_000_000 * args.scale)
+    cdf = np.zeros(nd)
+    A.check(L.kg_zipf_cdf(nd, 0.8, cdf.ctypes.data))
+    cdf_d = DeviceBuffer.from_numpy(ctx, cdf)
+    del cdf
+    bk, bp = DeviceBuffer(ctx, nb * 8), DeviceBuffer(ctx, nb * 8)
+    pk, pp = DeviceBuffer(ctx, npr * 8), DeviceBuffer(ctx, npr * 8)
+    A.check(L.kg_gen_agg(ctx.h, 6, 0, nb, cdf_d.ptr, nd, bk.ptr, None, None, None, None))
+    A.check(L.kg_gen_agg(ctx.h, 7, 0, npr, cdf_d.ptr, nd, pk.ptr, None, None, None, None))
+    # payloads: the amt column of the probe generator, uniform in [1, 1e4]
+    scratch = DeviceBuffer(ctx, max(nb, npr) * 8)
+    A.check(L.kg_gen_join_probe(ctx.h, 8, 0, nb, 1, 0, scratch.ptr, None, bp.ptr))
+    A.check(L.kg_gen_join_probe(ctx.h, 9, 0, npr, 1, 0, scratch.ptr, None, pp.ptr))
+    del scratch, cdf_d
+    ctx.synchronize()
+    batch = 1 << 26
+    res = {}
+
+    def step():
+        g = kgpu.GraceJoinAggregate(ctx, 64, table_budget=int((8 << 30) * args.scale))
+        for a in range(0, nb, batch):
+            m = min(batch, nb - a)
+            g.build_sink(Col(A.T_INT64, bk, length=m, offset=a),
+                         Col(A.T_INT64, bp, length=m, offset=a), m)
+        for a in range(0, npr, batch):
+            m = min(batch, npr - a)
+            g.probe_sink(Col(A.T_INT64, pk, length=m, offset=a),
+                         Col(A.T_INT64, pp, length=m, offset=a), m)
+        c, s_, v, spilled = g.finish()
+        res.update(pairs=int(c.sum()), spilled=spilled)
+        g.close()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    kgpu.prof_reset()
+    kgpu.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kgpu.prof_enable(False)
+    rows = nb + npr
+    step_ms = dt * 1e3 / args.steps
+    in_bytes = rows * 16
+    print(json.dumps({
+        "metric": "grace join + per-partition aggregate rows/s (build + probe rows)",
+        "value": rows * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded counter-based generator, generated in HBM)",
+        "config": {"workload": f"configuration 4 at scale {args.scale}: {nb} build rows, {npr} probe "
+                               f"rows, Zipf(0.8) over {nd} keys, 64 partitions, table budget "
+                               f"{int((8 << 30) * args.scale)} B",
+                   "pairs": res.get("pairs"), "spilled_bytes": res.get("spilled")},
+        "step_hbm_frac": (in_bytes / (step_ms / 1e3)) / HBM_PEAK,
+        "phases": phases(["hash_partition", "scatter", "grace_build", "grace_probe", "agg_update",
+                          "agg_partition", "agg_merge", "agg_output"]),
+    }), flush=True)
+
+
 def serve(args):
     """The batched/served mix: requests arrive as Poisson at --qps (seed 8);
     each is SELECT key, SUM(pay), COUNT(*) FROM build JOIN probe ON key = fk
@@ -507,6 +590,9 @@ def main():
         return
     if args.workload == "serve":
         serve(args)
+        return
+    if args.workload == "grace":
+        grace(args)
         return
     import kgpu
     from kgpu import Col, DeviceBuffer

@@ -1,17 +1,30 @@
 // kg_grace.hip -- the grace hash join (join/grace.rs:99-1031) fused with the
 // per-partition aggregate of the plan in include/kgpu.h (kg_grace_*).
 //
-//  sink     H_kg -> partition_of -> stable radix scatter (kg_repartition);
-//           each sunk batch stays partitioned in HBM, or, once the context's
-//           HBM budget would be passed, in pinned host memory (the spill).
-//  finish   per partition: the build rows aggregated per key (the device
-//           hash aggregation: row count, non-NULL payload count, payload
-//           sum), a read-only lookup table of those aggregates, then the
-//           probe rows streamed against it.  A probe row with key k adds
+//  sink     H_kg -> partition_of -> LDS-staged radix scatter (k_gp_hist,
+//           k_gp_scatter): a workgroup sorts each tile of rows by partition
+//           in LDS and writes every partition's run with consecutive lanes;
+//           rows become [key, payload] pairs (NULL keys, which join nothing,
+//           are dropped; a NULL payload is stored as -1, never a payload
+//           the finish accepts).  The histogram pass also takes the payload
+//           range for the factorization check.  Each sunk batch stays
+//           partitioned in HBM, or, once the context's HBM budget would be
+//           passed, in pinned host memory (the spill).
+//  finish   per partition, as one launch over all its pieces each: the
+//           build rows aggregated per key straight into the partition's
+//           lookup table (CAS insert, then one 64-bit atomic for the row
+//           and non-NULL payload counts and one for the payload sum), then
+//           the probe rows streamed against it.  A probe row with key k adds
 //           nb_k pairs and pay * Sb_k to its partition (wave-reduced, one
 //           atomic per wave), the factorized form of the pair walk
-//           (join/probe.rs:266-337) for a SUM of payload products.
+//           (join/probe.rs:266-337) for a SUM of payload products.  The
+//           partitions run back to back on the operator's stream, reusing
+//           one table buffer; the host waits once, at the end.
 #include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <climits>
 
 #include "kg_common.hpp"
 #include "kg_device.hpp"
@@ -20,9 +33,16 @@ namespace kg {
 
 struct GSlot {
   uint64_t key;
-  uint64_t nb;   // build rows with the key
-  uint64_t nbv;  // of which with a non-NULL payload
+  uint64_t cnt;  // build rows with the key (low 32 bits), of which with a
+                 // non-NULL payload (high 32 bits)
   uint64_t sb;   // their payload sum (wrapping)
+  uint64_t pad;
+};
+
+// A partition's rows of one sunk piece (device columns).
+struct GPiece {
+  DCol key, pay;
+  int64_t n;
 };
 
 constexpr uint64_t kGEmpty = 0x8000000000000000ULL;
@@ -31,30 +51,108 @@ __device__ __forceinline__ uint64_t gslot_start(uint64_t key, int log2cap) {
   return dev::table_hash(key, 0x51ED27A1D3C4B9E5ULL) >> (64 - log2cap);
 }
 
-// Lookup table of the build partition's per-key aggregates (keys are
-// distinct: no CAS race on a key).  Slot cap holds the key equal to EMPTY.
-__global__ void k_grace_table(const int64_t *keys, const uint8_t *key_valid, const int64_t *nb,
-                              const int64_t *nbv, const int64_t *sb, int64_t n, GSlot *slots,
-                              uint64_t mask, int log2cap, uint64_t cap) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (key_valid && !((key_valid[i >> 3] >> (i & 7)) & 1)) continue;  // the NULL key group
-    uint64_t k = (uint64_t)keys[i];
-    uint64_t s;
-    if (k == kGEmpty) {
-      s = cap;
-    } else {
-      s = gslot_start(k, log2cap);
-      while (true) {
-        unsigned long long prev = atomicCAS((unsigned long long *)&slots[s].key, kGEmpty, k);
-        if (prev == kGEmpty) break;
-        s = (s + 1) & mask;
+// Adds (row and non-NULL payload counts, payload sum) to key k's slot,
+// inserting the key first when it is absent.
+__device__ __forceinline__ void grace_add(GSlot *slots, uint64_t mask, int log2cap, uint64_t cap,
+                                          uint64_t k, unsigned long long c,
+                                          unsigned long long sb) {
+  uint64_t s;
+  if (k == kGEmpty) {
+    s = cap;
+  } else {
+    s = gslot_start(k, log2cap);
+    while (true) {
+      const uint64_t cur = __hip_atomic_load(&slots[s].key, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == k) break;
+      if (cur == kGEmpty) {
+        const unsigned long long prev = atomicCAS((unsigned long long *)&slots[s].key, kGEmpty, k);
+        if (prev == kGEmpty || prev == k) break;
       }
+      s = (s + 1) & mask;
     }
-    slots[s].key = k;
-    slots[s].nb = (uint64_t)nb[i];
-    slots[s].nbv = (uint64_t)nbv[i];
-    slots[s].sb = (uint64_t)sb[i];
+  }
+  if (c) atomicAdd((unsigned long long *)&slots[s].cnt, c);
+  if (sb) atomicAdd((unsigned long long *)&slots[s].sb, sb);
+}
+
+// The build pieces of a partition aggregated per key into its table (a
+// partition holds fewer than 2^32 build rows, so both counts fit one word).
+// A workgroup first combines each tile of kGBTile rows in an LDS table of
+// kGBSlots keys, then adds each of its keys to the global table once: a
+// Zipf-hot key (a quarter of some partition's build rows at s = 0.8) costs
+// one global atomic per tile instead of one per row, which the atomics on
+// one address would serialize.  Rows whose key finds no LDS slot within
+// kGBProbe steps go to the global table directly.
+constexpr int kGBBlock = 512;
+constexpr int kGBTile = 2048;
+constexpr int kGBSlots = 4096;
+constexpr int kGBProbe = 16;
+
+__global__ void __launch_bounds__(kGBBlock) k_grace_build(const GPiece *pcs, int npcs, GSlot *slots,
+                                                          uint64_t mask, int log2cap,
+                                                          uint64_t cap) {
+  extern __shared__ unsigned long long g_lds[];
+  unsigned long long *s_key = g_lds;               // kGBSlots (kGEmpty: free)
+  unsigned long long *s_cnt = s_key + kGBSlots;    // kGBSlots
+  unsigned long long *s_sb = s_cnt + kGBSlots;     // kGBSlots
+  for (int l = threadIdx.x; l < kGBSlots; l += kGBBlock) {
+    s_key[l] = kGEmpty;
+    s_cnt[l] = 0;
+    s_sb[l] = 0;
+  }
+  __syncthreads();
+  for (int q = 0; q < npcs; q++) {
+    const DCol key = pcs[q].key, pay = pcs[q].pay;
+    const int64_t n = pcs[q].n;
+    const int64_t ntiles = (n + kGBTile - 1) / kGBTile;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+#pragma unroll
+      for (int r = 0; r < kGBTile / kGBBlock; r++) {
+        const int64_t i = t * kGBTile + (int64_t)r * kGBBlock + threadIdx.x;
+        if (i >= n || !dev::valid(key, i)) continue;  // a NULL key joins nothing
+        const uint64_t k = (uint64_t)dev::load_i64(key, i);
+        const int64_t pval = dev::load_i64(pay, i);
+        const bool pv = dev::valid(pay, i) && pval >= 0;  // -1: a NULL payload
+        const unsigned long long c = 1ull | ((unsigned long long)pv << 32);
+        const unsigned long long sb = pv ? (unsigned long long)pval : 0ull;
+        int l = -1;
+        if (k != kGEmpty) {
+          int h = (int)((dev::table_hash(k, 0x9E3779B97F4A7C15ULL) >> 32) & (kGBSlots - 1));
+          for (int pr = 0; pr < kGBProbe; pr++) {
+            const unsigned long long cur = s_key[h];
+            if (cur == k) {
+              l = h;
+              break;
+            }
+            if (cur == kGEmpty) {
+              const unsigned long long prev = atomicCAS(&s_key[h], kGEmpty, k);
+              if (prev == kGEmpty || prev == k) {
+                l = h;
+                break;
+              }
+            }
+            h = (h + 1) & (kGBSlots - 1);
+          }
+        }
+        if (l >= 0) {
+          atomicAdd(&s_cnt[l], c);
+          if (sb) atomicAdd(&s_sb[l], sb);
+        } else {
+          grace_add(slots, mask, log2cap, cap, k, c, sb);
+        }
+      }
+      __syncthreads();
+      for (int l = threadIdx.x; l < kGBSlots; l += kGBBlock) {
+        const unsigned long long k = s_key[l];
+        if (k == kGEmpty) continue;
+        grace_add(slots, mask, log2cap, cap, k, s_cnt[l], s_sb[l]);
+        s_key[l] = kGEmpty;
+        s_cnt[l] = 0;
+        s_sb[l] = 0;
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -64,35 +162,41 @@ __global__ void k_grace_init(GSlot *slots, uint64_t n) {
     slots[s] = GSlot{kGEmpty, 0, 0, 0};
 }
 
-// Probe rows against the table; out[0..2] of the partition: pairs, pairs
-// with both payloads non-NULL, payload-product sum (all wrapping u64).
-__global__ void k_grace_probe(DCol key, DCol pay, int64_t n, const GSlot *slots, uint64_t mask,
+// The probe pieces of a partition against its table; out[0..2] of the
+// partition: pairs, pairs with both payloads non-NULL, payload-product sum
+// (all wrapping u64).
+__global__ void k_grace_probe(const GPiece *pcs, int npcs, const GSlot *slots, uint64_t mask,
                               int log2cap, uint64_t cap, unsigned long long *out) {
   uint64_t cnt = 0, vcnt = 0, sum = 0;
-  for (int64_t b = blockIdx.x * (int64_t)blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = b + threadIdx.x;
-    if (i < n && dev::valid(key, i)) {
-      uint64_t k = (uint64_t)dev::load_i64(key, i);
-      const GSlot *hit = nullptr;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int q = 0; q < npcs; q++) {
+    const DCol key = pcs[q].key, pay = pcs[q].pay;
+    const int64_t n = pcs[q].n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+      if (!dev::valid(key, i)) continue;
+      const uint64_t k = (uint64_t)dev::load_i64(key, i);
+      int64_t hit = -1;
       if (k == kGEmpty) {
-        if (slots[cap].nb) hit = &slots[cap];
+        if (slots[cap].cnt) hit = (int64_t)cap;
       } else {
         uint64_t s = gslot_start(k, log2cap);
         while (true) {
-          uint64_t sk = slots[s].key;
+          const uint64_t sk = slots[s].key;
           if (sk == k) {
-            hit = &slots[s];
+            hit = (int64_t)s;
             break;
           }
           if (sk == kGEmpty) break;
           s = (s + 1) & mask;
         }
       }
-      if (hit) {
-        cnt += hit->nb;
-        if (dev::valid(pay, i)) {
-          vcnt += hit->nbv;
-          sum += (uint64_t)dev::load_i64(pay, i) * hit->sb;
+      if (hit >= 0) {
+        const uint64_t c = slots[hit].cnt;
+        cnt += c & 0xFFFFFFFFull;
+        const int64_t pval = dev::load_i64(pay, i);
+        if (dev::valid(pay, i) && pval >= 0) {  // -1: a NULL payload
+          vcnt += c >> 32;
+          sum += (uint64_t)pval * slots[hit].sb;
         }
       }
     }
@@ -109,25 +213,168 @@ __global__ void k_grace_probe(DCol key, DCol pay, int64_t n, const GSlot *slots,
   }
 }
 
-// Range of the non-NULL payloads: [min, max] via atomics.
-__global__ void k_minmax_i64(DCol c, int64_t n, long long *mm) {
-  long long lo = INT64_MAX, hi = INT64_MIN;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (!dev::valid(c, i)) continue;
-    long long v = (long long)dev::load_i64(c, i);
-    lo = v < lo ? v : lo;
-    hi = v > hi ? v : hi;
+// ---------------------------------------------------------------- sink
+// Partition pass 1: per block (contiguous rows) the rows of each partition
+// (NULL keys dropped), in wave-private LDS counters; and the range of the
+// non-NULL payloads of rows with a key, one atomic per block.
+constexpr int kGPBlock = 256;
+
+__global__ void __launch_bounds__(kGPBlock) k_gp_hist(DCol key, DCol pay, int64_t n, uint32_t parts,
+                                                      int64_t per_block, uint32_t *hist,
+                                                      long long *mm) {
+  extern __shared__ uint32_t s_h[];  // [4][parts]
+  __shared__ long long s_lo[kGPBlock / 64], s_hi[kGPBlock / 64];
+  const int wave = threadIdx.x >> 6;
+  for (uint32_t p = threadIdx.x; p < 4 * parts; p += kGPBlock) s_h[p] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = lo + per_block < n ? lo + per_block : n;
+  long long vlo = LLONG_MAX, vhi = LLONG_MIN;
+  uint32_t *h = s_h + wave * parts;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kGPBlock) {
+    if (!dev::valid(key, i)) continue;
+    atomicAdd(&h[dev::partition_of(dev::value_hash(key, i), parts)], 1u);
+    if (dev::valid(pay, i)) {
+      const long long v = (long long)dev::load_i64(pay, i);
+      vlo = v < vlo ? v : vlo;
+      vhi = v > vhi ? v : vhi;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
-    long long a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
-    lo = a < lo ? a : lo;
-    hi = b > hi ? b : hi;
+    const long long a = __shfl_xor(vlo, off), b = __shfl_xor(vhi, off);
+    vlo = a < vlo ? a : vlo;
+    vhi = b > vhi ? b : vhi;
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicMin(mm, lo);
-    atomicMax(mm + 1, hi);
+    s_lo[wave] = vlo;
+    s_hi[wave] = vhi;
   }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < parts; p += kGPBlock)
+    hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_h[p] + s_h[parts + p] + s_h[2 * parts + p] +
+                                                 s_h[3 * parts + p];
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kGPBlock / 64; w++) {
+      vlo = s_lo[w] < vlo ? s_lo[w] : vlo;
+      vhi = s_hi[w] > vhi ? s_hi[w] : vhi;
+    }
+    if (vlo <= vhi) {
+      atomicMin(mm, vlo);
+      atomicMax(mm + 1, vhi);
+    }
+  }
+}
+
+// Exclusive scan of c[0..np) into s in LDS (np <= 8 * kGSBlock = 4096, the
+// most partitions); returns the total.
+constexpr int kGSBlock = 512;
+
+__device__ __forceinline__ uint32_t gs_block_scan(const uint32_t *c, uint32_t *s, uint32_t np,
+                                                  uint32_t *wsum) {
+  constexpr int PER = 8;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint32_t x[PER], tot = 0;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint32_t p = (uint32_t)t * PER + j;
+    x[j] = p < np ? c[p] : 0;
+    tot += x[j];
+  }
+  uint32_t inc = tot;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t woff = 0, all = 0;
+  for (int w = 0; w < kGSBlock / 64; w++) {
+    const uint32_t ws = wsum[w];
+    if (w < wv) woff += ws;
+    all += ws;
+  }
+  uint32_t run = woff + inc - tot;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint32_t p = (uint32_t)t * PER + j;
+    if (p < np) s[p] = run;
+    run += x[j];
+  }
+  __syncthreads();
+  return all;
+}
+
+__host__ __device__ constexpr size_t gs_lds_bytes(int R, uint32_t parts) {
+  return (size_t)kGSBlock * R * 16 + (size_t)parts * 16 + (size_t)kGSBlock * R * 2 +
+         kGSBlock / 64 * 4;
+}
+
+// Partition pass 2: tiles of kGSBlock * R rows sorted by partition in LDS,
+// then each partition's run written by consecutive lanes: out_key/out_pay
+// rows [key, payload or -1].
+template <int R>
+__global__ void __launch_bounds__(kGSBlock) k_gp_scatter(DCol key, DCol pay, int64_t n,
+                                                         uint32_t parts, int64_t per_block,
+                                                         const uint64_t *base, int64_t *out_key,
+                                                         int64_t *out_pay) {
+  constexpr int TT = kGSBlock * R;
+  extern __shared__ uint64_t smem[];
+  uint64_t *st_k = smem;                                       // TT
+  uint64_t *st_v = st_k + TT;                                  // TT
+  uint64_t *cur = st_v + TT;                                   // parts
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(cur + parts);   // parts
+  uint32_t *start = cnt + parts;                               // parts
+  uint16_t *st_p = reinterpret_cast<uint16_t *>(start + parts);  // TT
+  uint32_t *wsum = reinterpret_cast<uint32_t *>(st_p + TT);      // kGSBlock / 64
+  for (uint32_t p = threadIdx.x; p < parts; p += kGSBlock)
+    cur[p] = base[(uint64_t)p * gridDim.x + blockIdx.x];
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = lo + per_block < n ? lo + per_block : n;
+  for (int64_t t0 = lo; t0 < hi; t0 += TT) {
+    for (uint32_t p = threadIdx.x; p < parts; p += kGSBlock) cnt[p] = 0;
+    __syncthreads();
+    uint64_t kk[R], vv[R];
+    uint32_t pp[R], lp[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t i = t0 + (int64_t)r * kGSBlock + threadIdx.x;
+      const bool live = i < hi && dev::valid(key, i);
+      kk[r] = live ? (uint64_t)dev::load_i64(key, i) : 0;
+      vv[r] = live ? (dev::valid(pay, i) ? (uint64_t)dev::load_i64(pay, i) : ~0ull) : 0;
+      pp[r] = live ? dev::partition_of(dev::value_hash(key, i), parts) : 0xffffffffu;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) lp[r] = pp[r] != 0xffffffffu ? atomicAdd(&cnt[pp[r]], 1u) : 0u;
+    __syncthreads();
+    const uint32_t tn = gs_block_scan(cnt, start, parts, wsum);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (pp[r] == 0xffffffffu) continue;
+      const uint32_t sl = start[pp[r]] + lp[r];
+      st_k[sl] = kk[r];
+      st_v[sl] = vv[r];
+      st_p[sl] = (uint16_t)pp[r];
+    }
+    __syncthreads();
+    for (uint32_t sl = threadIdx.x; sl < tn; sl += kGSBlock) {
+      const uint32_t p = st_p[sl];
+      const uint64_t d = cur[p] + (sl - start[p]);
+      out_key[d] = (int64_t)st_k[sl];
+      out_pay[d] = (int64_t)st_v[sl];
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < parts; p += kGSBlock) cur[p] += cnt[p];
+    __syncthreads();
+  }
+}
+
+// Partition p's first row: base[p * nb] (the scan of the partition-major
+// histogram), and the row count at offs[parts].
+__global__ void k_gp_offsets(const uint64_t *base, const uint32_t *hist, uint64_t hn, uint32_t parts,
+                             uint32_t nb, int64_t *offs) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= parts; p += gridDim.x * blockDim.x)
+    offs[p] = p < parts ? (int64_t)base[(uint64_t)p * nb] : (int64_t)(base[hn - 1] + hist[hn - 1]);
 }
 
 static void ck(int rc) {
@@ -149,12 +396,12 @@ struct kg_grace {
   hipStream_t s = nullptr;
   uint32_t parts = 64;
   uint64_t table_budget = 0;
-  // one partitioned piece per sunk batch: device (a kg_batch) or host copy
+  // one partitioned piece per sunk batch: [key], [payload or -1 for NULL]
+  // in HBM, or their copies in pinned host memory (hipHostMalloc)
   struct Piece {
-    kg_batch *dev = nullptr;
-    // the spilled copy, in pinned host memory (hipHostMalloc)
-    uint8_t *hk = nullptr, *hv = nullptr, *hp = nullptr, *hpv = nullptr;
-    bool key_nullable = false, pay_nullable = false;
+    DevBuf key, pay;
+    int64_t *hk = nullptr, *hp = nullptr;
+    bool on_device = false;
     int64_t rows = 0;
     std::vector<int64_t> off;
   };
@@ -166,37 +413,29 @@ struct kg_grace {
 namespace kg {
 
 static void free_piece(kg_grace::Piece &p) {
-  if (p.dev) kg_batch_release(p.dev);
-  p.dev = nullptr;
-  for (uint8_t *q : {p.hk, p.hv, p.hp, p.hpv})
+  p.key.release();
+  p.pay.release();
+  p.on_device = false;
+  for (int64_t *q : {p.hk, p.hp})
     if (q) (void)hipHostFree(q);
-  p.hk = p.hv = p.hp = p.hpv = nullptr;
+  p.hk = p.hp = nullptr;
 }
 
 // Moves a partitioned piece from HBM to pinned host memory.
 static void spill_piece(kg_grace *g, kg_grace::Piece &piece) {
-  if (!piece.dev) return;
-  kg_column dk{}, dp{};
-  ck(kg_batch_column(piece.dev, 0, &dk));
-  ck(kg_batch_column(piece.dev, 1, &dp));
-  int64_t n = piece.rows;
-  size_t vb = (size_t)ceil_div(n, 32) * 4;
-  KG_HIP(hipHostMalloc((void **)&piece.hk, (size_t)n * 8, hipHostMallocDefault));
-  KG_HIP(hipHostMalloc((void **)&piece.hp, (size_t)n * 8, hipHostMallocDefault));
-  KG_HIP(hipMemcpyAsync(piece.hk, dk.values, (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
-  KG_HIP(hipMemcpyAsync(piece.hp, dp.values, (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
-  if (dk.validity) {
-    KG_HIP(hipHostMalloc((void **)&piece.hv, vb, hipHostMallocDefault));
-    KG_HIP(hipMemcpyAsync(piece.hv, dk.validity, vb, hipMemcpyDeviceToHost, g->s));
-  }
-  if (dp.validity) {
-    KG_HIP(hipHostMalloc((void **)&piece.hpv, vb, hipHostMallocDefault));
-    KG_HIP(hipMemcpyAsync(piece.hpv, dp.validity, vb, hipMemcpyDeviceToHost, g->s));
+  if (!piece.on_device) return;
+  const int64_t n = piece.rows;
+  KG_HIP(hipHostMalloc((void **)&piece.hk, (size_t)n * 8 + 8, hipHostMallocDefault));
+  KG_HIP(hipHostMalloc((void **)&piece.hp, (size_t)n * 8 + 8, hipHostMallocDefault));
+  if (n) {
+    KG_HIP(hipMemcpyAsync(piece.hk, piece.key.get(), (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
+    KG_HIP(hipMemcpyAsync(piece.hp, piece.pay.get(), (size_t)n * 8, hipMemcpyDeviceToHost, g->s));
   }
   KG_HIP(hipStreamSynchronize(g->s));
-  kg_batch_release(piece.dev);
-  piece.dev = nullptr;
-  g->spilled += (uint64_t)n * 16 + (dk.validity ? vb : 0) + (dp.validity ? vb : 0);
+  piece.key.release();
+  piece.pay.release();
+  piece.on_device = false;
+  g->spilled += (uint64_t)n * 16;
 }
 
 // HBM a sub-batch of `rows` needs while it is partitioned: the staged key and
@@ -235,26 +474,60 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
     v.offset += done;
     k.length -= done;
     v.length -= done;
-    // payload range for the factorization check
-    {
-      Staged st(g->ctx, g->s);
-      DCol pc = st.add(v, m);
-      k_minmax_i64<<<stream_grid(m, 256), 256, 0, g->s>>>(pc, m,
-                                                          g->mm.get<long long>() + 2 * side);
-      KG_HIP(hipGetLastError());
-      KG_HIP(hipStreamSynchronize(g->s));
-    }
-    kg_column cols[2] = {k, v};
-    int32_t kc = 0;
     kg_grace::Piece piece;
     piece.off.assign(g->parts + 1, 0);
-    ck(kg_repartition(g->ctx, 2, cols, m, 1, &kc, g->parts, piece.off.data(), &piece.dev));
-    piece.rows = m;
-    kg_column dk{}, dp{};
-    ck(kg_batch_column(piece.dev, 0, &dk));
-    ck(kg_batch_column(piece.dev, 1, &dp));
-    piece.key_nullable = dk.validity != nullptr;
-    piece.pay_nullable = dp.validity != nullptr;
+    {
+      Staged st(g->ctx, g->s);
+      const DCol dk = st.add(k, m), dv = st.add(v, m);
+      const uint32_t parts = g->parts;
+      const int64_t cus = g->ctx->num_cus > 0 ? g->ctx->num_cus : 256;
+      const int R = parts <= 1024 ? 12 : 8;
+      const int64_t tt = (int64_t)kGSBlock * R;
+      int64_t per_block = ceil_div(ceil_div(m, cus * 4), tt) * tt;
+      const int64_t nb = ceil_div(m, per_block);
+      const uint64_t hn = (uint64_t)parts * nb;
+      DevBuf hist(g->ctx, hn * 4 + 4), base(g->ctx, hn * 8 + 8), offs(g->ctx, (parts + 1) * 8);
+      {
+        ProfScope prof("hash_partition", g->s);
+        KG_HIP(hipFuncSetAttribute((const void *)k_gp_hist,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(parts * 16)));
+        k_gp_hist<<<(unsigned)nb, kGPBlock, parts * 16, g->s>>>(
+            dk, dv, m, parts, per_block, hist.get<uint32_t>(), g->mm.get<long long>() + 2 * side);
+        KG_HIP(hipGetLastError());
+      }
+      size_t tb = 0;
+      KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist.get<uint32_t>(),
+                                              base.get<uint64_t>(), (int)hn, g->s));
+      DevBuf temp(g->ctx, tb + 16);
+      KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, hist.get<uint32_t>(),
+                                              base.get<uint64_t>(), (int)hn, g->s));
+      piece.key.reset(g->ctx, (uint64_t)m * 8 + 8);
+      piece.pay.reset(g->ctx, (uint64_t)m * 8 + 8);
+      {
+        ProfScope prof("scatter", g->s);
+        const size_t lds = gs_lds_bytes(R, parts);
+        const void *fn = R == 12 ? (const void *)k_gp_scatter<12> : (const void *)k_gp_scatter<8>;
+        KG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (R == 12)
+          k_gp_scatter<12><<<(unsigned)nb, kGSBlock, lds, g->s>>>(
+              dk, dv, m, parts, per_block, base.get<uint64_t>(), piece.key.get<int64_t>(),
+              piece.pay.get<int64_t>());
+        else
+          k_gp_scatter<8><<<(unsigned)nb, kGSBlock, lds, g->s>>>(
+              dk, dv, m, parts, per_block, base.get<uint64_t>(), piece.key.get<int64_t>(),
+              piece.pay.get<int64_t>());
+        KG_HIP(hipGetLastError());
+      }
+      k_gp_offsets<<<(parts + 256) / 256, 256, 0, g->s>>>(base.get<uint64_t>(), hist.get<uint32_t>(),
+                                                          hn, parts, (uint32_t)nb,
+                                                          offs.get<int64_t>());
+      KG_HIP(hipGetLastError());
+      KG_HIP(hipMemcpyAsync(piece.off.data(), offs.get(), (parts + 1) * 8, hipMemcpyDeviceToHost,
+                            g->s));
+      KG_HIP(hipStreamSynchronize(g->s));
+    }
+    piece.rows = piece.off[g->parts];
+    piece.on_device = true;
     if (!fits(g, 0)) spill_piece(g, piece);
     store.push_back(std::move(piece));
     done += m;
@@ -263,18 +536,13 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
 
 // Partition p's rows of a piece as key and payload columns (device or host).
 static void slice(const kg_grace::Piece &pc, uint32_t p, kg_column &k, kg_column &v, int64_t &n) {
-  int64_t a = pc.off[p], b = pc.off[p + 1];
+  const int64_t a = pc.off[p], b = pc.off[p + 1];
   n = b - a;
-  if (pc.dev) {
-    ck(kg_batch_column(pc.dev, 0, &k));
-    ck(kg_batch_column(pc.dev, 1, &v));
-  } else {
-    k = kg_column{KG_T_INT64, 0, 0, 0, pc.rows, 0, pc.hk, pc.hv};
-    v = kg_column{KG_T_INT64, 0, 0, 0, pc.rows, 0, pc.hp, pc.hpv};
-  }
-  k.offset = a;
-  v.offset = a;
-  k.length = v.length = b;
+  const uint32_t fl = pc.on_device ? KG_COL_DEVICE : 0u;
+  const void *kv = pc.on_device ? (const void *)pc.key.get() : (const void *)pc.hk;
+  const void *pv = pc.on_device ? (const void *)pc.pay.get() : (const void *)pc.hp;
+  k = kg_column{KG_T_INT64, 0, 0, fl, b, a, kv, nullptr};
+  v = kg_column{KG_T_INT64, 0, 0, fl, b, a, pv, nullptr};
 }
 
 }  // namespace kg
@@ -311,7 +579,8 @@ int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_va
   return guarded([&] {
     StreamScope kg_scope(g->s);
     long long mm[4];
-    KG_HIP(hipMemcpy(mm, g->mm.get(), 32, hipMemcpyDeviceToHost));
+    KG_HIP(hipMemcpyAsync(mm, g->mm.get(), 32, hipMemcpyDeviceToHost, g->s));
+    KG_HIP(hipStreamSynchronize(g->s));
     bool any_b = mm[0] <= mm[1], any_p = mm[2] <= mm[3];
     if (any_b && any_p) {
       // every pair's product in [0, 2^31): the per-pair % 2^31 is the identity
@@ -324,69 +593,78 @@ int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_va
     }
     DevBuf out(g->ctx, (uint64_t)g->parts * 3 * 8);
     KG_HIP(hipMemsetAsync(out.get(), 0, out.bytes(), g->s));
-    const kg_column key_t{KG_T_INT64, 0, 0, 0, 0, 0, nullptr, nullptr};
-    const kg_column in_t{KG_T_INT64, 0, 0, 0, 0, 0, nullptr, nullptr};
-    kg_agg_spec aggs[3] = {{KG_AGG_COUNT_STAR, 0}, {KG_AGG_COUNT, 0}, {KG_AGG_SUM, 0}};
+    // one table buffer, sized for the partition with the most build rows
+    // (an upper bound of its distinct keys), reused by every partition
+    std::vector<int64_t> nbp(g->parts, 0), npp(g->parts, 0);
+    int64_t maxb = 0;
     for (uint32_t p = 0; p < g->parts; p++) {
-      int64_t nb = 0, np = 0;
-      for (auto &pc : g->build) nb += pc.off[p + 1] - pc.off[p];
-      for (auto &pc : g->probe) np += pc.off[p + 1] - pc.off[p];
-      if (nb == 0 || np == 0) continue;
-      // 1. the build partition aggregated per key
-      kg_hash_agg *agg = nullptr;
-      ck(kg_agg_create(g->ctx, &key_t, 1, &in_t, 3, aggs, 0, (uint64_t)nb, &agg));
-      std::unique_ptr<kg_hash_agg, void (*)(kg_hash_agg *)> agg_guard(agg, kg_agg_destroy);
+      for (auto &pc : g->build) nbp[p] += pc.off[p + 1] - pc.off[p];
+      for (auto &pc : g->probe) npp[p] += pc.off[p + 1] - pc.off[p];
+      if (nbp[p] >= (1ll << 32)) fail(KG_ERR_UNSUPPORTED, "a grace partition of 2^32 build rows");
+      if (npp[p] > 0 && nbp[p] > maxb) maxb = nbp[p];
+    }
+    const uint64_t maxcap = 1ull << glog2((uint64_t)(maxb < 256 ? 256 : maxb) * 2);
+    if ((maxcap + 1) * sizeof(GSlot) > g->table_budget)
+      fail(KG_ERR_MEMORY, "a grace partition's table exceeds the table budget");
+    DevBuf slots(g->ctx, (maxcap + 1) * sizeof(GSlot));
+    // piece descriptors of every partition, in pinned host memory until
+    // their uploads (enqueued in order on the stream) have run
+    const size_t per_part = g->build.size() + g->probe.size();
+    GPiece *hdesc = nullptr;
+    KG_HIP(hipHostMalloc((void **)&hdesc, sizeof(GPiece) * per_part * g->parts + 64,
+                         hipHostMallocDefault));
+    std::unique_ptr<GPiece, void (*)(GPiece *)> hguard(hdesc, [](GPiece *q) {
+      (void)hipHostFree(q);
+    });
+    DevBuf ddesc(g->ctx, sizeof(GPiece) * per_part * g->parts + 64);
+    for (uint32_t p = 0; p < g->parts; p++) {
+      if (nbp[p] == 0 || npp[p] == 0) continue;
+      // host (spilled) pieces of this partition are staged into HBM; the
+      // staging buffers are released in the stream's order after the launches
+      Staged st(g->ctx, g->s);
+      GPiece *hb = hdesc + per_part * p;
+      int nb = 0, np = 0;
       for (auto &pc : g->build) {
         kg_column k, v;
         int64_t n;
         slice(pc, p, k, v, n);
-        if (n) ck(kg_agg_sink(agg, &k, &v, n));
+        if (n) hb[nb++] = GPiece{st.add(k, n), st.add(v, n), n};
       }
-      ck(kg_agg_finalize(agg));
-      kg_batch *res = nullptr;
-      ck(kg_agg_result(agg, &res));
-      std::unique_ptr<kg_batch, void (*)(kg_batch *)> res_guard(res, kg_batch_release);
-      int64_t groups = kg_batch_num_rows(res);
-      kg_column rk, rn, rnv, rs;
-      ck(kg_batch_column(res, 0, &rk));
-      ck(kg_batch_column(res, 1, &rn));
-      ck(kg_batch_column(res, 2, &rnv));
-      ck(kg_batch_column(res, 3, &rs));
-      // 2. the lookup table
-      uint64_t cap = 1ull << glog2((uint64_t)(groups < 256 ? 256 : groups) * 2);
-      if ((cap + 1) * sizeof(GSlot) > g->table_budget)
-        fail(KG_ERR_MEMORY, "a grace partition's table exceeds the table budget");
-      DevBuf slots(g->ctx, (cap + 1) * sizeof(GSlot));
-      k_grace_init<<<stream_grid(cap + 1, 256), 256, 0, g->s>>>(slots.get<GSlot>(), cap + 1);
-      {
-        ProfScope prof("grace_build", g->s);
-        k_grace_table<<<stream_grid(groups, 256), 256, 0, g->s>>>(
-            static_cast<const int64_t *>(rk.values), rk.validity,
-            static_cast<const int64_t *>(rn.values), static_cast<const int64_t *>(rnv.values),
-            static_cast<const int64_t *>(rs.values), groups, slots.get<GSlot>(), cap - 1,
-            glog2(cap), cap);
-        KG_HIP(hipGetLastError());
-      }
-      // 3. the probe partition streamed against it
+      GPiece *hp = hb + nb;
       for (auto &pc : g->probe) {
         kg_column k, v;
         int64_t n;
         slice(pc, p, k, v, n);
-        if (!n) continue;
-        // rows [offset, offset + n): device slices in place, host slices
-        // copied to HBM
-        Staged st(g->ctx, g->s);
-        DCol dk = st.add(k, n);
-        DCol dv = st.add(v, n);
+        if (n) hp[np++] = GPiece{st.add(k, n), st.add(v, n), n};
+      }
+      GPiece *db = ddesc.get<GPiece>() + per_part * p;
+      KG_HIP(hipMemcpyAsync(db, hb, sizeof(GPiece) * (nb + np), hipMemcpyHostToDevice, g->s));
+      const uint64_t cap = 1ull << glog2((uint64_t)(nbp[p] < 256 ? 256 : nbp[p]) * 2);
+      const int lc = glog2(cap);
+      k_grace_init<<<stream_grid(cap + 1, 256), 256, 0, g->s>>>(slots.get<GSlot>(), cap + 1);
+      {
+        ProfScope prof("grace_build", g->s);
+        const size_t lds = (size_t)kGBSlots * 3 * 8;
+        KG_HIP(hipFuncSetAttribute((const void *)k_grace_build,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const int64_t tiles = ceil_div(nbp[p], kGBTile);
+        const int64_t cus = g->ctx->num_cus > 0 ? g->ctx->num_cus : 256;
+        k_grace_build<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, cus)), kGBBlock,
+                        lds, g->s>>>(db, nb, slots.get<GSlot>(), cap - 1, lc, cap);
+        KG_HIP(hipGetLastError());
+      }
+      {
         ProfScope prof("grace_probe", g->s);
-        k_grace_probe<<<stream_grid(n, 256), 256, 0, g->s>>>(
-            dk, dv, n, slots.get<GSlot>(), cap - 1, glog2(cap), cap,
+        k_grace_probe<<<stream_grid(npp[p], 256), 256, 0, g->s>>>(
+            db + nb, np, slots.get<GSlot>(), cap - 1, lc, cap,
             out.get<unsigned long long>() + 3 * p);
         KG_HIP(hipGetLastError());
-        if (st.staged_host()) KG_HIP(hipStreamSynchronize(g->s));
       }
-      KG_HIP(hipStreamSynchronize(g->s));
+      // spilled pieces were staged into HBM: let them go before the next
+      // partition's, so the staging stays within one partition's rows
+      if (st.staged_host()) KG_HIP(hipStreamSynchronize(g->s));
     }
+    KG_HIP(hipStreamSynchronize(g->s));
     std::vector<uint64_t> h((size_t)g->parts * 3);
     KG_HIP(hipMemcpy(h.data(), out.get(), h.size() * 8, hipMemcpyDeviceToHost));
     for (uint32_t p = 0; p < g->parts; p++) {
