@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
   const int WL = P.W | 1;  // odd LDS record stride: records spread over the LDS banks
   for (int64_t base = blockIdx.x * chunk; base < nrows; base += (int64_t)gridDim.x * chunk) {
     for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) s_keys[t] = kEmpty;
-    for (int t = threadIdx.x; t < SLOTS * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
+    fill_lds_records(P, s_rec, SLOTS, WL);
     __syncthreads();
     int64_t end = base + chunk < nrows ? base + chunk : nrows;
     if (P.direct || !P.has_key) {
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, S
   const int W = P.W;
   const int WL = W | 1;  // odd LDS record stride: records spread over the banks
   for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) s_keys[t] = kEmpty;
-  for (int t = threadIdx.x; t < (SLOTS + 2) * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
+  fill_lds_records(P, s_rec, SLOTS + 2, WL);
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < nrows ? lo + chunk : nrows;
@@ -583,7 +583,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_agg_merge(AggProg P, Table T, M
   const int limit = S - S / 8;
   const uint32_t np = 1u << M.log2p;
   for (int t = threadIdx.x; t < S; t += blockDim.x) s_keys[t] = kEmpty;
-  for (int t = threadIdx.x; t < S * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
+  fill_lds_records(P, s_rec, S, WL);
   if (threadIdx.x == 0) s_occ = 0;
   __syncthreads();
   for (uint32_t p = blockIdx.x; p < np; p += gridDim.x) {

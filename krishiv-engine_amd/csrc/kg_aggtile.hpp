@@ -23,6 +23,17 @@ struct SpillFmt {
   uint8_t off[kAggInputs];
 };
 
+// Fills n LDS records (stride WL words) with the program's initial state.
+// The word index is uniform across the workgroup: a lane-dependent index
+// into the kernel argument can make the compiler copy the whole program to
+// scratch.
+__device__ __forceinline__ void fill_lds_records(const AggProg &P, uint64_t *s_rec, int n, int WL) {
+  for (int w = 0; w < WL; w++) {
+    const uint64_t v = P.init[w];
+    for (int s = threadIdx.x; s < n; s += blockDim.x) s_rec[(size_t)s * WL + w] = v;
+  }
+}
+
 // Copies a spill row layout into a program (host side, before a launch whose
 // kernels read spill rows).
 inline void set_spill_layout(AggProg &P, const SpillFmt &F) {

@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(kJDBlock) k_joinagg_direct(KeySpec K, int64_t 
   const int span = (int)T.dspan;
   uint64_t *s_rec = smem;                                                // span x WL
   uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_rec + (size_t)span * WL);  // span
-  for (int t = threadIdx.x; t < span * WL; t += kJDBlock) s_rec[t] = P.init[t % WL];
+  fill_lds_records(P, s_rec, span, WL);
   for (int d = threadIdx.x; d < span; d += kJDBlock) {
     const int32_t sl = T.dslot[d];
     s_cnt[d] = sl >= 0 ? T.slots[sl].count : 0u;
@@ -695,7 +695,7 @@ __global__ void __launch_bounds__(kJMBlock) k_jp_merge(JTable T, AggProg P, Spil
       s_keys[l] = T.slots[s].key;
       s_cnt[l] = T.slots[s].count;
     }
-    for (int t = threadIdx.x; t < Wn * WL; t += blockDim.x) s_rec[t] = P.init[t % WL];
+    fill_lds_records(P, s_rec, Wn, WL);
     __syncthreads();
     const int64_t a = (int64_t)base[(uint64_t)p * nb];
     const int64_t b = p + 1 < np ? (int64_t)base[(uint64_t)(p + 1) * nb] : (int64_t)nrows;
