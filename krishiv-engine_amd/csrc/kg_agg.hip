@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
 // pending inputs for the host, which grows the table, merges them with
 // atomics (k_agg_retry) and resumes the partition.
 constexpr int kP1Block = 512;
-constexpr int kMergeBlock = 512;
+constexpr int kMergeBlock = 1024;  // 16 waves a CU: latency of the spill reads hidden
+constexpr int kMergeRows = 2;      // rows a lane per tile (96 VGPRs at 1024 lanes)
 constexpr int kTileRows = 4;
 constexpr int kWaveChunk = 512;
 constexpr int64_t kEvictRows = 32768;
@@ -635,7 +636,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_agg_merge(AggProg P, Table T, M
           for (int r = 0; r < R; r++)
             if (!done[r]) {
               const uint32_t j = atomicAdd(&M.pend_n[blockIdx.x], 1u);
-              M.pend[(uint64_t)blockIdx.x * R * kMergeBlock + j] = u[r];
+              M.pend[(uint64_t)blockIdx.x * kMergeRows * kMergeBlock + j] = u[r];
             }
           if (threadIdx.x == 0) {
             M.pend_part[blockIdx.x] = p;
@@ -680,7 +681,7 @@ __global__ void k_agg_retry(AggProg P, Table T, MergeDev M, int S, uint32_t nwg)
     part_range(M.cbase, M.cnb, M.nrecs, p, np, c0, c1);
     const int64_t nrec = c1 - c0;
     for (uint32_t j = threadIdx.x; j < npd; j += blockDim.x) {
-      const int64_t u = M.pend[(uint64_t)g * kTileRows * kMergeBlock + j];
+      const int64_t u = M.pend[(uint64_t)g * kMergeRows * kMergeBlock + j];
       if (u < nrec) {
         const uint64_t *d = M.recs + (c0 + u) * (1 + W);
         merge_record(P, T.rec + global_slot(T, d[0]) * (uint64_t)W, d + 1);
@@ -1440,7 +1441,7 @@ static void drain(kg_hash_agg *a) {
   const int64_t grid = std::min<int64_t>(np, (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256));
   DevBuf resume(a->ctx, (uint64_t)np * 8), inserted(a->ctx, (uint64_t)np * 4);
   DevBuf retry(a->ctx, (uint64_t)grid * a->m_slots * (1 + W) * 8);
-  DevBuf pend(a->ctx, (uint64_t)grid * kTileRows * kMergeBlock * 8);
+  DevBuf pend(a->ctx, (uint64_t)grid * kMergeRows * kMergeBlock * 8);
   DevBuf ctr(a->ctx, (uint64_t)grid * 12 + 16);  // retry_n, pend_n, pend_part, stops
   KG_HIP(hipMemsetAsync(resume.get(), 0, resume.bytes(), a->s));
   AggProg P = a->prog;
@@ -1474,7 +1475,7 @@ static void drain(kg_hash_agg *a) {
     KG_HIP(hipMemsetAsync(ctr.get(), 0, ctr.bytes(), a->s));
     {
       ProfScope prof("agg_merge", a->s);
-      k_agg_merge<kTileRows><<<(unsigned)grid, kMergeBlock, a->m_lds, a->s>>>(P, table_of(a), M,
+      k_agg_merge<kMergeRows><<<(unsigned)grid, kMergeBlock, a->m_lds, a->s>>>(P, table_of(a), M,
                                                                             a->m_slots);
       KG_HIP(hipGetLastError());
     }
@@ -1921,7 +1922,7 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       a->m_lds = (size_t)a->m_slots * (1 + WL) * 8;
       const std::pair<const void *, size_t> fns[2] = {
           {(const void *)k_agg_pass1<kTileRows>, a->p1_lds},
-          {(const void *)k_agg_merge<kTileRows>, a->m_lds}};
+          {(const void *)k_agg_merge<kMergeRows>, a->m_lds}};
       for (auto &f : fns) {
         hipError_t e = hipFuncSetAttribute(f.first, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)f.second);
