@@ -192,10 +192,12 @@ def _type_col(ctype, precision=0, scale=0) -> Column:
 
 # ---------------------------------------------------------------- batches
 class Batch:
-    """A device-resident batch (kg_batch) owned by the library."""
+    """A device-resident batch (kg_batch) owned by the library.  It keeps its
+    context alive: a batch must be released before its context."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, ctx=None):
         self.h = handle
+        self.ctx = ctx
 
     def __del__(self):
         try:
@@ -309,7 +311,7 @@ def repartition(ctx: Context, cols, key_cols, partitions: int):
     h = C.c_void_p()
     check(lib().kg_repartition(ctx.h, len(cols), ca, n, len(key_cols), kc, partitions,
                                offs.ctypes.data, C.byref(h)))
-    return Batch(h), offs
+    return Batch(h, ctx), offs
 
 
 def select_range(ctx: Context, col: Col, lo: int, hi: int, out_rows: DeviceBuffer | None = None):
@@ -327,7 +329,7 @@ def gather(ctx: Context, cols, rows: DeviceBuffer, n: int) -> Batch:
     """The rows (device int32 ids) of `cols` as a device batch."""
     h = C.c_void_p()
     check(lib().kg_gather(ctx.h, len(cols), _cols(cols), C.c_void_p(rows.ptr), n, C.byref(h)))
-    return Batch(h)
+    return Batch(h, ctx)
 
 
 # ---------------------------------------------------------------- aggregation
@@ -369,7 +371,7 @@ class HashAggregate:
     def result(self) -> Batch:
         h = C.c_void_p()
         check(lib().kg_agg_result(self.h, C.byref(h)))
-        return Batch(h)
+        return Batch(h, self.ctx)
 
     def close(self):
         if self.h:
@@ -414,14 +416,14 @@ class HashJoin:
         nrows = cols[0].length if nrows is None else nrows
         h = C.c_void_p()
         check(lib().kg_join_probe(self.h, len(cols), _cols(cols), nrows, C.byref(h)))
-        return Batch(h)
+        return Batch(h, self.ctx)
 
     def probe_selected(self, cols, sel: DeviceBuffer, nsel: int, nrows=None) -> Batch:
         nrows = cols[0].length if nrows is None else nrows
         h = C.c_void_p()
         check(lib().kg_join_probe_selected(self.h, len(cols), _cols(cols), nrows,
                                            C.c_void_p(sel.ptr), nsel, C.byref(h)))
-        return Batch(h)
+        return Batch(h, self.ctx)
 
     # join fused with GROUP BY the build key
     def aggregate(self, input_types, aggs, ansi=False):
@@ -438,7 +440,7 @@ class HashJoin:
     def aggregate_result(self) -> Batch:
         h = C.c_void_p()
         check(lib().kg_joinagg_result(self.h, C.byref(h)))
-        return Batch(h)
+        return Batch(h, self.ctx)
 
     def close(self):
         if self.h:
@@ -620,7 +622,7 @@ class JoinAggServer:
         """The finished request's (key, sum, count) batch, in HBM."""
         h = C.c_void_p()
         check(lib().kg_server_take_result(self.h, rid, C.byref(h)))
-        return Batch(h)
+        return Batch(h, self.ctx)
 
     def pending(self) -> int:
         return int(lib().kg_server_pending(self.h))
