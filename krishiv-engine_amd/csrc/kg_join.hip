@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(kJDBlock) k_joinagg_direct(KeySpec K, int64_t 
 // the workgroup of a key's home range has rows for it.  A probe that walks
 // past the window falls back to the global table with atomics.
 constexpr int kJPBlock = 256;
-constexpr int kJMBlock = 512;
+constexpr int kJMBlock = 1024;
 constexpr int kJMargin = 128;
 constexpr int kJTile = 4;
 
@@ -1224,9 +1224,9 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
     (void)hipGetLastError();
     optin = 64 * 1024;
   }
-  // two workgroups a CU measured faster than one with twice the window
-  const uint64_t lds_cap = std::min<uint64_t>((uint64_t)std::min(optin, 160 * 1024) - 1024,
-                                              80 * 1024);
+  // one 1024-lane workgroup a CU with the whole LDS: twice the window of two
+  // 512-lane ones, so half the partitions (longer runs for the scatter)
+  const uint64_t lds_cap = (uint64_t)std::min(optin, 160 * 1024) - 1024;
   const uint64_t per_slot = 8 + 4 + (uint64_t)WL * 8;
   uint64_t C = 1;
   while (C * 2 <= j->cap && (C * 2 + kJMargin) * per_slot <= lds_cap) C *= 2;
