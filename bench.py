@@ -39,7 +39,8 @@ def parse():
     p.add_argument("--key", choices=["int64", "int8"], default="int64")
     p.add_argument("--cpu-rows", type=int, default=20_000_000)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workload", choices=["agg", "joinagg", "bloom", "serve", "grace"], default="agg",
+    p.add_argument("--workload", choices=["agg", "joinagg", "bloom", "serve", "grace", "calib"],
+                   default="agg",
                    help="agg: the hash aggregation (default); joinagg: the join of "
                         "SELECT key, SUM(amt), COUNT(*) ... GROUP BY key over 1M build rows; "
                         "bloom: the runtime Bloom filter feeding a join (15M build, 600M probe); "
@@ -53,6 +54,8 @@ def parse():
     p.add_argument("--requests", type=int, default=10_000)
     p.add_argument("--qps", type=float, default=400.0)
     p.add_argument("--chunk", type=int, default=48)
+    p.add_argument("--latency-out", default="",
+                   help="serve: write each request's probe rows and latencies (npz) here")
     p.add_argument("--probe-rows", type=int, default=200_000_000)
     p.add_argument("--build-rows", type=int, default=1_000_000,
                    help="joinagg: build rows (keys a permutation of 0..n-1; fk uniform over 1.2n); "
@@ -326,6 +329,13 @@ def bloom(args):
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
                      "kernel": "bloom probe (k_bloom_probe_bits + expand)", "kernel_avg_ms": avg},
+        "random_roofline": random_ceiling(
+            ctx, (sbits + 7) // 8 if spark else nbytes, 8 if spark else 32,
+            n / max(bp_n / args.steps, 1) / (avg / 1e3) if bp_ms > 0 else 0.0,
+            "bloom probe (k_bloom_probe_bits + expand)",
+            "probe rows per second against random 8-byte reads (Spark layout: up to k words "
+            "a row, the first zero bit ends it)" if spark else
+            "probe rows per second against random 32-byte reads (one filter block a row)"),
         "families_ms_per_step": {f: v[0] / args.steps for f, v in fams.items()},
         **({} if args.no_cpu or spark else {"cpu_baseline": cpu_bloom(nbytes)}),
     }), flush=True)
@@ -392,6 +402,15 @@ def grace(args):
     rows = nb + npr
     step_ms = dt * 1e3 / args.steps
     in_bytes = rows * 16
+    ph = phases(["hash_partition", "scatter", "grace_build", "grace_probe", "agg_update",
+                 "agg_partition", "agg_merge", "agg_output"])
+    # a partition's table: 2x its build rows rounded up to a power of two, 32-byte slots
+    tab = (1 << ((2 * nb // 64) - 1).bit_length()) * 32
+    gp = ph.get("grace_probe")
+    rr = (random_ceiling(ctx, tab, 32, (npr / 64) / (gp["avg_ms"] / 1e3), "k_grace_probe",
+                         "probe rows per second of one partition (1/64 of the probe rows) "
+                         "against random 32-byte reads from a table of a mean partition's size")
+          if gp else None)
     print(json.dumps({
         "metric": "grace join + per-partition aggregate rows/s (build + probe rows)",
         "value": rows * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
@@ -403,8 +422,8 @@ def grace(args):
                                f"{int((8 << 30) * args.scale)} B",
                    "pairs": res.get("pairs"), "spilled_bytes": res.get("spilled")},
         "step_hbm_frac": (in_bytes / (step_ms / 1e3)) / HBM_PEAK,
-        "phases": phases(["hash_partition", "scatter", "grace_build", "grace_probe", "agg_update",
-                          "agg_partition", "agg_merge", "agg_output"]),
+        "random_roofline": rr,
+        "phases": ph,
     }), flush=True)
 
 
@@ -517,15 +536,23 @@ def serve(args):
         return lat, busy_s, gen_s
 
     out = {}
+    dump = {"probe_rows": np.array([10 * nb_of(r) for r in range(R)]), "resident": resident}
     for name, batched in (("plain", False), ("batched", True)):
+        m0 = ctx.mem_stats()
         lat_s, busy_s, _ = run_form(batched, saturated=True)
         lat, busy, gen = run_form(batched)
+        m1 = ctx.mem_stats()
+        dump[name + "_saturated"], dump[name + "_at_qps"] = lat_s, lat
         out[name] = {"queries_per_s": R / busy_s,
                      "saturated_p50_ms": float(np.percentile(lat_s, 50) * 1e3),
                      "at_qps": {"qps": args.qps, "achieved_qps": R / busy,
                                 "p50_ms": float(np.percentile(lat, 50) * 1e3),
                                 "p99_ms": float(np.percentile(lat, 99) * 1e3)},
-                     "wall_s_excl_gen": busy_s, "gen_s": gen}
+                     "wall_s_excl_gen": busy_s, "gen_s": gen,
+                     "allocator": {k: m1[k] - m0[k] for k in ("hip_mallocs", "hip_frees",
+                                                              "cache_hits", "releases")}}
+    if args.latency_out:
+        np.savez(args.latency_out, **dump)
     print(json.dumps({
         "metric": "batched vs plain join-aggregate queries/s (saturated) and p50/p99 latency at a fixed QPS",
         "value": out["batched"]["queries_per_s"], "unit": "queries/s", "n_gpus": 1,
@@ -537,6 +564,46 @@ def serve(args):
         "plain": out["plain"], "batched": out["batched"],
         "batched_over_plain_qps": out["batched"]["queries_per_s"] / out["plain"]["queries_per_s"],
     }), flush=True)
+
+
+def random_ceiling(ctx, table_bytes: int, read_bytes: int, achieved_per_s: float, kernel: str,
+                   note: str):
+    """A probe phase's random reads per second against the measured ceiling
+    of random reads of read_bytes from a device table of (the next power of
+    two of) table_bytes, kg_calib_random_read on this GPU."""
+    import kgpu
+    tb = 1 << max(12, (int(table_bytes) - 1).bit_length())
+    peak = kgpu.calib_random_read(ctx, tb, read_bytes, 1 << 27)
+    return {"bound": "random reads", "kernel": kernel, "table_bytes": int(table_bytes),
+            "calib_table_bytes": tb, "read_bytes": read_bytes, "achieved": achieved_per_s / 1e9,
+            "peak": peak / 1e9, "unit": "G reads/s", "frac": achieved_per_s / peak, "note": note}
+
+
+def calib(args):
+    """Measured ceilings of the random-access phases on this GPU: random
+    reads of 8, 16 and 32 bytes at hashed entries of device tables of 1 MB to
+    4 GB (the probe phases' roofline at their table size), and no-return
+    64-bit global atomic adds over 1 to 2^28 words (global-table updates)."""
+    import kgpu
+    ctx = kgpu.Context(0)
+    reads = []
+    for mb in (1, 4, 16, 64, 256, 1024, 4096):
+        for b in (8, 16, 32):
+            r = kgpu.calib_random_read(ctx, mb << 20, b, 1 << 28)
+            reads.append({"table_mb": mb, "read_bytes": b, "g_reads_per_s": r / 1e9,
+                          "gb_per_s": r * b / 1e9})
+    atomics = []
+    for a in (1, 64, 4096, 1 << 16, 1 << 20, 1 << 24, 1 << 28):
+        r = kgpu.calib_atomic_add(ctx, a, (1 << 22) if a < 4096 else (1 << 27))
+        atomics.append({"addresses": a, "g_ops_per_s": r / 1e9})
+    ref = [x for x in reads if x["table_mb"] == 256 and x["read_bytes"] == 32][0]
+    print(json.dumps({
+        "metric": "random-read and global-atomic ceilings (calibration)",
+        "value": ref["g_reads_per_s"] * 1e9, "unit": "random 32-byte reads/s from a 256 MB table",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "u64", "data": "synthetic (hashed indexes)",
+        "config": {"workload": "kg_calib_random_read over 2^28 reads a launch, "
+                               "kg_calib_atomic_add over 2^27 adds (2^22 below 4096 addresses)"},
+        "random_reads": reads, "atomic_adds": atomics}), flush=True)
 
 
 def traffic_of(summary: str, launch_rows: int):
@@ -593,6 +660,9 @@ def main():
         return
     if args.workload == "grace":
         grace(args)
+        return
+    if args.workload == "calib":
+        calib(args)
         return
     import kgpu
     from kgpu import Col, DeviceBuffer

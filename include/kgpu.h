@@ -87,6 +87,21 @@ int kg_ctx_create(int device, uint64_t hbm_budget_bytes, kg_ctx **out);
 void kg_ctx_destroy(kg_ctx *ctx);
 /* Bytes of HBM currently held by the context's operators. */
 uint64_t kg_ctx_hbm_in_use(const kg_ctx *ctx);
+/* The context's device allocator: freed blocks are kept by size class (at
+ * most 12.5 % over a request above 2 MB) and reused once the work enqueued
+ * before the free has completed; hipFree, which waits for the whole device,
+ * runs only past a quarter of the device's memory cached or when hipMalloc fails
+ * (then the whole cache is released).  The analogue of the reference's
+ * memory-pool statistics (PoolSnapshot, crates/krishiv-memory/src/pool.rs:826). */
+typedef struct kg_mem_stats {
+  uint64_t in_use;      /* bytes of live allocations (size classes) */
+  uint64_t cached;      /* freed bytes kept for reuse */
+  uint64_t hip_mallocs; /* hipMalloc calls */
+  uint64_t hip_frees;   /* hipFree calls */
+  uint64_t cache_hits;  /* allocations served from the cache */
+  uint64_t releases;    /* whole-cache releases after a failed hipMalloc */
+} kg_mem_stats;
+int kg_ctx_mem_stats(kg_ctx *ctx, kg_mem_stats *out);
 
 /* Plain device memory, for callers that keep inputs resident in HBM. */
 int kg_dev_alloc(kg_ctx *ctx, uint64_t bytes, void **out);
@@ -330,9 +345,10 @@ void kg_grace_destroy(kg_grace *g);
  * requests that name them (the broadcast build tables of join/broadcast.rs).
  * Two batches can be in flight (two streams), so requests that arrive while
  * one batch runs are admitted into the next as arena space frees.  A request
- * whose probe has 4M rows or more is not batched: kg_server_progress runs it
- * alone through the radix-partitioned fused aggregate (kg_joinagg_*) while
- * the batches run on the device. */
+ * whose probe has 4M rows or more is not batched: one of two worker threads
+ * runs it through the radix-partitioned fused aggregate (kg_joinagg_*), on
+ * its own stream, while the batches run on the device (two large requests on
+ * the same resident table run one after the other). */
 typedef struct kg_server kg_server;
 typedef struct kg_serve_request {
   int64_t id;
@@ -387,6 +403,21 @@ void kg_prof_reset(void);
  * ("agg_update", "agg_flush", "join_build", "join_probe", "joinagg_probe",
  * "bloom_insert", "bloom_probe", "hash_partition", "scatter"). */
 int kg_prof_get(const char *family, double *total_ms, int64_t *launches);
+
+/* ---- measured ceilings of the random-access phases ----------------------
+ * No counterpart in the reference: calibration kernels that the probe
+ * phases' rates are reported against (as the streaming phases' are against
+ * HBM bandwidth).  kg_calib_random_read: random reads of read_bytes (8, 16
+ * or 32) at hashed entries of a device table of table_bytes (a power of
+ * two), 8 in flight per lane on 8 workgroups a CU.  kg_calib_atomic_add:
+ * no-return 64-bit global atomic adds at hashed words among `addresses` (a
+ * power of two; 1 puts every add on one word).  Each times `reps` launches
+ * of `ops` operations (after one untimed launch) with HIP events on the
+ * context's stream and returns the mean device ms per launch. */
+int kg_calib_random_read(kg_ctx *ctx, uint64_t table_bytes, int32_t read_bytes, int64_t ops,
+                         int32_t reps, double *ms_per_launch);
+int kg_calib_atomic_add(kg_ctx *ctx, uint64_t addresses, int64_t ops, int32_t reps,
+                        double *ms_per_launch);
 
 #ifdef __cplusplus
 }

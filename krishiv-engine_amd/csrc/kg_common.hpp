@@ -89,6 +89,9 @@ struct kg_ctx {
   std::mutex cache_mu;
   std::multimap<uint64_t, void *> cache;
   uint64_t cached = 0;
+  // allocator counters (kg_ctx_mem_stats); the plain ones under cache_mu
+  std::atomic<uint64_t> n_mallocs{0};
+  uint64_t n_frees = 0, n_hits = 0, n_releases = 0;
   // blocks freed with stream order: reusable once every event completes
   struct Pending {
     uint64_t rb;

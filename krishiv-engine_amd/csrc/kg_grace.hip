@@ -143,10 +143,35 @@ __global__ void __launch_bounds__(kGBBlock) k_grace_build(const GPiece *pcs, int
         }
       }
       __syncthreads();
-      for (int l = threadIdx.x; l < kGBSlots; l += kGBBlock) {
-        const unsigned long long k = s_key[l];
-        if (k == kGEmpty) continue;
-        grace_add(slots, mask, log2cap, cap, k, s_cnt[l], s_sb[l]);
+      // flush: the home-slot loads of a lane's kGBSlots / kGBBlock keys are
+      // all issued before the first insert
+      constexpr int kF = kGBSlots / kGBBlock;
+      uint64_t fk[kF], fs[kF], fcur[kF];
+#pragma unroll
+      for (int r = 0; r < kF; r++) {
+        fk[r] = s_key[threadIdx.x + r * kGBBlock];
+        fs[r] = gslot_start(fk[r], log2cap);
+        fcur[r] = fk[r] != kGEmpty ? __hip_atomic_load(&slots[fs[r]].key, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                                   : kGEmpty;
+      }
+#pragma unroll
+      for (int r = 0; r < kF; r++) {
+        const int l = threadIdx.x + r * kGBBlock;
+        if (fk[r] != kGEmpty) {
+          uint64_t at = fs[r], cur = fcur[r];
+          while (cur != fk[r]) {
+            if (cur == kGEmpty) {
+              const unsigned long long prev =
+                  atomicCAS((unsigned long long *)&slots[at].key, kGEmpty, fk[r]);
+              if (prev == kGEmpty || prev == fk[r]) break;
+            }
+            at = (at + 1) & mask;
+            cur = __hip_atomic_load(&slots[at].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          if (s_cnt[l]) atomicAdd((unsigned long long *)&slots[at].cnt, s_cnt[l]);
+          if (s_sb[l]) atomicAdd((unsigned long long *)&slots[at].sb, s_sb[l]);
+        }
         s_key[l] = kGEmpty;
         s_cnt[l] = 0;
         s_sb[l] = 0;
@@ -164,39 +189,59 @@ __global__ void k_grace_init(GSlot *slots, uint64_t n) {
 
 // The probe pieces of a partition against its table; out[0..2] of the
 // partition: pairs, pairs with both payloads non-NULL, payload-product sum
-// (all wrapping u64).
-__global__ void k_grace_probe(const GPiece *pcs, int npcs, const GSlot *slots, uint64_t mask,
-                              int log2cap, uint64_t cap, unsigned long long *out) {
+// (all wrapping u64).  kGPU rows a lane: their key and payload loads, then
+// their home slots (32 bytes: key, counts, sum), are issued before any is
+// used.  The NULL-sentinel key lives in slot `cap`, whose key field stays
+// EMPTY; a slot with no rows adds nothing, so no count test is needed.
+constexpr int kGPU = 4;
+
+__global__ void __launch_bounds__(256) k_grace_probe(const GPiece *pcs, int npcs,
+                                                     const GSlot *slots, uint64_t mask,
+                                                     int log2cap, uint64_t cap,
+                                                     unsigned long long *out) {
   uint64_t cnt = 0, vcnt = 0, sum = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * kGPU;
   for (int q = 0; q < npcs; q++) {
     const DCol key = pcs[q].key, pay = pcs[q].pay;
     const int64_t n = pcs[q].n;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
-      if (!dev::valid(key, i)) continue;
-      const uint64_t k = (uint64_t)dev::load_i64(key, i);
-      int64_t hit = -1;
-      if (k == kGEmpty) {
-        if (slots[cap].cnt) hit = (int64_t)cap;
-      } else {
-        uint64_t s = gslot_start(k, log2cap);
-        while (true) {
-          const uint64_t sk = slots[s].key;
-          if (sk == k) {
-            hit = (int64_t)s;
-            break;
-          }
-          if (sk == kGEmpty) break;
-          s = (s + 1) & mask;
-        }
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x * kGPU + threadIdx.x; i0 < n;
+         i0 += stride) {
+      uint64_t k[kGPU], at[kGPU], sk[kGPU], sc[kGPU], ss[kGPU];
+      int64_t pv[kGPU];
+      bool act[kGPU];
+#pragma unroll
+      for (int u = 0; u < kGPU; u++) {
+        const int64_t i = i0 + (int64_t)u * blockDim.x;
+        act[u] = i < n && dev::valid(key, i);
+        k[u] = act[u] ? (uint64_t)dev::load_i64(key, i) : 0;
+        // -1: a NULL payload
+        pv[u] = act[u] && dev::valid(pay, i) ? dev::load_i64(pay, i) : -1;
       }
-      if (hit >= 0) {
-        const uint64_t c = slots[hit].cnt;
-        cnt += c & 0xFFFFFFFFull;
-        const int64_t pval = dev::load_i64(pay, i);
-        if (dev::valid(pay, i) && pval >= 0) {  // -1: a NULL payload
-          vcnt += c >> 32;
-          sum += (uint64_t)pval * slots[hit].sb;
+#pragma unroll
+      for (int u = 0; u < kGPU; u++) {
+        at[u] = k[u] == kGEmpty ? cap : gslot_start(k[u], log2cap);
+        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(slots + at[u]);
+        const ulonglong2 a = act[u] ? p[0] : make_ulonglong2(kGEmpty, 0);
+        const ulonglong2 b = act[u] ? p[1] : make_ulonglong2(0, 0);
+        sk[u] = a.x;
+        sc[u] = a.y;
+        ss[u] = b.x;
+      }
+#pragma unroll
+      for (int u = 0; u < kGPU; u++) {
+        while (act[u] && sk[u] != k[u] && sk[u] != kGEmpty) {
+          at[u] = (at[u] + 1) & mask;
+          const GSlot t = slots[at[u]];
+          sk[u] = t.key;
+          sc[u] = t.cnt;
+          ss[u] = t.sb;
+        }
+        if (act[u] && sk[u] == k[u]) {
+          cnt += sc[u] & 0xFFFFFFFFull;
+          if (pv[u] >= 0) {
+            vcnt += sc[u] >> 32;
+            sum += (uint64_t)pv[u] * ss[u];
+          }
         }
       }
     }
@@ -655,7 +700,7 @@ int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_va
       }
       {
         ProfScope prof("grace_probe", g->s);
-        k_grace_probe<<<stream_grid(npp[p], 256), 256, 0, g->s>>>(
+        k_grace_probe<<<stream_grid(npp[p], 256, kGPU * 2), 256, 0, g->s>>>(
             db + nb, np, slots.get<GSlot>(), cap - 1, lc, cap,
             out.get<unsigned long long>() + 3 * p);
         KG_HIP(hipGetLastError());

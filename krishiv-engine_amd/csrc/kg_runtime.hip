@@ -71,18 +71,29 @@ hipStream_t alloc_stream() { return g_alloc_stream; }
 StreamScope::StreamScope(hipStream_t s) : prev_(g_alloc_stream) { g_alloc_stream = s; }
 StreamScope::~StreamScope() { g_alloc_stream = prev_; }
 
+// Size classes: powers of two up to 2 MB; above, eight classes per power of
+// two (at most 12.5 % over the request) in whole 2 MB pages, so that blocks
+// freed by one request are reused by the next of a similar size.  A class
+// size maps to itself.
 static uint64_t round_alloc(uint64_t b) {
   if (b <= (2u << 20)) {
     uint64_t r = 256;
     while (r < b) r <<= 1;
     return r;
   }
-  return (b + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+  const int e = 63 - __builtin_clzll(b - 1);  // 2^e < b <= 2^(e+1)
+  const uint64_t step = std::max<uint64_t>(2u << 20, 1ull << (e - 3));
+  return (b + step - 1) / step * step;
 }
+
+// Freed bytes kept for reuse, at most: hipFree waits for every stream of
+// the device, so blocks are returned to HIP only past this or when hipMalloc
+// fails.
+static uint64_t cache_cap(const kg_ctx *ctx) { return ctx->total_hbm / 4; }
 
 // (cache_mu held) moves the pending blocks whose events have completed (all
 // of them with `all`, after a device synchronize) into the cache, keeping
-// at most a quarter of the device's memory cached.
+// at most cache_cap bytes cached.
 static void reclaim_locked(kg_ctx *ctx, bool all) {
   size_t keep = 0;
   for (size_t i = 0; i < ctx->pending.size(); i++) {
@@ -101,8 +112,9 @@ static void reclaim_locked(kg_ctx *ctx, bool all) {
       keep++;
       continue;
     }
-    if (ctx->cached + pb.rb > ctx->total_hbm / 4) {
+    if (ctx->cached + pb.rb > cache_cap(ctx)) {
       (void)hipFree(pb.p);
+      ctx->n_frees++;
     } else {
       ctx->cache.emplace(pb.rb, pb.p);
       ctx->cached += pb.rb;
@@ -120,6 +132,8 @@ static void cache_release_all(kg_ctx *ctx) {
   std::lock_guard<std::mutex> g(ctx->cache_mu);
   reclaim_locked(ctx, true);
   for (auto &kv : ctx->cache) (void)hipFree(kv.second);
+  ctx->n_frees += ctx->cache.size();
+  ctx->n_releases++;
   ctx->cache.clear();
   ctx->cached = 0;
 }
@@ -158,14 +172,17 @@ void *dev_alloc(kg_ctx *ctx, uint64_t bytes) {
       void *p = it->second;
       ctx->cache.erase(it);
       ctx->cached -= rb;
+      ctx->n_hits++;
       return p;
     }
   }
   void *p = nullptr;
+  ctx->n_mallocs++;
   hipError_t e = hipMalloc(&p, rb);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     cache_release_all(ctx);
+    ctx->n_mallocs++;
     e = hipMalloc(&p, rb);
   }
   if (e != hipSuccess) {
@@ -188,8 +205,9 @@ void dev_free(kg_ctx *ctx, void *p, uint64_t bytes, hipStream_t stream) {
     (void)hipDeviceSynchronize();
     std::lock_guard<std::mutex> g(ctx->cache_mu);
     reclaim_locked(ctx, true);
-    if (ctx->cached + rb > ctx->total_hbm / 4) {
+    if (ctx->cached + rb > cache_cap(ctx)) {
       (void)hipFree(p);
+      ctx->n_frees++;
       return;
     }
     ctx->cache.emplace(rb, p);
@@ -383,6 +401,19 @@ void kg_ctx_destroy(kg_ctx *ctx) {
 }
 
 uint64_t kg_ctx_hbm_in_use(const kg_ctx *ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+int kg_ctx_mem_stats(kg_ctx *ctx, kg_mem_stats *out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(KG_ERR_INTERNAL, "no context or output");
+    std::lock_guard<std::mutex> g(ctx->cache_mu);
+    out->in_use = ctx->in_use.load();
+    out->cached = ctx->cached;
+    out->hip_mallocs = ctx->n_mallocs.load();
+    out->hip_frees = ctx->n_frees;
+    out->cache_hits = ctx->n_hits;
+    out->releases = ctx->n_releases;
+  });
+}
 
 int kg_dev_alloc(kg_ctx *ctx, uint64_t bytes, void **out) {
   return guarded([&] { *out = dev_alloc(ctx, bytes); });

@@ -9,8 +9,11 @@
 //   init    the batch's own tables (16-byte slots {key, rows}) and records
 //   build   build keys inserted into their request's table (CAS); rows counted
 //   probe   probe rows looked up in their request's table (own or resident);
-//           a hit adds rows to COUNT and pay*rows to the exact 128-bit SUM
+//           a hit adds rows to COUNT and pay*rows to SUM (64-bit, wrapping:
+//           SUM(bigint) without ANSI overflow checks)
 //   output  occupied records compacted into the request's output region
+// Tiles are sized per batch so that each phase has about two per workgroup:
+// a batch of a few small requests still spreads over every CU.
 // The four launches read one device descriptor at a fixed address, so they
 // are captured once per stream slot as a hipGraph and replayed per batch
 // after the descriptor is uploaded.  Two slots (streams at default priority)
@@ -20,6 +23,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <set>
 #include <thread>
 
 #include "kg_aggops.hpp"
@@ -27,10 +31,12 @@
 namespace kg {
 
 constexpr int kSrvBlock = 256;
-constexpr int64_t kSrvTile = 1 << 16;  // rows (or slots) per tile
-constexpr int kSrvSlots = 2;           // batches in flight
+constexpr int64_t kSrvMinTile = 512;    // rows (or slots) per tile, at least
+constexpr int64_t kSrvTile = 1 << 16;   // and at most
+constexpr int kSrvSlots = 2;            // batches in flight
+constexpr int kSrvU = 4;                // probe rows in flight per lane
 
-struct SSlot {
+struct alignas(16) SSlot {
   uint64_t key;
   uint64_t rows;
 };
@@ -46,8 +52,7 @@ struct SReq {
   uint64_t cap;
   int32_t log2cap;
   int32_t own;
-  uint64_t *cnt;  // (cap + 1) pair counts (the output scans this array only)
-  uint64_t *sum;  // (cap + 1) x {sum lo, sum hi}
+  uint64_t *acc;  // (cap + 1) x {COUNT, SUM}; a group exists where COUNT != 0
   int64_t *out_key, *out_sum, *out_cnt;
 };
 
@@ -89,9 +94,8 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_init(SDesc *D) {
   for_tiles(D, 0, [&](const SReq &r, int, int64_t lo, int64_t hi) {
     for (int64_t s = lo + threadIdx.x; s < hi; s += blockDim.x) {
       if (r.own) r.slots[s] = SSlot{kEmpty, 0};
-      r.cnt[s] = 0;
-      r.sum[2 * s] = 0;
-      r.sum[2 * s + 1] = 0;
+      r.acc[2 * s] = 0;
+      r.acc[2 * s + 1] = 0;
     }
   });
 }
@@ -123,32 +127,44 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_build(SDesc *D) {
   });
 }
 
-__device__ __forceinline__ int64_t sfind(const SSlot *slots, uint64_t cap, int log2cap,
-                                         uint64_t key) {
-  if (key == kEmpty) return slots[cap].rows ? (int64_t)cap : -1;
-  uint64_t s = sslot_start(key, log2cap), mask = cap - 1;
-  while (true) {
-    uint64_t k = slots[s].key;
-    if (k == key) return (int64_t)s;
-    if (k == kEmpty) return -1;
-    s = (s + 1) & mask;
-  }
-}
-
+// kSrvU rows a lane: their key and payload loads, then their home-slot loads
+// (16 bytes: key and rows together), are issued before any is used.  The
+// key equal to EMPTY lives in slot `cap`, whose key field stays EMPTY.
 __global__ void __launch_bounds__(kSrvBlock) k_srv_probe(SDesc *D) {
   for_tiles(D, 2, [&](const SReq &r, int, int64_t lo, int64_t hi) {
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      if (r.pvalid && !((r.pvalid[i >> 3] >> (i & 7)) & 1)) continue;
-      int64_t s = sfind(r.slots, r.cap, r.log2cap, (uint64_t)r.pkeys[i]);
-      if (s < 0) continue;
-      uint64_t m = r.slots[s].rows;
-      if (!m) continue;
-      add_u64_nr<false>(r.cnt + s, m);
-      int64_t v = r.ppay[i];
-      // pay * m as 128 bits (two's complement)
-      uint64_t plo = (uint64_t)v * m;
-      uint64_t phi = __umul64hi((uint64_t)v, m) - (v < 0 ? m : 0);
-      add128<false>(r.sum + 2 * s, plo, phi);
+    const uint64_t mask = r.cap - 1;
+    for (int64_t i0 = lo; i0 < hi; i0 += (int64_t)kSrvBlock * kSrvU) {
+      uint64_t key[kSrvU], pay[kSrvU], at[kSrvU];
+      bool act[kSrvU];
+#pragma unroll
+      for (int u = 0; u < kSrvU; u++) {
+        const int64_t i = i0 + (int64_t)u * kSrvBlock + threadIdx.x;
+        act[u] = i < hi && (!r.pvalid || ((r.pvalid[i >> 3] >> (i & 7)) & 1));
+        key[u] = act[u] ? (uint64_t)r.pkeys[i] : 0;
+        pay[u] = act[u] ? (uint64_t)r.ppay[i] : 0;
+      }
+      uint64_t k0[kSrvU], m0[kSrvU];
+#pragma unroll
+      for (int u = 0; u < kSrvU; u++) {
+        at[u] = key[u] == kEmpty ? r.cap : sslot_start(key[u], r.log2cap);
+        const SSlot t = act[u] ? r.slots[at[u]] : SSlot{kEmpty, 0};
+        k0[u] = t.key;
+        m0[u] = t.rows;
+      }
+#pragma unroll
+      for (int u = 0; u < kSrvU; u++) {
+        uint64_t k = k0[u], m = m0[u], s = at[u];
+        while (act[u] && k != key[u] && k != kEmpty) {
+          s = (s + 1) & mask;
+          const SSlot t = r.slots[s];
+          k = t.key;
+          m = t.rows;
+        }
+        if (act[u] && k == key[u] && m) {
+          add_u64_nr<false>(r.acc + 2 * s, m);
+          add_u64_nr<false>(r.acc + 2 * s + 1, pay[u] * m);
+        }
+      }
     }
   });
 }
@@ -159,7 +175,7 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_output(SDesc *D) {
   for_tiles(D, 3, [&](const SReq &r, int q, int64_t lo, int64_t hi) {
     for (int64_t s0 = lo; s0 < hi; s0 += blockDim.x) {
       const int64_t s = s0 + threadIdx.x;
-      const bool has = s < hi && r.cnt[s] != 0;
+      const bool has = s < hi && r.acc[2 * s] != 0;
       const uint64_t m = __ballot(has);
       if (!m) continue;
       const int lane = __lane_id();
@@ -170,8 +186,8 @@ __global__ void __launch_bounds__(kSrvBlock) k_srv_output(SDesc *D) {
       if (!has) continue;
       const uint64_t o = base + __popcll(m & ((1ull << lane) - 1));
       r.out_key[o] = s == (int64_t)r.cap ? (int64_t)kEmpty : (int64_t)r.slots[s].key;
-      r.out_sum[o] = (int64_t)r.sum[2 * s];  // SUM(bigint) wraps (non-ANSI)
-      r.out_cnt[o] = (int64_t)r.cnt[s];
+      r.out_sum[o] = (int64_t)r.acc[2 * s + 1];
+      r.out_cnt[o] = (int64_t)r.acc[2 * s];
     }
   });
 }
@@ -208,8 +224,9 @@ struct kg_server {
     uint64_t bytes = 0;
   };
   std::deque<Pending> queue;
-  // Probes of kLargeProbe rows or more run one at a time on a worker
-  // thread (their own streams), so batches keep launching meanwhile.
+  // Probes of kLargeProbe rows or more run on kLargeWorkers worker threads
+  // (each request on its own stream), so batches keep launching meanwhile
+  // and one large request's host waits are filled by another's kernels.
   struct Large {
     kg_serve_request r;
     kg_hash_join *table = nullptr;  // the resident table's join, if any
@@ -223,10 +240,11 @@ struct kg_server {
   std::mutex large_mu;
   std::condition_variable large_cv;
   std::deque<Large> large;
+  std::set<kg_hash_join *> large_busy;  // resident joins a worker is using
   std::deque<LargeDone> large_done;
   int64_t large_outstanding = 0;  // queued + running (main thread's count)
   bool large_stop = false;
-  std::thread large_worker;
+  std::vector<std::thread> large_workers;
   struct Slot {
     hipStream_t s = nullptr;
     DevBuf arena;          // tables, records, outputs of the batch
@@ -253,6 +271,7 @@ namespace kg {
 // which the batch's tile kernels (one global atomic per matching row) would
 // only slow down.  Smaller requests are packed into batches.
 constexpr int64_t kLargeProbe = 4 << 20;
+constexpr int kLargeWorkers = 2;
 constexpr int64_t kBatchProbeRows = 8 << 20;
 
 static void ck(int rc) {
@@ -302,16 +321,25 @@ static kg_batch *run_large(kg_server *S, const kg_serve_request &r, kg_hash_join
   return b;
 }
 
+// A worker takes the oldest queued large request whose resident table (the
+// join object its aggregate state lives in) no other worker is using.
 static void large_loop(kg_server *S) {
   (void)hipSetDevice(S->ctx->device);
   while (true) {
     kg_server::Large L;
     {
       std::unique_lock<std::mutex> g(S->large_mu);
-      S->large_cv.wait(g, [&] { return S->large_stop || !S->large.empty(); });
-      if (S->large.empty()) return;  // stopping
-      L = S->large.front();
-      S->large.pop_front();
+      auto pick = [&] {
+        for (auto it = S->large.begin(); it != S->large.end(); ++it)
+          if (!it->table || !S->large_busy.count(it->table)) return it;
+        return S->large.end();
+      };
+      S->large_cv.wait(g, [&] { return S->large_stop || pick() != S->large.end(); });
+      if (S->large_stop) return;
+      auto it = pick();
+      L = *it;
+      S->large.erase(it);
+      if (L.table) S->large_busy.insert(L.table);
     }
     kg_server::LargeDone d{L.r.id, nullptr, KG_OK, {}};
     try {
@@ -324,6 +352,7 @@ static void large_loop(kg_server *S) {
       d.msg = e.what();
     }
     std::lock_guard<std::mutex> g(S->large_mu);
+    if (L.table) S->large_busy.erase(L.table);
     S->large_done.push_back(std::move(d));
     S->large_cv.notify_all();
   }
@@ -359,7 +388,7 @@ static uint64_t req_cap(kg_server *S, const kg_serve_request &r) {
 
 // Arena bytes of a request: own table slots, records and output columns.
 static uint64_t req_bytes(uint64_t cap, bool own) {
-  uint64_t per = (own ? 16 : 0) + 24 + 24;
+  uint64_t per = (own ? 16 : 0) + 16 + 24;
   return ((cap + 1) * per + 255) / 256 * 256;
 }
 
@@ -373,8 +402,13 @@ static void layout_desc(kg_server *S, kg_server::Slot &sl) {
 
 static SDesc *desc_ptr(kg_server::Slot &sl) { return sl.desc_dev.get<SDesc>(); }
 
+// Workgroups of each persistent phase launch: four a CU (16 waves).
+static unsigned srv_grid(kg_server *S) {
+  return (unsigned)((S->ctx->num_cus > 0 ? S->ctx->num_cus : 256) * 4);
+}
+
 static void launch_phases(kg_server *S, kg_server::Slot &sl) {
-  unsigned grid = (unsigned)((S->ctx->num_cus > 0 ? S->ctx->num_cus : 256) * 4);
+  const unsigned grid = srv_grid(S);
   SDesc *D = desc_ptr(sl);
   k_srv_init<<<grid, kSrvBlock, 0, sl.s>>>(D);
   k_srv_build<<<grid, kSrvBlock, 0, sl.s>>>(D);
@@ -431,14 +465,27 @@ static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
       dbase + sizeof(SDesc) + sizeof(SReq) * S->max_batch + 4 * sizeof(STile) * sl.max_tiles);
   uint8_t *arena = sl.arena.get<uint8_t>();
   uint64_t at = 0;
+  // per phase, tiles of about total / (2 x grid) rows (or slots)
+  int64_t tot[4] = {0, 0, 0, 0};
+  for (const auto &p : sl.reqs) {
+    tot[0] += (int64_t)p.cap + 1;
+    tot[1] += p.r.build_keys ? p.r.build_rows : 0;
+    tot[2] += p.r.probe_rows;
+    tot[3] += (int64_t)p.cap + 1;
+  }
+  int64_t tile[4];
+  for (int ph = 0; ph < 4; ph++) {
+    const int64_t want = ceil_div(ceil_div(tot[ph], 2 * (int64_t)srv_grid(S)), 256) * 256;
+    tile[ph] = std::min(kSrvTile, std::max(kSrvMinTile, want));
+  }
   int ntiles[4] = {0, 0, 0, 0};
   auto add_tiles = [&](int ph, int q, int64_t n) {
-    for (int64_t lo = 0; lo < n; lo += kSrvTile) {
+    for (int64_t lo = 0; lo < n; lo += tile[ph]) {
       if ((uint64_t)ntiles[ph] >= sl.max_tiles) fail(KG_ERR_UNSUPPORTED, "too many tiles in a batch");
       STile &t = HT[sl.max_tiles * ph + ntiles[ph]++];
       t.req = q;
       t.lo = lo;
-      t.hi = lo + kSrvTile < n ? lo + kSrvTile : n;
+      t.hi = lo + tile[ph] < n ? lo + tile[ph] : n;
     }
   };
   for (int q = 0; q < (int)sl.reqs.size(); q++) {
@@ -454,9 +501,7 @@ static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
     } else {
       s.slots = S->tables[r.table].slots.get<SSlot>();
     }
-    s.cnt = reinterpret_cast<uint64_t *>(arena + at);
-    at += (s.cap + 1) * 8;
-    s.sum = reinterpret_cast<uint64_t *>(arena + at);
+    s.acc = reinterpret_cast<uint64_t *>(arena + at);
     at += (s.cap + 1) * 16;
     s.out_key = reinterpret_cast<int64_t *>(arena + at);
     at += (s.cap + 1) * 8;
@@ -568,7 +613,7 @@ int kg_server_add_table(kg_server *S, const int64_t *keys, int64_t n, int32_t *t
     t.cap = 1ull << slog2(want * 2);
     t.log2cap = slog2(t.cap);
     t.slots.reset(S->ctx, (t.cap + 1) * 16);
-    DevBuf scratch(S->ctx, (t.cap + 1) * 24);
+    DevBuf scratch(S->ctx, (t.cap + 1) * 16);
     // a one-request descriptor of its own (the stream slots' descriptors
     // may still hold a batch's group counts): init + build phases
     std::vector<STile> tiles[2];
@@ -597,8 +642,7 @@ int kg_server_add_table(kg_server *S, const int64_t *keys, int64_t n, int32_t *t
     r.cap = t.cap;
     r.log2cap = t.log2cap;
     r.own = 1;
-    r.cnt = scratch.get<uint64_t>();
-    r.sum = r.cnt + (t.cap + 1);
+    r.acc = scratch.get<uint64_t>();
     memcpy(h.data(), &D, sizeof(SDesc));
     memcpy(h.data() + sizeof(SDesc), &r, sizeof(SReq));
     memcpy(h.data() + sizeof(SDesc) + sizeof(SReq), tiles[0].data(), sizeof(STile) * tiles[0].size());
@@ -607,7 +651,7 @@ int kg_server_add_table(kg_server *S, const int64_t *keys, int64_t n, int32_t *t
              tiles[1].data(), sizeof(STile) * tiles[1].size());
     hipStream_t st = S->slots[0].s;
     KG_HIP(hipMemcpyAsync(dbase, h.data(), bytes, hipMemcpyHostToDevice, st));
-    unsigned grid = (unsigned)((S->ctx->num_cus > 0 ? S->ctx->num_cus : 256) * 4);
+    const unsigned grid = srv_grid(S);
     k_srv_init<<<grid, kSrvBlock, 0, st>>>(d.get<SDesc>());
     k_srv_build<<<grid, kSrvBlock, 0, st>>>(d.get<SDesc>());
     KG_HIP(hipGetLastError());
@@ -626,7 +670,8 @@ int kg_server_submit(kg_server *S, const kg_serve_request *r) {
     if (r->probe_rows >= kLargeProbe) {
       kg_server::Large L{*r, r->build_keys ? nullptr : S->tables[r->table].join};
       std::lock_guard<std::mutex> g(S->large_mu);
-      if (!S->large_worker.joinable()) S->large_worker = std::thread(large_loop, S);
+      while (S->large_workers.size() < (size_t)kLargeWorkers)
+        S->large_workers.emplace_back(large_loop, S);
       S->large.push_back(L);
       S->large_outstanding++;
       S->large_cv.notify_all();
@@ -694,7 +739,7 @@ void kg_server_destroy(kg_server *S) {
     S->large.clear();
     S->large_cv.notify_all();
   }
-  if (S->large_worker.joinable()) S->large_worker.join();
+  for (auto &t : S->large_workers) t.join();
   for (auto &d : S->large_done) kg_batch_release(d.b);
   for (auto &sl : S->slots) {
     if (sl.s) (void)hipStreamSynchronize(sl.s);

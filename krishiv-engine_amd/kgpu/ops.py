@@ -51,6 +51,13 @@ class Context:
     def hbm_in_use(self) -> int:
         return lib().kg_ctx_hbm_in_use(self.h)
 
+    def mem_stats(self) -> dict:
+        """The device allocator's counters (kg_ctx_mem_stats)."""
+        v = (C.c_uint64 * 6)()
+        check(lib().kg_ctx_mem_stats(self.h, v))
+        return dict(zip(("in_use", "cached", "hip_mallocs", "hip_frees", "cache_hits",
+                         "releases"), list(v)))
+
     def synchronize(self):
         check(lib().kg_ctx_synchronize(self.h))
 
@@ -533,6 +540,21 @@ def prof_get(family: str):
     n = C.c_int64(0)
     check(lib().kg_prof_get(family.encode(), C.byref(ms), C.byref(n)))
     return ms.value, n.value
+
+
+def calib_random_read(ctx, table_bytes: int, read_bytes: int, ops: int, reps: int = 3) -> float:
+    """Random reads per second from a device table of table_bytes (the
+    ceiling a probe phase over a table of that size is measured against)."""
+    ms = C.c_double(0)
+    check(lib().kg_calib_random_read(ctx.h, table_bytes, read_bytes, ops, reps, C.byref(ms)))
+    return ops / (ms.value * 1e-3)
+
+
+def calib_atomic_add(ctx, addresses: int, ops: int, reps: int = 3) -> float:
+    """No-return 64-bit global atomic adds per second over `addresses` words."""
+    ms = C.c_double(0)
+    check(lib().kg_calib_atomic_add(ctx.h, addresses, ops, reps, C.byref(ms)))
+    return ops / (ms.value * 1e-3)
 
 
 # ---------------------------------------------------------------- grace join

@@ -62,9 +62,12 @@ def test_mixed_batch_own_and_resident_tables(use_graph):
         res_keys.append(k)
         res_dev.append((srv.add_table(d, nb), d))
     keep, want = [], {}
-    # requests 9 and 11 probe 4.5M and 5M rows: they run through the
-    # partitioned fused aggregate (own build, resident table)
-    sizes = [1000, 4096, 1, 20000, 250, 70000, 9000, 1000, 3, 450_000, 2000, 500_000]
+    # requests 9, 11, 12, 14 and 17 probe 4.2M to 5M rows: they run through
+    # the partitioned fused aggregate on the large-request workers (own
+    # builds and resident tables; 11 and 17 share resident table 1, so the
+    # workers must take them one after the other)
+    sizes = [1000, 4096, 1, 20000, 250, 70000, 9000, 1000, 3, 450_000, 2000, 500_000,
+             480_000, 100, 420_000, 7, 30, 460_000]
     for rid, nb in enumerate(sizes):
         if rid % 3 == 2:
             t = rid % 2
