@@ -54,6 +54,10 @@ def parse():
     p.add_argument("--requests", type=int, default=10_000)
     p.add_argument("--qps", type=float, default=400.0)
     p.add_argument("--chunk", type=int, default=48)
+    p.add_argument("--autotune", action="store_true",
+                   help="joinagg: run the shape autotuner for this build size before timing")
+    p.add_argument("--large-workers", type=int, default=0,
+                   help="serve: large-request worker threads (KG_SERVE_LARGE_WORKERS; 0: the default)")
     p.add_argument("--latency-out", default="",
                    help="serve: write each request's probe rows and latencies (npz) here")
     p.add_argument("--probe-rows", type=int, default=200_000_000)
@@ -188,6 +192,12 @@ def joinagg(args):
     bcol = Col(A.T_INT64, bk, length=nb)
     pcols = [Col(A.T_INT64, fk, fkv, length=n), Col(A.T_INT64, amt, length=n)]
     groups = [0]
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        shape, tms, ncand = kgpu.autotune_joinagg(ctx, nb, n)
+        tuned = {"shape": shape, "est_ms": tms, "candidates": ncand,
+                 "tuning_s": time.perf_counter() - t0}
 
     def step():
         j = kgpu.HashJoin(ctx, A.JOIN_INNER, [0], [0], expected_build_rows=nb)
@@ -231,7 +241,8 @@ def joinagg(args):
         "data": "synthetic (seeded counter-based generator, generated in HBM)",
         "config": {"workload": f"SELECT key, SUM(amt), COUNT(*) join-aggregate, {nb} build rows, "
                                f"{n} probe rows, fk uniform [0, {nb * 12 // 10}), 1% NULL",
-                   "groups": groups[0]},
+                   "groups": groups[0], "shape": kgpu.tune_get(ctx, nb),
+                   **({"autotune": tuned} if tuned else {})},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None,
@@ -462,6 +473,8 @@ def serve(args):
         j.build_sink([Col(A.T_INT64, res_keys[t], length=int(res_sizes[t]))], int(res_sizes[t]))
         j.build_finalize()
         plain_tables.append(j)
+    if args.large_workers:
+        os.environ["KG_SERVE_LARGE_WORKERS"] = str(args.large_workers)
     srv = kgpu.JoinAggServer(ctx, arena_bytes=64 << 30, max_batch=256, use_graph=True)
     srv_tables = [srv.add_table(res_keys[t], int(res_sizes[t])) for t in range(8)]
 
@@ -560,7 +573,8 @@ def serve(args):
         "data": "synthetic (seeded counter-based generator, generated in HBM per chunk)",
         "config": {"workload": f"{R} join+aggregate requests, Poisson {args.qps} q/s (seed 8), build "
                                "log-uniform [1K, 10M], probe 10x, 30 % on 8 resident builds, "
-                               "arena 64 GB", "chunk": args.chunk},
+                               "arena 64 GB", "chunk": args.chunk,
+                   "large_workers": int(os.environ.get("KG_SERVE_LARGE_WORKERS", "1"))},
         "plain": out["plain"], "batched": out["batched"],
         "batched_over_plain_qps": out["batched"]["queries_per_s"] / out["plain"]["queries_per_s"],
     }), flush=True)

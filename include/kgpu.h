@@ -345,10 +345,12 @@ void kg_grace_destroy(kg_grace *g);
  * requests that name them (the broadcast build tables of join/broadcast.rs).
  * Two batches can be in flight (two streams), so requests that arrive while
  * one batch runs are admitted into the next as arena space frees.  A request
- * whose probe has 4M rows or more is not batched: one of two worker threads
- * runs it through the radix-partitioned fused aggregate (kg_joinagg_*), on
- * its own stream, while the batches run on the device (two large requests on
- * the same resident table run one after the other). */
+ * whose probe has 4M rows or more is not batched: a worker thread runs it
+ * through the radix-partitioned fused aggregate (kg_joinagg_*), on its own
+ * stream, while the batches run on the device.  One worker by default;
+ * the environment variable KG_SERVE_LARGE_WORKERS (1 to 4, read at create)
+ * runs that many large requests at once (two on the same resident table run
+ * one after the other). */
 typedef struct kg_server kg_server;
 typedef struct kg_serve_request {
   int64_t id;
@@ -403,6 +405,34 @@ void kg_prof_reset(void);
  * ("agg_update", "agg_flush", "join_build", "join_probe", "joinagg_probe",
  * "bloom_insert", "bloom_probe", "hash_partition", "scatter"). */
 int kg_prof_get(const char *family, double *total_ms, int64_t *launches);
+
+/* ---- shape autotuner of the hash join and its fused aggregate -----------
+ * No counterpart in the reference (its operators have no device shapes to
+ * pick).  Per size class of the build side (ceil log2 of its rows) a
+ * context keeps one shape: the table's load factor (capacity >= rows * 100 /
+ * load_factor_pct, a power of two; 50 by default), the staged partition
+ * scatter's rows per lane (4, 8 or 12; 0 = by the row width) and the
+ * partition merge's workgroup (1024 lanes with the whole LDS, or 512 with
+ * half).  kg_autotune_joinagg times all 18 candidates (load 50/25/75 x rows
+ * 12/8/4 x lanes 1024/512) on synthetic inputs of the shape (unique build
+ * keys, probe keys uniform over 1.2x their range, 1 % NULL, SUM and COUNT;
+ * at most 2^26 probe rows, scaled to probe_rows) and keeps the fastest
+ * (build + probe time).  Results never depend on the shape. */
+typedef struct kg_tune_config {
+  int32_t load_factor_pct;
+  int32_t scatter_rows;
+  int32_t merge_lanes;
+  int32_t _pad;
+} kg_tune_config;
+int kg_tune_get(kg_ctx *ctx, int64_t build_rows, kg_tune_config *out);
+int kg_tune_set(kg_ctx *ctx, int64_t build_rows, const kg_tune_config *cfg);
+int kg_autotune_joinagg(kg_ctx *ctx, int64_t build_rows, int64_t probe_rows, kg_tune_config *best,
+                        double *best_ms, int32_t *candidates);
+/* The context's tuned shapes to and from a text file (one line a size class:
+ * "class load_pct scatter_rows merge_lanes"), so a process tunes once per
+ * device and later ones load the result. */
+int kg_tune_save(kg_ctx *ctx, const char *path);
+int kg_tune_load(kg_ctx *ctx, const char *path, int32_t *classes);
 
 /* ---- measured ceilings of the random-access phases ----------------------
  * No counterpart in the reference: calibration kernels that the probe

@@ -310,6 +310,9 @@ __global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, S
   const int64_t hi = lo + chunk < nrows ? lo + chunk : nrows;
   WaveChunk wc;
   int64_t next_evict = lo + kEvictRows;
+  const int kt = P.key.type;
+  const bool k64 = P.has_key && (kt == KG_T_INT64 || kt == KG_T_UINT64 || kt == KG_T_TIMESTAMP ||
+                                 kt == KG_T_DECIMAL64);
   for (int64_t base = lo; base < hi; base += (int64_t)R * blockDim.x) {
     if (base >= next_evict) {
       spill_lds_records(P, S, s_keys, s_rec, SLOTS, false);
@@ -320,16 +323,38 @@ __global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, S
     uint64_t key[R];
     int64_t pos[R];
     ColSrc<R> src;
+    // one-word integer keys: every row's key and validity loads are issued
+    // before the first probe (clamped row index, the validity byte kept raw)
+    uint32_t kvb[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      key[r] = 0;
+      kvb[r] = 0;
+    }
+    if (k64) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t i = base + (int64_t)r * blockDim.x + threadIdx.x, ic = i < hi ? i : hi - 1;
+        key[r] = dev::load_w64(P.key, ic);
+        kvb[r] = P.key.validity ? P.key.validity[(ic + P.key.offset) >> 3] : 0xFFu;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int64_t i = base + (int64_t)r * blockDim.x + threadIdx.x;
       src.idx[r] = i;
       ls[r] = -1;
       miss[r] = false;
-      key[r] = 0;
       pos[r] = 0;
       if (i < hi) {
-        const int cls = row_key(P, i, key[r]);
+        int cls;
+        if (k64) {
+          const bool ok = (kvb[r] >> ((i + P.key.offset) & 7)) & 1;
+          cls = !ok ? 2 : key[r] == kEmpty ? 1 : 0;
+          if (!ok) key[r] = 0;
+        } else {
+          cls = row_key(P, i, key[r]);
+        }
         if (cls) {
           ls[r] = SLOTS + cls - 1;
         } else {
@@ -380,14 +405,17 @@ __global__ void k_spill_hist(const uint64_t *rows, int w, uint64_t n, Table T, i
   const int64_t hi = lo + per_block < (int64_t)n ? lo + per_block : (int64_t)n;
   // four rows a thread: their loads are in flight together
   for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
-    uint64_t key[4];
+    uint64_t key[4], fl[4];
     bool live[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int64_t i = i0 + r * (int64_t)blockDim.x;
-      live[r] = i < hi && !(dead && (rows[i * w + 1] & kDeadRow));
-      key[r] = i < hi ? rows[i * w] : 0;
+    for (int r = 0; r < 4; r++) {  // clamped row index: no branch between the loads
+      const int64_t i = i0 + r * (int64_t)blockDim.x, ic = i < hi ? i : hi - 1;
+      key[r] = rows[ic * w];
+      fl[r] = dead ? rows[ic * w + 1] : 0;
     }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      live[r] = i0 + r * (int64_t)blockDim.x < hi && !(fl[r] & kDeadRow);
 #pragma unroll
     for (int r = 0; r < 4; r++)
       if (live[r]) atomicAdd(&s_h[spill_part(T, key[r], log2p)], 1u);
@@ -918,8 +946,9 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
         case R_COUNT: static_cast<int64_t *>(a.values)[o] = (int64_t)divisor; break;
         case R_SUM_INT: {
           ok = valid_rows > 0;
-          // exact 128-bit sum; the result wraps (non-ANSI) or must fit (ANSI)
-          bool fits = (int64_t)w[1] == ((int64_t)w[0] >> 63);
+          // ANSI: the exact 128-bit sum must fit; otherwise the one wrapping
+          // word is the result
+          bool fits = !O.ansi || (int64_t)w[1] == ((int64_t)w[0] >> 63);
           if (ok && (overflowed || (O.ansi && !fits))) atomicOr(err, 1u);
           static_cast<int64_t *>(a.values)[o] = (int64_t)w[0];
           break;
@@ -1621,6 +1650,14 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
     int kind, col, checked, avg;
   };
   std::vector<std::pair<Key, int>> shared;
+  // integral columns that an AVG sums exactly: a SUM of the same column
+  // shares that 128-bit sum rather than keeping a word of its own
+  std::vector<bool> avg_sums(in_types.size(), false);
+  if (!merge)
+    for (int i = 0; i < naggs; i++)
+      if (aggs[i].func == KG_AGG_AVG && aggs[i].input >= 0 &&
+          aggs[i].input < (int)in_types.size() && is_integral(in_types[aggs[i].input].type))
+        avg_sums[aggs[i].input] = true;
   auto op_words = [&](int kind, int col, bool chk, bool avg, int nwords, uint64_t iv,
                       int &st_word) {
     if (!chk)
@@ -1697,7 +1734,11 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
           o.kind = avg ? R_AVG_INT : R_SUM_INT;
           o.out_type = avg ? KG_T_FLOAT64 : KG_T_INT64;
           o.st_type = avg ? KG_T_FLOAT64 : KG_T_INT64;
-          o.word = op_words(OP_ADD128, c, chk, false, 2, 0, st);
+          // a SUM that wraps (not ANSI) keeps one word; AVG and the ANSI SUM
+          // keep the exact 128-bit sum
+          if (!avg && !chk && (merge || !avg_sums[c0]))
+            o.word = op_words(OP_ADD64, c, false, false, 1, 0, st);
+          else o.word = op_words(OP_ADD128, c, chk, false, 2, 0, st);
           o.nulls_word = nulls_of(c);
           if (chk) {
             o.st_word = st;

@@ -34,6 +34,9 @@ enum OpKind : uint8_t {
   OP_ADDV,        // word += int64 value (counts)
   OP_COUNT_TRUE,  // word += value != 0 (a `nonempty` flag)
   OP_STICKY,      // st = 2 when the sum state is NULL and col2 is nonzero
+  // SUM of an integral column without overflow checks (not ANSI): only the
+  // result's low 64 bits are kept, so one wrapping word is the whole state
+  OP_ADD64,  // word += v (mod 2^64)
 };
 
 struct AggOp {
@@ -207,7 +210,8 @@ __device__ __forceinline__ void apply_row(const AggProg &P, uint64_t *r, int64_t
     if (!ok || (op.checked && P.skip_checked)) continue;
     uint64_t *w = r + op.word;
     switch (op.kind) {
-      case OP_ADDV: add_u64_nr<LDS>(w, lo); break;
+      case OP_ADDV:
+      case OP_ADD64: add_u64_nr<LDS>(w, lo); break;
       case OP_COUNT_TRUE:
         if (lo) add_u64_nr<LDS>(w, 1);
         break;
@@ -322,7 +326,8 @@ __device__ __forceinline__ void apply_wave(const AggProg &P, uint64_t *r, int64_
         if (leader && n) add_u64_nr<LDS>(w, n);
         break;
       }
-      case OP_ADDV: {
+      case OP_ADDV:
+      case OP_ADD64: {
         uint64_t s = wave_sum_u64(ok ? lo : 0);
         if (leader && s) add_u64_nr<LDS>(w, s);
         break;
@@ -423,6 +428,7 @@ __device__ __forceinline__ void apply_row_mult(const AggProg &P, uint64_t *r, in
         add128<false>(w, plo, phi);
         break;
       }
+      case OP_ADD64: add_u64_nr<false>(w, lo * m); break;
       case OP_ADDF64: add_f64<false>(w, f * (double)m); break;
       case OP_MIN_I64: min_i64<false>(w, (int64_t)lo); break;
       case OP_MAX_I64: max_i64<false>(w, (int64_t)lo); break;
@@ -443,6 +449,7 @@ __device__ __forceinline__ void merge_record(const AggProg &P, uint64_t *g, cons
       case OP_COUNT_STAR:
       case OP_NULLS:
       case OP_ADDV:
+      case OP_ADD64:
       case OP_COUNT_TRUE:
         if (s[0]) add_u64_nr<false>(d, s[0]);
         break;

@@ -180,6 +180,7 @@ __device__ __forceinline__ void tile_apply(const AggProg &P, uint64_t *srec, int
     if (op.checked && P.skip_checked) continue;
     switch (op.kind) {
       case OP_ADDV:
+      case OP_ADD64:
 #pragma unroll
         for (int r = 0; r < R; r++)
           if (ok[r] && lo[r]) add_u64_nr<LDS>(srec + ls[r] * WL + op.word, lo[r]);
@@ -294,6 +295,11 @@ __device__ __forceinline__ void tile_apply_mult(const AggProg &P, uint64_t *srec
         for (int r = 0; r < R; r++)
           if (act[r] && !ok[r]) add_u64_nr<LDS>(base + ls[r] * WL, m[r]);
         break;
+      case OP_ADD64:
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (ok[r]) add_u64_nr<LDS>(base + ls[r] * WL, lo[r] * m[r]);
+        break;
       case OP_ADD128:
 #pragma unroll
         for (int r = 0; r < R; r++)
@@ -344,6 +350,7 @@ __device__ __forceinline__ void merge_into_lds(const AggProg &P, uint64_t *d, co
       case OP_COUNT_STAR:
       case OP_NULLS:
       case OP_ADDV:
+      case OP_ADD64:
       case OP_COUNT_TRUE:
         if (a[0]) add_u64_nr<true>(b, a[0]);
         break;
@@ -385,6 +392,7 @@ __device__ __forceinline__ void merge_plain(const AggProg &P, uint64_t *d, const
       case OP_COUNT_STAR:
       case OP_NULLS:
       case OP_ADDV:
+      case OP_ADD64:
       case OP_COUNT_TRUE: b[0] += a[0]; break;
       case OP_STICKY:
         if (s[op.st_word] == 2) d[op.st_word] = 2;

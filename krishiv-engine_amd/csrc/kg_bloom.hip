@@ -67,8 +67,34 @@ __device__ __forceinline__ bool sbbf_check(const uint32_t *words, uint64_t nbloc
   return miss == 0;
 }
 
+// The first four words of the block of h: the bits they miss (0: the row
+// still passes).
+__device__ __forceinline__ uint32_t sbbf_miss_lo(uint4 x, uint32_t k) {
+  uint32_t miss = 0;
+  miss |= (1u << ((k * dev::sbbf_salt(0)) >> 27)) & ~x.x;
+  miss |= (1u << ((k * dev::sbbf_salt(1)) >> 27)) & ~x.y;
+  miss |= (1u << ((k * dev::sbbf_salt(2)) >> 27)) & ~x.z;
+  miss |= (1u << ((k * dev::sbbf_salt(3)) >> 27)) & ~x.w;
+  return miss;
+}
+__device__ __forceinline__ uint32_t sbbf_miss_hi(uint4 y, uint32_t k) {
+  uint32_t miss = 0;
+  miss |= (1u << ((k * dev::sbbf_salt(4)) >> 27)) & ~y.x;
+  miss |= (1u << ((k * dev::sbbf_salt(5)) >> 27)) & ~y.y;
+  miss |= (1u << ((k * dev::sbbf_salt(6)) >> 27)) & ~y.z;
+  miss |= (1u << ((k * dev::sbbf_salt(7)) >> 27)) & ~y.w;
+  return miss;
+}
+
 // Pass 1: tile t covers rows [t*kProbeTile, (t+1)*kProbeTile); bitmap word
-// (row / 64) holds the answers of 64 consecutive rows.
+// (row / 64) holds the answers of 64 consecutive rows.  A lane takes kPU
+// rows at a time: their key loads, then their blocks' first halves (16
+// bytes), are in flight together; the second half is read only by the rows
+// the first half did not already reject (most rows of a selective filter).
+// W64: keys that hash as one raw 64-bit word (int64, uint64, temporal).
+constexpr int kPU = 4;
+
+template <bool W64>
 __global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_bits(DCol key, int64_t n,
                                                                   const uint32_t *words,
                                                                   uint64_t nblocks,
@@ -80,15 +106,58 @@ __global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_bits(DCol key, int6
   int64_t base = (int64_t)blockIdx.x * kProbeTile;
   uint32_t cnt = 0;
   const int lane = threadIdx.x & 63;
-  for (int k = 0; k < kProbeTile / kProbeBlock; k++) {
-    int64_t i = base + (int64_t)k * kProbeBlock + threadIdx.x;
-    bool pass = false;
-    if (i < n && dev::valid(key, i)) pass = sbbf_check(words, nblocks, dev::value_hash(key, i));
-    uint64_t m = __ballot(pass);
-    int64_t w0 = base + (int64_t)k * kProbeBlock + (threadIdx.x & ~63);
-    if (lane == 0 && w0 < n) {
-      bitmap[w0 >> 6] = m;
-      cnt += (uint32_t)__popcll(m);
+  for (int k0 = 0; k0 < kProbeTile / kProbeBlock; k0 += kPU) {
+    uint64_t h[kPU];
+    bool live[kPU];
+    if (W64) {
+      uint64_t kv[kPU];
+      uint32_t kb[kPU];
+#pragma unroll
+      for (int u = 0; u < kPU; u++) {
+        const int64_t i = base + (int64_t)(k0 + u) * kProbeBlock + threadIdx.x;
+        const int64_t ic = i < n ? i : n - 1;
+        kv[u] = dev::load_w64(key, ic);
+        kb[u] = key.validity ? key.validity[(ic + key.offset) >> 3] : 0xFFu;
+      }
+#pragma unroll
+      for (int u = 0; u < kPU; u++) {
+        const int64_t i = base + (int64_t)(k0 + u) * kProbeBlock + threadIdx.x;
+        const int64_t ic = i < n ? i : n - 1;
+        live[u] = i < n && ((kb[u] >> ((ic + key.offset) & 7)) & 1);
+        h[u] = dev::xxh64_fixed(kv[u], 0, 8);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kPU; u++) {
+        const int64_t i = base + (int64_t)(k0 + u) * kProbeBlock + threadIdx.x;
+        live[u] = i < n && dev::valid(key, i);
+        h[u] = live[u] ? dev::value_hash(key, i) : 0;
+      }
+    }
+    // every lane reads its first half (a block in range for any h), so the
+    // reads are not behind a per-lane branch; rows not live are masked after
+    uint4 x[kPU];
+#pragma unroll
+    for (int u = 0; u < kPU; u++)
+      x[u] = reinterpret_cast<const uint4 *>(words + sbbf_block(h[u], nblocks) * 8)[0];
+    uint32_t miss[kPU];
+#pragma unroll
+    for (int u = 0; u < kPU; u++) miss[u] = live[u] ? sbbf_miss_lo(x[u], (uint32_t)h[u]) : 1u;
+#pragma unroll
+    for (int u = 0; u < kPU; u++) {
+      if (!miss[u]) {
+        const uint4 *b = reinterpret_cast<const uint4 *>(words + sbbf_block(h[u], nblocks) * 8);
+        miss[u] = sbbf_miss_hi(b[1], (uint32_t)h[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kPU; u++) {
+      const uint64_t m = __ballot(miss[u] == 0);
+      const int64_t w0 = base + (int64_t)(k0 + u) * kProbeBlock + (threadIdx.x & ~63);
+      if (lane == 0 && w0 < n) {
+        bitmap[w0 >> 6] = m;
+        cnt += (uint32_t)__popcll(m);
+      }
     }
   }
   if (lane == 0 && cnt) atomicAdd(&s_cnt, cnt);
@@ -386,9 +455,15 @@ int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *ou
       KG_HIP(hipGetLastError());
     } else {
       ProfScope prof("bloom_probe", b->s);
-      k_bloom_probe_bits<<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
-          k, nrows, b->words.get<uint32_t>(), b->nblocks, b->bitmap.get<uint64_t>(),
-          b->tiles.get<uint32_t>());
+      const bool w64 = k.type == KG_T_INT64 || k.type == KG_T_UINT64 || k.type == KG_T_TIMESTAMP;
+      if (w64)
+        k_bloom_probe_bits<true><<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
+            k, nrows, b->words.get<uint32_t>(), b->nblocks, b->bitmap.get<uint64_t>(),
+            b->tiles.get<uint32_t>());
+      else
+        k_bloom_probe_bits<false><<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
+            k, nrows, b->words.get<uint32_t>(), b->nblocks, b->bitmap.get<uint64_t>(),
+            b->tiles.get<uint32_t>());
       KG_HIP(hipGetLastError());
     }
     size_t tb = 0;

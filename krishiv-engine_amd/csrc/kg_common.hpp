@@ -76,6 +76,15 @@ class Fence {
  private:
   hipEvent_t e_ = nullptr;
 };
+
+// Shape parameters of the hash join and its fused aggregate, per size class
+// of the build side (kg_tune_*): the table's load factor, the staged
+// partition scatter's rows per lane, and the partition merge's workgroup.
+struct TuneCfg {
+  int32_t load_pct = 50;     // table capacity >= rows * 100 / load_pct (a power of two)
+  int32_t scatter_rows = 0;  // 4, 8 or 12; 0: by the row width
+  int32_t merge_lanes = 1024;  // 1024 (one workgroup a CU, the whole LDS) or 512 (two)
+};
 }  // namespace kg
 
 struct kg_ctx {
@@ -101,11 +110,17 @@ struct kg_ctx {
   std::vector<Pending> pending;
   // events after the last enqueued reads of caller-owned device columns
   std::vector<std::shared_ptr<kg::Fence>> use_fences;
+  // tuned shapes by build-side size class (ceil log2 of the rows)
+  std::mutex tune_mu;
+  std::map<int, kg::TuneCfg> tune;
 };
 
 namespace kg {
 
 void *dev_alloc(kg_ctx *ctx, uint64_t bytes);
+// The tuned shape for a build side of `rows` (defaults where none is stored).
+TuneCfg tune_for(kg_ctx *ctx, int64_t rows);
+void tune_store(kg_ctx *ctx, int64_t rows, const TuneCfg &c);
 // Frees a block.  With a stream, the block is reused only after the work
 // enqueued on that stream so far, and every read of caller columns enqueued
 // so far (use fences), has completed; without one, after a device

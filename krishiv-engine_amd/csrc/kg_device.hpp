@@ -21,6 +21,14 @@ __device__ __forceinline__ bool valid(const DCol &c, int64_t i) {
   return (c.validity[j >> 3] >> (j & 7)) & 1;
 }
 
+// The 64-bit word of an 8-byte column at row i, loaded with no condition.
+// Streaming kernels clamp i into their row range and mask the result after
+// the loads of all their rows are issued: a load behind a per-lane branch
+// (a validity test, a range test) waits for the loads before it.
+__device__ __forceinline__ uint64_t load_w64(const DCol &c, int64_t i) {
+  return static_cast<const uint64_t *>(c.values)[i + c.offset];
+}
+
 // Integer-like value widened to int64 (bool, signed/unsigned ints, dates,
 // timestamps, decimal64).  UINT64 keeps its bit pattern.
 __device__ __forceinline__ int64_t load_i64(const DCol &c, int64_t i) {

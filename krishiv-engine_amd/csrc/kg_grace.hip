@@ -47,6 +47,15 @@ struct GPiece {
 
 constexpr uint64_t kGEmpty = 0x8000000000000000ULL;
 
+// Validity of row i in two steps: the byte (0xFF without a bitmap), loaded
+// with the row's other words, then its bit once every load is issued.
+__device__ __forceinline__ uint32_t vbyte(const DCol &c, int64_t i) {
+  return c.validity ? c.validity[(i + c.offset) >> 3] : 0xFFu;
+}
+__device__ __forceinline__ bool vbit(const DCol &c, int64_t i, uint32_t b) {
+  return (b >> ((i + c.offset) & 7)) & 1;
+}
+
 __device__ __forceinline__ uint64_t gslot_start(uint64_t key, int log2cap) {
   return dev::table_hash(key, 0x51ED27A1D3C4B9E5ULL) >> (64 - log2cap);
 }
@@ -107,39 +116,55 @@ __global__ void __launch_bounds__(kGBBlock) k_grace_build(const GPiece *pcs, int
     const int64_t n = pcs[q].n;
     const int64_t ntiles = (n + kGBTile - 1) / kGBTile;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      // the tile's loads are all issued before the first LDS insert (clamped
+      // row index; pieces are int64 key and payload columns, see slice)
+      constexpr int kR = kGBTile / kGBBlock;
+      uint64_t kr[kR];
+      int64_t pw[kR];
+      uint32_t kb[kR], pb[kR];
 #pragma unroll
-      for (int r = 0; r < kGBTile / kGBBlock; r++) {
-        const int64_t i = t * kGBTile + (int64_t)r * kGBBlock + threadIdx.x;
-        if (i >= n || !dev::valid(key, i)) continue;  // a NULL key joins nothing
-        const uint64_t k = (uint64_t)dev::load_i64(key, i);
-        const int64_t pval = dev::load_i64(pay, i);
-        const bool pv = dev::valid(pay, i) && pval >= 0;  // -1: a NULL payload
-        const unsigned long long c = 1ull | ((unsigned long long)pv << 32);
-        const unsigned long long sb = pv ? (unsigned long long)pval : 0ull;
-        int l = -1;
-        if (k != kGEmpty) {
-          int h = (int)((dev::table_hash(k, 0x9E3779B97F4A7C15ULL) >> 32) & (kGBSlots - 1));
-          for (int pr = 0; pr < kGBProbe; pr++) {
-            const unsigned long long cur = s_key[h];
-            if (cur == k) {
-              l = h;
-              break;
-            }
-            if (cur == kGEmpty) {
-              const unsigned long long prev = atomicCAS(&s_key[h], kGEmpty, k);
-              if (prev == kGEmpty || prev == k) {
+      for (int r = 0; r < kR; r++) {
+        const int64_t i = t * kGBTile + (int64_t)r * kGBBlock + threadIdx.x, ic = i < n ? i : n - 1;
+        kr[r] = dev::load_w64(key, ic);
+        kb[r] = vbyte(key, ic);
+        pw[r] = (int64_t)dev::load_w64(pay, ic);
+        pb[r] = vbyte(pay, ic);
+      }
+#pragma unroll
+      for (int r = 0; r < kR; r++) {
+        const int64_t i = t * kGBTile + (int64_t)r * kGBBlock + threadIdx.x, ic = i < n ? i : n - 1;
+        // a NULL key joins nothing
+        if (i < n && vbit(key, ic, kb[r])) {
+          const uint64_t k = kr[r];
+          const int64_t pval = pw[r];
+          const bool pv = vbit(pay, ic, pb[r]) && pval >= 0;  // -1: a NULL payload
+          const unsigned long long c = 1ull | ((unsigned long long)pv << 32);
+          const unsigned long long sb = pv ? (unsigned long long)pval : 0ull;
+          int l = -1;
+          if (k != kGEmpty) {
+            int h = (int)((dev::table_hash(k, 0x9E3779B97F4A7C15ULL) >> 32) & (kGBSlots - 1));
+            for (int pr = 0; pr < kGBProbe; pr++) {
+              const unsigned long long cur = s_key[h];
+              if (cur == k) {
                 l = h;
                 break;
               }
+              if (cur == kGEmpty) {
+                const unsigned long long prev = atomicCAS(&s_key[h], kGEmpty, k);
+                if (prev == kGEmpty || prev == k) {
+                  l = h;
+                  break;
+                }
+              }
+              h = (h + 1) & (kGBSlots - 1);
             }
-            h = (h + 1) & (kGBSlots - 1);
           }
-        }
-        if (l >= 0) {
-          atomicAdd(&s_cnt[l], c);
-          if (sb) atomicAdd(&s_sb[l], sb);
-        } else {
-          grace_add(slots, mask, log2cap, cap, k, c, sb);
+          if (l >= 0) {
+            atomicAdd(&s_cnt[l], c);
+            if (sb) atomicAdd(&s_sb[l], sb);
+          } else {
+            grace_add(slots, mask, log2cap, cap, k, c, sb);
+          }
         }
       }
       __syncthreads();
@@ -209,13 +234,23 @@ __global__ void __launch_bounds__(256) k_grace_probe(const GPiece *pcs, int npcs
       uint64_t k[kGPU], at[kGPU], sk[kGPU], sc[kGPU], ss[kGPU];
       int64_t pv[kGPU];
       bool act[kGPU];
+      uint32_t kb[kGPU], pb[kGPU];
+      // int64 piece columns (see slice): loads on a clamped row index, the
+      // validity bits tested once all are issued
 #pragma unroll
       for (int u = 0; u < kGPU; u++) {
-        const int64_t i = i0 + (int64_t)u * blockDim.x;
-        act[u] = i < n && dev::valid(key, i);
-        k[u] = act[u] ? (uint64_t)dev::load_i64(key, i) : 0;
-        // -1: a NULL payload
-        pv[u] = act[u] && dev::valid(pay, i) ? dev::load_i64(pay, i) : -1;
+        const int64_t i = i0 + (int64_t)u * blockDim.x, ic = i < n ? i : n - 1;
+        k[u] = dev::load_w64(key, ic);
+        kb[u] = vbyte(key, ic);
+        pv[u] = (int64_t)dev::load_w64(pay, ic);
+        pb[u] = vbyte(pay, ic);
+      }
+#pragma unroll
+      for (int u = 0; u < kGPU; u++) {
+        const int64_t i = i0 + (int64_t)u * blockDim.x, ic = i < n ? i : n - 1;
+        act[u] = i < n && vbit(key, ic, kb[u]);
+        if (!act[u]) k[u] = 0;
+        if (!(act[u] && vbit(pay, ic, pb[u]))) pv[u] = -1;  // -1: a NULL payload
       }
 #pragma unroll
       for (int u = 0; u < kGPU; u++) {
@@ -264,6 +299,12 @@ __global__ void __launch_bounds__(256) k_grace_probe(const GPiece *pcs, int npcs
 // non-NULL payloads of rows with a key, one atomic per block.
 constexpr int kGPBlock = 256;
 
+// W64: the key is one raw 64-bit word hashed as such (int64, uint64,
+// temporal) and the payload one raw 64-bit integer word: eight rows a lane
+// with every load issued before any is tested (clamped row index, validity
+// bytes kept raw until then).
+
+template <bool W64>
 __global__ void __launch_bounds__(kGPBlock) k_gp_hist(DCol key, DCol pay, int64_t n, uint32_t parts,
                                                       int64_t per_block, uint32_t *hist,
                                                       long long *mm) {
@@ -276,13 +317,41 @@ __global__ void __launch_bounds__(kGPBlock) k_gp_hist(DCol key, DCol pay, int64_
   const int64_t hi = lo + per_block < n ? lo + per_block : n;
   long long vlo = LLONG_MAX, vhi = LLONG_MIN;
   uint32_t *h = s_h + wave * parts;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kGPBlock) {
-    if (!dev::valid(key, i)) continue;
-    atomicAdd(&h[dev::partition_of(dev::value_hash(key, i), parts)], 1u);
-    if (dev::valid(pay, i)) {
-      const long long v = (long long)dev::load_i64(pay, i);
-      vlo = v < vlo ? v : vlo;
-      vhi = v > vhi ? v : vhi;
+  if (W64) {
+    constexpr int U = 8;
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)kGPBlock * U) {
+      uint64_t kv[U], pv[U];
+      uint32_t kb[U], pb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t i = i0 + (int64_t)u * kGPBlock, ic = i < hi ? i : hi - 1;
+        kv[u] = dev::load_w64(key, ic);
+        kb[u] = vbyte(key, ic);
+        pv[u] = dev::load_w64(pay, ic);
+        pb[u] = vbyte(pay, ic);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t i = i0 + (int64_t)u * kGPBlock, ic = i < hi ? i : hi - 1;
+        if (i < hi && vbit(key, ic, kb[u])) {
+          atomicAdd(&h[dev::partition_of(dev::xxh64_fixed(kv[u], 0, 8), parts)], 1u);
+          if (vbit(pay, ic, pb[u])) {
+            const long long v = (long long)pv[u];
+            vlo = v < vlo ? v : vlo;
+            vhi = v > vhi ? v : vhi;
+          }
+        }
+      }
+    }
+  } else {
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kGPBlock) {
+      if (!dev::valid(key, i)) continue;
+      atomicAdd(&h[dev::partition_of(dev::value_hash(key, i), parts)], 1u);
+      if (dev::valid(pay, i)) {
+        const long long v = (long long)dev::load_i64(pay, i);
+        vlo = v < vlo ? v : vlo;
+        vhi = v > vhi ? v : vhi;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -358,7 +427,7 @@ __host__ __device__ constexpr size_t gs_lds_bytes(int R, uint32_t parts) {
 // Partition pass 2: tiles of kGSBlock * R rows sorted by partition in LDS,
 // then each partition's run written by consecutive lanes: out_key/out_pay
 // rows [key, payload or -1].
-template <int R>
+template <int R, bool W64>
 __global__ void __launch_bounds__(kGSBlock) k_gp_scatter(DCol key, DCol pay, int64_t n,
                                                          uint32_t parts, int64_t per_block,
                                                          const uint64_t *base, int64_t *out_key,
@@ -381,13 +450,33 @@ __global__ void __launch_bounds__(kGSBlock) k_gp_scatter(DCol key, DCol pay, int
     __syncthreads();
     uint64_t kk[R], vv[R];
     uint32_t pp[R], lp[R];
+    if (W64) {
+      uint32_t kb[R], pb[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int64_t i = t0 + (int64_t)r * kGSBlock + threadIdx.x;
-      const bool live = i < hi && dev::valid(key, i);
-      kk[r] = live ? (uint64_t)dev::load_i64(key, i) : 0;
-      vv[r] = live ? (dev::valid(pay, i) ? (uint64_t)dev::load_i64(pay, i) : ~0ull) : 0;
-      pp[r] = live ? dev::partition_of(dev::value_hash(key, i), parts) : 0xffffffffu;
+      for (int r = 0; r < R; r++) {
+        const int64_t i = t0 + (int64_t)r * kGSBlock + threadIdx.x, ic = i < hi ? i : hi - 1;
+        kk[r] = dev::load_w64(key, ic);
+        kb[r] = vbyte(key, ic);
+        vv[r] = dev::load_w64(pay, ic);
+        pb[r] = vbyte(pay, ic);
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t i = t0 + (int64_t)r * kGSBlock + threadIdx.x, ic = i < hi ? i : hi - 1;
+        const bool live = i < hi && vbit(key, ic, kb[r]);
+        pp[r] = live ? dev::partition_of(dev::xxh64_fixed(kk[r], 0, 8), parts) : 0xffffffffu;
+        vv[r] = live ? (vbit(pay, ic, pb[r]) ? vv[r] : ~0ull) : 0;
+        kk[r] = live ? kk[r] : 0;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t i = t0 + (int64_t)r * kGSBlock + threadIdx.x;
+        const bool live = i < hi && dev::valid(key, i);
+        kk[r] = live ? (uint64_t)dev::load_i64(key, i) : 0;
+        vv[r] = live ? (dev::valid(pay, i) ? (uint64_t)dev::load_i64(pay, i) : ~0ull) : 0;
+        pp[r] = live ? dev::partition_of(dev::value_hash(key, i), parts) : 0xffffffffu;
+      }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) lp[r] = pp[r] != 0xffffffffu ? atomicAdd(&cnt[pp[r]], 1u) : 0u;
@@ -524,6 +613,11 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
     {
       Staged st(g->ctx, g->s);
       const DCol dk = st.add(k, m), dv = st.add(v, m);
+      // raw 64-bit key words hashed as such, raw 64-bit integer payload words
+      const bool w64 = (dk.type == KG_T_INT64 || dk.type == KG_T_UINT64 ||
+                        dk.type == KG_T_TIMESTAMP) &&
+                       (dv.type == KG_T_INT64 || dv.type == KG_T_UINT64 ||
+                        dv.type == KG_T_TIMESTAMP || dv.type == KG_T_DECIMAL64);
       const uint32_t parts = g->parts;
       const int64_t cus = g->ctx->num_cus > 0 ? g->ctx->num_cus : 256;
       const int R = parts <= 1024 ? 12 : 8;
@@ -534,10 +628,15 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
       DevBuf hist(g->ctx, hn * 4 + 4), base(g->ctx, hn * 8 + 8), offs(g->ctx, (parts + 1) * 8);
       {
         ProfScope prof("hash_partition", g->s);
-        KG_HIP(hipFuncSetAttribute((const void *)k_gp_hist,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(parts * 16)));
-        k_gp_hist<<<(unsigned)nb, kGPBlock, parts * 16, g->s>>>(
-            dk, dv, m, parts, per_block, hist.get<uint32_t>(), g->mm.get<long long>() + 2 * side);
+        const void *hf = w64 ? (const void *)k_gp_hist<true> : (const void *)k_gp_hist<false>;
+        KG_HIP(hipFuncSetAttribute(hf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(parts * 16)));
+        if (w64)
+          k_gp_hist<true><<<(unsigned)nb, kGPBlock, parts * 16, g->s>>>(
+              dk, dv, m, parts, per_block, hist.get<uint32_t>(), g->mm.get<long long>() + 2 * side);
+        else
+          k_gp_hist<false><<<(unsigned)nb, kGPBlock, parts * 16, g->s>>>(
+              dk, dv, m, parts, per_block, hist.get<uint32_t>(), g->mm.get<long long>() + 2 * side);
         KG_HIP(hipGetLastError());
       }
       size_t tb = 0;
@@ -551,16 +650,19 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
       {
         ProfScope prof("scatter", g->s);
         const size_t lds = gs_lds_bytes(R, parts);
-        const void *fn = R == 12 ? (const void *)k_gp_scatter<12> : (const void *)k_gp_scatter<8>;
-        KG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        if (R == 12)
-          k_gp_scatter<12><<<(unsigned)nb, kGSBlock, lds, g->s>>>(
-              dk, dv, m, parts, per_block, base.get<uint64_t>(), piece.key.get<int64_t>(),
-              piece.pay.get<int64_t>());
-        else
-          k_gp_scatter<8><<<(unsigned)nb, kGSBlock, lds, g->s>>>(
-              dk, dv, m, parts, per_block, base.get<uint64_t>(), piece.key.get<int64_t>(),
-              piece.pay.get<int64_t>());
+#define KG_GPS(RR, WW)                                                                        \
+  do {                                                                                       \
+    KG_HIP(hipFuncSetAttribute((const void *)k_gp_scatter<RR, WW>,                           \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
+    k_gp_scatter<RR, WW><<<(unsigned)nb, kGSBlock, lds, g->s>>>(                             \
+        dk, dv, m, parts, per_block, base.get<uint64_t>(), piece.key.get<int64_t>(),         \
+        piece.pay.get<int64_t>());                                                           \
+  } while (0)
+        if (R == 12 && w64) KG_GPS(12, true);
+        else if (R == 12) KG_GPS(12, false);
+        else if (w64) KG_GPS(8, true);
+        else KG_GPS(8, false);
+#undef KG_GPS
         KG_HIP(hipGetLastError());
       }
       k_gp_offsets<<<(parts + 256) / 256, 256, 0, g->s>>>(base.get<uint64_t>(), hist.get<uint32_t>(),

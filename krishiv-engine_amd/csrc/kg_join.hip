@@ -140,29 +140,103 @@ __device__ __forceinline__ int64_t jfind(const JTable &T, int cls, uint64_t key)
   }
 }
 
-// Pass 1: matches and output rows of each probe row.
+// Pass 1: matches and output rows of each probe row.  kJCU rows a lane,
+// in stages whose loads are all in flight together: the selection indexes,
+// the keys (FAST: one 8-byte integer key column, validity bytes kept raw),
+// then the first slot (or direct-index entry) of each key; a probe chain
+// past the first slot is followed row by row (rare at the table's load).
+constexpr int kJCU = 4;
+
+template <bool FAST>
 __global__ void k_join_count(KeySpec K, int64_t n, const int32_t *sel, JTable T, int join_type,
                              uint32_t *head, uint32_t *nout) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t key;
-    int cls = pack_key(K, sel ? (int64_t)sel[i] : i, key);
-    int64_t s = jfind(T, cls, key);
-    uint32_t h = kNoRow, m = 0;
-    if (s >= 0) {
-      Slot sl = T.slots[s];
-      h = sl.head;
-      m = sl.count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * kJCU;
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x * kJCU + threadIdx.x; i0 < n; i0 += stride) {
+    int64_t row[kJCU];
+#pragma unroll
+    for (int r = 0; r < kJCU; r++) {
+      const int64_t i = i0 + (int64_t)r * blockDim.x, ic = i < n ? i : n - 1;
+      row[r] = sel ? (int64_t)sel[ic] : ic;
     }
-    uint32_t o;
-    switch (join_type) {
-      case KG_JOIN_INNER: o = m; break;
-      case KG_JOIN_LEFT_OUTER: o = m ? m : 1; break;
-      case KG_JOIN_LEFT_SEMI: o = m ? 1 : 0; break;
-      default: o = m ? 0 : 1; break;  // anti
+    uint64_t key[kJCU];
+    int cls[kJCU];
+    if (FAST) {
+      const DCol &d = K.c[0];
+      uint32_t kb[kJCU];
+#pragma unroll
+      for (int r = 0; r < kJCU; r++) {
+        key[r] = dev::load_w64(d, row[r]);
+        kb[r] = d.validity ? d.validity[(row[r] + d.offset) >> 3] : 0xFFu;
+      }
+#pragma unroll
+      for (int r = 0; r < kJCU; r++) {
+        const bool ok = (kb[r] >> ((row[r] + d.offset) & 7)) & 1;
+        cls[r] = !ok ? (K.null_safe ? 2 : 3) : key[r] == kEmpty ? 1 : 0;
+        if (!ok) key[r] = 0;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kJCU; r++) cls[r] = pack_key(K, row[r], key[r]);
     }
-    head[i] = h;
-    nout[i] = o;
+    // first slot of each key (its fixed slot for classes 1 and 2, slot 0 as
+    // a harmless stand-in where the row finds nothing)
+    int64_t s[kJCU];
+#pragma unroll
+    for (int r = 0; r < kJCU; r++) {
+      if (T.dslot) {
+        const uint64_t dd = key[r] - (uint64_t)T.dmin;
+        s[r] = cls[r] != 3 && dd < T.dspan ? (int64_t)T.dslot[dd] : -1;
+      } else {
+        s[r] = cls[r] == 3 ? -1
+               : cls[r] != 0 ? (int64_t)(T.cap + (cls[r] == 2 ? 1 : 0))
+                             : (int64_t)jslot_start(T, key[r]);
+      }
+    }
+    uint64_t sk[kJCU];
+    uint32_t sh[kJCU], sc[kJCU];
+#pragma unroll
+    for (int r = 0; r < kJCU; r++) {
+      const Slot t = T.slots[s[r] >= 0 ? s[r] : 0];
+      sk[r] = t.key;
+      sh[r] = t.head;
+      sc[r] = t.count;
+    }
+#pragma unroll
+    for (int r = 0; r < kJCU; r++) {
+      const int64_t i = i0 + (int64_t)r * blockDim.x;
+      uint32_t h = kNoRow, m = 0;
+      if (s[r] >= 0) {
+        if (!T.dslot && cls[r] == 0) {
+          // linear probing from the home slot
+          uint64_t at = (uint64_t)s[r];
+          while (sk[r] != key[r] && sk[r] != kEmpty) {
+            at = (at + 1) & T.mask;
+            const Slot t = T.slots[at];
+            sk[r] = t.key;
+            sh[r] = t.head;
+            sc[r] = t.count;
+          }
+          if (sk[r] == key[r]) {
+            h = sh[r];
+            m = sc[r];
+          }
+        } else {
+          h = sh[r];
+          m = sc[r];
+        }
+      }
+      uint32_t o;
+      switch (join_type) {
+        case KG_JOIN_INNER: o = m; break;
+        case KG_JOIN_LEFT_OUTER: o = m ? m : 1; break;
+        case KG_JOIN_LEFT_SEMI: o = m ? 1 : 0; break;
+        default: o = m ? 0 : 1; break;  // anti
+      }
+      if (i < n) {
+        head[i] = h;
+        nout[i] = o;
+      }
+    }
   }
 }
 
@@ -222,19 +296,28 @@ __device__ __forceinline__ void copy_elem(const DCol &in, int64_t j, void *out, 
   }
 }
 
+// A wave takes 64 consecutive output rows (blocks of whole waves, wave-
+// aligned strides), so each nullable column's validity bits of those rows
+// are two whole 32-bit words: a ballot, stored by lanes 0 and 32 (no atomic
+// per row).  Padded rows (build row -1) get a 0 bit: NULL.
 __global__ void k_join_gather(GatherCols G, int64_t nout, const int32_t *pidx,
                               const int32_t *bidx) {
-  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < nout;
-       o += (int64_t)gridDim.x * blockDim.x) {
-    int32_t pi = pidx[o];
-    int32_t bi = bidx ? bidx[o] : -1;
+  const int lane = threadIdx.x & 63;
+  for (int64_t b0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); b0 < nout;
+       b0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = b0 + lane;
+    const bool in = o < nout;
+    const int32_t pi = in ? pidx[o] : -1;
+    const int32_t bi = in && bidx ? bidx[o] : -1;
     for (int c = 0; c < G.n; c++) {
-      int32_t r = G.from_build[c] ? bi : pi;
+      const int32_t r = G.from_build[c] ? bi : pi;
       const DCol &s = G.src[c];
-      if (r < 0) continue;  // padded NULL (validity bit stays 0)
-      copy_elem(s, r, G.dst[c], o);
-      if (G.dst_valid[c] && dev::valid(s, r))
-        atomicOr(G.dst_valid[c] + (o >> 5), 1u << (o & 31));
+      if (r >= 0) copy_elem(s, r, G.dst[c], o);
+      if (G.dst_valid[c]) {
+        const uint64_t m = __ballot(r >= 0 && dev::valid(s, r));
+        if (lane == 0) G.dst_valid[c][b0 >> 5] = (uint32_t)m;
+        if (lane == 32 && b0 + 32 < nout) G.dst_valid[c][(b0 >> 5) + 1] = (uint32_t)(m >> 32);
+      }
     }
   }
 }
@@ -382,6 +465,57 @@ __device__ __forceinline__ int jp_key(const KeySpec &K, int64_t i, uint64_t &key
   return pack_key(K, i, key);
 }
 
+constexpr int kJPre = 4;
+
+// Values a lane keeps for each of its rows in the FAST staged scatter with R
+// rows a lane (the register budget): 1 at 12 rows, 2 at 8, kJPre at 4.
+__host__ __device__ constexpr int js_nv(int R) { return R >= 12 ? 1 : R >= 8 ? 2 : kJPre; }
+
+// The FAST rows (one 8-byte key column, up to kJPre one-word inputs) i0 +
+// r * stride, r < R, below hi: every key, value and validity load is issued
+// before any is tested (clamped row index, no per-lane branch), then
+// cls[r] as jp_key (3 also past hi) and, for cls 0, the values with their
+// validity bits (0 and unset otherwise).  NV: whether values are wanted.
+template <int R, int NV>
+__device__ __forceinline__ void jp_rows_fast(const KeySpec &K, const DCol *in, int nv, int64_t i0,
+                                             int64_t stride, int64_t hi, uint64_t (&keys)[R],
+                                             int (&cls)[R], uint64_t (&val)[R][NV > 0 ? NV : 1],
+                                             uint64_t (&vb)[R]) {
+  constexpr int NA = NV > 0 ? NV : 1;
+  const DCol &kc = K.c[0];
+  // validity bytes stay raw until every load is issued (a compare on a
+  // loaded value at once would wait for it)
+  uint32_t kbyte[R], vbyte[R][NA];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int64_t i = i0 + r * stride;
+    const int64_t ic = i < hi ? i : hi - 1;
+    keys[r] = dev::load_w64(kc, ic);
+    kbyte[r] = kc.validity ? kc.validity[(ic + kc.offset) >> 3] : 0xFFu;
+#pragma unroll
+    for (int c = 0; c < NA; c++) {
+      const bool use = NV > 0 && c < nv;  // uniform
+      val[r][c] = use ? dev::load_w64(in[c], ic) : 0;
+      vbyte[r][c] = use ? (in[c].validity ? in[c].validity[(ic + in[c].offset) >> 3] : 0xFFu) : 0u;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int64_t i = i0 + r * stride;
+    const int64_t ic = i < hi ? i : hi - 1;
+    const bool kok = (kbyte[r] >> ((ic + kc.offset) & 7)) & 1;
+    cls[r] = i >= hi ? 3 : !kok ? (K.null_safe ? 2 : 3) : keys[r] == kEmpty ? 1 : 0;
+    if (!kok) keys[r] = 0;
+    vb[r] = 0;
+#pragma unroll
+    for (int c = 0; c < NA; c++) {
+      const bool ok = NV > 0 && cls[r] == 0 && ((vbyte[r][c] >> ((ic + in[c].offset) & 7)) & 1);
+      val[r][c] = ok ? val[r][c] : 0;
+      vb[r] |= (uint64_t)ok << c;
+    }
+  }
+}
+
 template <bool FAST>
 __global__ void k_jp_hist(KeySpec K, int64_t n, JTable T, int log2p, int64_t per_block,
                           uint32_t *hist) {
@@ -391,16 +525,24 @@ __global__ void k_jp_hist(KeySpec K, int64_t n, JTable T, int log2p, int64_t per
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * per_block;
   const int64_t hi = lo + per_block < n ? lo + per_block : n;
-  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * (int64_t)blockDim.x) {
-    uint64_t key[4];
-    bool live[4];
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 8 * (int64_t)blockDim.x) {
+    uint64_t key[8];
+    bool live[8];
+    if (FAST) {
+      int cls[8];
+      uint64_t val[8][1], vb[8];
+      jp_rows_fast<8, 0>(K, nullptr, 0, i0, blockDim.x, hi, key, cls, val, vb);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int64_t i = i0 + r * (int64_t)blockDim.x;
-      live[r] = i < hi && jp_key<FAST>(K, i, key[r]) == 0;
+      for (int r = 0; r < 8; r++) live[r] = cls[r] == 0;
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int64_t i = i0 + r * (int64_t)blockDim.x;
+        live[r] = i < hi && jp_key<FAST>(K, i, key[r]) == 0;
+      }
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int r = 0; r < 8; r++)
       if (live[r]) atomicAdd(&s_h[jpart(T, key[r], log2p)], 1u);
   }
   __syncthreads();
@@ -413,7 +555,6 @@ __global__ void k_jp_hist(KeySpec K, int64_t n, JTable T, int log2p, int64_t per
 // are applied here with atomics.  A thread takes four rows; their keys and
 // (up to kJPre inputs of one word) values are loaded before the cursor
 // atomics, so the loads are in flight together.
-constexpr int kJPre = 4;
 
 __device__ __forceinline__ void jp_value(const DCol &d, int64_t i, bool &v, uint64_t &lo,
                                          uint64_t &hi) {
@@ -448,20 +589,24 @@ __global__ void k_jp_scatter(KeySpec K, int64_t n, JTable T, AggProg P, SpillFmt
     int cls4[4];
     uint64_t val[4][kJPre];
     uint64_t vb[4];
+    if (FAST) {  // FAST inputs are one-word, at most kJPre: pre holds
+      jp_rows_fast<4, kJPre>(K, P.in, F.ncols, i0, blockDim.x, hi, keys, cls4, val, vb);
+    } else {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int64_t i = i0 + r * (int64_t)blockDim.x;
-      cls4[r] = i < hi ? jp_key<FAST>(K, i, keys[r]) : 3;
-      vb[r] = 0;
+      for (int r = 0; r < 4; r++) {
+        const int64_t i = i0 + r * (int64_t)blockDim.x;
+        cls4[r] = i < hi ? jp_key<FAST>(K, i, keys[r]) : 3;
+        vb[r] = 0;
 #pragma unroll
-      for (int c = 0; c < kJPre; c++) {
-        val[r][c] = 0;
-        if (pre && c < F.ncols && cls4[r] == 0) {
-          bool v;
-          uint64_t l, h;
-          jp_value(P.in[c], i, v, l, h);
-          val[r][c] = l;
-          if (v) vb[r] |= 1ull << c;
+        for (int c = 0; c < kJPre; c++) {
+          val[r][c] = 0;
+          if (pre && c < F.ncols && cls4[r] == 0) {
+            bool v;
+            uint64_t l, h;
+            jp_value(P.in[c], i, v, l, h);
+            val[r][c] = l;
+            if (v) vb[r] |= 1ull << c;
+          }
         }
       }
     }
@@ -578,31 +723,28 @@ __global__ void __launch_bounds__(kJSBlock)
   for (int64_t t0 = lo; t0 < hi; t0 += TT) {
     for (uint32_t p = threadIdx.x; p < np; p += kJSBlock) cnt[p] = 0;
     __syncthreads();
+    constexpr int NV = js_nv(R);  // FAST launches have nv <= NV
     uint64_t keys[R];
     uint32_t pp[R], lp[R];
-    uint64_t val[R][FAST ? kJPre : 1];
+    uint64_t val[R][NV];
     uint64_t vb[R];
+    int cls[R];
+    if (FAST) {
+      jp_rows_fast<R, NV>(K, P.in, nv, t0 + threadIdx.x, kJSBlock, hi, keys, cls, val, vb);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t i = t0 + (int64_t)r * kJSBlock + threadIdx.x;
+        cls[r] = i < hi ? jp_key<FAST>(K, i, keys[r]) : 3;
+        vb[r] = 0;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int64_t i = t0 + (int64_t)r * kJSBlock + threadIdx.x;
-      const int cls = i < hi ? jp_key<FAST>(K, i, keys[r]) : 3;
-      pp[r] = cls == 0 ? jpart(T, keys[r], log2p) : 0xffffffffu;
-      vb[r] = 0;
-      if (FAST) {
-#pragma unroll
-        for (int c = 0; c < kJPre; c++) {
-          val[r][c] = 0;
-          if (c < nv && cls == 0) {
-            const DCol &d = P.in[c];
-            if (dev::valid(d, i)) {
-              val[r][c] = static_cast<const uint64_t *>(d.values)[i + d.offset];
-              vb[r] |= 1ull << c;
-            }
-          }
-        }
-      }
-      if (cls == 1 || cls == 2) {  // fixed slots (rare): atomics
-        const uint64_t s = T.cap + (cls == 2 ? 1 : 0);
+      pp[r] = cls[r] == 0 ? jpart(T, keys[r], log2p) : 0xffffffffu;
+      if (cls[r] == 1 || cls[r] == 2) {  // fixed slots (rare): atomics
+        const int64_t i = t0 + (int64_t)r * kJSBlock + threadIdx.x;
+        const uint64_t s = T.cap + (cls[r] == 2 ? 1 : 0);
         const uint32_t m = T.slots[s].count;
         if (m) apply_row_mult(P, rec + s * (uint64_t)P.W, i, m);
       }
@@ -619,7 +761,7 @@ __global__ void __launch_bounds__(kJSBlock)
       st[s] = keys[r];
       if (FAST) {
 #pragma unroll
-        for (int c = 0; c < kJPre; c++)
+        for (int c = 0; c < NV; c++)
           if (c < nv) st[(size_t)F.off[c] * TT + s] = val[r][c];
       } else {
         const int64_t i = t0 + (int64_t)r * kJSBlock + threadIdx.x;
@@ -705,11 +847,18 @@ __global__ void __launch_bounds__(kJMBlock) k_jp_merge(JTable T, AggProg P, Spil
       bool fb[kJTile];
       SpillSrc<kJTile> src;
       src.rows = rows;
+      // every row's key load is issued before the first probe (clamped index)
+      uint64_t kk[kJTile];
+#pragma unroll
+      for (int r = 0; r < kJTile; r++) {
+        const int64_t i = t0 + (int64_t)r * blockDim.x + threadIdx.x;
+        kk[r] = rows[(i < b ? i : b - 1) * F.rw];
+      }
 #pragma unroll
       for (int r = 0; r < kJTile; r++) {
         const int64_t i = t0 + (int64_t)r * blockDim.x + threadIdx.x;
         src.idx[r] = i;
-        const int2 hit = i < b ? window_probe(T, s_keys, s_cnt, Wn, whole, start, rows[i * F.rw])
+        const int2 hit = i < b ? window_probe(T, s_keys, s_cnt, Wn, whole, start, kk[r])
                                : make_int2(-1, 0);
         ls[r] = hit.x >= 0 ? hit.x : -1;
         m[r] = hit.x >= 0 ? (uint64_t)(uint32_t)hit.y : 0;
@@ -1037,7 +1186,10 @@ int kg_join_build_finalize(kg_hash_join *j) {
     concat_build(j);
     uint64_t want = (uint64_t)(n > (int64_t)j->expected ? n : (int64_t)j->expected);
     if (want < 512) want = 512;
-    j->cap = 1ull << jlog2(want * 2);
+    // capacity for the tuned load factor (50 % unless tuned), a power of two
+    const TuneCfg tc = tune_for(j->ctx, (int64_t)want);
+    const uint64_t pct = (uint64_t)std::min(90, std::max(10, (int)tc.load_pct));
+    j->cap = 1ull << jlog2((want * 100 + pct - 1) / pct);
     j->log2cap = jlog2(j->cap);
     j->slots.reset(j->ctx, (j->cap + 2) * sizeof(Slot));
     j->next.reset(j->ctx, (uint64_t)(n > 0 ? n : 1) * 4);
@@ -1088,8 +1240,13 @@ static void probe_impl(kg_hash_join *j, int32_t ncols, const kg_column *cols, in
       j->off.ensure(j->ctx, (uint64_t)(np + 1) * 8);
       {
         ProfScope prof(j->dspan ? "join_probe_direct" : "join_probe", j->s);
-        k_join_count<<<stream_grid(np, 256), 256, 0, j->s>>>(
-            K, np, sel, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
+        const bool fast = K.n == 1 && type_width(K.c[0].type) == 8 && !is_float(K.c[0].type);
+        if (fast)
+          k_join_count<true><<<stream_grid(np, 256, 2 * kJCU), 256, 0, j->s>>>(
+              K, np, sel, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
+        else
+          k_join_count<false><<<stream_grid(np, 256, 2 * kJCU), 256, 0, j->s>>>(
+              K, np, sel, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
         KG_HIP(hipGetLastError());
       }
       scan_u32(j, j->nout.get<uint32_t>(), j->off.get<uint64_t>(), np);
@@ -1225,8 +1382,12 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
     optin = 64 * 1024;
   }
   // one 1024-lane workgroup a CU with the whole LDS: twice the window of two
-  // 512-lane ones, so half the partitions (longer runs for the scatter)
-  const uint64_t lds_cap = (uint64_t)std::min(optin, 160 * 1024) - 1024;
+  // 512-lane ones, so half the partitions (longer runs for the scatter);
+  // the tuner may pick the two 512-lane ones
+  const TuneCfg tc = tune_for(j->ctx, j->build_rows);
+  const int mlanes = tc.merge_lanes == 512 ? 512 : 1024;
+  const uint64_t lds_all = (uint64_t)std::min(optin, 160 * 1024);
+  const uint64_t lds_cap = (mlanes == 1024 ? lds_all : lds_all / 2) - 1024;
   const uint64_t per_slot = 8 + 4 + (uint64_t)WL * 8;
   uint64_t C = 1;
   while (C * 2 <= j->cap && (C * 2 + kJMargin) * per_slot <= lds_cap) C *= 2;
@@ -1243,8 +1404,15 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   if (lds > lds_cap) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the join window");
   KG_HIP(hipFuncSetAttribute((const void *)k_jp_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
-  // staged scatter: tiles of kJSBlock * R rows sorted in LDS
-  const int R = F.rw <= 2 ? 12 : F.rw <= 3 ? 8 : 4;
+  // staged scatter: tiles of kJSBlock * R rows sorted in LDS (R tuned, or
+  // by the row width; a tuned R whose tile does not fit falls back)
+  // (a FAST tile keeps at most js_nv(R) values a row in registers)
+  int R = F.rw <= 2 ? 12 : F.rw <= 3 ? 8 : 4;
+  if ((tc.scatter_rows == 4 || tc.scatter_rows == 8 || tc.scatter_rows == 12) &&
+      js_lds_bytes(tc.scatter_rows, F.rw, np) + 1024 <= (size_t)optin &&
+      (!fast || F.ncols <= js_nv(tc.scatter_rows)))
+    R = tc.scatter_rows;
+  if (fast && F.ncols > js_nv(R)) fail(KG_ERR_INTERNAL, "staged scatter: too many values a row");
   const size_t slds = js_lds_bytes(R, F.rw, np);
   const bool staged = np <= kJSMaxParts && F.rw <= 6 && slds + 1024 <= (size_t)optin;
   int64_t per_block;
@@ -1304,10 +1472,10 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
     }
     KG_HIP(hipGetLastError());
   }
-  const unsigned grid = (unsigned)std::min<int64_t>(np, cus * 2);
+  const unsigned grid = (unsigned)std::min<int64_t>(np, cus * (mlanes == 1024 ? 2 : 4));
   {
     ProfScope pm("jp_merge", j->s);
-    k_jp_merge<<<grid, kJMBlock, lds, j->s>>>(T, Q, F, rows.get<uint64_t>(), hist.get<uint32_t>(),
+    k_jp_merge<<<grid, mlanes, lds, j->s>>>(T, Q, F, rows.get<uint64_t>(), hist.get<uint32_t>(),
                                                hn, base.get<uint64_t>(), (uint32_t)nb, log2p,
                                                (int)C, Wn, j->rec.get<uint64_t>());
     KG_HIP(hipGetLastError());

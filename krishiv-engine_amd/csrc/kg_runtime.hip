@@ -222,6 +222,23 @@ void dev_free(kg_ctx *ctx, void *p, uint64_t bytes, hipStream_t stream) {
   ctx->pending.push_back(std::move(pb));
 }
 
+static int size_class(int64_t rows) {
+  int b = 0;
+  while (b < 62 && (1ll << b) < rows) b++;
+  return b;
+}
+
+TuneCfg tune_for(kg_ctx *ctx, int64_t rows) {
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  auto it = ctx->tune.find(size_class(rows));
+  return it == ctx->tune.end() ? TuneCfg{} : it->second;
+}
+
+void tune_store(kg_ctx *ctx, int64_t rows, const TuneCfg &c) {
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  ctx->tune[size_class(rows)] = c;
+}
+
 int type_width(int t) {
   switch (t) {
     case KG_T_BOOL: return 0;

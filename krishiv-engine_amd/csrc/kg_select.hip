@@ -75,14 +75,26 @@ struct TakeCols {
   int n;
 };
 
+// A wave takes 64 consecutive output rows (whole-wave blocks and strides):
+// a nullable column's validity bits of those rows are two whole 32-bit
+// words, one ballot stored by lanes 0 and 32.
 __global__ void k_take(TakeCols G, const int32_t *rows, int64_t n) {
-  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < n;
-       o += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = rows[o];
+  const int lane = threadIdx.x & 63;
+  for (int64_t b0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); b0 < n;
+       b0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = b0 + lane;
+    const bool in = o < n;
+    const int64_t r = in ? rows[o] : 0;
     for (int c = 0; c < G.n; c++) {
       const DCol &s = G.src[c];
       int64_t j = r + s.offset;
       void *out = G.dst[c];
+      if (G.dst_valid[c]) {
+        const uint64_t m = __ballot(in && dev::valid(s, r));
+        if (lane == 0) G.dst_valid[c][b0 >> 5] = (uint32_t)m;
+        if (lane == 32 && b0 + 32 < n) G.dst_valid[c][(b0 >> 5) + 1] = (uint32_t)(m >> 32);
+      }
+      if (!in) continue;
       switch (s.width) {
         case 1: static_cast<uint8_t *>(out)[o] = static_cast<const uint8_t *>(s.values)[j]; break;
         case 2: static_cast<uint16_t *>(out)[o] = static_cast<const uint16_t *>(s.values)[j]; break;
@@ -99,7 +111,6 @@ __global__ void k_take(TakeCols G, const int32_t *rows, int64_t n) {
           if ((static_cast<const uint8_t *>(s.values)[j >> 3] >> (j & 7)) & 1)
             atomicOr(static_cast<uint32_t *>(out) + (o >> 5), 1u << (o & 31));
       }
-      if (G.dst_valid[c] && dev::valid(s, r)) atomicOr(G.dst_valid[c] + (o >> 5), 1u << (o & 31));
     }
   }
 }

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <condition_variable>
+#include <cstdlib>
 #include <deque>
 #include <map>
 #include <set>
@@ -224,9 +225,10 @@ struct kg_server {
     uint64_t bytes = 0;
   };
   std::deque<Pending> queue;
-  // Probes of kLargeProbe rows or more run on kLargeWorkers worker threads
-  // (each request on its own stream), so batches keep launching meanwhile
-  // and one large request's host waits are filled by another's kernels.
+  // Probes of kLargeProbe rows or more run on `nworkers` worker threads
+  // (each request on its own stream), so batches keep launching meanwhile;
+  // with two, one large request's host waits are filled by another's
+  // kernels, at the price of each sharing the device with the other.
   struct Large {
     kg_serve_request r;
     kg_hash_join *table = nullptr;  // the resident table's join, if any
@@ -245,6 +247,7 @@ struct kg_server {
   int64_t large_outstanding = 0;  // queued + running (main thread's count)
   bool large_stop = false;
   std::vector<std::thread> large_workers;
+  int nworkers = 1;
   struct Slot {
     hipStream_t s = nullptr;
     DevBuf arena;          // tables, records, outputs of the batch
@@ -271,7 +274,7 @@ namespace kg {
 // which the batch's tile kernels (one global atomic per matching row) would
 // only slow down.  Smaller requests are packed into batches.
 constexpr int64_t kLargeProbe = 4 << 20;
-constexpr int kLargeWorkers = 2;
+constexpr int kLargeWorkers = 1;  // default; KG_SERVE_LARGE_WORKERS (1 to 4) overrides
 constexpr int64_t kBatchProbeRows = 8 << 20;
 
 static void ck(int rc) {
@@ -595,6 +598,12 @@ int kg_server_create(kg_ctx *ctx, uint64_t arena_bytes, int32_t max_batch, int32
     S->max_batch = max_batch;
     S->use_graph = use_graph != 0;
     S->arena_bytes = arena_bytes / kSrvSlots;
+    S->nworkers = kLargeWorkers;
+    if (const char *w = getenv("KG_SERVE_LARGE_WORKERS")) {
+      const int v = atoi(w);
+      if (v < 1 || v > 4) fail(KG_ERR_UNSUPPORTED, "KG_SERVE_LARGE_WORKERS must be 1 to 4");
+      S->nworkers = v;
+    }
     for (auto &sl : S->slots) {
       sl.s = make_stream(ctx->device);
       sl.arena.reset(ctx, S->arena_bytes);
@@ -670,7 +679,7 @@ int kg_server_submit(kg_server *S, const kg_serve_request *r) {
     if (r->probe_rows >= kLargeProbe) {
       kg_server::Large L{*r, r->build_keys ? nullptr : S->tables[r->table].join};
       std::lock_guard<std::mutex> g(S->large_mu);
-      while (S->large_workers.size() < (size_t)kLargeWorkers)
+      while (S->large_workers.size() < (size_t)S->nworkers)
         S->large_workers.emplace_back(large_loop, S);
       S->large.push_back(L);
       S->large_outstanding++;

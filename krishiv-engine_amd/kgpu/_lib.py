@@ -122,6 +122,12 @@ _SIGS = {
     "kg_prof_reset": (None, []),
     "kg_prof_get": (C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p]),
     "kg_ctx_mem_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_tune_get": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "kg_tune_set": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "kg_autotune_joinagg": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "kg_tune_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "kg_tune_load": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
     "kg_calib_random_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int64, C.c_int32,
                                        C.c_void_p]),
     "kg_calib_atomic_add": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int32, C.c_void_p]),

@@ -542,6 +542,50 @@ def prof_get(family: str):
     return ms.value, n.value
 
 
+class TuneConfig(C.Structure):
+    """kg_tune_config: the join / fused-aggregate shape of one build size class."""
+    _fields_ = [("load_factor_pct", C.c_int32), ("scatter_rows", C.c_int32),
+                ("merge_lanes", C.c_int32), ("_pad", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {"load_factor_pct": self.load_factor_pct, "scatter_rows": self.scatter_rows,
+                "merge_lanes": self.merge_lanes}
+
+
+def tune_get(ctx, build_rows: int) -> dict:
+    c = TuneConfig()
+    check(lib().kg_tune_get(ctx.h, build_rows, C.byref(c)))
+    return c.as_dict()
+
+
+def tune_set(ctx, build_rows: int, load_factor_pct: int = 50, scatter_rows: int = 0,
+             merge_lanes: int = 1024):
+    c = TuneConfig(load_factor_pct, scatter_rows, merge_lanes, 0)
+    check(lib().kg_tune_set(ctx.h, build_rows, C.byref(c)))
+
+
+def tune_save(ctx, path: str):
+    check(lib().kg_tune_save(ctx.h, path.encode()))
+
+
+def tune_load(ctx, path: str) -> int:
+    n = C.c_int32(0)
+    check(lib().kg_tune_load(ctx.h, path.encode(), C.byref(n)))
+    return n.value
+
+
+def autotune_joinagg(ctx, build_rows: int, probe_rows: int):
+    """Times every candidate shape for this build size class and keeps the
+    fastest (kg_autotune_joinagg); returns (shape, ms per build + probe,
+    candidates timed)."""
+    c = TuneConfig()
+    ms = C.c_double(0)
+    n = C.c_int32(0)
+    check(lib().kg_autotune_joinagg(ctx.h, build_rows, probe_rows, C.byref(c), C.byref(ms),
+                                    C.byref(n)))
+    return c.as_dict(), ms.value, n.value
+
+
 def calib_random_read(ctx, table_bytes: int, read_bytes: int, ops: int, reps: int = 3) -> float:
     """Random reads per second from a device table of table_bytes (the
     ceiling a probe phase over a table of that size is measured against)."""

@@ -48,11 +48,12 @@ def make_request(rng, nb, dup=False):
     return keys, fk, fkv, pay
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_mixed_batch_own_and_resident_tables(use_graph):
-    say(f"serve: mixed sizes, resident tables, graph={use_graph}")
+@pytest.mark.parametrize("use_graph,workers", [(True, 1), (False, 1), (True, 2)])
+def test_mixed_batch_own_and_resident_tables(use_graph, workers, monkeypatch):
+    say(f"serve: mixed sizes, resident tables, graph={use_graph}, large workers={workers}")
     rng = np.random.default_rng(8 + use_graph)
     c = ctx()
+    monkeypatch.setenv("KG_SERVE_LARGE_WORKERS", str(workers))
     srv = kgpu.JoinAggServer(c, arena_bytes=1 << 30, max_batch=16, use_graph=use_graph)
     # two resident build tables
     res_keys, res_dev = [], []
