@@ -322,6 +322,8 @@ __global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, S
     bool miss[R];
     uint64_t key[R];
     int64_t pos[R];
+    // (no `safe` row: the branch-free column loads need more registers
+    // than this kernel's 128 and would spill)
     ColSrc<R> src;
     // one-word integer keys: every row's key and validity loads are issued
     // before the first probe (clamped row index, the validity byte kept raw)
@@ -637,15 +639,22 @@ __global__ void __launch_bounds__(kMergeBlock) k_agg_merge(AggProg P, Table T, M
         int ls[R], lsw[R];
         SpillSrc<R> src;
         src.rows = M.rows;
+        src.safe = r1 > r0 ? r0 : -1;
+        // every row's key load is issued before the first insert (a row
+        // index clamped into the partition, so no per-lane branch)
+        uint64_t key[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int64_t uc = u[r] < total ? u[r] : total - 1;
+          const uint64_t *kp = uc < nrec ? M.recs + (c0 + uc) * (1 + W)
+                                         : M.rows + (r0 + uc - nrec) * M.F.rw;
+          key[r] = *kp;
+        }
 #pragma unroll
         for (int r = 0; r < R; r++) {
           ls[r] = -1;
           src.idx[r] = r0 + u[r] - nrec;
-          if (!done[r]) {
-            const uint64_t key = u[r] < nrec ? M.recs[(c0 + u[r]) * (1 + W)]
-                                             : M.rows[src.idx[r] * M.F.rw];
-            ls[r] = lds_insert(s_keys, (uint32_t)S, &s_occ, limit, T, key);
-          }
+          if (!done[r]) ls[r] = lds_insert(s_keys, (uint32_t)S, &s_occ, limit, T, key[r]);
           if (ls[r] >= 0 && u[r] < nrec)
             merge_into_lds(P, s_rec + ls[r] * WL, M.recs + (c0 + u[r]) * (1 + W) + 1);
           lsw[r] = (ls[r] >= 0 && u[r] >= nrec) ? ls[r] : -1;

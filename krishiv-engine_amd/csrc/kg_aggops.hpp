@@ -65,6 +65,13 @@ struct AggProg {
   uint8_t spill_off[kAggInputs];
 };
 
+// Op o of the program through a uniform index: the op fields then come
+// from the kernel argument segment with scalar loads, not one vector load
+// and wait per field and row tile.
+__device__ __forceinline__ AggOp op_at(const AggProg &P, int o) {
+  return P.ops[__builtin_amdgcn_readfirstlane(o)];
+}
+
 struct Table {
   uint64_t *keys;
   uint64_t *rec;
@@ -180,7 +187,7 @@ __device__ __forceinline__ void apply_row(const AggProg &P, uint64_t *r, int64_t
   uint64_t lo = 0, hi = 0, top = 0;
   double f = 0.0;
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     if (op.kind == OP_COUNT_STAR) {
       add_u64_nr<LDS>(r, 1);
       continue;
@@ -285,7 +292,7 @@ __device__ __forceinline__ void apply_wave(const AggProg &P, uint64_t *r, int64_
   uint64_t lo = 0, hi = 0, top = 0;
   double f = 0.0;
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     if (op.kind == OP_COUNT_STAR) {
       if (leader) add_u64_nr<LDS>(r, (uint64_t)__popcll(peers));
       continue;
@@ -397,7 +404,7 @@ __device__ __forceinline__ void apply_row_mult(const AggProg &P, uint64_t *r, in
   uint64_t lo = 0, hi = 0;
   double f = 0.0;
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     if (op.kind == OP_COUNT_STAR) {
       add_u64_nr<false>(r, m);
       continue;
@@ -442,7 +449,7 @@ __device__ __forceinline__ void apply_row_mult(const AggProg &P, uint64_t *r, in
 // Merges an LDS record into a global record.
 __device__ __forceinline__ void merge_record(const AggProg &P, uint64_t *g, const uint64_t *l) {
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     const uint64_t *s = l + op.word;
     uint64_t *d = g + op.word;
     switch (op.kind) {

@@ -47,12 +47,38 @@ inline void set_spill_layout(AggProg &P, const SpillFmt &F) {
 template <int R>
 struct ColSrc {
   int64_t idx[R];
+  // a row every lane may read (>= 0): rows not active then load it instead
+  // of branching, so the loads of a lane's rows are in flight together; -1
+  // when there is none (the loads are then taken per active row)
+  int64_t safe = -1;
   __device__ __forceinline__ void load(const AggProg &P, int c, const bool (&act)[R],
                                        uint64_t (&lo)[R], uint64_t (&hi)[R], double (&f)[R],
                                        bool (&ok)[R]) const {
     const DCol &d = P.in[c];
     const int t = d.type;
     const int64_t off = d.offset;
+    if (safe >= 0 && (t == KG_T_FLOAT64 || t == KG_T_INT64 || t == KG_T_DECIMAL64 ||
+                      t == KG_T_TIMESTAMP || t == KG_T_INT32 || t == KG_T_DATE32)) {
+      // one word a row, no per-lane branch (a row not active reads `safe`);
+      // the validity bytes stay raw until every load is issued
+      uint32_t vb[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t j = (act[r] ? idx[r] : safe) + off;
+        lo[r] = (t == KG_T_INT32 || t == KG_T_DATE32)
+                    ? (uint64_t)(int64_t) static_cast<const int32_t *>(d.values)[j]
+                    : static_cast<const uint64_t *>(d.values)[j];
+        vb[r] = d.validity ? d.validity[j >> 3] : 0xFFu;
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t j = (act[r] ? idx[r] : safe) + off;
+        f[r] = __longlong_as_double((long long)lo[r]);  // read for FLOAT64 only
+        hi[r] = (uint64_t)((int64_t)lo[r] >> 63);
+        ok[r] = act[r] && ((vb[r] >> (j & 7)) & 1);
+      }
+      return;
+    }
     if (t == KG_T_FLOAT64) {
       const double *v = static_cast<const double *>(d.values);
 #pragma unroll
@@ -108,24 +134,42 @@ template <int R>
 struct SpillSrc {
   const uint64_t *rows;
   int64_t idx[R];
+  // a row index every lane may read (>= 0): rows not active then load it
+  // instead of branching, so a lane's loads are all in flight together;
+  // -1 when there is none (the loads are then taken per active row)
+  int64_t safe = -1;
   __device__ __forceinline__ void load(const AggProg &P, int c, const bool (&act)[R],
                                        uint64_t (&lo)[R], uint64_t (&hi)[R], double (&f)[R],
                                        bool (&ok)[R]) const {
     const int t = P.in[c].type;
     const int rw = P.spill_rw, o = P.spill_off[c];
     const bool wide = t == KG_T_DECIMAL128, uns = t == KG_T_UINT64;
+    uint64_t v[R], vb[R], w2[R];
     // every array is indexed by the unrolled r only (a loop that is not
     // unrolled would put lo/hi/f in scratch)
+    if (safe >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const uint64_t *row = rows + (act[r] ? idx[r] : safe) * (int64_t)rw;
+        v[r] = row[o];
+        vb[r] = P.spill_vword ? row[1] : ~0ull;
+        w2[r] = wide ? row[o + 1] : 0;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const uint64_t *row = rows + idx[r] * (int64_t)rw;
+        v[r] = act[r] ? row[o] : 0;
+        vb[r] = act[r] && P.spill_vword ? row[1] : ~0ull;
+        w2[r] = act[r] && wide ? row[o + 1] : 0;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const uint64_t *row = rows + idx[r] * (int64_t)rw;
-      const uint64_t v = act[r] ? row[o] : 0;
-      const uint64_t vb = act[r] && P.spill_vword ? row[1] : ~0ull;
-      const uint64_t w2 = act[r] && wide ? row[o + 1] : 0;
-      ok[r] = act[r] && ((vb >> c) & 1);
-      f[r] = __longlong_as_double((long long)v);
-      lo[r] = v;
-      hi[r] = wide ? w2 : uns ? 0 : (uint64_t)((int64_t)v >> 63);
+      ok[r] = act[r] && ((vb[r] >> c) & 1);
+      f[r] = __longlong_as_double((long long)v[r]);
+      lo[r] = v[r];
+      hi[r] = wide ? w2[r] : uns ? 0 : (uint64_t)((int64_t)v[r] >> 63);
     }
   }
 };
@@ -149,7 +193,7 @@ __device__ __forceinline__ void tile_apply(const AggProg &P, uint64_t *srec, int
   }
   int last = -1;
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     if (op.kind == OP_COUNT_STAR) {
 #pragma unroll
       for (int r = 0; r < R; r++)
@@ -277,7 +321,7 @@ __device__ __forceinline__ void tile_apply_mult(const AggProg &P, uint64_t *srec
   }
   int last = -1;
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     if (op.kind == OP_COUNT_STAR) {
 #pragma unroll
       for (int r = 0; r < R; r++)
@@ -343,7 +387,7 @@ __device__ __forceinline__ void tile_apply_mult(const AggProg &P, uint64_t *srec
 // Merges a state record (W words) into an LDS record (LDS atomics).
 __device__ __forceinline__ void merge_into_lds(const AggProg &P, uint64_t *d, const uint64_t *s) {
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     const uint64_t *a = s + op.word;
     uint64_t *b = d + op.word;
     switch (op.kind) {
@@ -385,7 +429,7 @@ __device__ __forceinline__ void merge_into_lds(const AggProg &P, uint64_t *d, co
 // the calling lane alone (its key's owner).
 __device__ __forceinline__ void merge_plain(const AggProg &P, uint64_t *d, const uint64_t *s) {
   for (int o = 0; o < P.nops; o++) {
-    const AggOp op = P.ops[o];
+    const AggOp op = op_at(P, o);
     const uint64_t *a = s + op.word;
     uint64_t *b = d + op.word;
     switch (op.kind) {

@@ -381,6 +381,10 @@ __global__ void k_join_direct(JTable T, int32_t *dslot) {
 constexpr int kJDBlock = 512;
 constexpr int kJDTile = 4;
 
+// FAST: one 8-byte integer key column; its rows' key and validity loads are
+// issued together (clamped row index), as are the input loads (row 0 stands
+// in for rows past n).
+template <bool FAST>
 __global__ void __launch_bounds__(kJDBlock) k_joinagg_direct(KeySpec K, int64_t n, JTable T,
                                                              AggProg P, uint64_t *rec) {
   extern __shared__ uint64_t smem[];
@@ -399,15 +403,39 @@ __global__ void __launch_bounds__(kJDBlock) k_joinagg_direct(KeySpec K, int64_t 
     int ls[kJDTile];
     uint64_t m[kJDTile];
     ColSrc<kJDTile> src;
+    src.safe = 0;  // n >= 1
+    uint64_t key[kJDTile];
+    int cls[kJDTile];
+    if (FAST) {
+      const DCol &kc = K.c[0];
+      uint32_t kb[kJDTile];
+#pragma unroll
+      for (int r = 0; r < kJDTile; r++) {
+        const int64_t i = t0 + (int64_t)r * kJDBlock + threadIdx.x, ic = i < n ? i : n - 1;
+        key[r] = dev::load_w64(kc, ic);
+        kb[r] = kc.validity ? kc.validity[(ic + kc.offset) >> 3] : 0xFFu;
+      }
+#pragma unroll
+      for (int r = 0; r < kJDTile; r++) {
+        const int64_t i = t0 + (int64_t)r * kJDBlock + threadIdx.x, ic = i < n ? i : n - 1;
+        // a NULL key matches nothing (no `<=>` join has a direct index)
+        cls[r] = i < n && ((kb[r] >> ((ic + kc.offset) & 7)) & 1) ? 0 : 3;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kJDTile; r++) {
+        const int64_t i = t0 + (int64_t)r * kJDBlock + threadIdx.x;
+        cls[r] = i < n ? pack_key(K, i, key[r]) : 3;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < kJDTile; r++) {
       const int64_t i = t0 + (int64_t)r * kJDBlock + threadIdx.x;
       src.idx[r] = i < n ? i : 0;
       ls[r] = -1;
       m[r] = 0;
-      uint64_t key;
-      if (i < n && pack_key(K, i, key) != 3) {
-        const uint64_t d = key - (uint64_t)T.dmin;
+      if (cls[r] != 3) {
+        const uint64_t d = key[r] - (uint64_t)T.dmin;
         if (d < T.dspan && s_cnt[d]) {
           ls[r] = (int)d;
           m[r] = s_cnt[d];
@@ -847,6 +875,7 @@ __global__ void __launch_bounds__(kJMBlock) k_jp_merge(JTable T, AggProg P, Spil
       bool fb[kJTile];
       SpillSrc<kJTile> src;
       src.rows = rows;
+      src.safe = a;  // a < b inside this loop
       // every row's key load is issued before the first probe (clamped index)
       uint64_t kk[kJTile];
 #pragma unroll
@@ -1516,14 +1545,21 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
     if (j->dspan && dlds + 1024 <= (size_t)optin && nrows >= (1 << 16)) {
       // direct index, records privatized in LDS
       ProfScope prof("joinagg_direct", j->s);
-      KG_HIP(hipFuncSetAttribute((const void *)k_joinagg_direct,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)dlds));
+      const bool dfast = K.n == 1 && !K.null_safe && type_width(K.c[0].type) == 8 &&
+                         !is_float(K.c[0].type);
+      const void *dfn = dfast ? (const void *)k_joinagg_direct<true>
+                              : (const void *)k_joinagg_direct<false>;
+      KG_HIP(hipFuncSetAttribute(dfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dlds));
       const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
       const int64_t per_cu = dlds <= 40 * 1024 ? 4 : dlds <= 80 * 1024 ? 2 : 1;
       const int64_t grid = std::min<int64_t>(cus * per_cu,
                                              ceil_div(nrows, (int64_t)kJDBlock * kJDTile * 8));
-      k_joinagg_direct<<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
-          K, nrows, jtable(j), P, j->rec.get<uint64_t>());
+      if (dfast)
+        k_joinagg_direct<true><<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
+            K, nrows, jtable(j), P, j->rec.get<uint64_t>());
+      else
+        k_joinagg_direct<false><<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
+            K, nrows, jtable(j), P, j->rec.get<uint64_t>());
       KG_HIP(hipGetLastError());
     } else if (nrows < (1 << 20) || !joinagg_partitioned(j, K, P, nrows)) {
       ProfScope prof(j->dspan ? "joinagg_probe_direct" : "joinagg_probe", j->s);
