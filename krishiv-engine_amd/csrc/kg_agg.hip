@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
 // workgroup that would pass it stops, leaving its unmerged records and
 // pending inputs for the host, which grows the table, merges them with
 // atomics (k_agg_retry) and resumes the partition.
-constexpr int kP1Block = 512;
+constexpr int kP1Block = 1024;
 constexpr int kMergeBlock = 1024;  // 16 waves a CU: latency of the spill reads hidden
 constexpr int kMergeRows = 2;      // rows a lane per tile (96 VGPRs at 1024 lanes)
 constexpr int kTileRows = 4;
@@ -1383,7 +1383,7 @@ static void record_types(kg_hash_agg *a, const AggProg &P) {
 static void pass1(kg_hash_agg *a, const AggProg &P, int64_t nrows) {
   record_types(a, P);
   const int64_t per_wg = (int64_t)kTileRows * kP1Block * 8;
-  const int64_t maxb = (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256) * 2;
+  const int64_t maxb = (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256);
   // pieces whose row spill fits the budget
   const int64_t piece_max =
       std::max<int64_t>(1 << 20, (int64_t)(a->spill_budget / 2 / (8 * (uint64_t)a->fmt.rw)));
@@ -1954,9 +1954,16 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
     if (a->slots == 0) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the LDS table");
     a->hashed = a->has_key && !a->direct;
     if (a->hashed) {
-      // LDS tables sized so that two workgroups share a CU (80 KiB each)
-      const size_t budget = 80 * 1024 - 64;
+      // one 1,024-lane workgroup a CU with the whole LDS: one table of the
+      // heavy keys a CU (two 512-lane ones would each hold the same top keys)
       const int WL = W | 1;
+      int p1_optin = 0;
+      if (hipDeviceGetAttribute(&p1_optin, hipDeviceAttributeSharedMemPerBlockOptin,
+                                ctx->device) != hipSuccess || p1_optin < 64 * 1024) {
+        (void)hipGetLastError();
+        p1_optin = 64 * 1024;
+      }
+      const size_t budget = (size_t)std::min(p1_optin, 160 * 1024) - 1024;
       a->p1_slots = (int)((budget / 8 - 2 * (size_t)WL) / (1 + (size_t)WL));
       // the merge: one workgroup a CU with as much LDS as a workgroup may take
       int optin = 0;
