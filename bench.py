@@ -55,7 +55,8 @@ def parse():
     p.add_argument("--qps", type=float, default=400.0)
     p.add_argument("--chunk", type=int, default=48)
     p.add_argument("--autotune", action="store_true",
-                   help="joinagg: run the shape autotuner for this build size before timing")
+                   help="joinagg: run the shape autotuner for this build size before timing; "
+                        "serve: for the size classes of the large requests' builds")
     p.add_argument("--large-workers", type=int, default=0,
                    help="serve: large-request worker threads (KG_SERVE_LARGE_WORKERS; 0: the default)")
     p.add_argument("--latency-out", default="",
@@ -475,6 +476,15 @@ def serve(args):
         plain_tables.append(j)
     if args.large_workers:
         os.environ["KG_SERVE_LARGE_WORKERS"] = str(args.large_workers)
+    tuned = {}
+    if args.autotune:
+        # the size classes of the large requests' builds (4M probe rows and
+        # more: builds of 400K rows and more), tuned before either form runs
+        t0 = time.perf_counter()
+        for lg in range(19, 25):
+            shape, _, _ = kgpu.autotune_joinagg(ctx, 1 << lg, 10 << lg)
+            tuned[f"2^{lg}"] = shape
+        tuned["tuning_s"] = time.perf_counter() - t0
     srv = kgpu.JoinAggServer(ctx, arena_bytes=64 << 30, max_batch=256, use_graph=True)
     srv_tables = [srv.add_table(res_keys[t], int(res_sizes[t])) for t in range(8)]
 
@@ -574,7 +584,8 @@ def serve(args):
         "config": {"workload": f"{R} join+aggregate requests, Poisson {args.qps} q/s (seed 8), build "
                                "log-uniform [1K, 10M], probe 10x, 30 % on 8 resident builds, "
                                "arena 64 GB", "chunk": args.chunk,
-                   "large_workers": int(os.environ.get("KG_SERVE_LARGE_WORKERS", "1"))},
+                   "large_workers": int(os.environ.get("KG_SERVE_LARGE_WORKERS", "1")),
+                   **({"autotuned_classes": tuned} if tuned else {})},
         "plain": out["plain"], "batched": out["batched"],
         "batched_over_plain_qps": out["batched"]["queries_per_s"] / out["plain"]["queries_per_s"],
     }), flush=True)
