@@ -125,10 +125,14 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, in
 // it tile by tile (R rows a lane, op-major) to an LDS table of SLOTS records
 // plus two records for the key INT64_MIN and the NULL key.  Rows whose key
 // finds no LDS slot within kLdsProbe probes are appended to the row spill.
-// Every kEvictRows rows the records with fewer than kHotRows rows (keys seen
-// once or twice: the Zipf tail) move to the record spill and free their
-// slots, so the LDS table keeps the heavy keys; at the end every record
-// moves there.  The global table is not touched by ordinary keys.
+// A key takes an LDS slot only at its second sighting within an eviction
+// window: a 16K-bit LDS doorkeeper records first sightings (one atomicOr),
+// whose rows spill as 40-byte rows instead of becoming records that the next
+// eviction would spill at 8 x (W + 1) bytes.  Every kEvictRows rows the
+// records with fewer than kHotRows rows (the Zipf tail) move to the record
+// spill and free their slots, and the doorkeeper is cleared, so the LDS
+// table keeps the heavy keys; at the end every record moves there.  The
+// global table is not touched by ordinary keys.
 //
 // Spill space: each wave takes kWaveChunk rows of the row spill at a time
 // with one atomic; rows it leaves unused are marked dead (bit 63 of the
@@ -149,8 +153,8 @@ constexpr int kMergeBlock = 1024;  // 16 waves a CU: latency of the spill reads 
 constexpr int kMergeRows = 2;      // rows a lane per tile (96 VGPRs at 1024 lanes)
 constexpr int kTileRows = 4;
 constexpr int kWaveChunk = 512;
-constexpr int64_t kEvictRows = 32768;
-constexpr uint64_t kHotRows = 4;
+constexpr int64_t kEvictRows = 65536;
+constexpr uint64_t kHotRows = 2;
 constexpr uint64_t kDeadRow = 1ull << 63;
 
 struct SpillDev {
@@ -247,13 +251,21 @@ __device__ __forceinline__ uint32_t lds_home(const Table &T, uint64_t key, uint3
   return (uint32_t)(((slot_hash(T, key) & 0xFFFFFFFFull) * slots) >> 32);
 }
 
+// The doorkeeper: kDoorWords 32-bit words of LDS (16K bits), bit from the
+// key's multiplicative hash.
+constexpr int kDoorWords = 512;
+
 __device__ __forceinline__ int lds_probe(uint64_t *s_keys, uint32_t slots, const Table &T,
-                                         uint64_t key) {
+                                         uint64_t key, uint32_t *s_door) {
   uint32_t s = lds_home(T, key, slots);
   for (int p = 0; p < kLdsProbe; p++) {
     uint64_t k = s_keys[s];
     if (k == key) return (int)s;
     if (k == kEmpty) {
+      // a free slot: only a key already seen in this window takes it
+      const uint32_t b = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 50);
+      const uint32_t old = atomicOr(&s_door[b >> 5], 1u << (b & 31));
+      if (!((old >> (b & 31)) & 1)) return -1;  // first sighting: the row spills
       unsigned long long prev = atomicCAS((unsigned long long *)&s_keys[s], kEmpty, key);
       if (prev == kEmpty || prev == key) return (int)s;
     }
@@ -303,6 +315,8 @@ __global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, S
   uint64_t *s_rec = smem + SLOTS;  // SLOTS + 2 records
   const int W = P.W;
   const int WL = W | 1;  // odd LDS record stride: records spread over the banks
+  uint32_t *s_door = reinterpret_cast<uint32_t *>(s_rec + (size_t)(SLOTS + 2) * WL);
+  for (int t = threadIdx.x; t < kDoorWords; t += blockDim.x) s_door[t] = 0;
   for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) s_keys[t] = kEmpty;
   fill_lds_records(P, s_rec, SLOTS + 2, WL);
   __syncthreads();
@@ -316,6 +330,8 @@ __global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, S
   for (int64_t base = lo; base < hi; base += (int64_t)R * blockDim.x) {
     if (base >= next_evict) {
       spill_lds_records(P, S, s_keys, s_rec, SLOTS, false);
+      for (int t = threadIdx.x; t < kDoorWords; t += blockDim.x) s_door[t] = 0;
+      __syncthreads();
       next_evict = base + kEvictRows;
     }
     int ls[R];
@@ -360,7 +376,7 @@ __global__ void __launch_bounds__(kP1Block, 4) k_agg_pass1(AggProg P, Table T, S
         if (cls) {
           ls[r] = SLOTS + cls - 1;
         } else {
-          ls[r] = lds_probe(s_keys, (uint32_t)SLOTS, T, key[r]);
+          ls[r] = lds_probe(s_keys, (uint32_t)SLOTS, T, key[r], s_door);
           miss[r] = ls[r] < 0;
         }
       }
@@ -1963,7 +1979,7 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
         (void)hipGetLastError();
         p1_optin = 64 * 1024;
       }
-      const size_t budget = (size_t)std::min(p1_optin, 160 * 1024) - 1024;
+      const size_t budget = (size_t)std::min(p1_optin, 160 * 1024) - 1024 - 4 * kDoorWords;
       a->p1_slots = (int)((budget / 8 - 2 * (size_t)WL) / (1 + (size_t)WL));
       // the merge: one workgroup a CU with as much LDS as a workgroup may take
       int optin = 0;
@@ -1975,7 +1991,7 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       a->m_slots = (int)(((size_t)std::min(optin, 160 * 1024) - 1024) / 8 / (1 + (size_t)WL));
       if (a->p1_slots < 64 || a->m_slots < 64)
         fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the LDS table");
-      a->p1_lds = ((size_t)a->p1_slots + (size_t)(a->p1_slots + 2) * WL) * 8;
+      a->p1_lds = ((size_t)a->p1_slots + (size_t)(a->p1_slots + 2) * WL) * 8 + 4 * kDoorWords;
       a->m_lds = (size_t)a->m_slots * (1 + WL) * 8;
       const std::pair<const void *, size_t> fns[2] = {
           {(const void *)k_agg_pass1<kTileRows>, a->p1_lds},
