@@ -410,7 +410,9 @@ int kg_prof_get(const char *family, double *total_ms, int64_t *launches);
  * No counterpart in the reference (its operators have no device shapes to
  * pick).  Per size class of the build side (ceil log2 of its rows) a
  * context keeps one shape: the table's load factor (capacity >= rows * 100 /
- * load_factor_pct, a power of two; 50 by default), the staged partition
+ * load_factor_pct, a power of two; for a class nobody tuned 25 up to 2^20
+ * build rows, 75 up to 2^23, else 50: what the tuner picked for the join
+ * aggregate on an MI355X), the staged partition
  * scatter's rows per lane (4, 8 or 12; 0 = by the row width) and the
  * partition merge's workgroup (1024 lanes with the whole LDS, or 512 with
  * half).  kg_autotune_joinagg times all 18 candidates (load 50/25/75 x rows

@@ -81,7 +81,8 @@ class Fence {
 // of the build side (kg_tune_*): the table's load factor, the staged
 // partition scatter's rows per lane, and the partition merge's workgroup.
 struct TuneCfg {
-  int32_t load_pct = 50;     // table capacity >= rows * 100 / load_pct (a power of two)
+  int32_t load_pct = 50;     // table capacity >= rows * 100 / load_pct (a power of two);
+                             // untuned classes take a per-class value (tune_for)
   int32_t scatter_rows = 0;  // 4, 8 or 12; 0: by the row width
   int32_t merge_lanes = 1024;  // 1024 (one workgroup a CU, the whole LDS) or 512 (two)
 };

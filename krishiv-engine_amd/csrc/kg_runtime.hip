@@ -228,10 +228,20 @@ static int size_class(int64_t rows) {
   return b;
 }
 
+// A class nobody tuned: the load factors kg_autotune_joinagg picked on an
+// MI355X for the join aggregate (builds up to 2^20 keys: 25 %, to 2^23: 75 %,
+// larger: 50 %), the other fields at their defaults.
+static TuneCfg untuned(int cls) {
+  TuneCfg c;
+  c.load_pct = cls <= 20 ? 25 : cls <= 23 ? 75 : 50;
+  return c;
+}
+
 TuneCfg tune_for(kg_ctx *ctx, int64_t rows) {
+  const int cls = size_class(rows);
   std::lock_guard<std::mutex> g(ctx->tune_mu);
-  auto it = ctx->tune.find(size_class(rows));
-  return it == ctx->tune.end() ? TuneCfg{} : it->second;
+  auto it = ctx->tune.find(cls);
+  return it == ctx->tune.end() ? untuned(cls) : it->second;
 }
 
 void tune_store(kg_ctx *ctx, int64_t rows, const TuneCfg &c) {

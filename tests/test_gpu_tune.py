@@ -107,3 +107,11 @@ def test_tuned_shapes_save_and_load(tmp_path):
     finally:
         kgpu.tune_set(c, 5000)
         kgpu.tune_set(c, 3_000_000)
+
+
+def test_untuned_classes_take_the_measured_load_factors():
+    say("tune: untuned defaults")
+    c = ctx()
+    for rows, load in ((100, 25), (7_000_000, 75), (40_000_000, 50)):
+        assert kgpu.tune_get(c, rows) == {"load_factor_pct": load, "scatter_rows": 0,
+                                          "merge_lanes": 1024}, rows
