@@ -4,7 +4,8 @@
  * hash-repartition operators.
  *
  * The reference engine is Rust and has no FFI for this path; these entry
- * points are what a thin `extern "C"` crate binds.  Each
+ * points are what a thin `extern "C"` crate binds, one `extern` block per
+ * declaration below, with `#[repr(C)]` mirrors of the structs.  Each
  * declaration names the reference interface it replaces (paths relative to
  * the reference repository root).  Plain pointers and sizes only.
  *
