@@ -255,6 +255,19 @@ class Batch:
             vals = vals.astype(bool)
         return vals, valid.astype(bool)
 
+    def column_raw(self, i):
+        """(values, valid) on the host without decoding: decimal128 as an
+        (n, 2) int64 array of (low, high) words, other types as numpy."""
+        d = self.device_column(i)
+        n = self.num_rows
+        valid = np.ones(n, dtype=np.uint8)
+        if d.type == T_DECIMAL128:
+            vals = np.zeros((n, 2), dtype=np.int64)
+        else:
+            vals = np.zeros(n, dtype=_NP[d.type])
+        check(lib().kg_batch_column_to_host(self.h, i, vals.ctypes.data, valid.ctypes.data))
+        return vals, valid.astype(bool)
+
     def column_type(self, i):
         d = self.device_column(i)
         return d.type, d.precision, d.scale

@@ -508,16 +508,10 @@ int kgo_sbbf_build(const uint64_t *hashes, int64_t n, size_t bytes, double fpp, 
 }
 
 /* ------------------------------------------------------------------ */
-/* Spark-layout Bloom filter (optional).  The reference holds only the    */
-/* hash primitive: Murmur3 x86_32 as Spark runs it                        */
-/* (crates/krishiv-kernels/src/hash/murmur3.rs:9-52).  The filter is       */
-/* Spark's sketch BloomFilterImpl (version 1; Apache Spark, not in the     */
-/* reference): putLong(item) takes h1 = hashLong(item, 0), h2 =            */
-/* hashLong(item, h1) and sets bit (h1 + i*h2, negated by ~ when           */
-/* negative, 32-bit) mod bitSize for i = 1..k in a long[] bit array; a     */
-/* runtime filter's item is xxhash64(key) with seed 42 (Spark's            */
-/* InjectRuntimeFilter).  Parity of the layout is unpinned by the          */
-/* reference; the hashes are pinned by Spark's hash()/xxhash64() goldens.  */
+/* Spark's hash functions over int and bigint values: Murmur3 x86_32 as    */
+/* Spark runs it (crates/krishiv-kernels/src/hash/murmur3.rs:9-52) and     */
+/* Spark's xxhash64 (seed 42).  Pinned by Spark 4.2.0's hash()/xxhash64()   */
+/* goldens (tests/fixtures/spark-4.2.0/hash.json).                          */
 /* ------------------------------------------------------------------ */
 static uint32_t m3_rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
 
@@ -556,50 +550,6 @@ int64_t kgo_spark_xxhash64(int64_t v, int width) {
 
 void kgo_spark_xxhash64_batch(const int64_t *v, int64_t n, int width, int64_t *out) {
   for (int64_t i = 0; i < n; i++) out[i] = kgo_spark_xxhash64(v[i], width);
-}
-
-/* optimalNumOfBits: (long)(-n ln p / (ln 2)^2) */
-uint64_t kgo_spark_bloom_bits_for(uint64_t items, double fpp) {
-  double l2 = log(2.0);
-  return (uint64_t)(int64_t)(-(double)items * log(fpp) / (l2 * l2));
-}
-
-/* optimalNumOfHashFunctions: max(1, round(m / n * ln 2)) */
-int32_t kgo_spark_bloom_hashes_for(uint64_t items, uint64_t bits) {
-  double k = (double)bits / (double)(items ? items : 1) * log(2.0);
-  int32_t r = (int32_t)llround(k);
-  return r < 1 ? 1 : r;
-}
-
-/* words of the bit array: ceil(bits / 64); bitSize = words * 64 */
-uint64_t kgo_spark_bloom_words(uint64_t bits) { return (bits + 63) / 64; }
-
-static uint64_t spark_bit(int32_t h1, int32_t h2, int i, uint64_t bit_size) {
-  int32_t c = (int32_t)((uint32_t)h1 + (uint32_t)i * (uint32_t)h2);
-  if (c < 0) c = ~c;
-  return (uint64_t)c % bit_size;
-}
-
-void kgo_spark_bloom_put(uint64_t *words, uint64_t nwords, int32_t k, int64_t item) {
-  int32_t h1 = kgo_murmur3_long(item, 0), h2 = kgo_murmur3_long(item, h1);
-  for (int i = 1; i <= k; i++) {
-    uint64_t b = spark_bit(h1, h2, i, nwords * 64);
-    words[b >> 6] |= 1ull << (b & 63);
-  }
-}
-
-int kgo_spark_bloom_check(const uint64_t *words, uint64_t nwords, int32_t k, int64_t item) {
-  int32_t h1 = kgo_murmur3_long(item, 0), h2 = kgo_murmur3_long(item, h1);
-  for (int i = 1; i <= k; i++) {
-    uint64_t b = spark_bit(h1, h2, i, nwords * 64);
-    if (!((words[b >> 6] >> (b & 63)) & 1)) return 0;
-  }
-  return 1;
-}
-
-void kgo_spark_bloom_check_batch(const uint64_t *words, uint64_t nwords, int32_t k,
-                                 const int64_t *items, int64_t n, uint8_t *out) {
-  for (int64_t i = 0; i < n; i++) out[i] = (uint8_t)kgo_spark_bloom_check(words, nwords, k, items[i]);
 }
 
 /* ------------------------------------------------------------------ */
@@ -735,13 +685,22 @@ typedef struct {
 
 typedef struct {
   int64_t cnt;  /* COUNT, AVG count, SUM seen (as 0/1) */
-  i128 v;       /* int sum (low 64 bits wrap), decimal sum, min/max value */
-  double f;     /* float sum / float min/max */
+  union {
+    i128 v;     /* int sum (low 64 bits wrap), decimal sum, min/max value */
+    struct {
+      double f; /* float sum / float min/max */
+      double c; /* compensation of a float sum (exact_f64 mode only) */
+    };
+  };
   uint8_t st;   /* 0 none, 1 value, 2 overflowed (decimal) */
 } agg_state;
 
 struct kgo_agg {
   int has_key, key_type, naggs, ninputs, ansi;
+  /* float SUM/AVG as a compensated (TwoSum) sum: a reference value for
+   * checking sums whose order differs, not the reference's own row-order
+   * double adds */
+  int exact_f64;
   agg_desc *d;
   int64_t ngroups, cap_groups;
   int64_t *keys;
@@ -904,6 +863,18 @@ static int dec_add(i128 *sum, i128 x, i128 bound) {
   return 1;
 }
 
+/* A float sum's add: the plain double add (the reference), or with
+ * exact_f64 Knuth's TwoSum, whose rounding error collects in `c`. */
+static inline void fadd(const kgo_agg *a, agg_state *s, double x) {
+  if (!a->exact_f64) {
+    s->f += x;
+    return;
+  }
+  double t = s->f + x, bp = t - s->f;
+  s->c += (s->f - (t - bp)) + (x - bp);
+  s->f = t;
+}
+
 static int fold_row(kgo_agg *a, agg_state *s, const agg_desc *d, const kgo_col *in, int64_t i) {
   if (d->kind == K_COUNT) {
     if (d->func == KGO_AGG_COUNT_STAR || col_valid(&in[d->input], i)) s->cnt += 1;
@@ -925,7 +896,7 @@ static int fold_row(kgo_agg *a, agg_state *s, const agg_desc *d, const kgo_col *
     }
     case K_SUM_F:
       /* sum_f64 (dense.rs:289-345): row order, NULL skipped */
-      s->f += col_f64(c, i);
+      fadd(a, s, col_f64(c, i));
       s->cnt = 1;
       return 0;
     case K_SUM_DEC:
@@ -959,7 +930,7 @@ static int fold_row(kgo_agg *a, agg_state *s, const agg_desc *d, const kgo_col *
     }
     case K_AVG_F:
       /* AvgFloat (accumulator/fold.rs:401-424): double sum in row order */
-      s->f += col_f64(c, i);
+      fadd(a, s, col_f64(c, i));
       s->cnt += 1;
       return 0;
     case K_AVG_DEC:
@@ -972,20 +943,26 @@ static int fold_row(kgo_agg *a, agg_state *s, const agg_desc *d, const kgo_col *
   return 0;
 }
 
-int kgo_agg_update(kgo_agg *a, const kgo_col *key, const kgo_col *inputs, int64_t n) {
-  for (int64_t i = 0; i < n; i++) {
-    int64_t k = 0;
-    int kv = 1;
-    if (a->has_key) {
-      kv = col_valid(key, i);
-      k = kv ? (int64_t)col_int(key, i) : 0;
-    }
-    int64_t g = find_group(a, k, kv);
-    for (int j = 0; j < a->naggs; j++)
-      if (fold_row(a, &a->st[g * a->naggs + j], &a->d[j], inputs, i) != 0) return -1;
+static inline int update_row(kgo_agg *a, const kgo_col *key, const kgo_col *inputs, int64_t i) {
+  int64_t k = 0;
+  int kv = 1;
+  if (a->has_key) {
+    kv = col_valid(key, i);
+    k = kv ? (int64_t)col_int(key, i) : 0;
   }
+  int64_t g = find_group(a, k, kv);
+  for (int j = 0; j < a->naggs; j++)
+    if (fold_row(a, &a->st[g * a->naggs + j], &a->d[j], inputs, i) != 0) return -1;
   return 0;
 }
+
+int kgo_agg_update(kgo_agg *a, const kgo_col *key, const kgo_col *inputs, int64_t n) {
+  for (int64_t i = 0; i < n; i++)
+    if (update_row(a, key, inputs, i) != 0) return -1;
+  return 0;
+}
+
+void kgo_agg_set_exact_f64(kgo_agg *a, int on) { a->exact_f64 = on; }
 
 /* Merge partial states (the state-column folds of accumulator/fold.rs and
  * DecFold with `w = None`, helpers.rs:168-190,244-254). */
@@ -1009,7 +986,8 @@ int kgo_agg_merge(kgo_agg *a, const kgo_agg *o) {
           break;
         case K_SUM_F:
           if (t->cnt) {
-            s->f += t->f;
+            fadd(a, s, t->f);
+            s->c += t->c;
             s->cnt = 1;
           }
           break;
@@ -1046,7 +1024,10 @@ int kgo_agg_merge(kgo_agg *a, const kgo_agg *o) {
           }
           break;
         case K_AVG_F:
-          if (t->cnt) s->f += t->f;
+          if (t->cnt) {
+            fadd(a, s, t->f);
+            s->c += t->c;
+          }
           s->cnt += t->cnt;
           break;
         case K_AVG_DEC:
@@ -1154,7 +1135,7 @@ int kgo_agg_finish(const kgo_agg *a, int64_t *out_key, uint8_t *out_key_valid, k
           break;
         case K_SUM_F:
           valid = s->cnt != 0;
-          o->f64[g] = valid ? s->f : 0.0;
+          o->f64[g] = valid ? s->f + s->c : 0.0;
           break;
         case K_SUM_DEC:
           if (s->st == 0) valid = 0;
@@ -1180,7 +1161,7 @@ int kgo_agg_finish(const kgo_agg *a, int64_t *out_key, uint8_t *out_key_valid, k
           break;
         case K_AVG_F:
           valid = s->cnt != 0;
-          o->f64[g] = valid ? s->f / (double)s->cnt : 0.0;
+          o->f64[g] = valid ? (s->f + s->c) / (double)s->cnt : 0.0;
           break;
         case K_AVG_DEC: {
           if (s->cnt == 0) {
@@ -1210,6 +1191,206 @@ int kgo_agg_finish(const kgo_agg *a, int64_t *out_key, uint8_t *out_key_valid, k
     }
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Parallel hash aggregation, shaped as the reference runs it on a thread */
+/* pool: each driver thread folds its share of the rows into its own      */
+/* table, split into 16 radix partitions by the top 4 bits of the key's   */
+/* hash (aggregate/mod.rs:11-21, rowtable.rs:137-139,278-284); finalize    */
+/* merges partition p of every thread's table, in thread order, one task   */
+/* a partition on the pool (final_tasks, rowtable.rs:1805).  Integer and   */
+/* decimal results equal the single-threaded aggregate's; float sums       */
+/* depend on the merge order, as the reference's do (dense.rs:289-293).    */
+/* ------------------------------------------------------------------ */
+#include <pthread.h>
+
+#define KGO_PAGG_PARTS 16
+
+struct kgo_pagg {
+  int threads, has_key, naggs;
+  kgo_agg **tl; /* threads x KGO_PAGG_PARTS */
+  int64_t *off; /* KGO_PAGG_PARTS + 1 group offsets after finalize */
+  int finalized;
+  char err[256];
+  pthread_mutex_t mu;
+};
+
+static inline int pagg_part(const kgo_pagg *p, const kgo_col *key, int64_t i) {
+  if (!p->has_key || !col_valid(key, i)) return 0;
+  return (int)(key_slot_hash((int64_t)col_int(key, i)) >> 60);
+}
+
+typedef struct {
+  kgo_pagg *p;
+  int t;
+  const kgo_col *key, *inputs;
+  int64_t lo, hi;
+  int rc;
+} pagg_job;
+
+static void pagg_fail(kgo_pagg *p) {
+  pthread_mutex_lock(&p->mu);
+  if (!p->err[0]) snprintf(p->err, sizeof p->err, "%s", g_err);
+  pthread_mutex_unlock(&p->mu);
+}
+
+static void *pagg_update_job(void *arg) {
+  pagg_job *j = (pagg_job *)arg;
+  kgo_agg **tab = &j->p->tl[(size_t)j->t * KGO_PAGG_PARTS];
+  for (int64_t i = j->lo; i < j->hi; i++) {
+    if (update_row(tab[pagg_part(j->p, j->key, i)], j->key, j->inputs, i) != 0) {
+      j->rc = -1;
+      pagg_fail(j->p);
+      return NULL;
+    }
+  }
+  return NULL;
+}
+
+kgo_pagg *kgo_pagg_new(const kgo_col *key_type_like, int ninputs, const kgo_col *input_types,
+                       int naggs, const kgo_agg_spec *aggs, int ansi, int exact_f64,
+                       int threads) {
+  if (threads < 1) threads = 1;
+  kgo_pagg *p = (kgo_pagg *)calloc(1, sizeof(kgo_pagg));
+  p->threads = threads;
+  p->has_key = key_type_like != NULL;
+  p->naggs = naggs;
+  pthread_mutex_init(&p->mu, NULL);
+  p->tl = (kgo_agg **)calloc((size_t)threads * KGO_PAGG_PARTS, sizeof(kgo_agg *));
+  p->off = (int64_t *)calloc(KGO_PAGG_PARTS + 1, sizeof(int64_t));
+  for (int k = 0; k < threads * KGO_PAGG_PARTS; k++) {
+    p->tl[k] = kgo_agg_new(key_type_like, ninputs, input_types, naggs, aggs, ansi);
+    if (!p->tl[k]) {
+      kgo_pagg_free(p);
+      return NULL;
+    }
+    p->tl[k]->exact_f64 = exact_f64;
+  }
+  return p;
+}
+
+void kgo_pagg_free(kgo_pagg *p) {
+  if (!p) return;
+  for (int k = 0; k < p->threads * KGO_PAGG_PARTS; k++) kgo_agg_free(p->tl[k]);
+  free(p->tl);
+  free(p->off);
+  pthread_mutex_destroy(&p->mu);
+  free(p);
+}
+
+const char *kgo_pagg_error(const kgo_pagg *p) { return p->err; }
+
+int kgo_pagg_update(kgo_pagg *p, const kgo_col *key, const kgo_col *inputs, int64_t n) {
+  if (p->finalized) return set_err("INTERNAL_ERROR", "update after finalize");
+  int T = p->threads;
+  pthread_t *th = (pthread_t *)calloc((size_t)T, sizeof(pthread_t));
+  pagg_job *jobs = (pagg_job *)calloc((size_t)T, sizeof(pagg_job));
+  for (int t = 0; t < T; t++) {
+    jobs[t] = (pagg_job){p, t, key, inputs, n * t / T, n * (t + 1) / T, 0};
+    pthread_create(&th[t], NULL, pagg_update_job, &jobs[t]);
+  }
+  int rc = 0;
+  for (int t = 0; t < T; t++) {
+    pthread_join(th[t], NULL);
+    rc |= jobs[t].rc;
+  }
+  free(th);
+  free(jobs);
+  if (rc) snprintf(g_err, sizeof g_err, "%s", p->err);
+  return rc ? -1 : 0;
+}
+
+typedef struct {
+  kgo_pagg *p;
+  int part;
+  int64_t *out_key;
+  uint8_t *out_key_valid;
+  const kgo_out *outs;
+  int rc;
+} pagg_part_job;
+
+static void *pagg_merge_job(void *arg) {
+  pagg_part_job *j = (pagg_part_job *)arg;
+  kgo_pagg *p = j->p;
+  kgo_agg *dst = p->tl[j->part];
+  for (int t = 1; t < p->threads; t++) {
+    kgo_agg *src = p->tl[(size_t)t * KGO_PAGG_PARTS + j->part];
+    if (kgo_agg_merge(dst, src) != 0) {
+      j->rc = -1;
+      pagg_fail(p);
+      return NULL;
+    }
+    /* the thread's partition is merged: its memory goes back now */
+    kgo_agg *fresh = kgo_agg_new(NULL, 0, NULL, 0, NULL, 0);
+    kgo_agg_free(src);
+    p->tl[(size_t)t * KGO_PAGG_PARTS + j->part] = fresh;
+  }
+  return NULL;
+}
+
+static int run_parts(kgo_pagg *p, void *(*fn)(void *), pagg_part_job *jobs) {
+  pthread_t th[KGO_PAGG_PARTS];
+  int rc = 0;
+  /* at most `threads` partitions at once */
+  for (int a = 0; a < KGO_PAGG_PARTS; a += p->threads) {
+    int b = a + p->threads < KGO_PAGG_PARTS ? a + p->threads : KGO_PAGG_PARTS;
+    for (int q = a; q < b; q++) pthread_create(&th[q], NULL, fn, &jobs[q]);
+    for (int q = a; q < b; q++) {
+      pthread_join(th[q], NULL);
+      rc |= jobs[q].rc;
+    }
+  }
+  if (rc) snprintf(g_err, sizeof g_err, "%s", p->err);
+  return rc ? -1 : 0;
+}
+
+int kgo_pagg_finalize(kgo_pagg *p) {
+  if (p->finalized) return 0;
+  pagg_part_job jobs[KGO_PAGG_PARTS];
+  for (int q = 0; q < KGO_PAGG_PARTS; q++) jobs[q] = (pagg_part_job){p, q, NULL, NULL, NULL, 0};
+  if (run_parts(p, pagg_merge_job, jobs) != 0) return -1;
+  p->off[0] = 0;
+  for (int q = 0; q < KGO_PAGG_PARTS; q++) p->off[q + 1] = p->off[q] + p->tl[q]->ngroups;
+  p->finalized = 1;
+  return 0;
+}
+
+int64_t kgo_pagg_groups(const kgo_pagg *p) { return p->finalized ? p->off[KGO_PAGG_PARTS] : -1; }
+
+static void *pagg_finish_job(void *arg) {
+  pagg_part_job *j = (pagg_part_job *)arg;
+  kgo_pagg *p = j->p;
+  const int64_t o = p->off[j->part];
+  kgo_out outs[64];
+  int na = p->naggs;
+  for (int k = 0; k < na; k++) {
+    outs[k] = j->outs[k];
+    if (outs[k].i64) outs[k].i64 += o;
+    if (outs[k].hi) outs[k].hi += o;
+    if (outs[k].f64) outs[k].f64 += o;
+    if (outs[k].valid) outs[k].valid += o;
+  }
+  if (kgo_agg_finish(p->tl[j->part], j->out_key ? j->out_key + o : NULL,
+                     j->out_key_valid ? j->out_key_valid + o : NULL, outs) != 0) {
+    j->rc = -1;
+    pagg_fail(p);
+  }
+  return NULL;
+}
+
+int kgo_pagg_finish(kgo_pagg *p, int64_t *out_key, uint8_t *out_key_valid, kgo_out *outs) {
+  if (!p->finalized) return set_err("INTERNAL_ERROR", "finish before finalize");
+  if (p->naggs > 64) return set_err("KRISHIV_UNSUPPORTED_FEATURE", "more than 64 aggregates");
+  pagg_part_job jobs[KGO_PAGG_PARTS];
+  for (int q = 0; q < KGO_PAGG_PARTS; q++)
+    jobs[q] = (pagg_part_job){p, q, out_key, out_key_valid, outs, 0};
+  return run_parts(p, pagg_finish_job, jobs);
+}
+
+void kgo_pagg_result_type(const kgo_pagg *p, int i, int32_t *type, int32_t *precision,
+                          int32_t *scale) {
+  kgo_agg_result_type(p->tl[0], i, type, precision, scale);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1266,76 +1447,248 @@ static uint64_t row_key_hash(int nkeys, const kgo_col *keys, int64_t i) {
   return h;
 }
 
-/* Pair emission (join/probe.rs:266-337,395-501): inner pairs; left outer adds
- * each unmatched probe row with build -1; semi/anti emit a probe row once. */
-int64_t kgo_join(int join_type, int nkeys, const kgo_col *probe_keys, const kgo_col *build_keys,
-                 const uint8_t *null_safe, int64_t n_probe, int64_t n_build, int nested,
-                 int64_t cap, int64_t *out_probe, int64_t *out_build) {
+/* The residual filter (join/mod.rs:50-51, probe.rs:686+) as a postfix
+ * program over the candidate pair's columns (probe columns, then build
+ * columns), Spark's three-valued logic: a pair passes when the program
+ * yields TRUE; NULL and FALSE do not pass.  Integer-like columns are read as
+ * int64; + - * wrap at 64 bits (non-ANSI); % takes the dividend's sign and
+ * is NULL for a zero divisor. */
+typedef struct {
+  int64_t v;
+  int null;
+} xval;
+
+static int expr_pass(const kgo_expr *x, int nx, const kgo_col *pc, const kgo_col *bc, int64_t i,
+                     int64_t j) {
+  xval st[32];
+  int sp = 0;
+  for (int k = 0; k < nx; k++) {
+    const kgo_expr *e = &x[k];
+    if (e->op == KGO_X_PROBE || e->op == KGO_X_BUILD) {
+      const kgo_col *c = e->op == KGO_X_PROBE ? &pc[e->arg] : &bc[e->arg];
+      int64_t r = e->op == KGO_X_PROBE ? i : j;
+      int ok = col_valid(c, r);
+      st[sp++] = (xval){ok ? (int64_t)col_int(c, r) : 0, !ok};
+      continue;
+    }
+    if (e->op == KGO_X_CONST) {
+      st[sp++] = (xval){e->imm, 0};
+      continue;
+    }
+    if (e->op == KGO_X_NOT) {
+      st[sp - 1].v = !st[sp - 1].v;
+      continue;
+    }
+    if (e->op == KGO_X_IS_NULL) {
+      st[sp - 1] = (xval){st[sp - 1].null, 0};
+      continue;
+    }
+    xval b = st[--sp], a = st[sp - 1], r = {0, 0};
+    uint64_t ua = (uint64_t)a.v, ub = (uint64_t)b.v;
+    switch (e->op) {
+      case KGO_X_AND: /* Kleene: FALSE wins over NULL */
+        if ((!a.null && !a.v) || (!b.null && !b.v)) r = (xval){0, 0};
+        else r = (xval){1, a.null || b.null};
+        break;
+      case KGO_X_OR: /* TRUE wins over NULL */
+        if ((!a.null && a.v) || (!b.null && b.v)) r = (xval){1, 0};
+        else r = (xval){0, a.null || b.null};
+        break;
+      default:
+        if (a.null || b.null) {
+          r = (xval){0, 1};
+          break;
+        }
+        switch (e->op) {
+          case KGO_X_ADD: r.v = (int64_t)(ua + ub); break;
+          case KGO_X_SUB: r.v = (int64_t)(ua - ub); break;
+          case KGO_X_MUL: r.v = (int64_t)(ua * ub); break;
+          case KGO_X_MOD:
+            if (b.v == 0) r.null = 1;
+            else if (b.v == -1) r.v = 0;
+            else r.v = a.v % b.v;
+            break;
+          case KGO_X_EQ: r.v = a.v == b.v; break;
+          case KGO_X_NE: r.v = a.v != b.v; break;
+          case KGO_X_LT: r.v = a.v < b.v; break;
+          case KGO_X_LE: r.v = a.v <= b.v; break;
+          case KGO_X_GT: r.v = a.v > b.v; break;
+          case KGO_X_GE: r.v = a.v >= b.v; break;
+        }
+    }
+    st[sp - 1] = r;
+  }
+  return sp == 1 && !st[0].null && st[0].v != 0;
+}
+
+/* Spark's join types (join/mod.rs:1-60,217-249): pairs and tails as the
+ * probe (probe.rs:266-501,532-599) and the build-side tail (emit_build,
+ * probe.rs:611-632) return them.  Output (probe row, build row): -1 pads an
+ * unmatched side; semi/anti/null-aware anti return (probe row, -1); right
+ * semi/anti (-1, build row); existence (probe row, 1 or 0 = exists).  The
+ * probe rows come first, in order, then the build-side tail in build order. */
+int64_t kgo_join_ex(int join_type, int nkeys, const kgo_col *probe_keys, const kgo_col *build_keys,
+                    const uint8_t *null_safe, int64_t n_probe, int64_t n_build, int nested,
+                    int nfilter, const kgo_expr *filter, const kgo_col *probe_cols,
+                    const kgo_col *build_cols, int64_t cap, int64_t *out_probe,
+                    int64_t *out_build) {
   int64_t nout = 0;
   int64_t nslots = 16, *heads = NULL, *next = NULL;
-  uint64_t *bh = NULL;
+  uint8_t *bmatch = (uint8_t *)calloc((size_t)(n_build ? n_build : 1), 1);
+  int build_has_null_key = 0;
   if (!nested) {
     while (nslots < 2 * n_build) nslots *= 2;
     heads = (int64_t *)malloc(sizeof(int64_t) * (size_t)nslots);
     for (int64_t s = 0; s < nslots; s++) heads[s] = -1;
     next = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n_build ? n_build : 1));
-    bh = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n_build ? n_build : 1));
-    for (int64_t j = 0; j < n_build; j++) {
-      /* a build row whose `=` key is NULL matches nothing: not indexed */
-      int drop = 0;
-      for (int k = 0; k < nkeys; k++)
-        if (!col_valid(&build_keys[k], j) && !(null_safe && null_safe[k])) drop = 1;
-      next[j] = -1;
-      if (drop) continue;
-      bh[j] = row_key_hash(nkeys, build_keys, j);
-      int64_t s = (int64_t)(bh[j] & (uint64_t)(nslots - 1));
-      next[j] = heads[s];
-      heads[s] = j;
-    }
   }
+  for (int64_t j = 0; j < n_build; j++) {
+    /* a build row whose `=` key is NULL matches nothing: not indexed */
+    int drop = 0;
+    for (int k = 0; k < nkeys; k++)
+      if (!col_valid(&build_keys[k], j)) {
+        if (!(null_safe && null_safe[k])) drop = 1;
+        build_has_null_key = 1;
+      }
+    if (nested) continue;
+    next[j] = -1;
+    if (drop) continue;
+    int64_t s = (int64_t)(row_key_hash(nkeys, build_keys, j) & (uint64_t)(nslots - 1));
+    next[j] = heads[s];
+    heads[s] = j;
+  }
+  const int pairs = join_type == KGO_JOIN_INNER || join_type == KGO_JOIN_LEFT_OUTER ||
+                    join_type == KGO_JOIN_RIGHT_OUTER || join_type == KGO_JOIN_FULL_OUTER;
+  const int marks_build = join_type >= KGO_JOIN_RIGHT_SEMI;
+#define EMIT(a, b)              \
+  do {                          \
+    if (nout >= cap) goto full; \
+    out_probe[nout] = (a);      \
+    out_build[nout++] = (b);    \
+  } while (0)
   for (int64_t i = 0; i < n_probe; i++) {
     int64_t matches = 0;
-    if (nested) {
-      for (int64_t j = 0; j < n_build; j++) {
-        if (!keys_match(nkeys, probe_keys, build_keys, null_safe, i, j)) continue;
+    int64_t j = nested ? 0 : heads[row_key_hash(nkeys, probe_keys, i) & (uint64_t)(nslots - 1)];
+    while (nested ? j < n_build : j >= 0) {
+      if (keys_match(nkeys, probe_keys, build_keys, null_safe, i, j) &&
+          (nfilter == 0 || expr_pass(filter, nfilter, probe_cols, build_cols, i, j))) {
         matches++;
-        if (join_type == KGO_JOIN_INNER || join_type == KGO_JOIN_LEFT_OUTER) {
-          if (nout >= cap) return -1;
-          out_probe[nout] = i;
-          out_build[nout++] = j;
-        } else {
-          break;
-        }
+        if (marks_build) bmatch[j] = 1;
+        if (pairs) EMIT(i, j);
+        else if (!marks_build) break; /* semi/anti/existence: one match is enough */
       }
-    } else {
-      uint64_t h = row_key_hash(nkeys, probe_keys, i);
-      for (int64_t j = heads[h & (uint64_t)(nslots - 1)]; j >= 0; j = next[j]) {
-        if (!keys_match(nkeys, probe_keys, build_keys, null_safe, i, j)) continue;
-        matches++;
-        if (join_type == KGO_JOIN_INNER || join_type == KGO_JOIN_LEFT_OUTER) {
-          if (nout >= cap) goto full;
-          out_probe[nout] = i;
-          out_build[nout++] = j;
-        } else {
-          break;
-        }
-      }
+      j = nested ? j + 1 : next[j];
     }
-    if ((join_type == KGO_JOIN_LEFT_OUTER && matches == 0) ||
-        (join_type == KGO_JOIN_LEFT_SEMI && matches > 0) ||
-        (join_type == KGO_JOIN_LEFT_ANTI && matches == 0)) {
-      if (nout >= cap) goto full;
-      out_probe[nout] = i;
-      out_build[nout++] = -1;
+    int pkey_null = 0;
+    for (int k = 0; k < nkeys; k++) pkey_null |= !col_valid(&probe_keys[k], i);
+    switch (join_type) {
+      case KGO_JOIN_LEFT_OUTER:
+      case KGO_JOIN_FULL_OUTER:
+        if (!matches) EMIT(i, -1);
+        break;
+      case KGO_JOIN_LEFT_SEMI:
+        if (matches) EMIT(i, -1);
+        break;
+      case KGO_JOIN_LEFT_ANTI:
+        if (!matches) EMIT(i, -1);
+        break;
+      case KGO_JOIN_NULL_AWARE_ANTI:
+        /* NOT IN: an empty build keeps every row; a NULL build key keeps
+         * none; otherwise a non-NULL key that matches nothing */
+        if (n_build == 0 || (!build_has_null_key && !pkey_null && !matches)) EMIT(i, -1);
+        break;
+      case KGO_JOIN_EXISTENCE:
+        EMIT(i, matches ? 1 : 0);
+        break;
+      default:
+        break;
     }
   }
+  if (marks_build) {
+    for (int64_t j = 0; j < n_build; j++) {
+      if (join_type == KGO_JOIN_RIGHT_SEMI ? bmatch[j] : !bmatch[j]) EMIT(-1, j);
+    }
+  }
+#undef EMIT
   free(heads);
   free(next);
-  free(bh);
+  free(bmatch);
   return nout;
 full:
   free(heads);
   free(next);
-  free(bh);
+  free(bmatch);
   return -1;
+}
+
+int64_t kgo_join(int join_type, int nkeys, const kgo_col *probe_keys, const kgo_col *build_keys,
+                 const uint8_t *null_safe, int64_t n_probe, int64_t n_build, int nested,
+                 int64_t cap, int64_t *out_probe, int64_t *out_build) {
+  return kgo_join_ex(join_type, nkeys, probe_keys, build_keys, null_safe, n_probe, n_build, nested,
+                     0, NULL, NULL, NULL, cap, out_probe, out_build);
+}
+
+/* ------------------------------------------------------------------ */
+/* Grace join fused with the per-partition aggregate (configuration 4's  */
+/* plan): SELECT p, COUNT(*), SUM(probe.pay * build.pay) ... GROUP BY p,  */
+/* p = partition_of(H_kg(key), parts) (routing.rs:222-231), inner join on */
+/* `=` int64 keys (NULL keys never match).  Per key the pairs' count is   */
+/* nb * np and their payload-product sum Sb * Sp (pairs whose payloads    */
+/* are both non-NULL), so no pair is walked.  mode 0: the 64-bit wrapping */
+/* sum; mode 1: that sum mod 2^31.  Pinned by the pair walk (kgo_join)    */
+/* on small inputs.                                                       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int64_t key;
+  int64_t nb, nbv; /* build rows, build rows with a payload */
+  uint64_t sb;     /* wrapping sum of their payloads */
+  int used;
+} gkey;
+
+int kgo_grace_factorized(const kgo_col *bkey, const kgo_col *bpay, int64_t nb, const kgo_col *pkey,
+                         const kgo_col *ppay, int64_t np, uint32_t parts, int mode,
+                         int64_t *counts, int64_t *sums, uint8_t *sum_valid) {
+  int64_t cap = 16;
+  while (cap < 2 * nb) cap *= 2;
+  gkey *t = (gkey *)calloc((size_t)cap, sizeof(gkey));
+  if (!t) return set_err("UNABLE_TO_ACQUIRE_MEMORY", "grace oracle table");
+  for (int64_t i = 0; i < nb; i++) {
+    if (!col_valid(bkey, i)) continue;
+    int64_t k = (int64_t)col_int(bkey, i);
+    uint64_t h = key_slot_hash(k) & (uint64_t)(cap - 1);
+    while (t[h].used && t[h].key != k) h = (h + 1) & (uint64_t)(cap - 1);
+    t[h].used = 1;
+    t[h].key = k;
+    t[h].nb++;
+    if (col_valid(bpay, i)) {
+      t[h].nbv++;
+      t[h].sb += (uint64_t)(int64_t)col_int(bpay, i);
+    }
+  }
+  uint64_t *s = (uint64_t *)calloc(parts, sizeof(uint64_t));
+  int64_t *vc = (int64_t *)calloc(parts, sizeof(int64_t));
+  for (uint32_t q = 0; q < parts; q++) counts[q] = 0;
+  for (int64_t i = 0; i < np; i++) {
+    if (!col_valid(pkey, i)) continue;
+    int64_t k = (int64_t)col_int(pkey, i);
+    uint64_t h = key_slot_hash(k) & (uint64_t)(cap - 1);
+    while (t[h].used && t[h].key != k) h = (h + 1) & (uint64_t)(cap - 1);
+    if (!t[h].used) continue;
+    uint8_t b[8];
+    for (int q = 0; q < 8; q++) b[q] = (uint8_t)((uint64_t)k >> (8 * q));
+    uint32_t p = kgo_partition_of(kgo_xxh64(b, 8, 0), parts);
+    counts[p] += t[h].nb;
+    if (col_valid(ppay, i)) {
+      s[p] += (uint64_t)(int64_t)col_int(ppay, i) * t[h].sb;
+      vc[p] += t[h].nbv;
+    }
+  }
+  for (uint32_t q = 0; q < parts; q++) {
+    sums[q] = (int64_t)(mode == 1 ? (s[q] & 0x7FFFFFFFull) : s[q]);
+    sum_valid[q] = vc[q] > 0;
+  }
+  free(t);
+  free(s);
+  free(vc);
+  return 0;
 }

@@ -180,6 +180,29 @@ void kgo_agg_result_type(const kgo_agg *a, int i, int32_t *type, int32_t *precis
                          int32_t *scale);
 void kgo_agg_free(kgo_agg *a);
 const char *kgo_last_error(void);
+/* Float SUM/AVG as a compensated (TwoSum) sum: a fsum-grade reference value
+ * for checking float sums whose addition order differs (0: the reference's
+ * row-order double adds, the default). */
+void kgo_agg_set_exact_f64(kgo_agg *a, int on);
+
+/* The aggregate on `threads` threads, shaped as the reference's thread pool
+ * runs it: thread-local tables in 16 radix partitions (aggregate/mod.rs:11-21,
+ * rowtable.rs:137-139,278-284), then a parallel merge per partition
+ * (final_tasks, rowtable.rs:1805).  Integer and decimal results equal
+ * kgo_agg's; float sums differ by their order (or not at all with
+ * exact_f64).  finish writes the groups partition by partition. */
+typedef struct kgo_pagg kgo_pagg;
+kgo_pagg *kgo_pagg_new(const kgo_col *key_type_like, int ninputs, const kgo_col *input_types,
+                       int naggs, const kgo_agg_spec *aggs, int ansi, int exact_f64,
+                       int threads);
+int kgo_pagg_update(kgo_pagg *p, const kgo_col *key, const kgo_col *inputs, int64_t n);
+int kgo_pagg_finalize(kgo_pagg *p);
+int64_t kgo_pagg_groups(const kgo_pagg *p);
+int kgo_pagg_finish(kgo_pagg *p, int64_t *out_key, uint8_t *out_key_valid, kgo_out *outs);
+void kgo_pagg_result_type(const kgo_pagg *p, int i, int32_t *type, int32_t *precision,
+                          int32_t *scale);
+const char *kgo_pagg_error(const kgo_pagg *p);
+void kgo_pagg_free(kgo_pagg *p);
 
 /* decimal avg by one HALF_UP division (accumulator/helpers.rs:595-609).
  * Returns 1 and *out on success, 0 when the shortcut does not apply or the
@@ -187,35 +210,84 @@ const char *kgo_last_error(void);
 int kgo_avg_fast(int64_t sum_lo, int64_t sum_hi, int sum_scale, int64_t count, int p, int s,
                  int64_t *out_lo, int64_t *out_hi);
 
-/* Spark-layout Bloom filter (optional row; layout parity unpinned by the
- * reference, which holds only Murmur3 x86_32, murmur3.rs:9-52). */
+/* Spark's hash() (Murmur3 x86_32, crates/krishiv-kernels/src/hash/
+ * murmur3.rs:9-52) and xxhash64() (seed 42) of int and bigint values. */
 int32_t kgo_murmur3_int(int32_t v, int32_t seed);
 int32_t kgo_murmur3_long(int64_t v, int32_t seed);
 int64_t kgo_spark_xxhash64(int64_t v, int width);
 void kgo_spark_xxhash64_batch(const int64_t *v, int64_t n, int width, int64_t *out);
-uint64_t kgo_spark_bloom_bits_for(uint64_t items, double fpp);
-int32_t kgo_spark_bloom_hashes_for(uint64_t items, uint64_t bits);
-uint64_t kgo_spark_bloom_words(uint64_t bits);
-void kgo_spark_bloom_put(uint64_t *words, uint64_t nwords, int32_t k, int64_t item);
-int kgo_spark_bloom_check(const uint64_t *words, uint64_t nwords, int32_t k, int64_t item);
-void kgo_spark_bloom_check_batch(const uint64_t *words, uint64_t nwords, int32_t k,
-                                 const int64_t *items, int64_t n, uint8_t *out);
 
 /* ---------------------------------------------------------------------
- * Hash join (crates/krishiv-exec/src/join/mod.rs:21-60,217-293;
- * join/probe.rs:266-501).  Inner / left semi / left anti / left outer on
- * `=` (NULL never matches) or `<=>` keys over integer columns.
- * Pinned by: a nested-loop join on small inputs (the reference's own oracle,
- * crates/krishiv-exec/tests/hash_join/main.rs:130-195).
+ * Hash join (crates/krishiv-exec/src/join/mod.rs:1-60,217-293;
+ * join/probe.rs:266-501,532-632).  Spark's ten join types on `=` (NULL never
+ * matches) or `<=>` keys over integer-like or float columns, with an
+ * optional residual filter.
+ * Pinned by: a nested-loop join on small inputs, and a pure-Python
+ * restatement of the reference's own oracle and its edge cases
+ * (crates/krishiv-exec/tests/hash_join/main.rs:130-290, VLess filter).
  * --------------------------------------------------------------------- */
-enum { KGO_JOIN_INNER = 0, KGO_JOIN_LEFT_OUTER = 1, KGO_JOIN_LEFT_SEMI = 2, KGO_JOIN_LEFT_ANTI = 3 };
-/* Emits (probe row, build row) pairs; build row -1 for an unmatched probe
- * row of a left outer join; semi/anti emit probe rows only.  Returns the
- * pair count, or -1 if `cap` is too small.  `nested` selects the
- * nested-loop form (the reference test oracle) instead of the hash form. */
+enum {
+  KGO_JOIN_INNER = 0,
+  KGO_JOIN_LEFT_OUTER = 1,
+  KGO_JOIN_LEFT_SEMI = 2,
+  KGO_JOIN_LEFT_ANTI = 3,
+  KGO_JOIN_NULL_AWARE_ANTI = 4,
+  KGO_JOIN_EXISTENCE = 5,
+  KGO_JOIN_RIGHT_SEMI = 6,
+  KGO_JOIN_RIGHT_ANTI = 7,
+  KGO_JOIN_RIGHT_OUTER = 8,
+  KGO_JOIN_FULL_OUTER = 9
+};
+/* Residual filter program: postfix ops over (probe, build) columns. */
+enum {
+  KGO_X_PROBE = 1, /* push probe column `arg` */
+  KGO_X_BUILD = 2, /* push build column `arg` */
+  KGO_X_CONST = 3, /* push `imm` */
+  KGO_X_ADD = 4,
+  KGO_X_SUB = 5,
+  KGO_X_MUL = 6,
+  KGO_X_MOD = 7,
+  KGO_X_EQ = 8,
+  KGO_X_NE = 9,
+  KGO_X_LT = 10,
+  KGO_X_LE = 11,
+  KGO_X_GT = 12,
+  KGO_X_GE = 13,
+  KGO_X_AND = 14,
+  KGO_X_OR = 15,
+  KGO_X_NOT = 16,
+  KGO_X_IS_NULL = 17
+};
+typedef struct kgo_expr {
+  int32_t op;
+  int32_t arg;
+  int64_t imm;
+} kgo_expr;
+/* Emits (probe row, build row) pairs: -1 pads the unmatched side; semi,
+ * anti and null-aware anti emit (probe row, -1); right semi/anti (-1, build
+ * row); existence (probe row, exists as 1/0).  Probe rows first, then the
+ * build-side tail.  Returns the count, or -1 if `cap` is too small.
+ * `nested` selects the nested-loop form (the reference test oracle). */
+int64_t kgo_join_ex(int join_type, int nkeys, const kgo_col *probe_keys, const kgo_col *build_keys,
+                    const uint8_t *null_safe, int64_t n_probe, int64_t n_build, int nested,
+                    int nfilter, const kgo_expr *filter, const kgo_col *probe_cols,
+                    const kgo_col *build_cols, int64_t cap, int64_t *out_probe,
+                    int64_t *out_build);
+/* kgo_join_ex without a residual filter. */
 int64_t kgo_join(int join_type, int nkeys, const kgo_col *probe_keys, const kgo_col *build_keys,
                  const uint8_t *null_safe, int64_t n_probe, int64_t n_build, int nested,
                  int64_t cap, int64_t *out_probe, int64_t *out_build);
+
+/* ---------------------------------------------------------------------
+ * Configuration 4's plan through the grace join: per output partition
+ * p = partition_of(H_kg(key), parts), the join pairs' COUNT(*) and the sum
+ * of probe.pay * build.pay, factorized per key (nb * np, Sb * Sp).  mode 0:
+ * the wrapping 64-bit sum; mode 1: that sum mod 2^31.  sum_valid: some pair
+ * had both payloads.  Pinned by the pair walk of kgo_join on small inputs.
+ * --------------------------------------------------------------------- */
+int kgo_grace_factorized(const kgo_col *bkey, const kgo_col *bpay, int64_t nb, const kgo_col *pkey,
+                         const kgo_col *ppay, int64_t np, uint32_t parts, int mode,
+                         int64_t *counts, int64_t *sums, uint8_t *sum_valid);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,11 @@ T_FLOAT32, T_FLOAT64, T_DATE32, T_TIMESTAMP, T_DECIMAL64, T_DECIMAL128 = 10, 11,
 
 AGG_COUNT_STAR, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_AVG = 1, 2, 3, 4, 5, 6
 JOIN_INNER, JOIN_LEFT_OUTER, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI = 0, 1, 2, 3
+JOIN_NULL_AWARE_ANTI, JOIN_EXISTENCE, JOIN_RIGHT_SEMI, JOIN_RIGHT_ANTI = 4, 5, 6, 7
+JOIN_RIGHT_OUTER, JOIN_FULL_OUTER = 8, 9
+# residual filter program ops (postfix)
+(X_PROBE, X_BUILD, X_CONST, X_ADD, X_SUB, X_MUL, X_MOD, X_EQ, X_NE, X_LT, X_LE, X_GT, X_GE,
+ X_AND, X_OR, X_NOT, X_IS_NULL) = range(1, 18)
 
 _NP_OF = {
     T_INT8: np.int8, T_INT16: np.int16, T_INT32: np.int32, T_INT64: np.int64,
@@ -104,18 +109,6 @@ def lib():
         L.kgo_spark_xxhash64.argtypes = [i64, C.c_int]
         L.kgo_spark_xxhash64_batch.restype = None
         L.kgo_spark_xxhash64_batch.argtypes = [vp, i64, C.c_int, vp]
-        L.kgo_spark_bloom_bits_for.restype = u64
-        L.kgo_spark_bloom_bits_for.argtypes = [u64, C.c_double]
-        L.kgo_spark_bloom_hashes_for.restype = C.c_int32
-        L.kgo_spark_bloom_hashes_for.argtypes = [u64, u64]
-        L.kgo_spark_bloom_words.restype = u64
-        L.kgo_spark_bloom_words.argtypes = [u64]
-        L.kgo_spark_bloom_put.restype = None
-        L.kgo_spark_bloom_put.argtypes = [vp, u64, C.c_int32, i64]
-        L.kgo_spark_bloom_check.restype = C.c_int
-        L.kgo_spark_bloom_check.argtypes = [vp, u64, C.c_int32, i64]
-        L.kgo_spark_bloom_check_batch.restype = None
-        L.kgo_spark_bloom_check_batch.argtypes = [vp, u64, C.c_int32, vp, i64, vp]
         L.kgo_sbbf_check_batch.restype = None
         L.kgo_sbbf_check_batch.argtypes = [vp, C.c_size_t, vp, C.c_int64, vp]
         L.kgo_sbbf_expected_fpp.restype = C.c_double
@@ -156,6 +149,24 @@ def lib():
         L.kgo_avg_fast.argtypes = [i64, i64, C.c_int, i64, C.c_int, C.c_int, vp, vp]
         L.kgo_join.restype = i64
         L.kgo_join.argtypes = [C.c_int, C.c_int, vp, vp, vp, i64, i64, C.c_int, i64, vp, vp]
+        L.kgo_join_ex.restype = i64
+        L.kgo_join_ex.argtypes = [C.c_int, C.c_int, vp, vp, vp, i64, i64, C.c_int, C.c_int, vp,
+                                  vp, vp, i64, vp, vp]
+        L.kgo_agg_set_exact_f64.argtypes = [vp, C.c_int]
+        L.kgo_pagg_new.restype = vp
+        L.kgo_pagg_new.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int]
+        L.kgo_pagg_update.restype = C.c_int
+        L.kgo_pagg_update.argtypes = [vp, vp, vp, i64]
+        L.kgo_pagg_finalize.restype = C.c_int
+        L.kgo_pagg_finalize.argtypes = [vp]
+        L.kgo_pagg_groups.restype = i64
+        L.kgo_pagg_groups.argtypes = [vp]
+        L.kgo_pagg_finish.restype = C.c_int
+        L.kgo_pagg_finish.argtypes = [vp, vp, vp, vp]
+        L.kgo_pagg_result_type.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.kgo_pagg_free.argtypes = [vp]
+        L.kgo_grace_factorized.restype = C.c_int
+        L.kgo_grace_factorized.argtypes = [vp, vp, i64, vp, vp, i64, u32, C.c_int, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -289,23 +300,6 @@ def sbbf_check_batch(words: np.ndarray, hashes: np.ndarray) -> np.ndarray:
     return out.astype(bool)
 
 
-def spark_bloom_build(items: np.ndarray, nbits: int, k: int) -> np.ndarray:
-    """Spark BloomFilterImpl bit array (uint64 words) after putLong of each item."""
-    L = lib()
-    words = np.zeros(L.kgo_spark_bloom_words(nbits), dtype=np.uint64)
-    for it in np.asarray(items, dtype=np.int64):
-        L.kgo_spark_bloom_put(_ptr(words), len(words), k, int(it))
-    return words
-
-
-def spark_bloom_check_batch(words: np.ndarray, k: int, items: np.ndarray) -> np.ndarray:
-    words = np.ascontiguousarray(words, dtype=np.uint64)
-    items = np.ascontiguousarray(items, dtype=np.int64)
-    out = np.zeros(len(items), dtype=np.uint8)
-    lib().kgo_spark_bloom_check_batch(_ptr(words), len(words), k, _ptr(items), len(items), _ptr(out))
-    return out.astype(bool)
-
-
 def spark_xxhash64(values: np.ndarray, width: int) -> np.ndarray:
     """Spark's xxhash64 (seed 42) of int (width 4) or bigint (width 8) values."""
     v = np.ascontiguousarray(np.asarray(values).astype(np.int64))
@@ -353,7 +347,7 @@ class Agg:
     """The oracle's hash aggregate (Sink::sink = update, combine = merge,
     finalize = finish)."""
 
-    def __init__(self, key_type, input_types, aggs, ansi=False):
+    def __init__(self, key_type, input_types, aggs, ansi=False, exact_f64=False):
         self.key_type = key_type
         kc = _Col(key_type[0], key_type[1], key_type[2], 0, 0, 0, None, None) if key_type else None
         self._kc = kc
@@ -368,6 +362,8 @@ class Agg:
                                    C.addressof(its), len(aggs), C.addressof(specs), int(ansi))
         if not self.h:
             raise RuntimeError(error())
+        if exact_f64:
+            lib().kgo_agg_set_exact_f64(self.h, 1)
 
     def update(self, key: Col | None, inputs):
         n = key.length if key is not None else (inputs[0].length if inputs else 0)
@@ -383,37 +379,13 @@ class Agg:
         if lib().kgo_agg_merge(self.h, other.h) != 0:
             raise ArithmeticError(error())
 
-    def finish(self):
+    def finish(self, raw=False):
         """Returns (keys, key_valid, [(values, valid, (type, p, s))...]).
-        Decimal results are Python ints."""
-        L = lib()
-        g = L.kgo_agg_groups(self.h)
-        keys = np.zeros(max(g, 1), dtype=np.int64)
-        kv = np.zeros(max(g, 1), dtype=np.uint8)
-        outs = (_Out * max(1, self.naggs))()
-        bufs = []
-        for i in range(self.naggs):
-            t, p, s = C.c_int32(), C.c_int32(), C.c_int32()
-            L.kgo_agg_result_type(self.h, i, C.byref(t), C.byref(p), C.byref(s))
-            lo = np.zeros(max(g, 1), dtype=np.int64)
-            hi = np.zeros(max(g, 1), dtype=np.int64)
-            f = np.zeros(max(g, 1), dtype=np.float64)
-            v = np.zeros(max(g, 1), dtype=np.uint8)
-            outs[i] = _Out(t.value, p.value, s.value, 0, _ptr(lo), _ptr(hi), _ptr(f), _ptr(v))
-            bufs.append((lo, hi, f, v, (t.value, p.value, s.value)))
-        if L.kgo_agg_finish(self.h, _ptr(keys), _ptr(kv), C.addressof(outs)) != 0:
-            raise ArithmeticError(error())
-        res = []
-        for lo, hi, f, v, (t, p, s) in bufs:
-            if t in (T_FLOAT64, T_FLOAT32):  # float32 MIN/MAX come back widened
-                vals = f[:g].copy()
-            elif t in (T_DECIMAL64, T_DECIMAL128):
-                vals = [((int(h) << 64) | (int(l) & 0xFFFFFFFFFFFFFFFF)) for l, h in zip(lo[:g], hi[:g])]
-                vals = [x - (1 << 128) if x >= (1 << 127) else x for x in vals]
-            else:
-                vals = lo[:g].copy()
-            res.append((vals, v[:g].astype(bool), (t, p, s)))
-        return keys[:g].copy(), kv[:g].astype(bool), res
+        Decimal results are Python ints, or with raw=True an (n, 2) int64
+        array of (low, high) words."""
+        return _finish(lib().kgo_agg_groups(self.h), self.naggs,
+                       lambda i, t, p, s: lib().kgo_agg_result_type(self.h, i, t, p, s),
+                       lambda k, kv, outs: lib().kgo_agg_finish(self.h, k, kv, outs), raw)
 
     def __del__(self):
         try:
@@ -422,6 +394,119 @@ class Agg:
                 self.h = None
         except Exception:
             pass
+
+
+def _finish(g, naggs, result_type, finish, raw):
+    """Shared by Agg and AggPar: allocate the output columns, finish, decode."""
+    if True:
+        L = lib()
+        keys = np.zeros(max(g, 1), dtype=np.int64)
+        kv = np.zeros(max(g, 1), dtype=np.uint8)
+        outs = (_Out * max(1, naggs))()
+        bufs = []
+        for i in range(naggs):
+            t, p, s = C.c_int32(), C.c_int32(), C.c_int32()
+            result_type(i, C.byref(t), C.byref(p), C.byref(s))
+            lo = np.zeros(max(g, 1), dtype=np.int64)
+            hi = np.zeros(max(g, 1), dtype=np.int64)
+            f = np.zeros(max(g, 1), dtype=np.float64)
+            v = np.zeros(max(g, 1), dtype=np.uint8)
+            outs[i] = _Out(t.value, p.value, s.value, 0, _ptr(lo), _ptr(hi), _ptr(f), _ptr(v))
+            bufs.append((lo, hi, f, v, (t.value, p.value, s.value)))
+        if finish(_ptr(keys), _ptr(kv), C.addressof(outs)) != 0:
+            raise ArithmeticError(error())
+        res = []
+        for lo, hi, f, v, (t, p, s) in bufs:
+            if t in (T_FLOAT64, T_FLOAT32):  # float32 MIN/MAX come back widened
+                vals = f[:g].copy()
+            elif t == T_DECIMAL64 and raw:
+                vals = lo[:g].copy()
+            elif t == T_DECIMAL128 and raw:
+                vals = np.stack([lo[:g], hi[:g]], axis=1)
+            elif t in (T_DECIMAL64, T_DECIMAL128):
+                vals = [((int(h) << 64) | (int(l) & 0xFFFFFFFFFFFFFFFF)) for l, h in zip(lo[:g], hi[:g])]
+                vals = [x - (1 << 128) if x >= (1 << 127) else x for x in vals]
+            else:
+                vals = lo[:g].copy()
+            res.append((vals, v[:g].astype(bool), (t, p, s)))
+        return keys[:g].copy(), kv[:g].astype(bool), res
+
+
+class AggPar:
+    """The aggregate on `threads` threads as the reference's pool runs it:
+    thread-local tables in 16 radix partitions, then a parallel merge per
+    partition (kgo_pagg_*).  exact_f64: float sums compensated (fsum-grade)."""
+
+    def __init__(self, key_type, input_types, aggs, threads, ansi=False, exact_f64=False):
+        kc = _Col(key_type[0], key_type[1], key_type[2], 0, 0, 0, None, None) if key_type else None
+        its = (_Col * max(1, len(input_types)))()
+        for i, (t, p, s) in enumerate(input_types):
+            its[i] = _Col(t, p, s, 0, 0, 0, None, None)
+        specs = (_AggSpec * max(1, len(aggs)))()
+        for i, (f, inp) in enumerate(aggs):
+            specs[i] = _AggSpec(f, inp)
+        self.naggs = len(aggs)
+        self.h = lib().kgo_pagg_new(C.addressof(kc) if kc is not None else None,
+                                    len(input_types), C.addressof(its), len(aggs),
+                                    C.addressof(specs), int(ansi), int(exact_f64), int(threads))
+        if not self.h:
+            raise RuntimeError(error())
+
+    def update(self, key: Col | None, inputs):
+        n = key.length if key is not None else (inputs[0].length if inputs else 0)
+        kc = key.c() if key is not None else None
+        ca = _cols(inputs)
+        if lib().kgo_pagg_update(self.h, C.addressof(kc) if kc is not None else None,
+                                 C.addressof(ca), n) != 0:
+            raise ArithmeticError(error())
+
+    def finalize(self):
+        if lib().kgo_pagg_finalize(self.h) != 0:
+            raise ArithmeticError(error())
+
+    def finish(self, raw=False):
+        self.finalize()
+        return _finish(lib().kgo_pagg_groups(self.h), self.naggs,
+                       lambda i, t, p, s: lib().kgo_pagg_result_type(self.h, i, t, p, s),
+                       lambda k, kv, outs: lib().kgo_pagg_finish(self.h, k, kv, outs), raw)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().kgo_pagg_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class _Expr(C.Structure):
+    _fields_ = [("op", C.c_int32), ("arg", C.c_int32), ("imm", C.c_int64)]
+
+
+def join_ex(join_type, probe_keys, build_keys, null_safe=None, nested=False, filt=None,
+            probe_cols=None, build_cols=None, cap=None):
+    """All ten join types with an optional residual filter: `filt` is a list
+    of (op, arg, imm) postfix ops over probe_cols/build_cols.  Returns
+    (probe rows, build rows) as kgo_join_ex emits them."""
+    n_p = probe_keys[0].length
+    n_b = build_keys[0].length
+    if cap is None:
+        cap = (n_p + 1) * (n_b + 1) if nested else max(16, 4 * (n_p + n_b))
+    op = np.zeros(cap, dtype=np.int64)
+    ob = np.zeros(cap, dtype=np.int64)
+    pk, bk = _cols(probe_keys), _cols(build_keys)
+    ns = None if null_safe is None else np.asarray(null_safe, dtype=np.uint8)
+    nf = len(filt) if filt else 0
+    fx = (_Expr * max(1, nf))(*[_Expr(*x) for x in (filt or [])])
+    pc = _cols(probe_cols or [])
+    bc = _cols(build_cols or [])
+    n = lib().kgo_join_ex(join_type, len(probe_keys), C.addressof(pk), C.addressof(bk), _ptr(ns),
+                          n_p, n_b, int(nested), nf, C.addressof(fx), C.addressof(pc),
+                          C.addressof(bc), cap, _ptr(op), _ptr(ob))
+    if n < 0:
+        return join_ex(join_type, probe_keys, build_keys, null_safe, nested, filt, probe_cols,
+                       build_cols, cap * 4)
+    return op[:n].copy(), ob[:n].copy()
 
 
 def join(join_type, probe_keys, build_keys, null_safe=None, nested=False, cap=None):
@@ -438,3 +523,17 @@ def join(join_type, probe_keys, build_keys, null_safe=None, nested=False, cap=No
     if n < 0:
         return join(join_type, probe_keys, build_keys, null_safe, nested, cap * 4)
     return op[:n].copy(), ob[:n].copy()
+
+
+def grace_factorized(bkey: Col, bpay: Col, pkey: Col, ppay: Col, parts: int, mode: int = 0):
+    """Configuration 4's per-partition (COUNT, SUM, sum valid) factorized per
+    key (kgo_grace_factorized); mode 1 takes the sum mod 2^31."""
+    counts = np.zeros(parts, dtype=np.int64)
+    sums = np.zeros(parts, dtype=np.int64)
+    valid = np.zeros(parts, dtype=np.uint8)
+    bk, bp, pk, pp = bkey.c(), bpay.c(), pkey.c(), ppay.c()
+    if lib().kgo_grace_factorized(C.byref(bk), C.byref(bp), bkey.length, C.byref(pk),
+                                  C.byref(pp), pkey.length, parts, mode, _ptr(counts),
+                                  _ptr(sums), _ptr(valid)) != 0:
+        raise RuntimeError(error())
+    return counts, sums, valid.astype(bool)

@@ -223,7 +223,7 @@ def test_sbbf_sizing_and_false_positives():
 # ---------------------------------------------------------------- Spark-layout Bloom
 def test_spark_murmur3_and_xxhash64_match_spark_goldens():
     """Spark 4.2.0's hash() (Murmur3 x86_32, seed 42) and xxhash64() (seed
-    42) of int and bigint values: the hashes the Spark-layout filter uses."""
+    42) of int and bigint values."""
     with open(os.path.join(GOLD, "spark_hash_int_bigint.json")) as f:
         g = json.load(f)
     L = kgo.lib()
@@ -236,25 +236,6 @@ def test_spark_murmur3_and_xxhash64_match_spark_goldens():
         else:
             assert L.kgo_murmur3_long(v, 42) == c["hash"], c
             assert kgo.spark_xxhash64([v], 8)[0] == c["xxhash64"], c
-
-
-def test_spark_bloom_has_no_false_negatives_and_its_expected_fpp():
-    """Layout parity is unpinned (the reference has no Spark filter); what
-    any correct filter must do: every inserted item passes, and the rate of
-    passing non-members is near (1 - e^(-kn/m))^k."""
-    L = kgo.lib()
-    n = 20000
-    m = L.kgo_spark_bloom_bits_for(n, 0.03)
-    k = L.kgo_spark_bloom_hashes_for(n, m)
-    assert abs(m - (-n * math.log(0.03) / math.log(2) ** 2)) < 1 and k == 5
-    items = kgo.spark_xxhash64(np.arange(n), 8)
-    words = kgo.spark_bloom_build(items, m, k)
-    assert len(words) == (m + 63) // 64
-    assert kgo.spark_bloom_check_batch(words, k, items).all()
-    other = kgo.spark_xxhash64(np.arange(n, n + 200000), 8)
-    fp = kgo.spark_bloom_check_batch(words, k, other).mean()
-    want = (1 - math.exp(-k * n / (len(words) * 64))) ** k
-    assert 0.7 * want < fp < 1.3 * want
 
 
 # ---------------------------------------------------------------- generators
@@ -541,3 +522,253 @@ def test_join_null_safe_and_float_normalization():
     assert pairs == [(0, 0), (1, 1)]
     pairs = sorted(zip(*kgo.join(kgo.JOIN_INNER, P, B, null_safe=[1])))
     assert pairs == [(0, 0), (1, 1), (3, 3)]
+
+
+# ---------------------------------------------------------------- join types
+# The reference's nested-loop oracle (crates/krishiv-exec/tests/hash_join/
+# main.rs:130-195), restated in Python over rows (k1, k2, v): string keys are
+# mapped to small integers.  Existence follows tests/hash_join/existence.rs:
+# every probe row once, plus whether some build row matches it.
+ALL_TYPES = [kgo.JOIN_INNER, kgo.JOIN_LEFT_OUTER, kgo.JOIN_LEFT_SEMI, kgo.JOIN_LEFT_ANTI,
+             kgo.JOIN_NULL_AWARE_ANTI, kgo.JOIN_EXISTENCE, kgo.JOIN_RIGHT_SEMI,
+             kgo.JOIN_RIGHT_ANTI, kgo.JOIN_RIGHT_OUTER, kgo.JOIN_FULL_OUTER]
+# VLess (main.rs:72-91): (p + b) % 7 != 0 AND p < b, NULL when (p+b) % 7 == 0
+VLESS = [(kgo.X_PROBE, 2, 0), (kgo.X_BUILD, 2, 0), (kgo.X_ADD, 0, 0), (kgo.X_CONST, 0, 7),
+         (kgo.X_MOD, 0, 0), (kgo.X_CONST, 0, 0), (kgo.X_NE, 0, 0), (kgo.X_PROBE, 2, 0),
+         (kgo.X_BUILD, 2, 0), (kgo.X_LT, 0, 0), (kgo.X_AND, 0, 0)]
+
+
+def ref_oracle(jt, keys, safe, filt, probe, build):
+    def eq(s, x, y):
+        return x == y if s else (x is not None and x == y)
+
+    def key_eq(p, b):
+        return all(eq(safe[k], p[k], b[k]) for k in range(keys))
+
+    def vless(p, b):
+        return (p + b) % 7 != 0 and p < b
+
+    def matches(p, b):
+        return key_eq(p, b) and (not filt or vless(p[2], b[2]))
+
+    out = []
+    for bi, b in enumerate(build):
+        m = [pi for pi, p in enumerate(probe) if matches(p, b)]
+        if jt == kgo.JOIN_RIGHT_SEMI and m:
+            out.append((-1, bi))
+        elif jt == kgo.JOIN_RIGHT_ANTI and not m:
+            out.append((-1, bi))
+        elif jt == kgo.JOIN_FULL_OUTER and not m:
+            out.append((-1, bi))
+        elif jt == kgo.JOIN_RIGHT_OUTER:
+            out += [(pi, bi) for pi in m] or [(-1, bi)]
+    for pi, p in enumerate(probe):
+        m = [bi for bi, b in enumerate(build) if matches(p, b)]
+        if jt in (kgo.JOIN_INNER, kgo.JOIN_LEFT_OUTER, kgo.JOIN_FULL_OUTER):
+            out += [(pi, bi) for bi in m]
+            if not m and jt != kgo.JOIN_INNER:
+                out.append((pi, -1))
+        elif jt == kgo.JOIN_LEFT_SEMI and m:
+            out.append((pi, -1))
+        elif jt == kgo.JOIN_LEFT_ANTI and not m:
+            out.append((pi, -1))
+        elif jt == kgo.JOIN_NULL_AWARE_ANTI:
+            keep = not build or (all(b[0] is not None for b in build) and p[0] is not None
+                                 and not m)
+            if keep:
+                out.append((pi, -1))
+        elif jt == kgo.JOIN_EXISTENCE:
+            out.append((pi, 1 if m else 0))
+    return sorted(out)
+
+
+def _cols3(rows):
+    k1 = np.array([0 if r[0] is None else r[0] for r in rows], dtype=np.int64)
+    k2 = np.array([0 if r[1] is None else r[1] for r in rows], dtype=np.int64)
+    v = np.array([r[2] for r in rows], dtype=np.int64)
+    return [kgo.Col(kgo.T_INT64, k1, [r[0] is not None for r in rows]),
+            kgo.Col(kgo.T_INT64, k2, [r[1] is not None for r in rows]),
+            kgo.Col(kgo.T_INT64, v)]
+
+
+def _run(jt, keys, safe, filt, probe, build, nested):
+    P, B = _cols3(probe), _cols3(build)
+    op, ob = kgo.join_ex(jt, P[:keys], B[:keys], safe[:keys], nested=nested,
+                         filt=VLESS if filt else None, probe_cols=P, build_cols=B)
+    return sorted(zip(op.tolist(), ob.tolist()))
+
+
+A_, B_, X_ = 1, 2, 3  # the reference's string keys "a", "b", "x"
+EDGE_PROBE = [(1, A_, 1), (1, A_, 9), (2, None, 3), (None, A_, 4), (None, None, 5), (7, X_, 6)]
+EDGE_BUILD = [(1, A_, 8), (1, A_, 2), (1, B_, 3), (2, None, 12), (None, None, 10), (None, A_, 11)]
+ALL_NULL = [(None, None, 1), (None, None, 2)]
+
+
+@pytest.mark.parametrize("jt", ALL_TYPES)
+def test_join_types_match_the_reference_oracle_on_edge_cases(jt):
+    """The reference's edge cases (main.rs:232-290): NULL keys, duplicates on
+    both sides, empty and all-NULL inputs, one and two keys, `=` and `<=>`,
+    with and without the residual filter; hash and nested-loop forms."""
+    inputs = [(EDGE_PROBE, EDGE_BUILD), (EDGE_PROBE, []), ([], EDGE_BUILD),
+              (EDGE_PROBE, ALL_NULL), (ALL_NULL, EDGE_BUILD)]
+    for keys in (1, 2):
+        for s in (False, True):
+            for filt in (False, True):
+                if jt == kgo.JOIN_NULL_AWARE_ANTI and (keys > 1 or s or filt):
+                    continue
+                safe = [s] * keys
+                for p, b in inputs:
+                    want = ref_oracle(jt, keys, safe, filt, p, b)
+                    for nested in (False, True):
+                        assert _run(jt, keys, safe, filt, p, b, nested) == want, \
+                            (jt, keys, s, filt, len(p), len(b), nested)
+
+
+def test_not_in_follows_spark_null_semantics():
+    """main.rs:300-321: an empty subquery keeps every row (NULL too); a NULL
+    in it keeps none; otherwise a NULL probe value is dropped."""
+    probe = [(1, None, 0), (2, None, 0), (None, None, 0)]
+    jt = kgo.JOIN_NULL_AWARE_ANTI
+    assert [p for p, _ in _run(jt, 1, [False], False, probe, [], False)] == [0, 1, 2]
+    assert _run(jt, 1, [False], False, probe, [(1, None, 0), (None, None, 0)], False) == []
+    assert _run(jt, 1, [False], False, probe, [(1, None, 0)], False) == [(1, -1)]
+
+
+def test_random_joins_match_the_reference_oracle():
+    """main.rs:356-380 in small: random rows with NULLs, keys in 0..6, values
+    in 0..20, every join type, one or two keys, `=`/`<=>`, filter on/off."""
+    rng = random.Random(7)
+
+    def row():
+        k1 = None if rng.random() < 0.2 else rng.randrange(6)
+        k2 = None if rng.random() < 0.2 else rng.randrange(3)
+        return (k1, k2, rng.randrange(20))
+
+    for case in range(150):
+        jt = ALL_TYPES[case % len(ALL_TYPES)]
+        naaj = jt == kgo.JOIN_NULL_AWARE_ANTI
+        keys = 1 if naaj else rng.choice([1, 2])
+        safe = [False] if naaj else [rng.random() < 0.5 for _ in range(keys)]
+        filt = (not naaj) and rng.random() < 0.5
+        probe = [row() for _ in range(rng.randrange(0, 40))]
+        build = [row() for _ in range(rng.randrange(0, 40))]
+        want = ref_oracle(jt, keys, safe, filt, probe, build)
+        assert _run(jt, keys, safe, filt, probe, build, False) == want, case
+
+
+def test_residual_filter_three_valued_logic_and_arithmetic():
+    """The filter program's operators on single pairs: wrapping arithmetic,
+    % with the dividend's sign and NULL for a zero divisor, Kleene AND/OR,
+    NOT, IS NULL, comparisons with NULL."""
+    P = [kgo.Col(kgo.T_INT64, np.array([0], dtype=np.int64)),
+         kgo.Col(kgo.T_INT64, np.array([-7], dtype=np.int64)),
+         kgo.Col(kgo.T_INT64, np.array([0], dtype=np.int64), [False])]
+    B = [kgo.Col(kgo.T_INT64, np.array([0], dtype=np.int64)),
+         kgo.Col(kgo.T_INT64, np.array([3], dtype=np.int64))]
+    X = kgo
+
+    def passes(prog):
+        op, _ = kgo.join_ex(kgo.JOIN_LEFT_SEMI, P[:1], B[:1], filt=prog, probe_cols=P,
+                            build_cols=B)
+        return len(op) == 1
+
+    pm, bm, c = (X.X_PROBE, 1, 0), (X.X_BUILD, 1, 0), lambda v: (X.X_CONST, 0, v)
+    pnull = (X.X_PROBE, 2, 0)
+    assert passes([pm, bm, (X.X_MOD, 0, 0), c(-1), (X.X_EQ, 0, 0)])       # -7 % 3 == -1
+    assert not passes([pm, c(0), (X.X_MOD, 0, 0), c(0), (X.X_EQ, 0, 0)])  # x % 0 is NULL
+    assert passes([c(-(1 << 63)), c(-1), (X.X_ADD, 0, 0), c((1 << 63) - 1), (X.X_EQ, 0, 0)])
+    assert passes([c(1 << 62), c(4), (X.X_MUL, 0, 0), c(0), (X.X_EQ, 0, 0)])
+    assert not passes([pnull, c(1), (X.X_LT, 0, 0)])
+    assert not passes([pnull, c(1), (X.X_LT, 0, 0), (X.X_NOT, 0, 0)])
+    assert passes([pnull, (X.X_IS_NULL, 0, 0)])
+    # NULL AND FALSE = FALSE; NULL OR TRUE = TRUE; NULL AND TRUE = NULL
+    f, t, n = [c(1), c(2), (X.X_GT, 0, 0)], [c(1), c(2), (X.X_LT, 0, 0)], [pnull, c(1), (X.X_LT, 0, 0)]
+    assert not passes(n + f + [(X.X_AND, 0, 0)])
+    assert passes(n + f + [(X.X_AND, 0, 0), (X.X_NOT, 0, 0)])
+    assert passes(n + t + [(X.X_OR, 0, 0)])
+    assert not passes(n + t + [(X.X_AND, 0, 0)])
+    assert not passes(n + t + [(X.X_AND, 0, 0), (X.X_NOT, 0, 0)])
+
+
+def test_parallel_aggregate_equals_single_threaded_and_fsum():
+    """The thread-pool form (kgo_pagg) returns the single-threaded
+    aggregate's integer and decimal results bit for bit; with compensated
+    float sums its SUM/AVG equal math.fsum within one unit in the last
+    place, and the row-order double sums stay within 1e-12 of it."""
+    n, nd = 300_000, 20_000
+    cdf = kgo.zipf_cdf(nd, 1.1)
+    key, dec, f, i32, _ = kgo.gen_agg(3, 0, n, cdf, nd)
+    its = [(kgo.T_DECIMAL64, 12, 2), (kgo.T_FLOAT64, 0, 0), (kgo.T_INT32, 0, 0)]
+    aggs = [(kgo.AGG_COUNT_STAR, 0), (kgo.AGG_SUM, 0), (kgo.AGG_SUM, 1), (kgo.AGG_AVG, 1),
+            (kgo.AGG_MIN, 2), (kgo.AGG_MAX, 1), (kgo.AGG_AVG, 0), (kgo.AGG_SUM, 2)]
+    ins = lambda a, b: [kgo.Col(kgo.T_DECIMAL64, dec[a:b], precision=12, scale=2),
+                        kgo.Col(kgo.T_FLOAT64, f[a:b]), kgo.Col(kgo.T_INT32, i32[a:b])]
+    one = kgo.Agg((kgo.T_INT64, 0, 0), its, aggs)
+    one.update(kgo.Col(kgo.T_INT64, key), ins(0, n))
+    k1, _, r1 = one.finish(raw=True)
+    par = kgo.AggPar((kgo.T_INT64, 0, 0), its, aggs, 6, exact_f64=True)
+    for a in range(0, n, 70_000):
+        par.update(kgo.Col(kgo.T_INT64, key[a:a + 70_000]), ins(a, a + 70_000))
+    k2, _, r2 = par.finish(raw=True)
+    o1, o2 = np.argsort(k1), np.argsort(k2)
+    assert np.array_equal(k1[o1], k2[o2])
+    for j, ((v1, ok1, t), (v2, ok2, _)) in enumerate(zip(r1, r2)):
+        assert np.array_equal(ok1[o1], ok2[o2])
+        if t[0] != kgo.T_FLOAT64:
+            assert np.array_equal(np.asarray(v1)[o1], np.asarray(v2)[o2]), j
+    # exact sums against math.fsum on the 200 largest groups and a sample
+    order = np.argsort(key, kind="stable")
+    ks, fs = key[order], f[order]
+    starts = np.searchsorted(ks, k2)
+    ends = np.searchsorted(ks, k2, side="right")
+    for g in np.concatenate([np.argsort(ends - starts)[-200:], np.arange(0, len(k2), 97)]):
+        exact = math.fsum(fs[starts[g]:ends[g]])
+        assert abs(r2[2][0][g] - exact) <= 2 * math.ulp(exact), g
+        assert abs(r2[3][0][g] - exact / (ends[g] - starts[g])) <= 4 * math.ulp(r2[3][0][g])
+    naive = dict(zip(k1.tolist(), r1[2][0].tolist()))
+    for g in range(0, len(k2), 13):
+        assert math.isclose(naive[int(k2[g])], r2[2][0][g], rel_tol=1e-12)
+
+
+def test_grace_factorization_equals_the_pair_walk():
+    """Configuration 4's per-partition COUNT and SUM(probe.pay * build.pay),
+    factorized per key, against every pair walked by the join oracle: the
+    wrapping sum (mode 0), the sum mod 2^31 (mode 1), and the per-pair
+    SUM(p * b % 2^31) that mode 0 equals when every product is below 2^31."""
+    rng = np.random.default_rng(3)
+    cdf = kgo.zipf_cdf(700, 0.8)
+    bk = kgo.gen_agg(6, 0, 4000, cdf, 700)[0]
+    pk = kgo.gen_agg(7, 0, 9000, cdf, 700)[0]
+    bkv, pkv = rng.random(4000) > 0.02, rng.random(9000) > 0.02
+    for hi in (256, 1 << 40):
+        bp, pp = rng.integers(0, hi, 4000), rng.integers(-hi, hi, 9000)
+        bpv, ppv = rng.random(4000) > 0.05, rng.random(9000) > 0.05
+        op, ob = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, pk, pkv)],
+                          [kgo.Col(kgo.T_INT64, bk, bkv)])
+        pids = kgo.route([kgo.Col(kgo.T_INT64, pk[op])], 64)
+        both = ppv[op] & bpv[ob]
+        with np.errstate(over="ignore"):
+            prod = pp[op].astype(np.uint64) * bp[ob].astype(np.uint64)
+        s = np.zeros(64, dtype=np.uint64)
+        np.add.at(s, pids[both], prod[both])
+        v = np.bincount(pids[both], minlength=64) > 0
+        for mode in (0, 1):
+            c, got, gv = kgo.grace_factorized(kgo.Col(kgo.T_INT64, bk, bkv),
+                                              kgo.Col(kgo.T_INT64, bp, bpv),
+                                              kgo.Col(kgo.T_INT64, pk, pkv),
+                                              kgo.Col(kgo.T_INT64, pp, ppv), 64, mode)
+            assert np.array_equal(c, np.bincount(pids, minlength=64))
+            want = s if mode == 0 else s & np.uint64(0x7FFFFFFF)
+            assert np.array_equal(got.view(np.uint64), want) and np.array_equal(gv, v)
+        if hi == 256:
+            per_pair = np.zeros(64, dtype=np.int64)
+            ok = both & (pp[op] >= 0)
+            np.add.at(per_pair, pids[ok], (pp[op][ok] * bp[ob][ok]) % (1 << 31))
+            c, got, _ = kgo.grace_factorized(kgo.Col(kgo.T_INT64, bk, bkv),
+                                             kgo.Col(kgo.T_INT64, bp, bpv),
+                                             kgo.Col(kgo.T_INT64, pk, pkv),
+                                             kgo.Col(kgo.T_INT64, np.abs(pp), ppv), 64, 0)
+            pa = np.abs(pp)
+            per_pair = np.zeros(64, dtype=np.int64)
+            np.add.at(per_pair, pids[both], (pa[op][both] * bp[ob][both]) % (1 << 31))
+            assert np.array_equal(got, per_pair)
