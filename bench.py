@@ -375,8 +375,10 @@ def grace(args):
     del cdf
     bk, bp = DeviceBuffer(ctx, nb * 8), DeviceBuffer(ctx, nb * 8)
     pk, pp = DeviceBuffer(ctx, npr * 8), DeviceBuffer(ctx, npr * 8)
-    A.check(L.kg_gen_agg(ctx.h, 6, 0, nb, cdf_d.ptr, nd, bk.ptr, None, None, None, None))
-    A.check(L.kg_gen_agg(ctx.h, 7, 0, npr, cdf_d.ptr, nd, pk.ptr, None, None, None, None))
+    # one key distribution on both sides: the permutation of ranks to keys is
+    # shared (seed 6), the draws differ
+    A.check(L.kg_gen_zipf_keys(ctx.h, 6, 6, 0, nb, cdf_d.ptr, nd, bk.ptr))
+    A.check(L.kg_gen_zipf_keys(ctx.h, 7, 6, 0, npr, cdf_d.ptr, nd, pk.ptr))
     # payloads: the amt column of the probe generator, uniform in [1, 1e4]
     scratch = DeviceBuffer(ctx, max(nb, npr) * 8)
     A.check(L.kg_gen_join_probe(ctx.h, 8, 0, nb, 1, 0, scratch.ptr, None, bp.ptr))

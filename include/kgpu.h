@@ -394,6 +394,12 @@ int kg_zipf_cdf(uint64_t n, double s, double *cdf_host);
 int kg_gen_agg(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, const double *cdf_dev,
                uint64_t ndistinct, int64_t *key_dev, int64_t *dec_dev, double *f_dev,
                int32_t *i32_dev, int8_t *k8_dev);
+/* Zipf keys (ranks drawn from the row seed) whose rank-to-key permutation
+ * comes from key_seed: columns with different seeds and one key_seed share
+ * one key distribution (the same keys are hot on both sides of a join);
+ * key_seed == seed gives kg_gen_agg's key column. */
+int kg_gen_zipf_keys(kg_ctx *ctx, uint64_t seed, uint64_t key_seed, int64_t lo, int64_t hi,
+                     const double *cdf_dev, uint64_t ndistinct, int64_t *key_dev);
 
 /* ---- profiling ----------------------------------------------------------
  * Device time of the hot kernels, from HIP events recorded around each launch

@@ -105,11 +105,36 @@ __global__ void k_gen_agg(uint64_t seed, int64_t lo, int64_t hi, const double *c
   }
 }
 
+// Zipf keys whose rank-to-key permutation comes from key_seed: two columns
+// with different row seeds and one key_seed draw from one distribution (the
+// same keys are hot on both sides of a join).  key_seed == seed gives
+// k_gen_agg's key column.
+__global__ void k_gen_zipf_keys(uint64_t seed, uint64_t key_seed, int64_t lo, int64_t hi,
+                                const double *cdf, uint64_t ndistinct, int64_t *key) {
+  const uint64_t b0 = stream_base(seed, 0);
+  for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = zipf_rank(cdf, ndistinct, dev::unit(dev::rand_at(b0, (uint64_t)i)));
+    key[i - lo] = (int64_t)perm(r, ndistinct, key_seed ^ 0x5BD1E995ULL);
+  }
+}
+
 }  // namespace kg
 
 using namespace kg;
 
 extern "C" {
+
+int kg_gen_zipf_keys(kg_ctx *ctx, uint64_t seed, uint64_t key_seed, int64_t lo, int64_t hi,
+                     const double *cdf_dev, uint64_t ndistinct, int64_t *key_dev) {
+  return guarded([&] {
+    if (hi <= lo) return;
+    k_gen_zipf_keys<<<stream_grid(hi - lo, 256), 256, 0, ctx->stream>>>(
+        seed, key_seed, lo, hi, cdf_dev, ndistinct, key_dev);
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
 
 int kg_gen_join_build(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, int64_t n_total,
                       int64_t *key_dev, int64_t *val_dev) {

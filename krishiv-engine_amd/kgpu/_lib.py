@@ -118,6 +118,8 @@ _SIGS = {
     "kg_gen_agg": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_void_p,
                              C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_void_p]),
+    "kg_gen_zipf_keys": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, C.c_int64,
+                                   C.c_void_p, C.c_uint64, C.c_void_p]),
     "kg_prof_enable": (None, [C.c_int]),
     "kg_prof_reset": (None, []),
     "kg_prof_get": (C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p]),

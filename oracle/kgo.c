@@ -640,6 +640,14 @@ void kgo_gen_join_probe(uint64_t seed, int64_t lo, int64_t hi, int64_t key_range
 /* Aggregation input: key = a permutation of a Zipf rank over ndistinct
  * values; dec (decimal(12,2) cents) in [1, 1e7); f in [0, 1); i32 in
  * [0, 50]; k8 (the low-cardinality int8 key) in [0, 4). */
+void kgo_gen_zipf_keys(uint64_t seed, uint64_t key_seed, int64_t lo, int64_t hi,
+                       const double *cdf, uint64_t ndistinct, int64_t *key) {
+  for (int64_t i = lo; i < hi; i++) {
+    uint64_t r = kgo_zipf_rank(cdf, ndistinct, kgo_unit(kgo_rand(seed, 0, (uint64_t)i)));
+    key[i - lo] = (int64_t)kgo_perm(r, ndistinct, key_seed ^ 0x5BD1E995ULL);
+  }
+}
+
 void kgo_gen_agg(uint64_t seed, int64_t lo, int64_t hi, const double *cdf, uint64_t ndistinct,
                  int64_t *key, int64_t *dec, double *f, int32_t *i32, int8_t *k8) {
   for (int64_t i = lo; i < hi; i++) {

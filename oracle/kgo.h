@@ -126,6 +126,10 @@ void kgo_gen_join_probe(uint64_t seed, int64_t lo, int64_t hi, int64_t key_range
                         uint32_t null_permille, int64_t *fk, uint8_t *fk_valid, int64_t *amt);
 void kgo_gen_agg(uint64_t seed, int64_t lo, int64_t hi, const double *cdf, uint64_t ndistinct,
                  int64_t *key, int64_t *dec, double *f, int32_t *i32, int8_t *k8);
+/* Zipf keys whose rank-to-key permutation comes from key_seed (shared by
+ * both sides of a join); key_seed == seed gives kgo_gen_agg's keys. */
+void kgo_gen_zipf_keys(uint64_t seed, uint64_t key_seed, int64_t lo, int64_t hi,
+                       const double *cdf, uint64_t ndistinct, int64_t *key);
 
 /* ---------------------------------------------------------------------
  * Hash aggregation with Spark semantics

@@ -132,6 +132,7 @@ def lib():
         L.kgo_gen_join_build.argtypes = [u64, i64, i64, i64, vp, vp]
         L.kgo_gen_join_probe.argtypes = [u64, i64, i64, i64, u32, vp, vp, vp]
         L.kgo_gen_agg.argtypes = [u64, i64, i64, vp, u64, vp, vp, vp, vp, vp]
+        L.kgo_gen_zipf_keys.argtypes = [u64, u64, i64, i64, vp, u64, vp]
         L.kgo_agg_new.restype = vp
         L.kgo_agg_new.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int]
         L.kgo_agg_update.restype = C.c_int
@@ -328,6 +329,12 @@ def gen_join_probe(seed, lo, hi, key_range, null_permille=10):
     amt = np.zeros(hi - lo, dtype=np.int64)
     lib().kgo_gen_join_probe(seed, lo, hi, key_range, null_permille, _ptr(fk), _ptr(ok), _ptr(amt))
     return fk, ok.astype(bool), amt
+
+
+def gen_zipf_keys(seed, key_seed, lo, hi, cdf, ndistinct):
+    key = np.zeros(hi - lo, dtype=np.int64)
+    lib().kgo_gen_zipf_keys(seed, key_seed, lo, hi, _ptr(cdf), ndistinct, _ptr(key))
+    return key
 
 
 def gen_agg(seed, lo, hi, cdf, ndistinct):

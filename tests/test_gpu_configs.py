@@ -286,8 +286,9 @@ def test_config4_grace_at_one_hundredth():
     del cdf
     bk, bp = DeviceBuffer(c, nb * 8), DeviceBuffer(c, nb * 8)
     pk, pp = DeviceBuffer(c, npr * 8), DeviceBuffer(c, npr * 8)
-    A.check(L.kg_gen_agg(c.h, 6, 0, nb, cdf_d.ptr, nd, bk.ptr, None, None, None, None))
-    A.check(L.kg_gen_agg(c.h, 7, 0, npr, cdf_d.ptr, nd, pk.ptr, None, None, None, None))
+    # one key distribution on both sides (permutation seed 6)
+    A.check(L.kg_gen_zipf_keys(c.h, 6, 6, 0, nb, cdf_d.ptr, nd, bk.ptr))
+    A.check(L.kg_gen_zipf_keys(c.h, 7, 6, 0, npr, cdf_d.ptr, nd, pk.ptr))
     scratch = DeviceBuffer(c, npr * 8)
     A.check(L.kg_gen_join_probe(c.h, 8, 0, nb, 1, 0, scratch.ptr, None, bp.ptr))
     A.check(L.kg_gen_join_probe(c.h, 9, 0, npr, 1, 0, scratch.ptr, None, pp.ptr))
@@ -305,5 +306,6 @@ def test_config4_grace_at_one_hundredth():
     hb = [kgo.Col(kgo.T_INT64, host(x, np.int64, nb)) for x in (bk, bp)]
     hp = [kgo.Col(kgo.T_INT64, host(x, np.int64, npr)) for x in (pk, pp)]
     wc, ws, wv = kgo.grace_factorized(hb[0], hb[1], hp[0], hp[1], 64, 0)
-    assert wc.sum() > 10 ** 10  # about 2e10 pairs at this scale
+    assert wc.sum() > 10 ** 11  # about 1.3e11 pairs at this scale
+    assert np.array_equal(hb[0].values[:1000], kgo.gen_zipf_keys(6, 6, 0, 1000, kgo.zipf_cdf(nd, 0.8), nd))
     assert np.array_equal(gc, wc) and np.array_equal(gs, ws) and np.array_equal(gv, wv)

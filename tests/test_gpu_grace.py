@@ -15,9 +15,9 @@ pytestmark = pytest.mark.gpu
 
 
 def zipf_keys(seed, n, distinct, s):
-    cdf = kgo.zipf_cdf(distinct, s)
-    key, _, _, _, _ = kgo.gen_agg(seed, 0, n, cdf, distinct)
-    return key
+    """Zipf(s) keys over `distinct` values; both sides of a join share the
+    rank-to-key permutation (seed 6), so the same keys are hot on both."""
+    return kgo.gen_zipf_keys(seed, 6, 0, n, kgo.zipf_cdf(distinct, s), distinct)
 
 
 def factorized(bk, bp, bv, pk, pp, pv, parts):
