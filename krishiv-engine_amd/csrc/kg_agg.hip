@@ -26,12 +26,12 @@
 // tables directly.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <memory>
 
 #include "kg_aggops.hpp"
+#include "kg_scan.hpp"
 #include "kg_aggtile.hpp"
 
 namespace kg {
@@ -1447,12 +1447,9 @@ static uint32_t partition_spill(kg_hash_agg *a, const uint64_t *in, uint64_t n, 
   k_spill_hist<<<(unsigned)nb, 256, np * 4, a->s>>>(in, w, n, t, log2p, per_block, dead ? 1 : 0,
                                                     hist.get<uint32_t>());
   KG_HIP(hipGetLastError());
-  size_t tb = 0;
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist.get<uint32_t>(), base.get<uint64_t>(),
-                                          (int)hn, a->s));
-  DevBuf temp(a->ctx, tb + 16);
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, hist.get<uint32_t>(),
-                                          base.get<uint64_t>(), (int)hn, a->s));
+  DevBuf temp(a->ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
+  exclusive_scan(hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(),
+                 a->s);
   k_spill_scatter<<<(unsigned)nb, 256, np * 4, a->s>>>(in, w, n, t, log2p, per_block,
                                                        dead ? 1 : 0, base.get<uint64_t>(),
                                                        out.get<uint64_t>());
@@ -1877,12 +1874,8 @@ int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t ns, int W, bo
   DevBuf flags(ctx, ns * 4), pos(ctx, (ns + 1) * 8);
   k_group_flags<<<stream_grid(ns, 256), 256, 0, s>>>(rec, ns, W, global, flags.get<uint32_t>());
   KG_HIP(hipGetLastError());
-  size_t tb = 0;
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flags.get<uint32_t>(), pos.get<uint64_t>(),
-                                          (int)ns, s));
-  DevBuf temp(ctx, tb + 16);
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, flags.get<uint32_t>(),
-                                          pos.get<uint64_t>(), (int)ns, s));
+  DevBuf temp(ctx, (uint64_t)scan_partials((int64_t)ns) * 8);
+  exclusive_scan(flags.get<uint32_t>(), pos.get<uint64_t>(), (int64_t)ns, temp.get<uint64_t>(), s);
   uint64_t last_pos = 0;
   uint32_t last_flag = 0;
   KG_HIP(hipMemcpyAsync(&last_pos, pos.get<uint64_t>() + ns - 1, 8, hipMemcpyDeviceToHost, s));

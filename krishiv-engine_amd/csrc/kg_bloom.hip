@@ -12,12 +12,12 @@
 // bitmap into the surviving row ids in input order.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 
 #include "kg_common.hpp"
 #include "kg_device.hpp"
+#include "kg_scan.hpp"
 
 namespace kg {
 
@@ -165,12 +165,14 @@ __global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_bits(DCol key, int6
   if (threadIdx.x == 0) tile_count[blockIdx.x] = s_cnt;
 }
 
-// Pass 2: thread t of a tile owns 32 consecutive rows (half a bitmap word).
+// Pass 2: thread t of a tile owns 32 consecutive rows (half a bitmap word);
+// the tile's survivors go to out_rows from tile_off[tile] on, as row_base +
+// row (the row ids of a chunk that starts at row_base).
 __global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_rows(const uint64_t *bitmap, int64_t n,
                                                                   const uint64_t *tile_off,
+                                                                  int64_t row_base,
                                                                   int32_t *out_rows) {
-  typedef hipcub::BlockScan<uint32_t, kProbeBlock> Scan;
-  __shared__ typename Scan::TempStorage tmp;
+  __shared__ uint32_t s_w[kProbeBlock / 64];
   int64_t base = (int64_t)blockIdx.x * kProbeTile;
   int64_t r0 = base + (int64_t)threadIdx.x * 32;
   uint32_t bits = 0;
@@ -178,13 +180,13 @@ __global__ void __launch_bounds__(kProbeBlock) k_bloom_probe_rows(const uint64_t
     uint64_t w = bitmap[r0 >> 6];
     bits = (uint32_t)(((r0 >> 5) & 1) ? (w >> 32) : w);
   }
-  uint32_t cnt = (uint32_t)__popc(bits), off = 0;
-  Scan(tmp).ExclusiveSum(cnt, off);
+  uint32_t tot;
+  const uint32_t off = block_exclusive_scan<kProbeBlock>((uint32_t)__popc(bits), s_w, &tot);
   int64_t pos = (int64_t)tile_off[blockIdx.x] + off;
   while (bits) {
     int b = __ffs(bits) - 1;
     bits &= bits - 1;
-    out_rows[pos++] = (int32_t)(r0 + b);
+    out_rows[pos++] = (int32_t)(row_base + r0 + b);
   }
 }
 
@@ -281,6 +283,23 @@ struct kg_bloom {
   DevBuf words, items;
   DevBuf bitmap, tiles, tile_off, temp;  // probe scratch
 };
+
+// Pass 1 of a probe of rows [0, n) of k: the survivors' bitmap and a count
+// per tile, for either layout.
+static void probe_bits(kg_bloom *b, const DCol &k, int64_t n, uint64_t *bitmap, uint32_t *tiles) {
+  const unsigned grid = (unsigned)ceil_div(n, kProbeTile);
+  if (b->spark) {
+    k_spark_probe_bits<<<grid, kProbeBlock, 0, b->s>>>(k, n, b->words.get<unsigned long long>(),
+                                                      b->nwords, b->k, bitmap, tiles);
+  } else if (k.type == KG_T_INT64 || k.type == KG_T_UINT64 || k.type == KG_T_TIMESTAMP) {
+    k_bloom_probe_bits<true><<<grid, kProbeBlock, 0, b->s>>>(k, n, b->words.get<uint32_t>(),
+                                                            b->nblocks, bitmap, tiles);
+  } else {
+    k_bloom_probe_bits<false><<<grid, kProbeBlock, 0, b->s>>>(k, n, b->words.get<uint32_t>(),
+                                                             b->nblocks, bitmap, tiles);
+  }
+  KG_HIP(hipGetLastError());
+}
 
 static void check_spark_key(const kg_column &c) {
   const int t = c.type;
@@ -440,52 +459,31 @@ int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *ou
     if (nrows <= 0) return;
     if (nrows > INT32_MAX) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^31 rows or more");
     check_column(*key, nrows, "probe key");
+    if (b->spark) check_spark_key(*key);
     Staged st(b->ctx, b->s);
     DCol k = st.add(*key, nrows);
     int64_t tiles = ceil_div(nrows, kProbeTile);
     b->bitmap.ensure(b->ctx, (uint64_t)ceil_div(nrows, 64) * 8 + 8);
     b->tiles.ensure(b->ctx, (uint64_t)tiles * 4);
     b->tile_off.ensure(b->ctx, (uint64_t)(tiles + 1) * 8);
-    if (b->spark) {
-      check_spark_key(*key);
+    b->temp.ensure(b->ctx, (uint64_t)(scan_partials(tiles) + 2) * 8);
+    uint64_t *total = b->temp.get<uint64_t>() + scan_partials(tiles);
+    {
       ProfScope prof("bloom_probe", b->s);
-      k_spark_probe_bits<<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
-          k, nrows, b->words.get<unsigned long long>(), b->nwords, b->k, b->bitmap.get<uint64_t>(),
-          b->tiles.get<uint32_t>());
-      KG_HIP(hipGetLastError());
-    } else {
-      ProfScope prof("bloom_probe", b->s);
-      const bool w64 = k.type == KG_T_INT64 || k.type == KG_T_UINT64 || k.type == KG_T_TIMESTAMP;
-      if (w64)
-        k_bloom_probe_bits<true><<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
-            k, nrows, b->words.get<uint32_t>(), b->nblocks, b->bitmap.get<uint64_t>(),
-            b->tiles.get<uint32_t>());
-      else
-        k_bloom_probe_bits<false><<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
-            k, nrows, b->words.get<uint32_t>(), b->nblocks, b->bitmap.get<uint64_t>(),
-            b->tiles.get<uint32_t>());
-      KG_HIP(hipGetLastError());
+      probe_bits(b, k, nrows, b->bitmap.get<uint64_t>(), b->tiles.get<uint32_t>());
     }
-    size_t tb = 0;
-    KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, b->tiles.get<uint32_t>(),
-                                            b->tile_off.get<uint64_t>(), (int)tiles, b->s));
-    b->temp.ensure(b->ctx, tb + 16);
-    KG_HIP(hipcub::DeviceScan::ExclusiveSum(b->temp.get(), tb, b->tiles.get<uint32_t>(),
-                                            b->tile_off.get<uint64_t>(), (int)tiles, b->s));
+    exclusive_scan(b->tiles.get<uint32_t>(), b->tile_off.get<uint64_t>(), tiles,
+                   b->temp.get<uint64_t>(), b->s, nullptr, total);
     {
       ProfScope prof("bloom_compact", b->s);
       k_bloom_probe_rows<<<(unsigned)tiles, kProbeBlock, 0, b->s>>>(
-          b->bitmap.get<uint64_t>(), nrows, b->tile_off.get<uint64_t>(), out_rows);
+          b->bitmap.get<uint64_t>(), nrows, b->tile_off.get<uint64_t>(), 0, out_rows);
       KG_HIP(hipGetLastError());
     }
-    uint64_t last_off = 0;
-    uint32_t last_cnt = 0;
-    KG_HIP(hipMemcpyAsync(&last_off, b->tile_off.get<uint64_t>() + tiles - 1, 8,
-                          hipMemcpyDeviceToHost, b->s));
-    KG_HIP(hipMemcpyAsync(&last_cnt, b->tiles.get<uint32_t>() + tiles - 1, 4, hipMemcpyDeviceToHost,
-                          b->s));
+    uint64_t count = 0;
+    KG_HIP(hipMemcpyAsync(&count, total, 8, hipMemcpyDeviceToHost, b->s));
     KG_HIP(hipStreamSynchronize(b->s));
-    *out_count = (int64_t)(last_off + last_cnt);
+    *out_count = (int64_t)count;
   });
 }
 

@@ -5,7 +5,7 @@
 // Repartition is a stable counting sort by partition id in three launches:
 //   1. k_route_hist   H_kg -> partition id per row (4 B/row written) and a
 //                     per-block histogram (LDS atomics);
-//   2. an exclusive scan of the partition-major histogram (hipcub);
+//   2. an exclusive scan of the partition-major histogram (kg_scan.hpp);
 //   3. k_scatter      each wave replays its contiguous quarter of the
 //                     block's rows; the rank of a row among the rows of its
 //                     partition in the same 64-row round comes from a wave
@@ -13,10 +13,10 @@
 //                     keep their input order inside a partition.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
 
 #include "kg_common.hpp"
 #include "kg_device.hpp"
+#include "kg_scan.hpp"
 
 namespace kg {
 
@@ -425,12 +425,8 @@ int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nr
           k, nrows, partitions, per_block, pids.get<uint32_t>(), hist.get<uint32_t>());
       KG_HIP(hipGetLastError());
     }
-    size_t temp_bytes = 0;
-    KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, hist.get<uint32_t>(),
-                                            base.get<uint64_t>(), (int)hn, s));
-    DevBuf temp(ctx, temp_bytes + 16);
-    KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), temp_bytes, hist.get<uint32_t>(),
-                                            base.get<uint64_t>(), (int)hn, s));
+    DevBuf temp(ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
+    exclusive_scan(hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(), s);
     {
       ProfScope prof("scatter", s);
       k_scatter<<<(unsigned)nb, kPartBlock, 4 * partitions * 4, s>>>(

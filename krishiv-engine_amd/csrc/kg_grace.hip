@@ -22,11 +22,11 @@
 //           one table buffer; the host waits once, at the end.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
 
 #include <climits>
 
 #include "kg_common.hpp"
+#include "kg_scan.hpp"
 #include "kg_device.hpp"
 
 namespace kg {
@@ -639,12 +639,9 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
               dk, dv, m, parts, per_block, hist.get<uint32_t>(), g->mm.get<long long>() + 2 * side);
         KG_HIP(hipGetLastError());
       }
-      size_t tb = 0;
-      KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist.get<uint32_t>(),
-                                              base.get<uint64_t>(), (int)hn, g->s));
-      DevBuf temp(g->ctx, tb + 16);
-      KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, hist.get<uint32_t>(),
-                                              base.get<uint64_t>(), (int)hn, g->s));
+      DevBuf temp(g->ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
+      exclusive_scan(hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(),
+                     g->s);
       piece.key.reset(g->ctx, (uint64_t)m * 8 + 8);
       piece.pay.reset(g->ctx, (uint64_t)m * 8 + 8);
       {

@@ -18,9 +18,9 @@
 // the pairs are written and the output columns gathered.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
 
 #include "kg_aggops.hpp"
+#include "kg_scan.hpp"
 #include "kg_aggtile.hpp"
 
 namespace kg {
@@ -1166,10 +1166,8 @@ static void build_direct(kg_hash_join *j) {
 }
 
 static void scan_u32(kg_hash_join *j, const uint32_t *in, uint64_t *out, int64_t n) {
-  size_t tb = 0;
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, j->s));
-  j->temp.ensure(j->ctx, tb + 16);
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(j->temp.get(), tb, in, out, (int)n, j->s));
+  j->temp.ensure(j->ctx, (uint64_t)scan_partials(n) * 8);
+  exclusive_scan(in, out, n, j->temp.get<uint64_t>(), j->s);
 }
 
 }  // namespace kg
@@ -1468,12 +1466,8 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
                                                                  hist.get<uint32_t>());
     KG_HIP(hipGetLastError());
   }
-  size_t tb = 0;
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist.get<uint32_t>(), base.get<uint64_t>(),
-                                          (int)hn, j->s));
-  DevBuf temp(j->ctx, tb + 16);
-  KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, hist.get<uint32_t>(), base.get<uint64_t>(),
-                                          (int)hn, j->s));
+  DevBuf temp(j->ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
+  exclusive_scan(hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(), j->s);
   {
     ProfScope ps("jp_scatter", j->s);
     const uint64_t *bp = base.get<uint64_t>();

@@ -8,7 +8,7 @@
 // scans the counts and expands the bitmap into row ids in input order.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
+#include "kg_scan.hpp"
 
 #include "kg_common.hpp"
 #include "kg_device.hpp"
@@ -49,8 +49,7 @@ __global__ void __launch_bounds__(kSelBlock) k_range_bits(DCol c, int64_t n, int
 __global__ void __launch_bounds__(kSelBlock) k_bits_to_rows(const uint64_t *bitmap, int64_t n,
                                                             const uint64_t *tile_off,
                                                             int32_t *out_rows) {
-  typedef hipcub::BlockScan<uint32_t, kSelBlock> Scan;
-  __shared__ typename Scan::TempStorage tmp;
+  __shared__ uint32_t s_w[kSelBlock / 64];
   int64_t base = (int64_t)blockIdx.x * kSelTile;
   int64_t r0 = base + (int64_t)threadIdx.x * 32;
   uint32_t bits = 0;
@@ -58,8 +57,9 @@ __global__ void __launch_bounds__(kSelBlock) k_bits_to_rows(const uint64_t *bitm
     uint64_t w = bitmap[r0 >> 6];
     bits = (uint32_t)(((r0 >> 5) & 1) ? (w >> 32) : w);
   }
-  uint32_t cnt = (uint32_t)__popc(bits), off = 0;
-  Scan(tmp).ExclusiveSum(cnt, off);
+  uint32_t tot;
+  const uint32_t cnt = (uint32_t)__popc(bits);
+  const uint32_t off = block_exclusive_scan<kSelBlock>(cnt, s_w, &tot);
   int64_t pos = (int64_t)tile_off[blockIdx.x] + off;
   while (bits) {
     int b = __ffs(bits) - 1;
@@ -141,12 +141,8 @@ int kg_select_range(kg_ctx *ctx, const kg_column *col, int64_t nrows, int64_t lo
                                                          counts.get<uint32_t>());
       KG_HIP(hipGetLastError());
     }
-    size_t tb = 0;
-    KG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts.get<uint32_t>(), offs.get<uint64_t>(),
-                                            (int)tiles, s));
-    DevBuf temp(ctx, tb + 16);
-    KG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.get(), tb, counts.get<uint32_t>(),
-                                            offs.get<uint64_t>(), (int)tiles, s));
+    DevBuf temp(ctx, (uint64_t)scan_partials(tiles) * 8);
+    exclusive_scan(counts.get<uint32_t>(), offs.get<uint64_t>(), tiles, temp.get<uint64_t>(), s);
     k_bits_to_rows<<<(unsigned)tiles, kSelBlock, 0, s>>>(bitmap.get<uint64_t>(), nrows,
                                                          offs.get<uint64_t>(), out_rows);
     KG_HIP(hipGetLastError());
