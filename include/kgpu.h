@@ -225,6 +225,16 @@ int kg_bloom_export(kg_bloom *b, uint8_t *bytes, uint64_t *len);
  * *out_count (host). */
 int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *out_rows,
                    int64_t *out_count);
+/* The same probe with no device allocation: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The key column must be
+ * device-resident (KG_COL_DEVICE) and out_rows must hold nrows entries; at
+ * most 2^31 - 1 rows a call.  Its scratch (the survivor bitmap and tile
+ * counts of 16M-row chunks, 2.03 MB) is one file-scope device array per GPU,
+ * held until the process ends: calls that overlap on one GPU race, so the
+ * caller serializes them.  Results equal kg_bloom_probe's. */
+int kg_bloom_probe_noalloc(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *out_rows,
+                           int64_t *out_count);
 void kg_bloom_destroy(kg_bloom *b);
 
 /* Spark-layout Bloom filter (optional).  The reference holds only the hash

@@ -301,6 +301,51 @@ static void probe_bits(kg_bloom *b, const DCol &k, int64_t n, uint64_t *bitmap, 
   KG_HIP(hipGetLastError());
 }
 
+__global__ void k_fill_u64(uint64_t *p, int64_t n, uint64_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---- scratch of kg_bloom_probe_noalloc --------------------------------------
+// The allocation-free probe works through its input in chunks of kNaChunk
+// rows; everything beyond the caller's input and output lives in `scratch`,
+// one 2.03 MB array per GPU for the life of the process: the chunk's
+// survivor bitmap, its tile counts and offsets, the scan's segment sums and
+// a two-word running survivor count.  Calls that overlap on one GPU share
+// the array and race: the caller serializes them.
+constexpr int64_t kNaChunk = 1 << 24;                        // rows a chunk
+constexpr int64_t kNaTilesMax = kNaChunk / kProbeTile;       // 2048 tiles a chunk
+constexpr int64_t kNaBitmap = 0;                             // kNaChunk / 64 words
+constexpr int64_t kNaTiles = kNaBitmap + kNaChunk / 64;      // kNaTilesMax uint32 (half as many words)
+constexpr int64_t kNaTileOff = kNaTiles + kNaTilesMax / 2;   // kNaTilesMax words
+constexpr int64_t kNaPartials = kNaTileOff + kNaTilesMax;    // scan segment sums
+constexpr int64_t kNaTotal = kNaPartials + (kNaTilesMax + kScanSegment - 1) / kScanSegment;
+constexpr int64_t kNaWords = kNaTotal + 2;
+static_assert(kNaChunk % kProbeTile == 0, "chunks are whole tiles");
+
+static __device__ __align__(256) uint64_t scratch[kNaWords];
+
+// The device address of `scratch` on a GPU, looked up once per device with
+// that device current (each GPU holds its own copy of the array).
+static uint64_t *noalloc_scratch(int device) {
+  static std::mutex mu;
+  static std::map<int, uint64_t *> ptrs;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = ptrs.find(device);
+  if (it != ptrs.end()) return it->second;
+  int prev = 0;
+  KG_HIP(hipGetDevice(&prev));
+  KG_HIP(hipSetDevice(device));
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  KG_HIP(hipSetDevice(prev));
+  if (e != hipSuccess || !p)
+    fail(KG_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  ptrs[device] = static_cast<uint64_t *>(p);
+  return static_cast<uint64_t *>(p);
+}
+
 static void check_spark_key(const kg_column &c) {
   const int t = c.type;
   const int w = type_width(t);
@@ -482,6 +527,54 @@ int kg_bloom_probe(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *ou
     }
     uint64_t count = 0;
     KG_HIP(hipMemcpyAsync(&count, total, 8, hipMemcpyDeviceToHost, b->s));
+    KG_HIP(hipStreamSynchronize(b->s));
+    *out_count = (int64_t)count;
+  });
+}
+
+int kg_bloom_probe_noalloc(kg_bloom *b, const kg_column *key, int64_t nrows, int32_t *out_rows,
+                           int64_t *out_count) {
+  return guarded([&] {
+    *out_count = 0;
+    if (nrows < 0) fail(KG_ERR_INTERNAL, "negative row count");
+    if (nrows > INT32_MAX)
+      fail(KG_ERR_UNSUPPORTED, "kg_bloom_probe_noalloc takes at most 2^31 - 1 rows a call");
+    if (nrows == 0) return;
+    if (!(key->flags & KG_COL_DEVICE))
+      fail(KG_ERR_UNSUPPORTED, "kg_bloom_probe_noalloc takes device-resident key columns only");
+    if (!out_rows) fail(KG_ERR_INTERNAL, "no output row buffer");
+    check_column(*key, nrows, "probe key");
+    if (b->spark) check_spark_key(*key);
+    uint64_t *sc = noalloc_scratch(b->ctx->device);
+    uint64_t *bitmap = sc + kNaBitmap, *partials = sc + kNaPartials, *total = sc + kNaTotal;
+    uint32_t *tiles = reinterpret_cast<uint32_t *>(sc + kNaTiles);
+    uint64_t *tile_off = sc + kNaTileOff;
+    DCol k{};
+    k.type = key->type;
+    k.precision = key->precision;
+    k.scale = key->scale;
+    k.width = type_width(key->type);
+    k.offset = key->offset;
+    k.values = key->values;
+    k.validity = key->validity;
+    // the running survivor count starts at zero (a fill kernel, not a memset)
+    k_fill_u64<<<1, 64, 0, b->s>>>(total, 2, 0);
+    KG_HIP(hipGetLastError());
+    int64_t c = 0;
+    for (int64_t at = 0; at < nrows; at += kNaChunk, c++) {
+      const int64_t m = std::min<int64_t>(kNaChunk, nrows - at);
+      const int64_t tcount = ceil_div(m, kProbeTile);
+      DCol kc = k;
+      kc.offset += at;
+      probe_bits(b, kc, m, bitmap, tiles);
+      // survivors of the earlier chunks carry into this chunk's offsets
+      exclusive_scan(tiles, tile_off, tcount, partials, b->s, total + (c & 1), total + ((c + 1) & 1));
+      k_bloom_probe_rows<<<(unsigned)tcount, kProbeBlock, 0, b->s>>>(bitmap, m, tile_off, at,
+                                                                      out_rows);
+      KG_HIP(hipGetLastError());
+    }
+    uint64_t count = 0;
+    KG_HIP(hipMemcpyAsync(&count, total + (c & 1), 8, hipMemcpyDeviceToHost, b->s));
     KG_HIP(hipStreamSynchronize(b->s));
     *out_count = (int64_t)count;
   });

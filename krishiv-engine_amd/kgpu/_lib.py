@@ -93,6 +93,8 @@ _SIGS = {
     "kg_bloom_expected_fpp": (C.c_double, [C.c_void_p, C.c_uint64]),
     "kg_bloom_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_bloom_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "kg_bloom_probe_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                         C.c_void_p]),
     "kg_bloom_destroy": (None, [C.c_void_p]),
     "kg_select_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                   C.c_void_p, C.c_void_p]),
