@@ -266,13 +266,37 @@ int kg_gather(kg_ctx *ctx, int32_t ncols, const kg_column *cols, const int32_t *
 /* ---- hash join ----------------------------------------------------------
  * HashJoin (crates/krishiv-exec/src/join/mod.rs:1014-1211): the build side is
  * a Sink (kg_join_build_sink per batch, then kg_join_build_finalize,
- * join/build.rs:1225-1695); the probe side an Operator (kg_join_probe per
- * batch, join/probe.rs:784-948).  Output columns: the probe columns, then
- * (inner, left outer) every build column (join/mod.rs:793-819). */
+ * join/build.rs:1225-1695); the probe side an Operator (join/probe.rs:784-948)
+ * driven per driver through a local handle: kg_join_probe_push takes a batch,
+ * kg_join_probe_next returns its output in batches of at most max_rows rows
+ * (Cursor::next / pairs, probe.rs:266-501).  Once every probe batch is done,
+ * kg_join_tail_next returns the build rows the join type preserves
+ * (emit_build, probe.rs:611-632).  Spark's join types and NULL semantics
+ * (join/mod.rs:1-60,217-293):
+ *   INNER        pairs                       [probe cols, build cols]
+ *   LEFT_OUTER   pairs + unmatched probe rows (NULL build cols)
+ *   LEFT_SEMI    probe rows with a match     [probe cols]
+ *   LEFT_ANTI    probe rows without a match  [probe cols]
+ *   NULL_AWARE_ANTI  NOT IN (one `=` key, no residual filter): an empty build
+ *                keeps every probe row; a NULL build key keeps none; else a
+ *                probe row with a non-NULL key and no match [probe cols]
+ *   EXISTENCE    every probe row once        [probe cols, exists BOOL non-null]
+ *   RIGHT_SEMI   build rows with a match (tail)           [build cols]
+ *   RIGHT_ANTI   build rows without one, NULL keys too (tail) [build cols]
+ *   RIGHT_OUTER  pairs + unmatched build rows (tail; NULL probe cols)
+ *   FULL_OUTER   pairs + unmatched probe rows + unmatched build rows (tail)
+ * A residual filter (kg_join_set_filter) decides which key-equal pairs
+ * match, before any of the above. */
 #define KG_JOIN_INNER 0
 #define KG_JOIN_LEFT_OUTER 1
 #define KG_JOIN_LEFT_SEMI 2
 #define KG_JOIN_LEFT_ANTI 3
+#define KG_JOIN_NULL_AWARE_ANTI 4
+#define KG_JOIN_EXISTENCE 5
+#define KG_JOIN_RIGHT_SEMI 6
+#define KG_JOIN_RIGHT_ANTI 7
+#define KG_JOIN_RIGHT_OUTER 8
+#define KG_JOIN_FULL_OUTER 9
 
 typedef struct kg_join_spec {
   int32_t join_type;
@@ -293,12 +317,72 @@ int kg_join_create(kg_ctx *ctx, const kg_join_spec *spec, uint64_t expected_buil
                    kg_hash_join **out);
 int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows);
 int kg_join_build_finalize(kg_hash_join *j);
-/* Rows the build kept (NULL `=` keys dropped). */
+/* Rows sunk into the build side (all of them; those with a NULL `=` key
+ * are stored but never indexed). */
 int64_t kg_join_build_rows(const kg_hash_join *j);
+
+/* The residual filter (JoinSpec::filter, join/mod.rs:327-350; evaluated per
+ * key-equal pair, probe.rs:686+), as a postfix program over the pair's
+ * probe and build columns with Spark's three-valued logic: a pair matches
+ * when the program yields TRUE (NULL and FALSE do not).  Integer-like
+ * columns (integers, dates, timestamps, decimal64 as unscaled values) read
+ * as int64; + - * wrap at 64 bits; % takes the dividend's sign and is NULL
+ * for a zero divisor; comparisons with NULL are NULL; AND/OR are Kleene's.
+ * At most 32 ops, 8 distinct columns, a stack depth of 16.  Set before the
+ * first probe; not with NULL_AWARE_ANTI or the fused aggregate. */
+#define KG_X_PROBE 1 /* push probe column `arg` */
+#define KG_X_BUILD 2 /* push build column `arg` */
+#define KG_X_CONST 3 /* push `imm` */
+#define KG_X_ADD 4
+#define KG_X_SUB 5
+#define KG_X_MUL 6
+#define KG_X_MOD 7
+#define KG_X_EQ 8
+#define KG_X_NE 9
+#define KG_X_LT 10
+#define KG_X_LE 11
+#define KG_X_GT 12
+#define KG_X_GE 13
+#define KG_X_AND 14
+#define KG_X_OR 15
+#define KG_X_NOT 16
+#define KG_X_IS_NULL 17
+typedef struct kg_expr {
+  int32_t op;
+  int32_t arg;
+  int64_t imm;
+} kg_expr;
+int kg_join_set_filter(kg_hash_join *j, int32_t nops, const kg_expr *ops);
+
+/* A driver's probe state (the Operator's per-driver LocalState,
+ * crates/krishiv-exec/src/operator.rs:41-44): its own HIP stream and
+ * buffers over the shared, finalized build.  Distinct locals of one join may
+ * be used from different threads at once. */
+typedef struct kg_join_local kg_join_local;
+int kg_join_local_create(kg_hash_join *j, kg_join_local **out);
+void kg_join_local_destroy(kg_join_local *l);
+/* Operator::execute's input: a probe batch (all rows, or the rows of a
+ * selection vector: device row ids, the runtime filter's survivors, the
+ * fused RowSelect of join/mod.rs:641-657).  Host columns are copied to HBM
+ * before the call returns; device columns and sel_rows must stay valid
+ * until kg_join_probe_next reports done.  The previous batch must be done. */
+int kg_join_probe_push(kg_join_local *l, int32_t ncols, const kg_column *cols, int64_t nrows,
+                       const int32_t *sel_rows, int64_t nsel);
+/* The pushed batch's next output rows, at most max_rows (the bounded
+ * execute of operator.rs:7-13); *done = 1 once the batch's output is all
+ * returned (the last call may return a zero-row batch). */
+int kg_join_probe_next(kg_join_local *l, int64_t max_rows, kg_batch **out, int32_t *done);
+/* After every probe batch of every local: the build rows the join type
+ * returns by their matches (right semi: matched; right anti, right outer,
+ * full outer: unmatched, NULL `=` keys included), at most max_rows a call;
+ * the outer joins pad them with NULL probe columns of the given types.
+ * Other join types return done at once. */
+int kg_join_tail_next(kg_hash_join *j, int32_t nprobe_cols, const kg_column *probe_types,
+                      int64_t max_rows, kg_batch **out, int32_t *done);
+/* One batch's whole output at once on the join's own local (push + next
+ * without a bound). */
 int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                   kg_batch **out);
-/* The probe over the rows a selection vector keeps (device row ids, e.g. the
- * runtime filter's survivors): the fused RowSelect of join/mod.rs:641-657. */
 int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                            const int32_t *sel_rows, int64_t nsel, kg_batch **out);
 void kg_join_destroy(kg_hash_join *j);

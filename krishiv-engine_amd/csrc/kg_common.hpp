@@ -233,6 +233,8 @@ class Staged {
   }
   DCol add(const kg_column &c, int64_t nrows);
   bool staged_host() const { return !bufs_.empty(); }
+  // The staged copies, for a caller that reads them after this call.
+  std::vector<DevBuf> take_buffers() { return std::move(bufs_); }
 
  private:
   kg_ctx *ctx_;

@@ -55,6 +55,95 @@ struct JTable {
   uint64_t dspan;
 };
 
+// The residual filter as kernels evaluate it: the postfix program with each
+// column operand re-indexed into `cols` (side 0 probe, 1 build).
+constexpr int kFilterOps = 32, kFilterCols = 8, kFilterStack = 16;
+struct FilterProg {
+  int n;  // ops; 0: no filter
+  int ncols;
+  kg_expr op[kFilterOps];
+  DCol cols[kFilterCols];
+  int side[kFilterCols];
+};
+
+// Whether the pair (probe row prow, build row brow) passes the filter: the
+// program yields TRUE (Spark's three-valued logic; NULL does not pass).
+__device__ __noinline__ bool filter_pass(const FilterProg *__restrict__ Fp, int64_t prow,
+                                         int64_t brow) {
+  const FilterProg &F = *Fp;
+  int64_t v[kFilterStack];
+  uint32_t nul = 0;  // bit k: stack entry k is NULL
+  int sp = 0;
+  for (int k = 0; k < F.n; k++) {
+    const kg_expr e = F.op[k];
+    if (e.op == KG_X_PROBE || e.op == KG_X_BUILD) {
+      const DCol &c = F.cols[e.arg];
+      const int64_t r = F.side[e.arg] ? brow : prow;
+      const bool ok = dev::valid(c, r);
+      v[sp] = ok ? dev::load_i64(c, r) : 0;
+      nul = ok ? nul & ~(1u << sp) : nul | (1u << sp);
+      sp++;
+      continue;
+    }
+    if (e.op == KG_X_CONST) {
+      v[sp] = e.imm;
+      nul &= ~(1u << sp);
+      sp++;
+      continue;
+    }
+    if (e.op == KG_X_NOT) {
+      v[sp - 1] = !v[sp - 1];
+      continue;
+    }
+    if (e.op == KG_X_IS_NULL) {
+      v[sp - 1] = (nul >> (sp - 1)) & 1;
+      nul &= ~(1u << (sp - 1));
+      continue;
+    }
+    sp--;
+    const int64_t a = v[sp - 1], b = v[sp];
+    const bool an = (nul >> (sp - 1)) & 1, bn = (nul >> sp) & 1;
+    int64_t r = 0;
+    bool rn = false;
+    if (e.op == KG_X_AND) {  // FALSE wins over NULL
+      if ((!an && !a) || (!bn && !b)) r = 0;
+      else { r = 1; rn = an || bn; }
+    } else if (e.op == KG_X_OR) {  // TRUE wins over NULL
+      if ((!an && a) || (!bn && b)) r = 1;
+      else { r = 0; rn = an || bn; }
+    } else if (an || bn) {
+      rn = true;
+    } else {
+      const uint64_t ua = (uint64_t)a, ub = (uint64_t)b;
+      switch (e.op) {
+        case KG_X_ADD: r = (int64_t)(ua + ub); break;
+        case KG_X_SUB: r = (int64_t)(ua - ub); break;
+        case KG_X_MUL: r = (int64_t)(ua * ub); break;
+        case KG_X_MOD:
+          if (b == 0) rn = true;
+          else r = b == -1 ? 0 : a % b;
+          break;
+        case KG_X_EQ: r = a == b; break;
+        case KG_X_NE: r = a != b; break;
+        case KG_X_LT: r = a < b; break;
+        case KG_X_LE: r = a <= b; break;
+        case KG_X_GT: r = a > b; break;
+        default: r = a >= b; break;  // KG_X_GE
+      }
+    }
+    v[sp - 1] = r;
+    nul = rn ? nul | (1u << (sp - 1)) : nul & ~(1u << (sp - 1));
+  }
+  return !(nul & 1u) && v[0] != 0;
+}
+
+// Join types that return pairs, and those that mark the build rows they match.
+__device__ __host__ __forceinline__ bool emits_pairs(int jt) {
+  return jt == KG_JOIN_INNER || jt == KG_JOIN_LEFT_OUTER || jt == KG_JOIN_RIGHT_OUTER ||
+         jt == KG_JOIN_FULL_OUTER;
+}
+__device__ __host__ __forceinline__ bool marks_build(int jt) { return jt >= KG_JOIN_RIGHT_SEMI; }
+
 // Packed key of row i.  Returns 0 (ordinary), 1 (the EMPTY value), 2 (a
 // `<=>` NULL) or 3 (an `=` NULL: matches nothing).
 __device__ __forceinline__ int pack_key(const KeySpec &K, int64_t i, uint64_t &key) {
@@ -88,13 +177,14 @@ __global__ void k_join_init(Slot *slots, uint64_t n) {
     slots[s] = Slot{kEmpty, kNoRow, 0};
 }
 
-__global__ void k_join_build(KeySpec K, int64_t n, JTable T) {
+__global__ void k_join_build(KeySpec K, int64_t n, JTable T, unsigned long long *null_rows) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint64_t key;
     int cls = pack_key(K, i, key);
     if (cls == 3) {
       T.next[i] = kNoRow;
+      atomicAdd(null_rows, 1ull);  // a NOT IN build with a NULL keeps no probe row
       continue;  // an `=` key with a NULL is never indexed
     }
     uint64_t s;
@@ -147,9 +237,14 @@ __device__ __forceinline__ int64_t jfind(const JTable &T, int cls, uint64_t key)
 // past the first slot is followed row by row (rare at the table's load).
 constexpr int kJCU = 4;
 
-template <bool FAST>
+// With a residual filter, or for the joins that return build rows by their
+// matches (bmatch: one byte a build row, set by plain stores), a key's chain
+// is walked: m becomes the number of pairs that pass, and head is cleared
+// when none does.  naaj_all: NOT IN over an empty build keeps every row.
+template <bool FAST, bool WALK>
 __global__ void k_join_count(KeySpec K, int64_t n, const int32_t *sel, JTable T, int join_type,
-                             uint32_t *head, uint32_t *nout) {
+                             const FilterProg *F, uint8_t *bmatch, int naaj_all, uint32_t *head,
+                             uint32_t *nout) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x * kJCU;
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x * kJCU + threadIdx.x; i0 < n; i0 += stride) {
     int64_t row[kJCU];
@@ -225,12 +320,29 @@ __global__ void k_join_count(KeySpec K, int64_t n, const int32_t *sel, JTable T,
           m = sc[r];
         }
       }
+      if (WALK && h != kNoRow && i < n) {
+        uint32_t mp = 0;
+        uint32_t b = h;
+        for (uint32_t c = 0; c < m && b != kNoRow; c++, b = T.next[b]) {
+          if (F && !filter_pass(F, row[r], b)) continue;
+          mp++;
+          if (bmatch) bmatch[b] = 1;
+          else if (!emits_pairs(join_type)) break;  // one match decides
+        }
+        m = mp;
+        if (!mp) h = kNoRow;
+      }
       uint32_t o;
       switch (join_type) {
-        case KG_JOIN_INNER: o = m; break;
-        case KG_JOIN_LEFT_OUTER: o = m ? m : 1; break;
+        case KG_JOIN_INNER:
+        case KG_JOIN_RIGHT_OUTER: o = m; break;
+        case KG_JOIN_LEFT_OUTER:
+        case KG_JOIN_FULL_OUTER: o = m ? m : 1; break;
         case KG_JOIN_LEFT_SEMI: o = m ? 1 : 0; break;
-        default: o = m ? 0 : 1; break;  // anti
+        case KG_JOIN_LEFT_ANTI: o = m ? 0 : 1; break;
+        case KG_JOIN_NULL_AWARE_ANTI: o = naaj_all ? 1 : (cls[r] != 3 && !m) ? 1 : 0; break;
+        case KG_JOIN_EXISTENCE: o = 1; break;
+        default: o = 0; break;  // right semi / right anti: build rows only
       }
       if (i < n) {
         head[i] = h;
@@ -240,37 +352,82 @@ __global__ void k_join_count(KeySpec K, int64_t n, const int32_t *sel, JTable T,
   }
 }
 
-// Pass 2: the (probe row, build row) pairs; build row -1 pads.
-__global__ void k_join_pairs(int64_t n, const int32_t *sel, const uint32_t *head,
+// Pass 2: the (probe row, build row) pairs of output rows [w0, w1) of the
+// batch, written at their index - w0; probe rows [ia, ib) hold them.  Build
+// row -1 pads; an existence join's second column is its `exists` (1/0).
+template <bool FILT>
+__global__ void k_join_pairs(int64_t ia, int64_t ib, const int32_t *sel, const uint32_t *head,
                              const uint32_t *nout, const uint64_t *off, const uint32_t *next,
-                             int join_type, int32_t *pidx, int32_t *bidx) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+                             int join_type, const FilterProg *F, uint64_t w0, uint64_t w1,
+                             int32_t *pidx, int32_t *bidx) {
+  for (int64_t i = ia + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ib;
        i += (int64_t)gridDim.x * blockDim.x) {
-    uint32_t o = nout[i];
+    const uint32_t o = nout[i];
     if (!o) continue;
-    uint64_t p = off[i];
-    int32_t row = sel ? sel[i] : (int32_t)i;
-    if (join_type == KG_JOIN_LEFT_SEMI || join_type == KG_JOIN_LEFT_ANTI) {
-      pidx[p] = row;
-      if (bidx) bidx[p] = -1;
-      continue;
-    }
+    const uint64_t p = off[i];
+    if (p >= w1 || p + o <= w0) continue;
+    const int32_t row = sel ? sel[i] : (int32_t)i;
     uint32_t r = head[i];
-    if (r == kNoRow) {  // left outer, unmatched
-      pidx[p] = row;
-      bidx[p] = -1;
+    if (!emits_pairs(join_type)) {  // one output row, inside the window
+      pidx[p - w0] = row;
+      if (bidx) bidx[p - w0] = join_type == KG_JOIN_EXISTENCE ? (r != kNoRow) : -1;
       continue;
     }
-    for (uint32_t k = 0; k < o && r != kNoRow; k++, r = next[r]) {
-      pidx[p + k] = row;
-      bidx[p + k] = (int32_t)r;
+    if (r == kNoRow) {  // an outer join's unmatched probe row
+      pidx[p - w0] = row;
+      bidx[p - w0] = -1;
+      continue;
+    }
+    for (uint32_t k = 0; r != kNoRow && k < o; r = next[r]) {
+      if (FILT && !filter_pass(F, row, r)) continue;
+      const uint64_t q = p + k++;
+      if (q >= w1) break;
+      if (q >= w0) {
+        pidx[q - w0] = row;
+        bidx[q - w0] = (int32_t)r;
+      }
     }
   }
 }
 
+// The probe rows holding output rows [w0, w1): out[0] = the first row whose
+// outputs end past w0, out[1] = the first row whose outputs start at or past
+// w1 (off is non-decreasing).
+__global__ void k_join_window(const uint64_t *off, const uint32_t *nout, int64_t n, uint64_t w0,
+                              uint64_t w1, int64_t *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t m = (lo + hi) / 2;
+    if (off[m] + nout[m] > w0) hi = m;
+    else lo = m + 1;
+  }
+  out[0] = lo;
+  hi = n;
+  while (lo < hi) {
+    const int64_t m = (lo + hi) / 2;
+    if (off[m] >= w1) hi = m;
+    else lo = m + 1;
+  }
+  out[1] = lo;
+}
+
+// The tail of a join that returns build rows by their matches: flags of the
+// rows it returns, then their ids in build order.
+__global__ void k_tail_flags(const uint8_t *bmatch, int64_t n, int want_matched, uint32_t *flags) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x)
+    flags[r] = (bmatch[r] != 0) == (want_matched != 0);
+}
+__global__ void k_tail_list(const uint32_t *flags, const uint64_t *pos, int64_t n, int32_t *list) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x)
+    if (flags[r]) list[pos[r]] = (int32_t)r;
+}
+
 struct GatherCols {
   DCol src[kJoinCols];
-  int from_build[kJoinCols];
+  int from_build[kJoinCols];  // 0 probe column, 1 build column, 2 `exists` (from bidx)
   void *dst[kJoinCols];
   uint32_t *dst_valid[kJoinCols];  // NULL when the output column has no NULLs
   int n;
@@ -307,9 +464,16 @@ __global__ void k_join_gather(GatherCols G, int64_t nout, const int32_t *pidx,
        b0 += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = b0 + lane;
     const bool in = o < nout;
-    const int32_t pi = in ? pidx[o] : -1;
+    const int32_t pi = in && pidx ? pidx[o] : -1;
     const int32_t bi = in && bidx ? bidx[o] : -1;
     for (int c = 0; c < G.n; c++) {
+      if (G.from_build[c] == 2) {  // the existence join's BOOL column, never NULL
+        const uint64_t m = __ballot(in && bi == 1);
+        uint32_t *w = static_cast<uint32_t *>(G.dst[c]);
+        if (lane == 0) w[b0 >> 5] = (uint32_t)m;
+        if (lane == 32 && b0 + 32 < nout) w[(b0 >> 5) + 1] = (uint32_t)(m >> 32);
+        continue;
+      }
       const int32_t r = G.from_build[c] ? bi : pi;
       const DCol &s = G.src[c];
       if (r >= 0) copy_elem(s, r, G.dst[c], o);
@@ -922,6 +1086,23 @@ __global__ void __launch_bounds__(kJMBlock) k_jp_merge(JTable T, AggProg P, Spil
 
 using namespace kg;
 
+// A driver's probe state: its stream and buffers, and the batch it is
+// returning (kg_join_probe_push / kg_join_probe_next).
+struct kg_join_local {
+  kg_hash_join *j = nullptr;
+  hipStream_t s = nullptr;
+  DevBuf head, nout, off, temp, pidx, bidx, win;
+  bool active = false;
+  std::vector<DevBuf> held;  // host input columns staged to HBM
+  std::vector<DCol> d;
+  bool device_cols = false;
+  const int32_t *sel = nullptr;
+  int64_t np = 0;  // probe rows (the selected ones)
+  uint64_t total = 0, emitted = 0;
+  FilterProg F{};
+  DevBuf fprog;  // F in device memory (kernels read it through a pointer)
+};
+
 struct kg_hash_join {
   kg_ctx *ctx = nullptr;
   hipStream_t s = nullptr;
@@ -953,8 +1134,18 @@ struct kg_hash_join {
   OutProg out{};
   std::vector<kg_column> agg_types;
   DevBuf rec;
-  // probe scratch
-  DevBuf head, nout, off, temp, pidx, bidx;
+  // scan scratch of the fused aggregate and the tail
+  DevBuf temp;
+  // residual filter (column indices of the probe and build batches)
+  std::vector<kg_expr> filter;
+  // joins that return build rows by their matches: one byte a build row
+  DevBuf bmatch;
+  uint64_t null_rows = 0;  // build rows whose `=` key is NULL
+  bool probed = false;
+  std::unique_ptr<kg_join_local> own;  // the local of kg_join_probe
+  // the build-side tail
+  DevBuf tail_list;
+  int64_t tail_n = -1, tail_emitted = 0;
 };
 
 namespace kg {
@@ -1180,8 +1371,11 @@ int kg_join_create(kg_ctx *ctx, const kg_join_spec *spec, uint64_t expected_buil
     *out = nullptr;
     if (spec->num_keys < 1 || spec->num_keys > 2)
       fail(KG_ERR_UNSUPPORTED, "one or two join keys on the device");
-    if (spec->join_type < KG_JOIN_INNER || spec->join_type > KG_JOIN_LEFT_ANTI)
+    if (spec->join_type < KG_JOIN_INNER || spec->join_type > KG_JOIN_FULL_OUTER)
       fail(KG_ERR_UNSUPPORTED, "join type on the device");
+    if (spec->join_type == KG_JOIN_NULL_AWARE_ANTI &&
+        (spec->num_keys != 1 || spec->null_safe[0]))
+      fail(KG_ERR_UNSUPPORTED, "NOT IN takes one `=` key");
     if (spec->num_keys == 2 && (spec->null_safe[0] || spec->null_safe[1]))
       fail(KG_ERR_UNSUPPORTED, "null-safe keys in a two-key device join");
     auto j = std::make_unique<kg_hash_join>();
@@ -1201,7 +1395,9 @@ int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, in
     check_keys(j->spec, ncols, cols, j->spec.build_keys);
     if (j->build_rows + nrows > (int64_t)0x7FFFFFF0)
       fail(KG_ERR_UNSUPPORTED, "a build side of 2^31 rows or more (grace partitions first)");
-    if (nrows > 0) append_build(j, ncols, cols, nrows);
+    // a first batch of no rows still fixes the build schema (an empty build
+    // side's output columns)
+    if (nrows > 0 || j->bcols.empty()) append_build(j, ncols, cols, nrows);
   });
 }
 
@@ -1222,14 +1418,22 @@ int kg_join_build_finalize(kg_hash_join *j) {
     j->next.reset(j->ctx, (uint64_t)(n > 0 ? n : 1) * 4);
     k_join_init<<<stream_grid(j->cap + 2, 256), 256, 0, j->s>>>(j->slots.get<Slot>(), j->cap + 2);
     KG_HIP(hipGetLastError());
+    DevBuf nulls(j->ctx, 8);
+    KG_HIP(hipMemsetAsync(nulls.get(), 0, 8, j->s));
     if (n > 0) {
       std::vector<DCol> d(j->bcols.size());
       for (size_t c = 0; c < d.size(); c++) d[c] = build_dcol(j->bcols[c]);
       KeySpec K = key_spec(j, d.data(), j->spec.build_keys);
       ProfScope prof("join_build", j->s);
-      k_join_build<<<stream_grid(n, 256), 256, 0, j->s>>>(K, n, jtable(j));
+      k_join_build<<<stream_grid(n, 256), 256, 0, j->s>>>(K, n, jtable(j),
+                                                          nulls.get<unsigned long long>());
       KG_HIP(hipGetLastError());
     }
+    if (marks_build(j->spec.join_type)) {
+      j->bmatch.reset(j->ctx, (uint64_t)(n > 0 ? n : 1));
+      KG_HIP(hipMemsetAsync(j->bmatch.get(), 0, j->bmatch.bytes(), j->s));
+    }
+    KG_HIP(hipMemcpyAsync(&j->null_rows, nulls.get(), 8, hipMemcpyDeviceToHost, j->s));
     build_direct(j);
     KG_HIP(hipStreamSynchronize(j->s));
     j->finalized = true;
@@ -1238,105 +1442,305 @@ int kg_join_build_finalize(kg_hash_join *j) {
 
 int64_t kg_join_build_rows(const kg_hash_join *j) { return j ? j->build_rows : 0; }
 
-static void probe_impl(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
-                       const int32_t *sel, int64_t nsel, kg_batch **out) {
-  {
-    *out = nullptr;
-    if (!j->finalized) fail(KG_ERR_INTERNAL, "probe before the build finalized");
-    if (ncols < 1 || ncols + (int)j->bcols.size() > kJoinCols)
-      fail(KG_ERR_UNSUPPORTED, "too many output columns");
-    if (nrows > INT32_MAX) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^31 rows or more");
-    check_keys(j->spec, ncols, cols, j->spec.probe_keys);
-    for (int c = 0; c < j->spec.num_keys; c++) {
-      int bt = j->bcols.empty() ? cols[j->spec.probe_keys[c]].type
-                                : j->bcols[j->spec.build_keys[c]].type;
-      if (cols[j->spec.probe_keys[c]].type != bt)
-        fail(KG_ERR_INTERNAL, "probe and build keys differ in type");
-    }
-    Staged st(j->ctx, j->s);
-    std::vector<DCol> d;
-    stage_all(st, ncols, cols, nrows, d);
-    int jt = j->spec.join_type;
-    bool emits_build = jt == KG_JOIN_INNER || jt == KG_JOIN_LEFT_OUTER;
-    int64_t total = 0;
-    int64_t np = sel ? nsel : nrows;  // probe rows (selected ones)
-    if (np > 0) {
-      KeySpec K = key_spec(j, d.data(), j->spec.probe_keys);
-      j->head.ensure(j->ctx, (uint64_t)np * 4);
-      j->nout.ensure(j->ctx, (uint64_t)np * 4);
-      j->off.ensure(j->ctx, (uint64_t)(np + 1) * 8);
-      {
-        ProfScope prof(j->dspan ? "join_probe_direct" : "join_probe", j->s);
-        const bool fast = K.n == 1 && type_width(K.c[0].type) == 8 && !is_float(K.c[0].type);
-        if (fast)
-          k_join_count<true><<<stream_grid(np, 256, 2 * kJCU), 256, 0, j->s>>>(
-              K, np, sel, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
-        else
-          k_join_count<false><<<stream_grid(np, 256, 2 * kJCU), 256, 0, j->s>>>(
-              K, np, sel, jtable(j), jt, j->head.get<uint32_t>(), j->nout.get<uint32_t>());
-        KG_HIP(hipGetLastError());
+}  // extern "C"
+
+namespace kg {
+
+// The stored filter with its column operands resolved against a probe
+// batch's columns and the build store: each distinct column once in cols.
+static FilterProg resolve_filter(kg_hash_join *j, const std::vector<DCol> &probe) {
+  FilterProg F{};
+  F.n = (int)j->filter.size();
+  for (int k = 0; k < F.n; k++) {
+    kg_expr e = j->filter[k];
+    if (e.op == KG_X_PROBE || e.op == KG_X_BUILD) {
+      const int side = e.op == KG_X_BUILD;
+      const int ncol = side ? (int)j->bcols.size() : (int)probe.size();
+      if (e.arg < 0 || e.arg >= ncol) fail(KG_ERR_INTERNAL, "filter column index out of range");
+      const DCol c = side ? build_dcol(j->bcols[e.arg]) : probe[e.arg];
+      if (!is_int_like(c.type) || c.width <= 0)
+        fail(KG_ERR_UNSUPPORTED, "residual filter over a column that is not integer-like");
+      int slot = -1;
+      for (int q = 0; q < F.ncols; q++)
+        if (F.side[q] == side && F.cols[q].values == c.values && F.cols[q].offset == c.offset)
+          slot = q;
+      if (slot < 0) {
+        if (F.ncols == kFilterCols) fail(KG_ERR_UNSUPPORTED, "a filter over more than 8 columns");
+        slot = F.ncols++;
+        F.cols[slot] = c;
+        F.side[slot] = side;
       }
-      scan_u32(j, j->nout.get<uint32_t>(), j->off.get<uint64_t>(), np);
-      uint64_t last_off = 0;
-      uint32_t last_n = 0;
-      KG_HIP(hipMemcpyAsync(&last_off, j->off.get<uint64_t>() + np - 1, 8,
-                            hipMemcpyDeviceToHost, j->s));
-      KG_HIP(hipMemcpyAsync(&last_n, j->nout.get<uint32_t>() + np - 1, 4,
-                            hipMemcpyDeviceToHost, j->s));
-      KG_HIP(hipStreamSynchronize(j->s));
-      total = (int64_t)(last_off + last_n);
+      e.arg = slot;
     }
-    auto b = std::make_unique<kg_batch>();
-    b->ctx = j->ctx;
-    b->nrows = total;
-    int nb = emits_build ? (int)j->bcols.size() : 0;
-    b->cols.resize(ncols + nb);
-    GatherCols G{};
-    G.n = ncols + nb;
-    uint64_t vb = (uint64_t)ceil_div(total, 32) * 4 + 4;
-    for (int c = 0; c < ncols + nb; c++) {
-      bool from_build = c >= ncols;
-      DCol src = from_build ? build_dcol(j->bcols[c - ncols]) : d[c];
-      auto &oc = b->cols[c];
-      oc.type = src.type;
-      oc.precision = src.precision;
-      oc.scale = src.scale;
-      int w = type_width(src.type);
-      uint64_t bytes = w == 0 ? vb : (uint64_t)total * w + 16;
-      oc.values.reset(j->ctx, bytes);
-      if (w == 0) KG_HIP(hipMemsetAsync(oc.values.get(), 0, bytes, j->s));
-      bool nullable = src.validity != nullptr || (from_build && jt == KG_JOIN_LEFT_OUTER);
-      if (nullable) {
-        oc.validity.reset(j->ctx, vb);
-        KG_HIP(hipMemsetAsync(oc.validity.get(), 0, vb, j->s));
-      }
-      G.src[c] = src;
-      G.from_build[c] = from_build;
-      G.dst[c] = oc.values.get();
-      // padded rows (build row -1) keep a 0 validity bit: NULL
-      G.dst_valid[c] = nullable ? oc.validity.get<uint32_t>() : nullptr;
-    }
-    if (total > 0) {
-      j->pidx.ensure(j->ctx, (uint64_t)total * 4);
-      if (emits_build) j->bidx.ensure(j->ctx, (uint64_t)total * 4);
-      k_join_pairs<<<stream_grid(np, 256), 256, 0, j->s>>>(
-          np, sel, j->head.get<uint32_t>(), j->nout.get<uint32_t>(), j->off.get<uint64_t>(),
-          j->next.get<uint32_t>(), jt, j->pidx.get<int32_t>(),
-          emits_build ? j->bidx.get<int32_t>() : nullptr);
-      KG_HIP(hipGetLastError());
-      ProfScope prof("join_gather", j->s);
-      k_join_gather<<<stream_grid(total, 256), 256, 0, j->s>>>(
-          G, total, j->pidx.get<int32_t>(), emits_build ? j->bidx.get<int32_t>() : nullptr);
-      KG_HIP(hipGetLastError());
-    }
-    KG_HIP(hipStreamSynchronize(j->s));
-    *out = b.release();
+    F.op[k] = e;
   }
+  return F;
+}
+
+// The output columns a probe batch (or the tail) returns, allocated for
+// `rows` rows; G gets the matching gather program.  probe: the probe
+// batch's columns (for the tail, typeless stand-ins whose rows are all NULL).
+static std::unique_ptr<kg_batch> out_batch(kg_hash_join *j, const std::vector<DCol> &probe,
+                                           bool tail, int64_t rows, GatherCols &G,
+                                           hipStream_t s) {
+  const int jt = j->spec.join_type;
+  const bool with_probe = !(jt == KG_JOIN_RIGHT_SEMI || jt == KG_JOIN_RIGHT_ANTI);
+  const bool with_build = emits_pairs(jt) || !with_probe;
+  const bool with_exists = jt == KG_JOIN_EXISTENCE;
+  const int np = with_probe ? (int)probe.size() : 0;
+  const int nb = with_build ? (int)j->bcols.size() : 0;
+  const int nc = np + nb + (with_exists ? 1 : 0);
+  if (nc > kJoinCols) fail(KG_ERR_UNSUPPORTED, "more than 32 output columns");
+  auto b = std::make_unique<kg_batch>();
+  b->ctx = j->ctx;
+  b->nrows = rows;
+  b->cols.resize(nc);
+  G = GatherCols{};
+  G.n = nc;
+  const uint64_t vb = (uint64_t)ceil_div(rows, 32) * 4 + 4;
+  for (int c = 0; c < nc; c++) {
+    const int kind = c < np ? 0 : c < np + nb ? 1 : 2;
+    DCol src{};
+    if (kind == 0) src = probe[c];
+    else if (kind == 1) src = build_dcol(j->bcols[c - np]);
+    else src.type = KG_T_BOOL;
+    auto &oc = b->cols[c];
+    oc.type = src.type;
+    oc.precision = src.precision;
+    oc.scale = src.scale;
+    const int w = type_width(src.type);
+    const uint64_t bytes = w == 0 ? vb : (uint64_t)rows * w + 16;
+    oc.values.reset(j->ctx, bytes);
+    if (w == 0) KG_HIP(hipMemsetAsync(oc.values.get(), 0, bytes, s));
+    const bool pads_build = jt == KG_JOIN_LEFT_OUTER || jt == KG_JOIN_FULL_OUTER;
+    const bool nullable = kind == 2 ? false
+                          : kind == 0 ? (tail || src.validity != nullptr)
+                                      : (src.validity != nullptr || pads_build);
+    if (nullable) {
+      oc.validity.reset(j->ctx, vb);
+      KG_HIP(hipMemsetAsync(oc.validity.get(), 0, vb, s));
+    }
+    G.src[c] = src;
+    G.from_build[c] = kind;
+    G.dst[c] = oc.values.get();
+    // padded rows (row -1) keep a 0 validity bit: NULL
+    G.dst_valid[c] = nullable ? oc.validity.get<uint32_t>() : nullptr;
+  }
+  return b;
+}
+
+static void check_probe_columns(kg_hash_join *j, int32_t ncols, const kg_column *cols,
+                                int64_t nrows) {
+  if (!j->finalized) fail(KG_ERR_INTERNAL, "probe before the build finalized");
+  if (ncols < 1 || ncols > kJoinCols) fail(KG_ERR_UNSUPPORTED, "1 to 32 probe columns");
+  if (nrows > INT32_MAX) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^31 rows or more");
+  check_keys(j->spec, ncols, cols, j->spec.probe_keys);
+  for (int c = 0; c < j->spec.num_keys; c++) {
+    int bt = j->bcols.empty() ? cols[j->spec.probe_keys[c]].type
+                              : j->bcols[j->spec.build_keys[c]].type;
+    if (cols[j->spec.probe_keys[c]].type != bt)
+      fail(KG_ERR_INTERNAL, "probe and build keys differ in type");
+  }
+}
+
+static void probe_push(kg_join_local *l, int32_t ncols, const kg_column *cols, int64_t nrows,
+                       const int32_t *sel, int64_t nsel) {
+  kg_hash_join *j = l->j;
+  if (l->active) fail(KG_ERR_INTERNAL, "the previous probe batch is not done");
+  check_probe_columns(j, ncols, cols, nrows);
+  if (sel && nsel > nrows) fail(KG_ERR_INTERNAL, "more selected rows than rows");
+  if (nsel > 0 && !sel) fail(KG_ERR_INTERNAL, "no selection vector");
+  j->probed = true;
+  l->held.clear();
+  {
+    Staged st(j->ctx, l->s);
+    stage_all(st, ncols, cols, nrows, l->d);
+    l->held = st.take_buffers();
+  }
+  l->device_cols = false;
+  for (int c = 0; c < ncols; c++) l->device_cols |= (cols[c].flags & KG_COL_DEVICE) != 0;
+  l->F = resolve_filter(j, l->d);
+  if (l->F.n) {
+    l->fprog.ensure(j->ctx, sizeof(FilterProg));
+    KG_HIP(hipMemcpyAsync(l->fprog.get(), &l->F, sizeof(FilterProg), hipMemcpyHostToDevice, l->s));
+  }
+  const FilterProg *fp = l->F.n ? l->fprog.get<FilterProg>() : nullptr;
+  const int jt = j->spec.join_type;
+  const int64_t np = sel ? nsel : nrows;
+  uint64_t total = 0;
+  const bool none = jt == KG_JOIN_NULL_AWARE_ANTI && j->null_rows > 0;  // NOT IN (.., NULL)
+  if (np > 0 && !none) {
+    KeySpec K = key_spec(j, l->d.data(), j->spec.probe_keys);
+    l->head.ensure(j->ctx, (uint64_t)np * 4);
+    l->nout.ensure(j->ctx, (uint64_t)np * 4);
+    l->off.ensure(j->ctx, (uint64_t)(np + 1) * 8);
+    l->temp.ensure(j->ctx, (uint64_t)(scan_partials(np) + 1) * 8);
+    const int naaj_all = j->build_rows == 0;
+    uint8_t *bm = marks_build(jt) ? j->bmatch.get<uint8_t>() : nullptr;
+    {
+      ProfScope prof(j->dspan ? "join_probe_direct" : "join_probe", l->s);
+      const bool fast = K.n == 1 && type_width(K.c[0].type) == 8 && !is_float(K.c[0].type);
+      const bool walk = fp || bm;
+      const unsigned grid = stream_grid(np, 256, 2 * kJCU);
+      uint32_t *hd = l->head.get<uint32_t>(), *no = l->nout.get<uint32_t>();
+      if (fast && walk)
+        k_join_count<true, true><<<grid, 256, 0, l->s>>>(K, np, sel, jtable(j), jt, fp, bm,
+                                                         naaj_all, hd, no);
+      else if (fast)
+        k_join_count<true, false><<<grid, 256, 0, l->s>>>(K, np, sel, jtable(j), jt, fp, bm,
+                                                          naaj_all, hd, no);
+      else if (walk)
+        k_join_count<false, true><<<grid, 256, 0, l->s>>>(K, np, sel, jtable(j), jt, fp, bm,
+                                                          naaj_all, hd, no);
+      else
+        k_join_count<false, false><<<grid, 256, 0, l->s>>>(K, np, sel, jtable(j), jt, fp, bm,
+                                                           naaj_all, hd, no);
+      KG_HIP(hipGetLastError());
+    }
+    uint64_t *tot = l->temp.get<uint64_t>() + scan_partials(np);
+    exclusive_scan(l->nout.get<uint32_t>(), l->off.get<uint64_t>(), np, l->temp.get<uint64_t>(),
+                   l->s, nullptr, tot);
+    KG_HIP(hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, l->s));
+    KG_HIP(hipStreamSynchronize(l->s));
+  }
+  l->sel = sel;
+  l->np = np;
+  l->total = total;
+  l->emitted = 0;
+  l->active = true;
+}
+
+static void probe_next(kg_join_local *l, int64_t max_rows, kg_batch **out, int32_t *done) {
+  kg_hash_join *j = l->j;
+  *out = nullptr;
+  *done = 0;
+  if (!l->active) fail(KG_ERR_INTERNAL, "no probe batch pushed");
+  if (max_rows < 1) fail(KG_ERR_INTERNAL, "max_rows must be positive");
+  const int jt = j->spec.join_type;
+  const uint64_t w0 = l->emitted;
+  const uint64_t w1 = std::min<uint64_t>(l->total, w0 + (uint64_t)max_rows);
+  const int64_t rows = (int64_t)(w1 - w0);
+  GatherCols G;
+  auto b = out_batch(j, l->d, false, rows, G, l->s);
+  if (rows > 0) {
+    l->win.ensure(j->ctx, 16);
+    k_join_window<<<1, 64, 0, l->s>>>(l->off.get<uint64_t>(), l->nout.get<uint32_t>(), l->np, w0,
+                                      w1, l->win.get<int64_t>());
+    KG_HIP(hipGetLastError());
+    int64_t range[2] = {0, 0};
+    KG_HIP(hipMemcpyAsync(range, l->win.get(), 16, hipMemcpyDeviceToHost, l->s));
+    KG_HIP(hipStreamSynchronize(l->s));
+    const bool two = emits_pairs(jt) || jt == KG_JOIN_EXISTENCE;
+    l->pidx.ensure(j->ctx, (uint64_t)rows * 4);
+    if (two) l->bidx.ensure(j->ctx, (uint64_t)rows * 4);
+    const FilterProg *fp = l->F.n ? l->fprog.get<FilterProg>() : nullptr;
+    const auto pairs = fp ? k_join_pairs<true> : k_join_pairs<false>;
+    pairs<<<stream_grid(range[1] - range[0], 256), 256, 0, l->s>>>(
+        range[0], range[1], l->sel, l->head.get<uint32_t>(), l->nout.get<uint32_t>(),
+        l->off.get<uint64_t>(), j->next.get<uint32_t>(), jt, fp, w0, w1, l->pidx.get<int32_t>(),
+        two ? l->bidx.get<int32_t>() : nullptr);
+    KG_HIP(hipGetLastError());
+    ProfScope prof("join_gather", l->s);
+    k_join_gather<<<stream_grid(rows, 256), 256, 0, l->s>>>(
+        G, rows, l->pidx.get<int32_t>(), two ? l->bidx.get<int32_t>() : nullptr);
+    KG_HIP(hipGetLastError());
+  }
+  KG_HIP(hipStreamSynchronize(l->s));
+  if (l->device_cols) note_column_use(j->ctx, l->s);
+  l->emitted = w1;
+  if (l->emitted >= l->total) {
+    *done = 1;
+    l->active = false;
+    l->held.clear();
+  }
+  *out = b.release();
+}
+
+}  // namespace kg
+
+extern "C" {
+
+int kg_join_set_filter(kg_hash_join *j, int32_t nops, const kg_expr *ops) {
+  return guarded([&] {
+    if (j->probed) fail(KG_ERR_INTERNAL, "the filter is set before the first probe");
+    if (j->has_agg) fail(KG_ERR_UNSUPPORTED, "a residual filter with the fused aggregate");
+    if (nops > 0 && j->spec.join_type == KG_JOIN_NULL_AWARE_ANTI)
+      fail(KG_ERR_UNSUPPORTED, "NOT IN with a residual filter");
+    if (nops < 0 || nops > kFilterOps) fail(KG_ERR_UNSUPPORTED, "a filter of more than 32 ops");
+    int depth = 0;
+    for (int k = 0; k < nops; k++) {
+      const int op = ops[k].op;
+      if (op == KG_X_PROBE || op == KG_X_BUILD || op == KG_X_CONST) depth++;
+      else if (op == KG_X_NOT || op == KG_X_IS_NULL) {
+        if (depth < 1) fail(KG_ERR_INTERNAL, "filter op without an operand");
+      } else if (op >= KG_X_ADD && op <= KG_X_OR) {
+        if (depth < 2) fail(KG_ERR_INTERNAL, "filter op without two operands");
+        depth--;
+      } else {
+        fail(KG_ERR_INTERNAL, "unknown filter op");
+      }
+      if (depth > kFilterStack) fail(KG_ERR_UNSUPPORTED, "a filter deeper than 16");
+    }
+    if (nops > 0 && depth != 1) fail(KG_ERR_INTERNAL, "a filter must leave one value");
+    j->filter.assign(ops, ops + nops);
+  });
+}
+
+int kg_join_local_create(kg_hash_join *j, kg_join_local **out) {
+  return guarded([&] {
+    *out = nullptr;
+    auto l = std::make_unique<kg_join_local>();
+    l->j = j;
+    l->s = make_stream(j->ctx->device);
+    *out = l.release();
+  });
+}
+
+void kg_join_local_destroy(kg_join_local *l) {
+  if (!l) return;
+  if (l->s) (void)hipStreamSynchronize(l->s);
+  hipStream_t s = l->s;
+  {
+    StreamScope kg_scope(s);
+    delete l;
+  }
+  destroy_stream(s);
+}
+
+int kg_join_probe_push(kg_join_local *l, int32_t ncols, const kg_column *cols, int64_t nrows,
+                       const int32_t *sel_rows, int64_t nsel) {
+  return guarded([&] {
+    StreamScope kg_scope(l->s);
+    probe_push(l, ncols, cols, nrows, sel_rows, sel_rows ? nsel : 0);
+  });
+}
+
+int kg_join_probe_next(kg_join_local *l, int64_t max_rows, kg_batch **out, int32_t *done) {
+  return guarded([&] {
+    StreamScope kg_scope(l->s);
+    probe_next(l, max_rows, out, done);
+  });
+}
+
+static kg_join_local *own_local(kg_hash_join *j) {
+  if (!j->own) {
+    j->own = std::make_unique<kg_join_local>();
+    j->own->j = j;
+    j->own->s = j->s;
+  }
+  return j->own.get();
 }
 
 int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                   kg_batch **out) {
-  return guarded([&] { StreamScope kg_scope(j->s); probe_impl(j, ncols, cols, nrows, nullptr, 0, out); });
+  return guarded([&] {
+    StreamScope kg_scope(j->s);
+    kg_join_local *l = own_local(j);
+    l->active = false;
+    probe_push(l, ncols, cols, nrows, nullptr, 0);
+    int32_t done = 0;
+    probe_next(l, INT64_MAX, out, &done);
+  });
 }
 
 int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
@@ -1344,8 +1748,73 @@ int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols
   return guarded([&] {
     StreamScope kg_scope(j->s);
     if (nsel > 0 && !sel_rows) fail(KG_ERR_INTERNAL, "no selection vector");
-    if (nsel > nrows) fail(KG_ERR_INTERNAL, "more selected rows than rows");
-    probe_impl(j, ncols, cols, nrows, sel_rows, nsel, out);
+    kg_join_local *l = own_local(j);
+    l->active = false;
+    probe_push(l, ncols, cols, nrows, sel_rows ? sel_rows : nullptr, sel_rows ? nsel : 0);
+    int32_t done = 0;
+    probe_next(l, INT64_MAX, out, &done);
+  });
+}
+
+int kg_join_tail_next(kg_hash_join *j, int32_t nprobe_cols, const kg_column *probe_types,
+                      int64_t max_rows, kg_batch **out, int32_t *done) {
+  return guarded([&] {
+    StreamScope kg_scope(j->s);
+    *out = nullptr;
+    *done = 1;
+    const int jt = j->spec.join_type;
+    if (!marks_build(jt)) return;
+    if (!j->finalized) fail(KG_ERR_INTERNAL, "tail before the build finalized");
+    if (max_rows < 1) fail(KG_ERR_INTERNAL, "max_rows must be positive");
+    if (nprobe_cols < 0 || nprobe_cols > kJoinCols) fail(KG_ERR_UNSUPPORTED, "probe columns");
+    const int64_t n = j->build_rows;
+    if (j->tail_n < 0) {  // the returned build rows, in build order
+      j->tail_n = 0;
+      if (n > 0) {
+        DevBuf flags(j->ctx, (uint64_t)n * 4), pos(j->ctx, (uint64_t)n * 8);
+        j->temp.ensure(j->ctx, (uint64_t)(scan_partials(n) + 1) * 8);
+        uint64_t *tot = j->temp.get<uint64_t>() + scan_partials(n);
+        k_tail_flags<<<stream_grid(n, 256), 256, 0, j->s>>>(j->bmatch.get<uint8_t>(), n,
+                                                             jt == KG_JOIN_RIGHT_SEMI,
+                                                             flags.get<uint32_t>());
+        KG_HIP(hipGetLastError());
+        exclusive_scan(flags.get<uint32_t>(), pos.get<uint64_t>(), n, j->temp.get<uint64_t>(), j->s,
+                       nullptr, tot);
+        uint64_t cnt = 0;
+        KG_HIP(hipMemcpyAsync(&cnt, tot, 8, hipMemcpyDeviceToHost, j->s));
+        KG_HIP(hipStreamSynchronize(j->s));
+        j->tail_list.reset(j->ctx, cnt * 4 + 4);
+        k_tail_list<<<stream_grid(n, 256), 256, 0, j->s>>>(flags.get<uint32_t>(),
+                                                            pos.get<uint64_t>(), n,
+                                                            j->tail_list.get<int32_t>());
+        KG_HIP(hipGetLastError());
+        j->tail_n = (int64_t)cnt;
+      }
+      j->tail_emitted = 0;
+    }
+    std::vector<DCol> probe(nprobe_cols);
+    for (int c = 0; c < nprobe_cols; c++) {
+      probe[c] = DCol{};
+      probe[c].type = probe_types[c].type;
+      probe[c].precision = probe_types[c].precision;
+      probe[c].scale = probe_types[c].scale;
+      probe[c].width = type_width(probe_types[c].type);
+      if (probe[c].width < 0) fail(KG_ERR_UNSUPPORTED, "probe column type");
+    }
+    const int64_t w0 = j->tail_emitted;
+    const int64_t rows = std::min<int64_t>(j->tail_n - w0, max_rows);
+    GatherCols G;
+    auto b = out_batch(j, probe, true, rows, G, j->s);
+    if (rows > 0) {
+      ProfScope prof("join_gather", j->s);
+      k_join_gather<<<stream_grid(rows, 256), 256, 0, j->s>>>(
+          G, rows, nullptr, j->tail_list.get<int32_t>() + w0);
+      KG_HIP(hipGetLastError());
+    }
+    KG_HIP(hipStreamSynchronize(j->s));
+    j->tail_emitted = w0 + rows;
+    *done = j->tail_emitted >= j->tail_n;
+    *out = b.release();
   });
 }
 
@@ -1356,6 +1825,7 @@ int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_t
     if (!j->finalized) fail(KG_ERR_INTERNAL, "aggregate before the build finalized");
     if (j->spec.join_type != KG_JOIN_INNER || j->spec.num_keys != 1)
       fail(KG_ERR_UNSUPPORTED, "the fused aggregate needs an inner join on one key");
+    if (!j->filter.empty()) fail(KG_ERR_UNSUPPORTED, "the fused aggregate with a residual filter");
     if (ninputs > kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 32 aggregate inputs");
     j->agg_types.assign(input_types, input_types + ninputs);
     std::vector<CheckedSum> checked;
@@ -1623,7 +2093,10 @@ void kg_join_destroy(kg_hash_join *j) {
   if (!j) return;
   if (j->s) (void)hipStreamSynchronize(j->s);
   hipStream_t s = j->s;
-  delete j;
+  {
+    StreamScope kg_scope(s);
+    delete j;
+  }
   destroy_stream(s);
 }
 

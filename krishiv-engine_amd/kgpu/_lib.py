@@ -22,6 +22,11 @@ COL_DEVICE = 1
 AGG_COUNT_STAR, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_AVG = 1, 2, 3, 4, 5, 6
 AGG_ANSI = 1
 JOIN_INNER, JOIN_LEFT_OUTER, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI = 0, 1, 2, 3
+JOIN_NULL_AWARE_ANTI, JOIN_EXISTENCE, JOIN_RIGHT_SEMI, JOIN_RIGHT_ANTI = 4, 5, 6, 7
+JOIN_RIGHT_OUTER, JOIN_FULL_OUTER = 8, 9
+# residual filter ops (postfix)
+(X_PROBE, X_BUILD, X_CONST, X_ADD, X_SUB, X_MUL, X_MOD, X_EQ, X_NE, X_LT, X_LE, X_GT, X_GE,
+ X_AND, X_OR, X_NOT, X_IS_NULL) = range(1, 18)
 
 WIDTH = {T_BOOL: 0, T_INT8: 1, T_UINT8: 1, T_INT16: 2, T_UINT16: 2, T_INT32: 4, T_UINT32: 4,
          T_FLOAT32: 4, T_DATE32: 4, T_INT64: 8, T_UINT64: 8, T_FLOAT64: 8, T_TIMESTAMP: 8,
@@ -44,6 +49,11 @@ class Column(C.Structure):
 
 class AggSpec(C.Structure):
     _fields_ = [("func", C.c_int32), ("input", C.c_int32)]
+
+
+class Expr(C.Structure):
+    """kg_expr: one op of a residual filter program."""
+    _fields_ = [("op", C.c_int32), ("arg", C.c_int32), ("imm", C.c_int64)]
 
 
 class JoinSpec(C.Structure):
@@ -108,6 +118,14 @@ _SIGS = {
     "kg_join_probe_selected": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64,
                                          C.c_void_p, C.c_int64, C.c_void_p]),
     "kg_join_destroy": (None, [C.c_void_p]),
+    "kg_join_set_filter": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "kg_join_local_create": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_join_local_destroy": (None, [C.c_void_p]),
+    "kg_join_probe_push": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                     C.c_int64]),
+    "kg_join_probe_next": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "kg_join_tail_next": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_void_p]),
     "kg_joinagg_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                     C.c_uint32]),
     "kg_joinagg_probe": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]),

@@ -17,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, JoinSpec, KgError, check, lib,
+from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, Expr, JoinSpec, KgError, check, lib,
                    T_BOOL, T_DECIMAL128, T_DECIMAL64, T_INT8, T_INT16, T_INT32, T_INT64,
                    T_UINT8, T_UINT16, T_UINT32, T_UINT64, T_FLOAT32, T_FLOAT64, T_DATE32,
                    T_TIMESTAMP)
@@ -438,6 +438,31 @@ class HashJoin:
         check(lib().kg_join_probe(self.h, len(cols), _cols(cols), nrows, C.byref(h)))
         return Batch(h, self.ctx)
 
+    def set_filter(self, program):
+        """The residual filter: a list of (op, arg, imm) postfix ops (X_*)."""
+        ops = (Expr * max(1, len(program)))(*[Expr(*x) for x in program])
+        check(lib().kg_join_set_filter(self.h, len(program), ops))
+
+    def local(self) -> "JoinLocal":
+        """A driver's probe state (its own stream), for bounded output."""
+        return JoinLocal(self)
+
+    def tail(self, probe_types, max_rows=1 << 62):
+        """The build rows the join type returns after every probe (right
+        semi/anti/outer, full outer), as a list of batches of at most
+        max_rows rows; [] for the other join types."""
+        pt = (Column * max(1, len(probe_types)))(*[_type_col(*t) for t in probe_types])
+        out = []
+        while True:
+            h = C.c_void_p()
+            done = C.c_int32(0)
+            check(lib().kg_join_tail_next(self.h, len(probe_types), pt, max_rows, C.byref(h),
+                                          C.byref(done)))
+            if h.value:
+                out.append(Batch(h, self.ctx))
+            if done.value:
+                return out
+
     def probe_selected(self, cols, sel: DeviceBuffer, nsel: int, nrows=None) -> Batch:
         nrows = cols[0].length if nrows is None else nrows
         h = C.c_void_p()
@@ -465,6 +490,52 @@ class HashJoin:
     def close(self):
         if self.h:
             lib().kg_join_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class JoinLocal:
+    """A driver's probe state over a finalized HashJoin (kg_join_local):
+    push a batch, then take its output in batches of at most max_rows."""
+
+    def __init__(self, join: HashJoin):
+        self.join = join
+        h = C.c_void_p()
+        check(lib().kg_join_local_create(join.h, C.byref(h)))
+        self.h = h
+        self._keep = None
+
+    def push(self, cols, nrows=None, sel: DeviceBuffer | None = None, nsel: int = 0):
+        nrows = cols[0].length if nrows is None else nrows
+        self._keep = (cols, sel)
+        check(lib().kg_join_probe_push(self.h, len(cols), _cols(cols), nrows,
+                                       C.c_void_p(sel.ptr) if sel is not None else None, nsel))
+
+    def next(self, max_rows: int):
+        """(Batch, done)."""
+        h = C.c_void_p()
+        done = C.c_int32(0)
+        check(lib().kg_join_probe_next(self.h, max_rows, C.byref(h), C.byref(done)))
+        return Batch(h, self.join.ctx), bool(done.value)
+
+    def probe_all(self, cols, max_rows: int, nrows=None, sel=None, nsel=0):
+        """Every output batch of one probe batch."""
+        self.push(cols, nrows, sel, nsel)
+        out = []
+        while True:
+            b, done = self.next(max_rows)
+            out.append(b)
+            if done:
+                return out
+
+    def close(self):
+        if self.h:
+            lib().kg_join_local_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -395,3 +395,196 @@ def test_partitioned_fused_aggregate_type_dispatch():
                 assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)) + 1e-300, (k, c, a, b)
             else:
                 assert int(a) == int(b), (k, c, a, b)
+
+
+# ---------------------------------------------------------------- every join type
+ALL_TYPES = [A.JOIN_INNER, A.JOIN_LEFT_OUTER, A.JOIN_LEFT_SEMI, A.JOIN_LEFT_ANTI,
+             A.JOIN_NULL_AWARE_ANTI, A.JOIN_EXISTENCE, A.JOIN_RIGHT_SEMI, A.JOIN_RIGHT_ANTI,
+             A.JOIN_RIGHT_OUTER, A.JOIN_FULL_OUTER]
+# the reference's VLess residual filter (tests/hash_join/main.rs:72-91) over
+# column 2 of each side: (p + b) % 7 != 0 AND p < b
+VLESS = [(A.X_PROBE, 2, 0), (A.X_BUILD, 2, 0), (A.X_ADD, 0, 0), (A.X_CONST, 0, 7),
+         (A.X_MOD, 0, 0), (A.X_CONST, 0, 0), (A.X_NE, 0, 0), (A.X_PROBE, 2, 0),
+         (A.X_BUILD, 2, 0), (A.X_LT, 0, 0), (A.X_AND, 0, 0)]
+NULLKEY = None
+
+
+def _side(rows, kdt):
+    """Rows (k1, k2, v) -> columns k1, k2 (nullable, of dtype kdt) and v (int64)."""
+    n = len(rows)
+    k1 = np.array([0 if r[0] is None else r[0] for r in rows], dtype=kdt)
+    k2 = np.array([0 if r[1] is None else r[1] for r in rows], dtype=kdt)
+    v = np.array([r[2] for r in rows], dtype=np.int64)
+    ok1 = np.array([r[0] is not None for r in rows], dtype=bool)
+    ok2 = np.array([r[1] is not None for r in rows], dtype=bool)
+    return (k1, ok1), (k2, ok2), (v, np.ones(n, dtype=bool))
+
+
+def expected_any(jt, pairs, probe, build):
+    """Output rows of kgo.join_ex's pairs, as the device returns them."""
+    op, ob = pairs
+    out = []
+    for i, j in zip(op.tolist(), ob.tolist()):
+        pr = [None] * len(probe) if i < 0 else [int(v[i]) if ok[i] else None for v, ok in probe]
+        br = [] if jt == A.JOIN_EXISTENCE else \
+            [None] * len(build) if j < 0 else [int(v[j]) if ok[j] else None for v, ok in build]
+        if jt in (A.JOIN_LEFT_SEMI, A.JOIN_LEFT_ANTI, A.JOIN_NULL_AWARE_ANTI):
+            out.append(tuple(pr))
+        elif jt == A.JOIN_EXISTENCE:
+            out.append(tuple(pr) + (bool(j),))
+        elif jt in (A.JOIN_RIGHT_SEMI, A.JOIN_RIGHT_ANTI):
+            out.append(tuple(br))
+        else:
+            out.append(tuple(pr) + tuple(br))
+    return sorted(out, key=lambda t: tuple((x is None, x or 0) for x in t))
+
+
+def batch_rows(batches):
+    out = []
+    for b in batches:
+        if b.num_rows == 0:
+            continue
+        cols = [b.column(i) for i in range(b.num_columns)]
+        for r in range(b.num_rows):
+            out.append(tuple((bool(v[r]) if v.dtype == bool else int(v[r])) if ok[r] else None
+                             for v, ok in cols))
+    return sorted(out, key=lambda t: tuple((x is None, x or 0) for x in t))
+
+
+def run_any(jt, keys, safe, filt, probe_rows, build_rows, max_rows, build_batches=2,
+            probe_batches=2, direct=False):
+    # one key: int64 (the untyped 8-byte fast path); two keys: int32 + int32,
+    # packed into 8 bytes
+    kdt, kt, okt = (np.int64, A.T_INT64, kgo.T_INT64) if keys == 1 else \
+        (np.int32, A.T_INT32, kgo.T_INT32)
+    types = [kt, kt, A.T_INT64]
+    P, B = _side(probe_rows, kdt), _side(build_rows, kdt)
+    j = kgpu.HashJoin(ctx(), jt, list(range(keys)), list(range(keys)), safe[:keys],
+                      expected_build_rows=64, direct_index=direct)
+    nb = len(build_rows)
+    if nb == 0:  # an empty build still gives its schema
+        j.build_sink([Col(t, v, ok) for t, (v, ok) in zip(types, B)], 0)
+    cuts = np.linspace(0, nb, build_batches + 1).astype(int)
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        if b > a:
+            j.build_sink([Col(t, v[a:b], ok[a:b]) for t, (v, ok) in zip(types, B)], b - a)
+    j.build_finalize()
+    if filt:
+        j.set_filter(VLESS)
+    loc = j.local()
+    got = []
+    npr = len(probe_rows)
+    cuts = np.linspace(0, npr, probe_batches + 1).astype(int)
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        if b > a:
+            for out in loc.probe_all([Col(t, v[a:b], ok[a:b]) for t, (v, ok) in zip(types, P)],
+                                     max_rows, b - a):
+                assert out.num_rows <= max_rows
+                got.append(out)
+    got += j.tail([(t, 0, 0) for t in types], max_rows)
+    kc = lambda side: [kgo.Col(t, v, ok) for t, (v, ok) in zip([okt, okt, kgo.T_INT64], side)]
+    pairs = kgo.join_ex(jt, kc(P)[:keys], kc(B)[:keys], safe[:keys],
+                        filt=VLESS if filt else None, probe_cols=kc(P), build_cols=kc(B))
+    return batch_rows(got), expected_any(jt, pairs, P, B)
+
+
+A_, B_, X_ = 1, 2, 3
+EDGE_PROBE = [(1, A_, 1), (1, A_, 9), (2, None, 3), (None, A_, 4), (None, None, 5), (7, X_, 6)]
+EDGE_BUILD = [(1, A_, 8), (1, A_, 2), (1, B_, 3), (2, None, 12), (None, None, 10), (None, A_, 11)]
+ALL_NULL = [(None, None, 1), (None, None, 2)]
+
+
+@pytest.mark.parametrize("jt", ALL_TYPES)
+def test_every_join_type_on_the_reference_edge_cases(jt):
+    """The reference's edge cases (tests/hash_join/main.rs:232-290): NULL keys,
+    duplicates on both sides, empty and all-NULL sides, one and two keys,
+    `=` and `<=>`, with and without the residual filter, output bounded at
+    5 rows a batch (the reference's tiny batch size)."""
+    say(f"join types: edge cases, type {jt}")
+    inputs = [(EDGE_PROBE, EDGE_BUILD), (EDGE_PROBE, []), ([], EDGE_BUILD),
+              (EDGE_PROBE, ALL_NULL), (ALL_NULL, EDGE_BUILD)]
+    for keys in (1, 2):
+        for s in (False, True):
+            if keys == 2 and s:
+                continue  # two null-safe keys are refused on the device
+            for filt in (False, True):
+                if jt == A.JOIN_NULL_AWARE_ANTI and (keys > 1 or s or filt):
+                    continue
+                for p, b in inputs:
+                    got, want = run_any(jt, keys, [s, s], filt, p, b, 5)
+                    assert got == want, (jt, keys, s, filt, len(p), len(b))
+
+
+def test_not_in_spark_null_semantics_on_device():
+    """tests/hash_join/main.rs:300-321 on the device."""
+    probe = [(1, None, 0), (2, None, 0), (None, None, 0)]
+    jt = A.JOIN_NULL_AWARE_ANTI
+    assert run_any(jt, 1, [False], False, probe, [], 7)[0] == [(1, None, 0), (2, None, 0),
+                                                                (None, None, 0)]
+    assert run_any(jt, 1, [False], False, probe, [(1, None, 0), (None, None, 0)], 7)[0] == []
+    assert run_any(jt, 1, [False], False, probe, [(1, None, 0)], 7)[0] == [(2, None, 0)]
+
+
+@pytest.mark.parametrize("jt", ALL_TYPES)
+def test_every_join_type_random_larger(jt):
+    """Random rows with NULLs and duplicates (keys in 0..2000, values in
+    0..20), one or two keys, `=`/`<=>`, with and without the filter, output
+    bounded at 777 rows a batch; the direct index on for one-key builds."""
+    rng = np.random.default_rng(100 + jt)
+
+    def rows(n, nk):
+        return [(None if rng.random() < 0.05 else int(rng.integers(0, nk)),
+                 None if rng.random() < 0.05 else int(rng.integers(0, 3)), int(rng.integers(0, 20)))
+                for _ in range(n)]
+
+    for case in range(4):
+        naaj = jt == A.JOIN_NULL_AWARE_ANTI
+        keys = 1 if naaj or case % 2 == 0 else 2
+        safe = [False, False] if naaj or keys == 2 else [bool(case & 2)] * 2
+        filt = (not naaj) and case >= 2
+        say(f"join types: random, type {jt}, case {case}")
+        probe, build = rows(6000, 2000), rows(3000, 2000)
+        if naaj and case == 3:
+            build = [r for r in build if r[0] is not None]  # a NOT IN build without NULLs
+        got, want = run_any(jt, keys, safe, filt, probe, build, 777, build_batches=3,
+                            probe_batches=3, direct=(keys == 1 and case == 1))
+        assert got == want, (jt, case)
+
+
+def test_locals_probe_concurrently_from_threads():
+    """Two drivers' locals over one finalized build, probing different
+    batches from two threads at once (ctypes releases the GIL during the
+    calls): each gets the rows it gets alone."""
+    import threading
+    rng = np.random.default_rng(9)
+    bk = rng.integers(0, 50_000, 200_000)
+    pay = np.arange(len(bk), dtype=np.int64)
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0], expected_build_rows=len(bk))
+    j.build_sink([Col(A.T_INT64, bk), Col(A.T_INT64, pay)])
+    j.build_finalize()
+    batches = [rng.integers(0, 60_000, 400_000) for _ in range(6)]
+    alone = []
+    for pk in batches:
+        out = j.local().probe_all([Col(A.T_INT64, pk)], 1 << 30)[0]
+        alone.append(np.sort(out.column(2)[0] * 100_000 + out.column(0)[0]))
+    got = [None] * len(batches)
+    err = []
+
+    def drive(which):
+        try:
+            loc = j.local()
+            for k in which:
+                outs = loc.probe_all([Col(A.T_INT64, batches[k])], 50_000)
+                v = np.concatenate([o.column(2)[0] * 100_000 + o.column(0)[0] for o in outs])
+                got[k] = np.sort(v)
+        except Exception as e:  # noqa: BLE001
+            err.append(e)
+
+    ts = [threading.Thread(target=drive, args=(w,)) for w in ([0, 2, 4], [1, 3, 5])]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not err, err
+    for a, b in zip(alone, got):
+        assert np.array_equal(a, b)
