@@ -1186,6 +1186,7 @@ struct kg_hash_agg {
   // ordered sums
   std::vector<CheckedSum> checked;
   uint64_t rows_seen = 0;
+  uint64_t pending_rows = 0;  // rows of the sink call in progress
   DevBuf scratch_slots, scratch_abs;
   bool finalized = false;
   int64_t ngroups = 0;
@@ -1231,6 +1232,51 @@ static uint64_t read_occupied(kg_hash_agg *a) {
   KG_HIP(hipMemcpyAsync(&v, a->occ.get(), 8, hipMemcpyDeviceToHost, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
   return v + a->merged_keys;
+}
+
+// ---- NeedMemory: a call checks the HBM its worst case may take against the
+// context's budget before it changes any state, so that a refusal leaves the
+// aggregate as it was and the same batch can be retried (operator.rs:144-161,
+// 314-329: NeedMemory{bytes} with the state unchanged).
+static void need_room(kg_ctx *ctx, uint64_t need, const char *what) {
+  const uint64_t held = ctx->in_use.load();
+  if (ctx->budget && held + need > ctx->budget)
+    fail(KG_ERR_MEMORY, std::string(what) + " needs " + std::to_string(need) +
+                            " more bytes of HBM (" + std::to_string(held) + " of " +
+                            std::to_string(ctx->budget) + " held)");
+}
+
+static uint64_t table_bytes(const kg_hash_agg *a, uint64_t cap) {
+  const uint64_t ns = a->direct ? 257 : cap + 2;
+  return (a->direct ? 257 : cap) * 8 + ns * (uint64_t)a->prog.W * 8;
+}
+
+// At most how many distinct keys rows can hold: a one-to-four-byte key has
+// few values.
+static uint64_t key_domain(const kg_hash_agg *a) {
+  const int w = type_width(a->key_type.type);
+  return w >= 1 && w <= 4 ? (1ull << (8 * w)) + 1 : ~0ull;
+}
+
+// The peak HBM a drain of nr spilled rows and nc spilled records may take
+// beyond what is held: their partitioned copies, the merge's scratch, and
+// the table grown for the worst case of every entry being a new key (at
+// most `new_keys`; the old table is held while the new one fills).
+static uint64_t drain_need(kg_hash_agg *a, uint64_t nr, uint64_t nc, uint64_t occupied,
+                           uint64_t new_keys, uint64_t spill_bytes = 0) {
+  if (!nr && !nc) return 0;
+  const uint64_t W = (uint64_t)a->prog.W;
+  const uint64_t grid = (uint64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256);
+  // the partitioned copies of the spill (its bytes, when given)
+  uint64_t need = (spill_bytes ? spill_bytes : (nr * a->fmt.rw + nc * (1 + W)) * 8) + 16;
+  need += grid * a->m_slots * (1 + W) * 8 + grid * kMergeRows * kMergeBlock * 8;
+  need += (64ull << 20);  // histograms, bases, counters
+  uint64_t want = std::min<uint64_t>(occupied + new_keys, occupied + nr + nc);
+  want = std::min<uint64_t>(want, key_domain(a));
+  uint64_t cap = a->cap;
+  while (cap / 4 * 3 < want) cap *= 2;
+  if (cap > a->cap) need += table_bytes(a, cap);
+  return need;
 }
 
 // Doubles the table (rehash of every key and record).
@@ -1471,6 +1517,9 @@ static void drain(kg_hash_agg *a) {
   read_spill_counts(a, nr, nc);
   if (nr == 0 && nc == 0) return;
   if (nr + nc >= (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "more than 2^32 spilled entries");
+  // everything the merge may take, checked before the spill is touched
+  need_room(a->ctx, drain_need(a, nr, nc, read_occupied(a), a->rows_seen + a->pending_rows),
+            "the merge of the aggregation spill");
   if (read_occupied(a) > a->cap / 2) grow(a, a->cap * 2);
   const int W = a->prog.W;
   // partitions of about 4096 inputs (their keys fit the merge's LDS table)
@@ -1562,7 +1611,9 @@ static void drain(kg_hash_agg *a) {
 
 // Whether no running sum of the checked op can leave its bound during this
 // batch, in any row order (the per-batch proof of accumulator/helpers.rs:40-75).
-static bool proven(kg_hash_agg *a, CheckedSum &c, const DCol &col, int64_t n) {
+static bool proven(kg_hash_agg *a, const CheckedSum &c, const DCol &col, int64_t n,
+                   double &batch_abs) {
+  batch_abs = 0.0;
   // 1: rows so far x the type's largest magnitude
   unsigned __int128 rows = (unsigned __int128)(a->rows_seen + (uint64_t)n);
   unsigned __int128 tb = c.type_bound;
@@ -1581,12 +1632,50 @@ static bool proven(kg_hash_agg *a, CheckedSum &c, const DCol &col, int64_t n) {
   KG_HIP(hipStreamSynchronize(a->s));
   double total = (c.cum_abs + batch) * (1.0 + 1e-9) + 1.0;
   double bound = (double)c.bound;
-  if (total < bound * (1.0 - 1e-9)) {
-    c.cum_abs += batch;
-    return true;
+  batch_abs = batch;
+  return total < bound * (1.0 - 1e-9);
+}
+
+// The worst-case HBM a sink call of nrows takes beyond what is held (host
+// columns are already staged by then): the spill buffers it may grow to,
+// and (when the spill may pass its budget during the call, or the batch is
+// replayed) the merge of a full spill.
+static uint64_t sink_need(kg_hash_agg *a, const kg_column *inputs, int64_t nrows, bool replay) {
+  uint64_t need = 0;
+  const uint64_t new_keys = std::min<uint64_t>(a->rows_seen + (uint64_t)nrows, key_domain(a));
+  if (!a->hashed) {
+    if (replay) need += (uint64_t)nrows * 8;
+    return need;
   }
-  c.cum_abs += batch;
-  return false;
+  uint64_t nr, nc;
+  read_spill_counts(a, nr, nc);
+  const uint64_t occupied = read_occupied(a);
+  if (replay) {  // drain, then every row into the table directly
+    need += (uint64_t)nrows * 8 + drain_need(a, nr, nc, occupied, new_keys);
+    uint64_t cap = a->cap;
+    while (cap / 4 * 3 < std::min<uint64_t>(occupied + nr + nc + (uint64_t)nrows, key_domain(a)))
+      cap *= 2;
+    if (cap > a->cap) need += table_bytes(a, cap);
+    return need;
+  }
+  uint64_t rw = a->fmt.rw;
+  if (rw == 0) {  // spill row layout before the first sink: key, validity, inputs
+    rw = 2;
+    for (int c = 0; c < a->nsink; c++) rw += inputs[c].type == KG_T_DECIMAL128 ? 2 : 1;
+  }
+  const uint64_t cw = 1 + (uint64_t)a->prog.W;
+  const int64_t maxb = (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256);
+  const uint64_t slack_rows = (uint64_t)maxb * (kP1Block / 64) * kWaveChunk;
+  const uint64_t recs = (uint64_t)maxb * a->p1_slots *
+                        ((uint64_t)ceil_div(nrows, maxb) / kEvictRows + 2);
+  const uint64_t spill_after = ((nr + (uint64_t)nrows + slack_rows) * rw + (nc + recs) * cw) * 8;
+  if (spill_after <= a->spill_budget) return need + spill_after;
+  // pieces drain between them: fresh spill buffers up to the budget, and the
+  // merge of a spill that size
+  // (a full spill holds at most spill_budget bytes of rows and records together)
+  const uint64_t full_rows = a->spill_budget / (rw * 8), full_recs = a->spill_budget / (cw * 8);
+  return need + a->spill_budget +
+         drain_need(a, full_rows, full_recs, occupied, new_keys, a->spill_budget);
 }
 
 static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, int64_t nrows) {
@@ -1607,8 +1696,12 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
     P.in[c] = st.add(inputs[c], nrows);
   }
   bool replay = false;
-  for (auto &c : a->checked)
-    if (!proven(a, c, P.in[c.col], nrows)) replay = true;
+  std::vector<double> batch_abs(a->checked.size(), 0.0);
+  for (size_t k = 0; k < a->checked.size(); k++)
+    if (!proven(a, a->checked[k], P.in[a->checked[k].col], nrows, batch_abs[k])) replay = true;
+  // NeedMemory before any state changes
+  need_room(a->ctx, sink_need(a, inputs, nrows, replay), "the aggregation batch");
+  a->pending_rows = (uint64_t)nrows;
   if (!replay) {
     P.skip_checked = 0;
     if (a->hashed) {
@@ -1650,6 +1743,8 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
     KG_HIP(hipGetLastError());
   }
   a->rows_seen += (uint64_t)nrows;
+  a->pending_rows = 0;
+  for (size_t k = 0; k < a->checked.size(); k++) a->checked[k].cum_abs += batch_abs[k];
   // host inputs were staged into buffers freed on return: finish with them
   if (st.staged_host()) KG_HIP(hipStreamSynchronize(a->s));
 }

@@ -1212,7 +1212,26 @@ __global__ void k_copy_bits(const uint8_t *src, int64_t src_off, int64_t n, uint
   }
 }
 
+// Appends a build batch.  Every buffer is allocated before anything is
+// changed, so a refusal for memory (NeedMemory) leaves the build as it was
+// and the same batch can be sunk again (operator.rs:314-329).
 static void append_build(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows) {
+  if (!j->bcols.empty() && (int)j->bcols.size() != ncols)
+    fail(KG_ERR_INTERNAL, "build batches differ in columns");
+  for (int c = 0; c < (int)j->bcols.size(); c++)
+    if (cols[c].type != j->bcols[c].type) fail(KG_ERR_INTERNAL, "build column type changed");
+  Staged st(j->ctx, j->s);
+  std::vector<DCol> d;
+  stage_all(st, ncols, cols, nrows, d);
+  std::vector<DevBuf> vals(ncols), valids(ncols);
+  for (int c = 0; c < ncols; c++) {
+    const int w = type_width(cols[c].type);
+    // values: bit-packed booleans re-based to bit 0 as 32-bit words
+    const uint64_t vbytes = w == 0 ? (uint64_t)ceil_div(nrows, 32) * 4 : (uint64_t)nrows * w;
+    vals[c].reset(j->ctx, vbytes ? vbytes : 4);
+    if (d[c].validity) valids[c].reset(j->ctx, (uint64_t)ceil_div(nrows, 32) * 4 + 4);
+  }
+  // nothing below allocates: the batch is appended
   if (j->bcols.empty()) {
     j->bcols.resize(ncols);
     for (int c = 0; c < ncols; c++) {
@@ -1221,29 +1240,22 @@ static void append_build(kg_hash_join *j, int32_t ncols, const kg_column *cols, 
       j->bcols[c].scale = cols[c].scale;
     }
   }
-  if ((int)j->bcols.size() != ncols) fail(KG_ERR_INTERNAL, "build batches differ in columns");
-  Staged st(j->ctx, j->s);
-  std::vector<DCol> d;
-  stage_all(st, ncols, cols, nrows, d);
   for (int c = 0; c < ncols; c++) {
     auto &bc = j->bcols[c];
-    if (cols[c].type != bc.type) fail(KG_ERR_INTERNAL, "build column type changed");
-    int w = type_width(bc.type);
-    // values: bit-packed booleans re-based to bit 0 as 32-bit words
-    uint64_t vbytes = w == 0 ? (uint64_t)ceil_div(nrows, 32) * 4 : (uint64_t)nrows * w;
-    DevBuf v(j->ctx, vbytes ? vbytes : 4);
+    const int w = type_width(bc.type);
+    DevBuf &v = vals[c];
     if (w == 0) {
       KG_HIP(hipMemsetAsync(v.get(), 0, v.bytes(), j->s));
       k_copy_bits<<<stream_grid(nrows, 256), 256, 0, j->s>>>(
           static_cast<const uint8_t *>(d[c].values), d[c].offset, nrows, v.get<uint32_t>(), 0);
-    } else {
+    } else if (nrows > 0) {
       KG_HIP(hipMemcpyAsync(v.get(), static_cast<const uint8_t *>(d[c].values) + d[c].offset * w,
                             (size_t)nrows * w, hipMemcpyDeviceToDevice, j->s));
     }
     bc.chunks.emplace_back(std::move(v), nrows);
     if (d[c].validity) {
       bc.nullable = true;
-      DevBuf vb(j->ctx, (uint64_t)ceil_div(nrows, 32) * 4);
+      DevBuf &vb = valids[c];
       KG_HIP(hipMemsetAsync(vb.get(), 0, vb.bytes(), j->s));
       k_copy_bits<<<stream_grid(nrows, 256), 256, 0, j->s>>>(d[c].validity, d[c].offset, nrows,
                                                              vb.get<uint32_t>(), 0);
@@ -1265,16 +1277,37 @@ __global__ void k_set_bits(uint32_t *dst, int64_t off, int64_t n) {
   }
 }
 
-// Concatenates the chunks of every build column (the arena made contiguous).
-static void concat_build(kg_hash_join *j) {
-  int64_t n = j->build_rows;
-  for (auto &bc : j->bcols) {
+// The contiguous buffers of every build column, allocated before the chunks
+// are copied into them (concat_build), so finalize can refuse for memory
+// with the build unchanged.
+struct Arena {
+  std::vector<DevBuf> values, validity;
+};
+static Arena alloc_arena(kg_hash_join *j) {
+  const int64_t n = j->build_rows;
+  Arena A;
+  A.values.resize(j->bcols.size());
+  A.validity.resize(j->bcols.size());
+  for (size_t c = 0; c < j->bcols.size(); c++) {
+    const auto &bc = j->bcols[c];
+    const int w = type_width(bc.type);
+    const uint64_t vbytes = w == 0 ? (uint64_t)ceil_div(n, 32) * 4 : (uint64_t)n * w;
+    A.values[c].reset(j->ctx, vbytes + 16);
+    if (bc.nullable) A.validity[c].reset(j->ctx, (uint64_t)ceil_div(n, 32) * 4 + 4);
+  }
+  return A;
+}
+
+// Concatenates the chunks of every build column into the arena's buffers
+// (the row arena made contiguous).
+static void concat_build(kg_hash_join *j, Arena &A) {
+  for (size_t ci = 0; ci < j->bcols.size(); ci++) {
+    auto &bc = j->bcols[ci];
     int w = type_width(bc.type);
-    uint64_t vbytes = w == 0 ? (uint64_t)ceil_div(n, 32) * 4 : (uint64_t)n * w;
-    bc.values.reset(j->ctx, vbytes + 16);
+    bc.values = std::move(A.values[ci]);
     if (w == 0) KG_HIP(hipMemsetAsync(bc.values.get(), 0, bc.values.bytes(), j->s));
     if (bc.nullable) {
-      bc.validity.reset(j->ctx, (uint64_t)ceil_div(n, 32) * 4 + 4);
+      bc.validity = std::move(A.validity[ci]);
       KG_HIP(hipMemsetAsync(bc.validity.get(), 0, bc.validity.bytes(), j->s));
     }
     int64_t at = 0;
@@ -1406,19 +1439,26 @@ int kg_join_build_finalize(kg_hash_join *j) {
     StreamScope kg_scope(j->s);
     if (j->finalized) return;
     int64_t n = j->build_rows;
-    concat_build(j);
     uint64_t want = (uint64_t)(n > (int64_t)j->expected ? n : (int64_t)j->expected);
     if (want < 512) want = 512;
     // capacity for the tuned load factor (50 % unless tuned), a power of two
     const TuneCfg tc = tune_for(j->ctx, (int64_t)want);
     const uint64_t pct = (uint64_t)std::min(90, std::max(10, (int)tc.load_pct));
-    j->cap = 1ull << jlog2((want * 100 + pct - 1) / pct);
-    j->log2cap = jlog2(j->cap);
-    j->slots.reset(j->ctx, (j->cap + 2) * sizeof(Slot));
-    j->next.reset(j->ctx, (uint64_t)(n > 0 ? n : 1) * 4);
+    const uint64_t cap = 1ull << jlog2((want * 100 + pct - 1) / pct);
+    // every buffer first: a refusal for memory leaves the build unfinalized
+    // with its batches intact
+    Arena arena = alloc_arena(j);
+    DevBuf slots(j->ctx, (cap + 2) * sizeof(Slot)), next(j->ctx, (uint64_t)(n > 0 ? n : 1) * 4);
+    DevBuf nulls(j->ctx, 8), bmatch;
+    if (marks_build(j->spec.join_type)) bmatch.reset(j->ctx, (uint64_t)(n > 0 ? n : 1));
+    concat_build(j, arena);
+    j->cap = cap;
+    j->log2cap = jlog2(cap);
+    j->slots = std::move(slots);
+    j->next = std::move(next);
+    j->bmatch = std::move(bmatch);
     k_join_init<<<stream_grid(j->cap + 2, 256), 256, 0, j->s>>>(j->slots.get<Slot>(), j->cap + 2);
     KG_HIP(hipGetLastError());
-    DevBuf nulls(j->ctx, 8);
     KG_HIP(hipMemsetAsync(nulls.get(), 0, 8, j->s));
     if (n > 0) {
       std::vector<DCol> d(j->bcols.size());
@@ -1429,12 +1469,16 @@ int kg_join_build_finalize(kg_hash_join *j) {
                                                           nulls.get<unsigned long long>());
       KG_HIP(hipGetLastError());
     }
-    if (marks_build(j->spec.join_type)) {
-      j->bmatch.reset(j->ctx, (uint64_t)(n > 0 ? n : 1));
-      KG_HIP(hipMemsetAsync(j->bmatch.get(), 0, j->bmatch.bytes(), j->s));
-    }
+    if (j->bmatch.bytes()) KG_HIP(hipMemsetAsync(j->bmatch.get(), 0, j->bmatch.bytes(), j->s));
     KG_HIP(hipMemcpyAsync(&j->null_rows, nulls.get(), 8, hipMemcpyDeviceToHost, j->s));
-    build_direct(j);
+    // the direct index is an accelerator: without the memory for it the
+    // hashed table serves the probes
+    try {
+      build_direct(j);
+    } catch (const Error &e) {
+      if (e.code != KG_ERR_MEMORY) throw;
+      j->dspan = 0;
+    }
     KG_HIP(hipStreamSynchronize(j->s));
     j->finalized = true;
   });
