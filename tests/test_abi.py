@@ -50,3 +50,44 @@ def test_no_torch_types_in_header():
     text = open(os.path.join(ROOT, "include", "kgpu.h")).read()
     for bad in ("torch", "at::", "Tensor", "hipStream_t", "std::"):
         assert bad not in text
+
+
+def _c_decls():
+    """{function name: parameter count} and {struct: field count} of kgpu.h,
+    parsed here independently of the crate's generator."""
+    import re
+    text = open(os.path.join(ROOT, "include", "kgpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    fns = {}
+    for m in re.finditer(r"\b(kg_\w+)\s*\(([^;{}]*?)\)\s*;", text, flags=re.S):
+        args = " ".join(m.group(2).split())
+        fns[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    structs = {}
+    for m in re.finditer(r"typedef\s+struct\s+\w+\s*\{(.*?)\}\s*(\w+)\s*;", text, flags=re.S):
+        structs[m.group(2)] = len([f for f in m.group(1).split(";") if f.strip()])
+    return fns, structs
+
+
+def test_rust_sys_crate_declares_the_header():
+    """krishiv-gpu-sys (the crate a Rust build of the engine links) declares
+    every function of include/kgpu.h with the same number of parameters and
+    every struct with the same fields, and is what its generator writes from
+    the header today (regenerate after changing the header)."""
+    import importlib.util
+    import re
+    crate = os.path.join(ROOT, "krishiv-engine_amd", "rust", "krishiv-gpu-sys")
+    rs = open(os.path.join(crate, "src", "lib.rs")).read()
+    fns, structs = _c_decls()
+    got = {}
+    for m in re.finditer(r"pub fn (kg_\w+)\(([^)]*)\)", rs):
+        args = m.group(2).strip()
+        got[m.group(1)] = 0 if not args else args.count(",") + 1
+    assert got == fns
+    for name, nfields in structs.items():
+        m = re.search(r"pub struct " + name + r" \{(.*?)\n\}", rs, flags=re.S)
+        assert m, name
+        assert m.group(1).count("pub ") == nfields, name
+    spec = importlib.util.spec_from_file_location("gen_lib", os.path.join(crate, "gen_lib.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    assert gen.generate() == rs
