@@ -1,17 +1,27 @@
-"""Benchmark of the MI355X hash-aggregation hot path.
+"""Benchmark of the MI355X hash join / aggregate / Bloom / grace / served path.
 
-Workload (one step): GROUP BY an int64 key drawn Zipf(s=1.1) over 10M
-distinct values, 200M rows, with a decimal(12,2) column (int64 cents in
-[1, 1e7)), a float64 column in [0, 1) and an int32 column in [0, 50];
-aggregates COUNT(*) and SUM, MIN, MAX, AVG of each value column (13
-aggregates).  Inputs are generated once into HBM (synthetic, seeded); a step
-creates the aggregate, sinks the 200M rows, finalizes and materializes the
-result groups in HBM.
+With no --workload every configuration runs and one JSON line is printed:
+the headline is configuration 2 (GROUP BY an int64 key drawn Zipf(s=1.1)
+over 10M values, 200M rows, decimal(12,2) / float64 / int32 columns, COUNT(*)
+and SUM/MIN/MAX/AVG of each: 13 aggregates), and "configs" holds one record
+per other configuration, each with its roofline and its CPU baseline:
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--rows N]
-                    [--key int64|int8] [--cpu-rows N] [--no-cpu]
+  config1_join_sum     1M-row build x 8M-row probe, SELECT key, SUM(amt),
+                       COUNT(*) ... GROUP BY key (the fused join aggregate)
+  config2_agg_int8     the 200M-row aggregation over a 4-value int8 key
+  config3_bloom_join   15M-row build, 5 % kept, Bloom filter at fpp 0.03,
+                       600M probe rows screened, joined and summed
+  config4_grace        grace join + per-partition COUNT/SUM, 1.5B x 6B rows
+                       Zipf(0.8) over 1.2B keys, 64 partitions, 8 GB table
+                       budget (--scale shrinks it)
+  config5_served       10,000 join+aggregate requests, batched/served form
+                       against the plain one-launch-sequence-per-request form
 
-Prints one JSON line (rank 0).  Only one GPU is used; --gpus above 1 exits.
+Inputs are generated into HBM (synthetic, seeded) before each timed region.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload all|agg|...]
+
+Only one GPU is used; --gpus above 1 exits with a message.
 """
 from __future__ import annotations
 
@@ -38,16 +48,19 @@ def parse():
     p.add_argument("--distinct", type=int, default=10_000_000)
     p.add_argument("--key", choices=["int64", "int8"], default="int64")
     p.add_argument("--cpu-rows", type=int, default=20_000_000)
+    p.add_argument("--bloom-probe-rows", type=int, default=600_000_000)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workload", choices=["agg", "joinagg", "bloom", "serve", "grace", "calib"],
-                   default="agg",
-                   help="agg: the hash aggregation (default); joinagg: the join of "
+    p.add_argument("--workload", choices=["all", "agg", "joinagg", "bloom", "serve", "grace",
+                                          "calib"],
+                   default="all",
+                   help="all: every configuration, one line (default); agg: the hash "
+                        "aggregation; joinagg: the join of "
                         "SELECT key, SUM(amt), COUNT(*) ... GROUP BY key over 1M build rows; "
                         "bloom: the runtime Bloom filter feeding a join (15M build, 600M probe); "
                         "serve: the batched/served join-aggregate mix against the plain form; "
                         "grace: the grace join + per-partition aggregate at --scale of "
                         "configuration 4")
-    p.add_argument("--scale", type=float, default=0.1,
+    p.add_argument("--scale", type=float, default=1.0,
                    help="grace: fraction of configuration 4 (1.5B build, 6B probe rows)")
     p.add_argument("--grace-hbm-budget", type=int, default=0,
                    help="grace: context HBM budget in bytes (partitions past it go to pinned host)")
@@ -61,7 +74,8 @@ def parse():
                    help="serve: large-request worker threads (KG_SERVE_LARGE_WORKERS; 0: the default)")
     p.add_argument("--latency-out", default="",
                    help="serve: write each request's probe rows and latencies (npz) here")
-    p.add_argument("--probe-rows", type=int, default=200_000_000)
+    p.add_argument("--probe-rows", type=int, default=8_000_000,
+                   help="joinagg: probe rows (configuration 1: 8M)")
     p.add_argument("--build-rows", type=int, default=1_000_000,
                    help="joinagg: build rows (keys a permutation of 0..n-1; fk uniform over 1.2n); "
                         "4096 or fewer take the direct index")
@@ -85,90 +99,138 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": 1 << 30}), flush=True)
 
 
-def cpu_baseline(args, aggs):
-    """The oracle (the C restatement of the reference's CPU aggregate) on a
-    bounded sample of the same workload, run as the reference runs it:
-    thread-local tables over contiguous morsels, merged at finalize
-    (aggregate/mod.rs:11-21).  ctypes releases the GIL inside each C call."""
-    import concurrent.futures as cf
+def cpu_threads() -> int:
+    """Host threads of the CPU baselines: the GPU box gives one GPU 16 cores."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import kgo
+    return kgo
+
+
+def cpu_baseline(args, aggs, key="int64"):
+    """The oracle (the C restatement of the reference's CPU aggregate) on a
+    bounded sample of the same workload, run as the reference's pool runs
+    it: thread-local tables in 16 radix partitions over contiguous shares of
+    the rows, then a parallel merge per partition (aggregate/mod.rs:11-21,
+    rowtable.rs:1805; kgo_pagg_*)."""
+    kgo = oracle()
     n = args.cpu_rows
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     cdf = kgo.zipf_cdf(args.distinct, 1.1)
-    key, dec, f, i32, k8 = kgo.gen_agg(3, 0, n, cdf, args.distinct)
-    kcol = kgo.Col(kgo.T_INT64, key) if args.key == "int64" else kgo.Col(kgo.T_INT8, k8)
-    kt = kcol.type
-    kv = key if args.key == "int64" else k8
+    keyv, dec, f, i32, k8 = kgo.gen_agg(3, 0, n, cdf, args.distinct)
+    kt, kv = (kgo.T_INT64, keyv) if key == "int64" else (kgo.T_INT8, k8)
     types = [(kgo.T_DECIMAL64, 12, 2), (kgo.T_FLOAT64, 0, 0), (kgo.T_INT32, 0, 0)]
+    t0 = time.perf_counter()
+    a = kgo.AggPar((kt, 0, 0), types, aggs, threads)
+    a.update(kgo.Col(kt, kv), [kgo.Col(kgo.T_DECIMAL64, dec, precision=12, scale=2),
+                               kgo.Col(kgo.T_FLOAT64, f), kgo.Col(kgo.T_INT32, i32)])
+    a.finish(raw=True)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "rows/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} rows of the same generator ({key} key); the oracle's aggregate "
+                      f"on {threads} threads (thread-local tables in 16 partitions, parallel "
+                      f"merge per partition); {dt:.2f} s"}
+
+
+def cpu_join_pairs(kgo, probe_key, probe_valid, build_key, threads):
+    """The oracle's hash join of probe chunks on `threads` threads (ctypes
+    releases the GIL inside each C call): (probe rows, build rows)."""
+    import concurrent.futures as cf
+    n = len(probe_key)
     cuts = np.linspace(0, n, threads + 1).astype(np.int64)
+    bcol = [kgo.Col(kgo.T_INT64, build_key)]
 
     def part(t):
         lo, hi = int(cuts[t]), int(cuts[t + 1])
-        a = kgo.Agg((kt, 0, 0), types, aggs)
-        a.update(kgo.Col(kt, kv[lo:hi]),
-                 [kgo.Col(kgo.T_DECIMAL64, dec[lo:hi], precision=12, scale=2),
-                  kgo.Col(kgo.T_FLOAT64, f[lo:hi]), kgo.Col(kgo.T_INT32, i32[lo:hi])])
-        return a
+        pv = None if probe_valid is None else probe_valid[lo:hi]
+        pi, bi = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, probe_key[lo:hi], pv)], bcol,
+                          cap=(hi - lo) + 16)
+        return pi + lo, bi
 
-    t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(threads) as ex:
         parts = list(ex.map(part, range(threads)))
-    final = parts[0]
-    for p in parts[1:]:
-        final.merge(p)
-    final.finish()
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "rows/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} rows of the same generator; oracle C aggregate on {threads} "
-                      f"threads (per-thread tables merged at the end); {dt:.2f} s"}
+    return np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts])
 
 
 def cpu_joinagg():
-    """The oracle (C restatement of the reference's join and aggregate, one
-    thread) on SELECT key, SUM(amt), COUNT(*) over 1M build rows and the
-    first 8M probe rows of the same generators: the reference's own
-    CPU-runnable case of this workload."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import kgo
+    """The oracle on configuration 1 exactly (1M build rows, 8M probe rows):
+    the join of probe shares on the pool, then SUM(amt), COUNT(*) GROUP BY
+    key in the pool-shaped aggregate."""
+    kgo = oracle()
+    threads = cpu_threads()
     nb, n = 1_000_000, 8_000_000
     bk, _ = kgo.gen_join_build(1, 0, nb, nb)
     fk, ok, amt = kgo.gen_join_probe(2, 0, n, 1_200_000, 10)
     t0 = time.perf_counter()
-    pi, bi = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, fk, ok)], [kgo.Col(kgo.T_INT64, bk)],
-                      cap=n + 16)
-    a = kgo.Agg((kgo.T_INT64, 0, 0), [(kgo.T_INT64, 0, 0)], [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0)])
+    pi, bi = cpu_join_pairs(kgo, fk, ok, bk, threads)
+    a = kgo.AggPar((kgo.T_INT64, 0, 0), [(kgo.T_INT64, 0, 0)],
+                   [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0)], threads)
     a.update(kgo.Col(kgo.T_INT64, bk[bi]), [kgo.Col(kgo.T_INT64, amt[pi])])
-    a.finish()
+    a.finish(raw=True)
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"1M build rows and the first 8M probe rows of the same generators "
-                      f"(configuration 1's exact shape); oracle hash join + aggregate, one thread; "
-                      f"{dt:.2f} s"}
+    return {"value": n / dt, "unit": "rows/s", "cores": threads, "kind": "port",
+            "sample": f"configuration 1's exact shape (1M build, 8M probe rows, same generators); "
+                      f"the oracle's hash join on {threads} threads, then its pool-shaped "
+                      f"aggregate; {dt:.2f} s"}
 
 
 def cpu_bloom(nbytes):
-    """The oracle on the Bloom workload's first 20M probe rows: filter built
-    from the same 15M build rows (val < 50000), H_kg of each probe key, SBBF
-    check, then the survivors' join and SUM(amt), COUNT(*); one thread."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import kgo
+    """The oracle on the Bloom workload's first 20M probe rows: the filter
+    built from the same 15M build rows (val < 50000), H_kg and the SBBF check
+    of each probe key, the survivors' join and SUM(amt), COUNT(*), on the
+    pool (probe shares per thread)."""
+    import concurrent.futures as cf
+    kgo = oracle()
+    threads = cpu_threads()
     nb, n = 15_000_000, 20_000_000
     bk, bv = kgo.gen_join_build(4, 0, nb, nb)
     fk, ok, amt = kgo.gen_join_probe(5, 0, n, nb, 0)
     t0 = time.perf_counter()
     kept = bk[bv < 50_000]
     _, words = kgo.sbbf_build(kgo.hash_columns([kgo.Col(kgo.T_INT64, kept)]), nbytes=nbytes)
-    hit = kgo.sbbf_check_batch(words, kgo.hash_columns([kgo.Col(kgo.T_INT64, fk)]))
-    sel = np.nonzero(hit)[0]
-    pi, bi = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, fk[sel])], [kgo.Col(kgo.T_INT64, kept)])
-    a = kgo.Agg(None, [(kgo.T_INT64, 0, 0)], [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0)])
-    a.update(None, [kgo.Col(kgo.T_INT64, amt[sel][pi])])
-    a.finish()
+    cuts = np.linspace(0, n, threads + 1).astype(np.int64)
+
+    def part(t):
+        lo, hi = int(cuts[t]), int(cuts[t + 1])
+        hit = kgo.sbbf_check_batch(words, kgo.hash_columns([kgo.Col(kgo.T_INT64, fk[lo:hi])]))
+        sel = np.nonzero(hit)[0] + lo
+        pi, _ = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, fk[sel])],
+                         [kgo.Col(kgo.T_INT64, kept)])
+        return int(amt[sel][pi].sum()), len(pi)
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(part, range(threads)))
+    total = sum(p[0] for p in parts)
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"the same 15M build rows and the first 20M probe rows; oracle H_kg, SBBF "
-                      f"build/check, join and aggregate, one thread; {dt:.2f} s"}
+    return {"value": n / dt, "unit": "rows/s", "cores": threads, "kind": "port",
+            "sample": f"the same 15M build rows and the first 20M probe rows; the oracle's H_kg, "
+                      f"SBBF build and check, join and SUM/COUNT on {threads} threads; "
+                      f"{dt:.2f} s", "check_sum": total}
+
+
+def cpu_grace(scale):
+    """The oracle on configuration 4 at 1/1000 of its size (1.5M build x 6M
+    probe rows, Zipf(0.8) over 1.2M keys, payloads in [1, 1e4]): the
+    per-partition COUNT and SUM factorized per key (kgo_grace_factorized;
+    the reference's own pair walk over ~1e11 pairs at this size is out of
+    reach of a bounded sample), one thread."""
+    kgo = oracle()
+    nb, npr, nd = 1_500_000, 6_000_000, 1_200_000
+    cdf = kgo.zipf_cdf(nd, 0.8)
+    bk = kgo.gen_zipf_keys(6, 6, 0, nb, cdf, nd)
+    pk = kgo.gen_zipf_keys(7, 6, 0, npr, cdf, nd)
+    _, _, bp = kgo.gen_join_probe(8, 0, nb, 1, 0)
+    _, _, pp = kgo.gen_join_probe(9, 0, npr, 1, 0)
+    t0 = time.perf_counter()
+    c, _, _ = kgo.grace_factorized(kgo.Col(kgo.T_INT64, bk), kgo.Col(kgo.T_INT64, bp),
+                                   kgo.Col(kgo.T_INT64, pk), kgo.Col(kgo.T_INT64, pp), 64, 0)
+    dt = time.perf_counter() - t0
+    return {"value": (nb + npr) / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"configuration 4 at 1/1000 (1.5M x 6M rows, {int(c.sum())} pairs); the "
+                      f"oracle's per-key factorized grace aggregate, one thread; {dt:.2f} s"}
 
 
 def joinagg(args):
@@ -234,7 +296,7 @@ def joinagg(args):
     per_launch = in_bytes / max(sc_n / args.steps, 1)
     achieved = per_launch / (avg / 1e3) if sc_ms > 0 else None
     step_ms = dt * 1e3 / args.steps
-    print(json.dumps({
+    return {
         "metric": f"join probe rows/s (fused GROUP BY build key; {nb} build rows)",
         "value": n * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
@@ -247,7 +309,7 @@ def joinagg(args):
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None,
-                     "traffic": traffic_of(f"{rk_name}_joinagg_200M_traffic",
+                     "traffic": traffic_of(f"{rk_name}_joinagg_{n // 1_000_000}M_traffic",
                                            int(per_launch // 16.125) if sc_n else -1),
                      "kernel": rk_name, "kernel_avg_ms": avg,
                      "launches_per_step": sc_n / args.steps,
@@ -255,8 +317,9 @@ def joinagg(args):
         "step_hbm_frac": ((in_bytes + out_bytes) / (step_ms / 1e3)) / HBM_PEAK,
         "phases": phases(["join_build", "jp_hist", "jp_scatter", "jp_merge", "joinagg_direct",
                           "agg_output"]),
-        **({} if args.no_cpu or nb != 1_000_000 else {"cpu_baseline": cpu_joinagg()}),
-    }), flush=True)
+        **({} if args.no_cpu or nb != 1_000_000 or n != 8_000_000
+           else {"cpu_baseline": cpu_joinagg()}),
+    }
 
 
 def bloom(args):
@@ -270,7 +333,7 @@ def bloom(args):
     from kgpu import Col, DeviceBuffer
     from kgpu import abi as A
     ctx = kgpu.Context(0)
-    nb, n = 15_000_000, args.probe_rows if args.probe_rows != 200_000_000 else 600_000_000
+    nb, n = 15_000_000, args.bloom_probe_rows
     L = kgpu.lib()
     bk = DeviceBuffer(ctx, nb * 8)
     bv = DeviceBuffer(ctx, nb * 8)
@@ -328,7 +391,7 @@ def bloom(args):
     avg = bp_ms / max(bp_n, 1)
     per_launch = n * 8 / max(bp_n / args.steps, 1)  # the probe key column
     achieved = per_launch / (avg / 1e3) if bp_ms > 0 else None
-    print(json.dumps({
+    return {
         "metric": f"Bloom-screened probe rows/s (600M probe, 15M build, fpp 0.03, "
                   f"{'Spark murmur3' if spark else 'split-block'} layout)",
         "value": n * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
@@ -339,7 +402,9 @@ def bloom(args):
                    **stats},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK if achieved else None,
+                     "traffic": traffic_of("k_bloom_probe_bits_bloom_600M_traffic",
+                                           n // max(int(bp_n / args.steps), 1)),
                      "kernel": "bloom probe (k_bloom_probe_bits + expand)", "kernel_avg_ms": avg},
         "random_roofline": random_ceiling(
             ctx, (sbits + 7) // 8 if spark else nbytes, 8 if spark else 32,
@@ -350,7 +415,7 @@ def bloom(args):
             "probe rows per second against random 32-byte reads (one filter block a row)"),
         "families_ms_per_step": {f: v[0] / args.steps for f, v in fams.items()},
         **({} if args.no_cpu or spark else {"cpu_baseline": cpu_bloom(nbytes)}),
-    }), flush=True)
+    }
 
 
 def grace(args):
@@ -369,22 +434,23 @@ def grace(args):
     nb = int(1_500_000_000 * args.scale)
     npr = 4 * nb
     nd = int(1_200_000_000 * args.scale)
-    cdf = np.zeros(nd)
-    A.check(L.kg_zipf_cdf(nd, 0.8, cdf.ctypes.data))
-    cdf_d = DeviceBuffer.from_numpy(ctx, cdf)
-    del cdf
+    t_gen = time.perf_counter()
+    # the Zipf table of 1.2B ranks is built on the device (the host loop of
+    # kg_zipf_cdf over 1.2B terms would take tens of seconds)
+    cdf_d = DeviceBuffer(ctx, nd * 8)
+    A.check(L.kg_zipf_cdf_device(ctx.h, nd, 0.8, cdf_d.ptr))
     bk, bp = DeviceBuffer(ctx, nb * 8), DeviceBuffer(ctx, nb * 8)
     pk, pp = DeviceBuffer(ctx, npr * 8), DeviceBuffer(ctx, npr * 8)
     # one key distribution on both sides: the permutation of ranks to keys is
     # shared (seed 6), the draws differ
     A.check(L.kg_gen_zipf_keys(ctx.h, 6, 6, 0, nb, cdf_d.ptr, nd, bk.ptr))
     A.check(L.kg_gen_zipf_keys(ctx.h, 7, 6, 0, npr, cdf_d.ptr, nd, pk.ptr))
+    cdf_d.free()
     # payloads: the amt column of the probe generator, uniform in [1, 1e4]
-    scratch = DeviceBuffer(ctx, max(nb, npr) * 8)
-    A.check(L.kg_gen_join_probe(ctx.h, 8, 0, nb, 1, 0, scratch.ptr, None, bp.ptr))
-    A.check(L.kg_gen_join_probe(ctx.h, 9, 0, npr, 1, 0, scratch.ptr, None, pp.ptr))
-    del scratch, cdf_d
+    A.check(L.kg_gen_join_probe(ctx.h, 8, 0, nb, 1, 0, None, None, bp.ptr))
+    A.check(L.kg_gen_join_probe(ctx.h, 9, 0, npr, 1, 0, None, None, pp.ptr))
     ctx.synchronize()
+    t_gen = time.perf_counter() - t_gen
     batch = 1 << 26
     res = {}
 
@@ -425,7 +491,11 @@ def grace(args):
                          "probe rows per second of one partition (1/64 of the probe rows) "
                          "against random 32-byte reads from a table of a mean partition's size")
           if gp else None)
-    print(json.dumps({
+    sc_ms, sc_n = kgpu.prof_get("scatter")
+    sc_avg = sc_ms / max(sc_n, 1)
+    sc_rows = rows * args.steps / max(sc_n, 1)  # rows of one partition-pass launch
+    sc_ach = sc_rows * 16 / (sc_avg / 1e3) if sc_ms > 0 else None
+    return {
         "metric": "grace join + per-partition aggregate rows/s (build + probe rows)",
         "value": rows * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
@@ -435,10 +505,19 @@ def grace(args):
                                f"rows, Zipf(0.8) over {nd} keys, 64 partitions, table budget "
                                f"{int((8 << 30) * args.scale)} B",
                    "pairs": res.get("pairs"), "spilled_bytes": res.get("spilled")},
+        "roofline": {"bound": "hbm", "achieved": sc_ach / 1e9 if sc_ach else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": sc_ach / HBM_PEAK if sc_ach else None,
+                     "traffic": traffic_of("k_gp_scatter_grace_traffic", int(sc_rows)),
+                     "kernel": "k_gp_scatter (the partition pass: key and payload read once)",
+                     "kernel_avg_ms": sc_avg, "launches_per_step": sc_n / args.steps,
+                     "algorithmic_bytes_per_step": in_bytes},
         "step_hbm_frac": (in_bytes / (step_ms / 1e3)) / HBM_PEAK,
         "random_roofline": rr,
         "phases": ph,
-    }), flush=True)
+        "generation_s": t_gen,
+        **({} if args.no_cpu else {"cpu_baseline": cpu_grace(args.scale)}),
+    }
 
 
 def serve(args):
@@ -578,7 +657,14 @@ def serve(args):
                                                               "cache_hits", "releases")}}
     if args.latency_out:
         np.savez(args.latency_out, **dump)
-    print(json.dumps({
+    # algorithmic bytes of the mix: each request's build keys (8 B a row; the
+    # resident ones once), probe keys and payloads (16 B a row), result rows
+    # (24 B a group, at most the build rows)
+    build_rows = sum(nb_of(r) for r in range(R) if not resident[r]) + int(res_sizes.sum())
+    probe_rows = sum(10 * nb_of(r) for r in range(R))
+    mix_bytes = build_rows * 8 + probe_rows * 16 + sum(nb_of(r) for r in range(R)) * 24
+    bb = mix_bytes / out["batched"]["wall_s_excl_gen"]
+    return {
         "metric": "batched vs plain join-aggregate queries/s (saturated) and p50/p99 latency at a fixed QPS",
         "value": out["batched"]["queries_per_s"], "unit": "queries/s", "n_gpus": 1,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
@@ -590,7 +676,56 @@ def serve(args):
                    **({"autotuned_classes": tuned} if tuned else {})},
         "plain": out["plain"], "batched": out["batched"],
         "batched_over_plain_qps": out["batched"]["queries_per_s"] / out["plain"]["queries_per_s"],
-    }), flush=True)
+        "roofline": {"bound": "hbm", "achieved": bb / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": bb / HBM_PEAK, "traffic": None,
+                     "kernel": "the whole batched form, saturated (every kernel of the mix)",
+                     "algorithmic_bytes": mix_bytes},
+        **({} if args.no_cpu else {"cpu_baseline": cpu_serve(R)}),
+    }
+
+
+def cpu_serve(R, budget_s=10.0):
+    """The oracle running the served mix serially, in arrival order (the same
+    seeded sizes and generators): each request's join of probe shares on the
+    pool, then its pool-shaped SUM/COUNT GROUP BY key, until about budget_s
+    seconds of work; queries/s and the p50 of the per-request times."""
+    kgo = oracle()
+    threads = cpu_threads()
+    rng = np.random.default_rng(8)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e7), R)).astype(np.int64)
+    resident = rng.random(R) < 0.3
+    table_of = rng.integers(0, 8, R)
+    res_sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e7), 8)).astype(np.int64)
+    res_keys = {}
+    times = []
+    spent = 0.0
+    for r in range(R):
+        if resident[r]:
+            t = int(table_of[r])
+            nb = int(res_sizes[t])
+            if t not in res_keys:
+                res_keys[t] = kgo.gen_join_build(1000 + t, 0, nb, nb, with_val=False)[0]
+            bk = res_keys[t]
+        else:
+            nb = int(sizes[r])
+            bk = kgo.gen_join_build(2 * r + 1, 0, nb, nb, with_val=False)[0]
+        fk, _, pay = kgo.gen_join_probe(2 * r + 2, 0, 10 * nb, int(nb * 1.1), 0)
+        t0 = time.perf_counter()
+        pi, bi = cpu_join_pairs(kgo, fk, None, bk, threads)
+        a = kgo.AggPar((kgo.T_INT64, 0, 0), [(kgo.T_INT64, 0, 0)],
+                       [(kgo.AGG_SUM, 0), (kgo.AGG_COUNT_STAR, 0)], threads)
+        a.update(kgo.Col(kgo.T_INT64, bk[bi]), [kgo.Col(kgo.T_INT64, pay[pi])])
+        a.finish(raw=True)
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        spent += dt
+        if spent >= budget_s:
+            break
+    return {"value": len(times) / spent, "unit": "queries/s", "cores": threads, "kind": "port",
+            "p50_ms": float(np.percentile(times, 50) * 1e3),
+            "sample": f"the first {len(times)} requests of the same mix in arrival order, one at a "
+                      f"time; the oracle's join on {threads} threads and its pool-shaped "
+                      f"aggregate; {spent:.2f} s"}
 
 
 def random_ceiling(ctx, table_bytes: int, read_bytes: int, achieved_per_s: float, kernel: str,
@@ -668,33 +803,12 @@ def phases(names):
     return out
 
 
-def main():
-    args = parse()
-    if args.copy_calibration:
-        copy_calibration()
-        return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
-        sys.exit(2)
-    if args.workload == "joinagg":
-        joinagg(args)
-        return
-    if args.workload == "bloom":
-        bloom(args)
-        return
-    if args.workload == "serve":
-        serve(args)
-        return
-    if args.workload == "grace":
-        grace(args)
-        return
-    if args.workload == "calib":
-        calib(args)
-        return
+def agg(args, key="int64"):
+    """Configuration 2: the hash aggregation of 200M rows over the int64 Zipf
+    key, or (key int8) the 4-value int8 key."""
     import kgpu
     from kgpu import Col, DeviceBuffer
     from kgpu import abi as A
-    import ctypes as C
 
     ctx = kgpu.Context(0)
     n, nd = args.rows, args.distinct
@@ -708,9 +822,9 @@ def main():
     f_d = DeviceBuffer(ctx, n * 8)
     i32_d = DeviceBuffer(ctx, n * 4)
     A.check(kgpu.lib().kg_gen_agg(ctx.h, 3, 0, n, cdf_d.ptr, nd,
-                                  key_d.ptr if args.key == "int64" else None, dec_d.ptr, f_d.ptr,
-                                  i32_d.ptr, k8_d.ptr if args.key == "int8" else None))
-    if args.key == "int64":
+                                  key_d.ptr if key == "int64" else None, dec_d.ptr, f_d.ptr,
+                                  i32_d.ptr, k8_d.ptr if key == "int8" else None))
+    if key == "int64":
         kcol = Col(A.T_INT64, key_d, length=n)
         key_bytes = 8
     else:
@@ -722,7 +836,7 @@ def main():
     aggs = [(A.AGG_COUNT_STAR, 0)]
     for c in range(3):
         aggs += [(A.AGG_SUM, c), (A.AGG_MIN, c), (A.AGG_MAX, c), (A.AGG_AVG, c)]
-    expected = nd if args.key == "int64" else 4
+    expected = nd if key == "int64" else 4
 
     groups = [0]
 
@@ -757,8 +871,11 @@ def main():
     launches_per_step = upd_n / args.steps
     bytes_per_launch = in_bytes / max(launches_per_step, 1)
     achieved = bytes_per_launch / (upd_avg_ms / 1e3) if upd_ms > 0 else None
+    kname = "k_agg_pass1" if key == "int64" else "k_agg_update"
     out = {
-        "metric": "hash aggregation rows/s (200M rows, Zipf 1.1 over 10M int64 keys, 13 aggregates)",
+        "metric": "hash aggregation rows/s (200M rows, Zipf 1.1 over 10M int64 keys, 13 aggregates)"
+                  if key == "int64" else
+                  "hash aggregation rows/s (200M rows, 4-value int8 key, 13 aggregates)",
         "value": rows_per_s,
         "unit": "rows/s",
         "n_gpus": 1,
@@ -770,31 +887,79 @@ def main():
         "vs_baseline": None,
         "dtype": "int64/decimal(12,2)/f64/int32",
         "data": "synthetic (seeded counter-based generator, generated in HBM)",
-        "config": {"workload": f"hash aggregation, {n} rows, key {args.key} "
-                               f"{'Zipf(1.1) over %d' % nd if args.key == 'int64' else '4 values'}, "
+        "config": {"workload": f"hash aggregation, {n} rows, key {key} "
+                               f"{'Zipf(1.1) over %d' % nd if key == 'int64' else '4 values'}, "
                                "COUNT(*) + SUM/MIN/MAX/AVG of decimal(12,2), float64, int32",
                    "groups": groups[0]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None,
-                     "traffic": traffic_of("k_agg_pass1_hash_agg_200M_traffic",
-                                           int(bytes_per_launch // (key_bytes + 20)))
-                     if args.key == "int64" else None,
-                     "kernel": "k_agg_pass1" if args.key == "int64" else "k_agg_update",
+                     "traffic": traffic_of(f"{kname}_hash_agg_200M{'' if key == 'int64' else '_int8'}"
+                                           "_traffic", int(bytes_per_launch // (key_bytes + 20))),
+                     "kernel": kname,
                      "kernel_avg_ms": upd_avg_ms,
                      "launches_per_step": launches_per_step,
                      "algorithmic_bytes_per_step": in_bytes},
         # whole operator: inputs read once + result rows written once (key,
         # COUNT(*), and SUM 16 B, MIN, MAX, AVG of decimal(12,2), float64, int32:
         # 112 B a group)
-        "step_hbm_frac": ((in_bytes + groups[0] * (112 if args.key == "int64" else 105))
+        "step_hbm_frac": ((in_bytes + groups[0] * (112 if key == "int64" else 105))
                           / (ms_per_step / 1e3)) / HBM_PEAK,
         "phases": ph,
     }
+    del key_d, k8_d, dec_d, f_d, i32_d, cdf_d
+    ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, [(1, 0)] + [(f, c) for c in range(3)
-                                                             for f in (3, 4, 5, 6)])
-    print(json.dumps(out), flush=True)
+                                                             for f in (3, 4, 5, 6)], key)
+    return out
+
+
+def sub_record(rec):
+    """A configuration's record inside the headline line (the keys of a bench
+    line without the ones the headline already carries)."""
+    drop = {"n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"}
+    return {k: v for k, v in rec.items() if k not in drop}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(the multi-GPU path is design only)"}))
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    one = {"joinagg": lambda: joinagg(args), "bloom": lambda: bloom(args),
+           "serve": lambda: serve(args), "grace": lambda: grace(args),
+           "agg": lambda: agg(args, args.key)}
+    if args.workload == "calib":
+        calib(args)
+        return
+    if args.workload != "all":
+        print(json.dumps(one[args.workload]()), flush=True)
+        return
+    import gc
+    head = agg(args, "int64")
+    subs = {}
+    runs = [("config1_join_sum", lambda: joinagg(args)),
+            ("config2_agg_int8", lambda: agg(args, "int8")),
+            ("config3_bloom_join", lambda: bloom(args)),
+            ("config5_served", lambda: serve(args)),
+            ("config4_grace", lambda: grace(args))]
+    for name, fn in runs:
+        gc.collect()
+        t0 = time.perf_counter()
+        try:
+            subs[name] = sub_record(fn())
+        except Exception as e:  # a refused configuration is reported, not fatal
+            subs[name] = {"error": f"{type(e).__name__}: {e}"}
+        subs[name]["wall_s"] = time.perf_counter() - t0
+        print(f"[bench] {name} done in {subs[name]['wall_s']:.1f} s", file=sys.stderr,
+              flush=True)
+    head["configs"] = subs
+    print(json.dumps(head), flush=True)
 
 
 if __name__ == "__main__":

@@ -480,11 +480,16 @@ void kg_server_destroy(kg_server *s);
  * CPU generators used by the tests. */
 int kg_gen_join_build(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, int64_t n_total,
                       int64_t *key_dev, int64_t *val_dev);
+/* fk_dev, fk_validity_dev and amt_dev may each be NULL (not generated). */
 int kg_gen_join_probe(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, int64_t key_range,
                       uint32_t null_permille, int64_t *fk_dev, uint8_t *fk_validity_dev,
                       int64_t *amt_dev);
 /* The Zipf(s) cumulative distribution over n ranks (host, double). */
 int kg_zipf_cdf(uint64_t n, double s, double *cdf_host);
+/* The same table built on the device (for distributions of billions of
+ * ranks): the terms summed by a parallel scan, so its entries differ from
+ * kg_zipf_cdf's by rounding. */
+int kg_zipf_cdf_device(kg_ctx *ctx, uint64_t n, double s, double *cdf_dev);
 int kg_gen_agg(kg_ctx *ctx, uint64_t seed, int64_t lo, int64_t hi, const double *cdf_dev,
                uint64_t ndistinct, int64_t *key_dev, int64_t *dec_dev, double *f_dev,
                int32_t *i32_dev, int8_t *k8_dev);

@@ -9,6 +9,7 @@
 
 #include "kg_common.hpp"
 #include "kg_device.hpp"
+#include "kg_scan.hpp"
 
 namespace kg {
 
@@ -68,7 +69,7 @@ __global__ void k_gen_join_probe(uint64_t seed, int64_t lo, int64_t hi, int64_t 
       int64_t r = w * 32 + k;
       if (r >= n) break;
       uint64_t u = (uint64_t)(lo + r);
-      fk[r] = (int64_t)dev::below(dev::rand_at(b0, u), (uint64_t)key_range);
+      if (fk) fk[r] = (int64_t)dev::below(dev::rand_at(b0, u), (uint64_t)key_range);
       if (dev::below(dev::rand_at(b1, u), 1000) >= null_permille) bits |= 1u << k;
       if (amt) amt[r] = 1 + (int64_t)dev::below(dev::rand_at(b2, u), 10000);
     }
@@ -119,11 +120,41 @@ __global__ void k_gen_zipf_keys(uint64_t seed, uint64_t key_seed, int64_t lo, in
   }
 }
 
+// The Zipf table on the device: w[k] = (k+1)^-s in place, an exclusive
+// scan, then cdf[k] = (prefix + w[k]) / total with the last entry 1.
+__global__ void k_zipf_w(double *w, uint64_t n, double s) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    w[k] = pow((double)(k + 1), -s);
+}
+__global__ void k_zipf_norm(double *cdf, uint64_t n, double s, const double *total) {
+  const double inv = 1.0 / *total;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    cdf[k] = k + 1 == n ? 1.0 : (cdf[k] + pow((double)(k + 1), -s)) * inv;
+}
+
 }  // namespace kg
 
 using namespace kg;
 
 extern "C" {
+
+int kg_zipf_cdf_device(kg_ctx *ctx, uint64_t n, double s, double *cdf_dev) {
+  return guarded([&] {
+    if (n == 0) fail(KG_ERR_INTERNAL, "empty distribution");
+    StreamScope kg_scope(ctx->stream);
+    DevBuf partials(ctx, (uint64_t)(scan_partials((int64_t)n) + 1) * 8);
+    double *total = partials.get<double>() + scan_partials((int64_t)n);
+    k_zipf_w<<<stream_grid((int64_t)n, 256), 256, 0, ctx->stream>>>(cdf_dev, n, s);
+    KG_HIP(hipGetLastError());
+    exclusive_scan(cdf_dev, cdf_dev, (int64_t)n, partials.get<double>(), ctx->stream,
+                   nullptr, total);
+    k_zipf_norm<<<stream_grid((int64_t)n, 256), 256, 0, ctx->stream>>>(cdf_dev, n, s, total);
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
 
 int kg_gen_zipf_keys(kg_ctx *ctx, uint64_t seed, uint64_t key_seed, int64_t lo, int64_t hi,
                      const double *cdf_dev, uint64_t ndistinct, int64_t *key_dev) {

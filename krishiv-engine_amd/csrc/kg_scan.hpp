@@ -45,19 +45,20 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int64_t kScanSegment = (int64_t)kScanBlock * kScanItems;  // 4096 inputs a workgroup
 
-// Segment sums: partials[b] = sum of in[b * 4096, (b + 1) * 4096).
-template <class TI>
+// Segment sums: partials[b] = sum of in[b * 4096, (b + 1) * 4096), added in
+// TA (uint64 for counts; double for the Zipf table).
+template <class TI, class TA>
 __global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const TI *in, int64_t n,
-                                                            uint64_t *partials) {
-  __shared__ uint64_t s_w[kScanBlock / 64];
+                                                            TA *partials) {
+  __shared__ TA s_w[kScanBlock / 64];
   const int64_t base = (int64_t)blockIdx.x * kScanSegment;
-  uint64_t acc = 0;
+  TA acc = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
     const int64_t i = base + (int64_t)k * kScanBlock + threadIdx.x;
-    acc += i < n ? (uint64_t)in[i] : 0;
+    acc += i < n ? (TA)in[i] : (TA)0;
   }
-  uint64_t tot;
+  TA tot;
   (void)block_exclusive_scan<kScanBlock>(acc, s_w, &tot);
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
@@ -65,42 +66,43 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const TI *in, int64_
 // One 1024-lane workgroup turns the nb segment sums into exclusive offsets
 // in place, starting from *carry_in (0 without one); *total_out (if given)
 // gets the grand total including the carry.
-__global__ void __launch_bounds__(1024) k_scan_partials(uint64_t *partials, int64_t nb,
-                                                        const uint64_t *carry_in,
-                                                        uint64_t *total_out) {
-  __shared__ uint64_t s_w[16];
+template <class TA>
+__global__ void __launch_bounds__(1024) k_scan_partials(TA *partials, int64_t nb,
+                                                        const TA *carry_in, TA *total_out) {
+  __shared__ TA s_w[16];
   const int64_t per = (nb + 1023) / 1024;
   const int64_t lo = (int64_t)threadIdx.x * per;
   const int64_t hi = lo + per < nb ? lo + per : nb;
-  uint64_t mine = 0;
+  TA mine = 0;
   for (int64_t i = lo; i < hi; i++) mine += partials[i];
-  uint64_t tot;
-  uint64_t run = block_exclusive_scan<1024>(mine, s_w, &tot) + (carry_in ? *carry_in : 0);
+  TA tot;
+  TA run = block_exclusive_scan<1024>(mine, s_w, &tot) + (carry_in ? *carry_in : (TA)0);
   for (int64_t i = lo; i < hi; i++) {
-    const uint64_t v = partials[i];
+    const TA v = partials[i];
     partials[i] = run;
     run += v;
   }
-  if (threadIdx.x == 0 && total_out) *total_out = tot + (carry_in ? *carry_in : 0);
+  if (threadIdx.x == 0 && total_out) *total_out = tot + (carry_in ? *carry_in : (TA)0);
 }
 
 // Segment scans: out[i] = partials[segment] + the sum of the segment's
-// inputs before i.  Each lane owns kScanItems consecutive inputs.
-template <class TI>
+// inputs before i.  Each lane owns kScanItems consecutive inputs, read before
+// any is written (so out may be in when TI is TA).
+template <class TI, class TA>
 __global__ void __launch_bounds__(kScanBlock) k_scan_down(const TI *in, int64_t n,
-                                                          const uint64_t *partials, uint64_t *out) {
-  __shared__ uint64_t s_w[kScanBlock / 64];
+                                                          const TA *partials, TA *out) {
+  __shared__ TA s_w[kScanBlock / 64];
   const int64_t base = (int64_t)blockIdx.x * kScanSegment + (int64_t)threadIdx.x * kScanItems;
-  uint64_t v[kScanItems];
-  uint64_t acc = 0;
+  TA v[kScanItems];
+  TA acc = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
     const int64_t i = base + k;
-    v[k] = i < n ? (uint64_t)in[i] : 0;
+    v[k] = i < n ? (TA)in[i] : (TA)0;
     acc += v[k];
   }
-  uint64_t tot;
-  uint64_t run = block_exclusive_scan<kScanBlock>(acc, s_w, &tot) + partials[blockIdx.x];
+  TA tot;
+  TA run = block_exclusive_scan<kScanBlock>(acc, s_w, &tot) + partials[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
     const int64_t i = base + k;
@@ -114,18 +116,24 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_down(const TI *in, int64_t 
 // Segment sums a device-wide scan of n inputs needs (entries of uint64).
 inline int64_t scan_partials(int64_t n) { return ceil_div(n > 0 ? n : 1, kScanSegment); }
 
-// out[i] = carry + sum of in[0, i) for i < n (carry = *carry_in, or 0),
-// enqueued on s; partials must hold scan_partials(n) entries.  *total_out
-// (device, optional) gets carry + the sum of all n inputs.
-template <class TI>
-inline void exclusive_scan(const TI *in, uint64_t *out, int64_t n, uint64_t *partials,
-                           hipStream_t s, const uint64_t *carry_in = nullptr,
-                           uint64_t *total_out = nullptr) {
+// out[i] = carry + sum of in[0, i) for i < n (carry = *carry_in, or 0), the
+// sums taken in TA, enqueued on s; partials must hold scan_partials(n)
+// entries.  *total_out (device, optional) gets carry + the sum of all n
+// inputs.
+template <class T>
+struct same_type {
+  using type = T;
+};  // keeps the carry arguments out of template deduction (nullptr works)
+
+template <class TI, class TA>
+inline void exclusive_scan(const TI *in, TA *out, int64_t n, TA *partials, hipStream_t s,
+                           const typename same_type<TA>::type *carry_in = nullptr,
+                           typename same_type<TA>::type *total_out = nullptr) {
   if (n <= 0) return;
   const int64_t nb = scan_partials(n);
-  k_scan_reduce<TI><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, partials);
-  k_scan_partials<<<1, 1024, 0, s>>>(partials, nb, carry_in, total_out);
-  k_scan_down<TI><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, partials, out);
+  k_scan_reduce<TI, TA><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, partials);
+  k_scan_partials<TA><<<1, 1024, 0, s>>>(partials, nb, carry_in, total_out);
+  k_scan_down<TI, TA><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, partials, out);
   KG_HIP(hipGetLastError());
 }
 

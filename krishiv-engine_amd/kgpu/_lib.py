@@ -135,6 +135,7 @@ _SIGS = {
     "kg_gen_join_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                                     C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_zipf_cdf": (C.c_int, [C.c_uint64, C.c_double, C.c_void_p]),
+    "kg_zipf_cdf_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_void_p]),
     "kg_gen_agg": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_void_p,
                              C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_void_p]),

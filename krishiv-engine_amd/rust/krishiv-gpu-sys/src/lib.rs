@@ -250,6 +250,7 @@ extern "C" {
     pub fn kg_gen_join_build(ctx: *mut kg_ctx, seed: u64, lo: i64, hi: i64, n_total: i64, key_dev: *mut i64, val_dev: *mut i64) -> c_int;
     pub fn kg_gen_join_probe(ctx: *mut kg_ctx, seed: u64, lo: i64, hi: i64, key_range: i64, null_permille: u32, fk_dev: *mut i64, fk_validity_dev: *mut u8, amt_dev: *mut i64) -> c_int;
     pub fn kg_zipf_cdf(n: u64, s: f64, cdf_host: *mut f64) -> c_int;
+    pub fn kg_zipf_cdf_device(ctx: *mut kg_ctx, n: u64, s: f64, cdf_dev: *mut f64) -> c_int;
     pub fn kg_gen_agg(ctx: *mut kg_ctx, seed: u64, lo: i64, hi: i64, cdf_dev: *const f64, ndistinct: u64, key_dev: *mut i64, dec_dev: *mut i64, f_dev: *mut f64, i32_dev: *mut i32, k8_dev: *mut int8_t) -> c_int;
     pub fn kg_gen_zipf_keys(ctx: *mut kg_ctx, seed: u64, key_seed: u64, lo: i64, hi: i64, cdf_dev: *const f64, ndistinct: u64, key_dev: *mut i64) -> c_int;
     pub fn kg_prof_enable(on: c_int);
