@@ -35,6 +35,83 @@
 #include "kg_aggtile.hpp"
 
 namespace kg {
+// ---- few groups: a one-byte key or a global aggregate ----------------------
+// The used slots of a one-byte key (256 values and NULL at 256), as bits.
+__global__ void k_small_slots(AggProg P, int64_t nrows, unsigned int *bits) {
+  __shared__ unsigned int s_bits[9];
+  if (threadIdx.x < 9) s_bits[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrows;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t key;
+    const int cls = row_key(P, i, key);
+    const int slot = cls == 2 ? 256 : (int)(key & 0xFF);
+    if (!((s_bits[slot >> 5] >> (slot & 31)) & 1)) atomicOr(&s_bits[slot >> 5], 1u << (slot & 31));
+  }
+  __syncthreads();
+  if (threadIdx.x < 9 && s_bits[threadIdx.x]) atomicOr(&bits[threadIdx.x], s_bits[threadIdx.x]);
+}
+
+// Each workgroup keeps K copies of the D used records in LDS; lane l of
+// wave w writes copy (w * 4 + l % 4) % K, so LDS atomics of one instruction
+// meet on a record only when their rows share a key and a copy.  Rows are
+// applied op-major in tiles of R a lane (tile_apply); at the end the copies
+// are summed into copy 0 in LDS and each used record is merged into its
+// global record once.  remap: one-byte slot -> compact record (-1 unused);
+// slot_of: compact record -> slot.  A global aggregate has D = 1.
+constexpr int kSmallBlock = 256;
+template <int R>
+__global__ void __launch_bounds__(kSmallBlock) k_agg_small(AggProg P, Table T, int64_t nrows,
+                                                           int64_t chunk, const int16_t *remap,
+                                                           const int16_t *slot_of, int D,
+                                                           int K) {
+  extern __shared__ uint64_t smem[];
+  __shared__ int16_t s_map[257];
+  const int WL = P.W | 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  fill_lds_records(P, smem, K * D, WL);
+  if (P.direct)
+    for (int t = threadIdx.x; t < 257; t += blockDim.x) s_map[t] = remap[t];
+  __syncthreads();
+  uint64_t *mine = smem + (size_t)((wave * 4 + (lane & 3)) % K) * D * WL;
+  const int64_t base = blockIdx.x * chunk;
+  const int64_t end = base + chunk < nrows ? base + chunk : nrows;
+  for (int64_t t0 = base; t0 < end; t0 += (int64_t)R * kSmallBlock) {
+    int ls[R];
+    ColSrc<R> src;
+    src.safe = base;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t i = t0 + (int64_t)r * kSmallBlock + threadIdx.x;
+      src.idx[r] = i;
+      ls[r] = -1;
+      if (i < end) {
+        if (P.direct) {
+          uint64_t key;
+          const int cls = row_key(P, i, key);
+          ls[r] = s_map[cls == 2 ? 256 : (int)(key & 0xFF)];
+        } else {
+          ls[r] = 0;
+        }
+      }
+    }
+    tile_apply<R>(P, mine, WL, ls, src);
+  }
+  __syncthreads();
+  // copies 1..K-1 into copy 0 (one thread a record of a copy), then out
+  for (int k = 1; k < K; k++) {
+    for (int d = threadIdx.x; d < D; d += blockDim.x)
+      merge_record<true>(P, smem + (size_t)d * WL, smem + ((size_t)k * D + d) * WL);
+    __syncthreads();
+  }
+  for (int d = threadIdx.x; d < D; d += blockDim.x) {
+    const uint64_t *l = smem + (size_t)d * WL;
+    if (l[0] == 0) continue;  // no row
+    const int slot = P.direct ? slot_of[d] : 0;
+    merge_record(P, global_record(P, T, (P.direct && slot == 256) ? 2 : 0, (uint64_t)slot), l);
+  }
+}
+
 template <int SLOTS>
 __global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, int64_t nrows,
                                                           int64_t chunk) {
@@ -1187,6 +1264,7 @@ struct kg_hash_agg {
   std::vector<CheckedSum> checked;
   uint64_t rows_seen = 0;
   uint64_t pending_rows = 0;  // rows of the sink call in progress
+  DevBuf small_bits;          // the few-groups path: used-slot bits and its maps
   DevBuf scratch_slots, scratch_abs;
   bool finalized = false;
   int64_t ngroups = 0;
@@ -1327,12 +1405,61 @@ static void reserve(kg_hash_agg *a, int64_t n) {
   while (occ + (uint64_t)n > a->cap / 4 * 3) grow(a, a->cap * 2);
 }
 
+// The few-groups path (a one-byte key, or a global aggregate): the used
+// slots first (one-byte keys), then k_agg_small over every CU.
+static void launch_small(kg_hash_agg *a, const AggProg &Q, int64_t n) {
+  int D = 1;
+  int16_t remap[257], slot_of[257];
+  if (Q.direct) {
+    a->small_bits.ensure(a->ctx, 9 * 4 + 2 * 257 * 2 + 64);
+    KG_HIP(hipMemsetAsync(a->small_bits.get(), 0, 9 * 4, a->s));
+    k_small_slots<<<stream_grid(n, 256, 16), 256, 0, a->s>>>(Q, n, a->small_bits.get<unsigned>());
+    KG_HIP(hipGetLastError());
+    unsigned bits[9];
+    KG_HIP(hipMemcpyAsync(bits, a->small_bits.get(), 36, hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    D = 0;
+    for (int sl = 0; sl < 257; sl++) {
+      remap[sl] = -1;
+      if ((bits[sl >> 5] >> (sl & 31)) & 1) {
+        remap[sl] = (int16_t)D;
+        slot_of[D++] = (int16_t)sl;
+      }
+    }
+    if (D == 0) return;
+    int16_t *dev_map = reinterpret_cast<int16_t *>(a->small_bits.get<uint8_t>() + 64);
+    KG_HIP(hipMemcpyAsync(dev_map, remap, 257 * 2, hipMemcpyHostToDevice, a->s));
+    KG_HIP(hipMemcpyAsync(dev_map + 257, slot_of, (size_t)D * 2, hipMemcpyHostToDevice, a->s));
+  }
+  const int WL = Q.W | 1;
+  const size_t rec_bytes = (size_t)D * WL * 8;
+  // record copies: up to 4 a wave, within 48 KB of LDS a workgroup
+  int K = kSmallBlock / 64 * 4;
+  while (K > 1 && (size_t)K * rec_bytes > 48 * 1024) K--;
+  if ((size_t)K * rec_bytes > 64 * 1024) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide");
+  const int64_t cus = a->ctx->num_cus > 0 ? a->ctx->num_cus : 256;
+  constexpr int R = 4;
+  int64_t blocks = std::min<int64_t>(cus * 4, ceil_div(n, (int64_t)R * kSmallBlock * 4));
+  if (blocks < 1) blocks = 1;
+  const int64_t chunk = ceil_div(n, blocks);
+  const int16_t *dm = Q.direct ? reinterpret_cast<int16_t *>(a->small_bits.get<uint8_t>() + 64)
+                               : nullptr;
+  ProfScope prof("agg_update", a->s);
+  k_agg_small<R><<<(unsigned)blocks, kSmallBlock, (size_t)K * rec_bytes, a->s>>>(
+      Q, table_of(a), n, chunk, dm, dm ? dm + 257 : nullptr, D, K);
+  KG_HIP(hipGetLastError());
+}
+
 static void launch_update(kg_hash_agg *a, const AggProg &P, int64_t lo, int64_t n) {
   if (n <= 0) return;
   AggProg Q = P;
   // offset the columns to this piece of the batch
   for (int c = 0; c < kAggInputs; c++) Q.in[c].offset += lo;
   Q.key.offset += lo;
+  if (a->direct || !a->has_key) {
+    launch_small(a, Q, n);
+    return;
+  }
   Table t = table_of(a);
   int64_t chunk = 16384;
   int64_t blocks = ceil_div(n, chunk);

@@ -447,6 +447,8 @@ __device__ __forceinline__ void apply_row_mult(const AggProg &P, uint64_t *r, in
 }
 
 // Merges an LDS record into a global record.
+// Merges record l into g (global atomics; with LDS = true, LDS atomics).
+template <bool LDS = false>
 __device__ __forceinline__ void merge_record(const AggProg &P, uint64_t *g, const uint64_t *l) {
   for (int o = 0; o < P.nops; o++) {
     const AggOp op = op_at(P, o);
@@ -458,29 +460,29 @@ __device__ __forceinline__ void merge_record(const AggProg &P, uint64_t *g, cons
       case OP_ADDV:
       case OP_ADD64:
       case OP_COUNT_TRUE:
-        if (s[0]) add_u64_nr<false>(d, s[0]);
+        if (s[0]) add_u64_nr<LDS>(d, s[0]);
         break;
       case OP_STICKY:
-        if (l[op.st_word] == 2) max_u64<false>(g + op.st_word, 2);
+        if (l[op.st_word] == 2) max_u64<LDS>(g + op.st_word, 2);
         break;
       case OP_ADD128:
-        if (!(op.checked && P.skip_checked)) add128<false>(d, s[0], s[1]);
+        if (!(op.checked && P.skip_checked)) add128<LDS>(d, s[0], s[1]);
         break;
       case OP_ADD192:
-        if (!(op.checked && P.skip_checked)) add192<false>(d, s[0], s[1], s[2]);
+        if (!(op.checked && P.skip_checked)) add192<LDS>(d, s[0], s[1], s[2]);
         break;
-      case OP_ADDF64: add_f64<false>(d, __longlong_as_double((long long)s[0])); break;
+      case OP_ADDF64: add_f64<LDS>(d, __longlong_as_double((long long)s[0])); break;
       case OP_MIN_I64:
-        if (s[0] != P.init[op.word]) min_i64<false>(d, (int64_t)s[0]);
+        if (s[0] != P.init[op.word]) min_i64<LDS>(d, (int64_t)s[0]);
         break;
       case OP_MAX_I64:
-        if (s[0] != P.init[op.word]) max_i64<false>(d, (int64_t)s[0]);
+        if (s[0] != P.init[op.word]) max_i64<LDS>(d, (int64_t)s[0]);
         break;
       case OP_MIN_F:
-        if (s[0] != P.init[op.word]) min_u64<false>(d, s[0]);
+        if (s[0] != P.init[op.word]) min_u64<LDS>(d, s[0]);
         break;
       case OP_MAX_F:
-        if (s[0] != P.init[op.word]) max_u64<false>(d, s[0]);
+        if (s[0] != P.init[op.word]) max_u64<LDS>(d, s[0]);
         break;
       default: break;
     }
