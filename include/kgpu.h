@@ -416,7 +416,13 @@ int kg_joinagg_result(kg_hash_join *j, kg_batch **out);
  * (KRISHIV_UNSUPPORTED_FEATURE otherwise).  Sums wrap at 64 bits (Spark
  * non-ANSI bigint SUM).  Integer-like int64 keys and payloads. */
 typedef struct kg_grace kg_grace;
-int kg_grace_create(kg_ctx *ctx, uint32_t partitions, uint64_t table_budget_bytes, kg_grace **out);
+/* flags: KG_GRACE_SUM_MOD31 reads the SUM as ((sum of every pair's
+ * probe.pay * build.pay, wrapping at 2^64) mod 2^31), which factors per key
+ * for any payloads; without it, SUM((probe.pay * build.pay) % 2^31) per pair,
+ * accepted only when every product lies in [0, 2^31). */
+#define KG_GRACE_SUM_MOD31 1u
+int kg_grace_create(kg_ctx *ctx, uint32_t partitions, uint64_t table_budget_bytes,
+                    uint32_t flags, kg_grace **out);
 int kg_grace_build_sink(kg_grace *g, const kg_column *key, const kg_column *pay, int64_t nrows);
 int kg_grace_probe_sink(kg_grace *g, const kg_column *key, const kg_column *pay, int64_t nrows);
 /* Per output partition (host arrays of `partitions`): the pair count, the

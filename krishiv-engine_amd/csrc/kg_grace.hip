@@ -6,7 +6,8 @@
 //           in LDS and writes every partition's run with consecutive lanes;
 //           rows become [key, payload] pairs (NULL keys, which join nothing,
 //           are dropped; a NULL payload is stored as -1, never a payload
-//           the finish accepts).  The histogram pass also takes the payload
+//           the finish accepts; the mod 2^31 reading keeps only each
+//           payload's low 31 bits, on which it alone depends).  The histogram pass also takes the payload
 //           range for the factorization check.  Each sunk batch stays
 //           partitioned in HBM, or, once the context's HBM budget would be
 //           passed, in pinned host memory (the spill).
@@ -431,7 +432,7 @@ template <int R, bool W64>
 __global__ void __launch_bounds__(kGSBlock) k_gp_scatter(DCol key, DCol pay, int64_t n,
                                                          uint32_t parts, int64_t per_block,
                                                          const uint64_t *base, int64_t *out_key,
-                                                         int64_t *out_pay) {
+                                                         int64_t *out_pay, uint64_t pmask) {
   constexpr int TT = kGSBlock * R;
   extern __shared__ uint64_t smem[];
   uint64_t *st_k = smem;                                       // TT
@@ -465,7 +466,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gp_scatter(DCol key, DCol pay, int
         const int64_t i = t0 + (int64_t)r * kGSBlock + threadIdx.x, ic = i < hi ? i : hi - 1;
         const bool live = i < hi && vbit(key, ic, kb[r]);
         pp[r] = live ? dev::partition_of(dev::xxh64_fixed(kk[r], 0, 8), parts) : 0xffffffffu;
-        vv[r] = live ? (vbit(pay, ic, pb[r]) ? vv[r] : ~0ull) : 0;
+        vv[r] = live ? (vbit(pay, ic, pb[r]) ? vv[r] & pmask : ~0ull) : 0;
         kk[r] = live ? kk[r] : 0;
       }
     } else {
@@ -474,7 +475,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gp_scatter(DCol key, DCol pay, int
         const int64_t i = t0 + (int64_t)r * kGSBlock + threadIdx.x;
         const bool live = i < hi && dev::valid(key, i);
         kk[r] = live ? (uint64_t)dev::load_i64(key, i) : 0;
-        vv[r] = live ? (dev::valid(pay, i) ? (uint64_t)dev::load_i64(pay, i) : ~0ull) : 0;
+        vv[r] = live ? (dev::valid(pay, i) ? (uint64_t)dev::load_i64(pay, i) & pmask : ~0ull) : 0;
         pp[r] = live ? dev::partition_of(dev::value_hash(key, i), parts) : 0xffffffffu;
       }
     }
@@ -530,6 +531,7 @@ struct kg_grace {
   hipStream_t s = nullptr;
   uint32_t parts = 64;
   uint64_t table_budget = 0;
+  bool mod31 = false;  // SUM = ((sum of products) mod 2^64) mod 2^31
   // one partitioned piece per sunk batch: [key], [payload or -1 for NULL]
   // in HBM, or their copies in pinned host memory (hipHostMalloc)
   struct Piece {
@@ -647,13 +649,16 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
       {
         ProfScope prof("scatter", g->s);
         const size_t lds = gs_lds_bytes(R, parts);
+      // the mod 2^31 reading depends on the payloads' low 31 bits only (2^31
+      // divides 2^64): kept non-negative, so -1 still marks a NULL payload
+      const uint64_t pmask = g->mod31 ? 0x7FFFFFFFull : ~0ull;
 #define KG_GPS(RR, WW)                                                                        \
   do {                                                                                       \
     KG_HIP(hipFuncSetAttribute((const void *)k_gp_scatter<RR, WW>,                           \
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     k_gp_scatter<RR, WW><<<(unsigned)nb, kGSBlock, lds, g->s>>>(                             \
         dk, dv, m, parts, per_block, base.get<uint64_t>(), piece.key.get<int64_t>(),         \
-        piece.pay.get<int64_t>());                                                           \
+        piece.pay.get<int64_t>(), pmask);                                                    \
   } while (0)
         if (R == 12 && w64) KG_GPS(12, true);
         else if (R == 12) KG_GPS(12, false);
@@ -693,13 +698,16 @@ static void slice(const kg_grace::Piece &pc, uint32_t p, kg_column &k, kg_column
 
 extern "C" {
 
-int kg_grace_create(kg_ctx *ctx, uint32_t partitions, uint64_t table_budget_bytes, kg_grace **out) {
+int kg_grace_create(kg_ctx *ctx, uint32_t partitions, uint64_t table_budget_bytes,
+                    uint32_t flags, kg_grace **out) {
   return guarded([&] {
     *out = nullptr;
     if (partitions < 1 || partitions > 4096) fail(KG_ERR_UNSUPPORTED, "1 to 4096 partitions");
+    if (flags & ~KG_GRACE_SUM_MOD31) fail(KG_ERR_INTERNAL, "unknown grace flags");
     auto g = std::make_unique<kg_grace>();
     g->ctx = ctx;
     g->parts = partitions;
+    g->mod31 = (flags & KG_GRACE_SUM_MOD31) != 0;
     g->table_budget = table_budget_bytes ? table_budget_bytes : (8ull << 30);
     g->s = make_stream(ctx->device);
     StreamScope kg_scope(g->s);
@@ -728,10 +736,11 @@ int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_va
     bool any_b = mm[0] <= mm[1], any_p = mm[2] <= mm[3];
     if (any_b && any_p) {
       // every pair's product in [0, 2^31): the per-pair % 2^31 is the identity
-      bool nonneg = mm[0] >= 0 && mm[2] >= 0;
+      // (the mod 2^31 reading takes any payloads)
+      bool nonneg = g->mod31 || (mm[0] >= 0 && mm[2] >= 0);
       unsigned __int128 prod = (unsigned __int128)(uint64_t)(mm[1] > 0 ? mm[1] : 0) *
                                (unsigned __int128)(uint64_t)(mm[3] > 0 ? mm[3] : 0);
-      if (!nonneg || prod >= ((unsigned __int128)1 << 31))
+      if (!g->mod31 && (!nonneg || prod >= ((unsigned __int128)1 << 31)))
         fail(KG_ERR_UNSUPPORTED,
              "payload products reach 2^31 or are negative: the per-pair modulus does not factor");
     }
@@ -813,7 +822,7 @@ int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_va
     KG_HIP(hipMemcpy(h.data(), out.get(), h.size() * 8, hipMemcpyDeviceToHost));
     for (uint32_t p = 0; p < g->parts; p++) {
       counts[p] = (int64_t)h[3 * p];
-      sums[p] = (int64_t)h[3 * p + 2];
+      sums[p] = (int64_t)(g->mod31 ? (h[3 * p + 2] & 0x7FFFFFFFull) : h[3 * p + 2]);
       if (sum_valid) sum_valid[p] = h[3 * p + 1] != 0;
     }
     if (spilled_bytes) *spilled_bytes = g->spilled;

@@ -88,8 +88,11 @@ static uint64_t round_alloc(uint64_t b) {
 
 // Freed bytes kept for reuse, at most: hipFree waits for every stream of
 // the device, so blocks are returned to HIP only past this or when hipMalloc
-// fails.
-static uint64_t cache_cap(const kg_ctx *ctx) { return ctx->total_hbm / 4; }
+// fails (which releases the whole cache and retries).  Seven eighths of HBM:
+// an operator that cycles through more than a quarter of it (the grace
+// join's partition pieces at 1.5B x 6B rows, 120 GB a step) would otherwise
+// free and allocate half of them again every step.
+static uint64_t cache_cap(const kg_ctx *ctx) { return ctx->total_hbm / 8 * 7; }
 
 // (cache_mu held) moves the pending blocks whose events have completed (all
 // of them with `all`, after a device synchronize) into the cache, keeping

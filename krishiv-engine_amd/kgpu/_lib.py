@@ -189,7 +189,7 @@ def check(rc: int):
 
 
 _SIGS.update({
-    "kg_grace_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p]),
+    "kg_grace_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p]),
     "kg_grace_build_sink": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "kg_grace_probe_sink": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "kg_grace_finish": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

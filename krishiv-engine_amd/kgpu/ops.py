@@ -690,11 +690,14 @@ class GraceJoinAggregate:
     """Grace hash join fused with SELECT p, COUNT(*), SUM(probe.pay *
     build.pay % 2^31) ... GROUP BY p, p = partition_of(H_kg(key), partitions)."""
 
-    def __init__(self, ctx: Context, partitions=64, table_budget=8 << 30):
+    def __init__(self, ctx: Context, partitions=64, table_budget=8 << 30, sum_mod31=False):
+        """sum_mod31: SUM read as ((sum of products) mod 2^64) mod 2^31 (any
+        payloads); otherwise per-pair (p * b) % 2^31, products in [0, 2^31)."""
         self.ctx = ctx
         self.parts = partitions
         h = C.c_void_p()
-        check(lib().kg_grace_create(ctx.h, partitions, table_budget, C.byref(h)))
+        check(lib().kg_grace_create(ctx.h, partitions, table_budget, 1 if sum_mod31 else 0,
+                                    C.byref(h)))
         self.h = h
 
     def build_sink(self, key: Col, pay: Col, nrows=None):

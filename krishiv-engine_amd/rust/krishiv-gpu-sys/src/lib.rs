@@ -64,6 +64,7 @@ pub const KG_X_AND: i32 = 14;
 pub const KG_X_OR: i32 = 15;
 pub const KG_X_NOT: i32 = 16;
 pub const KG_X_IS_NULL: i32 = 17;
+pub const KG_GRACE_SUM_MOD31: u32 = 1;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug)]
@@ -235,7 +236,7 @@ extern "C" {
     pub fn kg_joinagg_create(j: *mut kg_hash_join, ninputs: i32, input_types: *const kg_column, naggs: i32, aggs: *const kg_agg_spec, flags: u32) -> c_int;
     pub fn kg_joinagg_probe(j: *mut kg_hash_join, ncols: i32, cols: *const kg_column, nrows: i64, input_cols: *const i32) -> c_int;
     pub fn kg_joinagg_result(j: *mut kg_hash_join, out: *mut *mut kg_batch) -> c_int;
-    pub fn kg_grace_create(ctx: *mut kg_ctx, partitions: u32, table_budget_bytes: u64, out: *mut *mut kg_grace) -> c_int;
+    pub fn kg_grace_create(ctx: *mut kg_ctx, partitions: u32, table_budget_bytes: u64, flags: u32, out: *mut *mut kg_grace) -> c_int;
     pub fn kg_grace_build_sink(g: *mut kg_grace, key: *const kg_column, pay: *const kg_column, nrows: i64) -> c_int;
     pub fn kg_grace_probe_sink(g: *mut kg_grace, key: *const kg_column, pay: *const kg_column, nrows: i64) -> c_int;
     pub fn kg_grace_finish(g: *mut kg_grace, counts: *mut i64, sums: *mut i64, sum_valid: *mut u8, spilled_bytes: *mut u64) -> c_int;

@@ -105,3 +105,25 @@ def test_products_that_do_not_factor_are_refused():
     g.probe_sink(Col(A.T_INT64, np.array([1, 2])), Col(A.T_INT64, np.array([1 << 12, 3])))
     with pytest.raises(kgpu.KgError, match="KRISHIV_UNSUPPORTED_FEATURE"):
         g.finish()
+
+
+def test_sum_mod31_reading_takes_any_payloads():
+    """Configuration 4's factorable reading: SUM = ((sum of p * b, wrapping)
+    mod 2^31) for payloads of any sign and size, against the oracle's
+    factorization (pinned by the pair walk in the oracle tests)."""
+    say("grace: mod 2^31 reading, wide payloads")
+    rng = np.random.default_rng(12)
+    nb, npr = 200_000, 900_000
+    bk = zipf_keys(6, nb, 150_000, 0.8)
+    pk = zipf_keys(7, npr, 150_000, 0.8)
+    bp = rng.integers(-(1 << 40), 1 << 40, nb)
+    pp = rng.integers(-(1 << 40), 1 << 40, npr)
+    bv, pv = rng.random(nb) > 0.02, rng.random(npr) > 0.02
+    g = kgpu.GraceJoinAggregate(ctx(), 64, sum_mod31=True)
+    g.build_sink(Col(A.T_INT64, bk), Col(A.T_INT64, bp, bv))
+    g.probe_sink(Col(A.T_INT64, pk), Col(A.T_INT64, pp, pv))
+    gc, gs, gv, _ = g.finish()
+    wc, ws, wv = kgo.grace_factorized(kgo.Col(kgo.T_INT64, bk), kgo.Col(kgo.T_INT64, bp, bv),
+                                      kgo.Col(kgo.T_INT64, pk), kgo.Col(kgo.T_INT64, pp, pv), 64, 1)
+    assert np.array_equal(gc, wc) and np.array_equal(gs, ws) and np.array_equal(gv, wv)
+    assert (gs >= 0).all() and (gs < (1 << 31)).all()
