@@ -201,6 +201,24 @@ int kg_agg_finalize(kg_hash_agg *a);
 int kg_agg_result_type(const kg_hash_agg *a, int32_t i, kg_column *out_type);
 /* Source: the final values as a device batch [key, agg 0, agg 1, ...]. */
 int kg_agg_result(kg_hash_agg *a, kg_batch **out);
+/* BypassPolicy of a PartialAggregate (aggregate/partial.rs:20-50, defaults
+ * 100000 rows, 80 percent, 16 MiB): after each batch is folded, once
+ * min_rows rows are seen and the groups (flushed and held) per 100 rows reach
+ * max_group_percent, the table is flushed and every later batch passes
+ * through as one partial state per row (AggTable::states_of_rows); otherwise
+ * a table whose records pass max_table_bytes is flushed (partial.rs:217-262).
+ * Flushed and passed-through state batches are taken with
+ * kg_agg_partial_next; finalize + result give what the table holds at the
+ * end.  KG_AGG_PARTIAL aggregates with a key only, set before the first
+ * sink. */
+int kg_agg_set_bypass(kg_hash_agg *a, uint64_t min_rows, uint32_t max_group_percent,
+                      uint64_t max_table_bytes);
+/* The next emitted partial-state batch (*out NULL if none); *done = 1 when no
+ * more are queued.  The caller owns the batch. */
+int kg_agg_partial_next(kg_hash_agg *a, kg_batch **out, int32_t *done);
+/* PartialAggregate's metrics (partial.rs:289-296): table flushes and rows
+ * passed through. */
+int kg_agg_partial_stats(const kg_hash_agg *a, uint64_t *flushes, uint64_t *bypassed_rows);
 void kg_agg_destroy(kg_hash_agg *a);
 
 /* ---- runtime Bloom filter (split-block, Parquet layout) ----------------

@@ -112,88 +112,14 @@ __global__ void __launch_bounds__(kSmallBlock) k_agg_small(AggProg P, Table T, i
   }
 }
 
-template <int SLOTS>
-__global__ void __launch_bounds__(kAggBlock) k_agg_update(AggProg P, Table T, int64_t nrows,
-                                                          int64_t chunk) {
-  extern __shared__ uint64_t smem[];
-  uint64_t *s_keys = smem;
-  uint64_t *s_rec = smem + SLOTS;
-  const int W = P.W;
-  const int WL = P.W | 1;  // odd LDS record stride: records spread over the LDS banks
-  for (int64_t base = blockIdx.x * chunk; base < nrows; base += (int64_t)gridDim.x * chunk) {
-    for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) s_keys[t] = kEmpty;
-    fill_lds_records(P, s_rec, SLOTS, WL);
-    __syncthreads();
-    int64_t end = base + chunk < nrows ? base + chunk : nrows;
-    if (P.direct || !P.has_key) {
-      // few groups: each wave reduces its rows per distinct group in
-      // registers and applies them once (lanes iterate in step: every lane
-      // takes part in the ballots and shuffles)
-      if (P.direct && threadIdx.x < 257) s_keys[threadIdx.x] = 0;  // every direct slot valid
-      for (int64_t b = base; b < end; b += blockDim.x) {
-        int64_t i = b + threadIdx.x;
-        bool active = i < end;
-        int ls = 0;
-        if (active && P.direct) {
-          uint64_t key;
-          int cls = row_key(P, i, key);
-          ls = cls == 2 ? 256 : (int)(key & 0xFF);
-        }
-        uint64_t todo = __ballot(active);
-        while (todo) {
-          int lead = __ffsll((unsigned long long)todo) - 1;
-          int ks = __shfl(ls, lead);
-          uint64_t peers = __ballot(active && ls == ks);
-          apply_wave<true>(P, s_rec + (uint64_t)ks * WL, i, peers);
-          todo &= ~peers;
-        }
-      }
-      __syncthreads();
-      for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) {
-        const uint64_t *l = s_rec + (uint64_t)t * WL;
-        if (l[0] == 0 || (!P.direct && t != 0)) continue;
-        merge_record(P, global_record(P, T, (P.direct && t == 256) ? 2 : 0, (uint64_t)t), l);
-      }
-      __syncthreads();
-      continue;
-    }
-    for (int64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
-      uint64_t key;
-      int cls = row_key(P, i, key);
-      int ls = -1;
-      if (cls == 0) {
-        uint64_t s = (P.has_key ? slot_hash(T, key) : 0) & (SLOTS - 1);
-        for (int p = 0; p < kLdsProbe; p++) {
-          uint64_t k = s_keys[s];
-          if (k == key) {
-            ls = (int)s;
-            break;
-          }
-          if (k == kEmpty) {
-            unsigned long long prev = atomicCAS((unsigned long long *)&s_keys[s], kEmpty, key);
-            if (prev == kEmpty || prev == key) {
-              ls = (int)s;
-              break;
-            }
-          }
-          s = (s + 1) & (SLOTS - 1);
-        }
-      }
-      if (ls >= 0) {
-        apply_row<true>(P, s_rec + (uint64_t)ls * WL, i);
-      } else {
-        apply_row<false>(P, global_record(P, T, cls, key), i);
-      }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < SLOTS; t += blockDim.x) {
-      uint64_t k = s_keys[t];
-      if (k == kEmpty) continue;
-      const uint64_t *l = s_rec + (uint64_t)t * WL;
-      if (l[0] == 0) continue;  // no row
-      merge_record(P, global_record(P, T, 0, k), l);
-    }
-    __syncthreads();
+// Few groups whose records are too wide for LDS copies (more than 64 KiB
+// for every used slot): each row applied to its global record with atomics.
+__global__ void __launch_bounds__(kAggBlock) k_agg_rows_atomic(AggProg P, Table T, int64_t nrows) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrows;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t key;
+    const int cls = row_key(P, i, key);
+    apply_row<false>(P, global_record(P, T, cls, key), i);
   }
 }
 
@@ -1244,8 +1170,6 @@ struct kg_hash_agg {
   int nsink = 0;  // input columns per sink call (state columns when merging)
   AggProg prog{};
   OutProg out{};
-  int slots = 512;  // LDS records per workgroup (one-byte and global aggregates)
-  size_t lds_bytes = 0;
   int p1_slots = 0, m_slots = 0;  // LDS tables of pass 1 and of the merge
   size_t p1_lds = 0, m_lds = 0;
   // table
@@ -1265,6 +1189,14 @@ struct kg_hash_agg {
   uint64_t rows_seen = 0;
   uint64_t pending_rows = 0;  // rows of the sink call in progress
   DevBuf small_bits;          // the few-groups path: used-slot bits and its maps
+  // PartialAggregate's BypassPolicy (aggregate/partial.rs:20-50): partial
+  // state batches emitted by flushes and by pass-through, taken with
+  // kg_agg_partial_next
+  bool has_policy = false, bypass = false;
+  uint64_t pol_min_rows = 0, pol_pct = 0, pol_max_bytes = 0;
+  uint64_t part_rows = 0, flushed_groups = 0, flushes = 0, bypassed_rows = 0;
+  std::vector<kg_batch *> emitted;
+  size_t emitted_next = 0;
   DevBuf scratch_slots, scratch_abs;
   bool finalized = false;
   int64_t ngroups = 0;
@@ -1436,7 +1368,12 @@ static void launch_small(kg_hash_agg *a, const AggProg &Q, int64_t n) {
   // record copies: up to 4 a wave, within 48 KB of LDS a workgroup
   int K = kSmallBlock / 64 * 4;
   while (K > 1 && (size_t)K * rec_bytes > 48 * 1024) K--;
-  if ((size_t)K * rec_bytes > 64 * 1024) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide");
+  if ((size_t)K * rec_bytes > 64 * 1024) {
+    ProfScope prof("agg_update", a->s);
+    k_agg_rows_atomic<<<stream_grid(n, kAggBlock), kAggBlock, 0, a->s>>>(Q, table_of(a), n);
+    KG_HIP(hipGetLastError());
+    return;
+  }
   const int64_t cus = a->ctx->num_cus > 0 ? a->ctx->num_cus : 256;
   constexpr int R = 4;
   int64_t blocks = std::min<int64_t>(cus * 4, ceil_div(n, (int64_t)R * kSmallBlock * 4));
@@ -1456,26 +1393,8 @@ static void launch_update(kg_hash_agg *a, const AggProg &P, int64_t lo, int64_t 
   // offset the columns to this piece of the batch
   for (int c = 0; c < kAggInputs; c++) Q.in[c].offset += lo;
   Q.key.offset += lo;
-  if (a->direct || !a->has_key) {
-    launch_small(a, Q, n);
-    return;
-  }
-  Table t = table_of(a);
-  int64_t chunk = 16384;
-  int64_t blocks = ceil_div(n, chunk);
-  int64_t maxb = (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256) * 4;
-  if (blocks > maxb) {
-    blocks = maxb;
-    chunk = ceil_div(n, blocks);
-  }
-  ProfScope prof("agg_update", a->s);
-  switch (a->slots) {
-    case 512: k_agg_update<512><<<(unsigned)blocks, kAggBlock, a->lds_bytes, a->s>>>(Q, t, n, chunk); break;
-    case 256: k_agg_update<256><<<(unsigned)blocks, kAggBlock, a->lds_bytes, a->s>>>(Q, t, n, chunk); break;
-    default: k_agg_update<128><<<(unsigned)blocks, kAggBlock, a->lds_bytes, a->s>>>(Q, t, n, chunk); break;
-  }
-  KG_HIP(hipGetLastError());
-  a->occ_bound += (uint64_t)n;
+  if (a->hashed) fail(KG_ERR_INTERNAL, "hashed keys are aggregated by pass 1");
+  launch_small(a, Q, n);
 }
 
 // ---------------------------------------------------------------- spill
@@ -1805,6 +1724,10 @@ static uint64_t sink_need(kg_hash_agg *a, const kg_column *inputs, int64_t nrows
          drain_need(a, full_rows, full_recs, occupied, new_keys, a->spill_budget);
 }
 
+static void emit_rows(kg_hash_agg *a, const AggProg &P, int64_t n);
+static void after_partial_batch(kg_hash_agg *a, uint64_t nrows);
+static uint64_t out_row_bytes(const kg_hash_agg *a);
+
 static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, int64_t nrows) {
   if (a->finalized) fail(KG_ERR_INTERNAL, "sink after finalize");
   if (nrows <= 0) return;
@@ -1821,6 +1744,17 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
     if (!a->merge && inputs[c].type != a->in_types[c].type)
       fail(KG_ERR_INTERNAL, "input column type changed");
     P.in[c] = st.add(inputs[c], nrows);
+  }
+  if (a->has_policy && a->bypass) {
+    // pass-through: the batch leaves as one partial state per row
+    P.skip_checked = 0;
+    const uint64_t rb = (uint64_t)a->prog.W * 8 + 8;
+    need_room(a->ctx, (uint64_t)nrows * (rb + out_row_bytes(a)) + (1u << 20),
+              "the pass-through batch");
+    emit_rows(a, P, nrows);
+    a->rows_seen += (uint64_t)nrows;
+    a->part_rows += (uint64_t)nrows;
+    return;
   }
   bool replay = false;
   std::vector<double> batch_abs(a->checked.size(), 0.0);
@@ -1874,6 +1808,7 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
   for (size_t k = 0; k < a->checked.size(); k++) a->checked[k].cum_abs += batch_abs[k];
   // host inputs were staged into buffers freed on return: finish with them
   if (st.staged_host()) KG_HIP(hipStreamSynchronize(a->s));
+  if (a->has_policy) after_partial_batch(a, (uint64_t)nrows);
 }
 
 int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg_agg_spec *aggs,
@@ -2114,6 +2049,37 @@ int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t ns, int W, bo
   return n;
 }
 
+// Bits [off, off + n) of src to dst from bit 0 (a column's validity, or a
+// BOOL column's values, at its offset).
+__global__ void k_copy_key_bits(const uint8_t *src, int64_t off, int64_t n, uint32_t *dst) {
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w * 32 < n;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t bits = 0;
+    for (int k = 0; k < 32 && w * 32 + k < n; k++) {
+      const int64_t i = off + w * 32 + k;
+      bits |= (uint32_t)((src[i >> 3] >> (i & 7)) & 1) << k;
+    }
+    dst[w] = bits;
+  }
+}
+
+__global__ void k_iota(uint64_t *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (uint64_t)i;
+}
+
+// Each row as its own record: the initial state, then the row applied (the
+// records of PartialAggregate's pass-through, AggTable::states_of_rows).
+__global__ void k_rows_as_records(AggProg P, int64_t n, uint64_t *rec) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t *r = rec + (uint64_t)i * P.W;
+    for (int w = 0; w < P.W; w++) r[w] = P.init[w];
+    apply_row<false>(P, r, i);
+  }
+}
+
 void launch_agg_output(const OutProg &O, const uint64_t *keys, int key_stride,
                        const uint64_t *rec, const uint64_t *list, int64_t n, void *out_key,
                        uint8_t *out_key_valid, int key_width, unsigned int *err, hipStream_t s) {
@@ -2126,9 +2092,228 @@ void launch_agg_output(const OutProg &O, const uint64_t *keys, int key_stride,
   KG_HIP(hipGetLastError());
 }
 
+// The aggregate's output batch [key, aggregates (or partial states)] of the
+// n records at rec + list[o] * W.  key_src: copy the key column from these
+// rows instead of the records' keys (pass-through rows).
+static std::unique_ptr<kg_batch> result_batch(kg_hash_agg *a, const uint64_t *keys,
+                                              const uint64_t *rec, const uint64_t *list,
+                                              int64_t n, uint64_t cap, const DCol *key_src) {
+  {
+    auto b = std::make_unique<kg_batch>();
+    b->ctx = a->ctx;
+    b->nrows = n;
+    uint64_t vb = (uint64_t)ceil_div(n, 32) * 4 + 4;
+    OutProg O = a->out;
+    O.cap = cap;
+    int ncols = (a->has_key ? 1 : 0);
+    for (int j = 0; j < a->out.n; j++) ncols += (a->partial && a->out.a[j].st2) ? 2 : 1;
+    b->cols.resize(ncols);
+    int kw = 8;
+    void *key_vals = nullptr;
+    uint8_t *key_valid = nullptr;
+    int base = 0;
+    if (a->has_key) {
+      auto &kc = b->cols[0];
+      kc.type = a->key_type.type;
+      kc.precision = a->key_type.precision;
+      kc.scale = a->key_type.scale;
+      kw = type_width(kc.type);
+      uint64_t bytes = kw == 0 ? (uint64_t)ceil_div(n, 32) * 4 + 4 : (uint64_t)n * kw + 8;
+      if (kw == 0) {
+        // booleans leave the table as one byte per group; repack as bits
+        kw = 1;
+      }
+      kc.values.reset(a->ctx, bytes);
+      kc.validity.reset(a->ctx, vb);
+      KG_HIP(hipMemsetAsync(kc.validity.get(), 0, vb, a->s));
+      key_vals = kc.values.get();
+      key_valid = kc.validity.get<uint8_t>();
+      base = 1;
+    }
+    int at = base;
+    std::vector<int> bool_cols;  // partial `nonempty` columns, written a byte per group
+    for (int j = 0; j < a->out.n; j++) {
+      OutAgg &o = O.a[j];
+      auto &c = b->cols[at++];
+      c.type = a->partial ? o.st_type : o.out_type;
+      c.precision = a->partial ? o.st_p : o.out_p;
+      c.scale = a->partial ? o.st_s : o.out_s;
+      int w = type_width(c.type);
+      c.values.reset(a->ctx, (uint64_t)n * (w > 0 ? w : 1) + 16);
+      c.validity.reset(a->ctx, vb);
+      KG_HIP(hipMemsetAsync(c.validity.get(), 0, vb, a->s));
+      o.values = c.values.get();
+      o.validity = c.validity.get<uint8_t>();
+      o.values2 = nullptr;
+      o.validity2 = nullptr;
+      if (a->partial && o.st2) {
+        auto &c2 = b->cols[at];
+        c2.type = o.st2 == 1 ? KG_T_BOOL : KG_T_INT64;
+        c2.values.reset(a->ctx, (uint64_t)n * 8 + 16);
+        c2.validity.reset(a->ctx, vb);
+        KG_HIP(hipMemsetAsync(c2.validity.get(), 0, vb, a->s));
+        o.values2 = c2.values.get();
+        o.validity2 = c2.validity.get<uint8_t>();
+        if (o.st2 == 1) bool_cols.push_back(at);
+        at++;
+      }
+    }
+    DevBuf err(a->ctx, 4);
+    KG_HIP(hipMemsetAsync(err.get(), 0, 4, a->s));
+    launch_agg_output(O, keys, 1, rec, list, n, key_src ? nullptr : key_vals,
+                      key_src ? nullptr : key_valid, kw, err.get<unsigned int>(), a->s);
+    if (key_src && n > 0) {  // the rows' own keys
+      const int w = type_width(key_src->type);
+      if (w == 0)
+        k_copy_key_bits<<<stream_grid(ceil_div(n, 32), 256, 1), 256, 0, a->s>>>(
+            static_cast<const uint8_t *>(key_src->values), key_src->offset, n,
+            static_cast<uint32_t *>(key_vals));
+      else
+        KG_HIP(hipMemcpyAsync(key_vals, static_cast<const uint8_t *>(key_src->values) +
+                                            key_src->offset * w,
+                              (size_t)n * w, hipMemcpyDeviceToDevice, a->s));
+      if (key_src->validity)
+        k_copy_key_bits<<<stream_grid(ceil_div(n, 32), 256, 1), 256, 0, a->s>>>(
+            key_src->validity, key_src->offset, n, reinterpret_cast<uint32_t *>(key_valid));
+      else
+        KG_HIP(hipMemsetAsync(key_valid, 0xFF, (size_t)ceil_div(n, 32) * 4, a->s));
+      KG_HIP(hipGetLastError());
+    }
+    unsigned int e = 0;
+    KG_HIP(hipMemcpyAsync(&e, err.get(), 4, hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    if (e & 1u) fail(KG_ERR_OVERFLOW, "Overflow in sum (use try_sum/try_avg to tolerate overflow)");
+    if (e & 2u) fail(KG_ERR_OUT_OF_RANGE, "average out of the result decimal's range");
+    // booleans (a boolean key, partial `nonempty` states) leave the kernel as
+    // one byte per group; Arrow packs them as bits
+    if (a->has_key && a->key_type.type == KG_T_BOOL && !key_src) bool_cols.push_back(0);
+    for (int c : bool_cols) {
+      DevBuf bits(a->ctx, vb);
+      k_bytes_to_bits<<<stream_grid(ceil_div(n, 32), 256, 1), 256, 0, a->s>>>(
+          b->cols[c].values.get<uint8_t>(), n, bits.get<uint32_t>());
+      KG_HIP(hipGetLastError());
+      b->cols[c].values = std::move(bits);
+    }
+    KG_HIP(hipStreamSynchronize(a->s));
+    return b;
+  }
+}
+
+
+// ---- PartialAggregate: flushes and pass-through (aggregate/partial.rs) -----
+static void reset_table(kg_hash_agg *a) {
+  alloc_table(a, a->cap, a->keys, a->rec);
+  KG_HIP(hipMemsetAsync(a->occ.get(), 0, 8, a->s));
+  if (a->hashed) KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 24, a->s));
+  a->occ_bound = a->merged_keys = 0;
+  a->bound_rows = a->bound_recs = 0;
+  a->finalized = false;
+  a->ngroups = 0;
+}
+
+// Output bytes per group: the key and every state column with validity.
+static uint64_t out_row_bytes(const kg_hash_agg *a) {
+  uint64_t b = 8 + 1;
+  for (int j = 0; j < a->out.n; j++) b += (a->out.a[j].st2 ? 2 : 1) * (16 + 1);
+  return b;
+}
+
+// Flush (PartialAggregate::start_drain): the table's partial states are
+// emitted and the table starts empty.  Emitting allocates its output first:
+// refused for memory, the table stays as it was and false is returned.
+static bool emit_table(kg_hash_agg *a) {
+  drain(a);
+  const int64_t groups = group_positions(a->ctx, a->rec.get<uint64_t>(), nslots(a), a->prog.W,
+                                         !a->has_key, a->pos, a->s);
+  if (groups > 0) {
+    try {
+      a->emitted.push_back(result_batch(a, a->keys.get<uint64_t>(), a->rec.get<uint64_t>(),
+                                        a->pos.get<uint64_t>(), groups, a->cap, nullptr)
+                               .release());
+    } catch (const Error &e) {
+      if (e.code != KG_ERR_MEMORY) throw;
+      return false;
+    }
+    a->flushed_groups += (uint64_t)groups;
+  }
+  a->flushes++;
+  reset_table(a);
+  for (auto &c : a->checked) c.cum_abs = 0;  // the sums start again from empty groups
+  return true;
+}
+
+// BypassPolicy after a batch is folded (partial.rs:246-262): groups per 100
+// rows, flushed and held, at or above max_group_percent once min_rows rows
+// are seen switch the aggregate to pass-through (the table flushed first);
+// otherwise a table whose groups' records pass max_table_bytes is flushed.
+static void after_partial_batch(kg_hash_agg *a, uint64_t nrows) {
+  a->part_rows += nrows;
+  drain(a);
+  const uint64_t held = a->direct ? (uint64_t)group_positions(a->ctx, a->rec.get<uint64_t>(),
+                                                               nslots(a), a->prog.W, false,
+                                                               a->pos, a->s)
+                                  : read_occupied(a);
+  const uint64_t groups = a->flushed_groups + held;
+  if (a->part_rows >= a->pol_min_rows && groups * 100 >= a->part_rows * a->pol_pct) {
+    if (emit_table(a)) a->bypass = true;
+    return;
+  }
+  if (held * ((uint64_t)a->prog.W * 8 + 8) > a->pol_max_bytes) (void)emit_table(a);
+}
+
+// Pass-through: every row of the batch as its own partial state.
+static void emit_rows(kg_hash_agg *a, const AggProg &P, int64_t n) {
+  DevBuf rec(a->ctx, (uint64_t)n * a->prog.W * 8 + 8), list(a->ctx, (uint64_t)n * 8 + 8);
+  k_rows_as_records<<<stream_grid(n, 256), 256, 0, a->s>>>(P, n, rec.get<uint64_t>());
+  k_iota<<<stream_grid(n, 256), 256, 0, a->s>>>(list.get<uint64_t>(), n);
+  KG_HIP(hipGetLastError());
+  const DCol *ks = a->has_key ? &P.key : nullptr;
+  // cap past every record index: no record is the NULL-key slot
+  a->emitted.push_back(result_batch(a, nullptr, rec.get<uint64_t>(), list.get<uint64_t>(), n,
+                                    (uint64_t)n + 16, ks)
+                           .release());
+  a->bypassed_rows += (uint64_t)n;
+}
+
 }  // namespace kg
 
 extern "C" {
+
+int kg_agg_set_bypass(kg_hash_agg *a, uint64_t min_rows, uint32_t max_group_percent,
+                      uint64_t max_table_bytes) {
+  return guarded([&] {
+    if (!a->partial) fail(KG_ERR_INTERNAL, "a bypass policy needs a partial aggregate");
+    if (a->rows_seen) fail(KG_ERR_INTERNAL, "the bypass policy is set before the first sink");
+    if (!a->has_key) fail(KG_ERR_UNSUPPORTED, "a bypass policy for a global aggregate");
+    a->has_policy = true;
+    a->pol_min_rows = min_rows;
+    a->pol_pct = max_group_percent;
+    a->pol_max_bytes = max_table_bytes;
+  });
+}
+
+int kg_agg_partial_next(kg_hash_agg *a, kg_batch **out, int32_t *done) {
+  return guarded([&] {
+    *out = nullptr;
+    *done = 1;
+    if (a->emitted_next < a->emitted.size()) {
+      *out = a->emitted[a->emitted_next];
+      a->emitted[a->emitted_next++] = nullptr;
+    }
+    if (a->emitted_next >= a->emitted.size()) {
+      a->emitted.clear();
+      a->emitted_next = 0;
+    }
+    *done = a->emitted.empty() ? 1 : 0;
+  });
+}
+
+int kg_agg_partial_stats(const kg_hash_agg *a, uint64_t *flushes, uint64_t *bypassed_rows) {
+  return guarded([&] {
+    if (flushes) *flushes = a->flushes;
+    if (bypassed_rows) *bypassed_rows = a->bypassed_rows;
+  });
+}
 
 int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
                   const kg_column *input_types, int32_t naggs, const kg_agg_spec *aggs,
@@ -2161,28 +2346,6 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
     a->s = make_stream(ctx->device);
     StreamScope kg_scope(a->s);
     int W = a->prog.W;
-    // the largest LDS table (records + keys) the device grants a workgroup:
-    // 64 KiB always, more where the dynamic-LDS attribute is accepted
-    a->slots = 0;
-    for (int slots : {512, 256, 128}) {
-      if (a->direct && slots < 512) break;
-      size_t bytes = (size_t)slots * ((W | 1) + 1) * 8;
-      if (bytes > 96 * 1024) continue;
-      hipError_t e = hipSuccess;
-      if (bytes > 64 * 1024) {
-        const void *fn = slots == 512 ? (const void *)k_agg_update<512>
-                         : slots == 256 ? (const void *)k_agg_update<256>
-                                        : (const void *)k_agg_update<128>;
-        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess) (void)hipGetLastError();
-      }
-      if (e == hipSuccess) {
-        a->slots = slots;
-        a->lds_bytes = bytes;
-        break;
-      }
-    }
-    if (a->slots == 0) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the LDS table");
     a->hashed = a->has_key && !a->direct;
     if (a->hashed) {
       // one 1,024-lane workgroup a CU with the whole LDS: one table of the
@@ -2265,93 +2428,16 @@ int kg_agg_result(kg_hash_agg *a, kg_batch **out) {
     StreamScope kg_scope(a->s);
     *out = nullptr;
     if (!a->finalized) fail(KG_ERR_INTERNAL, "result before finalize");
-    auto b = std::make_unique<kg_batch>();
-    b->ctx = a->ctx;
-    int64_t n = a->ngroups;
-    b->nrows = n;
-    uint64_t vb = (uint64_t)ceil_div(n, 32) * 4 + 4;
-    OutProg O = a->out;
-    O.cap = a->cap;
-    int ncols = (a->has_key ? 1 : 0);
-    for (int j = 0; j < a->out.n; j++) ncols += (a->partial && a->out.a[j].st2) ? 2 : 1;
-    b->cols.resize(ncols);
-    int kw = 8;
-    void *key_vals = nullptr;
-    uint8_t *key_valid = nullptr;
-    int base = 0;
-    if (a->has_key) {
-      auto &kc = b->cols[0];
-      kc.type = a->key_type.type;
-      kc.precision = a->key_type.precision;
-      kc.scale = a->key_type.scale;
-      kw = type_width(kc.type);
-      uint64_t bytes = kw == 0 ? (uint64_t)ceil_div(n, 32) * 4 + 4 : (uint64_t)n * kw + 8;
-      if (kw == 0) {
-        // booleans leave the table as one byte per group; repack as bits
-        kw = 1;
-      }
-      kc.values.reset(a->ctx, bytes);
-      kc.validity.reset(a->ctx, vb);
-      KG_HIP(hipMemsetAsync(kc.validity.get(), 0, vb, a->s));
-      key_vals = kc.values.get();
-      key_valid = kc.validity.get<uint8_t>();
-      base = 1;
-    }
-    int at = base;
-    std::vector<int> bool_cols;  // partial `nonempty` columns, written a byte per group
-    for (int j = 0; j < a->out.n; j++) {
-      OutAgg &o = O.a[j];
-      auto &c = b->cols[at++];
-      c.type = a->partial ? o.st_type : o.out_type;
-      c.precision = a->partial ? o.st_p : o.out_p;
-      c.scale = a->partial ? o.st_s : o.out_s;
-      int w = type_width(c.type);
-      c.values.reset(a->ctx, (uint64_t)n * (w > 0 ? w : 1) + 16);
-      c.validity.reset(a->ctx, vb);
-      KG_HIP(hipMemsetAsync(c.validity.get(), 0, vb, a->s));
-      o.values = c.values.get();
-      o.validity = c.validity.get<uint8_t>();
-      o.values2 = nullptr;
-      o.validity2 = nullptr;
-      if (a->partial && o.st2) {
-        auto &c2 = b->cols[at];
-        c2.type = o.st2 == 1 ? KG_T_BOOL : KG_T_INT64;
-        c2.values.reset(a->ctx, (uint64_t)n * 8 + 16);
-        c2.validity.reset(a->ctx, vb);
-        KG_HIP(hipMemsetAsync(c2.validity.get(), 0, vb, a->s));
-        o.values2 = c2.values.get();
-        o.validity2 = c2.validity.get<uint8_t>();
-        if (o.st2 == 1) bool_cols.push_back(at);
-        at++;
-      }
-    }
-    DevBuf err(a->ctx, 4);
-    KG_HIP(hipMemsetAsync(err.get(), 0, 4, a->s));
-    launch_agg_output(O, a->keys.get<uint64_t>(), 1, a->rec.get<uint64_t>(), a->pos.get<uint64_t>(),
-                      n, key_vals, key_valid, kw, err.get<unsigned int>(), a->s);
-    unsigned int e = 0;
-    KG_HIP(hipMemcpyAsync(&e, err.get(), 4, hipMemcpyDeviceToHost, a->s));
-    KG_HIP(hipStreamSynchronize(a->s));
-    if (e & 1u) fail(KG_ERR_OVERFLOW, "Overflow in sum (use try_sum/try_avg to tolerate overflow)");
-    if (e & 2u) fail(KG_ERR_OUT_OF_RANGE, "average out of the result decimal's range");
-    // booleans (a boolean key, partial `nonempty` states) leave the kernel as
-    // one byte per group; Arrow packs them as bits
-    if (a->has_key && a->key_type.type == KG_T_BOOL) bool_cols.push_back(0);
-    for (int c : bool_cols) {
-      DevBuf bits(a->ctx, vb);
-      k_bytes_to_bits<<<stream_grid(ceil_div(n, 32), 256, 1), 256, 0, a->s>>>(
-          b->cols[c].values.get<uint8_t>(), n, bits.get<uint32_t>());
-      KG_HIP(hipGetLastError());
-      b->cols[c].values = std::move(bits);
-    }
-    KG_HIP(hipStreamSynchronize(a->s));
-    *out = b.release();
+    *out = result_batch(a, a->keys.get<uint64_t>(), a->rec.get<uint64_t>(), a->pos.get<uint64_t>(),
+                        a->ngroups, a->cap, nullptr)
+               .release();
   });
 }
 
 void kg_agg_destroy(kg_hash_agg *a) {
   if (!a) return;
   if (a->s) (void)hipStreamSynchronize(a->s);
+  for (kg_batch *b : a->emitted) kg_batch_release(b);  // batches never taken
   hipStream_t s = a->s;
   delete a;
   destroy_stream(s);

@@ -93,6 +93,9 @@ _SIGS = {
     "kg_agg_finalize": (C.c_int, [C.c_void_p]),
     "kg_agg_result_type": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "kg_agg_result": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_agg_set_bypass": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64]),
+    "kg_agg_partial_next": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_agg_partial_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_agg_destroy": (None, [C.c_void_p]),
     "kg_bloom_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "kg_bloom_bytes_for": (C.c_uint64, [C.c_uint64, C.c_double]),

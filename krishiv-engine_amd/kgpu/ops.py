@@ -393,6 +393,27 @@ class HashAggregate:
         check(lib().kg_agg_result(self.h, C.byref(h)))
         return Batch(h, self.ctx)
 
+    def set_bypass(self, min_rows=100_000, max_group_percent=80, max_table_bytes=16 << 20):
+        """BypassPolicy (aggregate/partial.rs:20-50) of a partial aggregate."""
+        check(lib().kg_agg_set_bypass(self.h, min_rows, max_group_percent, max_table_bytes))
+
+    def partial_batches(self):
+        """The partial-state batches flushed or passed through so far."""
+        out = []
+        while True:
+            h, done = C.c_void_p(), C.c_int32()
+            check(lib().kg_agg_partial_next(self.h, C.byref(h), C.byref(done)))
+            if h.value:
+                out.append(Batch(h, self.ctx))
+            if done.value:
+                return out
+
+    def partial_stats(self):
+        """(flushes, bypassed_rows)."""
+        f, r = C.c_uint64(), C.c_uint64()
+        check(lib().kg_agg_partial_stats(self.h, C.byref(f), C.byref(r)))
+        return f.value, r.value
+
     def close(self):
         if self.h:
             lib().kg_agg_destroy(self.h)

@@ -206,6 +206,9 @@ extern "C" {
     pub fn kg_agg_finalize(a: *mut kg_hash_agg) -> c_int;
     pub fn kg_agg_result_type(a: *const kg_hash_agg, i: i32, out_type: *mut kg_column) -> c_int;
     pub fn kg_agg_result(a: *mut kg_hash_agg, out: *mut *mut kg_batch) -> c_int;
+    pub fn kg_agg_set_bypass(a: *mut kg_hash_agg, min_rows: u64, max_group_percent: u32, max_table_bytes: u64) -> c_int;
+    pub fn kg_agg_partial_next(a: *mut kg_hash_agg, out: *mut *mut kg_batch, done: *mut i32) -> c_int;
+    pub fn kg_agg_partial_stats(a: *const kg_hash_agg, flushes: *mut u64, bypassed_rows: *mut u64) -> c_int;
     pub fn kg_agg_destroy(a: *mut kg_hash_agg);
     pub fn kg_bloom_create(ctx: *mut kg_ctx, nbytes: u64, out: *mut *mut kg_bloom) -> c_int;
     pub fn kg_bloom_bytes_for(items: u64, fpp: f64) -> u64;
