@@ -109,6 +109,80 @@ int kgo_sbbf_build(const uint64_t *hashes, int64_t n, size_t bytes, double fpp, 
                    double max_fpp, uint32_t *words_out, size_t *nblocks_out);
 
 /* ---------------------------------------------------------------------
+ * Runtime KeyFilter (kgo_filter.c): KeyFilterBuilder's forms
+ * (crates/krishiv-io/src/runtime_filter/filter.rs:180-615, member.rs),
+ * evaluation (member.rs:439-560), the versioned encoding (codec.rs), the
+ * merge of published filters (merge.rs:35-169) and the scan row stage with
+ * its adaptive switch-off (crates/krishiv-exec/src/operators/runtime_filter.rs:240-395).
+ * Pinned by: brute-force membership (exact forms keep exactly the stored
+ * keys, every form keeps every stored key), the form cutoffs the reference
+ * states (8 / 4096 members, the dense bit-set size rule with the numbers of
+ * member.rs:138-146), hand-assembled encodings of the codec layout
+ * (codec.rs:6-20,104-180), decode refusals (form_tests.rs:474-537), and
+ * merge(a, b) against the filter built from the union of the keys.
+ * --------------------------------------------------------------------- */
+enum { KGO_SIDE_BOTH = 0, KGO_SIDE_UPPER = 1, KGO_SIDE_LOWER = 2 };
+enum { KGO_FORM_RANGE = 0, KGO_FORM_SMALL = 1, KGO_FORM_EXACT = 2, KGO_FORM_BLOOM = 3,
+       KGO_FORM_DENSE = 4 };
+#define KGO_SMALL_SET_MAX 8                /* member.rs:34 */
+#define KGO_DENSE_MAX_BITS (1u << 20)      /* member.rs:47 */
+#define KGO_DENSE_SIZE_FACTOR 16           /* member.rs:53 */
+#define KGO_DENSE_FREE_BYTES 4096          /* member.rs:56 */
+#define KGO_HASH_SET_BYTES_PER_KEY 9       /* member.rs:60 */
+#define KGO_FILTER_FORMAT_VERSION 1        /* codec.rs:34 */
+#define KGO_FILTER_ROW_HASH_ID 1           /* hash.rs:40-44 */
+#define KGO_FILTER_NORMALIZATION_VERSION 1 /* codec.rs:36 */
+
+typedef struct kgo_filter_opts {
+  uint64_t max_values;       /* the statistics IN list cap (50) */
+  uint64_t exact_max_values; /* exact set cap (4096) */
+  int32_t bloom;             /* collect hashes for a Bloom filter */
+  int32_t side;              /* KGO_SIDE_* */
+  uint64_t bloom_max_items;  /* 4,000,000 */
+  uint64_t bloom_max_bytes;  /* 8 MiB */
+  double bloom_fpp;          /* 0.01 */
+  double bloom_max_fpp;      /* 0.05 */
+  uint64_t bloom_bytes;      /* the planned size (0: sized for the keys) */
+} kgo_filter_opts;
+
+typedef struct kgo_kf_info {
+  int32_t form;     /* KGO_FORM_* */
+  int32_t has_range;
+  int64_t nmembers; /* small / exact members (a dense set: 0, see kgo_kf_members) */
+  int64_t values;   /* the IN list's length, -1 without one */
+  uint64_t lo_lo, lo_hi, hi_lo, hi_hi; /* integer / decimal range */
+  double flo, fhi;                     /* float range */
+  int32_t nulls_pass, empty;
+  uint64_t keys;
+  uint64_t bloom_bytes;
+} kgo_kf_info;
+
+typedef struct kgo_kf kgo_kf;
+typedef struct kgo_rs kgo_rs;
+void kgo_filter_default_opts(kgo_filter_opts *o);
+int kgo_kf_widens(int from, int fp, int fs, int to, int tp, int ts);
+kgo_kf *kgo_kf_new(int type, int precision, int scale, const kgo_filter_opts *o);
+void kgo_kf_free(kgo_kf *f);
+int kgo_kf_add(kgo_kf *f, const kgo_col *c, const uint8_t *include, int64_t n);
+int kgo_kf_merge_builder(kgo_kf *a, const kgo_kf *b);
+int kgo_kf_finish(kgo_kf *f);
+int kgo_kf_get_info(const kgo_kf *f, kgo_kf_info *o);
+int64_t kgo_kf_members(const kgo_kf *f, uint64_t *out);
+int64_t kgo_kf_bloom_words(const kgo_kf *f, uint32_t *out);
+double kgo_kf_row_cost(const kgo_kf *f);
+int64_t kgo_kf_evaluate(const kgo_kf *f, const kgo_col *c, int64_t n, uint8_t *out);
+int64_t kgo_kf_encode(const kgo_kf *f, uint8_t *out, int64_t cap);
+kgo_kf *kgo_kf_decode(const uint8_t *bytes, int64_t len, char *err);
+kgo_kf *kgo_kf_merge(const kgo_kf *a, const kgo_kf *b, const kgo_filter_opts *o, char *err);
+double kgo_sbbf_pass_rate(const uint32_t *words, size_t nblocks);
+kgo_rs *kgo_rs_new(uint64_t min_rows, double max_pass_rate);
+void kgo_rs_free(kgo_rs *s);
+int kgo_rs_set(kgo_rs *s, int slot, const kgo_kf *f, int column);
+void kgo_rs_start_partition(kgo_rs *s);
+void kgo_rs_stats(const kgo_rs *s, int slot, uint64_t *tested, uint64_t *dropped, int *off);
+int64_t kgo_rs_apply(kgo_rs *s, int ncols, const kgo_col *cols, int64_t n, uint8_t *out);
+
+/* ---------------------------------------------------------------------
  * Synthetic data generators (counter-based; identical integer arithmetic on
  * the device in libkgpu).
  * --------------------------------------------------------------------- */

@@ -168,6 +168,51 @@ def lib():
         L.kgo_pagg_free.argtypes = [vp]
         L.kgo_grace_factorized.restype = C.c_int
         L.kgo_grace_factorized.argtypes = [vp, vp, i64, vp, vp, i64, u32, C.c_int, vp, vp, vp]
+        f64 = C.c_double
+        L.kgo_filter_default_opts.restype = None
+        L.kgo_filter_default_opts.argtypes = [vp]
+        L.kgo_kf_widens.restype = C.c_int
+        L.kgo_kf_widens.argtypes = [C.c_int] * 6
+        L.kgo_kf_new.restype = vp
+        L.kgo_kf_new.argtypes = [C.c_int, C.c_int, C.c_int, vp]
+        L.kgo_kf_free.restype = None
+        L.kgo_kf_free.argtypes = [vp]
+        L.kgo_kf_add.restype = C.c_int
+        L.kgo_kf_add.argtypes = [vp, vp, vp, i64]
+        L.kgo_kf_merge_builder.restype = C.c_int
+        L.kgo_kf_merge_builder.argtypes = [vp, vp]
+        L.kgo_kf_finish.restype = C.c_int
+        L.kgo_kf_finish.argtypes = [vp]
+        L.kgo_kf_get_info.restype = C.c_int
+        L.kgo_kf_get_info.argtypes = [vp, vp]
+        L.kgo_kf_members.restype = i64
+        L.kgo_kf_members.argtypes = [vp, vp]
+        L.kgo_kf_bloom_words.restype = i64
+        L.kgo_kf_bloom_words.argtypes = [vp, vp]
+        L.kgo_kf_row_cost.restype = f64
+        L.kgo_kf_row_cost.argtypes = [vp]
+        L.kgo_kf_evaluate.restype = i64
+        L.kgo_kf_evaluate.argtypes = [vp, vp, i64, vp]
+        L.kgo_kf_encode.restype = i64
+        L.kgo_kf_encode.argtypes = [vp, vp, i64]
+        L.kgo_kf_decode.restype = vp
+        L.kgo_kf_decode.argtypes = [vp, i64, vp]
+        L.kgo_kf_merge.restype = vp
+        L.kgo_kf_merge.argtypes = [vp, vp, vp, vp]
+        L.kgo_sbbf_pass_rate.restype = f64
+        L.kgo_sbbf_pass_rate.argtypes = [vp, C.c_size_t]
+        L.kgo_rs_new.restype = vp
+        L.kgo_rs_new.argtypes = [u64, f64]
+        L.kgo_rs_free.restype = None
+        L.kgo_rs_free.argtypes = [vp]
+        L.kgo_rs_set.restype = C.c_int
+        L.kgo_rs_set.argtypes = [vp, C.c_int, vp, C.c_int]
+        L.kgo_rs_start_partition.restype = None
+        L.kgo_rs_start_partition.argtypes = [vp]
+        L.kgo_rs_stats.restype = None
+        L.kgo_rs_stats.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.kgo_rs_apply.restype = i64
+        L.kgo_rs_apply.argtypes = [vp, C.c_int, vp, i64, vp]
         _lib = L
     return _lib
 
@@ -544,3 +589,169 @@ def grace_factorized(bkey: Col, bpay: Col, pkey: Col, ppay: Col, parts: int, mod
                                   _ptr(sums), _ptr(valid)) != 0:
         raise RuntimeError(error())
     return counts, sums, valid.astype(bool)
+
+
+# --- runtime KeyFilter (kgo_filter.c) ------------------------------------
+SIDE_BOTH, SIDE_UPPER, SIDE_LOWER = 0, 1, 2
+FORM_RANGE, FORM_SMALL, FORM_EXACT, FORM_BLOOM, FORM_DENSE = 0, 1, 2, 3, 4
+
+
+class FilterOpts(C.Structure):
+    """KeyFilterOptions (runtime_filter/filter.rs:28-107)."""
+    _fields_ = [("max_values", C.c_uint64), ("exact_max_values", C.c_uint64),
+                ("bloom", C.c_int32), ("side", C.c_int32), ("bloom_max_items", C.c_uint64),
+                ("bloom_max_bytes", C.c_uint64), ("bloom_fpp", C.c_double),
+                ("bloom_max_fpp", C.c_double), ("bloom_bytes", C.c_uint64)]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        lib().kgo_filter_default_opts(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class _KfInfo(C.Structure):
+    _fields_ = [("form", C.c_int32), ("has_range", C.c_int32), ("nmembers", C.c_int64),
+                ("values", C.c_int64), ("lo_lo", C.c_uint64), ("lo_hi", C.c_uint64),
+                ("hi_lo", C.c_uint64), ("hi_hi", C.c_uint64), ("flo", C.c_double),
+                ("fhi", C.c_double), ("nulls_pass", C.c_int32), ("empty", C.c_int32),
+                ("keys", C.c_uint64), ("bloom_bytes", C.c_uint64)]
+
+
+def _i128(lo, hi):
+    v = (int(hi) << 64) | int(lo)
+    return v - (1 << 128) if v >= 1 << 127 else v
+
+
+class KeyFilter:
+    """A KeyFilterBuilder until finish(), then the KeyFilter it published
+    (runtime_filter/filter.rs:180-615)."""
+
+    def __init__(self, ctype, precision=0, scale=0, opts=None, _h=None):
+        self.opts = opts if opts is not None else FilterOpts.default()
+        self.h = _h if _h is not None else lib().kgo_kf_new(ctype, precision, scale,
+                                                           C.byref(self.opts))
+        if not self.h:
+            raise ValueError("unsupported runtime filter key type")
+
+    def add(self, col: Col, include=None):
+        inc = None if include is None else np.packbits(np.asarray(include, bool), bitorder="little")
+        c = col.c()
+        if lib().kgo_kf_add(self.h, C.byref(c), _ptr(inc), col.length) != 0:
+            raise ValueError("keys of another type")
+        return self
+
+    def merge_builder(self, other: "KeyFilter"):
+        if lib().kgo_kf_merge_builder(self.h, other.h) != 0:
+            raise ValueError("builder merge refused")
+        return self
+
+    def finish(self):
+        lib().kgo_kf_finish(self.h)
+        return self
+
+    def info(self) -> dict:
+        i = _KfInfo()
+        lib().kgo_kf_get_info(self.h, C.byref(i))
+        return {"form": i.form, "has_range": bool(i.has_range), "nmembers": i.nmembers,
+                "values": i.values, "lo": _i128(i.lo_lo, i.lo_hi), "hi": _i128(i.hi_lo, i.hi_hi),
+                "flo": i.flo, "fhi": i.fhi, "nulls_pass": bool(i.nulls_pass),
+                "empty": bool(i.empty), "keys": i.keys, "bloom_bytes": i.bloom_bytes}
+
+    def members(self):
+        n = lib().kgo_kf_members(self.h, None)
+        w = np.zeros(2 * max(n, 1), dtype=np.uint64)
+        lib().kgo_kf_members(self.h, _ptr(w))
+        return [_i128(w[2 * k], w[2 * k + 1]) for k in range(n)]
+
+    def bloom_words(self) -> np.ndarray:
+        n = lib().kgo_kf_bloom_words(self.h, None)
+        w = np.zeros(max(n, 1), dtype=np.uint32)
+        lib().kgo_kf_bloom_words(self.h, _ptr(w))
+        return w[:n]
+
+    def row_cost(self) -> float:
+        return lib().kgo_kf_row_cost(self.h)
+
+    def evaluate(self, col: Col):
+        """Per-row pass flags, or None when the column does not widen."""
+        out = np.zeros(max(col.length, 1), dtype=np.uint8)
+        c = col.c()
+        n = lib().kgo_kf_evaluate(self.h, C.byref(c), col.length, _ptr(out))
+        return None if n < 0 else out[:col.length].astype(bool)
+
+    def encode(self) -> bytes:
+        n = lib().kgo_kf_encode(self.h, None, 0)
+        buf = np.zeros(max(n, 1), dtype=np.uint8)
+        lib().kgo_kf_encode(self.h, _ptr(buf), n)
+        return buf[:n].tobytes()
+
+    @classmethod
+    def decode(cls, data: bytes) -> "KeyFilter":
+        err = C.create_string_buffer(128)
+        buf = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+        h = lib().kgo_kf_decode(_ptr(buf), len(data), err)
+        if not h:
+            raise ValueError(err.value.decode())
+        return cls(0, _h=h)
+
+    def merge(self, other: "KeyFilter", opts=None) -> "KeyFilter":
+        o = opts if opts is not None else self.opts
+        err = C.create_string_buffer(128)
+        h = lib().kgo_kf_merge(self.h, other.h, C.byref(o), err)
+        if not h:
+            raise ValueError(err.value.decode())
+        return KeyFilter(0, opts=o, _h=h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().kgo_kf_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def sbbf_pass_rate(words: np.ndarray) -> float:
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    return lib().kgo_sbbf_pass_rate(_ptr(w), len(w) // 8)
+
+
+class RowStage:
+    """A scan driver's row stage (operators/runtime_filter.rs:240-395)."""
+
+    def __init__(self, min_rows=65_536, max_pass_rate=0.9):
+        self.h = lib().kgo_rs_new(min_rows, max_pass_rate)
+        self.keep = {}
+
+    def set(self, slot, kf: KeyFilter | None, column):
+        self.keep[slot] = kf
+        lib().kgo_rs_set(self.h, slot, kf.h if kf is not None else None, column)
+
+    def start_partition(self):
+        lib().kgo_rs_start_partition(self.h)
+
+    def apply(self, cols):
+        """Pass flags of the batch's rows, or None when nothing is filtered."""
+        n = cols[0].length
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        arr = _cols(cols)
+        r = lib().kgo_rs_apply(self.h, len(cols), arr, n, _ptr(out))
+        if r == -2:
+            raise ValueError("a runtime filter's column is missing")
+        return None if r < 0 else out[:n].astype(bool)
+
+    def stats(self, slot):
+        t, d, o = C.c_uint64(), C.c_uint64(), C.c_int()
+        lib().kgo_rs_stats(self.h, slot, C.byref(t), C.byref(d), C.byref(o))
+        return t.value, d.value, bool(o.value)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().kgo_rs_free(self.h)
+                self.h = None
+        except Exception:
+            pass
