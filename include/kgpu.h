@@ -281,6 +281,112 @@ int kg_select_range(kg_ctx *ctx, const kg_column *col, int64_t nrows, int64_t lo
 int kg_gather(kg_ctx *ctx, int32_t ncols, const kg_column *cols, const int32_t *rows,
               int64_t nrows, kg_batch **out);
 
+/* ---- runtime KeyFilter: forms, codec, merge, row stage -----------------
+ * KeyFilterBuilder / KeyFilter (crates/krishiv-io/src/runtime_filter/
+ * filter.rs:180-615): a producer's keys (fixed-width key types: bool, ints,
+ * date32, timestamp, float32/64, decimal64/128) collected on the device into
+ * the min/max range, the distinct keys while few, and the row hashes for a
+ * Bloom filter; finish picks the row form (member.rs:1-20): the range for
+ * dense integers, a small set (<= 8), an exact set (<= exact_max_values; a
+ * bit per value for short integer ranges), else a split-block Bloom filter
+ * kept only if its expected fpp <= bloom_max_fpp. */
+#define KG_FORM_RANGE 0
+#define KG_FORM_SMALL 1
+#define KG_FORM_EXACT 2
+#define KG_FORM_BLOOM 3
+#define KG_FORM_DENSE 4 /* an exact set held as a bit per value */
+#define KG_SIDE_BOTH 0  /* RangeSide (filter.rs:76-90) */
+#define KG_SIDE_UPPER 1
+#define KG_SIDE_LOWER 2
+
+/* KeyFilterOptions (filter.rs:28-107). */
+typedef struct kg_filter_options {
+  uint64_t max_values;       /* statistics IN list cap (default 50) */
+  uint64_t exact_max_values; /* exact set cap (4096; at most 65536 here) */
+  int32_t bloom;             /* collect hashes for a Bloom filter (1) */
+  int32_t side;              /* KG_SIDE_* */
+  uint64_t bloom_max_items;  /* 4,000,000 */
+  uint64_t bloom_max_bytes;  /* 8 MiB */
+  double bloom_fpp;          /* 0.01 */
+  double bloom_max_fpp;      /* 0.05 */
+  uint64_t bloom_bytes;      /* the planned size; 0: sized for the keys */
+} kg_filter_options;
+void kg_filter_default_options(kg_filter_options *o);
+
+typedef struct kg_key_filter kg_key_filter;
+/* KeyFilterBuilder::new for keys of key_type (type, precision, scale). */
+int kg_key_filter_create(kg_ctx *ctx, const kg_column *key_type, const kg_filter_options *opts,
+                         kg_key_filter **out);
+/* KeyFilterBuilder::add_hashed (filter.rs:225-284): the keys of rows where
+ * `include` (optional; rows whose value is non-zero and valid) is set; a NULL
+ * key lets NULLs pass.  The column must be of the key type. */
+int kg_key_filter_add(kg_key_filter *f, const kg_column *key, const kg_column *include,
+                      int64_t nrows);
+/* KeyFilterBuilder::merge (filter.rs:478-510): `other`'s keys into `into`
+ * (both unfinished builders of one key type; `other` is left as it was). */
+int kg_key_filter_merge_builder(kg_key_filter *into, const kg_key_filter *other);
+/* KeyFilterBuilder::finish (filter.rs:526-602): the published filter. */
+int kg_key_filter_finish(kg_key_filter *f);
+
+typedef struct kg_key_filter_info {
+  int32_t form; /* KG_FORM_* */
+  int32_t has_range;
+  int64_t members; /* set forms: the members; otherwise 0 */
+  int64_t values;  /* the statistics IN list's length, -1 without one */
+  uint64_t lo[2], hi[2]; /* integer / decimal range (low, high words) */
+  double flo, fhi;       /* float range */
+  int32_t nulls_pass, empty;
+  uint64_t keys;        /* non-null keys stored, duplicates included */
+  uint64_t bloom_bytes; /* the Bloom form's size */
+  double row_cost_ns;   /* Member::row_cost_ns (member.rs:410-418) */
+} kg_key_filter_info;
+int kg_key_filter_info_get(const kg_key_filter *f, kg_key_filter_info *out);
+/* A set form's members as (low, high) words, ascending; *n in: capacity in
+ * members, out: the count (out_words NULL: the count only). */
+int kg_key_filter_members(const kg_key_filter *f, uint64_t *out_words, int64_t *n);
+/* KeyFilter::evaluate (filter.rs:764-797, member.rs:439-560): ids of the
+ * rows of `col` that may match, in order (device out_rows, host count).
+ * The column may be of a type that widens to the key type (key.rs:75-103:
+ * narrower ints, ints or float32 to float64, decimals to a wider scale);
+ * another type is KG_ERR_UNSUPPORTED (the filter does not apply). */
+int kg_key_filter_evaluate(kg_key_filter *f, const kg_column *col, int64_t nrows,
+                           int32_t *out_rows, int64_t *out_count);
+/* KeyFilter::encode (runtime_filter/codec.rs:104-180): the versioned bytes
+ * (format 1, row hash XXH64 seed 0, normalization 1).  bytes NULL: *len gets
+ * the size; else *len in: capacity, out: the size written.  Exact sets are
+ * written in ascending order. */
+int kg_key_filter_encode(const kg_key_filter *f, uint8_t *bytes, uint64_t *len);
+/* KeyFilter::decode (codec.rs:186-246): bytes of another version, type or
+ * layout are refused (KG_ERR_INTERNAL); a top-N bound is KG_ERR_UNSUPPORTED. */
+int kg_key_filter_decode(kg_ctx *ctx, const uint8_t *bytes, uint64_t len, kg_key_filter **out);
+/* KeyFilter::merge (merge.rs:35-169): the filter of the union of the keys of
+ * two published filters; equal-size planned Bloom filters OR on the device. */
+int kg_key_filter_merge(const kg_key_filter *a, const kg_key_filter *b,
+                        const kg_filter_options *opts, kg_key_filter **out);
+void kg_key_filter_destroy(kg_key_filter *f);
+
+/* RowStage (crates/krishiv-exec/src/operators/runtime_filter.rs:240-395): a
+ * scan driver's runtime filters over its batches in one fused pass, cheapest
+ * form first; a filter passing more than min(max_pass_percent / 100, 1 -
+ * row_cost / 20 ns) of a window of min_rows rows it tested is switched off
+ * until start_partition or a new filter in its slot. */
+#define KG_ROW_STAGE_SLOTS 8
+typedef struct kg_row_stage kg_row_stage;
+int kg_row_stage_create(kg_ctx *ctx, uint64_t min_rows, uint32_t max_pass_percent,
+                        kg_row_stage **out);
+/* Slot `slot` filters column `column` of the batches with `f` (a published
+ * filter the stage does not own; NULL: pending, applies nothing). */
+int kg_row_stage_set(kg_row_stage *s, int32_t slot, const kg_key_filter *f, int32_t column);
+int kg_row_stage_start_partition(kg_row_stage *s);
+/* The rows some producer may keep (device out_rows, in order); *out_count =
+ * -1 when nothing is filtered (every row passes; out_rows untouched). */
+int kg_row_stage_apply(kg_row_stage *s, int32_t ncols, const kg_column *cols, int64_t nrows,
+                       int32_t *out_rows, int64_t *out_count);
+/* Rows tested and dropped by a slot's filter, and whether it is off. */
+int kg_row_stage_stats(const kg_row_stage *s, int32_t slot, uint64_t *tested, uint64_t *dropped,
+                       int32_t *off);
+void kg_row_stage_destroy(kg_row_stage *s);
+
 /* ---- hash join ----------------------------------------------------------
  * HashJoin (crates/krishiv-exec/src/join/mod.rs:1014-1211): the build side is
  * a Sink (kg_join_build_sink per batch, then kg_join_build_finalize,

@@ -458,8 +458,15 @@ double kg_bloom_expected_fpp(const kg_bloom *b, uint64_t items) {
   if (b->spark)  // Spark's expectedFpp: (1 - e^(-k n / m))^k
     return std::pow(1.0 - std::exp(-(double)b->k * (double)items / (64.0 * (double)b->nwords)),
                     (double)b->k);
+  return kg::sbbf_expected_fpp(b->nblocks, items);
+}
+
+}  // extern "C"
+
+namespace kg {
+double sbbf_expected_fpp(uint64_t nblocks, uint64_t items) {
   // expected_fpp (bloom.rs:270-308): blocks receive Poisson(lambda) hashes
-  double lambda = (double)items / (double)b->nblocks;
+  double lambda = (double)items / (double)nblocks;
   if (lambda == 0.0) return 0.0;
   if (lambda > 1024.0) return std::pow(1.0 - std::pow(31.0 / 32.0, lambda), 8);
   double spread = 10.0 * std::sqrt(lambda) + 20.0;
@@ -480,6 +487,9 @@ double kg_bloom_expected_fpp(const kg_bloom *b, uint64_t items) {
   }
   return total < 1.0 ? total : 1.0;
 }
+}  // namespace kg
+
+extern "C" {
 
 int kg_bloom_export(kg_bloom *b, uint8_t *bytes, uint64_t *len) {
   return guarded([&] {

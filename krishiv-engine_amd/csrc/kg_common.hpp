@@ -265,6 +265,10 @@ void destroy_stream(hipStream_t s);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// RuntimeBloom::expected_fpp of a split-block filter of nblocks 32-byte
+// blocks after `items` hashes (bloom.rs:270-308).
+double sbbf_expected_fpp(uint64_t nblocks, uint64_t items);
+
 // Grid size for a streaming kernel over n items: enough workgroups to fill
 // the 256 CUs several times over, capped (grid-stride loops do the rest).
 inline unsigned stream_grid(int64_t n, int block, int64_t per_thread = 4) {

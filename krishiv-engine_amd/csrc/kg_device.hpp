@@ -93,9 +93,9 @@ constexpr uint64_t L_MIX = 0x243F6A8885A308D3ULL;      // layout.rs:45
 constexpr uint64_t L_FINISH = 0xB7E151628AED2A6BULL;   // layout.rs:46
 constexpr uint64_t L_FIN = 0x6A09E667F3BCC909ULL;      // layout.rs:50
 
-__device__ __forceinline__ uint64_t rotl(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+__host__ __device__ __forceinline__ uint64_t rotl(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
 
-__device__ __forceinline__ uint64_t avalanche(uint64_t h) {
+__host__ __device__ __forceinline__ uint64_t avalanche(uint64_t h) {
   h ^= h >> 33;
   h *= P2;
   h ^= h >> 29;
@@ -103,13 +103,13 @@ __device__ __forceinline__ uint64_t avalanche(uint64_t h) {
   return h ^ (h >> 32);
 }
 
-__device__ __forceinline__ uint64_t lane8(uint64_t h, uint64_t k) {
+__host__ __device__ __forceinline__ uint64_t lane8(uint64_t h, uint64_t k) {
   return rotl(h ^ (rotl(k * P2, 31) * P1), 27) * P1 + P4;
 }
 
 // XXH64 seed 0 of `width` (1, 2, 4, 8, 16) little-endian bytes held in
 // (lo, hi): the short-input path of routing.rs:112-142.
-__device__ __forceinline__ uint64_t xxh64_fixed(uint64_t lo, uint64_t hi, int width) {
+__host__ __device__ __forceinline__ uint64_t xxh64_fixed(uint64_t lo, uint64_t hi, int width) {
   uint64_t h = P5 + (uint64_t)width;
   if (width == 16) {
     h = lane8(lane8(h, lo), hi);
@@ -227,7 +227,7 @@ __device__ __forceinline__ uint64_t table_hash(uint64_t key, uint64_t seed) {
 }
 
 // ---------------------------------------------------------------- SBBF
-__device__ __forceinline__ uint32_t sbbf_salt(int w) {
+__host__ __device__ __forceinline__ uint32_t sbbf_salt(int w) {
   // bloom.rs:15-24 (the Parquet specification's salts)
   switch (w) {
     case 0: return 0x47b6137bu;
@@ -239,6 +239,32 @@ __device__ __forceinline__ uint32_t sbbf_salt(int w) {
     case 6: return 0x9efc4947u;
     default: return 0x5c6bfb31u;
   }
+}
+
+// RuntimeBloom::check_hash (bloom.rs:128-200): the block of h, (upper 32
+// bits * blocks) >> 32, holds all eight bits of its mask.
+__device__ __forceinline__ bool sbbf_test(const uint32_t *words, uint64_t nblocks, uint64_t h) {
+  const uint4 *b = reinterpret_cast<const uint4 *>(words + (((h >> 32) * nblocks) >> 32) * 8);
+  const uint4 x = b[0], y = b[1];
+  const uint32_t k = (uint32_t)h;
+  uint32_t miss = 0;
+  miss |= (1u << ((k * sbbf_salt(0)) >> 27)) & ~x.x;
+  miss |= (1u << ((k * sbbf_salt(1)) >> 27)) & ~x.y;
+  miss |= (1u << ((k * sbbf_salt(2)) >> 27)) & ~x.z;
+  miss |= (1u << ((k * sbbf_salt(3)) >> 27)) & ~x.w;
+  miss |= (1u << ((k * sbbf_salt(4)) >> 27)) & ~y.x;
+  miss |= (1u << ((k * sbbf_salt(5)) >> 27)) & ~y.y;
+  miss |= (1u << ((k * sbbf_salt(6)) >> 27)) & ~y.z;
+  miss |= (1u << ((k * sbbf_salt(7)) >> 27)) & ~y.w;
+  return miss == 0;
+}
+
+// RuntimeBloom::insert_hash (bloom.rs:134-145) with atomics (OR is order-free).
+__device__ __forceinline__ void sbbf_put(uint32_t *words, uint64_t nblocks, uint64_t h) {
+  uint32_t *b = words + (((h >> 32) * nblocks) >> 32) * 8;
+  const uint32_t k = (uint32_t)h;
+#pragma unroll
+  for (int w = 0; w < 8; w++) atomicOr(b + w, 1u << ((k * sbbf_salt(w)) >> 27));
 }
 
 // ---------------------------------------------------------------- generators

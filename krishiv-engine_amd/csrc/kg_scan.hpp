@@ -111,6 +111,31 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_down(const TI *in, int64_t 
   }
 }
 
+// A row bitmap (bit r % 64 of word r / 64: row r passes) expanded into the
+// passing row ids, in order: tile t holds rows [t * BLOCK * 32, (t + 1) *
+// BLOCK * 32), its first output at tile_off[t] (an exclusive scan of the
+// tiles' counts); row ids are written as row_base + row.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_bitmap_rows(const uint64_t *bitmap, int64_t n,
+                                                       const uint64_t *tile_off, int64_t row_base,
+                                                       int32_t *out_rows) {
+  __shared__ uint32_t s_w[BLOCK / 64];
+  const int64_t r0 = (int64_t)blockIdx.x * BLOCK * 32 + (int64_t)threadIdx.x * 32;
+  uint32_t bits = 0;
+  if (r0 < n) {
+    const uint64_t w = bitmap[r0 >> 6];
+    bits = (uint32_t)(((r0 >> 5) & 1) ? (w >> 32) : w);
+  }
+  uint32_t tot;
+  const uint32_t off = block_exclusive_scan<BLOCK>((uint32_t)__popc(bits), s_w, &tot);
+  int64_t pos = (int64_t)tile_off[blockIdx.x] + off;
+  while (bits) {
+    const int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    out_rows[pos++] = (int32_t)(row_base + r0 + b);
+  }
+}
+
 }  // namespace
 
 // Segment sums a device-wide scan of n inputs needs (entries of uint64).

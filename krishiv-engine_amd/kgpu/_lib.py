@@ -47,6 +47,22 @@ class Column(C.Structure):
                 ("values", C.c_void_p), ("validity", C.c_void_p)]
 
 
+class FilterOptions(C.Structure):
+    """kg_filter_options (KeyFilterOptions)."""
+    _fields_ = [("max_values", C.c_uint64), ("exact_max_values", C.c_uint64),
+                ("bloom", C.c_int32), ("side", C.c_int32), ("bloom_max_items", C.c_uint64),
+                ("bloom_max_bytes", C.c_uint64), ("bloom_fpp", C.c_double),
+                ("bloom_max_fpp", C.c_double), ("bloom_bytes", C.c_uint64)]
+
+
+class KeyFilterInfo(C.Structure):
+    _fields_ = [("form", C.c_int32), ("has_range", C.c_int32), ("members", C.c_int64),
+                ("values", C.c_int64), ("lo", C.c_uint64 * 2), ("hi", C.c_uint64 * 2),
+                ("flo", C.c_double), ("fhi", C.c_double), ("nulls_pass", C.c_int32),
+                ("empty", C.c_int32), ("keys", C.c_uint64), ("bloom_bytes", C.c_uint64),
+                ("row_cost_ns", C.c_double)]
+
+
 class AggSpec(C.Structure):
     _fields_ = [("func", C.c_int32), ("input", C.c_int32)]
 
@@ -109,6 +125,27 @@ _SIGS = {
     "kg_bloom_probe_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                          C.c_void_p]),
     "kg_bloom_destroy": (None, [C.c_void_p]),
+    "kg_filter_default_options": (None, [C.c_void_p]),
+    "kg_key_filter_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_key_filter_add": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "kg_key_filter_merge_builder": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_key_filter_finish": (C.c_int, [C.c_void_p]),
+    "kg_key_filter_info_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_key_filter_members": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_key_filter_evaluate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                         C.c_void_p]),
+    "kg_key_filter_encode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_key_filter_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kg_key_filter_merge": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_key_filter_destroy": (None, [C.c_void_p]),
+    "kg_row_stage_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
+    "kg_row_stage_set": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
+    "kg_row_stage_start_partition": (C.c_int, [C.c_void_p]),
+    "kg_row_stage_apply": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                     C.c_void_p]),
+    "kg_row_stage_stats": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "kg_row_stage_destroy": (None, [C.c_void_p]),
     "kg_select_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                   C.c_void_p, C.c_void_p]),
     "kg_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
@@ -222,3 +259,6 @@ _SIGS.update({
     "kg_join_partition": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_uint32,
                                     C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint32]),
 })
+
+FORM_RANGE, FORM_SMALL, FORM_EXACT, FORM_BLOOM, FORM_DENSE = 0, 1, 2, 3, 4
+SIDE_BOTH, SIDE_UPPER, SIDE_LOWER = 0, 1, 2

@@ -17,7 +17,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, Expr, JoinSpec, KgError, check, lib,
+from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, Expr, FilterOptions, JoinSpec,
+                   KeyFilterInfo, KgError, check, lib,
                    T_BOOL, T_DECIMAL128, T_DECIMAL64, T_INT8, T_INT16, T_INT32, T_INT64,
                    T_UINT8, T_UINT16, T_UINT32, T_UINT64, T_FLOAT32, T_FLOAT64, T_DATE32,
                    T_TIMESTAMP)
@@ -567,6 +568,155 @@ class JoinLocal:
 
 
 # ---------------------------------------------------------------- bloom
+def _i128(lo, hi):
+    v = (int(hi) << 64) | int(lo)
+    return v - (1 << 128) if v >= 1 << 127 else v
+
+
+class KeyFilter:
+    """A runtime KeyFilterBuilder on the device until finish(), then the
+    KeyFilter it published (runtime_filter/filter.rs:180-615)."""
+
+    def __init__(self, ctx: Context, key_type=None, options: FilterOptions | None = None,
+                 _handle=None):
+        self.ctx = ctx
+        if _handle is not None:
+            self.h = _handle
+            return
+        kt = _type_col(*key_type)
+        h = C.c_void_p()
+        check(lib().kg_key_filter_create(ctx.h, C.byref(kt),
+                                         C.byref(options) if options is not None else None,
+                                         C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def options(**kw) -> FilterOptions:
+        o = FilterOptions()
+        lib().kg_filter_default_options(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def add(self, key: Col, include: Col | None = None, nrows=None):
+        nrows = key.length if nrows is None else nrows
+        kc = key.c()
+        ic = include.c() if include is not None else None
+        check(lib().kg_key_filter_add(self.h, C.byref(kc),
+                                      C.byref(ic) if ic is not None else None, nrows))
+        return self
+
+    def merge_builder(self, other: "KeyFilter"):
+        check(lib().kg_key_filter_merge_builder(self.h, other.h))
+        return self
+
+    def finish(self):
+        check(lib().kg_key_filter_finish(self.h))
+        return self
+
+    def info(self) -> dict:
+        i = KeyFilterInfo()
+        check(lib().kg_key_filter_info_get(self.h, C.byref(i)))
+        return {"form": i.form, "has_range": bool(i.has_range), "nmembers": i.members,
+                "values": i.values, "lo": _i128(i.lo[0], i.lo[1]), "hi": _i128(i.hi[0], i.hi[1]),
+                "flo": i.flo, "fhi": i.fhi, "nulls_pass": bool(i.nulls_pass),
+                "empty": bool(i.empty), "keys": i.keys, "bloom_bytes": i.bloom_bytes,
+                "row_cost_ns": i.row_cost_ns}
+
+    def members(self):
+        n = C.c_int64(0)
+        check(lib().kg_key_filter_members(self.h, None, C.byref(n)))
+        w = np.zeros(2 * max(n.value, 1), dtype=np.uint64)
+        check(lib().kg_key_filter_members(self.h, C.c_void_p(w.ctypes.data), C.byref(n)))
+        return [_i128(w[2 * k], w[2 * k + 1]) for k in range(n.value)]
+
+    def evaluate(self, col: Col, nrows=None, out_rows: DeviceBuffer | None = None):
+        """Row ids that may match: (DeviceBuffer of int32 ids, count)."""
+        nrows = col.length if nrows is None else nrows
+        out = out_rows or DeviceBuffer(self.ctx, max(nrows, 1) * 4)
+        cnt = C.c_int64(0)
+        cc = col.c()
+        check(lib().kg_key_filter_evaluate(self.h, C.byref(cc), nrows, C.c_void_p(out.ptr),
+                                           C.byref(cnt)))
+        return out, cnt.value
+
+    def encode(self) -> bytes:
+        n = C.c_uint64(0)
+        check(lib().kg_key_filter_encode(self.h, None, C.byref(n)))
+        buf = (C.c_uint8 * max(n.value, 1))()
+        check(lib().kg_key_filter_encode(self.h, buf, C.byref(n)))
+        return bytes(buf)[:n.value]
+
+    @classmethod
+    def decode(cls, ctx: Context, data: bytes) -> "KeyFilter":
+        h = C.c_void_p()
+        buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data or b"\0")
+        check(lib().kg_key_filter_decode(ctx.h, buf, len(data), C.byref(h)))
+        return cls(ctx, _handle=h)
+
+    def merge(self, other: "KeyFilter", options: FilterOptions | None = None) -> "KeyFilter":
+        h = C.c_void_p()
+        check(lib().kg_key_filter_merge(self.h, other.h,
+                                        C.byref(options) if options is not None else None,
+                                        C.byref(h)))
+        return KeyFilter(self.ctx, _handle=h)
+
+    def close(self):
+        if self.h:
+            lib().kg_key_filter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RowStage:
+    """A scan driver's runtime-filter row stage (operators/runtime_filter.rs)."""
+
+    def __init__(self, ctx: Context, min_rows=65_536, max_pass_percent=90):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().kg_row_stage_create(ctx.h, min_rows, max_pass_percent, C.byref(h)))
+        self.h = h
+        self.keep = {}
+
+    def set(self, slot, f: KeyFilter | None, column):
+        self.keep[slot] = f
+        check(lib().kg_row_stage_set(self.h, slot, f.h if f is not None else None, column))
+
+    def start_partition(self):
+        check(lib().kg_row_stage_start_partition(self.h))
+
+    def apply(self, cols, nrows=None, out_rows: DeviceBuffer | None = None):
+        """(DeviceBuffer of row ids, count), or (None, -1) when nothing is
+        filtered."""
+        nrows = cols[0].length if nrows is None else nrows
+        out = out_rows or DeviceBuffer(self.ctx, max(nrows, 1) * 4)
+        cnt = C.c_int64(0)
+        check(lib().kg_row_stage_apply(self.h, len(cols), _cols(cols), nrows, C.c_void_p(out.ptr),
+                                       C.byref(cnt)))
+        return (None, -1) if cnt.value < 0 else (out, cnt.value)
+
+    def stats(self, slot):
+        t, d, o = C.c_uint64(), C.c_uint64(), C.c_int32()
+        check(lib().kg_row_stage_stats(self.h, slot, C.byref(t), C.byref(d), C.byref(o)))
+        return t.value, d.value, bool(o.value)
+
+    def close(self):
+        if self.h:
+            lib().kg_row_stage_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class RuntimeBloom:
     """The runtime filter's split-block Bloom filter (Parquet layout), or with
     spark_bits > 0 Spark's BloomFilterImpl layout of that many bits and

@@ -37,6 +37,15 @@ pub const KG_AGG_AVG: i32 = 6;
 pub const KG_AGG_ANSI: u32 = 1;
 pub const KG_AGG_PARTIAL: u32 = 2;
 pub const KG_AGG_MERGE: u32 = 4;
+pub const KG_FORM_RANGE: i32 = 0;
+pub const KG_FORM_SMALL: i32 = 1;
+pub const KG_FORM_EXACT: i32 = 2;
+pub const KG_FORM_BLOOM: i32 = 3;
+pub const KG_FORM_DENSE: i32 = 4;
+pub const KG_SIDE_BOTH: i32 = 0;
+pub const KG_SIDE_UPPER: i32 = 1;
+pub const KG_SIDE_LOWER: i32 = 2;
+pub const KG_ROW_STAGE_SLOTS: i32 = 8;
 pub const KG_JOIN_INNER: i32 = 0;
 pub const KG_JOIN_LEFT_OUTER: i32 = 1;
 pub const KG_JOIN_LEFT_SEMI: i32 = 2;
@@ -114,6 +123,45 @@ pub struct kg_hash_agg {
 
 #[repr(C)]
 pub struct kg_bloom {
+    _private: [u8; 0],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct kg_filter_options {
+    pub max_values: u64,
+    pub exact_max_values: u64,
+    pub bloom: i32,
+    pub side: i32,
+    pub bloom_max_items: u64,
+    pub bloom_max_bytes: u64,
+    pub bloom_fpp: f64,
+    pub bloom_max_fpp: f64,
+    pub bloom_bytes: u64,
+}
+
+#[repr(C)]
+pub struct kg_key_filter {
+    _private: [u8; 0],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct kg_key_filter_info {
+    pub form: i32,
+    pub has_range: i32,
+    pub members: i64,
+    pub values: i64,
+    pub hi: [uint64_t lo[2],; 2],
+    pub fhi: double flo,,
+    pub empty: int32_t nulls_pass,,
+    pub keys: u64,
+    pub bloom_bytes: u64,
+    pub row_cost_ns: f64,
+}
+
+#[repr(C)]
+pub struct kg_row_stage {
     _private: [u8; 0],
 }
 
@@ -223,6 +271,24 @@ extern "C" {
     pub fn kg_spark_bloom_create(ctx: *mut kg_ctx, num_bits: u64, num_hashes: i32, out: *mut *mut kg_bloom) -> c_int;
     pub fn kg_select_range(ctx: *mut kg_ctx, col: *const kg_column, nrows: i64, lo: i64, hi: i64, out_rows: *mut i32, out_count: *mut i64) -> c_int;
     pub fn kg_gather(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, rows: *const i32, nrows: i64, out: *mut *mut kg_batch) -> c_int;
+    pub fn kg_filter_default_options(o: *mut kg_filter_options);
+    pub fn kg_key_filter_create(ctx: *mut kg_ctx, key_type: *const kg_column, opts: *const kg_filter_options, out: *mut *mut kg_key_filter) -> c_int;
+    pub fn kg_key_filter_add(f: *mut kg_key_filter, key: *const kg_column, include: *const kg_column, nrows: i64) -> c_int;
+    pub fn kg_key_filter_merge_builder(into: *mut kg_key_filter, other: *const kg_key_filter) -> c_int;
+    pub fn kg_key_filter_finish(f: *mut kg_key_filter) -> c_int;
+    pub fn kg_key_filter_info_get(f: *const kg_key_filter, out: *mut kg_key_filter_info) -> c_int;
+    pub fn kg_key_filter_members(f: *const kg_key_filter, out_words: *mut u64, n: *mut i64) -> c_int;
+    pub fn kg_key_filter_evaluate(f: *mut kg_key_filter, col: *const kg_column, nrows: i64, out_rows: *mut i32, out_count: *mut i64) -> c_int;
+    pub fn kg_key_filter_encode(f: *const kg_key_filter, bytes: *mut u8, len: *mut u64) -> c_int;
+    pub fn kg_key_filter_decode(ctx: *mut kg_ctx, bytes: *const u8, len: u64, out: *mut *mut kg_key_filter) -> c_int;
+    pub fn kg_key_filter_merge(a: *const kg_key_filter, b: *const kg_key_filter, opts: *const kg_filter_options, out: *mut *mut kg_key_filter) -> c_int;
+    pub fn kg_key_filter_destroy(f: *mut kg_key_filter);
+    pub fn kg_row_stage_create(ctx: *mut kg_ctx, min_rows: u64, max_pass_percent: u32, out: *mut *mut kg_row_stage) -> c_int;
+    pub fn kg_row_stage_set(s: *mut kg_row_stage, slot: i32, f: *const kg_key_filter, column: i32) -> c_int;
+    pub fn kg_row_stage_start_partition(s: *mut kg_row_stage) -> c_int;
+    pub fn kg_row_stage_apply(s: *mut kg_row_stage, ncols: i32, cols: *const kg_column, nrows: i64, out_rows: *mut i32, out_count: *mut i64) -> c_int;
+    pub fn kg_row_stage_stats(s: *const kg_row_stage, slot: i32, tested: *mut u64, dropped: *mut u64, off: *mut i32) -> c_int;
+    pub fn kg_row_stage_destroy(s: *mut kg_row_stage);
     pub fn kg_join_create(ctx: *mut kg_ctx, spec: *const kg_join_spec, expected_build_rows: u64, out: *mut *mut kg_hash_join) -> c_int;
     pub fn kg_join_build_sink(j: *mut kg_hash_join, ncols: i32, cols: *const kg_column, nrows: i64) -> c_int;
     pub fn kg_join_build_finalize(j: *mut kg_hash_join) -> c_int;
