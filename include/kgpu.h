@@ -387,6 +387,43 @@ int kg_row_stage_stats(const kg_row_stage *s, int32_t slot, uint64_t *tested, ui
                        int32_t *off);
 void kg_row_stage_destroy(kg_row_stage *s);
 
+/* ---- exchange frames ------------------------------------------------------
+ * The unit of local shuffle files and the wire (crates/krishiv-shuffle/src/
+ * frame.rs:1-26): a 44-byte little-endian header -- magic "KXFR", format
+ * version 1, kind, compression 0, exchange id, stage attempt, map partition,
+ * task attempt, sequence, reduce partition, body length, CRC32C of the body
+ * -- then the body: for a Data frame an Arrow IPC stream of one batch
+ * (frame.rs:205-221), for an End frame nothing. */
+#define KG_FRAME_HEADER_BYTES 44
+#define KG_FRAME_DATA 1
+#define KG_FRAME_END 2
+#define KG_FRAME_VALUE 3
+typedef struct kg_frame_header {
+  uint64_t id; /* the exchange (or value) id */
+  uint32_t stage_attempt, map, attempt, sequence, reduce;
+  uint32_t kind; /* KG_FRAME_* (kg_exchange_frames writes Data frames) */
+} kg_frame_header;
+/* frame::header_bytes of an End frame (empty body, CRC 0): 44 bytes to out
+ * (host). */
+int kg_frame_end(const kg_frame_header *h, uint8_t *out);
+/* frame::crc32c (frame.rs:251-274) of len bytes of device memory. */
+int kg_crc32c(kg_ctx *ctx, const uint8_t *data, uint64_t len, uint32_t *out);
+/* ExchangeSink's Data frames of one split batch (exchange/mod.rs:447-501 then
+ * frame::encode, frame.rs:118-150): `cols` holds the rows grouped by reduce
+ * partition (kg_repartition's output), partition p's rows at
+ * [part_offsets[p], part_offsets[p+1]).  Each non-empty partition's rows
+ * leave as one frame: the header of `base` with reduce = p and kind Data,
+ * and an Arrow IPC stream (schema, one record batch, end marker) whose
+ * fields are named names[c] (NULL: "c0", "c1", ...), all nullable.  The
+ * frames are written back to back into `out` (device), partition p's at
+ * [frame_offsets[p], frame_offsets[p+1]) (host array of partitions + 1; an
+ * empty partition has no frame); *total gets the bytes.  out NULL: the sizes
+ * only. */
+int kg_exchange_frames(kg_ctx *ctx, int32_t ncols, const kg_column *cols,
+                       const char *const *names, const int64_t *part_offsets, uint32_t partitions,
+                       const kg_frame_header *base, uint8_t *out, uint64_t capacity,
+                       uint64_t *frame_offsets, uint64_t *total);
+
 /* ---- hash join ----------------------------------------------------------
  * HashJoin (crates/krishiv-exec/src/join/mod.rs:1014-1211): the build side is
  * a Sink (kg_join_build_sink per batch, then kg_join_build_finalize,

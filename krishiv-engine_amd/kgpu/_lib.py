@@ -63,6 +63,13 @@ class KeyFilterInfo(C.Structure):
                 ("row_cost_ns", C.c_double)]
 
 
+class FrameHeader(C.Structure):
+    """kg_frame_header (frame.rs FrameHeader)."""
+    _fields_ = [("id", C.c_uint64), ("stage_attempt", C.c_uint32), ("map", C.c_uint32),
+                ("attempt", C.c_uint32), ("sequence", C.c_uint32), ("reduce", C.c_uint32),
+                ("kind", C.c_uint32)]
+
+
 class AggSpec(C.Structure):
     _fields_ = [("func", C.c_int32), ("input", C.c_int32)]
 
@@ -146,6 +153,11 @@ _SIGS = {
     "kg_row_stage_stats": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
     "kg_row_stage_destroy": (None, [C.c_void_p]),
+    "kg_frame_end": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_crc32c": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kg_exchange_frames": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                     C.c_void_p]),
     "kg_select_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                   C.c_void_p, C.c_void_p]),
     "kg_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
@@ -262,3 +274,5 @@ _SIGS.update({
 
 FORM_RANGE, FORM_SMALL, FORM_EXACT, FORM_BLOOM, FORM_DENSE = 0, 1, 2, 3, 4
 SIDE_BOTH, SIDE_UPPER, SIDE_LOWER = 0, 1, 2
+FRAME_HEADER_BYTES = 44
+FRAME_DATA, FRAME_END, FRAME_VALUE = 1, 2, 3

@@ -17,7 +17,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, Expr, FilterOptions, JoinSpec,
+from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, Expr, FilterOptions, FrameHeader,
+                   JoinSpec,
                    KeyFilterInfo, KgError, check, lib,
                    T_BOOL, T_DECIMAL128, T_DECIMAL64, T_INT8, T_INT16, T_INT32, T_INT64,
                    T_UINT8, T_UINT16, T_UINT32, T_UINT64, T_FLOAT32, T_FLOAT64, T_DATE32,
@@ -715,6 +716,40 @@ class RowStage:
             self.close()
         except Exception:
             pass
+
+
+def exchange_frames(ctx: Context, cols, part_offsets, base: FrameHeader, names=None):
+    """The Data frames of one split batch (kg_exchange_frames): (DeviceBuffer
+    of the frames back to back, frame offsets (partitions + 1), total bytes)."""
+    offs = np.ascontiguousarray(part_offsets, dtype=np.int64)
+    parts = len(offs) - 1
+    fo = np.zeros(parts + 1, dtype=np.uint64)
+    total = C.c_uint64(0)
+    nm = None
+    if names is not None:
+        enc = [n.encode() for n in names]
+        nm = (C.c_char_p * len(enc))(*enc)
+    cc = _cols(cols)
+    check(lib().kg_exchange_frames(ctx.h, len(cols), cc, nm, C.c_void_p(offs.ctypes.data), parts,
+                                   C.byref(base), None, 0, C.c_void_p(fo.ctypes.data),
+                                   C.byref(total)))
+    out = DeviceBuffer(ctx, max(total.value, 8))
+    check(lib().kg_exchange_frames(ctx.h, len(cols), cc, nm, C.c_void_p(offs.ctypes.data), parts,
+                                   C.byref(base), C.c_void_p(out.ptr), out.nbytes,
+                                   C.c_void_p(fo.ctypes.data), C.byref(total)))
+    return out, fo, total.value
+
+
+def frame_end(h: FrameHeader) -> bytes:
+    buf = (C.c_uint8 * 44)()
+    check(lib().kg_frame_end(C.byref(h), buf))
+    return bytes(buf)
+
+
+def crc32c(ctx: Context, data: DeviceBuffer, nbytes: int) -> int:
+    out = C.c_uint32(0)
+    check(lib().kg_crc32c(ctx.h, C.c_void_p(data.ptr), nbytes, C.byref(out)))
+    return out.value
 
 
 class RuntimeBloom:

@@ -46,6 +46,10 @@ pub const KG_SIDE_BOTH: i32 = 0;
 pub const KG_SIDE_UPPER: i32 = 1;
 pub const KG_SIDE_LOWER: i32 = 2;
 pub const KG_ROW_STAGE_SLOTS: i32 = 8;
+pub const KG_FRAME_HEADER_BYTES: i32 = 44;
+pub const KG_FRAME_DATA: i32 = 1;
+pub const KG_FRAME_END: i32 = 2;
+pub const KG_FRAME_VALUE: i32 = 3;
 pub const KG_JOIN_INNER: i32 = 0;
 pub const KG_JOIN_LEFT_OUTER: i32 = 1;
 pub const KG_JOIN_LEFT_SEMI: i32 = 2;
@@ -163,6 +167,14 @@ pub struct kg_key_filter_info {
 #[repr(C)]
 pub struct kg_row_stage {
     _private: [u8; 0],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct kg_frame_header {
+    pub id: u64,
+    pub reduce: uint32_t stage_attempt, map, attempt, sequence,,
+    pub kind: u32,
 }
 
 #[repr(C)]
@@ -289,6 +301,9 @@ extern "C" {
     pub fn kg_row_stage_apply(s: *mut kg_row_stage, ncols: i32, cols: *const kg_column, nrows: i64, out_rows: *mut i32, out_count: *mut i64) -> c_int;
     pub fn kg_row_stage_stats(s: *const kg_row_stage, slot: i32, tested: *mut u64, dropped: *mut u64, off: *mut i32) -> c_int;
     pub fn kg_row_stage_destroy(s: *mut kg_row_stage);
+    pub fn kg_frame_end(h: *const kg_frame_header, out: *mut u8) -> c_int;
+    pub fn kg_crc32c(ctx: *mut kg_ctx, data: *const u8, len: u64, out: *mut u32) -> c_int;
+    pub fn kg_exchange_frames(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, names: *mut *const c_char, part_offsets: *const i64, partitions: u32, base: *const kg_frame_header, out: *mut u8, capacity: u64, frame_offsets: *mut u64, total: *mut u64) -> c_int;
     pub fn kg_join_create(ctx: *mut kg_ctx, spec: *const kg_join_spec, expected_build_rows: u64, out: *mut *mut kg_hash_join) -> c_int;
     pub fn kg_join_build_sink(j: *mut kg_hash_join, ncols: i32, cols: *const kg_column, nrows: i64) -> c_int;
     pub fn kg_join_build_finalize(j: *mut kg_hash_join) -> c_int;

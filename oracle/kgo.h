@@ -183,6 +183,25 @@ void kgo_rs_stats(const kgo_rs *s, int slot, uint64_t *tested, uint64_t *dropped
 int64_t kgo_rs_apply(kgo_rs *s, int ncols, const kgo_col *cols, int64_t n, uint8_t *out);
 
 /* ---------------------------------------------------------------------
+ * Exchange frames (kgo_frame.c; crates/krishiv-shuffle/src/frame.rs).
+ * Pinned by: the CRC32C check value of "123456789" (0xE3069283) and of the
+ * empty input (frame.rs tests.rs:122-126), a hand-assembled header, and the
+ * refusals of damaged frames (tests.rs:84-116).
+ * --------------------------------------------------------------------- */
+typedef struct kgo_frame_hdr {
+  uint64_t id;
+  uint32_t stage_attempt, map, attempt, sequence, reduce;
+  uint32_t kind; /* 1 Data, 2 End, 3 Value */
+} kgo_frame_hdr;
+enum { KGO_FRAME_ETRUNCATED = 1, KGO_FRAME_EMAGIC = 2, KGO_FRAME_EVERSION = 3,
+       KGO_FRAME_EKIND = 4, KGO_FRAME_ECOMPRESSION = 5, KGO_FRAME_ECHECKSUM = 6,
+       KGO_FRAME_EBODY = 7 };
+uint32_t kgo_crc32c(const uint8_t *p, uint64_t n);
+void kgo_frame_header(const kgo_frame_hdr *h, const uint8_t *body, uint64_t len, uint8_t out[44]);
+int kgo_frame_decode(const uint8_t *f, uint64_t n, kgo_frame_hdr *h, uint64_t *body_off,
+                     uint64_t *body_len);
+
+/* ---------------------------------------------------------------------
  * Synthetic data generators (counter-based; identical integer arithmetic on
  * the device in libkgpu).
  * --------------------------------------------------------------------- */

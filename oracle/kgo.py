@@ -213,6 +213,12 @@ def lib():
         L.kgo_rs_stats.argtypes = [vp, C.c_int, vp, vp, vp]
         L.kgo_rs_apply.restype = i64
         L.kgo_rs_apply.argtypes = [vp, C.c_int, vp, i64, vp]
+        L.kgo_crc32c.restype = u32
+        L.kgo_crc32c.argtypes = [vp, u64]
+        L.kgo_frame_header.restype = None
+        L.kgo_frame_header.argtypes = [vp, vp, u64, vp]
+        L.kgo_frame_decode.restype = C.c_int
+        L.kgo_frame_decode.argtypes = [vp, u64, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -755,3 +761,38 @@ class RowStage:
                 self.h = None
         except Exception:
             pass
+
+
+# --- exchange frames (kgo_frame.c) -----------------------------------------
+FRAME_DATA, FRAME_END, FRAME_VALUE = 1, 2, 3
+FRAME_ERRORS = {1: "truncated", 2: "magic", 3: "version", 4: "kind", 5: "compression",
+                6: "checksum", 7: "body"}
+
+
+class FrameHeader(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("stage_attempt", C.c_uint32), ("map", C.c_uint32),
+                ("attempt", C.c_uint32), ("sequence", C.c_uint32), ("reduce", C.c_uint32),
+                ("kind", C.c_uint32)]
+
+
+def crc32c(data: bytes) -> int:
+    buf = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+    return lib().kgo_crc32c(_ptr(buf), len(data))
+
+
+def frame_encode(h: FrameHeader, body: bytes) -> bytes:
+    out = np.zeros(44, dtype=np.uint8)
+    buf = np.frombuffer(body, dtype=np.uint8) if body else np.zeros(1, np.uint8)
+    lib().kgo_frame_header(C.byref(h), _ptr(buf), len(body), _ptr(out))
+    return out.tobytes() + body
+
+
+def frame_decode(frame: bytes):
+    """(header, body) of one frame; ValueError naming the refusal otherwise."""
+    h = FrameHeader()
+    off, ln = C.c_uint64(), C.c_uint64()
+    buf = np.frombuffer(frame, dtype=np.uint8) if frame else np.zeros(1, np.uint8)
+    r = lib().kgo_frame_decode(_ptr(buf), len(frame), C.byref(h), C.byref(off), C.byref(ln))
+    if r:
+        raise ValueError(FRAME_ERRORS[r])
+    return h, frame[off.value:off.value + ln.value]
