@@ -129,6 +129,79 @@ int kg_batch_column_to_host(const kg_batch *b, int32_t i, void *values, uint8_t 
  * caller enqueued itself on the columns must be complete first. */
 void kg_batch_release(kg_batch *b);
 
+/* ---- Arrow C Data Interface at the boundary ------------------------------
+ * The engine's batches are Arrow record batches (krishiv-types Batch over
+ * arrow_array); the structures below are the Arrow C Data Interface and C
+ * Device Data Interface as the Arrow specification defines them (the same
+ * layout everywhere, so a caller's own definitions are interchangeable). */
+#ifndef ARROW_C_DATA_INTERFACE
+#define ARROW_C_DATA_INTERFACE
+#define ARROW_FLAG_DICTIONARY_ORDERED 1
+#define ARROW_FLAG_NULLABLE 2
+#define ARROW_FLAG_MAP_KEYS_SORTED 4
+struct ArrowSchema {
+  const char *format;
+  const char *name;
+  const char *metadata;
+  int64_t flags;
+  int64_t n_children;
+  struct ArrowSchema **children;
+  struct ArrowSchema *dictionary;
+  void (*release)(struct ArrowSchema *);
+  void *private_data;
+};
+struct ArrowArray {
+  int64_t length;
+  int64_t null_count;
+  int64_t offset;
+  int64_t n_buffers;
+  int64_t n_children;
+  const void **buffers;
+  struct ArrowArray **children;
+  struct ArrowArray *dictionary;
+  void (*release)(struct ArrowArray *);
+  void *private_data;
+};
+#endif
+#ifndef ARROW_C_DEVICE_DATA_INTERFACE
+#define ARROW_C_DEVICE_DATA_INTERFACE
+typedef int32_t ArrowDeviceType;
+#define ARROW_DEVICE_CPU 1
+#define ARROW_DEVICE_ROCM 10
+#define ARROW_DEVICE_ROCM_HOST 11
+struct ArrowDeviceArray {
+  struct ArrowArray array;
+  int64_t device_id;
+  ArrowDeviceType device_type;
+  void *sync_event;
+  int64_t reserved[3];
+};
+#endif
+
+/* A kg_column viewing one primitive Arrow array (no copy): formats b c s i l
+ * C S I L f g tdD tsu:<tz> d:p,s[,128] d:p,s,64 (bool, ints, uints, floats,
+ * date32, microsecond timestamps, decimal128/64); other formats are
+ * KG_ERR_UNSUPPORTED.  on_device: the buffers are device pointers.  The view
+ * is valid while the array is. */
+int kg_column_from_arrow(const struct ArrowArray *array, const struct ArrowSchema *schema,
+                         int32_t on_device, kg_column *out);
+/* The same of an ArrowDeviceArray: ARROW_DEVICE_ROCM buffers are device
+ * columns (its sync_event, a hipEvent_t *, is waited for first), CPU and
+ * ROCM_HOST ones host columns. */
+int kg_column_from_arrow_device(const struct ArrowDeviceArray *array,
+                                const struct ArrowSchema *schema, kg_column *out);
+/* A batch as an Arrow struct array ("+s") of its columns, named names[c]
+ * (NULL: "c0", "c1", ...), without a copy: the batch is moved into the
+ * export (the caller no longer releases it; the array's release does).
+ * device_type ARROW_DEVICE_ROCM, device_id the context's device, sync_event
+ * NULL (the batch is complete when exported). */
+int kg_batch_export_device(kg_batch *b, const char *const *names, struct ArrowDeviceArray *out,
+                           struct ArrowSchema *schema_out);
+/* A batch copied to host memory as an Arrow struct array (the batch is left
+ * as it was). */
+int kg_batch_export_host(const kg_batch *b, const char *const *names, struct ArrowArray *out,
+                         struct ArrowSchema *schema_out);
+
 /* ---- routing hash and hash repartition (exchange) ---------------------
  * kg_hash_columns replaces routing::hash_columns
  * (crates/krishiv-kernels/src/hash/routing.rs:258-282): H_kg v1 of each row

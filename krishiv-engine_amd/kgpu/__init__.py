@@ -8,4 +8,4 @@ from .ops import (Batch, Col, Context, DeviceBuffer, GraceJoinAggregate, HashAgg
                   HashJoin, JoinAggServer, JoinLocal, KeyFilter, RowStage, autotune_joinagg, calib_atomic_add, calib_random_read,
                   tune_get, tune_load, tune_save, tune_set,
                   RuntimeBloom, gather, hash_columns, join_partition, prof_enable, prof_get, prof_reset,
-                  repartition, route, select_range, exchange_frames, frame_end, crc32c)
+                  repartition, route, select_range, exchange_frames, frame_end, crc32c, arrow_col)

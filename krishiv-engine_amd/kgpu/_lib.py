@@ -70,6 +70,39 @@ class FrameHeader(C.Structure):
                 ("kind", C.c_uint32)]
 
 
+class ArrowSchema(C.Structure):
+    pass
+
+
+ArrowSchema._fields_ = [("format", C.c_char_p), ("name", C.c_char_p), ("metadata", C.c_char_p),
+                        ("flags", C.c_int64), ("n_children", C.c_int64),
+                        ("children", C.POINTER(C.POINTER(ArrowSchema))),
+                        ("dictionary", C.POINTER(ArrowSchema)),
+                        ("release", C.CFUNCTYPE(None, C.POINTER(ArrowSchema))),
+                        ("private_data", C.c_void_p)]
+
+
+class ArrowArray(C.Structure):
+    pass
+
+
+ArrowArray._fields_ = [("length", C.c_int64), ("null_count", C.c_int64), ("offset", C.c_int64),
+                       ("n_buffers", C.c_int64), ("n_children", C.c_int64),
+                       ("buffers", C.POINTER(C.c_void_p)),
+                       ("children", C.POINTER(C.POINTER(ArrowArray))),
+                       ("dictionary", C.POINTER(ArrowArray)),
+                       ("release", C.CFUNCTYPE(None, C.POINTER(ArrowArray))),
+                       ("private_data", C.c_void_p)]
+
+
+class ArrowDeviceArray(C.Structure):
+    _fields_ = [("array", ArrowArray), ("device_id", C.c_int64), ("device_type", C.c_int32),
+                ("sync_event", C.c_void_p), ("reserved", C.c_int64 * 3)]
+
+
+ARROW_DEVICE_CPU, ARROW_DEVICE_ROCM, ARROW_DEVICE_ROCM_HOST = 1, 10, 11
+
+
 class AggSpec(C.Structure):
     _fields_ = [("func", C.c_int32), ("input", C.c_int32)]
 
@@ -153,6 +186,10 @@ _SIGS = {
     "kg_row_stage_stats": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
     "kg_row_stage_destroy": (None, [C.c_void_p]),
+    "kg_column_from_arrow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "kg_column_from_arrow_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_batch_export_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_batch_export_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_frame_end": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kg_crc32c": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kg_exchange_frames": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -17,8 +17,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, Column, Expr, FilterOptions, FrameHeader,
-                   JoinSpec,
+from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, ArrowArray, ArrowDeviceArray, ArrowSchema,
+                   Column, Expr, FilterOptions, FrameHeader, JoinSpec,
                    KeyFilterInfo, KgError, check, lib,
                    T_BOOL, T_DECIMAL128, T_DECIMAL64, T_INT8, T_INT16, T_INT32, T_INT64,
                    T_UINT8, T_UINT16, T_UINT32, T_UINT64, T_FLOAT32, T_FLOAT64, T_DATE32,
@@ -195,6 +195,52 @@ def _cols(cols):
     return arr
 
 
+def _names(names):
+    if names is None:
+        return None
+    enc = [n.encode() for n in names]
+    return (C.c_char_p * len(enc))(*enc)
+
+
+def arrow_col(array, on_device: bool = False, schema: ArrowSchema | None = None) -> "Col":
+    """A Col viewing an Arrow array without a copy (kg_column_from_arrow):
+    a pyarrow Array (host), or an exported (ArrowArray, ArrowSchema) /
+    (ArrowDeviceArray, ArrowSchema) pair."""
+    keep = [array]
+    out = Column()
+    if isinstance(array, ArrowDeviceArray):
+        check(lib().kg_column_from_arrow_device(C.byref(array), C.byref(schema), C.byref(out)))
+    elif isinstance(array, ArrowArray):
+        check(lib().kg_column_from_arrow(C.byref(array), C.byref(schema), int(on_device),
+                                         C.byref(out)))
+    else:
+        arr, sch = ArrowArray(), ArrowSchema()
+        array._export_to_c(C.addressof(arr), C.addressof(sch))
+        keep = [array, _Released(arr, sch)]
+        check(lib().kg_column_from_arrow(C.byref(arr), C.byref(sch), 0, C.byref(out)))
+    c = Col.__new__(Col)
+    c.type, c.precision, c.scale = out.type, out.precision, out.scale
+    c.length, c.offset, c.device = out.length, out.offset, bool(out.flags & COL_DEVICE)
+    c.vptr, c.bptr, c._keep = out.values, out.validity, keep
+    return c
+
+
+class _Released:
+    """Calls an exported Arrow array's and schema's release when dropped."""
+
+    def __init__(self, arr, sch):
+        self.arr, self.sch = arr, sch
+
+    def __del__(self):
+        try:
+            if self.arr.release:
+                self.arr.release(C.pointer(self.arr))
+            if self.sch.release:
+                self.sch.release(C.pointer(self.sch))
+        except Exception:
+            pass
+
+
 def _type_col(ctype, precision=0, scale=0) -> Column:
     return Column(ctype, precision, scale, 0, 0, 0, None, None)
 
@@ -237,6 +283,21 @@ class Batch:
         c.length, c.offset, c.device = d.length, 0, True
         c.vptr, c.bptr, c._keep = d.values, d.validity, [self]
         return c
+
+    def to_arrow(self, names=None):
+        """A pyarrow RecordBatch copied to host memory (kg_batch_export_host)."""
+        import pyarrow as pa
+        arr, sch = ArrowArray(), ArrowSchema()
+        check(lib().kg_batch_export_host(self.h, _names(names), C.byref(arr), C.byref(sch)))
+        return pa.RecordBatch._import_from_c(C.addressof(arr), C.addressof(sch))
+
+    def export_device(self, names=None):
+        """(ArrowDeviceArray, ArrowSchema) in HBM without a copy; the batch
+        moves into the export (call the array's release to free it)."""
+        arr, sch = ArrowDeviceArray(), ArrowSchema()
+        check(lib().kg_batch_export_device(self.h, _names(names), C.byref(arr), C.byref(sch)))
+        self.h = None
+        return arr, sch
 
     def column(self, i):
         """(values, valid) on the host: numpy values (python ints for

@@ -26,21 +26,42 @@ def strip_comments(text: str) -> str:
 
 
 def rust_type(ctype: str) -> str:
-    """A C parameter or field type (without its name) as a Rust type."""
-    t = ctype.strip()
-    stars = t.count("*")
-    t = t.replace("*", " ").split()
-    const = "const" in t
-    t = [w for w in t if w not in ("const", "struct")]
-    base = " ".join(t)
-    rb = SCALAR.get(base, base)
-    if stars == 0:
-        if rb == "c_void":
-            return "()"
-        return rb
-    if stars == 1:
-        return ("*const " if const else "*mut ") + rb
-    return "*mut " * (stars - 1) + ("*const " if const else "*mut ") + rb
+    """A C parameter or field type (without its name) as a Rust type: each
+    pointer is *const when what it points to is const."""
+    toks = ctype.replace("*", " * ").split()
+    if "*" not in toks:
+        base = " ".join(w for w in toks if w not in ("const", "struct"))
+        rb = SCALAR.get(base, base)
+        return "()" if rb == "c_void" else rb
+    first = toks.index("*")
+    base_toks = toks[:first]
+    base = " ".join(w for w in base_toks if w not in ("const", "struct"))
+    t = SCALAR.get(base, base)
+    pointee_const = "const" in base_toks
+    rest = toks[first:]
+    for k, tok in enumerate(rest):
+        if tok != "*":
+            continue
+        t = ("*const " if pointee_const else "*mut ") + t
+        pointee_const = k + 1 < len(rest) and rest[k + 1] == "const"
+    return t
+
+
+def field_line(f: str) -> str:
+    """One struct field as a Rust field (arrays and function pointers too)."""
+    fp = re.match(r"(.*?)\(\s*\*\s*(\w+)\s*\)\s*\((.*)\)$", f)
+    if fp:
+        ret, fname, args = fp.groups()
+        params = [rust_type(a) for a in args.split(",") if a.strip() and a.strip() != "void"]
+        rr = rust_type(ret)
+        sig = f"unsafe extern \"C\" fn({', '.join(params)})" + ("" if rr == "()" else f" -> {rr}")
+        return f"    pub {ident(fname)}: Option<{sig}>,"
+    arr = re.match(r"(.*?)(\w+)\s*\[(\d+)\]$", f)
+    if arr:
+        ft, fname, n = arr.groups()
+        return f"    pub {ident(fname)}: [{rust_type(ft)}; {n}],"
+    ft, fname = split_param(f)
+    return f"    pub {ident(fname)}: {rust_type(ft)},"
 
 
 RUST_KEYWORDS = {"type", "in", "fn", "ref", "match", "loop", "move", "where", "use", "mod",
@@ -69,7 +90,7 @@ def generate() -> str:
            "",
            "use std::os::raw::{c_char, c_int, c_void};",
            ""]
-    for m in re.finditer(r"#define\s+(KG_[A-Z0-9_]+)\s+([0-9]+)(u?)\b", src):
+    for m in re.finditer(r"#define\s+((?:KG|ARROW)_[A-Z0-9_]+)\s+([0-9]+)(u?)\b", src):
         name, val, uns = m.groups()
         out.append(f"pub const {name}: {'u32' if uns else 'i32'} = {val};")
     out.append("")
@@ -79,6 +100,10 @@ def generate() -> str:
         items.append((m.start(), "struct", m))
     for m in re.finditer(r"typedef\s+struct\s+(\w+)\s+(\w+)\s*;", src):
         items.append((m.start(), "opaque", m))
+    for m in re.finditer(r"(?<!typedef )\bstruct\s+(\w+)\s*\{(.*?)\}\s*;", src, flags=re.S):
+        items.append((m.start(), "cstruct", m))
+    for m in re.finditer(r"typedef\s+(int32_t|int64_t|uint32_t|uint64_t)\s+(\w+)\s*;", src):
+        items.append((m.start(), "alias", m))
     decl = re.compile(r"^\s*([A-Za-z_][\w\s\*]*?[\s\*])(kg_\w+)\s*\(([^;]*?)\)\s*;", flags=re.M | re.S)
     for m in decl.finditer(src):
         items.append((m.start(), "fn", m))
@@ -87,20 +112,13 @@ def generate() -> str:
     for _, kind, m in items:
         if kind == "opaque":
             out.append(f"#[repr(C)]\npub struct {m.group(2)} {{\n    _private: [u8; 0],\n}}\n")
-        elif kind == "struct":
-            fields = []
-            for f in m.group(2).split(";"):
-                f = " ".join(f.split())
-                if not f:
-                    continue
-                arr = re.match(r"(.*?)(\w+)\s*\[(\d+)\]$", f)
-                if arr:
-                    ft, fname, n = arr.groups()
-                    fields.append(f"    pub {ident(fname)}: [{rust_type(ft)}; {n}],")
-                else:
-                    ft, fname = split_param(f)
-                    fields.append(f"    pub {ident(fname)}: {rust_type(ft)},")
-            out.append("#[repr(C)]\n#[derive(Clone, Copy, Debug)]\npub struct " + m.group(3) +
+        elif kind == "alias":
+            out.append(f"pub type {m.group(2)} = {SCALAR[m.group(1)]};\n")
+        elif kind in ("struct", "cstruct"):
+            body = m.group(2)
+            name = m.group(3) if kind == "struct" else m.group(1)
+            fields = [field_line(" ".join(f.split())) for f in body.split(";") if f.strip()]
+            out.append("#[repr(C)]\n#[derive(Clone, Copy, Debug)]\npub struct " + name +
                        " {\n" + "\n".join(fields) + "\n}\n")
         else:
             ret, name, args = m.group(1), m.group(2), " ".join(m.group(3).split())

@@ -28,6 +28,12 @@ pub const KG_T_TIMESTAMP: i32 = 13;
 pub const KG_T_DECIMAL64: i32 = 14;
 pub const KG_T_DECIMAL128: i32 = 15;
 pub const KG_COL_DEVICE: u32 = 1;
+pub const ARROW_FLAG_DICTIONARY_ORDERED: i32 = 1;
+pub const ARROW_FLAG_NULLABLE: i32 = 2;
+pub const ARROW_FLAG_MAP_KEYS_SORTED: i32 = 4;
+pub const ARROW_DEVICE_CPU: i32 = 1;
+pub const ARROW_DEVICE_ROCM: i32 = 10;
+pub const ARROW_DEVICE_ROCM_HOST: i32 = 11;
 pub const KG_AGG_COUNT_STAR: i32 = 1;
 pub const KG_AGG_COUNT: i32 = 2;
 pub const KG_AGG_SUM: i32 = 3;
@@ -111,6 +117,47 @@ pub struct kg_mem_stats {
 #[repr(C)]
 pub struct kg_batch {
     _private: [u8; 0],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct ArrowSchema {
+    pub format: *const c_char,
+    pub name: *const c_char,
+    pub metadata: *const c_char,
+    pub flags: i64,
+    pub n_children: i64,
+    pub children: *mut *mut ArrowSchema,
+    pub dictionary: *mut ArrowSchema,
+    pub release: Option<unsafe extern "C" fn(*mut ArrowSchema)>,
+    pub private_data: *mut c_void,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct ArrowArray {
+    pub length: i64,
+    pub null_count: i64,
+    pub offset: i64,
+    pub n_buffers: i64,
+    pub n_children: i64,
+    pub buffers: *mut *const c_void,
+    pub children: *mut *mut ArrowArray,
+    pub dictionary: *mut ArrowArray,
+    pub release: Option<unsafe extern "C" fn(*mut ArrowArray)>,
+    pub private_data: *mut c_void,
+}
+
+pub type ArrowDeviceType = i32;
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct ArrowDeviceArray {
+    pub array: ArrowArray,
+    pub device_id: i64,
+    pub device_type: ArrowDeviceType,
+    pub sync_event: *mut c_void,
+    pub reserved: [i64; 3],
 }
 
 #[repr(C)]
@@ -257,6 +304,10 @@ extern "C" {
     pub fn kg_batch_column(b: *const kg_batch, i: i32, out: *mut kg_column) -> c_int;
     pub fn kg_batch_column_to_host(b: *const kg_batch, i: i32, values: *mut c_void, valid_bytes: *mut u8) -> c_int;
     pub fn kg_batch_release(b: *mut kg_batch);
+    pub fn kg_column_from_arrow(array: *const ArrowArray, schema: *const ArrowSchema, on_device: i32, out: *mut kg_column) -> c_int;
+    pub fn kg_column_from_arrow_device(array: *const ArrowDeviceArray, schema: *const ArrowSchema, out: *mut kg_column) -> c_int;
+    pub fn kg_batch_export_device(b: *mut kg_batch, names: *const *const c_char, out: *mut ArrowDeviceArray, schema_out: *mut ArrowSchema) -> c_int;
+    pub fn kg_batch_export_host(b: *const kg_batch, names: *const *const c_char, out: *mut ArrowArray, schema_out: *mut ArrowSchema) -> c_int;
     pub fn kg_hash_columns(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, nrows: i64, out_hashes: *mut u64, out_flags: u32) -> c_int;
     pub fn kg_partition(ctx: *mut kg_ctx, nkeys: i32, keys: *const kg_column, nrows: i64, partitions: u32, out_pids: *mut u32, out_flags: u32) -> c_int;
     pub fn kg_join_partition(ctx: *mut kg_ctx, nkeys: i32, keys: *const kg_column, nrows: i64, radix_bits: u32, depth: u32, seed: u64, out_hashes: *mut u64, out_parts: *mut u8, out_flags: u32) -> c_int;
@@ -303,7 +354,7 @@ extern "C" {
     pub fn kg_row_stage_destroy(s: *mut kg_row_stage);
     pub fn kg_frame_end(h: *const kg_frame_header, out: *mut u8) -> c_int;
     pub fn kg_crc32c(ctx: *mut kg_ctx, data: *const u8, len: u64, out: *mut u32) -> c_int;
-    pub fn kg_exchange_frames(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, names: *mut *const c_char, part_offsets: *const i64, partitions: u32, base: *const kg_frame_header, out: *mut u8, capacity: u64, frame_offsets: *mut u64, total: *mut u64) -> c_int;
+    pub fn kg_exchange_frames(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, names: *const *const c_char, part_offsets: *const i64, partitions: u32, base: *const kg_frame_header, out: *mut u8, capacity: u64, frame_offsets: *mut u64, total: *mut u64) -> c_int;
     pub fn kg_join_create(ctx: *mut kg_ctx, spec: *const kg_join_spec, expected_build_rows: u64, out: *mut *mut kg_hash_join) -> c_int;
     pub fn kg_join_build_sink(j: *mut kg_hash_join, ncols: i32, cols: *const kg_column, nrows: i64) -> c_int;
     pub fn kg_join_build_finalize(j: *mut kg_hash_join) -> c_int;
