@@ -223,6 +223,13 @@ int kg_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrow
 int kg_join_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrows,
                       uint32_t radix_bits, uint32_t depth, uint64_t seed, uint64_t *out_hashes,
                       uint8_t *out_parts, uint32_t out_flags);
+/* The split alone (exchange/mod.rs:480-501) with the partition of each row
+ * given, one byte a row in device memory (< partitions <= 256): the rows of
+ * `cols` regrouped by partition, stable within a partition; out_offsets
+ * (host, partitions+1 entries). */
+int kg_split_partitions(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
+                        const uint8_t *partition_of_row, uint32_t partitions,
+                        int64_t *out_offsets, kg_batch **out);
 /* ExchangeSink::sink's route + split (exchange/mod.rs:447-501): the rows of
  * `cols` regrouped by partition, stable within a partition.  The output batch
  * holds the permuted columns; out_offsets (host, partitions+1 entries) gives
@@ -620,6 +627,48 @@ int kg_join_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t
 int kg_join_probe_selected(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                            const int32_t *sel_rows, int64_t nsel, kg_batch **out);
 void kg_join_destroy(kg_hash_join *j);
+
+/* ---- grace hash join ------------------------------------------------------
+ * GraceHashJoin (crates/krishiv-exec/src/join/grace.rs:1-20,828-1031): a
+ * HashJoin (same spec, join types, residual filter and output as
+ * kg_hash_join) whose build rows may pass `table_budget` bytes of HBM.
+ * Under the budget it is the in-memory join.  Past it every build row, and
+ * then every probe row, is partitioned on the top 6 bits of the row hash h
+ * (Hashing::partition level 0, keys.rs:118-128) into pinned host memory;
+ * after the last probe batch (kg_grace_join_finish) each partition is
+ * joined in HBM -- a partition whose build rows do not fit is split 8 ways
+ * on the next 3 bits of g (levels 1..6, MAX_DEPTH, grace.rs:1481-1522),
+ * and a semi or anti partition with at least 8 build rows a probe row is
+ * joined with roles reversed (grace.rs:497-548).  Output rows are those of
+ * the in-memory join (order within the output aside). */
+typedef struct kg_grace_join kg_grace_join;
+int kg_grace_join_create(kg_ctx *ctx, const kg_join_spec *spec, uint64_t table_budget,
+                         kg_grace_join **out);
+int kg_grace_join_set_filter(kg_grace_join *g, int32_t nops, const kg_expr *ops);
+int kg_grace_join_build_sink(kg_grace_join *g, int32_t ncols, const kg_column *cols,
+                             int64_t nrows);
+int kg_grace_join_build_finalize(kg_grace_join *g);
+/* A probe batch (the previous one's output taken first): in memory its
+ * output follows with kg_grace_join_next; spilled, its rows go to their
+ * partitions and come out after finish. */
+int kg_grace_join_probe_push(kg_grace_join *g, int32_t ncols, const kg_column *cols,
+                             int64_t nrows);
+/* Every probe batch pushed: the preserved build rows and the spilled
+ * partitions follow with kg_grace_join_next. */
+int kg_grace_join_finish(kg_grace_join *g);
+/* The next output batch of at most max_rows rows; *done = 1 when what was
+ * pushed (or, after finish, everything) is out (*out may then be NULL). */
+int kg_grace_join_next(kg_grace_join *g, int64_t max_rows, kg_batch **out, int32_t *done);
+typedef struct kg_grace_join_stats {
+  int32_t spilled;            /* the build passed the budget */
+  uint32_t max_depth;         /* deepest repartitioning level joined */
+  uint64_t partitions_joined; /* partitions joined in HBM (Counter::GracePartitions) */
+  uint64_t repartitions;      /* partitions split again (Counter::Repartitions) */
+  uint64_t reversed;          /* joined with roles reversed (Counter::ReversedPartitions) */
+  uint64_t spill_bytes;       /* bytes written to host memory */
+} kg_grace_join_stats;
+int kg_grace_join_stats_get(const kg_grace_join *g, kg_grace_join_stats *out);
+void kg_grace_join_destroy(kg_grace_join *g);
 
 /* ---- join fused with GROUP BY the build key -----------------------------
  * The plan Aggregate(GROUP BY build key; SUM/COUNT/MIN/MAX/AVG of probe

@@ -77,6 +77,26 @@ __global__ void __launch_bounds__(kPartBlock) k_route_hist(KeyCols k, int64_t n,
     hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_hist[p];
 }
 
+// The same histogram over partition ids given one byte a row (a join's
+// grace partitions, Hashing::partition).
+__global__ void __launch_bounds__(kPartBlock) k_pid_hist(const uint8_t *given, int64_t n,
+                                                         uint32_t parts, int64_t per_block,
+                                                         uint32_t *pids, uint32_t *hist) {
+  extern __shared__ uint32_t s_hist[];
+  for (uint32_t p = threadIdx.x; p < parts; p += blockDim.x) s_hist[p] = 0;
+  __syncthreads();
+  int64_t lo = blockIdx.x * per_block;
+  int64_t hi = lo + per_block < n ? lo + per_block : n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const uint32_t p = given[i] < parts ? given[i] : parts - 1;
+    pids[i] = p;
+    atomicAdd(&s_hist[p], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < parts; p += blockDim.x)
+    hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_hist[p];
+}
+
 __device__ __forceinline__ void copy_value(const DCol &in, int64_t i, void *out, int64_t pos) {
   int64_t j = i + in.offset;
   switch (in.width) {
@@ -343,11 +363,17 @@ int kg_partition(kg_ctx *ctx, int32_t nkeys, const kg_column *keys, int64_t nrow
   });
 }
 
-int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
-                   int32_t nkeys, const int32_t *key_cols, uint32_t partitions,
-                   int64_t *out_offsets, kg_batch **out) {
-  return guarded([&] {
-    *out = nullptr;
+}  // extern "C"
+
+namespace kg {
+// The rows of `cols` regrouped by partition, stable within a partition:
+// partitions routed from the key columns (given NULL), or given one byte a
+// row (device).
+static std::unique_ptr<kg_batch> split_rows(kg_ctx *ctx, int32_t ncols, const kg_column *cols,
+                                            int64_t nrows, int32_t nkeys, const int32_t *key_cols,
+                                            const uint8_t *given, uint32_t partitions,
+                                            int64_t *out_offsets) {
+  {
     if (ncols < 1 || ncols > kMaxCols) fail(KG_ERR_UNSUPPORTED, "1 to 32 columns");
     if (partitions < 1) partitions = 1;
     if (partitions > kMaxPartitions)
@@ -389,7 +415,7 @@ int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nr
         cc.out_valid[c] = oc.validity.get<uint32_t>();
       }
     }
-    if (nrows == 0 || partitions == 1 || nkeys == 0) {
+    if (nrows == 0 || partitions == 1 || (nkeys == 0 && !given)) {
       // one partition: the rows in order (route() puts every row in 0)
       for (uint32_t p = 0; p <= partitions; p++) out_offsets[p] = p == 0 ? 0 : nrows;
       if (nrows > 0) {
@@ -405,8 +431,7 @@ int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nr
         KG_HIP(hipGetLastError());
         KG_HIP(hipStreamSynchronize(s));
       }
-      *out = b.release();
-      return;
+      return b;
     }
     // block count: the histogram is partitions x blocks entries
     int64_t nb = ceil_div(nrows, 4096);
@@ -421,8 +446,12 @@ int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nr
     DevBuf hist(ctx, hn * 4), base(ctx, hn * 8);
     {
       ProfScope prof("hash_partition", s);
-      k_route_hist<<<(unsigned)nb, kPartBlock, partitions * 4, s>>>(
-          k, nrows, partitions, per_block, pids.get<uint32_t>(), hist.get<uint32_t>());
+      if (given)
+        k_pid_hist<<<(unsigned)nb, kPartBlock, partitions * 4, s>>>(
+            given, nrows, partitions, per_block, pids.get<uint32_t>(), hist.get<uint32_t>());
+      else
+        k_route_hist<<<(unsigned)nb, kPartBlock, partitions * 4, s>>>(
+            k, nrows, partitions, per_block, pids.get<uint32_t>(), hist.get<uint32_t>());
       KG_HIP(hipGetLastError());
     }
     DevBuf temp(ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
@@ -444,8 +473,35 @@ int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nr
     KG_HIP(hipStreamSynchronize(s));
     for (uint32_t p = 0; p < partitions; p++) out_offsets[p] = (int64_t)firsts[p];
     out_offsets[partitions] = nrows;
-    *out = b.release();
+    return b;
+  }
+}
+}  // namespace kg
+
+extern "C" {
+
+int kg_repartition(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
+                   int32_t nkeys, const int32_t *key_cols, uint32_t partitions,
+                   int64_t *out_offsets, kg_batch **out) {
+  return guarded([&] {
+    *out = nullptr;
+    *out = split_rows(ctx, ncols, cols, nrows, nkeys, key_cols, nullptr, partitions, out_offsets)
+               .release();
   });
 }
+
+int kg_split_partitions(kg_ctx *ctx, int32_t ncols, const kg_column *cols, int64_t nrows,
+                        const uint8_t *partition_of_row, uint32_t partitions,
+                        int64_t *out_offsets, kg_batch **out) {
+  return guarded([&] {
+    *out = nullptr;
+    if (!partition_of_row) fail(KG_ERR_INTERNAL, "no partition ids");
+    if (partitions > 256) fail(KG_ERR_UNSUPPORTED, "one-byte partition ids: at most 256");
+    *out = split_rows(ctx, ncols, cols, nrows, 0, nullptr, partition_of_row, partitions,
+                      out_offsets)
+               .release();
+  });
+}
+
 
 }  // extern "C"

@@ -5,7 +5,7 @@ mirror of the engine's operator interface over that ABI."""
 from ._lib import KgError, lib, exported_symbols  # noqa: F401
 from . import _lib as abi  # noqa: F401
 from .ops import (Batch, Col, Context, DeviceBuffer, GraceJoinAggregate, HashAggregate,  # noqa: F401
-                  HashJoin, JoinAggServer, JoinLocal, KeyFilter, RowStage, autotune_joinagg, calib_atomic_add, calib_random_read,
+                  HashJoin, GraceJoin, JoinAggServer, JoinLocal, KeyFilter, RowStage, autotune_joinagg, calib_atomic_add, calib_random_read,
                   tune_get, tune_load, tune_save, tune_set,
                   RuntimeBloom, gather, hash_columns, join_partition, prof_enable, prof_get, prof_reset,
                   repartition, route, select_range, exchange_frames, frame_end, crc32c, arrow_col)

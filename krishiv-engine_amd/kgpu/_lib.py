@@ -103,6 +103,12 @@ class ArrowDeviceArray(C.Structure):
 ARROW_DEVICE_CPU, ARROW_DEVICE_ROCM, ARROW_DEVICE_ROCM_HOST = 1, 10, 11
 
 
+class GraceJoinStats(C.Structure):
+    _fields_ = [("spilled", C.c_int32), ("max_depth", C.c_uint32),
+                ("partitions_joined", C.c_uint64), ("repartitions", C.c_uint64),
+                ("reversed", C.c_uint64), ("spill_bytes", C.c_uint64)]
+
+
 class AggSpec(C.Structure):
     _fields_ = [("func", C.c_int32), ("input", C.c_int32)]
 
@@ -190,6 +196,17 @@ _SIGS = {
     "kg_column_from_arrow_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_batch_export_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_batch_export_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kg_split_partitions": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                      C.c_uint32, C.c_void_p, C.c_void_p]),
+    "kg_grace_join_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kg_grace_join_set_filter": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "kg_grace_join_build_sink": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    "kg_grace_join_build_finalize": (C.c_int, [C.c_void_p]),
+    "kg_grace_join_probe_push": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    "kg_grace_join_finish": (C.c_int, [C.c_void_p]),
+    "kg_grace_join_next": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "kg_grace_join_stats_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kg_grace_join_destroy": (None, [C.c_void_p]),
     "kg_frame_end": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kg_crc32c": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kg_exchange_frames": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,

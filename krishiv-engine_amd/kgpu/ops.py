@@ -18,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (AGG_ANSI, COL_DEVICE, WIDTH, AggSpec, ArrowArray, ArrowDeviceArray, ArrowSchema,
-                   Column, Expr, FilterOptions, FrameHeader, JoinSpec,
+                   Column, Expr, FilterOptions, FrameHeader, GraceJoinStats, JoinSpec,
                    KeyFilterInfo, KgError, check, lib,
                    T_BOOL, T_DECIMAL128, T_DECIMAL64, T_INT8, T_INT16, T_INT32, T_INT64,
                    T_UINT8, T_UINT16, T_UINT32, T_UINT64, T_FLOAT32, T_FLOAT64, T_DATE32,
@@ -574,6 +574,74 @@ class HashJoin:
     def close(self):
         if self.h:
             lib().kg_join_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GraceJoin:
+    """A HashJoin whose build side may pass an HBM budget (kg_grace_join):
+    partitions spill to pinned host memory and are joined after finish()."""
+
+    def __init__(self, ctx: Context, join_type, probe_keys, build_keys, table_budget,
+                 null_safe=None):
+        self.ctx = ctx
+        spec = JoinSpec()
+        spec.join_type = join_type
+        spec.num_keys = len(probe_keys)
+        spec.no_direct_index = 0
+        for i, (p, b) in enumerate(zip(probe_keys, build_keys)):
+            spec.probe_keys[i] = p
+            spec.build_keys[i] = b
+            spec.null_safe[i] = int(bool(null_safe[i])) if null_safe else 0
+        h = C.c_void_p()
+        check(lib().kg_grace_join_create(ctx.h, C.byref(spec), table_budget, C.byref(h)))
+        self.h = h
+
+    def set_filter(self, program):
+        ops = (Expr * max(1, len(program)))(*[Expr(*x) for x in program])
+        check(lib().kg_grace_join_set_filter(self.h, len(program), ops))
+
+    def build_sink(self, cols, nrows=None):
+        nrows = cols[0].length if nrows is None else nrows
+        check(lib().kg_grace_join_build_sink(self.h, len(cols), _cols(cols), nrows))
+
+    def build_finalize(self):
+        check(lib().kg_grace_join_build_finalize(self.h))
+
+    def _drain(self, max_rows):
+        out = []
+        while True:
+            h, done = C.c_void_p(), C.c_int32(0)
+            check(lib().kg_grace_join_next(self.h, max_rows, C.byref(h), C.byref(done)))
+            if h.value:
+                out.append(Batch(h, self.ctx))
+            if done.value:
+                return out
+
+    def probe(self, cols, nrows=None, max_rows=1 << 20):
+        """Pushes a probe batch; its output batches (none while spilled)."""
+        nrows = cols[0].length if nrows is None else nrows
+        check(lib().kg_grace_join_probe_push(self.h, len(cols), _cols(cols), nrows))
+        return self._drain(max_rows)
+
+    def finish(self, max_rows=1 << 20):
+        """Every probe batch pushed: the remaining output batches."""
+        check(lib().kg_grace_join_finish(self.h))
+        return self._drain(max_rows)
+
+    def stats(self) -> dict:
+        st = GraceJoinStats()
+        check(lib().kg_grace_join_stats_get(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in GraceJoinStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().kg_grace_join_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -254,6 +254,22 @@ pub struct kg_join_local {
 }
 
 #[repr(C)]
+pub struct kg_grace_join {
+    _private: [u8; 0],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct kg_grace_join_stats {
+    pub spilled: i32,
+    pub max_depth: u32,
+    pub partitions_joined: u64,
+    pub repartitions: u64,
+    pub reversed: u64,
+    pub spill_bytes: u64,
+}
+
+#[repr(C)]
 pub struct kg_grace {
     _private: [u8; 0],
 }
@@ -311,6 +327,7 @@ extern "C" {
     pub fn kg_hash_columns(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, nrows: i64, out_hashes: *mut u64, out_flags: u32) -> c_int;
     pub fn kg_partition(ctx: *mut kg_ctx, nkeys: i32, keys: *const kg_column, nrows: i64, partitions: u32, out_pids: *mut u32, out_flags: u32) -> c_int;
     pub fn kg_join_partition(ctx: *mut kg_ctx, nkeys: i32, keys: *const kg_column, nrows: i64, radix_bits: u32, depth: u32, seed: u64, out_hashes: *mut u64, out_parts: *mut u8, out_flags: u32) -> c_int;
+    pub fn kg_split_partitions(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, nrows: i64, partition_of_row: *const u8, partitions: u32, out_offsets: *mut i64, out: *mut *mut kg_batch) -> c_int;
     pub fn kg_repartition(ctx: *mut kg_ctx, ncols: i32, cols: *const kg_column, nrows: i64, nkeys: i32, key_cols: *const i32, partitions: u32, out_offsets: *mut i64, out: *mut *mut kg_batch) -> c_int;
     pub fn kg_agg_create(ctx: *mut kg_ctx, key_type: *const kg_column, ninputs: i32, input_types: *const kg_column, naggs: i32, aggs: *const kg_agg_spec, flags: u32, expected_groups: u64, out: *mut *mut kg_hash_agg) -> c_int;
     pub fn kg_agg_sink(a: *mut kg_hash_agg, key: *const kg_column, inputs: *const kg_column, nrows: i64) -> c_int;
@@ -368,6 +385,15 @@ extern "C" {
     pub fn kg_join_probe(j: *mut kg_hash_join, ncols: i32, cols: *const kg_column, nrows: i64, out: *mut *mut kg_batch) -> c_int;
     pub fn kg_join_probe_selected(j: *mut kg_hash_join, ncols: i32, cols: *const kg_column, nrows: i64, sel_rows: *const i32, nsel: i64, out: *mut *mut kg_batch) -> c_int;
     pub fn kg_join_destroy(j: *mut kg_hash_join);
+    pub fn kg_grace_join_create(ctx: *mut kg_ctx, spec: *const kg_join_spec, table_budget: u64, out: *mut *mut kg_grace_join) -> c_int;
+    pub fn kg_grace_join_set_filter(g: *mut kg_grace_join, nops: i32, ops: *const kg_expr) -> c_int;
+    pub fn kg_grace_join_build_sink(g: *mut kg_grace_join, ncols: i32, cols: *const kg_column, nrows: i64) -> c_int;
+    pub fn kg_grace_join_build_finalize(g: *mut kg_grace_join) -> c_int;
+    pub fn kg_grace_join_probe_push(g: *mut kg_grace_join, ncols: i32, cols: *const kg_column, nrows: i64) -> c_int;
+    pub fn kg_grace_join_finish(g: *mut kg_grace_join) -> c_int;
+    pub fn kg_grace_join_next(g: *mut kg_grace_join, max_rows: i64, out: *mut *mut kg_batch, done: *mut i32) -> c_int;
+    pub fn kg_grace_join_stats_get(g: *const kg_grace_join, out: *mut kg_grace_join_stats) -> c_int;
+    pub fn kg_grace_join_destroy(g: *mut kg_grace_join);
     pub fn kg_joinagg_create(j: *mut kg_hash_join, ninputs: i32, input_types: *const kg_column, naggs: i32, aggs: *const kg_agg_spec, flags: u32) -> c_int;
     pub fn kg_joinagg_probe(j: *mut kg_hash_join, ncols: i32, cols: *const kg_column, nrows: i64, input_cols: *const i32) -> c_int;
     pub fn kg_joinagg_result(j: *mut kg_hash_join, out: *mut *mut kg_batch) -> c_int;
