@@ -494,7 +494,9 @@ def grace(args):
     sc_ms, sc_n = kgpu.prof_get("scatter")
     sc_avg = sc_ms / max(sc_n, 1)
     sc_rows = rows * args.steps / max(sc_n, 1)  # rows of one partition-pass launch
-    sc_ach = sc_rows * 16 / (sc_avg / 1e3) if sc_ms > 0 else None
+    # a partition pass reads each row's key and payload (16 B) and writes them
+    # once into the row's partition (16 B)
+    sc_ach = sc_rows * 32 / (sc_avg / 1e3) if sc_ms > 0 else None
     return {
         "metric": "grace join + per-partition aggregate rows/s (build + probe rows)",
         "value": rows * args.steps / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
@@ -509,7 +511,8 @@ def grace(args):
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": sc_ach / HBM_PEAK if sc_ach else None,
                      "traffic": traffic_of("k_gp_scatter_grace_traffic", int(sc_rows)),
-                     "kernel": "k_gp_scatter (the partition pass: key and payload read once)",
+                     "kernel": "k_gp_scatter (the partition pass: key and payload read once, "
+                               "written once)",
                      "kernel_avg_ms": sc_avg, "launches_per_step": sc_n / args.steps,
                      "algorithmic_bytes_per_step": in_bytes},
         "step_hbm_frac": (in_bytes / (step_ms / 1e3)) / HBM_PEAK,
@@ -871,7 +874,7 @@ def agg(args, key="int64"):
     launches_per_step = upd_n / args.steps
     bytes_per_launch = in_bytes / max(launches_per_step, 1)
     achieved = bytes_per_launch / (upd_avg_ms / 1e3) if upd_ms > 0 else None
-    kname = "k_agg_pass1" if key == "int64" else "k_agg_update"
+    kname = "k_agg_pass1" if key == "int64" else "k_agg_small"
     out = {
         "metric": "hash aggregation rows/s (200M rows, Zipf 1.1 over 10M int64 keys, 13 aggregates)"
                   if key == "int64" else
