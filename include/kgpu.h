@@ -299,6 +299,18 @@ int kg_agg_partial_next(kg_hash_agg *a, kg_batch **out, int32_t *done);
 /* PartialAggregate's metrics (partial.rs:289-296): table flushes and rows
  * passed through. */
 int kg_agg_partial_stats(const kg_hash_agg *a, uint64_t *flushes, uint64_t *bypassed_rows);
+/* How a drain merges the spilled rows and records into the table (set
+ * before the first sink).  form: KG_AGG_MERGE_AUTO (each window of the table
+ * merged in LDS by its own workgroup when the spill's partitions hold many
+ * inputs a window, else by key), KG_AGG_MERGE_WINDOWS or KG_AGG_MERGE_BY_KEY.
+ * size_from_sketch 0: the table grows from expected_groups and when full
+ * only, not from the HyperLogLog estimate of the spilled keys.  No
+ * counterpart in the reference (its row tables are CPU hash indexes): for
+ * tuning and for tests of each form. */
+#define KG_AGG_MERGE_AUTO 0
+#define KG_AGG_MERGE_WINDOWS 1
+#define KG_AGG_MERGE_BY_KEY 2
+int kg_agg_set_merge(kg_hash_agg *a, int32_t form, int32_t size_from_sketch);
 void kg_agg_destroy(kg_hash_agg *a);
 
 /* ---- runtime Bloom filter (split-block, Parquet layout) ----------------

@@ -171,6 +171,23 @@ struct SpillDev {
   SpillFmt F;
 };
 
+// A HyperLogLog sketch of the spilled keys (Flajolet et al., 2007), counted
+// by the drain's partition histogram: register j = top kHllBits bits of an
+// independent 64-bit hash of the key keeps the largest rank (1 + leading
+// zeros) of the hash's remaining bits.  The drain sizes the table from its
+// estimate before the merge (standard error 1.04 / sqrt(kHllRegs), about
+// 3 %).
+constexpr int kHllBits = 10;
+constexpr int kHllRegs = 1 << kHllBits;
+constexpr uint64_t kHllSalt = 0xD6E8FEB86659FD93ULL;
+
+__device__ __forceinline__ void hll_add(unsigned int *reg, const Table &T, uint64_t key) {
+  const uint64_t h = dev::table_hash(key, T.seed ^ kHllSalt);
+  const uint32_t j = (uint32_t)(h >> (64 - kHllBits));
+  const unsigned int rho = (unsigned int)__clzll((h << kHllBits) | (1ull << (kHllBits - 1))) + 1u;
+  if (rho > reg[j]) atomicMax(&reg[j], rho);
+}
+
 // Slots for the lanes in `mask` (every lane of the wave calls it with the
 // same mask): one atomic per wave; returns this lane's index.
 __device__ __forceinline__ int64_t spill_reserve(unsigned long long *counter, uint64_t mask) {
@@ -416,11 +433,15 @@ __device__ __forceinline__ uint32_t spill_part(const Table &T, uint64_t key, int
 
 // Per-block partition histogram: hist[p * gridDim + b] (dead rows of the row
 // spill, `dead` set, are left out).
+// With hll, the entries' keys also go into the sketch of the spilled keys.
 __global__ void k_spill_hist(const uint64_t *rows, int w, uint64_t n, Table T, int log2p,
-                             int64_t per_block, int dead, uint32_t *hist) {
+                             int64_t per_block, int dead, uint32_t *hist, unsigned int *hll) {
   extern __shared__ uint32_t s_h[];
   const uint32_t np = 1u << log2p;
+  unsigned int *s_reg = s_h + np;
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) s_h[p] = 0;
+  if (hll)
+    for (int j = threadIdx.x; j < kHllRegs; j += blockDim.x) s_reg[j] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * per_block;
   const int64_t hi = lo + per_block < (int64_t)n ? lo + per_block : (int64_t)n;
@@ -439,10 +460,18 @@ __global__ void k_spill_hist(const uint64_t *rows, int w, uint64_t n, Table T, i
       live[r] = i0 + r * (int64_t)blockDim.x < hi && !(fl[r] & kDeadRow);
 #pragma unroll
     for (int r = 0; r < 4; r++)
-      if (live[r]) atomicAdd(&s_h[spill_part(T, key[r], log2p)], 1u);
+      if (live[r]) {
+        atomicAdd(&s_h[spill_part(T, key[r], log2p)], 1u);
+        if (hll) hll_add(s_reg, T, key[r]);
+      }
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) hist[(uint64_t)p * gridDim.x + blockIdx.x] = s_h[p];
+  if (hll)
+    for (int j = threadIdx.x; j < kHllRegs; j += blockDim.x) {
+      const unsigned int v = s_reg[j];
+      if (v && v > hll[j]) atomicMax(&hll[j], v);
+    }
 }
 
 // Moves each row to its partition (order within a partition is not kept).
@@ -527,26 +556,30 @@ __device__ __forceinline__ void part_range(const uint64_t *base, uint32_t nb, ui
 // inserted by other workgroups while this one probes).
 __device__ __forceinline__ int64_t table_find(const Table &T, uint64_t key) {
   uint64_t s = slot_hash(T, key) >> (64 - T.log2cap);
-  while (true) {
+  for (uint64_t p = 0; p <= T.wmask; p++) {
     uint64_t k = __hip_atomic_load(&T.keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (k == key) return (int64_t)s;
     if (k == kEmpty) return -1;
-    s = (s + 1) & T.mask;
+    s = next_in_window(s, T.wmask);
   }
+  return -1;
 }
 
-// Find-or-insert without counting (the caller accounts for new keys).
+// Find-or-insert without counting (the caller accounts for new keys).  A full
+// window flags the table (see global_slot).
 __device__ __forceinline__ uint64_t table_insert(const Table &T, uint64_t key) {
   uint64_t s = slot_hash(T, key) >> (64 - T.log2cap);
-  while (true) {
+  for (uint64_t p = 0; p <= T.wmask; p++) {
     uint64_t k = __hip_atomic_load(&T.keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (k == key) return s;
     if (k == kEmpty) {
       unsigned long long prev = atomicCAS((unsigned long long *)&T.keys[s], kEmpty, key);
       if (prev == kEmpty || prev == key) return s;
     }
-    s = (s + 1) & T.mask;
+    s = next_in_window(s, T.wmask);
   }
+  atomicOr(T.occupied + 1, 1ull);
+  return T.cap + 1;
 }
 
 // Insert into the merge kernel's LDS table, unbounded probing while fewer
@@ -749,6 +782,205 @@ __global__ void k_agg_retry(AggProg P, Table T, MergeDev M, int S, uint32_t nwg)
         int ls[1] = {0};
         tile_apply<1, false>(P, r, 0, ls, src);
       }
+    }
+  }
+}
+
+// ---- windowed merge ---------------------------------------------------------
+// One workgroup owns a window of the table (2^log2span slots: keys probe only
+// within their window, so the window holds exactly its own keys): the window
+// comes into LDS (its occupied slots loaded, or all empty while the table
+// holds no key), every spilled input of the window is applied there (an LDS
+// insert within the window, LDS atomics), and the window goes back with
+// coalesced stores -- its keys, and the records of its occupied slots.  A
+// partition of the spill covers 2^g windows, and its inputs are read once
+// per window (those homed in another window are skipped).  A window that
+// fills is not written back (the table keeps that window as it was) and is
+// counted in *failed; the host grows the table and runs the merge again over
+// the windows not yet done (done[w] = 1).
+struct WMergeDev {
+  const uint64_t *rows;
+  uint64_t nrows;
+  const uint64_t *rbase;
+  uint32_t rnb;
+  const uint64_t *recs;
+  uint64_t nrecs;
+  const uint64_t *cbase;
+  uint32_t cnb;
+  int log2p;     // spill partitions
+  int g;         // log2 windows a partition
+  int log2span;  // log2 slots a window
+  int load;      // 1: windows are read from the table; 0: the table holds no key
+  uint8_t *done;
+  unsigned int *failed;
+  SpillFmt F;
+};
+
+// Find-or-insert of key in an LDS window of `span` slots from slot s; -1
+// when the window is full.
+__device__ __forceinline__ int win_insert(uint64_t *s_keys, uint32_t span, uint32_t s,
+                                          uint64_t key) {
+  for (uint32_t p = 0; p < span; p++) {
+    const uint64_t k = s_keys[s];
+    if (k == key) return (int)s;
+    if (k == kEmpty) {
+      const unsigned long long prev = atomicCAS((unsigned long long *)&s_keys[s], kEmpty, key);
+      if (prev == kEmpty || prev == key) return (int)s;
+    }
+    s = (s + 1) & (span - 1);
+  }
+  return -1;
+}
+
+// Spill rows of up to kStageWords words are staged in LDS a tile at a time.
+constexpr int kStageWords = 8;
+
+template <int R, bool STAGE>
+__global__ void __launch_bounds__(kMergeBlock) k_agg_wmerge(AggProg P, Table T, WMergeDev M) {
+  extern __shared__ uint64_t smem[];
+  const uint32_t span = 1u << M.log2span;
+  uint64_t *s_keys = smem;
+  uint64_t *s_rec = smem + span;
+  uint64_t *s_stage = s_rec + (size_t)span * (P.W | 1);  // STAGE: blockDim rows
+  __shared__ int s_fail, s_keys_before, s_keys_after;
+  const int W = P.W, WL = W | 1;
+  const uint32_t np = 1u << M.log2p;
+  const uint32_t nw = 1u << M.g;
+  for (uint32_t p = blockIdx.x; p < np; p += gridDim.x) {
+    int64_t r0, r1, c0, c1;
+    part_range(M.rbase, M.rnb, M.nrows, p, np, r0, r1);
+    part_range(M.cbase, M.cnb, M.nrecs, p, np, c0, c1);
+    const int64_t nrec = c1 - c0, total = nrec + (r1 - r0);
+    for (uint32_t wi = 0; wi < nw; wi++) {
+      const uint64_t win = ((uint64_t)p << M.g) + wi;
+      if (M.done[win]) continue;  // the same for every lane
+      if (total == 0) {           // nothing changes in this window
+        if (threadIdx.x == 0) M.done[win] = 1;
+        continue;
+      }
+      const uint64_t s0 = win << M.log2span;
+      // the window into LDS: keys, then each slot's record (its own when
+      // occupied, the initial state otherwise)
+      if (threadIdx.x == 0) {
+        s_fail = 0;
+        s_keys_before = 0;
+        s_keys_after = 0;
+      }
+      for (uint32_t t = threadIdx.x; t < span; t += blockDim.x) {
+        const uint64_t k = M.load ? T.keys[s0 + t] : kEmpty;
+        s_keys[t] = k;
+        const uint64_t m = __ballot(k != kEmpty);
+        if (__lane_id() == 0 && m) atomicAdd(&s_keys_before, __popcll(m));
+      }
+      fill_lds_records(P, s_rec, (int)span, WL);
+      __syncthreads();
+      if (M.load) {  // the occupied slots' own records
+        for (uint32_t i = threadIdx.x; i < span * (uint32_t)W; i += blockDim.x) {
+          const uint32_t sl = i / (uint32_t)W, w = i - sl * (uint32_t)W;
+          if (s_keys[sl] != kEmpty) s_rec[sl * WL + w] = T.rec[s0 * W + i];
+        }
+        __syncthreads();
+      }
+      // inputs [0, nrec) are pass 1's spilled records, [nrec, total) the
+      // spilled rows; with STAGE the rows are taken a tile at a time from LDS
+      const int64_t end = STAGE ? nrec : total;
+      for (int64_t t0 = 0; t0 < end; t0 += (int64_t)R * blockDim.x) {
+        if (*(volatile int *)&s_fail) break;  // the window runs again in a larger table
+        int ls[R], lsw[R];
+        uint64_t key[R];
+        SpillSrc<R> src;
+        src.rows = M.rows;
+        src.safe = r1 > r0 ? r0 : -1;
+        // every row's key load is issued before the first insert (a row
+        // index clamped into the partition, so no per-lane branch)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int64_t u = t0 + (int64_t)r * blockDim.x + threadIdx.x;
+          const int64_t uc = u < end ? u : end - 1;
+          const uint64_t *kp = uc < nrec ? M.recs + (c0 + uc) * (1 + W)
+                                         : M.rows + (r0 + uc - nrec) * M.F.rw;
+          key[r] = *kp;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int64_t u = t0 + (int64_t)r * blockDim.x + threadIdx.x;
+          ls[r] = -1;
+          src.idx[r] = r0 + u - nrec;
+          if (u < end) {
+            const uint64_t home = slot_hash(T, key[r]) >> (64 - T.log2cap);
+            if ((home >> M.log2span) == win) {
+              ls[r] = win_insert(s_keys, span, (uint32_t)(home & (span - 1)), key[r]);
+              if (ls[r] < 0) s_fail = 1;
+            }
+          }
+          if (ls[r] >= 0 && u < nrec)
+            merge_into_lds(P, s_rec + ls[r] * WL, M.recs + (c0 + u) * (1 + W) + 1);
+          lsw[r] = (ls[r] >= 0 && u >= nrec) ? ls[r] : -1;
+        }
+        tile_apply<R>(P, s_rec, WL, lsw, src);
+      }
+      if (STAGE) {
+        // a tile of blockDim rows into LDS with coalesced loads (a lane's
+        // words all in flight together), then applied from LDS
+        const int rw = M.F.rw;
+        for (int64_t t0 = r0; t0 < r1; t0 += blockDim.x) {
+          // (also the barrier after the previous tile's reads of the stage)
+          if (__syncthreads_or(*(volatile int *)&s_fail)) break;
+          const int64_t nt = r1 - t0 < (int64_t)blockDim.x ? r1 - t0 : (int64_t)blockDim.x;
+          const int64_t lim = nt * rw;
+          const uint64_t *g = M.rows + t0 * rw;
+          uint64_t v[kStageWords];
+#pragma unroll
+          for (int k = 0; k < kStageWords; k++) {
+            const int64_t i = (int64_t)k * blockDim.x + threadIdx.x;
+            v[k] = g[i < lim ? i : lim - 1];
+          }
+#pragma unroll
+          for (int k = 0; k < kStageWords; k++) {
+            const int64_t i = (int64_t)k * blockDim.x + threadIdx.x;
+            if (i < lim) s_stage[i] = v[k];
+          }
+          __syncthreads();
+          int ls[1] = {-1};
+          if ((int64_t)threadIdx.x < nt) {
+            const uint64_t key = s_stage[threadIdx.x * rw];
+            const uint64_t home = slot_hash(T, key) >> (64 - T.log2cap);
+            if ((home >> M.log2span) == win) {
+              ls[0] = win_insert(s_keys, span, (uint32_t)(home & (span - 1)), key);
+              if (ls[0] < 0) s_fail = 1;
+            }
+          }
+          SpillSrc<1> src;
+          src.rows = s_stage;
+          src.idx[0] = threadIdx.x;
+          src.safe = 0;
+          tile_apply<1>(P, s_rec, WL, ls, src);
+        }
+      }
+      __syncthreads();
+      if (!s_fail) {
+        for (uint32_t t = threadIdx.x; t < span; t += blockDim.x) {
+          const uint64_t k = s_keys[t];
+          T.keys[s0 + t] = k;
+          const uint64_t m = __ballot(k != kEmpty);
+          if (__lane_id() == 0 && m) atomicAdd(&s_keys_after, __popcll(m));
+        }
+        for (uint32_t i = threadIdx.x; i < span * (uint32_t)W; i += blockDim.x) {
+          const uint32_t sl = i / (uint32_t)W, w = i - sl * (uint32_t)W;
+          if (s_keys[sl] != kEmpty) T.rec[s0 * W + i] = s_rec[sl * WL + w];
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (s_fail) {
+          atomicAdd(M.failed, 1u);
+        } else {
+          M.done[win] = 1;
+          if (s_keys_after != s_keys_before)
+            atomicAdd(T.occupied, (unsigned long long)(s_keys_after - s_keys_before));
+        }
+      }
+      __syncthreads();  // the LDS window is refilled next
     }
   }
 }
@@ -1136,6 +1368,24 @@ __global__ void k_rehash(AggProg P, const uint64_t *okeys, const uint64_t *orec,
   }
 }
 
+// The initial state into every record whose slot holds no key.
+__global__ void k_init_free_records(Table T, int W, const uint64_t *init) {
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < T.cap * (uint64_t)W;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t sl = t / (uint64_t)W;
+    if (T.keys[sl] == kEmpty) T.rec[t] = init[t - sl * (uint64_t)W];
+  }
+}
+
+// Group flags of a hashed table: slots [0, cap) hold a group when they hold
+// a key, the two fixed slots when they have rows.
+__global__ void k_group_flags_keys(const uint64_t *keys, const uint64_t *rec, uint64_t cap, int W,
+                                   uint32_t *flags) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < cap + 2;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    flags[s] = (s < cap ? keys[s] != kEmpty : rec[s * (uint64_t)W] > 0) ? 1u : 0u;
+}
+
 __global__ void k_group_flags(const uint64_t *rec, uint64_t nslots, int W, int global,
                               uint32_t *flags) {
   for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < nslots;
@@ -1172,6 +1422,13 @@ struct kg_hash_agg {
   OutProg out{};
   int p1_slots = 0, m_slots = 0;  // LDS tables of pass 1 and of the merge
   size_t p1_lds = 0, m_lds = 0;
+  int log2span_max = 0;  // the largest window (log2 slots) the windowed merge holds in LDS
+  size_t w_lds_max = 0;  // the windowed merge's dynamic LDS limit
+  bool stage_rows = false;  // the windowed merge stages spill rows in LDS
+  bool rec_init = false;  // every window record holds the initial state (else only occupied ones)
+  int merge_form = KG_AGG_MERGE_AUTO;
+  bool size_from_sketch = true;
+  DevBuf hll;             // kHllRegs registers of the spill's sketch
   // table
   DevBuf keys, rec, occ;
   uint64_t cap = 0;
@@ -1216,9 +1473,18 @@ static Table table_of(kg_hash_agg *a) {
   t.log2cap = a->log2cap;
   t.mask = a->cap - 1;
   t.seed = a->seed;
+  t.wmask = (a->hashed ? std::min<uint64_t>(a->cap, 1ull << a->log2span_max) : a->cap) - 1;
   return t;
 }
 
+static int log2_span(const kg_hash_agg *a) {
+  return a->hashed ? std::min(a->log2cap, a->log2span_max) : a->log2cap;
+}
+
+// A hashed table's window records are initialized lazily: every path that
+// reads a slot's record before a key is in it calls ensure_rec_init first
+// (the windowed merge never does: it starts empty slots from the initial
+// state in LDS).  The two fixed records are initialized at once.
 static void alloc_table(kg_hash_agg *a, uint64_t cap, DevBuf &keys, DevBuf &rec) {
   uint64_t ns = a->direct ? 257 : cap + 2;
   keys.reset(a->ctx, (a->direct ? 257 : cap) * 8);
@@ -1226,7 +1492,26 @@ static void alloc_table(kg_hash_agg *a, uint64_t cap, DevBuf &keys, DevBuf &rec)
   k_fill_u64<<<stream_grid(keys.bytes() / 8, 256), 256, 0, a->s>>>(keys.get<uint64_t>(),
                                                                    keys.bytes() / 8, kEmpty);
   KG_HIP(hipGetLastError());
-  init_records(a->ctx, rec.get<uint64_t>(), ns, a->prog, a->s);
+  if (a->hashed) {
+    init_records(a->ctx, rec.get<uint64_t>() + cap * (uint64_t)a->prog.W, 2, a->prog, a->s);
+    a->rec_init = false;
+  } else {
+    init_records(a->ctx, rec.get<uint64_t>(), ns, a->prog, a->s);
+  }
+}
+
+static void ensure_rec_init(kg_hash_agg *a) {
+  if (!a->hashed || a->rec_init) return;
+  // only slots without a key: occupied records hold state
+  const Table t = table_of(a);
+  DevBuf init(a->ctx, (uint64_t)a->prog.W * 8);
+  KG_HIP(hipMemcpyAsync(init.get(), a->prog.init, (size_t)a->prog.W * 8, hipMemcpyHostToDevice,
+                        a->s));
+  k_init_free_records<<<stream_grid(a->cap, 256), 256, 0, a->s>>>(t, a->prog.W,
+                                                                  init.get<uint64_t>());
+  KG_HIP(hipGetLastError());
+  KG_HIP(hipStreamSynchronize(a->s));
+  a->rec_init = true;
 }
 
 static int log2_ceil(uint64_t v) {
@@ -1238,10 +1523,11 @@ static int log2_ceil(uint64_t v) {
 // Keys in the table: the device count (inserts made with global_slot) plus
 // the merge's inserts.
 static uint64_t read_occupied(kg_hash_agg *a) {
-  uint64_t v = 0;
-  KG_HIP(hipMemcpyAsync(&v, a->occ.get(), 8, hipMemcpyDeviceToHost, a->s));
+  uint64_t v[2] = {0, 0};
+  KG_HIP(hipMemcpyAsync(v, a->occ.get(), 16, hipMemcpyDeviceToHost, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
-  return v + a->merged_keys;
+  if (v[1]) fail(KG_ERR_INTERNAL, "a window of the aggregation table filled");
+  return v[0] + a->merged_keys;
 }
 
 // ---- NeedMemory: a call checks the HBM its worst case may take against the
@@ -1272,19 +1558,51 @@ static uint64_t key_domain(const kg_hash_agg *a) {
 // beyond what is held: their partitioned copies, the merge's scratch, and
 // the table grown for the worst case of every entry being a new key (at
 // most `new_keys`; the old table is held while the new one fills).
+// The table's size for a drain: at most half full once the keys held and
+// the new keys (the sketch's estimate with a tenth more, at most one a
+// spilled entry) are in.
+static uint64_t drain_cap(kg_hash_agg *a, uint64_t nr, uint64_t nc, uint64_t occupied,
+                          double est) {
+  uint64_t want = occupied + std::min<uint64_t>((uint64_t)(est * 1.1) + 64, nr + nc);
+  want = std::min<uint64_t>(want, key_domain(a));
+  uint64_t cap = a->cap;
+  while (cap / 2 < want) cap *= 2;
+  return cap;
+}
+
+// The peak HBM a drain of nr spilled rows and nc spilled records may take
+// beyond what is held: their partitioned copies, the merge's scratch, and
+// the grown table (from the sketch's estimate est of new keys when known;
+// otherwise for the worst case of every entry being a new key, at most
+// `new_keys`; the old table is held while the new one fills).
 static uint64_t drain_need(kg_hash_agg *a, uint64_t nr, uint64_t nc, uint64_t occupied,
-                           uint64_t new_keys, uint64_t spill_bytes = 0) {
+                           uint64_t new_keys, uint64_t spill_bytes = 0, double est = -1.0) {
   if (!nr && !nc) return 0;
   const uint64_t W = (uint64_t)a->prog.W;
   const uint64_t grid = (uint64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256);
   // the partitioned copies of the spill (its bytes, when given)
   uint64_t need = (spill_bytes ? spill_bytes : (nr * a->fmt.rw + nc * (1 + W)) * 8) + 16;
   need += grid * a->m_slots * (1 + W) * 8 + grid * kMergeRows * kMergeBlock * 8;
-  need += (64ull << 20);  // histograms, bases, counters
-  uint64_t want = std::min<uint64_t>(occupied + new_keys, occupied + nr + nc);
-  want = std::min<uint64_t>(want, key_domain(a));
+  // histograms and their scans (4 + 8 bytes a partition and block), counters,
+  // window flags
+  int log2p = 0;
+  while (log2p < 15 && (1ull << log2p) * 4096 < nr + nc) log2p++;
+  const uint64_t fill = grid * 2;
+  for (uint64_t n : {nr, nc}) {
+    if (!n) continue;
+    const uint64_t per_block = std::max<uint64_t>(
+        4096, std::min<uint64_t>((1ull << log2p) * 16, (n + fill - 1) / fill));
+    need += ((n + per_block - 1) / per_block) * (12ull << log2p) * 2;
+  }
+  need += (16ull << 20);
   uint64_t cap = a->cap;
-  while (cap / 4 * 3 < want) cap *= 2;
+  if (est >= 0.0) {
+    cap = drain_cap(a, nr, nc, occupied, est);
+  } else {
+    uint64_t want = std::min<uint64_t>(occupied + new_keys, occupied + nr + nc);
+    want = std::min<uint64_t>(want, key_domain(a));
+    while (cap / 2 < want) cap *= 2;
+  }
   if (cap > a->cap) need += table_bytes(a, cap);
   return need;
 }
@@ -1309,6 +1627,7 @@ static void grow(kg_hash_agg *a, uint64_t min_cap) {
   KG_HIP(hipMemcpyAsync(a->occ.get(), &keys_now, 8, hipMemcpyHostToDevice, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
   a->merged_keys = 0;
+  (void)read_occupied(a);  // a window that filled during the rehash
 }
 
 // Rows the next launch may take so that the table stays below 3/4 full
@@ -1335,6 +1654,7 @@ static void reserve(kg_hash_agg *a, int64_t n) {
   uint64_t occ = read_occupied(a);
   a->occ_bound = occ;
   while (occ + (uint64_t)n > a->cap / 4 * 3) grow(a, a->cap * 2);
+  ensure_rec_init(a);  // the rows' new keys start from their slots' records
 }
 
 // The few-groups path (a one-byte key, or a global aggregate): the used
@@ -1516,11 +1836,23 @@ static void pass1(kg_hash_agg *a, const AggProg &P, int64_t nrows) {
   }
 }
 
-// Partitions one spill stream (n entries of w words) by home-slot range into
-// `out`; base receives the scanned per-block histogram (partition p starts
-// at base[p * nb]).  Returns nb.
-static uint32_t partition_spill(kg_hash_agg *a, const uint64_t *in, uint64_t n, int w, int log2p,
-                                bool dead, DevBuf &out, DevBuf &base, uint64_t &live) {
+// One spill stream (n entries of w words) partitioned by home-slot range:
+// the per-block histogram first (spill_hist; with the sketch, the keys also
+// go into a->hll), then the scan and the scatter into `out` (spill_scatter;
+// partition p starts at base[p * nb]).
+struct SpillPart {
+  const uint64_t *in = nullptr;
+  uint64_t n = 0;
+  int w = 0;
+  bool dead = false;
+  int log2p = 0;
+  int64_t per_block = 0;
+  uint32_t nb = 0;
+  DevBuf hist;
+};
+
+static void spill_hist(kg_hash_agg *a, SpillPart &sp, const uint64_t *in, uint64_t n, int w,
+                       bool dead, int log2p, bool sketch) {
   const uint32_t np = 1u << log2p;
   // blocks enough to fill the device, each with about 16 entries a partition
   const int64_t fill = (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256) * 2;
@@ -1530,60 +1862,69 @@ static uint32_t partition_spill(kg_hash_agg *a, const uint64_t *in, uint64_t n, 
   if (nb < 1) nb = 1;
   per_block = ceil_div((int64_t)n, nb);
   if (per_block < 1) per_block = 1;
-  const uint64_t hn = (uint64_t)np * nb;
-  DevBuf hist(a->ctx, hn * 4 + 4);
-  base.reset(a->ctx, hn * 8 + 8);
-  out.reset(a->ctx, n * w * 8 + 8);
+  sp.in = in;
+  sp.n = n;
+  sp.w = w;
+  sp.dead = dead;
+  sp.log2p = log2p;
+  sp.per_block = per_block;
+  sp.nb = (uint32_t)nb;
+  sp.hist.reset(a->ctx, (uint64_t)np * nb * 4 + 4);
   Table t = table_of(a);
   ProfScope prof("agg_partition", a->s);
-  k_spill_hist<<<(unsigned)nb, 256, np * 4, a->s>>>(in, w, n, t, log2p, per_block, dead ? 1 : 0,
-                                                    hist.get<uint32_t>());
+  k_spill_hist<<<(unsigned)nb, 256, np * 4 + (sketch ? 4 * kHllRegs : 0), a->s>>>(
+      in, w, n, t, log2p, per_block, dead ? 1 : 0, sp.hist.get<uint32_t>(),
+      sketch ? a->hll.get<unsigned int>() : nullptr);
   KG_HIP(hipGetLastError());
+}
+
+// Returns the live entries (the scanned total).
+static uint64_t spill_scatter(kg_hash_agg *a, SpillPart &sp, DevBuf &out, DevBuf &base) {
+  const uint32_t np = 1u << sp.log2p;
+  const uint64_t hn = (uint64_t)np * sp.nb;
+  base.reset(a->ctx, hn * 8 + 8);
+  out.reset(a->ctx, sp.n * sp.w * 8 + 8);
+  Table t = table_of(a);
+  ProfScope prof("agg_partition", a->s);
   DevBuf temp(a->ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
-  exclusive_scan(hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(),
+  exclusive_scan(sp.hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(),
                  a->s);
-  k_spill_scatter<<<(unsigned)nb, 256, np * 4, a->s>>>(in, w, n, t, log2p, per_block,
-                                                       dead ? 1 : 0, base.get<uint64_t>(),
-                                                       out.get<uint64_t>());
+  k_spill_scatter<<<sp.nb, 256, np * 4, a->s>>>(sp.in, sp.w, sp.n, t, sp.log2p, sp.per_block,
+                                                sp.dead ? 1 : 0, base.get<uint64_t>(),
+                                                out.get<uint64_t>());
   KG_HIP(hipGetLastError());
-  // live entries: the scanned total (last base + last count)
   uint64_t lb = 0;
   uint32_t lc = 0;
   KG_HIP(hipMemcpyAsync(&lb, base.get<uint64_t>() + hn - 1, 8, hipMemcpyDeviceToHost, a->s));
-  KG_HIP(hipMemcpyAsync(&lc, hist.get<uint32_t>() + hn - 1, 4, hipMemcpyDeviceToHost, a->s));
+  KG_HIP(hipMemcpyAsync(&lc, sp.hist.get<uint32_t>() + hn - 1, 4, hipMemcpyDeviceToHost, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
-  live = lb + lc;
-  return (uint32_t)nb;
+  return lb + lc;
 }
 
-// Merges both spills into the table (see k_agg_merge) and empties them.
-static void drain(kg_hash_agg *a) {
-  if (!a->hashed) return;
-  uint64_t nr, nc;
-  read_spill_counts(a, nr, nc);
-  if (nr == 0 && nc == 0) return;
-  if (nr + nc >= (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "more than 2^32 spilled entries");
-  // everything the merge may take, checked before the spill is touched
-  need_room(a->ctx, drain_need(a, nr, nc, read_occupied(a), a->rows_seen + a->pending_rows),
-            "the merge of the aggregation spill");
-  if (read_occupied(a) > a->cap / 2) grow(a, a->cap * 2);
+// The sketch's estimate of the distinct keys spilled since the last drain
+// (HyperLogLog with the small-range correction of linear counting).
+static double spilled_keys_estimate(kg_hash_agg *a) {
+  std::vector<uint32_t> reg(kHllRegs);
+  KG_HIP(hipMemcpyAsync(reg.data(), a->hll.get(), (size_t)kHllRegs * 4, hipMemcpyDeviceToHost,
+                        a->s));
+  KG_HIP(hipStreamSynchronize(a->s));
+  const double m = kHllRegs;
+  double sum = 0.0;
+  int zeros = 0;
+  for (uint32_t v : reg) {
+    sum += std::ldexp(1.0, -(int)v);
+    zeros += v == 0;
+  }
+  double e = 0.7213 / (1.0 + 1.079 / m) * m * m / sum;
+  if (e <= 2.5 * m && zeros) e = m * std::log(m / zeros);
+  return e;
+}
+
+// The old merge (k_agg_merge: an LDS hash table per partition, merged into
+// the global table by key), for drains whose partitions cover many windows
+// each: few spilled inputs a window.
+static void merge_by_key(kg_hash_agg *a, const MergeDev &M0, int log2p, uint32_t np) {
   const int W = a->prog.W;
-  // partitions of about 4096 inputs (their keys fit the merge's LDS table)
-  int log2p = 0;
-  while (log2p < 14 && (1ull << log2p) * 4096 < nr + nc && (1ull << (log2p + 1)) <= a->cap) log2p++;
-  const uint32_t np = 1u << log2p;
-  DevBuf prow, pbase, prec, cbase;
-  uint32_t rnb = 0, cnb = 0;
-  uint64_t nr_live = 0, nc_live = 0;
-  if (nr)
-    rnb = partition_spill(a, a->sp_rows.get<uint64_t>(), nr, a->fmt.rw, log2p, true, prow, pbase,
-                          nr_live);
-  if (nc)
-    cnb = partition_spill(a, a->sp_recs.get<uint64_t>(), nc, 1 + W, log2p, false, prec, cbase,
-                          nc_live);
-  nr = nr_live;
-  nc = nc_live;
-  // merge, resuming stopped partitions after growth
   const int64_t grid = std::min<int64_t>(np, (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256));
   DevBuf resume(a->ctx, (uint64_t)np * 8), inserted(a->ctx, (uint64_t)np * 4);
   DevBuf retry(a->ctx, (uint64_t)grid * a->m_slots * (1 + W) * 8);
@@ -1593,15 +1934,7 @@ static void drain(kg_hash_agg *a) {
   AggProg P = a->prog;
   P.skip_checked = 0;
   set_spill_layout(P, a->fmt);
-  MergeDev M{};
-  M.rows = prow.get<uint64_t>();
-  M.nrows = nr;
-  M.rbase = nr ? pbase.get<uint64_t>() : nullptr;
-  M.rnb = rnb;
-  M.recs = prec.get<uint64_t>();
-  M.nrecs = nc;
-  M.cbase = nc ? cbase.get<uint64_t>() : nullptr;
-  M.cnb = cnb;
+  MergeDev M = M0;
   M.log2p = log2p;
   M.resume = resume.get<int64_t>();
   M.inserted = inserted.get<uint32_t>();
@@ -1611,7 +1944,7 @@ static void drain(kg_hash_agg *a) {
   M.pend_part = M.pend_n + grid;
   M.stops = M.pend_part + grid;
   M.pend = pend.get<int64_t>();
-  M.F = a->fmt;
+  ensure_rec_init(a);  // new keys' records are merged into, not written
   for (int round = 0;; round++) {
     if (round > 40) fail(KG_ERR_INTERNAL, "aggregation merge made no progress");
     const uint64_t keys_now = read_occupied(a);
@@ -1645,11 +1978,146 @@ static void drain(kg_hash_agg *a) {
     uint64_t want = a->cap * 2;
     while (want / 4 * 3 < need) want *= 2;
     grow(a, want);
+    ensure_rec_init(a);
     ProfScope prof("agg_merge", a->s);
     k_agg_retry<<<(unsigned)grid, 256, 0, a->s>>>(P, table_of(a), M, a->m_slots, (uint32_t)grid);
     KG_HIP(hipGetLastError());
     KG_HIP(hipStreamSynchronize(a->s));
   }
+}
+
+// The windowed merge (k_agg_wmerge) over partitions of 2^g windows, run
+// again over the windows not done after each growth of the table (a window
+// that filled).
+static void merge_windows(kg_hash_agg *a, const MergeDev &M0, int log2p, uint32_t np, bool load) {
+  AggProg P = a->prog;
+  P.skip_checked = 0;
+  set_spill_layout(P, a->fmt);
+  WMergeDev M{};
+  M.rows = M0.rows;
+  M.nrows = M0.nrows;
+  M.rbase = M0.rbase;
+  M.rnb = M0.rnb;
+  M.recs = M0.recs;
+  M.nrecs = M0.nrecs;
+  M.cbase = M0.cbase;
+  M.cnb = M0.cnb;
+  M.log2p = log2p;
+  M.F = a->fmt;
+  M.load = load ? 1 : 0;
+  int lw = a->log2cap - log2_span(a);
+  DevBuf done(a->ctx, (1ull << lw) + 8), failed(a->ctx, 8);
+  KG_HIP(hipMemsetAsync(done.get(), 0, done.bytes(), a->s));
+  const int64_t grid = std::min<int64_t>(np, (int64_t)(a->ctx->num_cus > 0 ? a->ctx->num_cus : 256));
+  for (int round = 0;; round++) {
+    M.log2span = log2_span(a);
+    M.g = lw - log2p;
+    M.done = done.get<uint8_t>();
+    M.failed = failed.get<unsigned int>();
+    const size_t lds = ((size_t)1 << M.log2span) * (1 + (size_t)(a->prog.W | 1)) * 8;
+    // spill rows staged in LDS when a tile of them fits beside the window
+    const size_t stage = (size_t)kMergeBlock * a->fmt.rw * 8;
+    const bool staged = a->stage_rows && a->fmt.rw <= kStageWords && lds + stage <= a->w_lds_max;
+    KG_HIP(hipMemsetAsync(failed.get(), 0, 8, a->s));
+    {
+      ProfScope prof("agg_merge", a->s);
+      if (staged)
+        k_agg_wmerge<kMergeRows, true><<<(unsigned)grid, kMergeBlock, lds + stage, a->s>>>(
+            P, table_of(a), M);
+      else
+        k_agg_wmerge<kMergeRows, false><<<(unsigned)grid, kMergeBlock, lds, a->s>>>(
+            P, table_of(a), M);
+      KG_HIP(hipGetLastError());
+    }
+    unsigned int nf = 0;
+    KG_HIP(hipMemcpyAsync(&nf, failed.get(), 4, hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    if (nf == 0) return;
+    if (round >= 40) fail(KG_ERR_INTERNAL, "aggregation table windows kept filling");
+    // twice the slots: the windows done keep their keys (rehashed), the rest
+    // run again with the table's keys loaded
+    std::vector<uint8_t> d0((size_t)1 << lw);
+    KG_HIP(hipMemcpyAsync(d0.data(), done.get(), d0.size(), hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    try {
+      // part of the spill is merged by now: a refusal cannot be retried
+      need_room(a->ctx, table_bytes(a, a->cap * 2), "the growth of the aggregation table");
+      grow(a, a->cap * 2);
+    } catch (const Error &e) {
+      if (e.code != KG_ERR_MEMORY) throw;
+      fail(KG_ERR_INTERNAL, "cannot finish the aggregation merge: " + e.msg);
+    }
+    const int lw2 = a->log2cap - log2_span(a);
+    std::vector<uint8_t> d1((size_t)1 << lw2);
+    for (size_t w = 0; w < d1.size(); w++) d1[w] = d0[w >> (lw2 - lw)];
+    DevBuf done2(a->ctx, d1.size() + 8);
+    KG_HIP(hipMemcpyAsync(done2.get(), d1.data(), d1.size(), hipMemcpyHostToDevice, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    done = std::move(done2);
+    lw = lw2;
+    M.load = 1;
+  }
+}
+
+// Merges both spills into the table and empties them.  The spill is
+// partitioned by home-slot range into partitions of about 4096 entries; the
+// partition histogram also counts the spilled keys' sketch, from which the
+// table is sized (at most half full after the merge) with whole windows a
+// partition.  Then each window is merged in LDS by its own workgroup, or,
+// when the partitions hold few entries a window, by key (k_agg_merge).
+static void drain(kg_hash_agg *a) {
+  if (!a->hashed) return;
+  uint64_t nr, nc;
+  read_spill_counts(a, nr, nc);
+  if (nr == 0 && nc == 0) return;
+  if (nr + nc >= (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "more than 2^32 spilled entries");
+  const uint64_t occ0 = read_occupied(a);
+  const int W = a->prog.W;
+  int log2p = 0;
+  while (log2p < 15 && (1ull << log2p) * 4096 < nr + nc) log2p++;
+  SpillPart pr, pc;
+  auto hists = [&](bool sketch) {
+    if (sketch) KG_HIP(hipMemsetAsync(a->hll.get(), 0, (size_t)kHllRegs * 4, a->s));
+    if (nr) spill_hist(a, pr, a->sp_rows.get<uint64_t>(), nr, a->fmt.rw, true, log2p, sketch);
+    if (nc) spill_hist(a, pc, a->sp_recs.get<uint64_t>(), nc, 1 + W, false, log2p, sketch);
+  };
+  hists(a->size_from_sketch);
+  const double est = a->size_from_sketch ? spilled_keys_estimate(a) : -1.0;
+  // everything the merge may take, checked before the spill or the table changes
+  need_room(a->ctx, drain_need(a, nr, nc, occ0, a->rows_seen + a->pending_rows, 0, est),
+            "the merge of the aggregation spill");
+  if (est >= 0.0) {
+    const uint64_t cap = drain_cap(a, nr, nc, occ0, est);
+    if (cap > a->cap) grow(a, cap);
+  }
+  // whole windows a partition: fewer partitions when the table has fewer
+  // windows (histograms again)
+  const int lw = a->log2cap - log2_span(a);  // log2 windows
+  if (log2p > lw) {
+    log2p = lw;
+    hists(false);
+  }
+  const uint32_t np = 1u << log2p;
+  DevBuf prow, pbase, prec, cbase;
+  uint64_t nr_live = 0, nc_live = 0;
+  if (nr) nr_live = spill_scatter(a, pr, prow, pbase);
+  if (nc) nc_live = spill_scatter(a, pc, prec, cbase);
+  MergeDev M{};
+  M.rows = prow.get<uint64_t>();
+  M.nrows = nr_live;
+  M.rbase = nr_live ? pbase.get<uint64_t>() : nullptr;
+  M.rnb = pr.nb;
+  M.recs = prec.get<uint64_t>();
+  M.nrecs = nc_live;
+  M.cbase = nc_live ? cbase.get<uint64_t>() : nullptr;
+  M.cnb = pc.nb;
+  M.F = a->fmt;
+  const bool windows = a->merge_form == KG_AGG_MERGE_WINDOWS ||
+                       (a->merge_form == KG_AGG_MERGE_AUTO && lw - log2p <= 2);
+  if (windows)
+    merge_windows(a, M, log2p, np, occ0 > 0);
+  else
+    merge_by_key(a, M, log2p, np);
   KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 24, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
   a->bound_rows = a->bound_recs = 0;
@@ -2027,9 +2495,13 @@ void init_records(kg_ctx *ctx, uint64_t *rec, uint64_t nrec, const AggProg &P, h
 // The slots holding a group, in slot order: list[0..n) (flags, an exclusive
 // scan, then each flagged slot writes itself at its position).
 int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t ns, int W, bool global,
-                        DevBuf &list, hipStream_t s) {
+                        DevBuf &list, hipStream_t s, const uint64_t *keys) {
   DevBuf flags(ctx, ns * 4), pos(ctx, (ns + 1) * 8);
-  k_group_flags<<<stream_grid(ns, 256), 256, 0, s>>>(rec, ns, W, global, flags.get<uint32_t>());
+  if (keys)  // a hashed table of ns - 2 slots and the two fixed ones
+    k_group_flags_keys<<<stream_grid(ns, 256), 256, 0, s>>>(keys, rec, ns - 2, W,
+                                                            flags.get<uint32_t>());
+  else
+    k_group_flags<<<stream_grid(ns, 256), 256, 0, s>>>(rec, ns, W, global, flags.get<uint32_t>());
   KG_HIP(hipGetLastError());
   DevBuf temp(ctx, (uint64_t)scan_partials((int64_t)ns) * 8);
   exclusive_scan(flags.get<uint32_t>(), pos.get<uint64_t>(), (int64_t)ns, temp.get<uint64_t>(), s);
@@ -2203,8 +2675,11 @@ static std::unique_ptr<kg_batch> result_batch(kg_hash_agg *a, const uint64_t *ke
 // ---- PartialAggregate: flushes and pass-through (aggregate/partial.rs) -----
 static void reset_table(kg_hash_agg *a) {
   alloc_table(a, a->cap, a->keys, a->rec);
-  KG_HIP(hipMemsetAsync(a->occ.get(), 0, 8, a->s));
-  if (a->hashed) KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 24, a->s));
+  KG_HIP(hipMemsetAsync(a->occ.get(), 0, 16, a->s));
+  if (a->hashed) {
+    KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 24, a->s));
+    KG_HIP(hipMemsetAsync(a->hll.get(), 0, (size_t)kHllRegs * 4, a->s));
+  }
   a->occ_bound = a->merged_keys = 0;
   a->bound_rows = a->bound_recs = 0;
   a->finalized = false;
@@ -2224,7 +2699,8 @@ static uint64_t out_row_bytes(const kg_hash_agg *a) {
 static bool emit_table(kg_hash_agg *a) {
   drain(a);
   const int64_t groups = group_positions(a->ctx, a->rec.get<uint64_t>(), nslots(a), a->prog.W,
-                                         !a->has_key, a->pos, a->s);
+                                         !a->has_key, a->pos, a->s,
+                                         a->hashed ? a->keys.get<uint64_t>() : nullptr);
   if (groups > 0) {
     try {
       a->emitted.push_back(result_batch(a, a->keys.get<uint64_t>(), a->rec.get<uint64_t>(),
@@ -2371,9 +2847,23 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
         fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the LDS table");
       a->p1_lds = ((size_t)a->p1_slots + (size_t)(a->p1_slots + 2) * WL) * 8 + 4 * kDoorWords;
       a->m_lds = (size_t)a->m_slots * (1 + WL) * 8;
-      const std::pair<const void *, size_t> fns[2] = {
+      // windows of the windowed merge: the largest power of two of slots
+      // (key + record) within the merge's LDS
+      while ((2ull << a->log2span_max) <= (uint64_t)a->m_slots) a->log2span_max++;
+      a->w_lds_max = (size_t)std::min(optin, 160 * 1024) - 1024;
+      const std::pair<const void *, size_t> fns[4] = {
           {(const void *)k_agg_pass1<kTileRows>, a->p1_lds},
-          {(const void *)k_agg_merge<kMergeRows>, a->m_lds}};
+          {(const void *)k_agg_merge<kMergeRows>, a->m_lds},
+          {(const void *)k_agg_wmerge<kMergeRows, false>, a->w_lds_max},
+          {(const void *)k_agg_wmerge<kMergeRows, true>, a->w_lds_max}};
+      // spill partition histograms (and the sketch's registers) and cursors:
+      // 4 bytes a partition, up to 2^15
+      for (const void *f : {(const void *)k_spill_hist, (const void *)k_spill_scatter})
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                128 * 1024 + 4 * kHllRegs) != hipSuccess) {
+          (void)hipGetLastError();
+          fail(KG_ERR_DEVICE, "dynamic LDS of 132 KiB refused");
+        }
       for (auto &f : fns) {
         hipError_t e = hipFuncSetAttribute(f.first, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)f.second);
@@ -2384,6 +2874,8 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       }
       a->sp_cnt.reset(ctx, 32);
       KG_HIP(hipMemsetAsync(a->sp_cnt.get(), 0, 32, a->s));
+      a->hll.reset(ctx, (uint64_t)kHllRegs * 4);
+      KG_HIP(hipMemsetAsync(a->hll.get(), 0, (size_t)kHllRegs * 4, a->s));
       a->spill_budget = std::min<uint64_t>(16ull << 30, ctx->total_hbm / 16);
       if (ctx->budget) a->spill_budget = std::min<uint64_t>(a->spill_budget, ctx->budget / 4);
       if (a->spill_budget < (64u << 20)) a->spill_budget = 64u << 20;
@@ -2391,10 +2883,19 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
     uint64_t eg = expected_groups < 512 ? 512 : expected_groups;
     a->cap = a->direct ? 256 : (1ull << log2_ceil(eg + eg / 4));
     a->log2cap = log2_ceil(a->cap);
-    a->occ.reset(ctx, 8);
-    KG_HIP(hipMemsetAsync(a->occ.get(), 0, 8, a->s));
+    a->occ.reset(ctx, 16);
+    KG_HIP(hipMemsetAsync(a->occ.get(), 0, 16, a->s));
     alloc_table(a.get(), a->cap, a->keys, a->rec);
     *out = a.release();
+  });
+}
+
+int kg_agg_set_merge(kg_hash_agg *a, int32_t form, int32_t size_from_sketch) {
+  return guarded([&] {
+    if (form < KG_AGG_MERGE_AUTO || form > KG_AGG_MERGE_BY_KEY) fail(KG_ERR_INTERNAL, "merge form");
+    if (a->rows_seen) fail(KG_ERR_INTERNAL, "the merge form is set before the first sink");
+    a->merge_form = form;
+    a->size_from_sketch = size_from_sketch != 0;
   });
 }
 
@@ -2408,7 +2909,8 @@ int kg_agg_finalize(kg_hash_agg *a) {
     if (a->finalized) return;
     drain(a);
     a->ngroups = group_positions(a->ctx, a->rec.get<uint64_t>(), nslots(a), a->prog.W,
-                                 !a->has_key, a->pos, a->s);
+                                 !a->has_key, a->pos, a->s,
+                                 a->hashed ? a->keys.get<uint64_t>() : nullptr);
     a->finalized = true;
   });
 }

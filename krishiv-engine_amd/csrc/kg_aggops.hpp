@@ -72,15 +72,27 @@ __device__ __forceinline__ AggOp op_at(const AggProg &P, int o) {
   return P.ops[__builtin_amdgcn_readfirstlane(o)];
 }
 
+// The hashed table's slots are cut into windows of wmask + 1 slots (a power
+// of two): a key's home slot is the top log2cap bits of its slot hash, and it
+// probes linearly within the window of its home slot, so each window holds
+// exactly the keys homed in it (one workgroup can own a window, and a window
+// fits in LDS).  A window with no free slot left makes an insert fail: it
+// sets the window-full flag (occupied[1]) and the host reports it.
 struct Table {
   uint64_t *keys;
   uint64_t *rec;
-  unsigned long long *occupied;
+  unsigned long long *occupied;  // [0] keys inserted with atomics, [1] window-full flag
   uint64_t mask;
   int log2cap;
   uint64_t seed;
   uint64_t cap;
+  uint64_t wmask;
 };
+
+// The next slot of a probe: the slot after s within s's window.
+__host__ __device__ __forceinline__ uint64_t next_in_window(uint64_t s, uint64_t wmask) {
+  return (s & ~wmask) | ((s + 1) & wmask);
+}
 
 // ---------------------------------------------------------------- encodings
 __device__ __forceinline__ uint64_t f64_order(double f) {
@@ -493,11 +505,12 @@ __device__ __forceinline__ uint64_t slot_hash(const Table &T, uint64_t key) {
   return dev::table_hash(key, T.seed);
 }
 
-// Find-or-insert of `key` in the global table (never full: the host keeps
-// the load below 3/4).
+// Find-or-insert of `key` in the global table (the host keeps the load below
+// 3/4).  A full window (which that load makes all but impossible) flags the
+// table and returns the NULL key's slot: the host then fails the operator.
 __device__ __forceinline__ uint64_t global_slot(const Table &T, uint64_t key) {
   uint64_t s = slot_hash(T, key) >> (64 - T.log2cap);
-  while (true) {
+  for (uint64_t p = 0; p <= T.wmask; p++) {
     uint64_t k = __hip_atomic_load(&T.keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (k == key) return s;
     if (k == kEmpty) {
@@ -508,8 +521,10 @@ __device__ __forceinline__ uint64_t global_slot(const Table &T, uint64_t key) {
       }
       if (prev == key) return s;
     }
-    s = (s + 1) & T.mask;
+    s = next_in_window(s, T.wmask);
   }
+  atomicOr(T.occupied + 1, 1ull);
+  return T.cap + 1;
 }
 
 // Key of row i as the 64-bit table key, and its slot class:
@@ -687,8 +702,10 @@ void init_records(kg_ctx *ctx, uint64_t *rec, uint64_t nrec, const AggProg &P, h
 
 // The occupied records (rows > 0, or slot 0 of a global aggregate) in slot
 // order: list[0..n) holds their slot indexes.  Returns n, the group count.
+// A hashed table (keys given) flags its slots [0, nslots - 2) by their keys
+// and the two fixed slots by their rows.
 int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t nslots, int W, bool global,
-                        DevBuf &list, hipStream_t s);
+                        DevBuf &list, hipStream_t s, const uint64_t *keys = nullptr);
 
 // Writes the final values of the n records named by list (kernel k_agg_output).
 void launch_agg_output(const OutProg &O, const uint64_t *keys, int key_stride,

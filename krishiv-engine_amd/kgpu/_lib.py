@@ -21,6 +21,7 @@ COL_DEVICE = 1
 
 AGG_COUNT_STAR, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_AVG = 1, 2, 3, 4, 5, 6
 AGG_ANSI = 1
+AGG_MERGE_AUTO, AGG_MERGE_WINDOWS, AGG_MERGE_BY_KEY = 0, 1, 2
 JOIN_INNER, JOIN_LEFT_OUTER, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI = 0, 1, 2, 3
 JOIN_NULL_AWARE_ANTI, JOIN_EXISTENCE, JOIN_RIGHT_SEMI, JOIN_RIGHT_ANTI = 4, 5, 6, 7
 JOIN_RIGHT_OUTER, JOIN_FULL_OUTER = 8, 9
@@ -156,6 +157,7 @@ _SIGS = {
     "kg_agg_result_type": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "kg_agg_result": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kg_agg_set_bypass": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64]),
+    "kg_agg_set_merge": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "kg_agg_partial_next": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_agg_partial_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "kg_agg_destroy": (None, [C.c_void_p]),

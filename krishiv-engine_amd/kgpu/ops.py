@@ -460,6 +460,11 @@ class HashAggregate:
         """BypassPolicy (aggregate/partial.rs:20-50) of a partial aggregate."""
         check(lib().kg_agg_set_bypass(self.h, min_rows, max_group_percent, max_table_bytes))
 
+    def set_merge(self, form=0, size_from_sketch=True):
+        """The drain's merge form (0 auto, 1 windows in LDS, 2 by key) and
+        whether the table is sized from the sketch of the spilled keys."""
+        check(lib().kg_agg_set_merge(self.h, form, 1 if size_from_sketch else 0))
+
     def partial_batches(self):
         """The partial-state batches flushed or passed through so far."""
         out = []

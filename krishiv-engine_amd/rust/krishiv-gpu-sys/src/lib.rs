@@ -43,6 +43,9 @@ pub const KG_AGG_AVG: i32 = 6;
 pub const KG_AGG_ANSI: u32 = 1;
 pub const KG_AGG_PARTIAL: u32 = 2;
 pub const KG_AGG_MERGE: u32 = 4;
+pub const KG_AGG_MERGE_AUTO: i32 = 0;
+pub const KG_AGG_MERGE_WINDOWS: i32 = 1;
+pub const KG_AGG_MERGE_BY_KEY: i32 = 2;
 pub const KG_FORM_RANGE: i32 = 0;
 pub const KG_FORM_SMALL: i32 = 1;
 pub const KG_FORM_EXACT: i32 = 2;
@@ -337,6 +340,7 @@ extern "C" {
     pub fn kg_agg_set_bypass(a: *mut kg_hash_agg, min_rows: u64, max_group_percent: u32, max_table_bytes: u64) -> c_int;
     pub fn kg_agg_partial_next(a: *mut kg_hash_agg, out: *mut *mut kg_batch, done: *mut i32) -> c_int;
     pub fn kg_agg_partial_stats(a: *const kg_hash_agg, flushes: *mut u64, bypassed_rows: *mut u64) -> c_int;
+    pub fn kg_agg_set_merge(a: *mut kg_hash_agg, form: i32, size_from_sketch: i32) -> c_int;
     pub fn kg_agg_destroy(a: *mut kg_hash_agg);
     pub fn kg_bloom_create(ctx: *mut kg_ctx, nbytes: u64, out: *mut *mut kg_bloom) -> c_int;
     pub fn kg_bloom_bytes_for(items: u64, fpp: f64) -> u64;

@@ -24,7 +24,7 @@ T_MAP = {kgo.T_INT64: A.T_INT64, kgo.T_INT32: A.T_INT32, kgo.T_INT8: A.T_INT8,
 
 
 def run_both(key, kvalid, cols, types, aggs, ansi=False, batches=1, device=False,
-             expected_groups=1 << 12):
+             expected_groups=1 << 12, merge=None):
     """Runs the oracle and the GPU aggregate on the same rows; returns
     (oracle result, gpu batch)."""
     n = len(key) if key is not None else len(cols[0][0])
@@ -38,6 +38,8 @@ def run_both(key, kvalid, cols, types, aggs, ansi=False, batches=1, device=False
     g = kgpu.HashAggregate(ctx(), (T_MAP[ktype], 0, 0) if key is not None else None,
                            [(T_MAP[t[0]], t[1], t[2]) for t in types], aggs, ansi=ansi,
                            expected_groups=expected_groups)
+    if merge is not None:
+        g.set_merge(*merge)
     bounds = np.linspace(0, n, batches + 1).astype(np.int64)
     keep = []
     for b in range(batches):
@@ -365,11 +367,18 @@ def test_two_phase_equals_single_phase():
     compare(ref, r, aggs)
 
 
-def test_merge_resumes_after_table_growth():
-    """200K distinct keys into a table sized for 16 groups: the spill merge
-    runs out of table room, stops, the table grows, the leftover records and
-    pending rows merge with atomics and the stopped partitions resume."""
-    say("agg: merge quota stops and resumes")
+@pytest.mark.parametrize("merge", [(A.AGG_MERGE_BY_KEY, False), (A.AGG_MERGE_WINDOWS, False),
+                                   (A.AGG_MERGE_AUTO, True)],
+                         ids=["by-key-grows", "windows-grow", "auto-sketch"])
+def test_merge_resumes_after_table_growth(merge):
+    """200K distinct keys into a table sized for 16 groups.  By key, with the
+    sketch's sizing off: the spill merge runs out of table room, stops, the
+    table grows, the leftover records and pending rows merge with atomics
+    and the stopped partitions resume.  In windows, sizing off: windows fill,
+    are left as they were, and run again after each growth of the table,
+    the windows done skipped.  Auto: the sketch of the spilled keys sizes
+    the table before the merge."""
+    say(f"agg: merge with table growth {merge}")
     rng = np.random.default_rng(11)
     n = 600_000
     key = (rng.integers(0, 200_000, n) * 7919 - (1 << 40)).astype(np.int64)
@@ -380,5 +389,5 @@ def test_merge_resumes_after_table_growth():
     aggs = [(kgo.AGG_COUNT_STAR, 0), (kgo.AGG_SUM, 0), (kgo.AGG_MIN, 0), (kgo.AGG_MAX, 1),
             (kgo.AGG_AVG, 1), (kgo.AGG_COUNT, 0)]
     ref, b, _ = run_both(key, None, [(v, ok), (f, None)], types, aggs, batches=4,
-                         expected_groups=16)
+                         expected_groups=16, merge=merge)
     compare(ref, b, aggs)
