@@ -247,12 +247,17 @@ __device__ __forceinline__ void write_spill_rows(const AggProg &P, const SpillDe
   for (int c = 0; c < S.F.ncols; c++) {
     src.load(P, c, miss, lo, hi, f, ok);
     const bool fl = dev::is_f(P.in[c].type), wide = P.in[c].type == KG_T_DECIMAL128;
+    const bool packed = S.F.packed == c + 1;
     const int o = S.F.off[c];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (!miss[r]) continue;
+      if (ok[r]) vbits[r] |= 1ull << (S.F.vshift + c);
+      if (packed) {
+        if (ok[r]) vbits[r] |= (uint32_t)lo[r];
+        continue;
+      }
       uint64_t *w = S.rows + pos[r] * rw + o;
-      if (ok[r]) vbits[r] |= 1ull << c;
       w[0] = !ok[r] ? 0 : fl ? (uint64_t)__double_as_longlong(f[r]) : lo[r];
       if (wide) w[1] = ok[r] ? hi[r] : 0;
     }
@@ -1793,11 +1798,26 @@ static void record_types(kg_hash_agg *a, const AggProg &P) {
     int w = 2;
     a->fmt.vword = 1;
     a->fmt.ncols = a->nsink;
+    // the first 32-bit integer column rides in word 1's low half when the
+    // validity bits fit above it (bits 32-62)
+    a->fmt.packed = 0;
+    a->fmt.vshift = 0;
+    if (a->nsink <= 31)
+      for (int c = 0; c < a->nsink; c++)
+        if (P.in[c].type == KG_T_INT32 || P.in[c].type == KG_T_DATE32) {
+          a->fmt.packed = (uint8_t)(c + 1);
+          a->fmt.vshift = 32;
+          break;
+        }
     for (int c = 0; c < a->nsink; c++) {
       a->prog.in[c].type = P.in[c].type;
       a->prog.in[c].precision = P.in[c].precision;
       a->prog.in[c].scale = P.in[c].scale;
       a->prog.in[c].width = P.in[c].width;
+      if (a->fmt.packed == c + 1) {
+        a->fmt.off[c] = 1;
+        continue;
+      }
       a->fmt.off[c] = (uint8_t)w;
       w += P.in[c].type == KG_T_DECIMAL128 ? 2 : 1;
     }
@@ -1840,6 +1860,11 @@ static void pass1(kg_hash_agg *a, const AggProg &P, int64_t nrows) {
 // the per-block histogram first (spill_hist; with the sketch, the keys also
 // go into a->hll), then the scan and the scatter into `out` (spill_scatter;
 // partition p starts at base[p * nb]).
+// Workgroup of the partition kernels: their LDS holds a counter a partition,
+// so past 2^13 partitions one workgroup fills a CU; it then takes 1,024
+// lanes (16 waves) to keep enough loads and stores in flight.
+static unsigned spill_block(uint32_t np) { return np > (1u << 13) ? 1024u : 256u; }
+
 struct SpillPart {
   const uint64_t *in = nullptr;
   uint64_t n = 0;
@@ -1872,7 +1897,7 @@ static void spill_hist(kg_hash_agg *a, SpillPart &sp, const uint64_t *in, uint64
   sp.hist.reset(a->ctx, (uint64_t)np * nb * 4 + 4);
   Table t = table_of(a);
   ProfScope prof("agg_partition", a->s);
-  k_spill_hist<<<(unsigned)nb, 256, np * 4 + (sketch ? 4 * kHllRegs : 0), a->s>>>(
+  k_spill_hist<<<(unsigned)nb, spill_block(np), np * 4 + (sketch ? 4 * kHllRegs : 0), a->s>>>(
       in, w, n, t, log2p, per_block, dead ? 1 : 0, sp.hist.get<uint32_t>(),
       sketch ? a->hll.get<unsigned int>() : nullptr);
   KG_HIP(hipGetLastError());
@@ -1889,7 +1914,8 @@ static uint64_t spill_scatter(kg_hash_agg *a, SpillPart &sp, DevBuf &out, DevBuf
   DevBuf temp(a->ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
   exclusive_scan(sp.hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(),
                  a->s);
-  k_spill_scatter<<<sp.nb, 256, np * 4, a->s>>>(sp.in, sp.w, sp.n, t, sp.log2p, sp.per_block,
+  k_spill_scatter<<<sp.nb, spill_block(np), np * 4, a->s>>>(sp.in, sp.w, sp.n, t, sp.log2p,
+                                                           sp.per_block,
                                                 sp.dead ? 1 : 0, base.get<uint64_t>(),
                                                 out.get<uint64_t>());
   KG_HIP(hipGetLastError());

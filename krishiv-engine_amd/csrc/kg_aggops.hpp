@@ -63,6 +63,8 @@ struct AggProg {
   int spill_rw;
   int spill_vword;
   uint8_t spill_off[kAggInputs];
+  uint8_t spill_packed;  // 1 + the 32-bit column packed into word 1 (0: none)
+  uint8_t spill_vshift;  // the first validity bit of word 1
 };
 
 // Op o of the program through a uniform index: the op fields then come

@@ -6,9 +6,13 @@
 // values are loaded once per tile with one type dispatch.  The rows come
 // either from the input columns (ColSrc) or from spill rows (SpillSrc).
 //
-// Spill row (64-bit words): [0] key, [1] validity of input column c in bit c,
-// then each input column's value at SpillFmt::off[c]: one word (integers
-// sign-extended, floats as float64 bits) or two (decimal128 lo, hi).
+// Spill row (64-bit words): [0] key, [1] validity of input column c in bit
+// vshift + c, then each input column's value at SpillFmt::off[c]: one word
+// (integers sign-extended, floats as float64 bits) or two (decimal128 lo,
+// hi).  One 32-bit integer column may be packed into the low half of word 1
+// (packed = its index + 1; the validity bits then start at bit 32), which
+// makes the benchmark's row 32 bytes instead of 40.  Bit 63 of word 1 marks
+// a dead row (validity bits stop below it).
 // Spilled record: [0] key, then the W state words.
 #pragma once
 
@@ -21,6 +25,8 @@ struct SpillFmt {
   int ncols;   // input columns carried
   int vword;   // 1: word 1 holds the validity bits; 0: every value is valid
   uint8_t off[kAggInputs];
+  uint8_t packed = 0;  // 1 + the column held in word 1's low half (0: none)
+  uint8_t vshift = 0;  // the first validity bit of word 1
 };
 
 // Fills n LDS records (stride WL words) with the program's initial state.
@@ -39,6 +45,8 @@ __device__ __forceinline__ void fill_lds_records(const AggProg &P, uint64_t *s_r
 inline void set_spill_layout(AggProg &P, const SpillFmt &F) {
   P.spill_rw = F.rw;
   P.spill_vword = F.vword;
+  P.spill_packed = F.packed;
+  P.spill_vshift = F.vshift;
   for (int c = 0; c < kAggInputs; c++) P.spill_off[c] = F.off[c];
 }
 
@@ -164,9 +172,12 @@ struct SpillSrc {
         w2[r] = act[r] && wide ? row[o + 1] : 0;
       }
     }
+    const bool packed = P.spill_packed == c + 1;
+    const int vs = P.spill_vshift + c;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      ok[r] = act[r] && ((vb[r] >> c) & 1);
+      ok[r] = act[r] && ((vb[r] >> vs) & 1);
+      if (packed) v[r] = (uint64_t)(int64_t)(int32_t)(uint32_t)v[r];
       f[r] = __longlong_as_double((long long)v[r]);
       lo[r] = v[r];
       hi[r] = wide ? w2[r] : uns ? 0 : (uint64_t)((int64_t)v[r] >> 63);
