@@ -154,6 +154,7 @@ __global__ void __launch_bounds__(kAggBlock) k_agg_rows_atomic(AggProg P, Table 
 constexpr int kP1Block = 1024;
 constexpr int kMergeBlock = 1024;  // 16 waves a CU: latency of the spill reads hidden
 constexpr int kMergeRows = 2;      // rows a lane per tile (96 VGPRs at 1024 lanes)
+constexpr int kWMergeRows = 3;     // rows a lane per tile of the windowed merge
 constexpr int kTileRows = 4;
 constexpr int kWaveChunk = 512;
 constexpr int64_t kEvictRows = 65536;
@@ -2048,10 +2049,10 @@ static void merge_windows(kg_hash_agg *a, const MergeDev &M0, int log2p, uint32_
     {
       ProfScope prof("agg_merge", a->s);
       if (staged)
-        k_agg_wmerge<kMergeRows, true><<<(unsigned)grid, kMergeBlock, lds + stage, a->s>>>(
+        k_agg_wmerge<kWMergeRows, true><<<(unsigned)grid, kMergeBlock, lds + stage, a->s>>>(
             P, table_of(a), M);
       else
-        k_agg_wmerge<kMergeRows, false><<<(unsigned)grid, kMergeBlock, lds, a->s>>>(
+        k_agg_wmerge<kWMergeRows, false><<<(unsigned)grid, kMergeBlock, lds, a->s>>>(
             P, table_of(a), M);
       KG_HIP(hipGetLastError());
     }
@@ -2099,8 +2100,15 @@ static void drain(kg_hash_agg *a) {
   if (nr + nc >= (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "more than 2^32 spilled entries");
   const uint64_t occ0 = read_occupied(a);
   const int W = a->prog.W;
-  int log2p = 0;
-  while (log2p < 15 && (1ull << log2p) * 4096 < nr + nc) log2p++;
+  // partitions of about 4096 entries, whole windows each (of the table as
+  // it is; the sketch may grow it)
+  auto want_bits = [&]() {
+    const int lw = a->log2cap - log2_span(a);  // log2 windows
+    int b = 0;
+    while (b < 15 && b < lw && (1ull << b) * 4096 < nr + nc) b++;
+    return b;
+  };
+  int log2p = want_bits();
   SpillPart pr, pc;
   auto hists = [&](bool sketch) {
     if (sketch) KG_HIP(hipMemsetAsync(a->hll.get(), 0, (size_t)kHllRegs * 4, a->s));
@@ -2116,11 +2124,10 @@ static void drain(kg_hash_agg *a) {
     const uint64_t cap = drain_cap(a, nr, nc, occ0, est);
     if (cap > a->cap) grow(a, cap);
   }
-  // whole windows a partition: fewer partitions when the table has fewer
-  // windows (histograms again)
+  // a table grown well past the partitions: histograms again with more
   const int lw = a->log2cap - log2_span(a);  // log2 windows
-  if (log2p > lw) {
-    log2p = lw;
+  if (want_bits() > log2p + 2) {
+    log2p = want_bits();
     hists(false);
   }
   const uint32_t np = 1u << log2p;
@@ -2322,7 +2329,10 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
   struct Key {
     int kind, col, checked, avg;
   };
-  std::vector<std::pair<Key, int>> shared;
+  struct Words {
+    int w, st;
+  };
+  std::vector<std::pair<Key, Words>> shared;
   // integral columns that an AVG sums exactly: a SUM of the same column
   // shares that 128-bit sum rather than keeping a word of its own
   std::vector<bool> avg_sums(in_types.size(), false);
@@ -2331,18 +2341,33 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
       if (aggs[i].func == KG_AGG_AVG && aggs[i].input >= 0 &&
           aggs[i].input < (int)in_types.size() && is_integral(in_types[aggs[i].input].type))
         avg_sums[aggs[i].input] = true;
+  // one set of words per (op, column, checked): a decimal SUM and AVG of one
+  // column share their checked sum and its state word (both results are
+  // NULL, or an error under ANSI, once the running sum leaves the bound)
   auto op_words = [&](int kind, int col, bool chk, bool avg, int nwords, uint64_t iv,
                       int &st_word) {
-    if (!chk)
-      for (auto &kv : shared)
-        if (kv.first.kind == kind && kv.first.col == col && !kv.first.checked && kv.first.avg == avg)
-          return kv.second;
+    for (auto &kv : shared)
+      if (kv.first.kind == kind && kv.first.col == col && kv.first.checked == (int)chk &&
+          kv.first.avg == (int)avg) {
+        st_word = kv.second.st;
+        return kv.second.w;
+      }
     int w = words(nwords, iv);
     st_word = chk ? words(1, 0) : 0;
     ops.push_back(AggOp{(uint8_t)kind, (uint8_t)col, (uint8_t)chk, 0, (uint16_t)w,
                         (uint16_t)st_word});
-    shared.push_back({Key{kind, col, chk, avg}, w});
+    shared.push_back({Key{kind, col, chk, avg}, Words{w, st_word}});
     return w;
+  };
+  // the ordered replay of a checked sum, once per set of words (SUM's
+  // semantics when a SUM shares the words of an AVG)
+  auto add_checked = [&](const CheckedSum &cs) {
+    for (auto &x : checked)
+      if (x.word == cs.word) {
+        x.avg = x.avg && cs.avg;
+        return;
+      }
+    checked.push_back(cs);
   };
   auto nulls_of = [&](int col) {
     if (col >= kAggInputs) fail(KG_ERR_UNSUPPORTED, "at most 32 input columns per sink");
@@ -2416,8 +2441,8 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
           if (chk) {
             o.st_word = st;
             int tw = merge ? 8 : type_width(t.type);
-            checked.push_back({c, o.word, st, false, false, (unsigned __int128)1 << 63,
-                               (unsigned __int128)1 << (8 * tw - 1)});
+            add_checked({c, o.word, st, false, false, (unsigned __int128)1 << 63,
+                         (unsigned __int128)1 << (8 * tw - 1)});
           }
         } else if (is_decimal(t.type)) {
           if (t.scale < 0) fail(KG_ERR_UNSUPPORTED, "negative decimal scale");
@@ -2425,12 +2450,13 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
           // the sum state's carrier (decimal_on, aggregate/mod.rs:1004-1011)
           int sum_type = (t.type == KG_T_DECIMAL64 && sum_p <= 18) ? KG_T_DECIMAL64 : KG_T_DECIMAL128;
           bool wide = merge ? sum_type == KG_T_DECIMAL128 : t.type == KG_T_DECIMAL128;
-          o.word = op_words(wide ? OP_ADD192 : OP_ADD128, c, true, avg, wide ? 3 : 2, 0, st);
+          // (merged partial states of a SUM and an AVG are separate columns)
+          o.word = op_words(wide ? OP_ADD192 : OP_ADD128, c, true, false, wide ? 3 : 2, 0, st);
           o.wide = wide;
           o.st_word = st;
           o.nulls_word = nulls_of(c);
-          checked.push_back({c, o.word, st, avg, wide, pow10_u128(sum_p),
-                             pow10_u128(merge ? sum_p : t.precision)});
+          add_checked({c, o.word, st, avg, wide, pow10_u128(sum_p),
+                       pow10_u128(merge ? sum_p : t.precision)});
           o.st_type = sum_type;
           o.st_p = sum_p;
           o.st_s = t.scale;
@@ -2880,8 +2906,8 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       const std::pair<const void *, size_t> fns[4] = {
           {(const void *)k_agg_pass1<kTileRows>, a->p1_lds},
           {(const void *)k_agg_merge<kMergeRows>, a->m_lds},
-          {(const void *)k_agg_wmerge<kMergeRows, false>, a->w_lds_max},
-          {(const void *)k_agg_wmerge<kMergeRows, true>, a->w_lds_max}};
+          {(const void *)k_agg_wmerge<kWMergeRows, false>, a->w_lds_max},
+          {(const void *)k_agg_wmerge<kWMergeRows, true>, a->w_lds_max}};
       // spill partition histograms (and the sketch's registers) and cursors:
       // 4 bytes a partition, up to 2^15
       for (const void *f : {(const void *)k_spill_hist, (const void *)k_spill_scatter})
