@@ -839,7 +839,9 @@ def agg(args, key="int64"):
     aggs = [(A.AGG_COUNT_STAR, 0)]
     for c in range(3):
         aggs += [(A.AGG_SUM, c), (A.AGG_MIN, c), (A.AGG_MAX, c), (A.AGG_AVG, c)]
-    expected = nd if key == "int64" else 4
+    # no group-count estimate: the table is sized at the drain from the
+    # sketch of the spilled keys (the one-byte key's table is fixed)
+    expected = 0
 
     groups = [0]
 

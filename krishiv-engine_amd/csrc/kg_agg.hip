@@ -2100,15 +2100,11 @@ static void drain(kg_hash_agg *a) {
   if (nr + nc >= (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "more than 2^32 spilled entries");
   const uint64_t occ0 = read_occupied(a);
   const int W = a->prog.W;
-  // partitions of about 4096 entries, whole windows each (of the table as
-  // it is; the sketch may grow it)
-  auto want_bits = [&]() {
-    const int lw = a->log2cap - log2_span(a);  // log2 windows
-    int b = 0;
-    while (b < 15 && b < lw && (1ull << b) * 4096 < nr + nc) b++;
-    return b;
-  };
-  int log2p = want_bits();
+  // partitions of 4K-8K entries (at most 2^15), and whole windows each:
+  // counted before the sketch sizes the table, again if that leaves fewer
+  // windows than partitions
+  int log2p = 0;
+  while (log2p < 15 && (1ull << log2p) * 8192 < nr + nc) log2p++;
   SpillPart pr, pc;
   auto hists = [&](bool sketch) {
     if (sketch) KG_HIP(hipMemsetAsync(a->hll.get(), 0, (size_t)kHllRegs * 4, a->s));
@@ -2124,10 +2120,9 @@ static void drain(kg_hash_agg *a) {
     const uint64_t cap = drain_cap(a, nr, nc, occ0, est);
     if (cap > a->cap) grow(a, cap);
   }
-  // a table grown well past the partitions: histograms again with more
   const int lw = a->log2cap - log2_span(a);  // log2 windows
-  if (want_bits() > log2p + 2) {
-    log2p = want_bits();
+  if (log2p > lw) {
+    log2p = lw;
     hists(false);
   }
   const uint32_t np = 1u << log2p;
