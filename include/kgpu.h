@@ -766,13 +766,15 @@ int kg_server_add_table(kg_server *s, const int64_t *keys, int64_t n, int32_t *t
 /* Queues a request (its inputs must stay valid until it completes). */
 int kg_server_submit(kg_server *s, const kg_serve_request *r);
 /* Non-blocking: retires finished batches (their ids to done_ids, at most
- * max), then launches the next batch from the queue if a stream is free.
- * wait != 0 blocks until at least one batch retires (or nothing is left). */
+ * max; ids beyond max are reported by the next calls), then launches the
+ * next batch from the queue if a stream is free.  wait != 0 blocks until at
+ * least one batch retires (or nothing is left).  A failed large request
+ * fails the call that meets it with no id reported yet. */
 int kg_server_progress(kg_server *s, int32_t wait, int64_t *done_ids, int32_t max,
                        int32_t *ndone);
 /* The finished request's result [key, sum, count] (once; then forgotten). */
 int kg_server_take_result(kg_server *s, int64_t id, kg_batch **out);
-/* Requests queued and in flight. */
+/* Requests queued, in flight, or finished with their ids not yet reported. */
 int64_t kg_server_pending(const kg_server *s);
 void kg_server_destroy(kg_server *s);
 
@@ -834,6 +836,7 @@ typedef struct kg_tune_config {
   int32_t _pad;
 } kg_tune_config;
 int kg_tune_get(kg_ctx *ctx, int64_t build_rows, kg_tune_config *out);
+/* load_factor_pct 0 returns the class to its untuned default. */
 int kg_tune_set(kg_ctx *ctx, int64_t build_rows, const kg_tune_config *cfg);
 int kg_autotune_joinagg(kg_ctx *ctx, int64_t build_rows, int64_t probe_rows, kg_tune_config *best,
                         double *best_ms, int32_t *candidates);

@@ -122,6 +122,10 @@ void *dev_alloc(kg_ctx *ctx, uint64_t bytes);
 // The tuned shape for a build side of `rows` (defaults where none is stored).
 TuneCfg tune_for(kg_ctx *ctx, int64_t rows);
 void tune_store(kg_ctx *ctx, int64_t rows, const TuneCfg &c);
+// Returns a size class to its untuned default.
+void tune_clear(kg_ctx *ctx, int64_t rows);
+// Whether a size class has a tuned (or set) shape.
+bool tune_has(kg_ctx *ctx, int64_t rows);
 // Frees a block.  With a stream, the block is reused only after the work
 // enqueued on that stream so far, and every read of caller columns enqueued
 // so far (use fences), has completed; without one, after a device

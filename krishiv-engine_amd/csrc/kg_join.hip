@@ -1130,6 +1130,8 @@ struct kg_hash_join {
   uint64_t dspan = 0;
   // fused aggregate
   bool has_agg = false;
+  TuneCfg shape;  // the size class's shape when the table was built (its load
+                  // factor sized the table; the partitioned probe takes the rest)
   AggProg prog{};
   OutProg out{};
   std::vector<kg_column> agg_types;
@@ -1443,6 +1445,7 @@ int kg_join_build_finalize(kg_hash_join *j) {
     if (want < 512) want = 512;
     // capacity for the tuned load factor (50 % unless tuned), a power of two
     const TuneCfg tc = tune_for(j->ctx, (int64_t)want);
+    j->shape = tc;
     const uint64_t pct = (uint64_t)std::min(90, std::max(10, (int)tc.load_pct));
     const uint64_t cap = 1ull << jlog2((want * 100 + pct - 1) / pct);
     // every buffer first: a refusal for memory leaves the build unfinalized
@@ -1925,7 +1928,7 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   // one 1024-lane workgroup a CU with the whole LDS: twice the window of two
   // 512-lane ones, so half the partitions (longer runs for the scatter);
   // the tuner may pick the two 512-lane ones
-  const TuneCfg tc = tune_for(j->ctx, j->build_rows);
+  const TuneCfg tc = j->shape;
   const int mlanes = tc.merge_lanes == 512 ? 512 : 1024;
   const uint64_t lds_all = (uint64_t)std::min(optin, 160 * 1024);
   const uint64_t lds_cap = (mlanes == 1024 ? lds_all : lds_all / 2) - 1024;

@@ -252,6 +252,16 @@ void tune_store(kg_ctx *ctx, int64_t rows, const TuneCfg &c) {
   ctx->tune[size_class(rows)] = c;
 }
 
+void tune_clear(kg_ctx *ctx, int64_t rows) {
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  ctx->tune.erase(size_class(rows));
+}
+
+bool tune_has(kg_ctx *ctx, int64_t rows) {
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  return ctx->tune.count(size_class(rows)) != 0;
+}
+
 int type_width(int t) {
   switch (t) {
     case KG_T_BOOL: return 0;

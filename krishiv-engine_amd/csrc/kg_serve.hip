@@ -245,6 +245,9 @@ struct kg_server {
   std::set<kg_hash_join *> large_busy;  // resident joins a worker is using
   std::deque<LargeDone> large_done;
   int64_t large_outstanding = 0;  // queued + running (main thread's count)
+  // finished requests whose ids did not fit a progress call's id buffer:
+  // reported first by the next call
+  std::deque<int64_t> unreported;
   bool large_stop = false;
   std::vector<std::thread> large_workers;
   int nworkers = 1;
@@ -361,13 +364,22 @@ static void large_loop(kg_server *S) {
   }
 }
 
+// An id to the caller's buffer, or to the next call's when it is full.
+static void report(kg_server *S, int64_t id, int64_t *done_ids, int32_t max, int32_t *ndone) {
+  if (*ndone < max) done_ids[(*ndone)++] = id;
+  else S->unreported.push_back(id);
+}
+
 // Moves finished large requests into the results (waiting for one when
-// `block`); a failed one is reported by this call.
+// `block`).  A failed one fails the call that meets it first, unless this
+// call has already reported ids: it then stays queued for the next call, so
+// no reported id is lost behind an error.
 static void collect_large(kg_server *S, bool block, int64_t *done_ids, int32_t max,
                           int32_t *ndone) {
   std::unique_lock<std::mutex> g(S->large_mu);
   if (block) S->large_cv.wait(g, [&] { return !S->large_done.empty(); });
   while (!S->large_done.empty() && *ndone < max) {
+    if (S->large_done.front().code != KG_OK && *ndone > 0) return;
     kg_server::LargeDone d = std::move(S->large_done.front());
     S->large_done.pop_front();
     S->large_outstanding--;
@@ -576,7 +588,7 @@ static void retire(kg_server *S, kg_server::Slot &sl, int64_t *done_ids, int32_t
                               hipMemcpyDeviceToDevice, sl.s));
     }
     S->results[sl.reqs[q].r.id] = b.release();
-    if (*ndone < max) done_ids[(*ndone)++] = sl.reqs[q].r.id;
+    report(S, sl.reqs[q].r.id, done_ids, max, ndone);
   }
   KG_HIP(hipStreamSynchronize(sl.s));
   S->inflight -= (int64_t)sl.reqs.size();
@@ -700,6 +712,10 @@ int kg_server_progress(kg_server *S, int32_t wait, int64_t *done_ids, int32_t ma
                        int32_t *ndone) {
   return guarded([&] {
     *ndone = 0;
+    while (!S->unreported.empty() && *ndone < max) {
+      done_ids[(*ndone)++] = S->unreported.front();
+      S->unreported.pop_front();
+    }
     bool waited = false;
     for (int pass = 0; pass < 2; pass++) {
       for (auto &sl : S->slots) {
@@ -737,7 +753,9 @@ int kg_server_take_result(kg_server *S, int64_t id, kg_batch **out) {
 }
 
 int64_t kg_server_pending(const kg_server *S) {
-  return S ? (int64_t)S->queue.size() + S->inflight + S->large_outstanding : 0;
+  return S ? (int64_t)S->queue.size() + S->inflight + S->large_outstanding +
+                 (int64_t)S->unreported.size()
+           : 0;
 }
 
 void kg_server_destroy(kg_server *S) {

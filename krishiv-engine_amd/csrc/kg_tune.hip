@@ -64,6 +64,10 @@ int kg_tune_get(kg_ctx *ctx, int64_t build_rows, kg_tune_config *out) {
 int kg_tune_set(kg_ctx *ctx, int64_t build_rows, const kg_tune_config *cfg) {
   return guarded([&] {
     if (!ctx || !cfg) fail(KG_ERR_INTERNAL, "no context or config");
+    if (cfg->load_factor_pct == 0) {  // back to the class's untuned default
+      tune_clear(ctx, build_rows);
+      return;
+    }
     check_cfg(*cfg);
     TuneCfg c;
     c.load_pct = cfg->load_factor_pct;
@@ -141,17 +145,22 @@ int kg_autotune_joinagg(kg_ctx *ctx, int64_t build_rows, int64_t probe_rows, kg_
     TuneCfg win;
     double win_ms = -1.0;
     int n = 0;
-    // a failed candidate leaves the class's shape as it was
+    // a failed candidate leaves the class's shape as it was (untuned stays
+    // untuned)
     const TuneCfg saved = tune_for(ctx, nb);
+    const bool had = tune_has(ctx, nb);
     struct Restore {
       kg_ctx *ctx;
       int64_t nb;
       TuneCfg c;
+      bool had;
       bool armed = true;
       ~Restore() {
-        if (armed) tune_store(ctx, nb, c);
+        if (!armed) return;
+        if (had) tune_store(ctx, nb, c);
+        else tune_clear(ctx, nb);
       }
-    } restore{ctx, nb, saved};
+    } restore{ctx, nb, saved, had};
     for (int li = 0; li < 3; li++)
       for (int ri = 0; ri < 3; ri++)
         for (int mi = 0; mi < 2; mi++) {

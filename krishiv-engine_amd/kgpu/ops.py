@@ -982,8 +982,12 @@ def tune_get(ctx, build_rows: int) -> dict:
     return c.as_dict()
 
 
-def tune_set(ctx, build_rows: int, load_factor_pct: int = 50, scatter_rows: int = 0,
+def tune_set(ctx, build_rows: int, load_factor_pct: int = None, scatter_rows: int = 0,
              merge_lanes: int = 1024):
+    """Forces the shape of build_rows' size class; with nothing but the
+    rows given the class returns to its untuned default."""
+    if load_factor_pct is None:
+        load_factor_pct = 0 if (scatter_rows, merge_lanes) == (0, 1024) else 50
     c = TuneConfig(load_factor_pct, scatter_rows, merge_lanes, 0)
     check(lib().kg_tune_set(ctx.h, build_rows, C.byref(c)))
 

@@ -126,3 +126,34 @@ def test_admission_splits_batches_and_refuses_oversize():
     with pytest.raises(kgpu.KgError, match="arena"):
         srv.submit(99, big, big, 8, build_keys=big, build_rows=4_000_000)
     srv.close()
+
+
+def test_every_id_reported_once_through_a_one_id_buffer():
+    """A batch retires more requests than the caller's id buffer holds: the
+    ids past it are reported by the next calls, none is lost, and the
+    server counts them as pending until they are."""
+    say("serve: ids through a one-id buffer")
+    rng = np.random.default_rng(5)
+    c = ctx()
+    srv = kgpu.JoinAggServer(c, arena_bytes=256 << 20, max_batch=16)
+    keep, want = [], {}
+    for rid in range(10):
+        bkeys, fk, fkv, pay = make_request(rng, 2000)
+        bufs = [DeviceBuffer.from_numpy(c, x) for x in
+                (bkeys, fk, np.packbits(fkv, bitorder="little"), pay)]
+        keep += bufs
+        srv.submit(rid, bufs[1], bufs[3], len(fk), build_keys=bufs[0], build_rows=2000,
+                   probe_valid=bufs[2])
+        want[rid] = oracle_request(bkeys, fk, fkv, pay)
+    seen = []
+    for _ in range(200):
+        if not srv.pending():
+            break
+        ids = srv.progress(wait=True, max_ids=1)
+        assert len(ids) <= 1
+        seen += ids
+    assert sorted(seen) == list(range(10)), seen
+    assert srv.pending() == 0
+    for rid in range(10):
+        assert got_request(srv.result(rid)) == want[rid]
+    srv.close()

@@ -59,7 +59,7 @@ def test_every_shape_gives_the_oracle_result():
                     kgpu.tune_set(c, NB, load, rows, lanes)
                     assert _gpu(bk, pk, pkv, amt) == want, (load, rows, lanes)
     finally:
-        kgpu.tune_set(c, NB)  # the defaults again for the other tests
+        kgpu.tune_set(c, NB)  # the class's default again for the other tests
 
 
 def test_autotune_times_every_candidate_and_keeps_one():
@@ -75,7 +75,8 @@ def test_autotune_times_every_candidate_and_keeps_one():
         assert kgpu.tune_get(c, 150_000) == best  # the same size class (2^18)
     finally:
         kgpu.tune_set(c, 200_000)
-    assert kgpu.tune_get(c, 200_000) == {"load_factor_pct": 50, "scatter_rows": 0,
+    # back to the untuned default of the class (2^18 rows: 25 %), not a tuned 50 %
+    assert kgpu.tune_get(c, 200_000) == {"load_factor_pct": 25, "scatter_rows": 0,
                                          "merge_lanes": 1024}
     for bad in ({"load_factor_pct": 5}, {"scatter_rows": 6}, {"merge_lanes": 256}):
         with pytest.raises(kgpu.KgError):
@@ -104,6 +105,11 @@ def test_tuned_shapes_save_and_load(tmp_path):
         bad.write_text("13 50 7 1024\n")
         with pytest.raises(kgpu.KgError):
             kgpu.tune_load(c, str(bad))
+        # a class returned to its default is not saved as tuned
+        kgpu.tune_set(c, 5000)
+        kgpu.tune_save(c, path)
+        classes = [int(line.split()[0]) for line in open(path) if line.strip()]
+        assert 13 not in classes and 22 in classes, classes
     finally:
         kgpu.tune_set(c, 5000)
         kgpu.tune_set(c, 3_000_000)
