@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkgpu.so")
+# KGPU_LIB: another build of the same sources (the host-sanitized one)
+LIB_PATH = os.environ.get("KGPU_LIB") or os.path.join(HERE, "libkgpu.so")
 
 OK = 0
 ERR_UNSUPPORTED, ERR_OVERFLOW, ERR_OUT_OF_RANGE, ERR_MEMORY, ERR_INTERNAL, ERR_DEVICE = 1, 2, 3, 4, 5, 6

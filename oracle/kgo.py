@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libkgo.so")
+# KGO_LIB: another build of the same sources (the sanitized one)
+_LIB_PATH = os.environ.get("KGO_LIB") or os.path.join(_HERE, "libkgo.so")
 
 T_BOOL, T_INT8, T_INT16, T_INT32, T_INT64 = 1, 2, 3, 4, 5
 T_UINT8, T_UINT16, T_UINT32, T_UINT64 = 6, 7, 8, 9

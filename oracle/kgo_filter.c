@@ -525,12 +525,12 @@ int kgo_kf_finish(kgo_kf *f) {
   if (f->distinct_ok) {
     f->nmembers = f->ndist;
     f->members = malloc((size_t)(f->ndist > 0 ? f->ndist : 1) * sizeof(i128));
-    memcpy(f->members, f->dist, (size_t)f->ndist * sizeof(i128));
+    if (f->ndist > 0) memcpy(f->members, f->dist, (size_t)f->ndist * sizeof(i128));
     if ((uint64_t)f->ndist <= f->opt.max_values) {
       f->values_ok = 1;
       f->nvalues = f->ndist;
       f->values = malloc((size_t)(f->ndist > 0 ? f->ndist : 1) * sizeof(i128));
-      memcpy(f->values, f->dist, (size_t)f->ndist * sizeof(i128));
+      if (f->ndist > 0) memcpy(f->values, f->dist, (size_t)f->ndist * sizeof(i128));
     }
   }
   int dense = f->distinct_ok && f->has_range && f->cls == CLS_INT &&
