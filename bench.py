@@ -65,6 +65,8 @@ def parse():
     p.add_argument("--grace-hbm-budget", type=int, default=0,
                    help="grace: context HBM budget in bytes (partitions past it go to pinned host)")
     p.add_argument("--requests", type=int, default=10_000)
+    p.add_argument("--serve-max-build", type=float, default=1e7,
+                   help="serve: builds log-uniform in [1K, this] rows")
     p.add_argument("--qps", type=float, default=400.0)
     p.add_argument("--chunk", type=int, default=48)
     p.add_argument("--autotune", action="store_true",
@@ -541,10 +543,11 @@ def serve(args):
     L = kgpu.lib()
     rng = np.random.default_rng(8)
     R = args.requests
-    sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e7), R)).astype(np.int64)
+    top = args.serve_max_build
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(top), R)).astype(np.int64)
     resident = rng.random(R) < 0.3
     table_of = rng.integers(0, 8, R)
-    res_sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e7), 8)).astype(np.int64)
+    res_sizes = np.exp(rng.uniform(np.log(1e3), np.log(top), 8)).astype(np.int64)
     gaps = rng.exponential(1.0 / args.qps, R)
     # resident builds (keys 0..n-1 permuted), built once for both forms
     res_keys = []
@@ -673,7 +676,7 @@ def serve(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic (seeded counter-based generator, generated in HBM per chunk)",
         "config": {"workload": f"{R} join+aggregate requests, Poisson {args.qps} q/s (seed 8), build "
-                               "log-uniform [1K, 10M], probe 10x, 30 % on 8 resident builds, "
+                               f"log-uniform [1K, {int(top):,}], probe 10x, 30 % on 8 resident builds, "
                                "arena 64 GB", "chunk": args.chunk,
                    "large_workers": int(os.environ.get("KG_SERVE_LARGE_WORKERS", "1")),
                    **({"autotuned_classes": tuned} if tuned else {})},
@@ -952,6 +955,10 @@ def main():
             ("config2_agg_int8", lambda: agg(args, "int8")),
             ("config3_bloom_join", lambda: bloom(args)),
             ("config5_served", lambda: serve(args)),
+            # the same mix with builds of at most 100K rows: requests whose
+            # cost is the per-query launch and host work the batched form saves
+            ("config5_served_small_builds", lambda: serve(argparse.Namespace(
+                **{**vars(args), "serve_max_build": 1e5, "no_cpu": True}))),
             ("config4_grace", lambda: grace(args))]
     for name, fn in runs:
         gc.collect()
