@@ -68,7 +68,6 @@ def parse():
     p.add_argument("--serve-max-build", type=float, default=1e7,
                    help="serve: builds log-uniform in [1K, this] rows")
     p.add_argument("--qps", type=float, default=400.0)
-    p.add_argument("--chunk", type=int, default=48)
     p.add_argument("--autotune", action="store_true",
                    help="joinagg: run the shape autotuner for this build size before timing; "
                         "serve: for the size classes of the large requests' builds")
@@ -530,11 +529,12 @@ def serve(args):
     each is SELECT key, SUM(pay), COUNT(*) FROM build JOIN probe ON key = fk
     GROUP BY key with a build of log-uniform [1K, 10M] unique int64 keys and
     a probe 10x as long (fk uniform over 1.1x the key range, int64 payload);
-    30 % probe one of 8 resident builds.  Inputs are generated in HBM per
-    chunk of --chunk requests before the chunk's arrivals start (generation
-    is timed apart and excluded from both forms).  The plain form runs one
-    join-aggregate launch sequence per request in arrival order on one
-    stream; the batched form submits each request to the server as it
+    30 % probe one of 8 resident builds.  Build sizes are snapped to 64
+    geometric size classes whose inputs (and the resident tables' probe
+    inputs) are generated in HBM once before either form runs (timed apart),
+    so all requests arrive as one continuous Poisson stream.  The plain form
+    runs one join-aggregate launch sequence per request in arrival order on
+    one stream; the batched form submits each request to the server as it
     arrives.  Latency = completion - arrival."""
     import kgpu
     from kgpu import Col, DeviceBuffer
@@ -575,75 +575,74 @@ def serve(args):
     srv = kgpu.JoinAggServer(ctx, arena_bytes=64 << 30, max_batch=256, use_graph=True)
     srv_tables = [srv.add_table(res_keys[t], int(res_sizes[t])) for t in range(8)]
 
+    # each request's build size snapped to one of 64 geometric size classes
+    # over [1K, top]; the inputs of every class (and of every resident
+    # table's probes) are generated once, before either form runs, so the
+    # whole stream of requests arrives as one Poisson process with nothing
+    # else on the GPU
+    grid = np.unique(np.exp(np.linspace(np.log(1e3), np.log(top), 64)).astype(np.int64))
+    sizes = grid[np.abs(np.log(sizes)[:, None] - np.log(grid)[None, :]).argmin(axis=1)]
+
     def nb_of(r):
         return int(res_sizes[table_of[r]]) if resident[r] else int(sizes[r])
 
-    def gen_chunk(lo, hi):
-        bufs = {}
-        for r in range(lo, hi):
-            nb = nb_of(r)
-            npr = 10 * nb
-            bk = None
-            if not resident[r]:
-                bk = DeviceBuffer(ctx, nb * 8)
-                A.check(L.kg_gen_join_build(ctx.h, 2 * r + 1, 0, nb, nb, bk.ptr, None))
-            fk = DeviceBuffer(ctx, npr * 8)
-            pay = DeviceBuffer(ctx, npr * 8)
-            A.check(L.kg_gen_join_probe(ctx.h, 2 * r + 2, 0, npr, int(nb * 1.1), 0, fk.ptr, None, pay.ptr))
-            bufs[r] = (bk, fk, pay, nb, npr)
-        ctx.synchronize()
-        return bufs
+    t0 = time.perf_counter()
+    pool = {}
+    for nb in sorted({nb_of(r) for r in range(R)}):
+        npr = 10 * nb
+        bk = DeviceBuffer(ctx, nb * 8)
+        A.check(L.kg_gen_join_build(ctx.h, 2 * nb + 1, 0, nb, nb, bk.ptr, None))
+        fk = DeviceBuffer(ctx, npr * 8)
+        pay = DeviceBuffer(ctx, npr * 8)
+        A.check(L.kg_gen_join_probe(ctx.h, 2 * nb + 2, 0, npr, int(nb * 1.1), 0, fk.ptr, None,
+                                    pay.ptr))
+        pool[nb] = (bk, fk, pay, nb, npr)
+    ctx.synchronize()
+    gen_s = time.perf_counter() - t0
+    arrive_at = np.cumsum(gaps) - gaps[0]
 
     def run_form(batched, saturated=False):
         lat = np.zeros(R)
-        gen_s = 0.0
-        busy_s = 0.0
-        for lo in range(0, R, args.chunk):
-            hi = min(R, lo + args.chunk)
-            t0 = time.perf_counter()
-            bufs = gen_chunk(lo, hi)
-            gen_s += time.perf_counter() - t0
-            arrive = np.zeros(hi - lo) if saturated else np.cumsum(gaps[lo:hi]) - gaps[lo]
-            start = time.perf_counter()
-            if not batched:
-                for r in range(lo, hi):
-                    wait = arrive[r - lo] - (time.perf_counter() - start)
-                    if wait > 0:
-                        time.sleep(wait)
-                    bk, fk, pay, nb, npr = bufs[r]
-                    if resident[r]:
-                        j = plain_tables[table_of[r]]
+        arrive = np.zeros(R) if saturated else arrive_at
+        start = time.perf_counter()
+        if not batched:
+            for r in range(R):
+                wait = arrive[r] - (time.perf_counter() - start)
+                if wait > 0:
+                    time.sleep(wait)
+                bk, fk, pay, nb, npr = pool[nb_of(r)]
+                if resident[r]:
+                    j = plain_tables[table_of[r]]
+                else:
+                    j = kgpu.HashJoin(ctx, A.JOIN_INNER, [0], [0], expected_build_rows=nb)
+                    j.build_sink([Col(A.T_INT64, bk, length=nb)], nb)
+                    j.build_finalize()
+                j.aggregate([(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)])
+                j.probe_aggregate([Col(A.T_INT64, fk, length=npr), Col(A.T_INT64, pay, length=npr)],
+                                  [1], npr)
+                res = j.aggregate_result()
+                lat[r] = time.perf_counter() - start - arrive[r]
+                del res
+                if not resident[r]:
+                    j.close()
+        else:
+            nxt, done = 0, 0
+            while done < R:
+                now = time.perf_counter() - start
+                while nxt < R and arrive[nxt] <= now:
+                    bk, fk, pay, nb, npr = pool[nb_of(nxt)]
+                    if resident[nxt]:
+                        srv.submit(nxt, fk, pay, npr, table=srv_tables[table_of[nxt]])
                     else:
-                        j = kgpu.HashJoin(ctx, A.JOIN_INNER, [0], [0], expected_build_rows=nb)
-                        j.build_sink([Col(A.T_INT64, bk, length=nb)], nb)
-                        j.build_finalize()
-                    j.aggregate([(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)])
-                    j.probe_aggregate([Col(A.T_INT64, fk, length=npr), Col(A.T_INT64, pay, length=npr)], [1], npr)
-                    res = j.aggregate_result()
-                    lat[r] = time.perf_counter() - start - arrive[r - lo]
-                    del res
-                    if not resident[r]:
-                        j.close()
-            else:
-                nxt, done = lo, 0
-                while done < hi - lo:
-                    now = time.perf_counter() - start
-                    while nxt < hi and arrive[nxt - lo] <= now:
-                        bk, fk, pay, nb, npr = bufs[nxt]
-                        if resident[nxt]:
-                            srv.submit(nxt, fk, pay, npr, table=srv_tables[table_of[nxt]])
-                        else:
-                            srv.submit(nxt, fk, pay, npr, build_keys=bk, build_rows=nb)
-                        nxt += 1
-                    ids = srv.progress(wait=nxt >= hi)
-                    t = time.perf_counter() - start
-                    for r in ids:
-                        lat[r] = t - arrive[r - lo]
-                        srv.result(r)
-                        done += 1
-            busy_s += time.perf_counter() - start
-            del bufs
-        return lat, busy_s, gen_s
+                        srv.submit(nxt, fk, pay, npr, build_keys=bk, build_rows=nb)
+                    nxt += 1
+                ids = srv.progress(wait=nxt >= R)
+                t = time.perf_counter() - start
+                for r in ids:
+                    lat[r] = t - arrive[r]
+                    srv.result(r)
+                    done += 1
+        return lat, time.perf_counter() - start, gen_s
 
     out = {}
     dump = {"probe_rows": np.array([10 * nb_of(r) for r in range(R)]), "resident": resident}
@@ -674,10 +673,12 @@ def serve(args):
         "metric": "batched vs plain join-aggregate queries/s (saturated) and p50/p99 latency at a fixed QPS",
         "value": out["batched"]["queries_per_s"], "unit": "queries/s", "n_gpus": 1,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
-        "data": "synthetic (seeded counter-based generator, generated in HBM per chunk)",
+        "data": "synthetic (seeded counter-based generator; one input set per build size class, "
+                "generated in HBM before the stream)",
         "config": {"workload": f"{R} join+aggregate requests, Poisson {args.qps} q/s (seed 8), build "
                                f"log-uniform [1K, {int(top):,}], probe 10x, 30 % on 8 resident builds, "
-                               "arena 64 GB", "chunk": args.chunk,
+                               "arena 64 GB", "size_classes": int(len(grid)),
+                   "generation_s": gen_s,
                    "large_workers": int(os.environ.get("KG_SERVE_LARGE_WORKERS", "1")),
                    **({"autotuned_classes": tuned} if tuned else {})},
         "plain": out["plain"], "batched": out["batched"],
