@@ -273,7 +273,13 @@ typedef struct kg_hash_agg kg_hash_agg;
 int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
                   const kg_column *input_types, int32_t naggs, const kg_agg_spec *aggs,
                   uint32_t flags, uint64_t expected_groups, kg_hash_agg **out);
-/* Sink::sink of raw rows. */
+/* Sink::sink of raw rows.  Host batches of fewer rows than a staging slot
+ * (512K rows at up to 64 B a row) are copied to pinned staging and gathered
+ * in HBM (up to 4M rows or 256 MB), and aggregated when the next batch does
+ * not fit, a device batch arrives, or at finalize: an error of that work
+ * (NeedMemory, overflow) is returned by that later call.  NeedMemory leaves
+ * the gathered rows in place and the call's own batch not taken, so the
+ * same batch retried is counted once.  Without a bypass policy only. */
 int kg_agg_sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, int64_t nrows);
 /* Sink::finalize: checks overflow, fixes the group count. */
 int kg_agg_finalize(kg_hash_agg *a);

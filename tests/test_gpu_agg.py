@@ -391,3 +391,99 @@ def test_merge_resumes_after_table_growth(merge):
     ref, b, _ = run_both(key, None, [(v, ok), (f, None)], types, aggs, batches=4,
                          expected_groups=16, merge=merge)
     compare(ref, b, aggs)
+
+
+def test_host_batches_are_gathered():
+    """Host batches smaller than a staging slot (512K rows here) are gathered
+    in HBM before they are aggregated: ragged batches at odd row and bit
+    offsets into the caller's arrays, NULL keys and values, the table's
+    sentinel key, batches of a slot or more (the gathered rows are
+    aggregated first), a device batch, and more rows than the 4M-row HBM
+    buffer holds; bit-exact against the oracle."""
+    say("agg: gathered host batches")
+    rng = np.random.default_rng(12)
+    n = 5_000_000
+    key = rng.integers(-3000, 3000, n).astype(np.int64)
+    key[rng.random(n) < 0.001] = np.iinfo(np.int64).min
+    kv = rng.random(n) > 0.02
+    v = rng.integers(-(1 << 40), 1 << 40, n)
+    vv = rng.random(n) > 0.05
+    types = [(kgo.T_INT64, 0, 0)]
+    aggs = [(kgo.AGG_COUNT_STAR, 0), (kgo.AGG_SUM, 0), (kgo.AGG_MIN, 0), (kgo.AGG_MAX, 0),
+            (kgo.AGG_COUNT, 0)]
+    o = kgo.Agg((kgo.T_INT64, 0, 0), types, aggs)
+    o.update(kgo.Col(kgo.T_INT64, key, kv), [kgo.Col(kgo.T_INT64, v, vv)])
+    ref = o.finish()
+    g = kgpu.HashAggregate(ctx(), (A.T_INT64, 0, 0), [(A.T_INT64, 0, 0)], aggs,
+                           expected_groups=1 << 10)
+    sizes = []
+    left = n
+    while left > 0:
+        r = rng.random()
+        m = int(rng.integers(600_000, 700_000)) if r < 0.02 else int(rng.integers(1, 70_000))
+        m = min(m, left)
+        sizes.append(m)
+        left -= m
+    dev_at = len(sizes) // 2
+    lo = 0
+    keep = []
+    # host batches view the whole arrays at element (and bit) offset lo
+    kc = Col(A.T_INT64, key, kv, length=n)
+    vc = Col(A.T_INT64, v, vv, length=n)
+    for b, m in enumerate(sizes):
+        if b == dev_at:
+            kb = kgpu.DeviceBuffer.from_numpy(ctx(), key[lo:lo + m])
+            kvb = kgpu.DeviceBuffer.from_numpy(ctx(), np.packbits(kv[lo:lo + m], bitorder="little"))
+            vb = kgpu.DeviceBuffer.from_numpy(ctx(), v[lo:lo + m])
+            vvb = kgpu.DeviceBuffer.from_numpy(ctx(), np.packbits(vv[lo:lo + m], bitorder="little"))
+            keep += [kb, kvb, vb, vvb]
+            g.sink(Col(A.T_INT64, kb, kvb, length=m), [Col(A.T_INT64, vb, vvb, length=m)], m)
+        else:
+            kc.offset = vc.offset = lo
+            g.sink(kc, [vc], m)
+        lo += m
+    g.finalize()
+    compare(ref, g.result(), aggs)
+
+
+def test_host_partial_states_are_gathered():
+    """The final phase of a two-phase aggregate fed host copies of the
+    partial states (decimal128 sums, BOOL nonempty flags, counts) in small
+    batches, which it gathers; against the oracle over all rows."""
+    say("agg: gathered host partial states")
+    rng = np.random.default_rng(13)
+    n = 120_000
+    key = rng.integers(0, 5000, n)
+    dec = rng.integers(-(10 ** 12) + 1, 10 ** 12, n)
+    i32 = rng.integers(-(1 << 31), (1 << 31) - 1, n).astype(np.int32)
+    oks = [rng.random(n) > 0.1, rng.random(n) > 0.3]
+    types = [(kgo.T_DECIMAL64, 12, 2), (kgo.T_INT32, 0, 0)]
+    aggs = [(kgo.AGG_COUNT_STAR, 0)] + [(fn, c) for c in range(2) for fn in ALL6]
+    o = kgo.Agg((kgo.T_INT64, 0, 0), types, aggs)
+    o.update(kgo.Col(kgo.T_INT64, key), [kgo.Col(t[0], vals, ok, precision=t[1], scale=t[2])
+                                         for vals, ok, t in zip((dec, i32), oks, types)])
+    ref = o.finish()
+    at = [(T_MAP[t[0]], t[1], t[2]) for t in types]
+    final = kgpu.HashAggregate(ctx(), (A.T_INT64, 0, 0), at, aggs, merge=True)
+    cuts = [0, 40_000, 95_000, n]
+    for a_, b_ in zip(cuts[:-1], cuts[1:]):
+        p = kgpu.HashAggregate(ctx(), (A.T_INT64, 0, 0), at, aggs, partial=True)
+        p.sink(Col(A.T_INT64, key[a_:b_]),
+               [Col(T_MAP[t[0]], vals[a_:b_], ok[a_:b_], t[1], t[2])
+                for vals, ok, t in zip((dec, i32), oks, types)])
+        p.finalize()
+        st = p.result()
+        host = []
+        for i in range(st.num_columns):
+            ct, cp, cs = st.column_type(i)
+            vals, ok = st.column_raw(i)
+            if ct == A.T_DECIMAL128:
+                vals = vals.view(np.uint64)
+            host.append((ct, cp, cs, vals, ok))
+        m = st.num_rows
+        for lo in range(0, m, 777):
+            hi = min(m, lo + 777)
+            cols = [Col(ct, vals[lo:hi], ok[lo:hi], cp, cs) for ct, cp, cs, vals, ok in host]
+            final.sink(cols[0], cols[1:], hi - lo)
+    final.finalize()
+    compare(ref, final.result(), aggs)

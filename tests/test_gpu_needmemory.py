@@ -88,6 +88,52 @@ def test_aggregate_sink_and_finalize_refused_then_retried():
             assert np.array_equal(gv[go], rv[ro]), j
 
 
+def test_gathered_host_batches_refused_then_retried():
+    """Small host batches are gathered in HBM; the batch that does not fit
+    makes the gathered rows aggregate first.  Refused for memory there, the
+    gathered rows stay and the batch is not taken: retried, it is counted
+    once."""
+    say("needmemory: gathered host batches")
+    c = kgpu.Context(0, 2 << 30)
+    n, nd, m = 4_400_000, 200_000, 100_000
+    cdf = kgo.zipf_cdf(nd, 1.1)
+    key, dec, f, i32, _ = kgo.gen_agg(22, 0, n, cdf, nd)
+    types = [(A.T_DECIMAL64, 12, 2), (A.T_FLOAT64, 0, 0), (A.T_INT32, 0, 0)]
+    aggs = [(A.AGG_COUNT_STAR, 0), (A.AGG_SUM, 0), (A.AGG_MAX, 0), (A.AGG_SUM, 2),
+            (A.AGG_MIN, 2)]
+    g = kgpu.HashAggregate(c, (A.T_INT64, 0, 0), types, aggs, expected_groups=1 << 12)
+    kc = Col(A.T_INT64, key)
+    ins = [Col(A.T_DECIMAL64, dec, precision=12, scale=2), Col(A.T_FLOAT64, f),
+           Col(A.T_INT32, i32)]
+    refused = 0
+    for lo in range(0, n, m):
+        kc.offset = lo
+        for col in ins:
+            col.offset = lo
+        if lo == 41 * m:  # 4.1M rows gathered: 100K more do not fit its 2^22 rows
+            held = fill_budget(c, 8)
+            with pytest.raises(kgpu.KgError, match="UNABLE_TO_ACQUIRE_MEMORY"):
+                g.sink(kc, ins, min(m, n - lo))
+            refused += 1
+            release(held)
+        g.sink(kc, ins, min(m, n - lo))
+    assert refused == 1
+    g.finalize()
+    r = g.result()
+    o = kgo.Agg((kgo.T_INT64, 0, 0), [(kgo.T_DECIMAL64, 12, 2), (kgo.T_FLOAT64, 0, 0),
+                                      (kgo.T_INT32, 0, 0)], aggs)
+    o.update(kgo.Col(kgo.T_INT64, key), [kgo.Col(kgo.T_DECIMAL64, dec, precision=12, scale=2),
+                                         kgo.Col(kgo.T_FLOAT64, f), kgo.Col(kgo.T_INT32, i32)])
+    rk, _, rres = o.finish(raw=True)
+    gk, _ = r.column_raw(0)
+    assert len(gk) == len(rk)
+    go, ro = np.argsort(gk), np.argsort(rk)
+    assert np.array_equal(gk[go], rk[ro])
+    for j in range(len(aggs)):
+        gv, _ = r.column_raw(1 + j)
+        assert np.array_equal(gv[go], np.asarray(rres[j][0])[ro]), j
+
+
 def test_join_build_sink_and_finalize_refused_then_retried():
     say("needmemory: join build")
     c = kgpu.Context(0, 512 << 20)
