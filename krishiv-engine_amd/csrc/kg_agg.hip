@@ -1107,11 +1107,12 @@ __device__ __forceinline__ double i128_to_double(uint64_t lo, uint64_t hi) {
 
 // One lane per output row o (the compacted slot list[o]).  Validity words
 // are whole 32-bit words from wave ballots (no atomics): lanes past the end
-// redo the last row, whose stores repeat the same values.
+// redo the last row, whose stores repeat the same values; a wave wholly past
+// the end stores nothing.
 __device__ __forceinline__ void wave_bits(uint8_t *bm, int64_t obase, int64_t n, bool bit) {
   uint64_t m = __ballot(bit);
   const int lane = __lane_id();
-  if (lane == 0) reinterpret_cast<uint32_t *>(bm)[obase >> 5] = (uint32_t)m;
+  if (lane == 0 && obase < n) reinterpret_cast<uint32_t *>(bm)[obase >> 5] = (uint32_t)m;
   if (lane == 32 && obase + 32 < n) reinterpret_cast<uint32_t *>(bm)[(obase >> 5) + 1] = (uint32_t)(m >> 32);
 }
 
@@ -1571,7 +1572,8 @@ static int log2_ceil(uint64_t v) {
 // Keys in the table: the device count (inserts made with global_slot) plus
 // the merge's inserts.
 static uint64_t read_occupied(kg_hash_agg *a) {
-  uint64_t v[2] = {0, 0};
+  PinnedReads pin;
+  uint64_t *v = pin.take<uint64_t>(2);
   KG_HIP(hipMemcpyAsync(v, a->occ.get(), 16, hipMemcpyDeviceToHost, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
   if (v[1]) fail(KG_ERR_INTERNAL, "a window of the aggregation table filled");
@@ -1655,9 +1657,10 @@ static uint64_t drain_need(kg_hash_agg *a, uint64_t nr, uint64_t nc, uint64_t oc
   return need;
 }
 
-// Doubles the table (rehash of every key and record).
-static void grow(kg_hash_agg *a, uint64_t min_cap) {
-  uint64_t keys_now = read_occupied(a);
+// Doubles the table (rehash of every key and record).  keys_now: the
+// occupied count when the caller has just read it.
+static void grow(kg_hash_agg *a, uint64_t min_cap, uint64_t keys_now = ~0ull) {
+  if (keys_now == ~0ull) keys_now = read_occupied(a);
   uint64_t ncap = a->cap * 2;
   while (ncap < min_cap) ncap *= 2;
   DevBuf nkeys, nrec;
@@ -1672,8 +1675,8 @@ static void grow(kg_hash_agg *a, uint64_t min_cap) {
   k_rehash<<<stream_grid(ocap + 2, 256), 256, 0, a->s>>>(a->prog, okeys.get<uint64_t>(),
                                                           orec.get<uint64_t>(), ocap, t);
   KG_HIP(hipGetLastError());
-  KG_HIP(hipMemcpyAsync(a->occ.get(), &keys_now, 8, hipMemcpyHostToDevice, a->s));
-  KG_HIP(hipStreamSynchronize(a->s));
+  k_fill_u64<<<1, 64, 0, a->s>>>(a->occ.get<uint64_t>(), 1, keys_now);
+  KG_HIP(hipGetLastError());
   a->merged_keys = 0;
   (void)read_occupied(a);  // a window that filled during the rehash
 }
@@ -1716,8 +1719,13 @@ static void launch_small(kg_hash_agg *a, const AggProg &Q, int64_t n) {
     k_small_slots<<<stream_grid(n, 256, 16), 256, 0, a->s>>>(Q, n, a->small_bits.get<unsigned>());
     KG_HIP(hipGetLastError());
     unsigned bits[9];
-    KG_HIP(hipMemcpyAsync(bits, a->small_bits.get(), 36, hipMemcpyDeviceToHost, a->s));
-    KG_HIP(hipStreamSynchronize(a->s));
+    {
+      PinnedReads pin;
+      unsigned *p = pin.take<unsigned>(9);
+      KG_HIP(hipMemcpyAsync(p, a->small_bits.get(), 36, hipMemcpyDeviceToHost, a->s));
+      KG_HIP(hipStreamSynchronize(a->s));
+      memcpy(bits, p, sizeof(bits));
+    }
     D = 0;
     for (int sl = 0; sl < 257; sl++) {
       remap[sl] = -1;
@@ -1779,8 +1787,23 @@ static SpillDev spill_of(kg_hash_agg *a) {
   return S;
 }
 
+// The spill counters and the table's occupied count with one round trip.
+static void read_counts(kg_hash_agg *a, uint64_t &nrows, uint64_t &nrecs, uint64_t &occupied) {
+  PinnedReads pin;
+  uint64_t *c = pin.take<uint64_t>(3), *v = pin.take<uint64_t>(2);
+  KG_HIP(hipMemcpyAsync(c, a->sp_cnt.get(), 24, hipMemcpyDeviceToHost, a->s));
+  KG_HIP(hipMemcpyAsync(v, a->occ.get(), 16, hipMemcpyDeviceToHost, a->s));
+  KG_HIP(hipStreamSynchronize(a->s));
+  if (c[2]) fail(KG_ERR_INTERNAL, "aggregation spill overflow");
+  if (v[1]) fail(KG_ERR_INTERNAL, "a window of the aggregation table filled");
+  nrows = c[0];
+  nrecs = c[1];
+  occupied = v[0] + a->merged_keys;
+}
+
 static void read_spill_counts(kg_hash_agg *a, uint64_t &nrows, uint64_t &nrecs) {
-  uint64_t c[3];
+  PinnedReads pin;
+  uint64_t *c = pin.take<uint64_t>(3);
   KG_HIP(hipMemcpyAsync(c, a->sp_cnt.get(), 24, hipMemcpyDeviceToHost, a->s));
   KG_HIP(hipStreamSynchronize(a->s));
   if (c[2]) fail(KG_ERR_INTERNAL, "aggregation spill overflow");
@@ -1946,41 +1969,57 @@ static void spill_hist(kg_hash_agg *a, SpillPart &sp, const uint64_t *in, uint64
   KG_HIP(hipGetLastError());
 }
 
-// Returns the live entries (the scanned total).
-static uint64_t spill_scatter(kg_hash_agg *a, SpillPart &sp, DevBuf &out, DevBuf &base) {
-  const uint32_t np = 1u << sp.log2p;
-  const uint64_t hn = (uint64_t)np * sp.nb;
+// The scan of the histogram into base (partition p of block b starts at
+// base[p * nb + b]); the live total is read back with the caller's next
+// round trip: base[hn - 1] + hist[hn - 1] (enqueue_total).
+static void spill_scan(kg_hash_agg *a, SpillPart &sp, DevBuf &base) {
+  const uint64_t hn = ((uint64_t)1 << sp.log2p) * sp.nb;
   base.reset(a->ctx, hn * 8 + 8);
-  out.reset(a->ctx, sp.n * sp.w * 8 + 8);
-  Table t = table_of(a);
   ProfScope prof("agg_partition", a->s);
   DevBuf temp(a->ctx, (uint64_t)scan_partials((int64_t)hn) * 8);
   exclusive_scan(sp.hist.get<uint32_t>(), base.get<uint64_t>(), (int64_t)hn, temp.get<uint64_t>(),
                  a->s);
+}
+
+// The last partition's base and count (pinned host memory, read after the
+// caller's sync).
+struct SpillTotal {
+  uint64_t *last_base = nullptr;
+  uint32_t *last_count = nullptr;
+  uint64_t live() const { return *last_base + *last_count; }
+};
+
+static void enqueue_total(kg_hash_agg *a, const SpillPart &sp, const DevBuf &base,
+                          const SpillTotal &t) {
+  const uint64_t hn = ((uint64_t)1 << sp.log2p) * sp.nb;
+  KG_HIP(hipMemcpyAsync(t.last_base, base.get<uint64_t>() + hn - 1, 8, hipMemcpyDeviceToHost,
+                        a->s));
+  KG_HIP(hipMemcpyAsync(t.last_count, sp.hist.get<uint32_t>() + hn - 1, 4,
+                        hipMemcpyDeviceToHost, a->s));
+}
+
+// The live entries to `out`, partitioned by the scan in base.
+static void spill_scatter(kg_hash_agg *a, SpillPart &sp, DevBuf &out, const DevBuf &base) {
+  const uint32_t np = 1u << sp.log2p;
+  out.reset(a->ctx, sp.n * sp.w * 8 + 8);
+  Table t = table_of(a);
+  ProfScope prof("agg_partition", a->s);
   k_spill_scatter<<<sp.nb, spill_block(np), np * 4, a->s>>>(sp.in, sp.w, sp.n, t, sp.log2p,
-                                                           sp.per_block,
-                                                sp.dead ? 1 : 0, base.get<uint64_t>(),
-                                                out.get<uint64_t>());
+                                                           sp.per_block, sp.dead ? 1 : 0,
+                                                           base.get<uint64_t>(),
+                                                           out.get<uint64_t>());
   KG_HIP(hipGetLastError());
-  uint64_t lb = 0;
-  uint32_t lc = 0;
-  KG_HIP(hipMemcpyAsync(&lb, base.get<uint64_t>() + hn - 1, 8, hipMemcpyDeviceToHost, a->s));
-  KG_HIP(hipMemcpyAsync(&lc, sp.hist.get<uint32_t>() + hn - 1, 4, hipMemcpyDeviceToHost, a->s));
-  KG_HIP(hipStreamSynchronize(a->s));
-  return lb + lc;
 }
 
 // The sketch's estimate of the distinct keys spilled since the last drain
 // (HyperLogLog with the small-range correction of linear counting).
-static double spilled_keys_estimate(kg_hash_agg *a) {
-  std::vector<uint32_t> reg(kHllRegs);
-  KG_HIP(hipMemcpyAsync(reg.data(), a->hll.get(), (size_t)kHllRegs * 4, hipMemcpyDeviceToHost,
-                        a->s));
-  KG_HIP(hipStreamSynchronize(a->s));
+// reg: the kHllRegs registers read back.
+static double spilled_keys_estimate(const uint32_t *reg) {
   const double m = kHllRegs;
   double sum = 0.0;
   int zeros = 0;
-  for (uint32_t v : reg) {
+  for (int r = 0; r < kHllRegs; r++) {
+    const uint32_t v = reg[r];
     sum += std::ldexp(1.0, -(int)v);
     zeros += v == 0;
   }
@@ -2099,8 +2138,13 @@ static void merge_windows(kg_hash_agg *a, const MergeDev &M0, int log2p, uint32_
       KG_HIP(hipGetLastError());
     }
     unsigned int nf = 0;
-    KG_HIP(hipMemcpyAsync(&nf, failed.get(), 4, hipMemcpyDeviceToHost, a->s));
-    KG_HIP(hipStreamSynchronize(a->s));
+    {
+      PinnedReads pin;
+      unsigned int *p = pin.take<unsigned int>(1);
+      KG_HIP(hipMemcpyAsync(p, failed.get(), 4, hipMemcpyDeviceToHost, a->s));
+      KG_HIP(hipStreamSynchronize(a->s));
+      nf = *p;
+    }
     if (nf == 0) return;
     if (round >= 40) fail(KG_ERR_INTERNAL, "aggregation table windows kept filling");
     // twice the slots: the windows done keep their keys (rehashed), the rest
@@ -2136,31 +2180,43 @@ static void merge_windows(kg_hash_agg *a, const MergeDev &M0, int log2p, uint32_
 // when the partitions hold few entries a window, by key (k_agg_merge).
 static void drain(kg_hash_agg *a) {
   if (!a->hashed) return;
-  uint64_t nr, nc;
-  read_spill_counts(a, nr, nc);
+  uint64_t nr, nc, occ0;
+  read_counts(a, nr, nc, occ0);
   if (nr == 0 && nc == 0) return;
   if (nr + nc >= (1ull << 32)) fail(KG_ERR_UNSUPPORTED, "more than 2^32 spilled entries");
-  const uint64_t occ0 = read_occupied(a);
   const int W = a->prog.W;
   // partitions of 4K-8K entries (at most 2^15), and whole windows each:
   // counted before the sketch sizes the table, again if that leaves fewer
-  // windows than partitions
+  // windows than partitions.  The histograms' scans, their totals and the
+  // sketch come back with one round trip.
   int log2p = 0;
   while (log2p < 15 && (1ull << log2p) * 8192 < nr + nc) log2p++;
   SpillPart pr, pc;
+  DevBuf pbase, cbase;
+  PinnedReads pin;
+  uint32_t *reg = pin.take<uint32_t>(kHllRegs);
+  SpillTotal tr{pin.take<uint64_t>(1), pin.take<uint32_t>(1)};
+  SpillTotal tc{pin.take<uint64_t>(1), pin.take<uint32_t>(1)};
   auto hists = [&](bool sketch) {
     if (sketch) KG_HIP(hipMemsetAsync(a->hll.get(), 0, (size_t)kHllRegs * 4, a->s));
     if (nr) spill_hist(a, pr, a->sp_rows.get<uint64_t>(), nr, a->fmt.rw, true, log2p, sketch);
     if (nc) spill_hist(a, pc, a->sp_recs.get<uint64_t>(), nc, 1 + W, false, log2p, sketch);
+    if (nr) spill_scan(a, pr, pbase);
+    if (nc) spill_scan(a, pc, cbase);
+    if (sketch)
+      KG_HIP(hipMemcpyAsync(reg, a->hll.get(), (size_t)kHllRegs * 4, hipMemcpyDeviceToHost, a->s));
+    if (nr) enqueue_total(a, pr, pbase, tr);
+    if (nc) enqueue_total(a, pc, cbase, tc);
+    KG_HIP(hipStreamSynchronize(a->s));
   };
   hists(a->size_from_sketch);
-  const double est = a->size_from_sketch ? spilled_keys_estimate(a) : -1.0;
+  const double est = a->size_from_sketch ? spilled_keys_estimate(reg) : -1.0;
   // everything the merge may take, checked before the spill or the table changes
   need_room(a->ctx, drain_need(a, nr, nc, occ0, a->rows_seen + a->pending_rows, 0, est),
             "the merge of the aggregation spill");
   if (est >= 0.0) {
     const uint64_t cap = drain_cap(a, nr, nc, occ0, est);
-    if (cap > a->cap) grow(a, cap);
+    if (cap > a->cap) grow(a, cap, occ0);
   }
   const int lw = a->log2cap - log2_span(a);  // log2 windows
   if (log2p > lw) {
@@ -2168,10 +2224,10 @@ static void drain(kg_hash_agg *a) {
     hists(false);
   }
   const uint32_t np = 1u << log2p;
-  DevBuf prow, pbase, prec, cbase;
-  uint64_t nr_live = 0, nc_live = 0;
-  if (nr) nr_live = spill_scatter(a, pr, prow, pbase);
-  if (nc) nc_live = spill_scatter(a, pc, prec, cbase);
+  DevBuf prow, prec;
+  if (nr) spill_scatter(a, pr, prow, pbase);
+  if (nc) spill_scatter(a, pc, prec, cbase);
+  const uint64_t nr_live = nr ? tr.live() : 0, nc_live = nc ? tc.live() : 0;
   MergeDev M{};
   M.rows = prow.get<uint64_t>();
   M.nrows = nr_live;
@@ -2212,8 +2268,13 @@ static bool proven(kg_hash_agg *a, const CheckedSum &c, const DCol &col, int64_t
   k_abs_sum<<<stream_grid(n, 256), 256, 0, a->s>>>(col, n, a->scratch_abs.get<double>());
   KG_HIP(hipGetLastError());
   double batch = 0.0;
-  KG_HIP(hipMemcpyAsync(&batch, a->scratch_abs.get(), 8, hipMemcpyDeviceToHost, a->s));
-  KG_HIP(hipStreamSynchronize(a->s));
+  {
+    PinnedReads pin;
+    double *p = pin.take<double>(1);
+    KG_HIP(hipMemcpyAsync(p, a->scratch_abs.get(), 8, hipMemcpyDeviceToHost, a->s));
+    KG_HIP(hipStreamSynchronize(a->s));
+    batch = *p;
+  }
   double total = (c.cum_abs + batch) * (1.0 + 1e-9) + 1.0;
   double bound = (double)c.bound;
   batch_abs = batch;
@@ -2231,9 +2292,8 @@ static uint64_t sink_need(kg_hash_agg *a, const kg_column *inputs, int64_t nrows
     if (replay) need += (uint64_t)nrows * 8;
     return need;
   }
-  uint64_t nr, nc;
-  read_spill_counts(a, nr, nc);
-  const uint64_t occupied = read_occupied(a);
+  uint64_t nr = 0, nc = 0, occupied = 0;
+  if (a->rows_seen) read_counts(a, nr, nc, occupied);  // else nothing is spilled or held
   if (replay) {  // drain, then every row into the table directly
     need += (uint64_t)nrows * 8 + drain_need(a, nr, nc, occupied, new_keys);
     uint64_t cap = a->cap;
@@ -2797,20 +2857,20 @@ int64_t group_positions(kg_ctx *ctx, const uint64_t *rec, uint64_t ns, int W, bo
   KG_HIP(hipGetLastError());
   DevBuf temp(ctx, (uint64_t)scan_partials((int64_t)ns) * 8);
   exclusive_scan(flags.get<uint32_t>(), pos.get<uint64_t>(), (int64_t)ns, temp.get<uint64_t>(), s);
-  uint64_t last_pos = 0;
-  uint32_t last_flag = 0;
-  KG_HIP(hipMemcpyAsync(&last_pos, pos.get<uint64_t>() + ns - 1, 8, hipMemcpyDeviceToHost, s));
-  KG_HIP(hipMemcpyAsync(&last_flag, flags.get<uint32_t>() + ns - 1, 4, hipMemcpyDeviceToHost, s));
+  PinnedReads pin;
+  uint64_t *last_pos = pin.take<uint64_t>(1);
+  uint32_t *last_flag = pin.take<uint32_t>(1);
+  KG_HIP(hipMemcpyAsync(last_pos, pos.get<uint64_t>() + ns - 1, 8, hipMemcpyDeviceToHost, s));
+  KG_HIP(hipMemcpyAsync(last_flag, flags.get<uint32_t>() + ns - 1, 4, hipMemcpyDeviceToHost, s));
   KG_HIP(hipStreamSynchronize(s));
-  int64_t n = (int64_t)(last_pos + last_flag);
+  int64_t n = (int64_t)(*last_pos + *last_flag);
   list.reset(ctx, (uint64_t)n * 8 + 8);
   if (n > 0) {
     k_compact_slots<<<stream_grid(ns, 256), 256, 0, s>>>(flags.get<uint32_t>(), pos.get<uint64_t>(),
                                                          ns, list.get<uint64_t>());
     KG_HIP(hipGetLastError());
-    KG_HIP(hipStreamSynchronize(s));
   }
-  return n;
+  return n;  // list is ready in stream order (the scratch is freed behind it)
 }
 
 // Bits [off, off + n) of src to dst from bit 0 (a column's validity, or a
@@ -2866,6 +2926,8 @@ static std::unique_ptr<kg_batch> result_batch(kg_hash_agg *a, const uint64_t *ke
     auto b = std::make_unique<kg_batch>();
     b->ctx = a->ctx;
     b->nrows = n;
+    // validity words are not cleared: the output kernel (and the key copy)
+    // writes every word that holds a bit below n
     uint64_t vb = (uint64_t)ceil_div(n, 32) * 4 + 4;
     OutProg O = a->out;
     O.cap = cap;
@@ -2889,7 +2951,6 @@ static std::unique_ptr<kg_batch> result_batch(kg_hash_agg *a, const uint64_t *ke
       }
       kc.values.reset(a->ctx, bytes);
       kc.validity.reset(a->ctx, vb);
-      KG_HIP(hipMemsetAsync(kc.validity.get(), 0, vb, a->s));
       key_vals = kc.values.get();
       key_valid = kc.validity.get<uint8_t>();
       base = 1;
@@ -2905,7 +2966,6 @@ static std::unique_ptr<kg_batch> result_batch(kg_hash_agg *a, const uint64_t *ke
       int w = type_width(c.type);
       c.values.reset(a->ctx, (uint64_t)n * (w > 0 ? w : 1) + 16);
       c.validity.reset(a->ctx, vb);
-      KG_HIP(hipMemsetAsync(c.validity.get(), 0, vb, a->s));
       o.values = c.values.get();
       o.validity = c.validity.get<uint8_t>();
       o.values2 = nullptr;
@@ -2915,7 +2975,6 @@ static std::unique_ptr<kg_batch> result_batch(kg_hash_agg *a, const uint64_t *ke
         c2.type = o.st2 == 1 ? KG_T_BOOL : KG_T_INT64;
         c2.values.reset(a->ctx, (uint64_t)n * 8 + 16);
         c2.validity.reset(a->ctx, vb);
-        KG_HIP(hipMemsetAsync(c2.validity.get(), 0, vb, a->s));
         o.values2 = c2.values.get();
         o.validity2 = c2.validity.get<uint8_t>();
         if (o.st2 == 1) bool_cols.push_back(at);
@@ -2944,8 +3003,13 @@ static std::unique_ptr<kg_batch> result_batch(kg_hash_agg *a, const uint64_t *ke
       KG_HIP(hipGetLastError());
     }
     unsigned int e = 0;
-    KG_HIP(hipMemcpyAsync(&e, err.get(), 4, hipMemcpyDeviceToHost, a->s));
-    KG_HIP(hipStreamSynchronize(a->s));
+    {
+      PinnedReads pin;
+      unsigned int *p = pin.take<unsigned int>(1);
+      KG_HIP(hipMemcpyAsync(p, err.get(), 4, hipMemcpyDeviceToHost, a->s));
+      KG_HIP(hipStreamSynchronize(a->s));
+      e = *p;
+    }
     if (e & 1u) fail(KG_ERR_OVERFLOW, "Overflow in sum (use try_sum/try_avg to tolerate overflow)");
     if (e & 2u) fail(KG_ERR_OUT_OF_RANGE, "average out of the result decimal's range");
     // booleans (a boolean key, partial `nonempty` states) leave the kernel as
@@ -3152,14 +3216,12 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
       // spill partition histograms (and the sketch's registers) and cursors:
       // 4 bytes a partition, up to 2^15
       for (const void *f : {(const void *)k_spill_hist, (const void *)k_spill_scatter})
-        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                128 * 1024 + 4 * kHllRegs) != hipSuccess) {
+        if (set_dynamic_lds(f, 128 * 1024 + 4 * kHllRegs) != hipSuccess) {
           (void)hipGetLastError();
           fail(KG_ERR_DEVICE, "dynamic LDS of 132 KiB refused");
         }
       for (auto &f : fns) {
-        hipError_t e = hipFuncSetAttribute(f.first, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)f.second);
+        hipError_t e = set_dynamic_lds(f.first, (int)f.second);
         if (e != hipSuccess) {
           (void)hipGetLastError();
           fail(KG_ERR_DEVICE, "dynamic LDS of " + std::to_string(f.second) + " bytes refused");

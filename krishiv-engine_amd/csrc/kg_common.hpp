@@ -250,6 +250,32 @@ class Staged {
 
 void check_column(const kg_column &c, int64_t nrows, const char *what);
 
+// hipFuncSetAttribute(MaxDynamicSharedMemorySize) for the current device,
+// made only when `bytes` passes what was set before for this kernel.
+hipError_t set_dynamic_lds(const void *fn, size_t bytes);
+
+// Small device-to-host readbacks through this thread's pinned buffer.  A
+// copy into pageable memory waits for itself (the runtime stages it), a copy
+// into pinned memory queues on the stream: a call that reads several values
+// back waits once.  Space taken is returned when the PinnedReads goes out of
+// scope, so the copies into it must have finished by then (the caller
+// synchronizes its stream before it reads the values).
+class PinnedReads {
+ public:
+  PinnedReads();
+  ~PinnedReads();
+  PinnedReads(const PinnedReads &) = delete;
+  PinnedReads &operator=(const PinnedReads &) = delete;
+  template <class T>
+  T *take(size_t n) {
+    return static_cast<T *>(take_bytes(n * sizeof(T)));
+  }
+
+ private:
+  void *take_bytes(size_t bytes);
+  size_t mark_;
+};
+
 // ---------------------------------------------------------------- profiling
 // Records a HIP event pair around a launch when profiling is on.
 class ProfScope {

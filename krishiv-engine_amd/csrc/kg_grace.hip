@@ -631,8 +631,7 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
       {
         ProfScope prof("hash_partition", g->s);
         const void *hf = w64 ? (const void *)k_gp_hist<true> : (const void *)k_gp_hist<false>;
-        KG_HIP(hipFuncSetAttribute(hf, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(parts * 16)));
+        KG_HIP(set_dynamic_lds(hf, (int)(parts * 16)));
         if (w64)
           k_gp_hist<true><<<(unsigned)nb, kGPBlock, parts * 16, g->s>>>(
               dk, dv, m, parts, per_block, hist.get<uint32_t>(), g->mm.get<long long>() + 2 * side);
@@ -654,8 +653,7 @@ static void sink_rows(kg_grace *g, std::vector<kg_grace::Piece> &store, const kg
       const uint64_t pmask = g->mod31 ? 0x7FFFFFFFull : ~0ull;
 #define KG_GPS(RR, WW)                                                                        \
   do {                                                                                       \
-    KG_HIP(hipFuncSetAttribute((const void *)k_gp_scatter<RR, WW>,                           \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
+    KG_HIP(set_dynamic_lds((const void *)k_gp_scatter<RR, WW>, lds));                        \
     k_gp_scatter<RR, WW><<<(unsigned)nb, kGSBlock, lds, g->s>>>(                             \
         dk, dv, m, parts, per_block, base.get<uint64_t>(), piece.key.get<int64_t>(),         \
         piece.pay.get<int64_t>(), pmask);                                                    \
@@ -798,8 +796,7 @@ int kg_grace_finish(kg_grace *g, int64_t *counts, int64_t *sums, uint8_t *sum_va
       {
         ProfScope prof("grace_build", g->s);
         const size_t lds = (size_t)kGBSlots * 3 * 8;
-        KG_HIP(hipFuncSetAttribute((const void *)k_grace_build,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        KG_HIP(set_dynamic_lds((const void *)k_grace_build, (int)lds));
         const int64_t tiles = ceil_div(nbp[p], kGBTile);
         const int64_t cus = g->ctx->num_cus > 0 ? g->ctx->num_cus : 256;
         k_grace_build<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, cus)), kGBBlock,

@@ -1946,8 +1946,7 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
   const int Wn = (int)std::min<uint64_t>(C + kJMargin, j->cap);
   const size_t lds = (size_t)Wn * per_slot;
   if (lds > lds_cap) fail(KG_ERR_UNSUPPORTED, "aggregate state too wide for the join window");
-  KG_HIP(hipFuncSetAttribute((const void *)k_jp_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds));
+  KG_HIP(set_dynamic_lds((const void *)k_jp_merge, (int)lds));
   // staged scatter: tiles of kJSBlock * R rows sorted in LDS (R tuned, or
   // by the row width; a tuned R whose tile does not fit falls back)
   // (a FAST tile keeps at most js_nv(R) values a row in registers)
@@ -1994,7 +1993,7 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
       const void *fn;
 #define KG_JS(RR, FF)                                                                          \
   fn = (const void *)k_jp_scatter_staged<RR, FF>;                                              \
-  KG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));     \
+  KG_HIP(set_dynamic_lds(fn, slds));                                                           \
   k_jp_scatter_staged<RR, FF><<<g, kJSBlock, slds, j->s>>>(K, nrows, T, P, F, log2p, per_block, \
                                                            bp, rp, rec)
       if (fast) {
@@ -2060,7 +2059,7 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
                          !is_float(K.c[0].type);
       const void *dfn = dfast ? (const void *)k_joinagg_direct<true>
                               : (const void *)k_joinagg_direct<false>;
-      KG_HIP(hipFuncSetAttribute(dfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dlds));
+      KG_HIP(set_dynamic_lds(dfn, (int)dlds));
       const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
       const int64_t per_cu = dlds <= 40 * 1024 ? 4 : dlds <= 80 * 1024 ? 2 : 1;
       const int64_t grid = std::min<int64_t>(cus * per_cu,

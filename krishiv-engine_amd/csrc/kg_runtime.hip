@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <string>
 
 #include "kg_common.hpp"
@@ -297,6 +298,44 @@ void check_column(const kg_column &c, int64_t nrows, const char *what) {
   if (c.length < nrows) fail(KG_ERR_INTERNAL, std::string(what) + ": column shorter than the batch");
   if (c.offset < 0) fail(KG_ERR_INTERNAL, std::string(what) + ": negative offset");
   if (nrows > 0 && !c.values) fail(KG_ERR_INTERNAL, std::string(what) + ": no values buffer");
+}
+
+namespace {
+std::mutex g_lds_mu;
+std::map<std::pair<int, const void *>, size_t> g_lds_set;
+}  // namespace
+
+hipError_t set_dynamic_lds(const void *fn, size_t bytes) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(g_lds_mu);
+  auto it = g_lds_set.find({dev, fn});
+  if (it != g_lds_set.end() && it->second >= bytes) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) g_lds_set[{dev, fn}] = bytes;
+  return e;
+}
+
+// Each calling thread keeps one 256 KB pinned buffer for the life of the
+// process (operators are driven by a few long-lived threads).
+namespace {
+constexpr size_t kPinnedReadBytes = 256u << 10;
+thread_local uint8_t *t_pinned = nullptr;
+thread_local size_t t_pinned_used = 0;
+}  // namespace
+
+PinnedReads::PinnedReads() : mark_(t_pinned_used) {
+  if (!t_pinned) KG_HIP(hipHostMalloc((void **)&t_pinned, kPinnedReadBytes, hipHostMallocDefault));
+}
+
+PinnedReads::~PinnedReads() { t_pinned_used = mark_; }
+
+void *PinnedReads::take_bytes(size_t bytes) {
+  const size_t at = (t_pinned_used + 63) & ~(size_t)63;
+  if (at + bytes > kPinnedReadBytes) fail(KG_ERR_INTERNAL, "pinned readback space exhausted");
+  t_pinned_used = at + bytes;
+  return t_pinned + at;
 }
 
 DCol Staged::add(const kg_column &c, int64_t nrows) {
