@@ -1369,8 +1369,10 @@ static void build_direct(kg_hash_join *j) {
   const int w = type_width(t);
   if (w <= 0 || w > 8 || is_float(t)) return;  // integers, dates, timestamps, decimal64
   DevBuf mm(j->ctx, 16);
-  const long long init[2] = {LLONG_MAX, LLONG_MIN};
-  long long got[2];
+  PinnedReads pin;
+  long long *init = pin.take<long long>(2), *got = pin.take<long long>(2);
+  init[0] = LLONG_MAX;
+  init[1] = LLONG_MIN;
   KG_HIP(hipMemcpyAsync(mm.get(), init, 16, hipMemcpyHostToDevice, j->s));
   k_slot_minmax<<<stream_grid(j->cap + 1, 256), 256, 0, j->s>>>(jtable(j), mm.get<long long>());
   KG_HIP(hipGetLastError());
@@ -1473,7 +1475,9 @@ int kg_join_build_finalize(kg_hash_join *j) {
       KG_HIP(hipGetLastError());
     }
     if (j->bmatch.bytes()) KG_HIP(hipMemsetAsync(j->bmatch.get(), 0, j->bmatch.bytes(), j->s));
-    KG_HIP(hipMemcpyAsync(&j->null_rows, nulls.get(), 8, hipMemcpyDeviceToHost, j->s));
+    PinnedReads pin;
+    uint64_t *null_rows = pin.take<uint64_t>(1);
+    KG_HIP(hipMemcpyAsync(null_rows, nulls.get(), 8, hipMemcpyDeviceToHost, j->s));
     // the direct index is an accelerator: without the memory for it the
     // hashed table serves the probes
     try {
@@ -1483,6 +1487,7 @@ int kg_join_build_finalize(kg_hash_join *j) {
       j->dspan = 0;
     }
     KG_HIP(hipStreamSynchronize(j->s));
+    j->null_rows = *null_rows;
     j->finalized = true;
   });
 }
@@ -2101,9 +2106,10 @@ int kg_joinagg_result(kg_hash_join *j, kg_batch **out) {
     int kw = type_width(kb.type);
     if (kw <= 0) fail(KG_ERR_UNSUPPORTED, "boolean join key in the fused aggregate");
     uint64_t vb = (uint64_t)ceil_div(n, 32) * 4 + 4;
+    // validity words are not cleared: the output kernel writes every word
+    // that holds a bit below n
     kc.values.reset(j->ctx, (uint64_t)n * kw + 8);
     kc.validity.reset(j->ctx, vb);
-    KG_HIP(hipMemsetAsync(kc.validity.get(), 0, vb, j->s));
     OutProg O = j->out;
     O.cap = j->cap;
     O.key_type = kb.type;
@@ -2117,7 +2123,6 @@ int kg_joinagg_result(kg_hash_join *j, kg_batch **out) {
       int w = type_width(c.type);
       c.values.reset(j->ctx, (uint64_t)n * (w > 0 ? w : 1) + 16);
       c.validity.reset(j->ctx, vb);
-      KG_HIP(hipMemsetAsync(c.validity.get(), 0, vb, j->s));
       O.a[a].values = c.values.get();
       O.a[a].validity = c.validity.get<uint8_t>();
     }
@@ -2126,11 +2131,12 @@ int kg_joinagg_result(kg_hash_join *j, kg_batch **out) {
     launch_agg_output(O, &j->slots.get<Slot>()->key, (int)(sizeof(Slot) / 8),
                       j->rec.get<uint64_t>(), pos.get<uint64_t>(), n, kc.values.get(),
                       kc.validity.get<uint8_t>(), kw, err.get<unsigned int>(), j->s);
-    unsigned int e = 0;
-    KG_HIP(hipMemcpyAsync(&e, err.get(), 4, hipMemcpyDeviceToHost, j->s));
+    PinnedReads pin;
+    unsigned int *e = pin.take<unsigned int>(1);
+    KG_HIP(hipMemcpyAsync(e, err.get(), 4, hipMemcpyDeviceToHost, j->s));
     KG_HIP(hipStreamSynchronize(j->s));
-    if (e & 1u) fail(KG_ERR_OVERFLOW, "Overflow in sum");
-    if (e & 2u) fail(KG_ERR_OUT_OF_RANGE, "average out of the result decimal's range");
+    if (*e & 1u) fail(KG_ERR_OVERFLOW, "Overflow in sum");
+    if (*e & 2u) fail(KG_ERR_OUT_OF_RANGE, "average out of the result decimal's range");
     *out = b.release();
   });
 }
