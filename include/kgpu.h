@@ -693,7 +693,13 @@ void kg_grace_join_destroy(kg_grace_join *g);
  * columns) over an inner HashJoin (the shape of SELECT key, SUM(amt),
  * COUNT(*) FROM build JOIN probe ON key = fk GROUP BY key): the aggregate
  * state lives in the build table's slots, so no join pair is materialized.
- * Equal to kg_join_probe followed by kg_agg_sink, without the pairs. */
+ * Equal to kg_join_probe followed by kg_agg_sink, without the pairs.
+ * Decimal SUM/AVG and ANSI integer SUM (checked sums) are taken while each
+ * probe batch is proven unable to overflow them (rows x the most build rows
+ * of a key x the largest magnitude, or the batch's sum of magnitudes, within
+ * the bound); a batch that cannot be proven is refused with
+ * KG_ERR_UNSUPPORTED before anything is applied.  Decimal128 inputs are
+ * refused. */
 int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_types,
                       int32_t naggs, const kg_agg_spec *aggs, uint32_t flags);
 int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,

@@ -2249,36 +2249,40 @@ static void drain(kg_hash_agg *a) {
   a->bound_rows = a->bound_recs = 0;
 }
 
-// Whether no running sum of the checked op can leave its bound during this
-// batch, in any row order (the per-batch proof of accumulator/helpers.rs:40-75).
-static bool proven(kg_hash_agg *a, const CheckedSum &c, const DCol &col, int64_t n,
-                   double &batch_abs) {
-  batch_abs = 0.0;
-  // 1: rows so far x the type's largest magnitude
-  unsigned __int128 rows = (unsigned __int128)(a->rows_seen + (uint64_t)n);
-  unsigned __int128 tb = c.type_bound;
+bool checked_batch_proven(kg_ctx *ctx, hipStream_t s, DevBuf &scratch, const CheckedSum &c,
+                          const DCol &col, int64_t n, uint64_t rows_before, uint64_t mult,
+                          double &batch_abs) {
+  // 1: every row so far at the type's largest magnitude, counted mult times
+  const unsigned __int128 rows = (unsigned __int128)(rows_before + (uint64_t)n) * mult;
+  const unsigned __int128 tb = c.type_bound;
   if (tb == 0 || rows <= (((unsigned __int128)0) - 1) / tb) {
     if (rows * tb < c.bound) {
+      batch_abs = (double)n * (double)mult * (double)tb;
       return true;
     }
   }
   // 2: the stored sums' bound plus this batch's exact sum of magnitudes
-  a->scratch_abs.ensure(a->ctx, 8);
-  KG_HIP(hipMemsetAsync(a->scratch_abs.get(), 0, 8, a->s));
-  k_abs_sum<<<stream_grid(n, 256), 256, 0, a->s>>>(col, n, a->scratch_abs.get<double>());
+  scratch.ensure(ctx, 8);
+  KG_HIP(hipMemsetAsync(scratch.get(), 0, 8, s));
+  k_abs_sum<<<stream_grid(n, 256), 256, 0, s>>>(col, n, scratch.get<double>());
   KG_HIP(hipGetLastError());
   double batch = 0.0;
   {
     PinnedReads pin;
     double *p = pin.take<double>(1);
-    KG_HIP(hipMemcpyAsync(p, a->scratch_abs.get(), 8, hipMemcpyDeviceToHost, a->s));
-    KG_HIP(hipStreamSynchronize(a->s));
-    batch = *p;
+    KG_HIP(hipMemcpyAsync(p, scratch.get(), 8, hipMemcpyDeviceToHost, s));
+    KG_HIP(hipStreamSynchronize(s));
+    batch = *p * (double)mult;
   }
-  double total = (c.cum_abs + batch) * (1.0 + 1e-9) + 1.0;
-  double bound = (double)c.bound;
+  const double total = (c.cum_abs + batch) * (1.0 + 1e-9) + 1.0;
   batch_abs = batch;
-  return total < bound * (1.0 - 1e-9);
+  return total < (double)c.bound * (1.0 - 1e-9);
+}
+
+static bool proven(kg_hash_agg *a, const CheckedSum &c, const DCol &col, int64_t n,
+                   double &batch_abs) {
+  return checked_batch_proven(a->ctx, a->s, a->scratch_abs, c, col, n, a->rows_seen, 1,
+                              batch_abs);
 }
 
 // The worst-case HBM a sink call of nrows takes beyond what is held (host

@@ -689,6 +689,16 @@ struct CheckedSum {
   double cum_abs = 0.0;          // upper bound of every stored |sum|
 };
 
+// Whether no running sum of the checked op can leave its bound during a
+// batch of n rows of col, each counted up to `mult` times, in any row order
+// (the per-batch proof of accumulator/helpers.rs:40-75): rows_before + n
+// rows at the type's largest magnitude, or the stored sums' bound (c.cum_abs)
+// plus the batch's exact sum of magnitudes.  batch_abs: what the batch adds
+// to c.cum_abs.  scratch: 8 bytes of device memory (grown if smaller).
+bool checked_batch_proven(kg_ctx *ctx, hipStream_t s, DevBuf &scratch, const CheckedSum &c,
+                          const DCol &col, int64_t n, uint64_t rows_before, uint64_t mult,
+                          double &batch_abs);
+
 // Builds the update program (prog) and the final-value program (out) of an
 // aggregate list over input columns of the given types.
 // merge = the inputs are the partial-state columns of a first phase (in

@@ -486,6 +486,19 @@ __global__ void k_join_gather(GatherCols G, int64_t nout, const int32_t *pidx,
   }
 }
 
+// The most build rows one key holds (slot counts), into *out.
+__global__ void k_slot_max_count(const Slot *slots, uint64_t n, unsigned int *out) {
+  unsigned int m = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    m = slots[i].count > m ? slots[i].count : m;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned int o = __shfl_xor(m, off);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
 // The join fused with GROUP BY the build key: each probe row's aggregate
 // ops go to its build key's record, once per matching build row.
 __global__ void k_joinagg_probe(KeySpec K, int64_t n, JTable T, AggProg P, uint64_t *rec) {
@@ -1136,6 +1149,12 @@ struct kg_hash_join {
   OutProg out{};
   std::vector<kg_column> agg_types;
   DevBuf rec;
+  // checked sums (decimal, ANSI integer) of the fused aggregate: each probe
+  // batch is proven unable to overflow them before it is applied
+  std::vector<CheckedSum> checked;
+  uint64_t agg_rows = 0;  // probe rows applied so far
+  uint64_t max_dup = 1;   // the most build rows of one key (a probe row's pairs)
+  DevBuf abs_scratch;
   // scan scratch of the fused aggregate and the tail
   DevBuf temp;
   // residual filter (column indices of the probe and build batches)
@@ -1883,11 +1902,24 @@ int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_t
     std::vector<CheckedSum> checked;
     compile_aggs(j->agg_types, naggs, aggs, (flags & KG_AGG_ANSI) != 0, false, false, j->prog,
                  j->out, checked);
-    if (!checked.empty())
-      fail(KG_ERR_UNSUPPORTED, "checked sums (decimal, ANSI) in the fused join aggregate");
     for (int o = 0; o < j->prog.nops; o++)
       if (j->prog.ops[o].kind == OP_ADD192)
         fail(KG_ERR_UNSUPPORTED, "decimal128 sums in the fused join aggregate");
+    j->checked = checked;
+    j->agg_rows = 0;
+    j->max_dup = 1;
+    if (!checked.empty()) {  // a probe row counts once per build row of its key
+      DevBuf mx(j->ctx, 8);
+      KG_HIP(hipMemsetAsync(mx.get(), 0, 8, j->s));
+      k_slot_max_count<<<stream_grid(j->cap + 2, 256), 256, 0, j->s>>>(
+          j->slots.get<Slot>(), j->cap + 2, mx.get<unsigned int>());
+      KG_HIP(hipGetLastError());
+      PinnedReads pin;
+      unsigned int *m = pin.take<unsigned int>(1);
+      KG_HIP(hipMemcpyAsync(m, mx.get(), 4, hipMemcpyDeviceToHost, j->s));
+      KG_HIP(hipStreamSynchronize(j->s));
+      j->max_dup = *m > 1 ? *m : 1;
+    }
     j->rec.reset(j->ctx, (j->cap + 2) * (uint64_t)j->prog.W * 8);
     init_records(j->ctx, j->rec.get<uint64_t>(), j->cap + 2, j->prog, j->s);
     j->has_agg = true;
@@ -2049,6 +2081,15 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
       P.in[c] = d[ic];
     }
     if (nrows >= (1ll << 32)) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^32 rows or more");
+    // checked sums: proven for this batch before anything is applied (a
+    // batch that cannot be proven is refused, the state unchanged: the CPU
+    // path replays such sums in row order)
+    std::vector<double> add(j->checked.size(), 0.0);
+    for (size_t k = 0; k < j->checked.size(); k++)
+      if (!checked_batch_proven(j->ctx, j->s, j->abs_scratch, j->checked[k],
+                                P.in[j->checked[k].col], nrows, j->agg_rows, j->max_dup, add[k]))
+        fail(KG_ERR_UNSUPPORTED,
+             "a decimal or ANSI sum of the fused join aggregate that this batch could overflow");
     const int WL = P.W | 1;
     const size_t dlds = (size_t)j->dspan * (WL * 8 + 4);
     int optin = 0;
@@ -2083,6 +2124,8 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
       KG_HIP(hipGetLastError());
     }
     if (st.staged_host()) KG_HIP(hipStreamSynchronize(j->s));
+    j->agg_rows += (uint64_t)nrows;
+    for (size_t k = 0; k < j->checked.size(); k++) j->checked[k].cum_abs += add[k];
   });
 }
 

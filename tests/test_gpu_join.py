@@ -279,6 +279,79 @@ def test_fused_aggregate_with_duplicate_build_keys():
         assert abs(float(r.column(5)[0][i]) - float(rres[4][0][ri])) <= 1e-12 * abs(float(rres[4][0][ri]))
 
 
+# probe rows: the per-row probe (global atomics), the direct index (dense
+# build keys, LDS records) and the radix-partitioned path
+@pytest.mark.parametrize("path,nb_keys,npr", [("probe", 500, 20_000), ("direct", 500, 200_000),
+                                              ("partitioned", 300_000, 1_200_000)])
+def test_fused_aggregate_decimal_and_ansi_sums(path, nb_keys, npr):
+    """Decimal SUM/AVG and the ANSI integer SUM (checked sums) in the fused
+    aggregate, with duplicate build keys (each probe row counts once per
+    pair) and NULL inputs: every batch is proven unable to overflow, so the
+    sums are exact; against the oracle's join + ANSI aggregate."""
+    say(f"join: fused decimal/ANSI sums, {path} path")
+    rng = np.random.default_rng(nb_keys + npr)
+    nb = nb_keys * 3
+    bk = rng.integers(0, nb_keys, nb).astype(np.int64)
+    if path == "probe":
+        bk = bk * 7919 + 12345  # sparse keys: no direct index
+    pk = bk[rng.integers(0, nb, npr)]
+    pk[rng.random(npr) < 0.2] += 1
+    dec = rng.integers(-(10 ** 12) + 1, 10 ** 12, npr)
+    decv = rng.random(npr) > 0.05
+    # (|value| x rows x pairs a row stays below 2^63: the ANSI sum is proven)
+    i64 = rng.integers(-(1 << 30), 1 << 30, npr)
+    i64v = rng.random(npr) > 0.05
+    op, ob = kgo.join(kgo.JOIN_INNER, [kgo.Col(kgo.T_INT64, pk)], [kgo.Col(kgo.T_INT64, bk)])
+    aggs = [(kgo.AGG_SUM, 0), (kgo.AGG_AVG, 0), (kgo.AGG_SUM, 1), (kgo.AGG_COUNT_STAR, 0),
+            (kgo.AGG_MAX, 0)]
+    types = [(kgo.T_DECIMAL64, 12, 2), (kgo.T_INT64, 0, 0)]
+    o = kgo.Agg((kgo.T_INT64, 0, 0), types, aggs, ansi=True)
+    o.update(kgo.Col(kgo.T_INT64, bk[ob]),
+             [kgo.Col(kgo.T_DECIMAL64, dec[op], decv[op], precision=12, scale=2),
+              kgo.Col(kgo.T_INT64, i64[op], i64v[op])])
+    rk, _, rres = o.finish()
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0], expected_build_rows=nb)
+    j.build_sink([Col(A.T_INT64, bk)])
+    j.build_finalize()
+    j.aggregate([(A.T_DECIMAL64, 12, 2), (A.T_INT64, 0, 0)], aggs, ansi=True)
+    half = npr // 2 if path != "partitioned" else npr
+    for a, b in ((0, half), (half, npr)):
+        if b > a:
+            j.probe_aggregate([Col(A.T_INT64, pk[a:b]),
+                               Col(A.T_DECIMAL64, dec[a:b], decv[a:b], 12, 2),
+                               Col(A.T_INT64, i64[a:b], i64v[a:b])], [1, 2])
+    r = j.aggregate_result()
+    assert r.num_rows == len(rk)
+    gk = r.column(0)[0]
+    gi = {int(k): i for i, k in enumerate(gk)}
+    cols = [r.column(1 + c) for c in range(len(aggs))]
+    for ri, k in enumerate(rk):
+        i = gi[int(k)]
+        for c in range(len(aggs)):
+            rv, rok, _ = rres[c]
+            gv, gok = cols[c]
+            assert bool(rok[ri]) == bool(gok[i]), (k, c)
+            if rok[ri]:
+                assert int(rv[ri]) == int(gv[i]), (k, c, rv[ri], gv[i])
+
+
+def test_fused_ansi_sum_that_could_overflow_is_refused():
+    """An ANSI SUM whose batch could pass 2^63 (values near 2^61, four
+    build rows a key) is refused before anything is applied; a batch that
+    is proven safe is then taken as usual."""
+    j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0])
+    j.build_sink([Col(A.T_INT64, np.repeat(np.arange(10, dtype=np.int64), 4))])
+    j.build_finalize()
+    j.aggregate([(A.T_INT64, 0, 0)], [(A.AGG_SUM, 0), (A.AGG_COUNT_STAR, 0)], ansi=True)
+    pk = np.arange(10, dtype=np.int64)
+    with pytest.raises(kgpu.KgError, match="could overflow"):
+        j.probe_aggregate([Col(A.T_INT64, pk), Col(A.T_INT64, np.full(10, 1 << 61))], [1])
+    j.probe_aggregate([Col(A.T_INT64, pk), Col(A.T_INT64, np.arange(10) * 3)], [1])
+    r = j.aggregate_result()
+    got = dict(zip(r.column(0)[0].tolist(), zip(r.column(1)[0].tolist(), r.column(2)[0].tolist())))
+    assert got == {k: (4 * 3 * k, 4) for k in range(10)}
+
+
 def test_empty_probe_and_no_match():
     j = kgpu.HashJoin(ctx(), A.JOIN_INNER, [0], [0])
     j.build_sink([Col(A.T_INT64, np.array([1, 2, 3]))])
