@@ -273,10 +273,11 @@ typedef struct kg_hash_agg kg_hash_agg;
 int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
                   const kg_column *input_types, int32_t naggs, const kg_agg_spec *aggs,
                   uint32_t flags, uint64_t expected_groups, kg_hash_agg **out);
-/* Sink::sink of raw rows.  Host batches of fewer rows than a staging slot
- * (512K rows at up to 64 B a row) are copied to pinned staging and gathered
- * in HBM (up to 4M rows or 256 MB), and aggregated when the next batch does
- * not fit, a device batch arrives, or at finalize: an error of that work
+/* Sink::sink of raw rows.  Host batches of fewer than 512K rows (at up to
+ * 64 B a row) are copied to pinned staging and gathered in HBM (64 times the
+ * first batch's rows, growing to 4M rows or 256 MB), and aggregated when the
+ * next batch does not fit, a device batch arrives, or at finalize: an error
+ * of that work
  * (NeedMemory, overflow) is returned by that later call.  NeedMemory leaves
  * the gathered rows in place and the call's own batch not taken, so the
  * same batch retried is counted once.  Without a bypass policy only. */

@@ -2517,19 +2517,40 @@ static bool gathers(const kg_hash_agg *a, const kg_column *key, const kg_column 
   if (a->has_key && (key->flags & KG_COL_DEVICE)) return false;
   for (int c = 0; c < a->nsink; c++)
     if (inputs[c].flags & KG_COL_DEVICE) return false;
-  if (a->gather) {  // the staging is laid out for the first batch's widths
-    const HostGather &g = *a->gather;
-    const int k0 = a->has_key ? 1 : 0;
-    if (g.ncols != k0 + a->nsink) return false;
-    for (int c = 0; c < g.ncols; c++)
-      if (type_width(c < k0 ? key->type : inputs[c - k0].type) != g.width[c]) return false;
-    return nrows < g.slot_rows;
-  }
-  return nrows < gather_cap(a, key, inputs) / 8;
+  return nrows < gather_cap(a, key, inputs) / 8;  // a slot of the largest gather
 }
 
-// A host batch into the gather: checked as sink checks it, flushed ahead
-// when it does not fit, then copied to pinned staging.
+// A gather of cap rows (a power of two) for columns of these widths: its HBM
+// buffers and two pinned staging slots of cap / 8 rows.
+static std::unique_ptr<HostGather> make_gather(kg_hash_agg *a, int ncols, const int *width,
+                                               int64_t cap) {
+  auto g = std::make_unique<HostGather>();
+  g->ncols = ncols;
+  g->cap = cap;
+  g->slot_rows = cap / 8;
+  size_t off = 0;
+  for (int c = 0; c < ncols; c++) {
+    g->width[c] = width[c];
+    const uint64_t w = (uint64_t)width[c];
+    g->vals[c].reset(a->ctx, w ? cap * w : cap / 8);
+    g->valid[c].reset(a->ctx, cap / 8);
+    g->voff[c] = off;
+    off += ((w ? g->slot_rows * w : g->slot_rows / 8) + 255) & ~(size_t)255;
+    g->boff[c] = off;
+    off += ((size_t)g->slot_rows / 8 + 255) & ~(size_t)255;
+  }
+  for (auto &sl : g->slot) {
+    KG_HIP(hipHostMalloc((void **)&sl.host, off, hipHostMallocDefault));
+    KG_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  }
+  return g;
+}
+
+// A host batch into the gather: checked as sink checks it, the gathered rows
+// aggregated first when it does not fit, then copied to pinned staging.  A
+// gather starts at 64 times the first batch's rows (64K to the largest), and
+// is replaced by one four times larger when it fills, or by one sized for a
+// batch that does not fit its slots, up to the largest for the row width.
 static void gather_batch(kg_hash_agg *a, const kg_column *key, const kg_column *inputs,
                          int64_t nrows) {
   if (a->finalized) fail(KG_ERR_INTERNAL, "sink after finalize");
@@ -2545,36 +2566,29 @@ static void gather_batch(kg_hash_agg *a, const kg_column *key, const kg_column *
     if (!a->merge && inputs[c].type != a->in_types[c].type)
       fail(KG_ERR_INTERNAL, "input column type changed");
   }
-  if (a->gather && a->gather->rows > 0) {
+  int width[1 + kAggInputs];
+  for (int c = 0; c < ncols; c++) width[c] = type_width(col(c).type);
+  const int64_t maxcap = gather_cap(a, key, inputs);
+  int64_t want = 65536;
+  while (want < nrows * 64 && want < maxcap) want <<= 1;
+  want = std::min(want, maxcap);
+  int64_t cap = want;
+  if (a->gather) {
     HostGather &g = *a->gather;
-    bool same = true;
-    for (int c = 0; c < ncols; c++)
-      same = same && col(c).type == g.type[c].type && col(c).precision == g.type[c].precision &&
-             col(c).scale == g.type[c].scale;
-    if (!same || g.rows + nrows > g.cap) flush_gather(a);
-  }
-  if (!a->gather) {
-    auto g = std::make_unique<HostGather>();
-    g->ncols = ncols;
-    g->cap = gather_cap(a, key, inputs);
-    g->slot_rows = g->cap / 8;
-    size_t off = 0;
-    for (int c = 0; c < ncols; c++) {
-      g->width[c] = type_width(col(c).type);
-      const uint64_t w = (uint64_t)g->width[c];
-      g->vals[c].reset(a->ctx, w ? g->cap * w : g->cap / 8);
-      g->valid[c].reset(a->ctx, g->cap / 8);
-      g->voff[c] = off;
-      off += ((w ? g->slot_rows * w : g->slot_rows / 8) + 255) & ~(size_t)255;
-      g->boff[c] = off;
-      off += ((size_t)g->slot_rows / 8 + 255) & ~(size_t)255;
+    bool same = g.ncols == ncols, widths = g.ncols == ncols;
+    for (int c = 0; c < ncols && same; c++) {
+      widths = widths && width[c] == g.width[c];
+      same = widths && col(c).type == g.type[c].type &&
+             col(c).precision == g.type[c].precision && col(c).scale == g.type[c].scale;
     }
-    for (auto &sl : g->slot) {
-      KG_HIP(hipHostMalloc((void **)&sl.host, off, hipHostMallocDefault));
-      KG_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-    }
-    a->gather = std::move(g);
+    const bool fits = g.rows + nrows <= g.cap, in_slot = nrows < g.slot_rows;
+    if (g.rows > 0 && (!same || !fits || !in_slot)) flush_gather(a);
+    cap = g.cap;
+    if (!fits && cap < maxcap) cap = std::min(cap * 4, maxcap);
+    if (!in_slot || !widths) cap = std::max(cap, want);
+    if (cap != g.cap || !widths) a->gather.reset();  // empty by now
   }
+  if (!a->gather) a->gather = make_gather(a, ncols, width, cap);
   HostGather &g = *a->gather;
   if (g.rows == 0)
     for (int c = 0; c < ncols; c++) {
