@@ -575,6 +575,10 @@ typedef struct kg_join_spec {
 typedef struct kg_hash_join kg_hash_join;
 int kg_join_create(kg_ctx *ctx, const kg_join_spec *spec, uint64_t expected_build_rows,
                    kg_hash_join **out);
+/* BuildSink::sink.  Host batches of fewer than 512K rows are gathered as
+ * kg_agg_sink gathers them (taken at finalize or when the next batch does
+ * not fit; NeedMemory then leaves them gathered and the new batch untaken);
+ * kg_join_build_rows counts them. */
 int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows);
 int kg_join_build_finalize(kg_hash_join *j);
 /* Rows sunk into the build side (all of them; those with a NULL `=` key
@@ -703,6 +707,10 @@ void kg_grace_join_destroy(kg_grace_join *g);
  * refused. */
 int kg_joinagg_create(kg_hash_join *j, int32_t ninputs, const kg_column *input_types,
                       int32_t naggs, const kg_agg_spec *aggs, uint32_t flags);
+/* Host probe batches of fewer than 512K rows are gathered and probed when
+ * the next batch does not fit, the input columns change, a device batch
+ * arrives, or at kg_joinagg_result, which then returns an error of that
+ * work (a checked sum's refusal leaves the gathered rows in place). */
 int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                      const int32_t *input_cols);
 int kg_joinagg_result(kg_hash_join *j, kg_batch **out);

@@ -34,6 +34,7 @@
 #include "kg_aggops.hpp"
 #include "kg_scan.hpp"
 #include "kg_aggtile.hpp"
+#include "kg_gather.hpp"
 
 namespace kg {
 // ---- few groups: a one-byte key or a global aggregate ----------------------
@@ -1418,46 +1419,6 @@ static unsigned __int128 pow10_u128(int p) {
 
 using namespace kg;
 
-namespace kg {
-
-// Host batches smaller than a staging slot are gathered before they are
-// aggregated: the CPU copies each into pinned staging (so the caller's
-// buffers are free when kg_agg_sink returns), a full staging slot goes to an
-// HBM buffer by DMA on the operator's stream, and the aggregation runs over
-// the HBM buffer when the next batch does not fit (or at finalize).  A small
-// batch alone costs a synchronous copy and a pass 1 over a few workgroups.
-// The HBM buffer holds up to 4M rows and 256 MB, a staging slot an eighth of
-// its rows (two slots alternate).
-constexpr int64_t kGatherRows = 1 << 22;
-constexpr uint64_t kGatherBytes = 256ull << 20;
-
-struct HostGather {
-  int64_t cap = 0, slot_rows = 0;  // rows of the HBM buffer, of a staging slot
-  int ncols = 0;  // the grouping column (when there is one), then the sink inputs
-  kg_column type[1 + kAggInputs]{};
-  int width[1 + kAggInputs]{};        // bytes a value; 0: bit-packed BOOL
-  size_t voff[1 + kAggInputs]{};      // a column's values within a staging slot
-  size_t boff[1 + kAggInputs]{};      // its validity bits within a staging slot
-  bool any_valid[1 + kAggInputs]{};   // some gathered batch had a validity bitmap
-  DevBuf vals[1 + kAggInputs], valid[1 + kAggInputs];  // cap rows each
-  int64_t rows = 0;                   // rows gathered (the open slot's included)
-  struct Slot {
-    uint8_t *host = nullptr;          // pinned
-    hipEvent_t done = nullptr;        // its last DMA
-    int64_t base = 0, rows = 0;       // first gathered row, rows staged
-  } slot[2];
-  int cur = 0;
-  ~HostGather() {
-    for (Slot &sl : slot) {
-      if (sl.done) (void)hipEventSynchronize(sl.done);
-      if (sl.done) (void)hipEventDestroy(sl.done);
-      if (sl.host) (void)hipHostFree(sl.host);
-    }
-  }
-};
-
-}  // namespace kg
-
 struct kg_hash_agg {
   kg_ctx *ctx = nullptr;
   hipStream_t s = nullptr;
@@ -2413,103 +2374,43 @@ static void sink(kg_hash_agg *a, const kg_column *key, const kg_column *inputs, 
   if (a->has_policy) after_partial_batch(a, (uint64_t)nrows);
 }
 
-// n bits from src (starting at bit sbit) to dst (starting at bit dbit), LSB
-// first, as Arrow bitmaps; src == nullptr writes ones.
-static void copy_bits(uint8_t *dst, int64_t dbit, const uint8_t *src, int64_t sbit, int64_t n) {
-  auto get = [&](int64_t i) { return src ? (src[(sbit + i) >> 3] >> ((sbit + i) & 7)) & 1 : 1; };
-  auto put = [&](int64_t i, int b) {
-    uint8_t &d = dst[(dbit + i) >> 3];
-    const int sh = (int)((dbit + i) & 7);
-    d = (uint8_t)((d & ~(1u << sh)) | ((unsigned)b << sh));
-  };
-  int64_t i = 0;
-  for (; i < n && ((dbit + i) & 7); i++) put(i, get(i));
-  const int64_t full = (n - i) >> 3;
-  uint8_t *d = dst + ((dbit + i) >> 3);
-  if (!src) {
-    memset(d, 0xFF, (size_t)full);
-  } else if (((sbit + i) & 7) == 0) {
-    memcpy(d, src + ((sbit + i) >> 3), (size_t)full);
-  } else {
-    const int sh = (int)((sbit + i) & 7);
-    const uint8_t *p = src + ((sbit + i) >> 3);
-    for (int64_t k = 0; k < full; k++) d[k] = (uint8_t)((p[k] >> sh) | (p[k + 1] << (8 - sh)));
-  }
-  i += full * 8;
-  for (; i < n; i++) put(i, get(i));
-}
-
-static HostGather::Slot &open_slot(HostGather &g) { return g.slot[g.cur]; }
-
-// The open slot's rows to the HBM buffer (DMA on the operator's stream).
-static void submit_slot(kg_hash_agg *a, HostGather &g) {
-  HostGather::Slot &sl = open_slot(g);
-  if (sl.rows == 0) return;
-  for (int c = 0; c < g.ncols; c++) {
-    const int w = g.width[c];
-    const uint64_t vb = w ? (uint64_t)sl.rows * w : (uint64_t)ceil_div(sl.rows, 8);
-    uint8_t *dv = g.vals[c].get<uint8_t>() + (w ? (uint64_t)sl.base * w : (uint64_t)sl.base / 8);
-    KG_HIP(hipMemcpyAsync(dv, sl.host + g.voff[c], vb, hipMemcpyHostToDevice, a->s));
-    KG_HIP(hipMemcpyAsync(g.valid[c].get<uint8_t>() + sl.base / 8, sl.host + g.boff[c],
-                          (size_t)ceil_div(sl.rows, 8), hipMemcpyHostToDevice, a->s));
-  }
-  KG_HIP(hipEventRecord(sl.done, a->s));
-  const int64_t next = sl.base + sl.rows;
-  g.cur ^= 1;
-  HostGather::Slot &nx = open_slot(g);
-  KG_HIP(hipEventSynchronize(nx.done));  // its previous DMA has read the staging
-  nx.base = next;
-  nx.rows = 0;
-}
-
 // Aggregates the gathered rows.  The rows stay gathered when this fails, so
 // a NeedMemory error leaves the operator as it was.
 static void flush_gather(kg_hash_agg *a) {
-  if (!a->gather || a->gather->rows == 0) return;
+  if (!a->gather || a->gather->rows() == 0) return;
   HostGather &g = *a->gather;
-  const int open = g.cur;
-  submit_slot(a, g);
-  kg_column cols[1 + kAggInputs];
-  for (int c = 0; c < g.ncols; c++) {
-    cols[c] = g.type[c];
-    cols[c].flags = KG_COL_DEVICE;
-    cols[c].length = g.rows;
-    cols[c].offset = 0;
-    cols[c].values = g.vals[c].get();
-    cols[c].validity = g.any_valid[c] ? g.valid[c].get<uint8_t>() : nullptr;
-  }
+  kg_column cols[kGatherCols];
+  g.submit(a->s, cols);
   const int k0 = a->has_key ? 1 : 0;
   try {
-    sink(a, a->has_key ? &cols[0] : nullptr, cols + k0, g.rows);
+    sink(a, a->has_key ? &cols[0] : nullptr, cols + k0, g.rows());
   } catch (...) {
-    // the slot stays open (its next submit copies its rows again), so later
-    // rows keep their place in it; its copy in flight finishes first
-    g.cur = open;
-    (void)hipEventSynchronize(open_slot(g).done);
+    g.keep();
     throw;
   }
-  g.rows = 0;
-  for (int c = 0; c < g.ncols; c++) g.any_valid[c] = false;
-  // the open slot (emptied by submit_slot) starts the buffer again; the
-  // stream orders its DMA after the aggregation that read the buffer
-  open_slot(g).base = 0;
-  open_slot(g).rows = 0;
+  g.clear();
 }
 
-// The gather's HBM rows for these column types (a power of two).
-static int64_t gather_cap(const kg_hash_agg *a, const kg_column *key, const kg_column *inputs) {
-  uint64_t bits = 0;  // bits a row: values and validity
-  const int k0 = a->has_key ? 1 : 0;
-  for (int c = 0; c < k0 + a->nsink; c++) {
-    const int w = type_width(c < k0 ? key->type : inputs[c - k0].type);
-    if (w < 0) return 0;
-    bits += (w ? 8u * (uint64_t)w : 1u) + 1;
-  }
-  int64_t cap = kGatherRows;
-  while (cap > 4096 && (uint64_t)cap * bits / 8 > kGatherBytes) cap >>= 1;
-  return cap;
+// The columns of a sink call: the grouping column (when there is one), then
+// the inputs.
+static int sink_columns(const kg_hash_agg *a, const kg_column *key, const kg_column *inputs,
+                        const kg_column **cols) {
+  int n = 0;
+  if (a->has_key) cols[n++] = key;
+  for (int c = 0; c < a->nsink; c++) cols[n++] = &inputs[c];
+  return n;
 }
 
+static int64_t gather_limit(const kg_hash_agg *a, const kg_column *key, const kg_column *inputs) {
+  const kg_column *cols[kGatherCols];
+  const int n = sink_columns(a, key, inputs, cols);
+  int width[kGatherCols];
+  for (int c = 0; c < n; c++) width[c] = type_width(cols[c]->type);
+  return gather_max_rows(width, n);
+}
+
+// Host batches of fewer rows than a slot of the largest gather are gathered
+// (without a bypass policy, whose decisions follow each batch).
 static bool gathers(const kg_hash_agg *a, const kg_column *key, const kg_column *inputs,
                     int64_t nrows) {
   if (a->has_policy || nrows <= 0 || (!a->has_key && a->nsink == 0)) return false;
@@ -2517,33 +2418,7 @@ static bool gathers(const kg_hash_agg *a, const kg_column *key, const kg_column 
   if (a->has_key && (key->flags & KG_COL_DEVICE)) return false;
   for (int c = 0; c < a->nsink; c++)
     if (inputs[c].flags & KG_COL_DEVICE) return false;
-  return nrows < gather_cap(a, key, inputs) / 8;  // a slot of the largest gather
-}
-
-// A gather of cap rows (a power of two) for columns of these widths: its HBM
-// buffers and two pinned staging slots of cap / 8 rows.
-static std::unique_ptr<HostGather> make_gather(kg_hash_agg *a, int ncols, const int *width,
-                                               int64_t cap) {
-  auto g = std::make_unique<HostGather>();
-  g->ncols = ncols;
-  g->cap = cap;
-  g->slot_rows = cap / 8;
-  size_t off = 0;
-  for (int c = 0; c < ncols; c++) {
-    g->width[c] = width[c];
-    const uint64_t w = (uint64_t)width[c];
-    g->vals[c].reset(a->ctx, w ? cap * w : cap / 8);
-    g->valid[c].reset(a->ctx, cap / 8);
-    g->voff[c] = off;
-    off += ((w ? g->slot_rows * w : g->slot_rows / 8) + 255) & ~(size_t)255;
-    g->boff[c] = off;
-    off += ((size_t)g->slot_rows / 8 + 255) & ~(size_t)255;
-  }
-  for (auto &sl : g->slot) {
-    KG_HIP(hipHostMalloc((void **)&sl.host, off, hipHostMallocDefault));
-    KG_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-  }
-  return g;
+  return nrows < gather_limit(a, key, inputs) / 8;
 }
 
 // A host batch into the gather: checked as sink checks it, the gathered rows
@@ -2554,9 +2429,6 @@ static std::unique_ptr<HostGather> make_gather(kg_hash_agg *a, int ncols, const 
 static void gather_batch(kg_hash_agg *a, const kg_column *key, const kg_column *inputs,
                          int64_t nrows) {
   if (a->finalized) fail(KG_ERR_INTERNAL, "sink after finalize");
-  const int k0 = a->has_key ? 1 : 0;
-  const int ncols = k0 + a->nsink;
-  auto col = [&](int c) -> const kg_column & { return c < k0 ? *key : inputs[c - k0]; };
   if (a->has_key) {
     check_column(*key, nrows, "grouping column");
     if (key->type != a->key_type.type) fail(KG_ERR_INTERNAL, "grouping column type changed");
@@ -2566,58 +2438,26 @@ static void gather_batch(kg_hash_agg *a, const kg_column *key, const kg_column *
     if (!a->merge && inputs[c].type != a->in_types[c].type)
       fail(KG_ERR_INTERNAL, "input column type changed");
   }
-  int width[1 + kAggInputs];
-  for (int c = 0; c < ncols; c++) width[c] = type_width(col(c).type);
-  const int64_t maxcap = gather_cap(a, key, inputs);
-  int64_t want = 65536;
-  while (want < nrows * 64 && want < maxcap) want <<= 1;
-  want = std::min(want, maxcap);
+  const kg_column *cols[kGatherCols];
+  const int ncols = sink_columns(a, key, inputs, cols);
+  int width[kGatherCols];
+  for (int c = 0; c < ncols; c++) width[c] = type_width(cols[c]->type);
+  const int64_t maxcap = gather_max_rows(width, ncols);
+  const int64_t want = gather_rows_for(nrows, maxcap);
   int64_t cap = want;
   if (a->gather) {
     HostGather &g = *a->gather;
-    bool same = g.ncols == ncols, widths = g.ncols == ncols;
-    for (int c = 0; c < ncols && same; c++) {
-      widths = widths && width[c] == g.width[c];
-      same = widths && col(c).type == g.type[c].type &&
-             col(c).precision == g.type[c].precision && col(c).scale == g.type[c].scale;
-    }
-    const bool fits = g.rows + nrows <= g.cap, in_slot = nrows < g.slot_rows;
-    if (g.rows > 0 && (!same || !fits || !in_slot)) flush_gather(a);
-    cap = g.cap;
+    bool widths = g.ncols() == ncols;
+    for (int c = 0; c < ncols && widths; c++) widths = width[c] == g.width(c);
+    const bool fits = g.rows() + nrows <= g.cap(), in_slot = nrows < g.slot_rows();
+    if (g.rows() > 0 && (!widths || !g.same_types(cols) || !fits || !in_slot)) flush_gather(a);
+    cap = g.cap();
     if (!fits && cap < maxcap) cap = std::min(cap * 4, maxcap);
     if (!in_slot || !widths) cap = std::max(cap, want);
-    if (cap != g.cap || !widths) a->gather.reset();  // empty by now
+    if (cap != g.cap() || !widths) a->gather.reset();  // empty by now
   }
-  if (!a->gather) a->gather = make_gather(a, ncols, width, cap);
-  HostGather &g = *a->gather;
-  if (g.rows == 0)
-    for (int c = 0; c < ncols; c++) {
-      g.type[c] = kg_column{};
-      g.type[c].type = col(c).type;
-      g.type[c].precision = col(c).precision;
-      g.type[c].scale = col(c).scale;
-    }
-  for (int64_t done = 0; done < nrows;) {
-    HostGather::Slot &sl = open_slot(g);
-    const int64_t take = std::min<int64_t>(nrows - done, g.slot_rows - sl.rows);
-    for (int c = 0; c < ncols; c++) {
-      const kg_column &src = col(c);
-      const int w = g.width[c];
-      const int64_t first = src.offset + done;
-      if (w)
-        memcpy(sl.host + g.voff[c] + (size_t)sl.rows * w,
-               static_cast<const uint8_t *>(src.values) + (size_t)first * w, (size_t)take * w);
-      else
-        copy_bits(sl.host + g.voff[c], sl.rows, static_cast<const uint8_t *>(src.values), first,
-                  take);
-      copy_bits(sl.host + g.boff[c], sl.rows, src.validity, first, take);
-      if (src.validity) g.any_valid[c] = true;
-    }
-    sl.rows += take;
-    g.rows += take;
-    done += take;
-    if (sl.rows == g.slot_rows) submit_slot(a, g);
-  }
+  if (!a->gather) a->gather = std::make_unique<HostGather>(a->ctx, ncols, width, cap);
+  a->gather->append(a->s, cols, nrows);
 }
 
 static void sink_or_gather(kg_hash_agg *a, const kg_column *key, const kg_column *inputs,
@@ -3133,7 +2973,7 @@ int kg_agg_set_bypass(kg_hash_agg *a, uint64_t min_rows, uint32_t max_group_perc
                       uint64_t max_table_bytes) {
   return guarded([&] {
     if (!a->partial) fail(KG_ERR_INTERNAL, "a bypass policy needs a partial aggregate");
-    if (a->rows_seen || (a->gather && a->gather->rows))
+    if (a->rows_seen || (a->gather && a->gather->rows()))
       fail(KG_ERR_INTERNAL, "the bypass policy is set before the first sink");
     if (!a->has_key) fail(KG_ERR_UNSUPPORTED, "a bypass policy for a global aggregate");
     a->has_policy = true;
@@ -3266,7 +3106,7 @@ int kg_agg_create(kg_ctx *ctx, const kg_column *key_type, int32_t ninputs,
 int kg_agg_set_merge(kg_hash_agg *a, int32_t form, int32_t size_from_sketch) {
   return guarded([&] {
     if (form < KG_AGG_MERGE_AUTO || form > KG_AGG_MERGE_BY_KEY) fail(KG_ERR_INTERNAL, "merge form");
-    if (a->rows_seen || (a->gather && a->gather->rows))
+    if (a->rows_seen || (a->gather && a->gather->rows()))
       fail(KG_ERR_INTERNAL, "the merge form is set before the first sink");
     a->merge_form = form;
     a->size_from_sketch = size_from_sketch != 0;

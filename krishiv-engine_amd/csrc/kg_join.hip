@@ -22,6 +22,7 @@
 #include "kg_aggops.hpp"
 #include "kg_scan.hpp"
 #include "kg_aggtile.hpp"
+#include "kg_gather.hpp"
 
 namespace kg {
 
@@ -1155,6 +1156,10 @@ struct kg_hash_join {
   uint64_t agg_rows = 0;  // probe rows applied so far
   uint64_t max_dup = 1;   // the most build rows of one key (a probe row's pairs)
   DevBuf abs_scratch;
+  // small host batches gathered in HBM: build batches until finalize, fused
+  // aggregate probe batches (with their input columns) until the result
+  std::unique_ptr<kg::HostGather> bgather, pgather;
+  std::vector<int32_t> pgather_inputs;
   // scan scratch of the fused aggregate and the tail
   DevBuf temp;
   // residual filter (column indices of the probe and build batches)
@@ -1417,6 +1422,64 @@ static void scan_u32(kg_hash_join *j, const uint32_t *in, uint64_t *out, int64_t
   exclusive_scan(in, out, n, j->temp.get<uint64_t>(), j->s);
 }
 
+// Whether a batch of host columns is small enough to gather (every column
+// in host memory, fewer rows than a slot of the largest gather).
+static bool gathers_host(int32_t ncols, const kg_column *cols, int64_t nrows) {
+  if (nrows <= 0 || ncols < 1 || ncols > kGatherCols) return false;
+  int width[kGatherCols];
+  for (int c = 0; c < ncols; c++) {
+    if (cols[c].flags & KG_COL_DEVICE) return false;
+    width[c] = type_width(cols[c].type);
+  }
+  return nrows < gather_max_rows(width, ncols) / 8;
+}
+
+// A host batch into gather g: the gathered rows are taken first (flush)
+// when it does not fit or differs in types; g starts at 64 times the first
+// batch's rows and is replaced by one four times larger when it fills, up to
+// the largest for the row width.
+template <class Flush>
+static void gather_into(kg_hash_join *j, std::unique_ptr<HostGather> &g, int32_t ncols,
+                        const kg_column *cols, int64_t nrows, Flush flush) {
+  const kg_column *cp[kGatherCols];
+  int width[kGatherCols];
+  for (int c = 0; c < ncols; c++) {
+    cp[c] = &cols[c];
+    width[c] = type_width(cols[c].type);
+  }
+  const int64_t maxcap = gather_max_rows(width, ncols);
+  const int64_t want = gather_rows_for(nrows, maxcap);
+  int64_t cap = want;
+  if (g) {
+    bool widths = g->ncols() == ncols;
+    for (int c = 0; c < ncols && widths; c++) widths = width[c] == g->width(c);
+    const bool fits = g->rows() + nrows <= g->cap(), in_slot = nrows < g->slot_rows();
+    if (g->rows() > 0 && (!widths || !g->same_types(cp) || !fits || !in_slot)) flush();
+    cap = g->cap();
+    if (!fits && cap < maxcap) cap = std::min(cap * 4, maxcap);
+    if (!in_slot || !widths) cap = std::max(cap, want);
+    if (cap != g->cap() || !widths) g.reset();  // empty by now
+  }
+  if (!g) g = std::make_unique<HostGather>(j->ctx, ncols, width, cap);
+  g->append(j->s, cp, nrows);
+}
+
+// The gathered build rows appended as one device batch (kept gathered when
+// that fails, so a NeedMemory refusal leaves the build as it was).
+static void flush_build_gather(kg_hash_join *j) {
+  if (!j->bgather || j->bgather->rows() == 0) return;
+  HostGather &g = *j->bgather;
+  kg_column cols[kGatherCols];
+  g.submit(j->s, cols);
+  try {
+    append_build(j, g.ncols(), cols, g.rows());
+  } catch (...) {
+    g.keep();
+    throw;
+  }
+  g.clear();
+}
+
 }  // namespace kg
 
 extern "C" {
@@ -1449,10 +1512,20 @@ int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, in
     if (j->finalized) fail(KG_ERR_INTERNAL, "build sink after finalize");
     if (ncols < 1 || ncols > kJoinCols) fail(KG_ERR_UNSUPPORTED, "1 to 32 build columns");
     check_keys(j->spec, ncols, cols, j->spec.build_keys);
-    if (j->build_rows + nrows > (int64_t)0x7FFFFFF0)
+    if (j->build_rows + (j->bgather ? j->bgather->rows() : 0) + nrows > (int64_t)0x7FFFFFF0)
       fail(KG_ERR_UNSUPPORTED, "a build side of 2^31 rows or more (grace partitions first)");
     // a first batch of no rows still fixes the build schema (an empty build
     // side's output columns)
+    if (gathers_host(ncols, cols, nrows)) {
+      if (!j->bcols.empty() && (int)j->bcols.size() != ncols)
+        fail(KG_ERR_INTERNAL, "build batches differ in columns");
+      for (int c = 0; c < (int)j->bcols.size(); c++)
+        if (cols[c].type != j->bcols[c].type) fail(KG_ERR_INTERNAL, "build column type changed");
+      for (int c = 0; c < ncols; c++) check_column(cols[c], nrows, "join column");
+      gather_into(j, j->bgather, ncols, cols, nrows, [&] { flush_build_gather(j); });
+      return;
+    }
+    flush_build_gather(j);
     if (nrows > 0 || j->bcols.empty()) append_build(j, ncols, cols, nrows);
   });
 }
@@ -1461,6 +1534,7 @@ int kg_join_build_finalize(kg_hash_join *j) {
   return guarded([&] {
     StreamScope kg_scope(j->s);
     if (j->finalized) return;
+    flush_build_gather(j);
     int64_t n = j->build_rows;
     uint64_t want = (uint64_t)(n > (int64_t)j->expected ? n : (int64_t)j->expected);
     if (want < 512) want = 512;
@@ -1511,7 +1585,9 @@ int kg_join_build_finalize(kg_hash_join *j) {
   });
 }
 
-int64_t kg_join_build_rows(const kg_hash_join *j) { return j ? j->build_rows : 0; }
+int64_t kg_join_build_rows(const kg_hash_join *j) {
+  return j ? j->build_rows + (j->bgather ? j->bgather->rows() : 0) : 0;
+}
 
 }  // extern "C"
 
@@ -2062,6 +2138,84 @@ static bool joinagg_partitioned(kg_hash_join *j, const KeySpec &K, const AggProg
 
 }  // namespace kg
 
+// One probe batch through the fused aggregate.
+static void joinagg_probe_batch(kg_hash_join *j, int32_t ncols, const kg_column *cols,
+                                int64_t nrows, const int32_t *input_cols) {
+  Staged st(j->ctx, j->s);
+  std::vector<DCol> d;
+  stage_all(st, ncols, cols, nrows, d);
+  KeySpec K = key_spec(j, d.data(), j->spec.probe_keys);
+  AggProg P = j->prog;
+  for (int c = 0; c < (int)j->agg_types.size(); c++) {
+    int ic = input_cols[c];
+    if (ic < 0 || ic >= ncols) fail(KG_ERR_INTERNAL, "aggregate input column index");
+    if (d[ic].type != j->agg_types[c].type) fail(KG_ERR_INTERNAL, "aggregate input type");
+    P.in[c] = d[ic];
+  }
+  if (nrows >= (1ll << 32)) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^32 rows or more");
+  // checked sums: proven for this batch before anything is applied (a
+  // batch that cannot be proven is refused, the state unchanged: the CPU
+  // path replays such sums in row order)
+  std::vector<double> add(j->checked.size(), 0.0);
+  for (size_t k = 0; k < j->checked.size(); k++)
+    if (!checked_batch_proven(j->ctx, j->s, j->abs_scratch, j->checked[k],
+                              P.in[j->checked[k].col], nrows, j->agg_rows, j->max_dup, add[k]))
+      fail(KG_ERR_UNSUPPORTED,
+           "a decimal or ANSI sum of the fused join aggregate that this batch could overflow");
+  const int WL = P.W | 1;
+  const size_t dlds = (size_t)j->dspan * (WL * 8 + 4);
+  int optin = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, j->ctx->device) !=
+          hipSuccess || optin < 64 * 1024) {
+    (void)hipGetLastError();
+    optin = 64 * 1024;
+  }
+  if (j->dspan && dlds + 1024 <= (size_t)optin && nrows >= (1 << 16)) {
+    // direct index, records privatized in LDS
+    ProfScope prof("joinagg_direct", j->s);
+    const bool dfast = K.n == 1 && !K.null_safe && type_width(K.c[0].type) == 8 &&
+                       !is_float(K.c[0].type);
+    const void *dfn = dfast ? (const void *)k_joinagg_direct<true>
+                            : (const void *)k_joinagg_direct<false>;
+    KG_HIP(set_dynamic_lds(dfn, (int)dlds));
+    const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
+    const int64_t per_cu = dlds <= 40 * 1024 ? 4 : dlds <= 80 * 1024 ? 2 : 1;
+    const int64_t grid = std::min<int64_t>(cus * per_cu,
+                                           ceil_div(nrows, (int64_t)kJDBlock * kJDTile * 8));
+    if (dfast)
+      k_joinagg_direct<true><<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
+          K, nrows, jtable(j), P, j->rec.get<uint64_t>());
+    else
+      k_joinagg_direct<false><<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
+          K, nrows, jtable(j), P, j->rec.get<uint64_t>());
+    KG_HIP(hipGetLastError());
+  } else if (nrows < (1 << 20) || !joinagg_partitioned(j, K, P, nrows)) {
+    ProfScope prof(j->dspan ? "joinagg_probe_direct" : "joinagg_probe", j->s);
+    k_joinagg_probe<<<stream_grid(nrows, 256), 256, 0, j->s>>>(K, nrows, jtable(j), P,
+                                                               j->rec.get<uint64_t>());
+    KG_HIP(hipGetLastError());
+  }
+  if (st.staged_host()) KG_HIP(hipStreamSynchronize(j->s));
+  j->agg_rows += (uint64_t)nrows;
+  for (size_t k = 0; k < j->checked.size(); k++) j->checked[k].cum_abs += add[k];
+}
+
+// The gathered probe batches through the fused aggregate (kept gathered when
+// that fails: a refusal leaves the aggregate as it was).
+static void flush_probe_gather(kg_hash_join *j) {
+  if (!j->pgather || j->pgather->rows() == 0) return;
+  HostGather &g = *j->pgather;
+  kg_column cols[kGatherCols];
+  g.submit(j->s, cols);
+  try {
+    joinagg_probe_batch(j, g.ncols(), cols, g.rows(), j->pgather_inputs.data());
+  } catch (...) {
+    g.keep();
+    throw;
+  }
+  g.clear();
+}
+
 int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int64_t nrows,
                      const int32_t *input_cols) {
   return guarded([&] {
@@ -2069,63 +2223,25 @@ int kg_joinagg_probe(kg_hash_join *j, int32_t ncols, const kg_column *cols, int6
     if (!j->has_agg) fail(KG_ERR_INTERNAL, "no fused aggregate");
     if (nrows <= 0) return;
     check_keys(j->spec, ncols, cols, j->spec.probe_keys);
-    Staged st(j->ctx, j->s);
-    std::vector<DCol> d;
-    stage_all(st, ncols, cols, nrows, d);
-    KeySpec K = key_spec(j, d.data(), j->spec.probe_keys);
-    AggProg P = j->prog;
-    for (int c = 0; c < (int)j->agg_types.size(); c++) {
-      int ic = input_cols[c];
+    const int na = (int)j->agg_types.size();
+    for (int c = 0; c < na; c++) {
+      const int ic = input_cols[c];
       if (ic < 0 || ic >= ncols) fail(KG_ERR_INTERNAL, "aggregate input column index");
-      if (d[ic].type != j->agg_types[c].type) fail(KG_ERR_INTERNAL, "aggregate input type");
-      P.in[c] = d[ic];
+      if (cols[ic].type != j->agg_types[c].type) fail(KG_ERR_INTERNAL, "aggregate input type");
     }
-    if (nrows >= (1ll << 32)) fail(KG_ERR_UNSUPPORTED, "a probe batch of 2^32 rows or more");
-    // checked sums: proven for this batch before anything is applied (a
-    // batch that cannot be proven is refused, the state unchanged: the CPU
-    // path replays such sums in row order)
-    std::vector<double> add(j->checked.size(), 0.0);
-    for (size_t k = 0; k < j->checked.size(); k++)
-      if (!checked_batch_proven(j->ctx, j->s, j->abs_scratch, j->checked[k],
-                                P.in[j->checked[k].col], nrows, j->agg_rows, j->max_dup, add[k]))
-        fail(KG_ERR_UNSUPPORTED,
-             "a decimal or ANSI sum of the fused join aggregate that this batch could overflow");
-    const int WL = P.W | 1;
-    const size_t dlds = (size_t)j->dspan * (WL * 8 + 4);
-    int optin = 0;
-    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, j->ctx->device) !=
-            hipSuccess || optin < 64 * 1024) {
-      (void)hipGetLastError();
-      optin = 64 * 1024;
+    if (gathers_host(ncols, cols, nrows)) {
+      for (int c = 0; c < ncols; c++) check_column(cols[c], nrows, "join column");
+      const bool same_inputs = j->pgather_inputs.size() == (size_t)na &&
+                               std::equal(input_cols, input_cols + na, j->pgather_inputs.begin());
+      if (!same_inputs) {
+        flush_probe_gather(j);
+        j->pgather_inputs.assign(input_cols, input_cols + na);
+      }
+      gather_into(j, j->pgather, ncols, cols, nrows, [&] { flush_probe_gather(j); });
+      return;
     }
-    if (j->dspan && dlds + 1024 <= (size_t)optin && nrows >= (1 << 16)) {
-      // direct index, records privatized in LDS
-      ProfScope prof("joinagg_direct", j->s);
-      const bool dfast = K.n == 1 && !K.null_safe && type_width(K.c[0].type) == 8 &&
-                         !is_float(K.c[0].type);
-      const void *dfn = dfast ? (const void *)k_joinagg_direct<true>
-                              : (const void *)k_joinagg_direct<false>;
-      KG_HIP(set_dynamic_lds(dfn, (int)dlds));
-      const int64_t cus = j->ctx->num_cus > 0 ? j->ctx->num_cus : 256;
-      const int64_t per_cu = dlds <= 40 * 1024 ? 4 : dlds <= 80 * 1024 ? 2 : 1;
-      const int64_t grid = std::min<int64_t>(cus * per_cu,
-                                             ceil_div(nrows, (int64_t)kJDBlock * kJDTile * 8));
-      if (dfast)
-        k_joinagg_direct<true><<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
-            K, nrows, jtable(j), P, j->rec.get<uint64_t>());
-      else
-        k_joinagg_direct<false><<<(unsigned)std::max<int64_t>(grid, 1), kJDBlock, dlds, j->s>>>(
-            K, nrows, jtable(j), P, j->rec.get<uint64_t>());
-      KG_HIP(hipGetLastError());
-    } else if (nrows < (1 << 20) || !joinagg_partitioned(j, K, P, nrows)) {
-      ProfScope prof(j->dspan ? "joinagg_probe_direct" : "joinagg_probe", j->s);
-      k_joinagg_probe<<<stream_grid(nrows, 256), 256, 0, j->s>>>(K, nrows, jtable(j), P,
-                                                                 j->rec.get<uint64_t>());
-      KG_HIP(hipGetLastError());
-    }
-    if (st.staged_host()) KG_HIP(hipStreamSynchronize(j->s));
-    j->agg_rows += (uint64_t)nrows;
-    for (size_t k = 0; k < j->checked.size(); k++) j->checked[k].cum_abs += add[k];
+    flush_probe_gather(j);
+    joinagg_probe_batch(j, ncols, cols, nrows, input_cols);
   });
 }
 
@@ -2134,6 +2250,7 @@ int kg_joinagg_result(kg_hash_join *j, kg_batch **out) {
     StreamScope kg_scope(j->s);
     *out = nullptr;
     if (!j->has_agg) fail(KG_ERR_INTERNAL, "no fused aggregate");
+    flush_probe_gather(j);
     DevBuf pos;
     uint64_t ns = j->cap + 2;
     int64_t n = group_positions(j->ctx, j->rec.get<uint64_t>(), ns, j->prog.W, false, pos, j->s);
