@@ -42,6 +42,8 @@ class HostGather {
   // Whether these columns have the gathered rows' types (type, precision,
   // scale); true when nothing is gathered.
   bool same_types(const kg_column *const *cols) const;
+  // The type of gathered column c (valid while rows() > 0).
+  int type(int c) const { return type_[c].type; }
   // Rows [0, n) of each host column (at its offset) into the staging; full
   // slots go to HBM on s.  The caller checked rows() + n <= cap().
   void append(hipStream_t s, const kg_column *const *cols, int64_t n);

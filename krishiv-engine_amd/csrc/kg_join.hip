@@ -1521,6 +1521,12 @@ int kg_join_build_sink(kg_hash_join *j, int32_t ncols, const kg_column *cols, in
         fail(KG_ERR_INTERNAL, "build batches differ in columns");
       for (int c = 0; c < (int)j->bcols.size(); c++)
         if (cols[c].type != j->bcols[c].type) fail(KG_ERR_INTERNAL, "build column type changed");
+      if (j->bcols.empty() && j->bgather && j->bgather->rows() > 0) {  // the gathered schema
+        if (j->bgather->ncols() != ncols) fail(KG_ERR_INTERNAL, "build batches differ in columns");
+        for (int c = 0; c < ncols; c++)
+          if (cols[c].type != j->bgather->type(c))
+            fail(KG_ERR_INTERNAL, "build column type changed");
+      }
       for (int c = 0; c < ncols; c++) check_column(cols[c], nrows, "join column");
       gather_into(j, j->bgather, ncols, cols, nrows, [&] { flush_build_gather(j); });
       return;
