@@ -263,6 +263,9 @@ struct kg_server {
     bool busy = false;
     std::vector<Pending> reqs;
     std::vector<SReq> sreqs;
+    // each request's result columns (key, SUM, COUNT; cap + 1 rows), written
+    // by the output phase and handed to the caller's batch as they are
+    std::vector<DevBuf> outs;
   };
   Slot slots[kSrvSlots];
   std::map<int64_t, kg_batch *> results;
@@ -401,9 +404,10 @@ static uint64_t req_cap(kg_server *S, const kg_serve_request &r) {
   return 1ull << slog2(n * 2);
 }
 
-// Arena bytes of a request: own table slots, records and output columns.
+// Arena bytes of a request: own table slots and records (its result
+// columns are allocated apart, and handed over with its result).
 static uint64_t req_bytes(uint64_t cap, bool own) {
-  uint64_t per = (own ? 16 : 0) + 16 + 24;
+  uint64_t per = (own ? 16 : 0) + 16;
   return ((cap + 1) * per + 255) / 256 * 256;
 }
 
@@ -462,6 +466,19 @@ static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
     sl.reqs.push_back(p);
     S->queue.pop_front();
   }
+  // the result columns; refused for memory, the requests go back to the
+  // front of the queue
+  try {
+    sl.outs.clear();
+    sl.outs.resize(3 * sl.reqs.size());
+    for (size_t q = 0; q < sl.reqs.size(); q++)
+      for (int c = 0; c < 3; c++) sl.outs[3 * q + c].reset(S->ctx, (sl.reqs[q].cap + 1) * 8 + 8);
+  } catch (...) {
+    for (size_t q = sl.reqs.size(); q-- > 0;) S->queue.push_front(sl.reqs[q]);
+    sl.reqs.clear();
+    sl.outs.clear();
+    throw;
+  }
   // host descriptor
   uint8_t *h = sl.desc_host;
   SDesc *HD = reinterpret_cast<SDesc *>(h);
@@ -518,12 +535,9 @@ static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
     }
     s.acc = reinterpret_cast<uint64_t *>(arena + at);
     at += (s.cap + 1) * 16;
-    s.out_key = reinterpret_cast<int64_t *>(arena + at);
-    at += (s.cap + 1) * 8;
-    s.out_sum = reinterpret_cast<int64_t *>(arena + at);
-    at += (s.cap + 1) * 8;
-    s.out_cnt = reinterpret_cast<int64_t *>(arena + at);
-    at += (s.cap + 1) * 8;
+    s.out_key = sl.outs[3 * q].get<int64_t>();
+    s.out_sum = sl.outs[3 * q + 1].get<int64_t>();
+    s.out_cnt = sl.outs[3 * q + 2].get<int64_t>();
     at = (at + 255) / 256 * 256;
     s.bkeys = r.build_keys;
     s.nb = own ? r.build_rows : 0;
@@ -567,30 +581,28 @@ static bool launch_batch(kg_server *S, kg_server::Slot &sl) {
 static void retire(kg_server *S, kg_server::Slot &sl, int64_t *done_ids, int32_t max,
                    int32_t *ndone) {
   StreamScope kg_scope(sl.s);
-  std::vector<unsigned long long> groups(sl.reqs.size());
-  uint8_t *gdev = sl.desc_dev.get<uint8_t>() + sizeof(SDesc) + sizeof(SReq) * S->max_batch +
-                  4 * sizeof(STile) * sl.max_tiles;
-  KG_HIP(hipMemcpyAsync(groups.data(), gdev, 8 * groups.size(), hipMemcpyDeviceToHost, sl.s));
+  // the group counts, into the pinned mirror of the descriptor (the batch
+  // that used it has finished); each request's result columns were written
+  // in place by the output phase
+  const size_t nq = sl.reqs.size();
+  const size_t goff = sizeof(SDesc) + sizeof(SReq) * S->max_batch + 4 * sizeof(STile) * sl.max_tiles;
+  unsigned long long *groups = reinterpret_cast<unsigned long long *>(sl.desc_host + goff);
+  KG_HIP(hipMemcpyAsync(groups, sl.desc_dev.get<uint8_t>() + goff, 8 * nq, hipMemcpyDeviceToHost,
+                        sl.s));
   KG_HIP(hipStreamSynchronize(sl.s));
-  for (size_t q = 0; q < sl.reqs.size(); q++) {
-    int64_t n = (int64_t)groups[q];
+  for (size_t q = 0; q < nq; q++) {
     auto b = std::make_unique<kg_batch>();
     b->ctx = S->ctx;
-    b->nrows = n;
+    b->nrows = (int64_t)groups[q];
     b->cols.resize(3);
-    const SReq &s = sl.sreqs[q];
-    const int64_t *src[3] = {s.out_key, s.out_sum, s.out_cnt};
     for (int c = 0; c < 3; c++) {
       b->cols[c].type = KG_T_INT64;
-      b->cols[c].values.reset(S->ctx, (uint64_t)n * 8 + 8);
-      if (n)
-        KG_HIP(hipMemcpyAsync(b->cols[c].values.get(), src[c], (size_t)n * 8,
-                              hipMemcpyDeviceToDevice, sl.s));
+      b->cols[c].values = std::move(sl.outs[3 * q + c]);
     }
     S->results[sl.reqs[q].r.id] = b.release();
     report(S, sl.reqs[q].r.id, done_ids, max, ndone);
   }
-  KG_HIP(hipStreamSynchronize(sl.s));
+  sl.outs.clear();
   S->inflight -= (int64_t)sl.reqs.size();
   sl.busy = false;
   sl.reqs.clear();
