@@ -182,12 +182,13 @@ struct SpillDev {
 // 3 %).
 constexpr int kHllBits = 10;
 constexpr int kHllRegs = 1 << kHllBits;
-constexpr uint64_t kHllSalt = 0xD6E8FEB86659FD93ULL;
-
-__device__ __forceinline__ void hll_add(unsigned int *reg, const Table &T, uint64_t key) {
-  const uint64_t h = dev::table_hash(key, T.seed ^ kHllSalt);
-  const uint32_t j = (uint32_t)(h >> (64 - kHllBits));
-  const unsigned int rho = (unsigned int)__clzll((h << kHllBits) | (1ull << (kHllBits - 1))) + 1u;
+// A key's sketch update from its slot hash h (the hash that also places it
+// in the table and the spill's partitions): the register from the low
+// kHllBits bits, the rank from the leading zeros of the other 54.
+__device__ __forceinline__ void hll_add(unsigned int *reg, uint64_t h) {
+  const uint32_t j = (uint32_t)(h & (kHllRegs - 1));
+  const uint64_t w = h >> kHllBits;
+  const unsigned int rho = w ? (unsigned int)(__clzll(w) - kHllBits + 1) : 64u - kHllBits + 1u;
   if (rho > reg[j]) atomicMax(&reg[j], rho);
 }
 
@@ -469,8 +470,9 @@ __global__ void k_spill_hist(const uint64_t *rows, int w, uint64_t n, Table T, i
 #pragma unroll
     for (int r = 0; r < 4; r++)
       if (live[r]) {
-        atomicAdd(&s_h[spill_part(T, key[r], log2p)], 1u);
-        if (hll) hll_add(s_reg, T, key[r]);
+        const uint64_t h = slot_hash(T, key[r]);
+        atomicAdd(&s_h[log2p ? (uint32_t)(h >> (64 - log2p)) : 0u], 1u);
+        if (hll) hll_add(s_reg, h);
       }
   }
   __syncthreads();
