@@ -1339,11 +1339,28 @@ __global__ void k_agg_output(OutProg O, const uint64_t *keys, int key_stride, co
 // ======================================================================
 namespace kg {
 
-__global__ void k_fill_words(uint64_t *rec, uint64_t nrec, int W, const uint64_t *init_dev) {
+// A record's initial words, passed by value (no upload, no wait).
+struct InitWords {
+  uint64_t w[kAggWords];
+};
+
+// The words into LDS first (uniform indices into the argument), then each
+// lane's words from LDS.
+__global__ void k_fill_words(uint64_t *rec, uint64_t nrec, int W, InitWords iw) {
+  __shared__ uint64_t s_w[kAggWords];
+  for (int w = 0; w < W; w++)
+    if (threadIdx.x == 0) s_w[w] = iw.w[w];
+  __syncthreads();
   uint64_t total = nrec * (uint64_t)W;
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
        t += (uint64_t)gridDim.x * blockDim.x)
-    rec[t] = init_dev[t % W];
+    rec[t] = s_w[t % W];
+}
+
+static InitWords init_words(const AggProg &P) {
+  InitWords iw{};
+  for (int w = 0; w < P.W && w < kAggWords; w++) iw.w[w] = P.init[w];
+  return iw;
 }
 
 // One byte per row (0/1) to an LSB bitmap: each thread owns a 32-bit word.
@@ -1380,11 +1397,15 @@ __global__ void k_rehash(AggProg P, const uint64_t *okeys, const uint64_t *orec,
 }
 
 // The initial state into every record whose slot holds no key.
-__global__ void k_init_free_records(Table T, int W, const uint64_t *init) {
+__global__ void k_init_free_records(Table T, int W, InitWords iw) {
+  __shared__ uint64_t s_w[kAggWords];
+  for (int w = 0; w < W; w++)
+    if (threadIdx.x == 0) s_w[w] = iw.w[w];
+  __syncthreads();
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < T.cap * (uint64_t)W;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t sl = t / (uint64_t)W;
-    if (T.keys[sl] == kEmpty) T.rec[t] = init[t - sl * (uint64_t)W];
+    if (T.keys[sl] == kEmpty) T.rec[t] = s_w[t - sl * (uint64_t)W];
   }
 }
 
@@ -1516,13 +1537,9 @@ static void ensure_rec_init(kg_hash_agg *a) {
   if (!a->hashed || a->rec_init) return;
   // only slots without a key: occupied records hold state
   const Table t = table_of(a);
-  DevBuf init(a->ctx, (uint64_t)a->prog.W * 8);
-  KG_HIP(hipMemcpyAsync(init.get(), a->prog.init, (size_t)a->prog.W * 8, hipMemcpyHostToDevice,
-                        a->s));
   k_init_free_records<<<stream_grid(a->cap, 256), 256, 0, a->s>>>(t, a->prog.W,
-                                                                  init.get<uint64_t>());
+                                                                  init_words(a->prog));
   KG_HIP(hipGetLastError());
-  KG_HIP(hipStreamSynchronize(a->s));
   a->rec_init = true;
 }
 
@@ -2697,11 +2714,9 @@ int compile_aggs(const std::vector<kg_column> &in_types, int32_t naggs, const kg
 }
 
 void init_records(kg_ctx *ctx, uint64_t *rec, uint64_t nrec, const AggProg &P, hipStream_t s) {
-  DevBuf init(ctx, (uint64_t)P.W * 8);
-  KG_HIP(hipMemcpyAsync(init.get(), P.init, (size_t)P.W * 8, hipMemcpyHostToDevice, s));
-  k_fill_words<<<stream_grid(nrec * P.W, 256), 256, 0, s>>>(rec, nrec, P.W, init.get<uint64_t>());
+  (void)ctx;
+  k_fill_words<<<stream_grid(nrec * P.W, 256), 256, 0, s>>>(rec, nrec, P.W, init_words(P));
   KG_HIP(hipGetLastError());
-  KG_HIP(hipStreamSynchronize(s));
 }
 
 // The slots holding a group, in slot order: list[0..n) (flags, an exclusive
